@@ -1,0 +1,166 @@
+"""Benchmark: leaf-cell RK3 updates/s of the hot path on one MI355X.
+
+A step is one pass of the reference's solute loop body without regrid (solution.F90:183-215):
+lambda (running max), dt from CFL, three SSP-RK3 stages (level-jump slots, fused WCNS3 flux +
+update, ghost fill) and the residual, all on the device. `value` is the throughput of the
+fast path (FP32 WCNS reconstruction / flux, FP64 RK update); the plain FP64 path is timed on
+the same mesh and state beside it and the L1(rho) difference between the two is reported.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "fdm-cell-based-amr-cartesian_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def workload_config(name: str, precision: int):
+    import fdmamr as F
+
+    if name == "riemann4":
+        # seeded 4-quadrant Riemann problem, [0,1]^2, base 256x256, lvl <= 4 (density gradient)
+        return F.Config(xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, h=1.0 / 256, gamma=1.4,
+                        bc=(F.BC_TRANSMISSIVE,) * 4, ic_kind=F.IC_RIEMANN4, seed=1,
+                        ic_param=(0.5, 0.5), localdt=1, cfl=0.5, t_end=0.2, amr=1, lvl_amr=4,
+                        pstep_adapt=10, amr_sensor=1, pre_adapt=4, flux_precision=precision)
+    if name == "shock4bubbles":
+        # Mach 3 shock through 4 seeded bubbles, [0,4]x[0,2], base 1024x512, lvl <= 4
+        g = 1.4
+        M = 3.0
+        p2 = 1 + 2 * g / (g + 1) * (M * M - 1)
+        r2 = (g + 1) * M * M / ((g - 1) * M * M + 2)
+        u2 = M * np.sqrt(g) * (1 - 1 / r2)
+        return F.Config(xmin=0.0, xmax=4.0, ymin=0.0, ymax=2.0, h=1.0 / 256, gamma=g,
+                        bc=(F.BC_INFLOW, F.BC_TRANSMISSIVE, F.BC_TRANSMISSIVE, F.BC_TRANSMISSIVE),
+                        inflow=(r2, u2, 0.0, p2), ic_kind=F.IC_SHOCK_BUBBLES, seed=2,
+                        ic_param=(0.5, M, 4, 0.05, 0.15, 0.1, 5.0), localdt=1, cfl=0.5,
+                        t_end=1e9, amr=1, lvl_amr=4, pstep_adapt=10, amr_sensor=1, pre_adapt=4,
+                        flux_precision=precision)
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(budget_s: float = 12.0):
+    """The CPU oracle (scalar FP64 restatement) on a bounded sample: vortex, base 64x64,
+    uniform, fixed dt, steps until the budget is used."""
+    from oracle import oracle as O
+
+    o = O.Oracle(h=10.0 / 64, tstep=1e-3)
+    n = o.n_leaf
+    o.step()
+    t0 = time.perf_counter()
+    steps = 0
+    while time.perf_counter() - t0 < budget_s:
+        o.step()
+        steps += 1
+    dt = time.perf_counter() - t0
+    return dict(value=steps * n / dt, unit="leaf-updates/s", cores=1, kind="port",
+                sample=f"oracle (C, FP64, 1 thread): vortex base 64x64 uniform, {n} leaves, "
+                       f"{steps} steps in {dt:.1f}s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="riemann4")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp64-only", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        raise SystemExit(2)
+
+    import fdmamr as F
+
+    t0 = time.perf_counter()
+    s64 = F.Solver(workload_config(args.workload, F.FLUX_FP64)).setup()
+    t_setup = time.perf_counter() - t0
+    tree = s64.tree()
+    m, n = int(tree["num"][:, 1].max()), int(tree["num"][:, 0].max())
+    cnt = s64.counts()
+    N = cnt["n_leaf"]
+    print(f"[bench] workload {args.workload}: {N} leaves, {cnt['n_slot_wls']} WLS slots, "
+          f"{cnt['n_slot_restrict']} restriction slots, l_max {cnt['l_max']}, setup {t_setup:.1f}s",
+          file=sys.stderr, flush=True)
+
+    def timed(s, label):
+        s.run_steps(args.warmup)
+        s.profile(True)
+        t0 = time.perf_counter()
+        s.run_steps(args.steps)  # synchronises at the end
+        wall = time.perf_counter() - t0
+        ms, nl = s.profile_read()
+        s.profile(False)
+        print(f"[bench] {label}: {args.steps} steps {wall*1e3:.1f} ms, stage kernel "
+              f"{ms/nl:.3f} ms avg over {nl} launches", file=sys.stderr, flush=True)
+        return wall, ms / max(nl, 1)
+
+    prim0 = s64.get_state()
+    wall64, k64 = timed(s64, "fp64")
+    res = {}
+    if not args.fp64_only:
+        s32 = F.Solver(workload_config(args.workload, F.FLUX_FP32))
+        s32.import_tree(m, n, tree)
+        s32.build_tables()
+        s32.set_state(prim0)
+        wall32, k32 = timed(s32, "fp32")
+        # accuracy on identical meshes: same start, same number of steps
+        a = s64.get_state()[:, 0]
+        b = s32.get_state()[:, 0]
+        geo = s64.geometry()
+        area = (1.0 / 2.0 ** geo["lvl"]) ** 2
+        l1 = float(np.sum(np.abs(a - b) * area) / np.sum(area))
+        linf = float(np.max(np.abs(a - b)))
+        res = dict(wall=wall32, kernel_ms=k32, l1=l1, linf=linf)
+
+    value_wall = res.get("wall", wall64)
+    kernel_ms = res.get("kernel_ms", k64)
+    ups = args.steps * N / value_wall
+    ups64 = args.steps * N / wall64
+    # algorithmic HBM bytes of one fused stage launch, averaged over the 3 stages:
+    # stencil ids 32 + level 1 + own state 32 + state out 32 + T out 8 + T_n 8 + U_n 32 (stages 2,3)
+    bytes_per_leaf = 32 + 1 + 32 + 32 + 8 + 8 + 32 * 2.0 / 3.0
+    achieved = N * bytes_per_leaf / (kernel_ms * 1e-3) / 1e9
+    out = {
+        "metric": "leaf-cell RK3 updates/s: learned+FP32 vs plain FP64 at L1(rho) err <=1e-4",
+        "value": ups,
+        "unit": "leaf-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": value_wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-flux/fp64-update" if res else "f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "n_leaf": N, "l_max": cnt["l_max"],
+                   "wls_slots": cnt["n_slot_wls"], "path": "fp32-flux (selector not yet built)"},
+        "fp64_value": ups64,
+        "speedup_vs_fp64": ups / ups64,
+        "l1_rho_vs_fp64": res.get("l1"),
+        "linf_rho_vs_fp64": res.get("linf"),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_stage", "kernel_ms": kernel_ms,
+                     "bytes_per_leaf_launch": bytes_per_leaf},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,647 @@
+// fdmamr_api.cpp -- the C-ABI (include/fdmamr.h): context, device buffers, stage sequencing
+// and the regrid crossing (device state -> host tree -> new tables -> device).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/fdmamr.h"
+#include "fdmamr_device.hpp"
+#include "host_tables.hpp"
+#include "host_tree.hpp"
+
+using namespace fdm;
+
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+};
+
+struct fdmamr_ctx {
+    fdmamr_params p{};
+    Tree tree;
+    Tables tb;
+    hipStream_t stream = nullptr;
+    bool host_only = false;  // mesh and table building only; no device calls
+    bool tables_ready = false;
+    std::string err;
+
+    DBuf<double4> Q[3];
+    DBuf<double> T[3];
+    DBuf<int32_t> stencil, wls_i, rst, bcp, son, state;
+    DBuf<int8_t> lvl;
+    DBuf<double> wls_d, partials, lampart, rot, div, tmp;
+    DBuf<int> nbad;
+    DevScalars* sc = nullptr;
+    DevTables dt{};
+
+    // kernel timing with HIP events on the context's stream (bench / profiling)
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, int>> ev_stage;  // (start, stop) indices around k_stage launches
+    int ev_used = 0;
+};
+
+namespace {
+
+int fail(fdmamr_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(ctx, -2, std::string("HIP error in ") + #expr + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+hipError_t ensure(DBuf<T>& b, size_t n) {
+    if (n == 0) n = 1;
+    if (b.cap >= n) return hipSuccess;
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+    size_t want = n + n / 4 + 64;
+    hipError_t e = hipMalloc((void**)&b.p, want * sizeof(T));
+    if (e == hipSuccess) b.cap = want;
+    return e;
+}
+template <typename T>
+void release(DBuf<T>& b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+int nblocks(int n, int b) { return (n + b - 1) / b; }
+
+int tree_depth(const Tree& t) {
+    int d = 0;
+    for (size_t c = 0; c < t.n_alloc(); c++)
+        if (t.father[c] != -2) d = std::max(d, t.lvl[c]);
+    return d;
+}
+
+int upload_tables(fdmamr_ctx* c) {
+    Tree& t = c->tree;
+    Tables& tb = c->tb;
+    const int slots = tb.S_w + tb.S_r;
+    const size_t ext = (size_t)tb.L + slots;
+    for (int k = 0; k < 3; k++) {
+        HIPCHK(c, ensure(c->Q[k], ext));
+        HIPCHK(c, ensure(c->T[k], ext));
+    }
+    HIPCHK(c, ensure(c->stencil, tb.stencil.size()));
+    HIPCHK(c, ensure(c->lvl, tb.lvl_leaf.size()));
+    HIPCHK(c, ensure(c->wls_d, tb.wls_d.size()));
+    HIPCHK(c, ensure(c->wls_i, tb.wls_i.size()));
+    HIPCHK(c, ensure(c->rst, tb.rst_node.size()));
+    HIPCHK(c, ensure(c->bcp, tb.bc_partner.size()));
+    HIPCHK(c, ensure(c->son, tb.node_son.size()));
+    HIPCHK(c, ensure(c->state, tb.state_of_node.size()));
+    const int nb = std::max(1, nblocks(tb.N, 256));
+    HIPCHK(c, ensure(c->partials, (size_t)nb * 8));
+    HIPCHK(c, ensure(c->lampart, (size_t)nb * 2));
+    HIPCHK(c, ensure(c->rot, tb.N));
+    HIPCHK(c, ensure(c->div, tb.N));
+    HIPCHK(c, ensure(c->tmp, 64));
+    HIPCHK(c, ensure(c->nbad, 1));
+    hipStream_t s = c->stream;
+    auto up = [&](void* d, const void* h, size_t bytes) -> hipError_t {
+        if (bytes == 0) return hipSuccess;
+        return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s);
+    };
+    HIPCHK(c, up(c->stencil.p, tb.stencil.data(), tb.stencil.size() * 4));
+    HIPCHK(c, up(c->lvl.p, tb.lvl_leaf.data(), tb.lvl_leaf.size()));
+    HIPCHK(c, up(c->wls_d.p, tb.wls_d.data(), tb.wls_d.size() * 8));
+    HIPCHK(c, up(c->wls_i.p, tb.wls_i.data(), tb.wls_i.size() * 4));
+    HIPCHK(c, up(c->rst.p, tb.rst_node.data(), tb.rst_node.size() * 4));
+    HIPCHK(c, up(c->bcp.p, tb.bc_partner.data(), tb.bc_partner.size() * 4));
+    HIPCHK(c, up(c->son.p, tb.node_son.data(), tb.node_son.size() * 4));
+    HIPCHK(c, up(c->state.p, tb.state_of_node.data(), tb.state_of_node.size() * 4));
+    // state of every leaf (interior, ghost, corner) into all three stage buffers
+    std::vector<double4> q(tb.L);
+    std::vector<double> tv(tb.L);
+    for (int i = 0; i < tb.L; i++) {
+        const auto& u = t.U[tb.leaf_node[i]];
+        q[i] = make_double4(u[0], u[1], u[2], u[4]);
+        tv[i] = u[3];
+    }
+    for (int k = 0; k < 3; k++) {
+        HIPCHK(c, up(c->Q[k].p, q.data(), q.size() * sizeof(double4)));
+        HIPCHK(c, up(c->T[k].p, tv.data(), tv.size() * 8));
+    }
+    DevTables& d = c->dt;
+    d.N = tb.N; d.G = tb.G; d.P = tb.P; d.L = tb.L; d.S_w = tb.S_w; d.S_r = tb.S_r;
+    d.gamma = t.gamma; d.cv = t.cv; d.h = t.h; d.xf = t.xf; d.Ma = t.Ma;
+    for (int k = 0; k < 4; k++) d.inflow[k] = t.inflow[k];
+    d.stencil = c->stencil.p; d.lvl = c->lvl.p; d.wls_d = c->wls_d.p; d.wls_i = c->wls_i.p;
+    d.rst_node = c->rst.p; d.bc_partner = c->bcp.p; d.node_son = c->son.p; d.node_state = c->state.p;
+    HIPCHK(c, hipStreamSynchronize(s));
+    return 0;
+}
+
+// device state (interior + ghost leaves of buffer 0) -> host tree
+int pull_state(fdmamr_ctx* c) {
+    Tables& tb = c->tb;
+    const int n = tb.N + tb.G;
+    std::vector<double4> q(n);
+    std::vector<double> tv(n);
+    if (n > 0) {
+        HIPCHK(c, hipMemcpyAsync(q.data(), c->Q[0].p, n * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(tv.data(), c->T[0].p, n * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n; i++) {
+        auto& u = c->tree.U[tb.leaf_node[i]];
+        u[0] = q[i].x; u[1] = q[i].y; u[2] = q[i].z; u[3] = tv[i]; u[4] = q[i].w;
+    }
+    return 0;
+}
+
+int check_ready(fdmamr_ctx* c) {
+    if (!c) return -1;
+    if (c->host_only) return fail(c, -5, "host-only context: no device");
+    if (!c->tables_ready) return fail(c, -1, "tables not built: call fdmamr_build_tables first");
+    return 0;
+}
+
+int next_event(fdmamr_ctx* c) {
+    if (c->ev_used == (int)c->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_used++;
+}
+
+// one SSP-RK3 step from buffer 0 back into buffer 0 (solution.F90:267-298)
+void enqueue_rk3(fdmamr_ctx* c, int advance_time) {
+    static const double ra[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0};   // solution.F90:392-393
+    static const double rb[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
+    static const int in[3] = {0, 1, 2}, out[3] = {1, 2, 0};
+    const DevTables& d = c->dt;
+    hipStream_t s = c->stream;
+    for (int k = 0; k < 3; k++) {
+        launch_slots(d, c->Q[in[k]].p, c->T[in[k]].p, s);
+        int e0 = -1, e1 = -1;
+        if (c->profiling && (e0 = next_event(c)) >= 0) (void)hipEventRecord(c->ev_pool[e0], s);
+        launch_stage(d, c->p.flux_precision, c->Q[in[k]].p, c->Q[0].p, c->T[0].p, c->Q[out[k]].p,
+                     c->T[out[k]].p, c->sc, ra[k], rb[k], k == 2 ? 1 : 0, c->partials.p, s);
+        if (e0 >= 0 && (e1 = next_event(c)) >= 0) {
+            (void)hipEventRecord(c->ev_pool[e1], s);
+            c->ev_stage.emplace_back(e0, e1);
+        }
+        launch_bc(d, c->Q[out[k]].p, c->T[out[k]].p, s);
+    }
+    launch_step_end(c->partials.p, nblocks(d.N, 256), d.N, advance_time, c->sc, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t fdmamr_create(const fdmamr_params* p, fdmamr_ctx** out) {
+    if (!p || !out) return -1;
+    *out = nullptr;
+    if (!(p->h > 0) || !(p->gamma > 1.0) || p->xmax <= p->xmin || p->ymax <= p->ymin) return -1;
+    if (p->flux_precision != FDMAMR_FLUX_FP64 && p->flux_precision != FDMAMR_FLUX_FP32) return -1;
+    const int kinds[4] = {p->bc_xm, p->bc_xp, p->bc_ym, p->bc_yp};
+    for (int k : kinds)
+        if (k != FDMAMR_BC_PERIODIC && k != FDMAMR_BC_TRANSMISSIVE && k != FDMAMR_BC_INFLOW) return -1;
+    const bool host_only = p->device == FDMAMR_DEVICE_HOST_ONLY;
+    if (!host_only) {
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return -5;
+        if (p->device < 0 || p->device >= ndev) return -5;
+    }
+    fdmamr_ctx* c = new (std::nothrow) fdmamr_ctx();
+    if (!c) return -1;
+    c->p = *p;
+    c->host_only = host_only;
+    if (!host_only && (hipSetDevice(p->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+                       hipMalloc((void**)&c->sc, sizeof(DevScalars)) != hipSuccess)) {
+        delete c;
+        return -2;
+    }
+    c->tree.configure(p->xmin, p->xmax, p->ymin, p->ymax, p->h, p->gamma, p->Ma > 0 ? p->Ma : 1.0);
+    for (int k = 0; k < 4; k++) { c->tree.bc[k] = kinds[k]; c->tree.inflow[k] = p->inflow[k]; }
+    *out = c;
+    return 0;
+}
+
+void fdmamr_destroy(fdmamr_ctx* c) {
+    if (!c) return;
+    if (c->host_only) { delete c; return; }
+    (void)hipSetDevice(c->p.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int k = 0; k < 3; k++) { release(c->Q[k]); release(c->T[k]); }
+    release(c->stencil); release(c->wls_i); release(c->rst); release(c->bcp); release(c->son);
+    release(c->state); release(c->lvl); release(c->wls_d); release(c->partials); release(c->lampart);
+    release(c->rot); release(c->div); release(c->tmp); release(c->nbad);
+    for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    if (c->sc) (void)hipFree(c->sc);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int32_t fdmamr_last_error(const fdmamr_ctx* c, char* buf, int32_t len) {
+    if (!c || !buf || len <= 0) return -1;
+    std::snprintf(buf, (size_t)len, "%s", c->err.c_str());
+    return 0;
+}
+
+int32_t fdmamr_creat_mesh(fdmamr_ctx* c) {
+    if (!c) return -1;
+    try {
+        c->tree.creat_mesh();
+        c->tree.idgrid();
+    } catch (const std::exception& e) {
+        return fail(c, -1, e.what());
+    }
+    c->tables_ready = false;
+    return 0;
+}
+
+int32_t fdmamr_initial(fdmamr_ctx* c, int32_t ic_kind, uint64_t seed, const double* ic_param8) {
+    if (!c) return -1;
+    if (c->tree.total == 0) return fail(c, -1, "no mesh: call fdmamr_creat_mesh first");
+    if (ic_kind < 0 || ic_kind > 3) return fail(c, -1, "unknown initial condition");
+    double z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    c->tree.initial(ic_kind, seed, ic_param8 ? ic_param8 : z);
+    c->tree.idgrid();
+    c->tree.boundary_fill_host();
+    c->tables_ready = false;
+    return 0;
+}
+
+int32_t fdmamr_tree_import(fdmamr_ctx* c, int32_t m, int32_t n, int32_t n_node,
+                           const int32_t* father, const int32_t* son, const int32_t* lvl,
+                           const int32_t* sp, const int32_t* num2, const int32_t* cross,
+                           const double* x, const double* y, const double* prim7) {
+    if (!c || m < 7 || n < 7 || n_node < m * n || !father || !son || !lvl || !sp || !num2 ||
+        !cross || !x || !y || !prim7)
+        return fail(c, -1, "fdmamr_tree_import: invalid argument");
+    Tree& t = c->tree;
+    t.m = m; t.n = n; t.total = m * n;
+    t.father.assign(father, father + n_node);
+    t.son.assign(son, son + n_node);
+    t.lvl.assign(lvl, lvl + n_node);
+    t.sp.assign(sp, sp + n_node);
+    t.cross.assign(cross, cross + n_node);
+    t.ni.resize(n_node); t.nj.resize(n_node);
+    for (int k = 0; k < n_node; k++) { t.ni[k] = num2[2 * k]; t.nj[k] = num2[2 * k + 1]; }
+    t.x.assign(x, x + n_node);
+    t.y.assign(y, y + n_node);
+    t.U.resize(n_node);
+    for (int k = 0; k < n_node; k++)
+        for (int i = 0; i < 7; i++) t.U[k][i] = prim7[7 * (size_t)k + i];
+    t.rot.assign(n_node, 0.0);
+    t.div.assign(n_node, 0.0);
+    t.free_blocks.clear();
+    for (int k = 0; k < n_node; k++) {
+        if (son[k] >= 0 && (son[k] + 3 >= n_node || son[k] < t.total))
+            return fail(c, -1, "fdmamr_tree_import: son index out of range");
+    }
+    t.idneighbor();
+    t.idgrid();
+    c->tables_ready = false;
+    return 0;
+}
+
+int32_t fdmamr_tree_size(fdmamr_ctx* c, int32_t* n_node) {
+    if (!c || !n_node) return -1;
+    c->tree.canonical_ids(n_node);
+    return 0;
+}
+
+int32_t fdmamr_tree_export(fdmamr_ctx* c, int32_t* n_node, int32_t* father, int32_t* son,
+                           int32_t* lvl, int32_t* sp, int32_t* num2, int32_t* cross, double* x,
+                           double* y, double* prim7) {
+    if (!c) return -1;
+    if (c->tables_ready && !c->host_only) { int r = pull_state(c); if (r) return r; }
+    const Tree& t = c->tree;
+    int32_t nn = 0;
+    std::vector<int32_t> id = t.canonical_ids(&nn);
+    if (n_node) *n_node = nn;
+    for (size_t k = 0; k < t.n_alloc(); k++) {
+        int j = id[k];
+        if (j < 0) continue;
+        if (father) father[j] = t.father[k] >= 0 ? id[t.father[k]] : -1;
+        if (son) son[j] = t.son[k] >= 0 ? id[t.son[k]] : -1;
+        if (lvl) lvl[j] = t.lvl[k];
+        if (sp) sp[j] = t.sp[k];
+        if (num2) { num2[2 * j] = t.ni[k]; num2[2 * j + 1] = t.nj[k]; }
+        if (cross) cross[j] = t.cross[k];
+        if (x) x[j] = t.x[k];
+        if (y) y[j] = t.y[k];
+        if (prim7) for (int i = 0; i < 7; i++) prim7[7 * (size_t)j + i] = t.U[k][i];
+    }
+    return 0;
+}
+
+int32_t fdmamr_build_tables(fdmamr_ctx* c) {
+    if (!c) return -1;
+    if (c->tree.total == 0) return fail(c, -1, "no mesh");
+    if (!c->host_only && hipSetDevice(c->p.device) != hipSuccess) return fail(c, -2, "hipSetDevice failed");
+    try {
+        c->tree.idgrid();
+        if (tree_depth(c->tree) >= MAX_TREE_DEPTH - 1) return fail(c, -1, "quadtree deeper than supported");
+        build_tables(c->tree, c->tb);
+    } catch (const std::exception& e) {
+        return fail(c, -1, std::string("table build: ") + e.what());
+    }
+    if (c->host_only) {
+        c->tables_ready = true;
+        return 0;
+    }
+    int r = upload_tables(c);
+    if (r) return r;
+    c->tables_ready = true;
+    return 0;
+}
+
+int32_t fdmamr_counts(fdmamr_ctx* c, int32_t* n_leaf, int32_t* n_ghost, int32_t* n_slot_wls,
+                      int32_t* n_slot_restrict, int32_t* l_max) {
+    if (!c) return -1;
+    if (n_leaf) *n_leaf = c->tb.N;
+    if (n_ghost) *n_ghost = c->tb.G;
+    if (n_slot_wls) *n_slot_wls = c->tb.S_w;
+    if (n_slot_restrict) *n_slot_restrict = c->tb.S_r;
+    if (l_max) *l_max = c->tb.l_max;
+    return 0;
+}
+
+int32_t fdmamr_set_state(fdmamr_ctx* c, const double* prim5) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!prim5) return fail(c, -1, "null state");
+    const int N = c->tb.N;
+    std::vector<double4> q(N);
+    std::vector<double> tv(N);
+    for (int i = 0; i < N; i++) {
+        const double* u = prim5 + 5 * (size_t)i;
+        q[i] = make_double4(u[0], u[1], u[2], u[4]);
+        tv[i] = u[3];
+    }
+    HIPCHK(c, hipMemcpyAsync(c->Q[0].p, q.data(), N * sizeof(double4), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->T[0].p, tv.data(), N * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int get_range(fdmamr_ctx* c, int off, int n, double* prim5) {
+    std::vector<double4> q(n);
+    std::vector<double> tv(n);
+    if (n > 0) {
+        HIPCHK(c, hipMemcpyAsync(q.data(), c->Q[0].p + off, n * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(tv.data(), c->T[0].p + off, n * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n; i++) {
+        double* u = prim5 + 5 * (size_t)i;
+        u[0] = q[i].x; u[1] = q[i].y; u[2] = q[i].z; u[3] = tv[i]; u[4] = q[i].w;
+    }
+    return 0;
+}
+
+int32_t fdmamr_get_state(fdmamr_ctx* c, double* prim5) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!prim5) return fail(c, -1, "null state");
+    return get_range(c, 0, c->tb.N, prim5);
+}
+
+int32_t fdmamr_get_ghost_state(fdmamr_ctx* c, double* prim5) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!prim5) return fail(c, -1, "null state");
+    return get_range(c, c->tb.N, c->tb.G, prim5);
+}
+
+int32_t fdmamr_leaf_geometry(fdmamr_ctx* c, double* x, double* y, int32_t* lvl, int32_t* node) {
+    if (!c) return -1;
+    const Tree& t = c->tree;
+    std::vector<int32_t> id;
+    if (node) id = t.canonical_ids(nullptr);
+    for (size_t i = 0; i < t.ns.size(); i++) {
+        int k = t.ns[i];
+        if (x) x[i] = t.x[k];
+        if (y) y[i] = t.y[k];
+        if (lvl) lvl[i] = t.lvl[k];
+        if (node) node[i] = id[k];
+    }
+    return 0;
+}
+
+int32_t fdmamr_max_wavespeed(fdmamr_ctx* c, double* lam_x, double* lam_y, int32_t* l_max) {
+    int r = check_ready(c);
+    if (r) return r;
+    const int nb = std::max(1, nblocks(c->tb.N, 256));
+    if (c->tb.N > 0) launch_wavespeed(c->dt, c->Q[0].p, c->lampart.p, c->stream);
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> part((size_t)nb * 2, 0.0);
+    if (c->tb.N > 0)
+        HIPCHK(c, hipMemcpyAsync(part.data(), c->lampart.p, part.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double lx = 0.0, ly = 0.0;
+    for (int b = 0; b < nb; b++) { lx = std::max(lx, part[2 * b]); ly = std::max(ly, part[2 * b + 1]); }
+    if (lam_x) *lam_x = lx;
+    if (lam_y) *lam_y = ly;
+    if (l_max) *l_max = c->tb.l_max;
+    return 0;
+}
+
+int32_t fdmamr_advance_rk3(fdmamr_ctx* c, double dt, double lam_x, double lam_y,
+                           double* res_max4, double* res_rms_sum4) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (c->tb.N == 0) return 0;
+    DevScalars s{};
+    s.dt = dt; s.lam_x = lam_x; s.lam_y = lam_y; s.h = c->tree.h; s.xf = c->tree.xf;
+    HIPCHK(c, hipMemcpyAsync(c->sc, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
+    enqueue_rk3(c, 0);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(&s, c->sc, sizeof s, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 4; k++) {
+        if (res_max4) res_max4[k] = s.res_max[k];
+        if (res_rms_sum4) res_rms_sum4[k] = s.res_sum[k];
+    }
+    for (int k = 0; k < 4; k++)
+        if (s.res_sum[k] != s.res_sum[k]) return fail(c, -3, "NaN residual");
+    return 0;
+}
+
+int32_t fdmamr_run_steps(fdmamr_ctx* c, int32_t n_steps, int32_t localdt, double cfl,
+                         double dt_fixed, double t_end, double* tt_inout, double* lam_inout2,
+                         double* scalars8) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (n_steps < 0) return fail(c, -1, "negative step count");
+    DevScalars s{};
+    s.tt = tt_inout ? *tt_inout : 0.0;
+    s.t_end = t_end; s.cfl = cfl; s.dt_fixed = dt_fixed; s.localdt = localdt;
+    s.l_max = c->tb.l_max; s.h = c->tree.h; s.xf = c->tree.xf;
+    s.lam_x = lam_inout2 ? lam_inout2[0] : 0.0;
+    s.lam_y = lam_inout2 ? lam_inout2[1] : 0.0;
+    HIPCHK(c, hipMemcpyAsync(c->sc, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
+    const int nb = std::max(1, nblocks(c->tb.N, 256));
+    for (int it = 0; it < n_steps && c->tb.N > 0; it++) {
+        launch_wavespeed(c->dt, c->Q[0].p, c->lampart.p, c->stream);
+        launch_step_begin(c->lampart.p, nb, c->sc, c->stream);
+        enqueue_rk3(c, 1);
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(&s, c->sc, sizeof s, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (tt_inout) *tt_inout = s.tt;
+    if (lam_inout2) { lam_inout2[0] = s.lam_x; lam_inout2[1] = s.lam_y; }
+    if (scalars8) {
+        scalars8[0] = s.tt; scalars8[1] = s.dt; scalars8[2] = s.lam_x; scalars8[3] = s.lam_y;
+        for (int k = 0; k < 4; k++) scalars8[4 + k] = s.res_rms[k];
+    }
+    if (s.status != 0) return fail(c, -3, "NaN residual");
+    return 0;
+}
+
+int32_t fdmamr_check_positivity(fdmamr_ctx* c, int32_t* n_bad) {
+    int r = check_ready(c);
+    if (r) return r;
+    HIPCHK(c, hipMemsetAsync(c->nbad.p, 0, sizeof(int), c->stream));
+    if (c->tb.N > 0) launch_positivity(c->dt, c->Q[0].p, c->nbad.p, c->stream);
+    int nb = 0;
+    HIPCHK(c, hipMemcpyAsync(&nb, c->nbad.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (n_bad) *n_bad = nb;
+    return nb ? fail(c, -4, "non-positive density or pressure") : 0;
+}
+
+int32_t fdmamr_sensors(fdmamr_ctx* c, double* rot, double* div) {
+    int r = check_ready(c);
+    if (r) return r;
+    const int N = c->tb.N;
+    launch_slots(c->dt, c->Q[0].p, c->T[0].p, c->stream);
+    if (N > 0) launch_sensors(c->dt, c->Q[0].p, c->rot.p, c->div.p, c->stream);
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> a(N), b(N);
+    if (N > 0) {
+        HIPCHK(c, hipMemcpyAsync(a.data(), c->rot.p, N * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(b.data(), c->div.p, N * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    Tree& t = c->tree;
+    for (int i = 0; i < N; i++) {
+        int k = t.ns[i];
+        double dy = t.h / (double)(1 << t.lvl[k]);
+        double f = std::pow(dy, 1.5);                 // dy**1.5, solution.F90:565,578
+        t.rot[k] = a[i] * f;
+        t.div[k] = b[i] * f;
+        if (rot) rot[i] = t.rot[k];
+        if (div) div[i] = t.div[k];
+    }
+    return 0;
+}
+
+int32_t fdmamr_regrid(fdmamr_ctx* c, int32_t lvl_amr, double refine_rot, double refine_div,
+                      double coarse_rot, double coarse_div, int32_t sensor_mode) {
+    int r = check_ready(c);
+    if (r) return r;
+    r = pull_state(c);
+    if (r) return r;
+    r = fdmamr_sensors(c, nullptr, nullptr);
+    if (r) return r;
+    Tree& t = c->tree;
+    if (sensor_mode == 1)
+        for (int k : t.ns) t.rot[k] = t.div[k];
+    // Srot, Sdiv, Ne in rot_div's visiting order (solution.F90:430-465)
+    std::vector<int32_t> order;
+    t.rot_div_order(order);
+    double srot = 0, sdiv = 0;
+    for (int k : order) { srot = srot + t.rot[k] * t.rot[k]; sdiv = sdiv + t.div[k] * t.div[k]; }
+    const int ne = (int)order.size();
+    double arot = std::sqrt(srot / ne), adiv = std::sqrt(sdiv / ne);
+    try {
+        t.solute_ref(arot, adiv, lvl_amr, refine_rot, refine_div, coarse_rot, coarse_div);
+        t.solute_gridmodify(lvl_amr);
+        t.idneighbor();
+    } catch (const std::exception& e) {
+        return fail(c, -1, e.what());
+    }
+    r = fdmamr_build_tables(c);
+    if (r) return r;
+    launch_bc(c->dt, c->Q[0].p, c->T[0].p, c->stream);   // solution.F90:224-229
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int32_t fdmamr_stencil_info(fdmamr_ctx* c, int32_t i, int32_t* kind8, int32_t* node8,
+                            double* px8, double* py8, int32_t* m1_8, int32_t* m2_8,
+                            int32_t* nb8x9) {
+    if (!c || !c->tables_ready || i < 0 || i >= c->tb.N) return fail(c, -1, "fdmamr_stencil_info: bad leaf");
+    std::vector<int32_t> id = c->tree.canonical_ids(nullptr);
+    const Tables& tb = c->tb;
+    for (int k = 0; k < 8; k++) {
+        size_t d = (size_t)i * 8 + k;
+        if (kind8) kind8[k] = tb.st_kind[d];
+        if (node8) node8[k] = id[tb.st_node[d]];
+        if (px8) px8[k] = tb.st_px[d];
+        if (py8) py8[k] = tb.st_py[d];
+        if (m1_8) m1_8[k] = tb.st_m1[d];
+        if (m2_8) m2_8[k] = tb.st_m2[d];
+        if (nb8x9)
+            for (int j = 0; j < 9; j++) {
+                int q = tb.st_nb[d * 9 + j];
+                nb8x9[k * 9 + j] = q >= 0 ? id[q] : -1;
+            }
+    }
+    return 0;
+}
+
+int32_t fdmamr_profile(fdmamr_ctx* c, int32_t enable) {
+    if (!c || c->host_only) return -1;
+    c->profiling = enable != 0;
+    c->ev_stage.clear();
+    c->ev_used = 0;
+    return 0;
+}
+
+int32_t fdmamr_profile_read(fdmamr_ctx* c, double* stage_ms_total, int32_t* stage_launches) {
+    if (!c || c->host_only) return -1;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    for (auto& pr : c->ev_stage) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev_pool[pr.first], c->ev_pool[pr.second]));
+        tot += ms;
+    }
+    if (stage_ms_total) *stage_ms_total = tot;
+    if (stage_launches) *stage_launches = (int32_t)c->ev_stage.size();
+    c->ev_stage.clear();
+    c->ev_used = 0;
+    return 0;
+}
+
+int32_t fdmamr_stencil_values(fdmamr_ctx* c, int32_t i, double* val8x4) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (i < 0 || i >= c->tb.N || !val8x4) return fail(c, -1, "fdmamr_stencil_values: bad leaf");
+    launch_slots(c->dt, c->Q[0].p, c->T[0].p, c->stream);
+    launch_stencil_values(c->dt, c->Q[0].p, i, c->tmp.p, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(val8x4, c->tmp.p, 32 * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,49 @@
+// fdmamr_device.hpp -- device-side table views and step scalars shared by the kernels and the
+// host-side context.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "host_tables.hpp"
+
+namespace fdm {
+
+// maximum quadtree depth handled by the on-device restriction walk
+constexpr int MAX_TREE_DEPTH = 24;
+
+// Device views of the flattened tables (all pointers are device memory).
+struct DevTables {
+    int N, G, P, L, S_w, S_r;
+    double gamma, cv, h, xf, Ma;
+    double inflow[4];
+    const int32_t* stencil;     // [8][N]
+    const int8_t* lvl;          // [N]
+    const double* wls_d;        // [S_w][WLS_STRIDE]
+    const int32_t* wls_i;       // [S_w][WLSI_STRIDE]
+    const int32_t* rst_node;    // [S_r]
+    const int32_t* bc_partner;  // [G]
+    const int32_t* node_son;    // [n_node]
+    const int32_t* node_state;  // [n_node]
+};
+
+// Scalars of the device-resident step loop (solution.F90:183-215).
+struct DevScalars {
+    double dt, lam_x, lam_y, tt, t_end, cfl, dt_fixed, h, xf;
+    double res_max[4], res_sum[4], res_rms[4];
+    int32_t localdt, l_max, steps_done, status;
+};
+
+void launch_slots(const DevTables& t, double4* Q, const double* T, hipStream_t st);
+void launch_stage(const DevTables& t, int precision, const double4* Qin, const double4* Qn,
+                  const double* Tn, double4* Qout, double* Tout, const DevScalars* sc,
+                  double alpha, double beta, int reduce, double* partials, hipStream_t st);
+void launch_bc(const DevTables& t, double4* Q, double* T, hipStream_t st);
+void launch_wavespeed(const DevTables& t, const double4* Q, double* partials, hipStream_t st);
+void launch_step_begin(const double* partials, int nb, DevScalars* sc, hipStream_t st);
+void launch_step_end(const double* partials, int nb, int n_leaf, int advance_time,
+                     DevScalars* sc, hipStream_t st);
+void launch_sensors(const DevTables& t, const double4* Q, double* rot, double* div, hipStream_t st);
+void launch_positivity(const DevTables& t, const double4* Q, int* nbad, hipStream_t st);
+void launch_stencil_values(const DevTables& t, const double4* Q, int i, double* out, hipStream_t st);
+
+}  // namespace fdm

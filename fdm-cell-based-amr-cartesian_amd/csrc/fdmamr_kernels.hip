@@ -1,0 +1,595 @@
+// fdmamr_kernels.hip -- gfx950 kernels of the per-step hot path.
+//
+// One RK stage is three launches on the context's stream:
+//   1. k_slots_wls / k_slots_restrict: level-jump stencil values (WLS-WENO,
+//      XuanGua.F90:179-356, and restriction means, XuanGua.F90:359-398) from the stage input,
+//      written behind the leaf states so the flux kernel gathers all eight stencil entries
+//      from one array;
+//   2. k_stage: one thread per interior leaf, gathers the 5-point x and y columns, computes
+//      the WCNS3 flux derivative in both directions (nnd_center_RK.F90:66-353), the
+//      forward-Euler substep (nnd_center_RK.F90:15-44) and the primitive RK blend
+//      (solution.F90:386-404), writes the stage output; stage 3 also reduces the residual;
+//   3. k_bc_fill: ghost leaves from their partner cells (solution.F90:109-168,314-356).
+//
+// Layout in HBM: state as AoS double4 (rho,u,v,p) per leaf -- one 32-byte gather per stencil
+// entry -- plus a separate double array for the passive temperature T. Leaves are in the
+// reference's cell_NS order (Morton inside each base cell) followed by ghost and corner
+// leaves, then the slot values. Stencil ids are int32 [8][N] (coalesced per position).
+//
+// FP64 kernels are compiled with -ffp-contract=off and follow the reference's evaluation
+// order, so they reproduce the CPU restatement bit for bit.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fdmamr_device.hpp"
+
+namespace fdm {
+
+// ---------------------------------------------------------------- constants
+#define WCNS_D1 0x1.745d18p-4                     // REAL(4) 1./11.  nnd_center_RK.F90:331
+#define WCNS_D2 0x1.d1745ep-1                     // REAL(4) 10./11.
+#define WCNS_EPS 0x1.16c2p-133                    // REAL(4) 1.e-40 (subnormal) nnd_center_RK.F90:319
+
+// ---------------------------------------------------------------- reductions
+__device__ __forceinline__ double wave_max(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o);
+    return v;
+}
+
+// ---------------------------------------------------------------- flux math
+template <typename S>
+struct Pt { S U[4]; S F[4]; };
+
+// compute_E / compute_F (nnd_center_RK.F90:171-185,293-307) for one point
+template <typename S, int DIR>
+__device__ __forceinline__ void cons_flux(double4 q, S gm1, Pt<S>& o) {
+    S r = (S)q.x, u = (S)q.y, v = (S)q.z, p = (S)q.w;
+    o.U[0] = r;
+    o.U[1] = r * u;
+    o.U[2] = r * v;
+    o.U[3] = p / gm1 + (S)0.5 * r * (u * u + v * v);
+    if (DIR == 0) {
+        o.F[0] = r * u;
+        o.F[1] = r * u * u + p;
+        o.F[2] = r * u * v;
+        o.F[3] = u * (p / gm1 + (S)0.5 * r * (u * u + v * v) + p);
+    } else {
+        o.F[0] = r * v;
+        o.F[1] = r * u * v;
+        o.F[2] = r * v * v + p;
+        o.F[3] = v * (p / gm1 + (S)0.5 * r * (u * u + v * v) + p);
+    }
+}
+
+// REcalc / RFcalc (nnd_center_RK.F90:114-169,236-291). R and Ri are row-major here:
+// R[r][c] = R_i(r+1, c+1).
+template <typename S, int DIR>
+__device__ __forceinline__ void eigen(const S* U, S gama, S R[4][4], S Ri[4][4]) {
+    S r = U[0], u = U[1] / U[0], v = U[2] / U[0];
+    S p = (gama - (S)1.0) * r * (U[3] / r - (S)0.5 * (u * u + v * v));
+    S a = sqrt(gama * p / r);
+    S q2 = (S)0.5 * (u * u + v * v);
+    S g1 = gama - (S)1.0, mg = (S)1.0 - gama;
+    S vn = DIR == 0 ? u : v;  // normal velocity
+    R[0][0] = 1; R[0][1] = 1; R[0][2] = 1; R[0][3] = 0;
+    R[3][0] = a * a / g1 + q2 - vn * a;
+    R[3][1] = q2;
+    R[3][2] = a * a / g1 + q2 + vn * a;
+    if (DIR == 0) {
+        R[1][0] = u - a; R[1][1] = u; R[1][2] = u + a; R[1][3] = 0;
+        R[2][0] = v; R[2][1] = v; R[2][2] = v; R[2][3] = -1;
+        R[3][3] = -v;
+        Ri[0][0] = (g1 * q2 + a * u) / ((S)2.0 * a * a);
+        Ri[1][0] = (a * a - g1 * q2) / (a * a);
+        Ri[2][0] = (g1 * q2 - a * u) / ((S)2.0 * a * a);
+        Ri[3][0] = v;
+        Ri[0][1] = (mg * u - a) / ((S)2.0 * a * a);
+        Ri[1][1] = (g1 * u) / (a * a);
+        Ri[2][1] = (mg * u + a) / ((S)2.0 * a * a);
+        Ri[3][1] = 0;
+        Ri[0][2] = (mg * v) / ((S)2.0 * a * a);
+        Ri[1][2] = (g1 * v) / (a * a);
+        Ri[2][2] = (mg * v) / ((S)2.0 * a * a);
+        Ri[3][2] = -1;
+    } else {
+        R[1][0] = u; R[1][1] = u; R[1][2] = u; R[1][3] = 1;
+        R[2][0] = v - a; R[2][1] = v; R[2][2] = v + a; R[2][3] = 0;
+        R[3][3] = u;
+        Ri[0][0] = (g1 * q2 + a * v) / ((S)2.0 * a * a);
+        Ri[1][0] = (a * a - g1 * q2) / (a * a);
+        Ri[2][0] = (g1 * q2 - a * v) / ((S)2.0 * a * a);
+        Ri[3][0] = -u;
+        Ri[0][1] = (mg * u) / ((S)2.0 * a * a);
+        Ri[1][1] = (g1 * u) / (a * a);
+        Ri[2][1] = (mg * u) / ((S)2.0 * a * a);
+        Ri[3][1] = 1;
+        Ri[0][2] = (mg * v - a) / ((S)2.0 * a * a);
+        Ri[1][2] = (g1 * v) / (a * a);
+        Ri[2][2] = (mg * v + a) / ((S)2.0 * a * a);
+        Ri[3][2] = 0;
+    }
+    Ri[0][3] = g1 / ((S)2.0 * a * a);
+    Ri[1][3] = mg / (a * a);
+    Ri[2][3] = g1 / ((S)2.0 * a * a);
+    Ri[3][3] = 0;
+}
+
+// WCNS_Z3 nonlinear weights for one component (nnd_center_RK.F90:326-346)
+template <typename S>
+__device__ __forceinline__ S wcns_component(S u1, S u2, S u3) {
+    S IS1 = (-u1 + u2) * (-u1 + u2);
+    S IS2 = (-u2 + u3) * (-u2 + u3);
+    S t3 = u1 - (S)2. * u2 + u3;
+    S IS3 = t3 * t3;
+    S p1 = u2;
+    S p2 = ((-u1 + (S)6.0 * u2 + (S)3.0 * u3) / (S)8.0 - (S)WCNS_D1 * p1) / (S)WCNS_D2;
+    IS1 = IS1 < IS2 ? IS1 : IS2;
+    IS2 = IS3 + u1 * u1 + u2 * u2 + u3 * u3 - u1 * u2 - u2 * u3 - u1 * u3;
+    S tq = (S)10. * fabs(IS2 - (S)3. * IS1);
+    S tau3 = tq * tq;
+    S r1, r2;
+    if constexpr (sizeof(S) == 8) {
+        r1 = tau3 / (WCNS_EPS + IS1);
+        r2 = tau3 / (WCNS_EPS + IS2);
+    } else {
+        // FP32: the REAL(4) epsilon is a subnormal; use a normal floor and clamp the ratios
+        // so that large jumps cannot overflow to inf/inf
+        r1 = fminf(tau3 / (1.0e-30f + IS1), 1.0e30f);
+        r2 = fminf(tau3 / (1.0e-30f + IS2), 1.0e30f);
+    }
+    S a1 = (S)WCNS_D1 * ((S)1.0 + r1);
+    S a2 = (S)WCNS_D2 * ((S)1.0 + r2);
+    S w1 = a1 / (a1 + a2), w2 = a2 / (a1 + a2);
+    return w1 * p1 + w2 * p2;
+}
+
+template <typename S>
+__device__ __forceinline__ void wcns(const S* u1, const S* u2, const S* u3, const S R[4][4], S* hp) {
+    S uL[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) uL[i] = wcns_component<S>(u1[i], u2[i], u3[i]);
+#pragma unroll
+    for (int r = 0; r < 4; r++) hp[r] = R[r][0] * uL[0] + R[r][1] * uL[1] + R[r][2] * uL[2] + R[r][3] * uL[3];
+}
+
+// LXL_X_wcns3 / LXL_Y_wcns3 (nnd_center_RK.F90:66-112,187-234): q[0..4] = points i-2..i+2
+template <typename S, int DIR>
+__device__ __forceinline__ void flux_derivative(const double4* q, double d, S lam, S gama, double* dEF) {
+    Pt<S> P[5];
+    const S gm1 = gama - (S)1.0;
+#pragma unroll
+    for (int j = 0; j < 5; j++) cons_flux<S, DIR>(q[j], gm1, P[j]);
+    S flux[2][4];
+#pragma unroll
+    for (int f = 0; f < 2; f++) {
+        S avg[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) avg[k] = (P[f + 1].U[k] + P[f + 2].U[k]) / (S)2.0;
+        S R[4][4], Ri[4][4];
+        eigen<S, DIR>(avg, gama, R, Ri);
+        S w[4][4], g[4][4];  // characteristic variables at points f..f+3
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                S s = (S)0.0, t = (S)0.0;
+#pragma unroll
+                for (int c = 0; c < 4; c++) { s = s + Ri[r][c] * P[f + j].U[c]; t = t + Ri[r][c] * P[f + j].F[c]; }
+                w[j][r] = s; g[j][r] = t;
+            }
+        S wL[4], wR[4], gL[4], gR[4];
+        wcns<S>(w[0], w[1], w[2], R, wL);
+        wcns<S>(w[3], w[2], w[1], R, wR);
+        wcns<S>(g[0], g[1], g[2], R, gL);
+        wcns<S>(g[3], g[2], g[1], R, gR);
+#pragma unroll
+        for (int k = 0; k < 4; k++) flux[f][k] = (S)0.5 * (gL[k] + gR[k] - lam * (wR[k] - wL[k]));
+    }
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        double df = (double)(flux[1][n] - flux[0][n]);
+        double de = (double)(P[3].F[n] - P[1].F[n]);
+        if (DIR == 0) dEF[n] = 4.0 / (3.0 * d) * df - 1.0 / (6.0 * d) * de;
+        else dEF[n] = 4.0 / 3.0 / d * df - 1.0 / 6.0 / d * de;
+    }
+}
+
+// ---------------------------------------------------------------- stage kernel
+// Qn/Tn alias Qout/Tout in stage 3 (in-place final stage), so they carry no __restrict__.
+template <typename S>
+__global__ __launch_bounds__(256) void k_stage(DevTables t, const double4* __restrict__ Qin,
+                                               const double4* Qn, const double* Tn,
+                                               double4* Qout, double* Tout,
+                                               const DevScalars* __restrict__ sc, double alpha,
+                                               double beta, int reduce, double* __restrict__ partials) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double rmax[4] = {0, 0, 0, 0}, rsum[4] = {0, 0, 0, 0};
+    if (i < t.N) {
+        const double gama = t.gamma, cv = t.cv;
+        const double dt = sc->dt;
+        int s[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) s[k] = t.stencil[(size_t)k * t.N + i];
+        const double4 qc = Qin[i];
+        const int lv = t.lvl[i];
+        const double dy = t.h / (double)(1 << lv);
+        const double dx = t.xf * dy;
+        double dE[4], dF[4];
+        {
+            double4 col[5] = {Qin[s[5]], Qin[s[1]], qc, Qin[s[0]], Qin[s[4]]};
+            flux_derivative<S, 0>(col, dx, (S)sc->lam_x, (S)gama, dE);
+        }
+        {
+            double4 col[5] = {Qin[s[7]], Qin[s[3]], qc, Qin[s[2]], Qin[s[6]]};
+            flux_derivative<S, 1>(col, dy, (S)sc->lam_y, (S)gama, dF);
+        }
+        // LXL (nnd_center_RK.F90:22-42)
+        double U[4], Ut[4], fu[4];
+        U[0] = qc.x;
+        U[1] = qc.x * qc.y;
+        U[2] = qc.x * qc.z;
+        U[3] = qc.w / (gama - 1.0) + 0.5 * qc.x * (qc.y * qc.y + qc.z * qc.z);
+#pragma unroll
+        for (int k = 0; k < 4; k++) fu[k] = -dE[k] - dF[k];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            Ut[k] = U[k] + fu[k] * dt;
+            fu[k] = fu[k] * dx * dy;
+            rmax[k] = fabs(fu[k]);
+            rsum[k] = fu[k] * fu[k];
+        }
+        double r = Ut[0];
+        double u = Ut[1] / Ut[0];
+        double v = Ut[2] / Ut[0];
+        double T = (Ut[3] - 0.5 * Ut[0] * (u * u + v * v)) / (cv * Ut[0]);
+        double p = (gama - 1.0) * r * (Ut[3] / r - 0.5 * (u * u + v * v));
+        // updateRK (solution.F90:398-400): U = alpha*Un + beta*Ut on (rho,u,v,T,p)
+        const double4 qn = Qn[i];
+        double4 o;
+        o.x = alpha * qn.x + beta * r;
+        o.y = alpha * qn.y + beta * u;
+        o.z = alpha * qn.z + beta * v;
+        o.w = alpha * qn.w + beta * p;
+        Qout[i] = o;
+        Tout[i] = alpha * Tn[i] + beta * T;
+    }
+    if (reduce) {
+        __shared__ double red[4][8];
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            double a = wave_max(rmax[k]);
+            double b = wave_sum(rsum[k]);
+            if (lane == 0) { red[wv][k] = a; red[wv][4 + k] = b; }
+        }
+        __syncthreads();
+        if (threadIdx.x < 8) {
+            const int k = threadIdx.x;
+            const int nw = blockDim.x >> 6;
+            double acc = red[0][k];
+            for (int w = 1; w < nw; w++) acc = k < 4 ? fmax(acc, red[w][k]) : acc + red[w][k];
+            partials[(size_t)blockIdx.x * 8 + k] = acc;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- WLS-WENO slots
+__device__ __forceinline__ double isq_poly(double k1, double k2, double k3, const double* b) {
+    // XuanGua.F90:342-355; b = dxb, dyb, x2, y2, x3, y3
+    double term1 = k1 * k1 * b[0] * b[1];
+    double term2 = 2 * k1 * k2 * b[2] / 2 * b[1] + 2 * k1 * k3 * b[0] * b[3] / 2;
+    double term3 = k2 * k2 * b[4] / 3 * b[1] + 2 * k2 * k3 * b[2] / 2 * b[3] / 2 + k3 * k3 * b[0] * b[5] / 3;
+    return term1 + term2 + term3;
+}
+
+__global__ __launch_bounds__(128) void k_slots_wls(DevTables t, double4* __restrict__ Q) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= t.S_w) return;
+    const double* __restrict__ d = t.wls_d + (size_t)s * WLS_STRIDE;
+    const int32_t* __restrict__ iv = t.wls_i + (size_t)s * WLSI_STRIDE;
+    const int m = iv[WLSI_M], m1 = iv[WLSI_M1];
+    const double xx = d[WLS_XX], yy = d[WLS_YY];
+    const double* box = d + WLS_BOX;
+    double val[9][4];
+    for (int i = 0; i < 9; i++) {
+        if (i < m) {
+            double4 q = Q[iv[WLSI_IDS + i]];
+            val[i][0] = q.x; val[i][1] = q.y; val[i][2] = q.z; val[i][3] = q.w;
+        }
+    }
+    const double lw0 = 0.98, lw1 = 0.01, lw2 = 0.01;
+    double out[4];
+    for (int k = 0; k < 4; k++) {
+        double wphi[9];
+        for (int i = 0; i < 9; i++) wphi[i] = i < m ? d[WLS_W + i] * val[i][k] : 0.0;
+        double b6[6] = {0, 0, 0, 0, 0, 0}, b3a[3] = {0, 0, 0}, b3b[3] = {0, 0, 0};
+        for (int i = 0; i < 9; i++) {
+            if (i >= m) break;
+            const double px = d[WLS_PX + i], py = d[WLS_PY + i];
+            const double P[6] = {1.0, px, py, px * py, px * px, py * py};
+            for (int r = 0; r < 6; r++) b6[r] = b6[r] + P[r] * wphi[i];
+            if (i < m1) for (int r = 0; r < 3; r++) b3a[r] = b3a[r] + P[r] * wphi[i];
+            else for (int r = 0; r < 3; r++) b3b[r] = b3b[r] + P[r] * wphi[i];
+        }
+        double al[6], as1[3], as2[3];
+        for (int r = 0; r < 6; r++) {
+            double a = 0.0;
+            for (int c = 0; c < 6; c++) a = a + d[WLS_INV6 + c * 6 + r] * b6[c];
+            al[r] = a;
+        }
+        for (int r = 0; r < 3; r++) {
+            double a = 0.0, b = 0.0;
+            for (int c = 0; c < 3; c++) { a = a + d[WLS_INV3A + c * 3 + r] * b3a[c]; b = b + d[WLS_INV3B + c * 3 + r] * b3b[c]; }
+            as1[r] = a; as2[r] = b;
+        }
+        double vl = al[0] + xx * al[1] + yy * al[2] + xx * yy * al[3] + xx * xx * al[4] + yy * yy * al[5];
+        double sl = 0.0;                            // compute_smoothness order 2, XuanGua.F90:325-334
+        sl = sl + isq_poly(al[1], 2 * al[3], al[4], box);
+        sl = sl + isq_poly(al[2], al[4], 2 * al[5], box);
+        sl = sl + (2 * al[3]) * (2 * al[3]) * box[0] * box[1];
+        sl = sl + al[4] * al[4] * box[0] * box[1];
+        sl = sl + (2 * al[5]) * (2 * al[5]) * box[0] * box[1];
+        double vs1 = as1[0] + xx * as1[1] + yy * as1[2];
+        double vs2 = as2[0] + xx * as2[1] + yy * as2[2];
+        double be1 = (as1[1] * as1[1] + as1[2] * as1[2]) * box[0] * box[1];
+        double be2 = (as2[1] * as2[1] + as2[2] * as2[2]) * box[0] * box[1];
+        double tau = (fabs(sl - be1) + fabs(sl - be2)) / 2.0;
+        tau = tau * tau;
+        double a0 = lw0 * (1.0 + tau / (1e-6 + sl));
+        double a1 = lw1 * (1.0 + tau / (1e-6 + be1));
+        double a2 = lw2 * (1.0 + tau / (1e-6 + be2));
+        double sa = a0 + a1 + a2;
+        double o0 = a0 / sa, o1 = a1 / sa, o2 = a2 / sa;
+        out[k] = o0 * (1.0 / lw0 * vl - (lw1 / lw0 * vs1 + lw2 / lw0 * vs2)) + (o1 * vs1 + o2 * vs2);
+    }
+    Q[t.L + s] = make_double4(out[0], out[1], out[2], out[3]);
+}
+
+// ---------------------------------------------------------------- restriction means
+// interpA (XuanGua.F90:359-398) on the device: depth-first mean over a subtree, keeping the
+// reference's ((s1+s2)+s3)+s4 order at every level.
+__device__ void restrict_mean(const DevTables& t, const double4* Q, const double* T, int node,
+                              double out[5]) {
+    struct Frame { int node, k; double acc[5]; };
+    Frame st[MAX_TREE_DEPTH];
+    int top = 0;
+    st[0].node = node; st[0].k = 0;
+    for (;;) {
+        Frame& f = st[top];
+        if (f.k == 4) {
+            double r[5];
+            for (int v = 0; v < 5; v++) r[v] = 0.25 * f.acc[v];
+            if (top == 0) { for (int v = 0; v < 5; v++) out[v] = r[v]; return; }
+            top--;
+            Frame& g = st[top];
+            for (int v = 0; v < 5; v++) g.acc[v] = g.k == 0 ? r[v] : g.acc[v] + r[v];
+            g.k++;
+            continue;
+        }
+        const int c = t.node_son[f.node] + f.k;
+        if (t.node_son[c] < 0) {
+            const int li = t.node_state[c];
+            const double4 q = Q[li];
+            const double val[5] = {q.x, q.y, q.z, T[li], q.w};
+            for (int v = 0; v < 5; v++) f.acc[v] = f.k == 0 ? val[v] : f.acc[v] + val[v];
+            f.k++;
+        } else {
+            if (top + 1 >= MAX_TREE_DEPTH) { f.k = 4; continue; }  // cannot happen (depth checked on host)
+            top++;
+            st[top].node = c; st[top].k = 0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(128) void k_slots_restrict(DevTables t, double4* __restrict__ Q,
+                                                        const double* __restrict__ T) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= t.S_r) return;
+    double o[5];
+    restrict_mean(t, Q, T, t.rst_node[s], o);
+    Q[t.L + t.S_w + s] = make_double4(o[0], o[1], o[2], o[4]);
+}
+
+// ---------------------------------------------------------------- ghost fill
+__global__ __launch_bounds__(128) void k_bc_fill(DevTables t, double4* __restrict__ Q,
+                                                 double* __restrict__ T) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= t.G) return;
+    const int p = t.bc_partner[g];
+    double4 q;
+    double tv;
+    if (p < 0) {  // fixed inflow state (new boundary kind)
+        q = make_double4(t.inflow[0], t.inflow[1], t.inflow[2], t.inflow[3]);
+        tv = (t.inflow[3] * t.gamma * t.Ma * t.Ma) / t.inflow[0];
+    } else if (t.node_son[p] < 0) {
+        const int li = t.node_state[p];
+        q = Q[li];
+        tv = T[li];
+    } else {
+        double o[5];
+        restrict_mean(t, Q, T, p, o);
+        q = make_double4(o[0], o[1], o[2], o[4]);
+        tv = o[3];
+    }
+    Q[t.N + g] = q;
+    T[t.N + g] = tv;
+}
+
+// ---------------------------------------------------------------- lambda / scalars
+__global__ __launch_bounds__(256) void k_wavespeed(DevTables t, const double4* __restrict__ Q,
+                                                   double* __restrict__ partials) {
+    // solution.F90:186-193: max |u|+c, |v|+c over interior leaves
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double lx = 0.0, ly = 0.0;
+    if (i < t.N) {
+        const double4 q = Q[i];
+        const double a = sqrt(t.gamma * q.w / q.x);
+        lx = fabs(q.y) + a;
+        ly = fabs(q.z) + a;
+    }
+    lx = wave_max(lx);
+    ly = wave_max(ly);
+    __shared__ double red[4][2];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) { red[wv][0] = lx; red[wv][1] = ly; }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        double acc = red[0][threadIdx.x];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) acc = fmax(acc, red[w][threadIdx.x]);
+        partials[(size_t)blockIdx.x * 2 + threadIdx.x] = acc;
+    }
+}
+
+// reduce lambda partials; running max; dt from CFL (solution.F90:185-201)
+__global__ __launch_bounds__(256) void k_step_begin(const double* __restrict__ partials, int nblk,
+                                                    DevScalars* sc) {
+    __shared__ double sx[256], sy[256];
+    double ax = 0.0, ay = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+        ax = fmax(ax, partials[2 * b]);
+        ay = fmax(ay, partials[2 * b + 1]);
+    }
+    sx[threadIdx.x] = ax; sy[threadIdx.x] = ay;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            sx[threadIdx.x] = fmax(sx[threadIdx.x], sx[threadIdx.x + o]);
+            sy[threadIdx.x] = fmax(sy[threadIdx.x], sy[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (sc->status != 0) return;
+        // lambda_x = max(lambda_x, ...) keeps the running maximum (solution.F90:189-190)
+        if (sx[0] > sc->lam_x) sc->lam_x = sx[0];
+        if (sy[0] > sc->lam_y) sc->lam_y = sy[0];
+        double dt = sc->dt_fixed;
+        if (sc->localdt) {
+            const double p2 = (double)(1 << sc->l_max);
+            dt = sc->cfl / (sc->lam_x / (sc->xf * sc->h / p2) + sc->lam_y / (sc->h / p2));
+        }
+        if (sc->tt + dt > sc->t_end) dt = sc->t_end - sc->tt;
+        sc->dt = dt;
+    }
+}
+
+// residual finalize; tt advance; NaN status (solution.F90:205-214)
+__global__ __launch_bounds__(256) void k_step_end(const double* __restrict__ partials, int nblk,
+                                                  int n_leaf, int advance_time, DevScalars* sc) {
+    __shared__ double sh[8][256];
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x)
+        for (int k = 0; k < 8; k++) acc[k] = k < 4 ? fmax(acc[k], partials[8 * b + k]) : acc[k] + partials[8 * b + k];
+    for (int k = 0; k < 8; k++) sh[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int k = 0; k < 8; k++)
+                sh[k][threadIdx.x] = k < 4 ? fmax(sh[k][threadIdx.x], sh[k][threadIdx.x + o])
+                                           : sh[k][threadIdx.x] + sh[k][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 4; k++) { sc->res_max[k] = sh[k][0]; sc->res_sum[k] = sh[4 + k][0]; }
+        if (advance_time && sc->status == 0) {
+            sc->tt = sc->tt + sc->dt;
+            sc->steps_done = sc->steps_done + 1;
+            for (int k = 0; k < 4; k++) {
+                double r = sqrt(sc->res_sum[k] / (1.0 * n_leaf));
+                sc->res_rms[k] = r;
+                if (r != r) sc->status = -3;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- sensors
+// rot_div raw parts (solution.F90:558-580): |vx-uy| and |ux|+|vy| from the +-1 stencil values;
+// the host multiplies by dy**1.5.
+__global__ __launch_bounds__(256) void k_sensors(DevTables t, const double4* __restrict__ Q,
+                                                 double* __restrict__ rot, double* __restrict__ div) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= t.N) return;
+    const int lv = t.lvl[i];
+    const double dy = t.h / (double)(1 << lv);
+    const double dx = t.xf * dy;
+    const double4 ip = Q[t.stencil[i]], im = Q[t.stencil[(size_t)t.N + i]];
+    const double4 jp = Q[t.stencil[(size_t)2 * t.N + i]], jm = Q[t.stencil[(size_t)3 * t.N + i]];
+    const double vx = (ip.z - im.z) / (2.0 * dx), uy = (jp.y - jm.y) / (2.0 * dy);
+    rot[i] = fabs(vx - uy);
+    const double ux = (ip.x - im.x) / (2.0 * dx), vy = (jp.x - jm.x) / (2.0 * dy);
+    div[i] = fabs(ux) + fabs(vy);
+}
+
+// positivity of rho and p (FP64), counts bad leaves
+__global__ __launch_bounds__(256) void k_positivity(DevTables t, const double4* __restrict__ Q,
+                                                    int* __restrict__ nbad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= t.N) return;
+    const double4 q = Q[i];
+    if (!(q.x > 0.0) || !(q.w > 0.0)) atomicAdd(nbad, 1);
+}
+
+// gather the eight stencil values of one leaf (diagnostics)
+__global__ void k_stencil_values(DevTables t, const double4* __restrict__ Q, int i, double* out) {
+    const int k = threadIdx.x;
+    if (k < 8 && i < t.N) {
+        const double4 q = Q[t.stencil[(size_t)k * t.N + i]];
+        out[4 * k + 0] = q.x; out[4 * k + 1] = q.y; out[4 * k + 2] = q.z; out[4 * k + 3] = q.w;
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+static inline int nblk(int n, int b) { return (n + b - 1) / b; }
+
+void launch_slots(const DevTables& t, double4* Q, const double* T, hipStream_t st) {
+    if (t.S_w > 0) hipLaunchKernelGGL(k_slots_wls, dim3(nblk(t.S_w, 128)), dim3(128), 0, st, t, Q);
+    if (t.S_r > 0) hipLaunchKernelGGL(k_slots_restrict, dim3(nblk(t.S_r, 128)), dim3(128), 0, st, t, Q, T);
+}
+
+void launch_stage(const DevTables& t, int precision, const double4* Qin, const double4* Qn,
+                  const double* Tn, double4* Qout, double* Tout, const DevScalars* sc,
+                  double alpha, double beta, int reduce, double* partials, hipStream_t st) {
+    if (t.N <= 0) return;
+    if (precision == 0)
+        hipLaunchKernelGGL(k_stage<double>, dim3(nblk(t.N, 256)), dim3(256), 0, st, t, Qin, Qn,
+                           Tn, Qout, Tout, sc, alpha, beta, reduce, partials);
+    else
+        hipLaunchKernelGGL(k_stage<float>, dim3(nblk(t.N, 256)), dim3(256), 0, st, t, Qin, Qn, Tn,
+                           Qout, Tout, sc, alpha, beta, reduce, partials);
+}
+
+void launch_bc(const DevTables& t, double4* Q, double* T, hipStream_t st) {
+    if (t.G > 0) hipLaunchKernelGGL(k_bc_fill, dim3(nblk(t.G, 128)), dim3(128), 0, st, t, Q, T);
+}
+
+void launch_wavespeed(const DevTables& t, const double4* Q, double* partials, hipStream_t st) {
+    hipLaunchKernelGGL(k_wavespeed, dim3(nblk(t.N, 256)), dim3(256), 0, st, t, Q, partials);
+}
+
+void launch_step_begin(const double* partials, int nb, DevScalars* sc, hipStream_t st) {
+    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(256), 0, st, partials, nb, sc);
+}
+
+void launch_step_end(const double* partials, int nb, int n_leaf, int advance_time,
+                     DevScalars* sc, hipStream_t st) {
+    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(256), 0, st, partials, nb, n_leaf, advance_time, sc);
+}
+
+void launch_sensors(const DevTables& t, const double4* Q, double* rot, double* div, hipStream_t st) {
+    hipLaunchKernelGGL(k_sensors, dim3(nblk(t.N, 256)), dim3(256), 0, st, t, Q, rot, div);
+}
+
+void launch_positivity(const DevTables& t, const double4* Q, int* nbad, hipStream_t st) {
+    hipLaunchKernelGGL(k_positivity, dim3(nblk(t.N, 256)), dim3(256), 0, st, t, Q, nbad);
+}
+
+void launch_stencil_values(const DevTables& t, const double4* Q, int i, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_stencil_values, dim3(1), dim3(64), 0, st, t, Q, i, out);
+}
+
+}  // namespace fdm

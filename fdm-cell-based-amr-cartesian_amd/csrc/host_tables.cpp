@@ -1,0 +1,318 @@
+// host_tables.cpp -- regrid-time flattening of the quadtree (see host_tables.hpp).
+#include "host_tables.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <unordered_map>
+
+namespace fdm {
+
+static const double PI_REF = 3.1415926535897932;   // module.F90:9
+static const double DIR_EPS = 0x1.b7cdfep-34;      // REAL(4) 1e-10, XuanGua.F90:40
+
+static inline double pow2i(int k) { return (double)(1 << k); }
+
+int brinv(int n, const double* a, double* r) {     // XuanGua.F90:593-660 (column-major)
+#define R(i, j) r[((j) - 1) * n + ((i) - 1)]
+    int is[16], js[16];
+    for (int k = 0; k < n * n; k++) r[k] = a[k];
+    for (int k = 1; k <= n; k++) {
+        double d = 0.0;
+        for (int i = k; i <= n; i++)
+            for (int j = k; j <= n; j++)
+                if (std::fabs(R(i, j)) > d) { d = std::fabs(R(i, j)); is[k - 1] = i; js[k - 1] = j; }
+        if (d + 1.0 == 1.0) return 0;  // singular: the reference returns the partial matrix
+        for (int j = 1; j <= n; j++) { double t = R(k, j); R(k, j) = R(is[k - 1], j); R(is[k - 1], j) = t; }
+        for (int i = 1; i <= n; i++) { double t = R(i, k); R(i, k) = R(i, js[k - 1]); R(i, js[k - 1]) = t; }
+        R(k, k) = 1.0 / R(k, k);
+        for (int j = 1; j <= n; j++) if (j != k) R(k, j) = R(k, j) * R(k, k);
+        for (int i = 1; i <= n; i++)
+            if (i != k)
+                for (int j = 1; j <= n; j++)
+                    if (j != k) R(i, j) = R(i, j) - R(i, k) * R(k, j);
+        for (int i = 1; i <= n; i++) if (i != k) R(i, k) = -R(i, k) * R(k, k);
+    }
+    for (int k = n; k >= 1; k--) {
+        for (int j = 1; j <= n; j++) { double t = R(k, j); R(k, j) = R(js[k - 1], j); R(js[k - 1], j) = t; }
+        for (int i = 1; i <= n; i++) { double t = R(i, k); R(i, k) = R(i, is[k - 1]); R(i, is[k - 1]) = t; }
+    }
+#undef R
+    return 1;
+}
+
+int get_direction(double xx, double yy, double x, double y) {   // XuanGua.F90:33-43
+    double dx = xx - x, dy = yy - y;
+    double angle = (std::atan2(dy, dx) + DIR_EPS) * 180.0 / PI_REF;
+    int fl = (int)std::floor((angle + 22.5) / 45.0 + 8);
+    return fl % 8 + 1;
+}
+
+namespace {
+
+struct List { int a[16]; int n = 0; };
+
+void add_to(List& l, int c) {                       // XuanGua.F90:11-30
+    for (int i = 0; i < l.n; i++) if (l.a[i] == c) return;
+    l.a[l.n++] = c;
+}
+
+// XuanGua.F90:152-177 (one pass; the reference's 4x repeat adds nothing after the first)
+void search5(const Tree& t, int c, double x, double y, List& l1, List& l2, bool* used) {
+    if (t.son[c] < 0) {
+        int dir = get_direction(t.x[c], t.y[c], x, y);
+        if (!used[dir - 1]) {
+            if (dir == 1 || dir == 5 || dir == 3 || dir == 7) add_to(l1, c);
+            else add_to(l2, c);
+            used[dir - 1] = true;
+        }
+    } else {
+        for (int k = 0; k < 4; k++) search5(t, t.son[c] + k, x, y, l1, l2, used);
+    }
+}
+
+void parent_lists(const Tree& t, int c, List& l1, List& l2) {  // XuanGua.F90:45-86
+    bool used[8] = {false};
+    l1.n = l2.n = 0;
+    for (int k = 0; k < 4; k++) search5(t, t.son[c] + k, t.x[c], t.y[c], l1, l2, used);
+    if (l1.n + l2.n < 8) {
+        const int32_t* q = &t.nb[8 * (size_t)c];
+        const int order[4] = {NB_LEFT, NB_RIGHT, NB_DOWN, NB_UP};
+        for (int k = 0; k < 4; k++)
+            if (q[order[k]] >= 0) search5(t, q[order[k]], t.x[c], t.y[c], l1, l2, used);
+    }
+}
+
+void child_lists(const Tree& t, int c, List& l1, List& l2) {   // XuanGua.F90:96-141
+    bool used[8] = {false};
+    l1.n = l2.n = 0;
+    add_to(l1, c);
+    const int32_t* q = &t.nb[8 * (size_t)c];
+    const int order[8] = {NB_DOWNLEFT, NB_LEFT, NB_UPLEFT, NB_DOWN, NB_UP, NB_DOWNRIGHT, NB_RIGHT, NB_UPRIGHT};
+    for (int k = 0; k < 8; k++)
+        if (q[order[k]] >= 0) search5(t, q[order[k]], t.x[c], t.y[c], l1, l2, used);
+}
+
+// WLS-WENO geometry of one slot (XuanGua.F90:206-255 and the box of :318-323)
+void wls_geometry(const Tree& t, int cell, double xx, double yy, const List& l1, const List& l2,
+                  const std::vector<int32_t>& state_of_node, double* d, int32_t* iv) {
+    int m1 = l1.n, m = l1.n + l2.n;
+    double px[16], py[16], w[16], P[16][6];
+    for (int i = 0; i < m; i++) {
+        int q = i < m1 ? l1.a[i] : l2.a[i - m1];
+        px[i] = t.x[q]; py[i] = t.y[q];
+        double dd = std::sqrt((px[i] - xx) * (px[i] - xx) + (py[i] - yy) * (py[i] - yy));
+        w[i] = 1.0 / (dd + 1e-6);
+        iv[WLSI_IDS + i] = state_of_node[q];
+        d[WLS_PX + i] = px[i]; d[WLS_PY + i] = py[i]; d[WLS_W + i] = w[i];
+        P[i][0] = 1.0; P[i][1] = px[i]; P[i][2] = py[i];
+        P[i][3] = px[i] * py[i]; P[i][4] = px[i] * px[i]; P[i][5] = py[i] * py[i];
+    }
+    iv[WLSI_M] = m; iv[WLSI_M1] = m1;
+    double A6[36], A3a[9], A3b[9];
+    for (int r = 0; r < 6; r++)
+        for (int c = 0; c < 6; c++) {
+            double s = 0.0;
+            for (int i = 0; i < m; i++) s = s + P[i][r] * (w[i] * P[i][c]);
+            A6[c * 6 + r] = s;
+        }
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            double s = 0.0;
+            for (int i = 0; i < m1; i++) s = s + P[i][r] * (w[i] * P[i][c]);
+            A3a[c * 3 + r] = s;
+            s = 0.0;
+            for (int i = m1; i < m; i++) s = s + P[i][r] * (w[i] * P[i][c]);
+            A3b[c * 3 + r] = s;
+        }
+    brinv(6, A6, d + WLS_INV6);
+    brinv(3, A3a, d + WLS_INV3A);
+    brinv(3, A3b, d + WLS_INV3B);
+    d[WLS_XX] = xx; d[WLS_YY] = yy;
+    double hx = t.xf * t.h / std::ldexp(1.0, t.lvl[cell] + 1);
+    double hy = t.h / std::ldexp(1.0, t.lvl[cell] + 1);
+    double x_min = t.x[cell] - hx, x_max = t.x[cell] + hx;
+    double y_min = t.y[cell] - hy, y_max = t.y[cell] + hy;
+    d[WLS_BOX + 0] = x_max - x_min;
+    d[WLS_BOX + 1] = y_max - y_min;
+    d[WLS_BOX + 2] = x_max * x_max - x_min * x_min;
+    d[WLS_BOX + 3] = y_max * y_max - y_min * y_min;
+    d[WLS_BOX + 4] = x_max * x_max * x_max - x_min * x_min * x_min;
+    d[WLS_BOX + 5] = y_max * y_max * y_max - y_min * y_min * y_min;
+}
+
+struct Key {
+    int kind, node;
+    uint64_t xb, yb;
+    bool operator==(const Key& o) const { return kind == o.kind && node == o.node && xb == o.xb && yb == o.yb; }
+};
+struct KeyHash {
+    size_t operator()(const Key& k) const {
+        uint64_t h = (uint64_t)k.node * 0x9E3779B97F4A7C15ULL ^ (uint64_t)k.kind;
+        h ^= k.xb + 0x9E3779B97F4A7C15ULL + (h << 6) + (h >> 2);
+        h ^= k.yb + 0x9E3779B97F4A7C15ULL + (h << 6) + (h >> 2);
+        return (size_t)h;
+    }
+};
+uint64_t bits(double v) { uint64_t b; std::memcpy(&b, &v, 8); return b; }
+
+}  // namespace
+
+void build_tables(const Tree& t, Tables& tb) {
+    const size_t NA = t.n_alloc();
+    tb = Tables();
+    tb.N = (int)t.ns.size();
+    tb.G = (int)t.bcl.size();
+    tb.state_of_node.assign(NA, -1);
+    tb.leaf_node.clear();
+    for (int i = 0; i < tb.N; i++) { tb.state_of_node[t.ns[i]] = i; tb.leaf_node.push_back(t.ns[i]); }
+    for (int i = 0; i < tb.G; i++) { tb.state_of_node[t.bcl[i]] = tb.N + i; tb.leaf_node.push_back(t.bcl[i]); }
+    // passive leaves: corners (never updated, solution.F90:644,682-686)
+    for (size_t c = 0; c < NA; c++)
+        if (t.father[c] != -2 && t.son[c] < 0 && tb.state_of_node[c] < 0) {
+            tb.state_of_node[c] = (int)tb.leaf_node.size();
+            tb.leaf_node.push_back((int)c);
+        }
+    tb.L = (int)tb.leaf_node.size();
+    tb.P = tb.L - tb.N - tb.G;
+    tb.node_son.assign(t.son.begin(), t.son.end());
+
+    const int N = tb.N;
+    tb.stencil.assign(8 * (size_t)N, -1);
+    tb.lvl_leaf.assign(N, 0);
+    tb.st_kind.assign(8 * (size_t)N, 0);
+    tb.st_node.assign(8 * (size_t)N, -1);
+    tb.st_px.assign(8 * (size_t)N, 0.0);
+    tb.st_py.assign(8 * (size_t)N, 0.0);
+    tb.st_m1.assign(8 * (size_t)N, 0);
+    tb.st_m2.assign(8 * (size_t)N, 0);
+    tb.st_nb.assign(8 * 9 * (size_t)N, -1);
+
+    std::unordered_map<Key, int, KeyHash> wls_map, rst_map;
+    std::vector<std::vector<double>> wd;
+    std::vector<std::vector<int32_t>> wi;
+    int lmax = 0;
+
+    for (int i = 0; i < N; i++) {
+        const int c = t.ns[i];
+        tb.lvl_leaf[i] = (int8_t)t.lvl[c];
+        if (t.lvl[c] > lmax) lmax = t.lvl[c];
+        const int32_t* q = &t.nb[8 * (size_t)c];
+        for (int k = 0; k < 8; k++) {
+            // Tip/Tim/Tjp/Tjm (k<4) and Tipp/Timm/Tjpp/Tjmm (k>=4), XuanGua.F90:401-590
+            const int dir = k & 3;
+            const bool two = k >= 4;
+            const int nbk = dir == 0 ? NB_RIGHT : dir == 1 ? NB_LEFT : dir == 2 ? NB_UP : NB_DOWN;
+            const int cn = q[nbk];
+            if (cn < 0) throw std::runtime_error("interior leaf without a face neighbour");
+            double off = two ? (dir < 2 ? 2.0 * t.xf * t.h : 2.0 * t.h) / pow2i(t.lvl[c])
+                             : (dir < 2 ? t.xf * t.h : t.h) / pow2i(t.lvl[c]);
+            double xx = t.x[c], yy = t.y[c];
+            if (dir == 0) xx = t.x[c] + off;
+            else if (dir == 1) xx = t.x[c] - off;
+            else if (dir == 2) yy = t.y[c] + off;
+            else yy = t.y[c] - off;
+            int tg;
+            bool same;
+            if (!two) { tg = cn; same = t.lvl[cn] == t.lvl[c]; }
+            else if (t.lvl[cn] == t.lvl[c]) {
+                int cnn = t.nb[8 * (size_t)cn + nbk];
+                if (cnn < 0) throw std::runtime_error("interior leaf without a second neighbour");
+                tg = cnn; same = t.lvl[cnn] == t.lvl[cn];
+            } else { tg = cn; same = false; }
+
+            List l1, l2;
+            int kind, ref;
+            if (same) {
+                if (t.son[tg] < 0) { kind = ST_COPY; ref = tb.state_of_node[tg]; }
+                else {
+                    parent_lists(t, tg, l1, l2);
+                    xx = t.x[tg]; yy = t.y[tg];
+                    if (l1.n + l2.n < 7) {
+                        kind = ST_PARENT_RESTRICT;
+                        Key key{2, tg, 0, 0};
+                        auto it = rst_map.find(key);
+                        int s;
+                        if (it == rst_map.end()) { s = (int)tb.rst_node.size(); rst_map.emplace(key, s); tb.rst_node.push_back(tg); }
+                        else s = it->second;
+                        ref = -2 - s;  // patched to L + S_w + s below
+                    } else {
+                        kind = ST_PARENT_WLS;
+                        Key key{1, tg, 0, 0};
+                        auto it = wls_map.find(key);
+                        int s;
+                        if (it == wls_map.end()) {
+                            s = (int)wd.size(); wls_map.emplace(key, s);
+                            wd.emplace_back(WLS_STRIDE, 0.0); wi.emplace_back(WLSI_STRIDE, -1);
+                            wls_geometry(t, tg, t.x[tg], t.y[tg], l1, l2, tb.state_of_node, wd.back().data(), wi.back().data());
+                        } else s = it->second;
+                        ref = 1000000000 + s;  // patched to L + s below
+                    }
+                }
+            } else {
+                child_lists(t, tg, l1, l2);
+                if (l1.n + l2.n < 7) { kind = ST_CHILD_COPY; ref = tb.state_of_node[tg]; }
+                else {
+                    kind = ST_CHILD_WLS;
+                    Key key{3, tg, bits(xx), bits(yy)};
+                    auto it = wls_map.find(key);
+                    int s;
+                    if (it == wls_map.end()) {
+                        s = (int)wd.size(); wls_map.emplace(key, s);
+                        wd.emplace_back(WLS_STRIDE, 0.0); wi.emplace_back(WLSI_STRIDE, -1);
+                        wls_geometry(t, tg, xx, yy, l1, l2, tb.state_of_node, wd.back().data(), wi.back().data());
+                    } else s = it->second;
+                    ref = 1000000000 + s;
+                }
+            }
+            size_t e = (size_t)k * N + i;
+            tb.stencil[e] = ref;
+            size_t d = (size_t)i * 8 + k;
+            tb.st_kind[d] = (int8_t)kind;
+            tb.st_node[d] = tg;
+            tb.st_px[d] = xx; tb.st_py[d] = yy;
+            tb.st_m1[d] = l1.n; tb.st_m2[d] = l2.n;
+            for (int j = 0; j < l1.n; j++) tb.st_nb[d * 9 + j] = l1.a[j];
+            for (int j = 0; j < l2.n && l1.n + j < 9; j++) tb.st_nb[d * 9 + l1.n + j] = l2.a[j];
+        }
+    }
+    tb.S_w = (int)wd.size();
+    tb.S_r = (int)tb.rst_node.size();
+    for (auto& r : tb.stencil) {
+        if (r >= 1000000000) r = tb.L + (r - 1000000000);
+        else if (r <= -2) r = tb.L + tb.S_w + (-2 - r);
+    }
+    tb.wls_d.resize((size_t)tb.S_w * WLS_STRIDE);
+    tb.wls_i.resize((size_t)tb.S_w * WLSI_STRIDE);
+    for (int s = 0; s < tb.S_w; s++) {
+        std::memcpy(&tb.wls_d[(size_t)s * WLS_STRIDE], wd[s].data(), sizeof(double) * WLS_STRIDE);
+        std::memcpy(&tb.wls_i[(size_t)s * WLSI_STRIDE], wi[s].data(), sizeof(int32_t) * WLSI_STRIDE);
+    }
+    tb.bc_partner.resize(tb.G);
+    for (int g = 0; g < tb.G; g++) tb.bc_partner[g] = t.bc_partner(t.bcl[g]);
+    tb.l_max = lmax;
+
+    // every index the kernels dereference is checked here, on the host
+    const long ext = (long)tb.L + tb.S_w + tb.S_r;
+    for (int32_t r : tb.stencil)
+        if (r < 0 || r >= ext) throw std::runtime_error("stencil index out of range");
+    for (int s = 0; s < tb.S_w; s++) {
+        const int32_t* iv = &tb.wls_i[(size_t)s * WLSI_STRIDE];
+        if (iv[WLSI_M] < 1 || iv[WLSI_M] > 9 || iv[WLSI_M1] < 0 || iv[WLSI_M1] > iv[WLSI_M])
+            throw std::runtime_error("WLS stencil size out of range");
+        for (int i = 0; i < iv[WLSI_M]; i++)
+            if (iv[WLSI_IDS + i] < 0 || iv[WLSI_IDS + i] >= tb.L) throw std::runtime_error("WLS id out of range");
+    }
+    for (int32_t q : tb.rst_node)
+        if (q < 0 || (size_t)q >= NA || t.son[q] < 0) throw std::runtime_error("restriction node invalid");
+    for (int32_t q : tb.bc_partner)
+        if (q >= 0 && ((size_t)q >= NA || t.father[q] == -2 || (t.son[q] < 0 && tb.state_of_node[q] < 0)))
+            throw std::runtime_error("boundary partner invalid");
+    for (size_t c = 0; c < NA; c++) {
+        if (t.father[c] == -2) continue;
+        if (t.son[c] >= 0 && (size_t)t.son[c] + 3 >= NA) throw std::runtime_error("son index out of range");
+        if (t.son[c] < 0 && tb.state_of_node[c] < 0) throw std::runtime_error("leaf without state slot");
+    }
+}
+
+}  // namespace fdm

@@ -1,0 +1,64 @@
+// host_tables.hpp -- flattening of the quadtree into the device tables of the hot path.
+//
+// The reference resolves every stencil value afresh in every stage by walking cached
+// neighbour pointers (Tip..Tjmm, XuanGua.F90:401-590) and, at refinement-level jumps, by
+// gathering a WLS-WENO stencil and inverting three normal-equation matrices
+// (XuanGua.F90:45-356). All of that depends only on the mesh, so it is done once per regrid
+// here: each interior leaf gets eight state indices; each distinct level-jump value becomes a
+// "slot" whose geometry (weights, inverted matrices, smoothness box) is precomputed in FP64.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "host_tree.hpp"
+
+namespace fdm {
+
+// per-slot record layout (doubles), shared with the device code
+enum WlsLayout {
+    WLS_INV6 = 0,    // 36: inverse of P^T W P (6x6), column-major
+    WLS_INV3A = 36,  // 9: inverse for list 1 (3x3)
+    WLS_INV3B = 45,  // 9: inverse for list 2 (3x3)
+    WLS_PX = 54,     // 9 stencil x
+    WLS_PY = 63,     // 9 stencil y
+    WLS_W = 72,      // 9 inverse-distance weights
+    WLS_XX = 81, WLS_YY = 82,
+    WLS_BOX = 83,    // 6: dxb, dyb, x2, y2, x3, y3
+    WLS_STRIDE = 96
+};
+enum WlsIntLayout { WLSI_M = 0, WLSI_M1 = 1, WLSI_IDS = 2, WLSI_STRIDE = 12 };
+
+// stencil classification, identical to the reference's gathers
+enum StKind { ST_COPY = 0, ST_PARENT_WLS = 1, ST_PARENT_RESTRICT = 2, ST_CHILD_WLS = 3, ST_CHILD_COPY = 4 };
+
+struct Tables {
+    int N = 0, G = 0, P = 0, L = 0;    // interior, ghost, passive (corner) leaves, L = N+G+P
+    int S_w = 0, S_r = 0;              // WLS slots, restriction slots
+    int l_max = 0;
+    std::vector<int32_t> state_of_node;  // node -> state index (leaves) or -1
+    std::vector<int32_t> leaf_node;      // state index -> node
+    std::vector<int32_t> stencil;        // [8][N]: state index, or L + slot index
+    std::vector<int8_t> lvl_leaf;        // [N]
+    // diagnostics (host only)
+    std::vector<int8_t> st_kind;         // [N*8]
+    std::vector<int32_t> st_node;        // [N*8]
+    std::vector<double> st_px, st_py;    // [N*8]
+    std::vector<int32_t> st_m1, st_m2;   // [N*8]
+    std::vector<int32_t> st_nb;          // [N*8*9] node ids
+    // slots
+    std::vector<double> wls_d;           // [S_w * WLS_STRIDE]
+    std::vector<int32_t> wls_i;          // [S_w * WLSI_STRIDE]
+    std::vector<int32_t> rst_node;       // [S_r]
+    // ghost fill: partner node per ghost (-1: inflow state)
+    std::vector<int32_t> bc_partner;     // [G]
+    // device topology for restriction means
+    std::vector<int32_t> node_son;       // [n_alloc]
+};
+
+void build_tables(const Tree& t, Tables& tb);
+
+// reference helpers reused by diagnostics
+int brinv(int n, const double* a, double* r);                   // XuanGua.F90:593-660
+int get_direction(double xx, double yy, double x, double y);    // XuanGua.F90:33-43
+
+}  // namespace fdm

@@ -1,0 +1,100 @@
+// host_tree.hpp -- host-side quadtree of the MI355X solver library.
+//
+// The reference keeps a pointer-linked quadtree (type(grid), module.F90:24-35). Here the
+// tree is a structure of arrays indexed by node id; the four sons of a node occupy four
+// consecutive ids (son1..son4 = SW, SE, NE, NW, mesh.F90:204-248). The tree lives on the
+// host and changes only at regrid time; the per-step hot path runs on flattened device
+// tables built from it (host_tables.cpp).
+#pragma once
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace fdm {
+
+enum Nb { NB_UP = 0, NB_DOWN, NB_RIGHT, NB_LEFT, NB_UPLEFT, NB_UPRIGHT, NB_DOWNLEFT, NB_DOWNRIGHT };
+
+struct Tree {
+    // ---- configuration (set_parameter, MAIN.F90:29-71) ----
+    double h = 0.5, xf = 1.0;
+    double Xmin = 0, Xmax = 0, Ymin = 0, Ymax = 0;  // ghost-extended (MAIN.F90:55-56)
+    double pxmin = 0, pxmax = 0, pymin = 0, pymax = 0;  // physical domain (MAIN.F90:35-36)
+    double gamma = 1.4, Ma = 1.0, cv = 0;
+    int bc[4] = {6, 6, 6, 6};                       // xm, xp, ym, yp
+    double inflow[4] = {0, 0, 0, 0};
+    int m = 0, n = 0, total = 0;
+
+    // ---- node arrays ----
+    std::vector<int32_t> father, son, lvl, sp, ni, nj, cross;
+    std::vector<double> x, y;
+    std::vector<std::array<double, 7>> U;           // U(1:7) of every node
+    std::vector<double> rot, div;
+    std::vector<int32_t> nb;                        // 8 cached neighbours per node (idnei)
+    std::vector<int32_t> free_blocks;               // first ids of released son blocks
+
+    // ---- leaf lists (IDgrid) ----
+    std::vector<int32_t> ns, bcl;                   // cell_NS, cell_BC
+
+    void configure(double xmin, double xmax, double ymin, double ymax, double h_, double gamma_,
+                   double Ma_);
+    void creat_mesh();                              // mesh.F90:9-47
+    int gridcross(int ni_, int nj_) const;          // mesh.F90:89-142
+    bool is_leaf(int c) const { return son[c] < 0; }
+    size_t n_alloc() const { return father.size(); }
+
+    void refine(int c);                             // mesh.F90:199-260
+    void coarse(int c);                             // mesh.F90:397-408
+    void release_subtree(int c);
+
+    // neighbour finders (neighbor.F90:6-396); -1 = null
+    int init_up(int c) const;
+    int init_down(int c) const;
+    int init_left(int c) const;
+    int init_right(int c) const;
+    int up(int c) const;
+    int down(int c) const;
+    int left(int c) const;
+    int right(int c) const;
+    int upleft(int c) const;
+    int upright(int c) const;
+    int downright(int c) const;
+    int downleft(int c) const;
+    void idneighbor();                              // neighbor.F90:399-449
+    void idgrid();                                  // solution.F90:593-693
+
+    void restrict_mean(int c, double* var7) const;  // interpA, XuanGua.F90:359-398
+    int bc_partner(int g) const;                    // BC_cyc partner after the son-path descent
+    int bc_kind_of(int cross_) const;
+
+    // canonical numbering: base cells, then son blocks in a block pre-order walk
+    std::vector<int32_t> canonical_ids(int32_t* n_out) const;
+
+    // ---- regrid ----
+    // solute_ref (solution.F90:468-535) on per-node rot/div; returns nothing
+    void solute_ref(double arot, double adiv, int lvl_amr, double rg_rot, double rg_div,
+                    double cg_rot, double cg_div);
+    void gridmodify1(int c);                        // mesh.F90:305-353
+    void gridmodify4(int c);                        // mesh.F90:355-395
+    void solute_gridmodify(int lvl_amr);            // solution.F90:537-556
+    // order in which rot_div visits interior leaves (solution.F90:433-465)
+    void rot_div_order(std::vector<int32_t>& out) const;
+
+    // ---- initial conditions (host, driver support) ----
+    void initial(int ic_kind, uint64_t seed, const double* ic_param8);
+    void boundary_fill_host();                      // initial_boundary (solution.F90:71-107)
+
+   private:
+    int alloc_block();
+    void ns_walk(int c, bool fill, int& cnt);
+    void bc_walk(int c, bool fill, int& cnt);
+    void idnei(int c);
+    void coarse_sub(int c, double arot, double adiv, int lvl_amr, double cg_rot, double cg_div);
+    void ref_sub(int c, double arot, double adiv, int lvl_amr, double rg_rot, double rg_div);
+    void rot_div_walk(int c, std::vector<int32_t>& out) const;
+};
+
+// SplitMix64 shared by the IC generators
+uint64_t splitmix64(uint64_t& s);
+
+}  // namespace fdm

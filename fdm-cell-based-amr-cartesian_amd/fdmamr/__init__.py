@@ -1,0 +1,356 @@
+"""ctypes binding of libfdmamr (include/fdmamr.h) and a small driver mirroring the reference.
+
+`Solver` follows the reference driver's call order (MAIN.F90:8-22 and solute,
+solution.F90:171-238): creat_mesh -> initial -> IDgrid/table flattening -> the step loop
+(lambda, dt, Time_advance_RK, residual) with AMR every `pstep_adapt` steps. Every numerical
+step runs in the HIP library; this module only moves arguments. The library must be built
+(build.py); loading fails loudly otherwise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "lib", "libfdmamr.so")
+
+BC_TRANSMISSIVE, BC_INFLOW, BC_PERIODIC = 1, 2, 6
+FLUX_FP64, FLUX_FP32 = 0, 1
+IC_VORTEX, IC_RIEMANN4, IC_SHOCK_BUBBLES, IC_DENSITY_WAVE = 0, 1, 2, 3
+GAMMA_REF = float(np.float32(1.4))  # the reference's REAL(4) literal (MAIN.F90:38)
+
+# every symbol include/fdmamr.h declares
+EXPORTS = (
+    "fdmamr_create", "fdmamr_destroy", "fdmamr_last_error", "fdmamr_creat_mesh",
+    "fdmamr_initial", "fdmamr_tree_import", "fdmamr_tree_export", "fdmamr_tree_size",
+    "fdmamr_build_tables", "fdmamr_counts", "fdmamr_set_state", "fdmamr_get_state",
+    "fdmamr_get_ghost_state", "fdmamr_leaf_geometry", "fdmamr_max_wavespeed",
+    "fdmamr_advance_rk3", "fdmamr_run_steps", "fdmamr_check_positivity", "fdmamr_sensors",
+    "fdmamr_regrid", "fdmamr_stencil_info", "fdmamr_stencil_values", "fdmamr_profile",
+    "fdmamr_profile_read",
+)
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("xmin", C.c_double), ("xmax", C.c_double), ("ymin", C.c_double), ("ymax", C.c_double),
+        ("h", C.c_double), ("gamma", C.c_double), ("Ma", C.c_double),
+        ("bc_xm", C.c_int32), ("bc_xp", C.c_int32), ("bc_ym", C.c_int32), ("bc_yp", C.c_int32),
+        ("inflow", C.c_double * 4),
+        ("flux_precision", C.c_int32), ("selector", C.c_int32), ("device", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class FdmError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libfdmamr.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FdmError(f"{LIB_PATH} not built: run python fdm-cell-based-amr-cartesian_amd/build.py")
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int32)
+    sig = {
+        "fdmamr_create": ([C.POINTER(Params), C.POINTER(P)], C.c_int32),
+        "fdmamr_destroy": ([P], None),
+        "fdmamr_last_error": ([P, C.c_char_p, C.c_int32], C.c_int32),
+        "fdmamr_creat_mesh": ([P], C.c_int32),
+        "fdmamr_initial": ([P, C.c_int32, C.c_uint64, dp], C.c_int32),
+        "fdmamr_tree_import": ([P, C.c_int32, C.c_int32, C.c_int32, ip, ip, ip, ip, ip, ip, dp, dp, dp], C.c_int32),
+        "fdmamr_tree_export": ([P, ip, ip, ip, ip, ip, ip, ip, dp, dp, dp], C.c_int32),
+        "fdmamr_tree_size": ([P, ip], C.c_int32),
+        "fdmamr_build_tables": ([P], C.c_int32),
+        "fdmamr_counts": ([P, ip, ip, ip, ip, ip], C.c_int32),
+        "fdmamr_set_state": ([P, dp], C.c_int32),
+        "fdmamr_get_state": ([P, dp], C.c_int32),
+        "fdmamr_get_ghost_state": ([P, dp], C.c_int32),
+        "fdmamr_leaf_geometry": ([P, dp, dp, ip, ip], C.c_int32),
+        "fdmamr_max_wavespeed": ([P, dp, dp, ip], C.c_int32),
+        "fdmamr_advance_rk3": ([P, C.c_double, C.c_double, C.c_double, dp, dp], C.c_int32),
+        "fdmamr_run_steps": ([P, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, dp, dp, dp], C.c_int32),
+        "fdmamr_check_positivity": ([P, ip], C.c_int32),
+        "fdmamr_sensors": ([P, dp, dp], C.c_int32),
+        "fdmamr_regrid": ([P, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32], C.c_int32),
+        "fdmamr_stencil_info": ([P, C.c_int32, ip, ip, dp, dp, ip, ip, ip], C.c_int32),
+        "fdmamr_stencil_values": ([P, C.c_int32, dp], C.c_int32),
+        "fdmamr_profile": ([P, C.c_int32], C.c_int32),
+        "fdmamr_profile_read": ([P, dp, ip], C.c_int32),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32)) if a is not None else None
+
+
+@dataclass
+class Config:
+    """Solver configuration; defaults follow set_parameter (MAIN.F90:29-71)."""
+    xmin: float = 0.0
+    xmax: float = 10.0
+    ymin: float = 0.0
+    ymax: float = 10.0
+    h: float = 0.5
+    gamma: float = GAMMA_REF
+    Ma: float = 1.0
+    bc: tuple = (BC_PERIODIC,) * 4          # xm, xp, ym, yp
+    inflow: tuple = (0.0, 0.0, 0.0, 0.0)
+    flux_precision: int = FLUX_FP64
+    selector: int = 0
+    device: int = 0
+    ic_kind: int = IC_VORTEX
+    seed: int = 0
+    ic_param: tuple = (0.0,) * 8
+    # time loop (solute)
+    tstep: float = float(np.float32(1e-4))
+    cfl: float = float(np.float32(0.45))
+    t_end: float = 10.0
+    localdt: int = 0
+    # AMR
+    amr: int = 0
+    lvl_amr: int = 0
+    pstep_adapt: int = 30
+    nstep_adapt: int = 0
+    refine_goal_rot: float = GAMMA_REF
+    refine_goal_div: float = GAMMA_REF
+    coarse_goal_rot: float = float(np.float32(0.4))
+    coarse_goal_div: float = float(np.float32(0.4))
+    amr_sensor: int = 0
+    pre_adapt: int = 0
+
+
+class Solver:
+    """One solver context on one GPU (one HIP stream)."""
+
+    def __init__(self, cfg: Config):
+        self.L = lib()
+        self.cfg = cfg
+        p = Params()
+        p.xmin, p.xmax, p.ymin, p.ymax = cfg.xmin, cfg.xmax, cfg.ymin, cfg.ymax
+        p.h, p.gamma, p.Ma = cfg.h, cfg.gamma, cfg.Ma
+        p.bc_xm, p.bc_xp, p.bc_ym, p.bc_yp = cfg.bc
+        for i, v in enumerate(cfg.inflow):
+            p.inflow[i] = v
+        p.flux_precision, p.selector, p.device = cfg.flux_precision, cfg.selector, cfg.device
+        self.ctx = C.c_void_p()
+        rc = self.L.fdmamr_create(C.byref(p), C.byref(self.ctx))
+        if rc != 0:
+            raise FdmError(f"fdmamr_create failed ({rc})")
+        self.tt = 0.0
+        self.it = 0
+        self.lam = np.zeros(2)
+        self.tstep = cfg.tstep
+        self.res_max = np.zeros(4)
+        self.res_rms = np.zeros(4)
+
+    # -- plumbing --
+    def _check(self, rc, what):
+        if rc != 0:
+            buf = C.create_string_buffer(512)
+            self.L.fdmamr_last_error(self.ctx, buf, 512)
+            raise FdmError(f"{what} failed ({rc}): {buf.value.decode()}")
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.L.fdmamr_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- driver (MAIN.F90:18-22) --
+    def setup(self):
+        """creat_mesh, initial, optional IC pre-adaptation, IDgrid + tables."""
+        cfg = self.cfg
+        self._check(self.L.fdmamr_creat_mesh(self.ctx), "creat_mesh")
+        self.initial()
+        self.build_tables()
+        for _ in range(cfg.pre_adapt):
+            self.regrid()
+            self.initial()
+            self.build_tables()
+        return self
+
+    def initial(self):
+        cfg = self.cfg
+        prm = np.array(list(cfg.ic_param) + [0.0] * (8 - len(cfg.ic_param)), np.float64)
+        self._check(self.L.fdmamr_initial(self.ctx, cfg.ic_kind, cfg.seed, _dp(prm)), "initial")
+
+    def build_tables(self):
+        self._check(self.L.fdmamr_build_tables(self.ctx), "build_tables")
+
+    def counts(self):
+        a = [C.c_int32() for _ in range(5)]
+        self._check(self.L.fdmamr_counts(self.ctx, *[C.byref(x) for x in a]), "counts")
+        return dict(n_leaf=a[0].value, n_ghost=a[1].value, n_slot_wls=a[2].value,
+                    n_slot_restrict=a[3].value, l_max=a[4].value)
+
+    @property
+    def n_leaf(self):
+        return self.counts()["n_leaf"]
+
+    # -- state --
+    def get_state(self):
+        out = np.zeros((self.n_leaf, 5))
+        self._check(self.L.fdmamr_get_state(self.ctx, _dp(out)), "get_state")
+        return out
+
+    def get_ghost_state(self):
+        out = np.zeros((self.counts()["n_ghost"], 5))
+        self._check(self.L.fdmamr_get_ghost_state(self.ctx, _dp(out)), "get_ghost_state")
+        return out
+
+    def set_state(self, prim):
+        prim = np.ascontiguousarray(prim, np.float64)
+        self._check(self.L.fdmamr_set_state(self.ctx, _dp(prim)), "set_state")
+
+    def geometry(self):
+        n = self.n_leaf
+        x = np.zeros(n); y = np.zeros(n)
+        lvl = np.zeros(n, np.int32); node = np.zeros(n, np.int32)
+        self._check(self.L.fdmamr_leaf_geometry(self.ctx, _dp(x), _dp(y), _ip(lvl), _ip(node)),
+                    "leaf_geometry")
+        return dict(x=x, y=y, lvl=lvl, node=node)
+
+    def tree(self):
+        nn = C.c_int32()
+        self._check(self.L.fdmamr_tree_size(self.ctx, C.byref(nn)), "tree_size")
+        n = nn.value
+        t = dict(father=np.zeros(n, np.int32), son=np.zeros(n, np.int32),
+                 lvl=np.zeros(n, np.int32), sp=np.zeros(n, np.int32),
+                 num=np.zeros((n, 2), np.int32), cross=np.zeros(n, np.int32),
+                 x=np.zeros(n), y=np.zeros(n), prim7=np.zeros((n, 7)))
+        self._check(self.L.fdmamr_tree_export(self.ctx, C.byref(nn), _ip(t["father"]), _ip(t["son"]),
+                                              _ip(t["lvl"]), _ip(t["sp"]), _ip(t["num"]),
+                                              _ip(t["cross"]), _dp(t["x"]), _dp(t["y"]),
+                                              _dp(t["prim7"])), "tree_export")
+        return t
+
+    def import_tree(self, m, n, t):
+        self._check(self.L.fdmamr_tree_import(
+            self.ctx, m, n, len(t["father"]), _ip(np.ascontiguousarray(t["father"], np.int32)),
+            _ip(np.ascontiguousarray(t["son"], np.int32)), _ip(np.ascontiguousarray(t["lvl"], np.int32)),
+            _ip(np.ascontiguousarray(t["sp"], np.int32)), _ip(np.ascontiguousarray(t["num"], np.int32)),
+            _ip(np.ascontiguousarray(t["cross"], np.int32)), _dp(np.ascontiguousarray(t["x"], np.float64)),
+            _dp(np.ascontiguousarray(t["y"], np.float64)),
+            _dp(np.ascontiguousarray(t["prim7"], np.float64))), "tree_import")
+
+    # -- hot path --
+    def max_wavespeed(self):
+        a, b, l = C.c_double(), C.c_double(), C.c_int32()
+        self._check(self.L.fdmamr_max_wavespeed(self.ctx, C.byref(a), C.byref(b), C.byref(l)),
+                    "max_wavespeed")
+        return a.value, b.value, l.value
+
+    def advance_rk3(self, dt, lam_x, lam_y):
+        rm = np.zeros(4); rs = np.zeros(4)
+        self._check(self.L.fdmamr_advance_rk3(self.ctx, dt, lam_x, lam_y, _dp(rm), _dp(rs)),
+                    "advance_rk3")
+        return rm, rs
+
+    def step(self):
+        """One pass of the solute loop body (solution.F90:183-230), host-synchronous."""
+        cfg = self.cfg
+        self.it += 1
+        lx, ly, lmax = self.max_wavespeed()
+        self.lam[0] = max(self.lam[0], lx)
+        self.lam[1] = max(self.lam[1], ly)
+        if cfg.localdt:
+            p2 = float(2 ** lmax)
+            self.tstep = cfg.cfl / (self.lam[0] / (1.0 * cfg.h / p2) + self.lam[1] / (cfg.h / p2))
+        if self.tt + self.tstep > cfg.t_end:
+            self.tstep = cfg.t_end - self.tt
+        rm, rs = self.advance_rk3(self.tstep, self.lam[0], self.lam[1])
+        self.tt = self.tt + self.tstep
+        self.res_max = rm
+        self.res_rms = np.sqrt(rs / (1.0 * self.n_leaf))
+        if np.isnan(self.res_rms).any():
+            raise FdmError(f"NaN residual at it={self.it}")
+        if cfg.amr and self.it >= cfg.nstep_adapt and self.it % cfg.pstep_adapt == 1:
+            self.regrid()
+
+    def run_steps(self, n):
+        """n steps without AMR and without host synchronisation inside (device dt)."""
+        cfg = self.cfg
+        tt = np.array([self.tt]); lam = self.lam.copy(); sc = np.zeros(8)
+        self._check(self.L.fdmamr_run_steps(self.ctx, n, cfg.localdt, cfg.cfl, cfg.tstep, cfg.t_end,
+                                            _dp(tt), _dp(lam), _dp(sc)), "run_steps")
+        self.tt = float(tt[0]); self.lam = lam; self.it += n
+        self.tstep = sc[1]
+        self.res_rms = sc[4:8].copy()
+        return sc
+
+    def regrid(self):
+        cfg = self.cfg
+        self._check(self.L.fdmamr_regrid(self.ctx, cfg.lvl_amr, cfg.refine_goal_rot,
+                                         cfg.refine_goal_div, cfg.coarse_goal_rot,
+                                         cfg.coarse_goal_div, cfg.amr_sensor), "regrid")
+
+    def sensors(self):
+        n = self.n_leaf
+        rot = np.zeros(n); div = np.zeros(n)
+        self._check(self.L.fdmamr_sensors(self.ctx, _dp(rot), _dp(div)), "sensors")
+        return rot, div
+
+    def check_positivity(self):
+        nb = C.c_int32()
+        rc = self.L.fdmamr_check_positivity(self.ctx, C.byref(nb))
+        if rc not in (0, -4):
+            self._check(rc, "check_positivity")
+        return nb.value
+
+    # -- measurement --
+    def profile(self, enable=True):
+        self._check(self.L.fdmamr_profile(self.ctx, 1 if enable else 0), "profile")
+
+    def profile_read(self):
+        ms = C.c_double(); n = C.c_int32()
+        self._check(self.L.fdmamr_profile_read(self.ctx, C.byref(ms), C.byref(n)), "profile_read")
+        return ms.value, n.value
+
+    # -- diagnostics --
+    def stencil_info(self, i):
+        kind = np.zeros(8, np.int32); node = np.zeros(8, np.int32)
+        px = np.zeros(8); py = np.zeros(8); m1 = np.zeros(8, np.int32); m2 = np.zeros(8, np.int32)
+        nb = np.zeros(72, np.int32)
+        self._check(self.L.fdmamr_stencil_info(self.ctx, i, _ip(kind), _ip(node), _dp(px), _dp(py),
+                                               _ip(m1), _ip(m2), _ip(nb)), "stencil_info")
+        return dict(kind=kind, node=node, px=px, py=py, m1=m1, m2=m2, nb=nb.reshape(8, 9))
+
+    def stencil_values(self, i):
+        out = np.zeros((8, 4))
+        self._check(self.L.fdmamr_stencil_values(self.ctx, i, _dp(out)), "stencil_values")
+        return out
+
+
+def config_c1(**kw) -> Config:
+    """Isentropic vortex, periodic [0,10]^2, base 64x64, lvlAMR=2, CFL 0.5 local dt."""
+    d = dict(h=10.0 / 64, localdt=1, cfl=0.5, amr=1, lvl_amr=2, pstep_adapt=30, t_end=1.0e9)
+    d.update(kw)
+    return Config(**d)

@@ -1,0 +1,162 @@
+/*
+ * fdmamr.h -- C-ABI of the MI355X (gfx950) hot path of the cell-based quadtree AMR
+ * finite-difference Euler solver (reference: Jian-cd/FDM-cell-based-AMR-Cartesian, Fortran).
+ *
+ * The reference has no FFI: its hot path is the argument-less module procedure
+ * Time_advance_RK (solution.F90:267-298) working on module globals and a pointer quadtree
+ * (module.F90:24-42). This header is the boundary that replaces it. A Fortran driver binds
+ * it with ISO_C_BINDING (fdm-cell-based-amr-cartesian_amd/fortran/fdmamr_c.f90); Python
+ * binds it with ctypes (fdm-cell-based-amr-cartesian_amd/fdmamr/).
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success and a negative code on failure; the message is
+ *    available from fdmamr_last_error. Codes: -1 invalid argument, -2 HIP error,
+ *    -3 NaN residual (the reference stops, solution.F90:209-214), -4 positivity failure,
+ *    -5 no device / library built without a usable GPU.
+ *  - Host arrays are borrowed for the duration of the call only. Device memory and the
+ *    context are owned by the library. One context = one HIP stream; not thread-safe per
+ *    context; call from the driver's serial part (as the reference calls Time_advance_RK).
+ *  - Leaf order is the reference's: interior leaves in cell_NS order (base cells row-major,
+ *    sons son1,son2,son4,son3; solution.F90:593-670), ghost leaves in cell_BC order
+ *    (solution.F90:672-693).
+ *  - State arrays are primitive (rho,u,v,T,p) as in U(1:5) (module.F90:32), AoS: prim[5*i+k].
+ */
+#ifndef FDMAMR_H
+#define FDMAMR_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fdmamr_ctx fdmamr_ctx;
+
+/* boundary kinds: 6 periodic (the reference's BCcyc, module.F90:7); 1 transmissive and
+ * 2 fixed inflow state are new (no reference counterpart) */
+#define FDMAMR_BC_TRANSMISSIVE 1
+#define FDMAMR_BC_INFLOW 2
+#define FDMAMR_BC_PERIODIC 6
+
+#define FDMAMR_DEVICE_HOST_ONLY (-1)
+
+/* flux precision modes */
+#define FDMAMR_FLUX_FP64 0   /* plain method, reference arithmetic */
+#define FDMAMR_FLUX_FP32 1   /* WCNS reconstruction + LF flux in FP32, RK update in FP64 */
+
+/* initial conditions (host-side, driver support): 0 isentropic vortex (solution.F90:31-69),
+ * 1 seeded 4-quadrant Riemann, 2 shock + seeded bubbles, 3 density wave (1..3 are new) */
+#define FDMAMR_IC_VORTEX 0
+#define FDMAMR_IC_RIEMANN4 1
+#define FDMAMR_IC_SHOCK_BUBBLES 2
+#define FDMAMR_IC_DENSITY_WAVE 3
+
+typedef struct fdmamr_params {
+    /* physical domain before the 3-cell ghost extension (MAIN.F90:35-36,55-56) */
+    double xmin, xmax, ymin, ymax;
+    double h;                 /* base spacing (MAIN.F90:37) */
+    double gamma;             /* MAIN.F90:38 */
+    double Ma;                /* MAIN.F90:61, cv = 1/(gamma (gamma-1) Ma^2) */
+    int32_t bc_xm, bc_xp, bc_ym, bc_yp;  /* kind_xmbc, kind_xpbc, kind_ymbc, kind_ypbc (MAIN.F90:48) */
+    double inflow[4];         /* rho,u,v,p for FDMAMR_BC_INFLOW */
+    int32_t flux_precision;   /* FDMAMR_FLUX_FP64 / FDMAMR_FLUX_FP32 */
+    int32_t selector;         /* 0 off; 1 learned troubled-cell selector (FP32 mode only) */
+    int32_t device;           /* HIP device ordinal, or FDMAMR_DEVICE_HOST_ONLY for a context
+                                 that only builds meshes and tables (no device calls) */
+    int32_t reserved;
+} fdmamr_params;
+
+/* ---- lifecycle ---- */
+/* replaces set_parameter's derived quantities (MAIN.F90:55-61) and allocates a stream */
+int32_t fdmamr_create(const fdmamr_params* p, fdmamr_ctx** out);
+void    fdmamr_destroy(fdmamr_ctx* ctx);
+int32_t fdmamr_last_error(const fdmamr_ctx* ctx, char* buf, int32_t len);
+
+/* ---- mesh (host quadtree owned by the context; regrid-time, not per step) ---- */
+/* creat_mesh (mesh.F90:9-47) + idneighbor (neighbor.F90:399-409) */
+int32_t fdmamr_creat_mesh(fdmamr_ctx* ctx);
+/* initial (solution.F90:9-29): host IC on every leaf, then initial_boundary. ic_param as in
+ * the driver's config (vortex: none; Riemann: split x,y; shock: x_s, Mach, n_bubble,
+ * r_min, r_max, ratio_min, ratio_max) */
+int32_t fdmamr_initial(fdmamr_ctx* ctx, int32_t ic_kind, uint64_t seed, const double* ic_param8);
+/* Replace the whole quadtree with a caller-flattened one (for drivers that own their tree):
+ * n_node nodes; base cells are nodes 0..m*n-1 in the reference's cell(i) order; son[k] is the
+ * id of son1 of node k (son1..son4 consecutive) or -1; lvl/sp/num/cross as type(grid)
+ * (module.F90:24-35); prim7 = U(1:7) of every node. Rebuilds neighbour caches. */
+int32_t fdmamr_tree_import(fdmamr_ctx* ctx, int32_t m, int32_t n, int32_t n_node,
+                           const int32_t* father, const int32_t* son, const int32_t* lvl,
+                           const int32_t* sp, const int32_t* num2, const int32_t* cross,
+                           const double* x, const double* y, const double* prim7);
+/* Export in canonical numbering (base cells, then son blocks in a pre-order walk). Any
+ * pointer may be NULL. Returns the node count through n_node. */
+int32_t fdmamr_tree_export(fdmamr_ctx* ctx, int32_t* n_node, int32_t* father, int32_t* son,
+                           int32_t* lvl, int32_t* sp, int32_t* num2, int32_t* cross, double* x,
+                           double* y, double* prim7);
+int32_t fdmamr_tree_size(fdmamr_ctx* ctx, int32_t* n_node);
+
+/* IDgrid (solution.F90:593-693) + flattening of the neighbour tables into device-resident
+ * leaf arrays: stencil ids for +-1,+-2 in x and y, WLS-WENO slot tables (FP64 coefficient
+ * setup, XuanGua.F90:179-255), restriction and boundary tables. Uploads the host state. */
+int32_t fdmamr_build_tables(fdmamr_ctx* ctx);
+int32_t fdmamr_counts(fdmamr_ctx* ctx, int32_t* n_leaf, int32_t* n_ghost, int32_t* n_slot_wls,
+                      int32_t* n_slot_restrict, int32_t* l_max);
+
+/* ---- state transfer (at regrid / output only) ---- */
+int32_t fdmamr_set_state(fdmamr_ctx* ctx, const double* prim5_leaf);   /* interior leaves */
+int32_t fdmamr_get_state(fdmamr_ctx* ctx, double* prim5_leaf);         /* interior leaves */
+int32_t fdmamr_get_ghost_state(fdmamr_ctx* ctx, double* prim5_ghost);  /* cell_BC leaves */
+/* leaf geometry: centre x,y and level, interior leaves; canonical node ids optional */
+int32_t fdmamr_leaf_geometry(fdmamr_ctx* ctx, double* x, double* y, int32_t* lvl, int32_t* node);
+
+/* ---- the hot path ---- */
+/* lambda/l_max loop of solute (solution.F90:185-193): max over interior leaves of |u|+c and
+ * |v|+c. The caller keeps the running maximum, as the reference does. */
+int32_t fdmamr_max_wavespeed(fdmamr_ctx* ctx, double* lam_x, double* lam_y, int32_t* l_max);
+/* Time_advance_RK (solution.F90:267-298): three SSP-RK3 stages, each = level-jump slot
+ * values (WLS-WENO / restriction), fused WCNS3 x+y flux derivative + forward-Euler substep +
+ * primitive RK blend (nnd_center_RK.F90:15-112, solution.F90:386-404), then the ghost fill
+ * (solution.F90:288-293). res_max/res_rms_sum receive Res_max(1:4) and Res_rms(1:4) of the
+ * last stage before the sqrt (nnd_center_RK.F90:31-36). */
+int32_t fdmamr_advance_rk3(fdmamr_ctx* ctx, double dt, double lam_x, double lam_y,
+                           double* res_max4, double* res_rms_sum4);
+/* n passes of the solute loop body without AMR (solution.F90:183-215) kept on the device:
+ * running-max lambda, dt = cfl/(lx/dx_min + ly/dy_min) when localdt != 0 (else fixed dt),
+ * clamp to t_end, RK3. No host synchronisation inside; scalars[8] (may be NULL) receive
+ * tt, dt, lam_x, lam_y, Res_rms(1:4) after the last step (sqrt applied). */
+int32_t fdmamr_run_steps(fdmamr_ctx* ctx, int32_t n_steps, int32_t localdt, double cfl,
+                         double dt_fixed, double t_end, double* tt_inout, double* lam_inout2,
+                         double* scalars8);
+/* FP64 positivity check of rho and p over interior leaves; returns count of bad leaves
+ * through n_bad (new; the reference only checks for NaN residuals) */
+int32_t fdmamr_check_positivity(fdmamr_ctx* ctx, int32_t* n_bad);
+
+/* ---- regrid (host refine/coarsen with device sensors) ---- */
+/* rot_div (solution.F90:424-466,558-580): per-interior-leaf vorticity and density-gradient
+ * sensors from the +-1 stencil values on the device; rot/div may be NULL */
+int32_t fdmamr_sensors(fdmamr_ctx* ctx, double* rot, double* div);
+/* AMR body (solution.F90:413-419): sensors, solute_ref (refine/coarsen by goals,
+ * solution.F90:468-535), solute_gridmodify (mesh.F90:305-395), idneighbor, IDgrid, tables,
+ * ghost fill. sensor_mode 0: rot OR div (reference); 1: density gradient only. */
+int32_t fdmamr_regrid(fdmamr_ctx* ctx, int32_t lvl_amr, double refine_rot, double refine_div,
+                      double coarse_rot, double coarse_div, int32_t sensor_mode);
+
+/* ---- diagnostics for parity tests ---- */
+/* stencil table of interior leaf i: kind[8] (0 copy, 1 parent WLS, 2 parent restriction,
+ * 3 child WLS, 4 child fallback copy), canonical node[8]; same classification as the
+ * reference's Tip..Tjmm (XuanGua.F90:401-590) */
+int32_t fdmamr_stencil_info(fdmamr_ctx* ctx, int32_t i, int32_t* kind8, int32_t* node8,
+                            double* px8, double* py8, int32_t* m1_8, int32_t* m2_8,
+                            int32_t* nb8x9);
+/* device-evaluated stencil values (rho,u,v,p) of interior leaf i, 8 positions */
+int32_t fdmamr_stencil_values(fdmamr_ctx* ctx, int32_t i, double* val8x4);
+
+/* ---- measurement (no reference counterpart; the reference times only block refinement,
+ * mesh.F90:55,76-77) ---- */
+/* bracket every fused stage-kernel launch with HIP events on the context's stream */
+int32_t fdmamr_profile(fdmamr_ctx* ctx, int32_t enable);
+/* total milliseconds and number of stage-kernel launches recorded since the last read */
+int32_t fdmamr_profile_read(fdmamr_ctx* ctx, double* stage_ms_total, int32_t* stage_launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
