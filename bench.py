@@ -33,7 +33,8 @@ def workload_config(name: str, precision: int):
         return F.Config(xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, h=1.0 / 256, gamma=1.4,
                         bc=(F.BC_TRANSMISSIVE,) * 4, ic_kind=F.IC_RIEMANN4, seed=1,
                         ic_param=(0.5, 0.5), localdt=1, cfl=0.5, t_end=0.2, amr=1, lvl_amr=4,
-                        pstep_adapt=10, amr_sensor=1, pre_adapt=4, flux_precision=precision)
+                        pstep_adapt=10, amr_sensor=1, pre_adapt=4, wls_local=1,
+                        flux_precision=precision)
     if name == "shock4bubbles":
         # Mach 3 shock through 4 seeded bubbles, [0,4]x[0,2], base 1024x512, lvl <= 4
         g = 1.4
@@ -45,7 +46,7 @@ def workload_config(name: str, precision: int):
                         bc=(F.BC_INFLOW, F.BC_TRANSMISSIVE, F.BC_TRANSMISSIVE, F.BC_TRANSMISSIVE),
                         inflow=(r2, u2, 0.0, p2), ic_kind=F.IC_SHOCK_BUBBLES, seed=2,
                         ic_param=(0.5, M, 4, 0.05, 0.15, 0.1, 5.0), localdt=1, cfl=0.5,
-                        t_end=1e9, amr=1, lvl_amr=4, pstep_adapt=10, amr_sensor=1, pre_adapt=4,
+                        t_end=1e9, amr=1, lvl_amr=4, pstep_adapt=10, amr_sensor=1, pre_adapt=4, wls_local=1,
                         flux_precision=precision)
     raise SystemExit(f"unknown workload {name}")
 

@@ -235,6 +235,7 @@ int32_t fdmamr_create(const fdmamr_params* p, fdmamr_ctx** out) {
     }
     c->tree.configure(p->xmin, p->xmax, p->ymin, p->ymax, p->h, p->gamma, p->Ma > 0 ? p->Ma : 1.0);
     for (int k = 0; k < 4; k++) { c->tree.bc[k] = kinds[k]; c->tree.inflow[k] = p->inflow[k]; }
+    c->tree.wls_local = p->wls_local ? 1 : 0;
     *out = c;
     return 0;
 }

@@ -98,9 +98,16 @@ void wls_geometry(const Tree& t, int cell, double xx, double yy, const List& l1,
                   const std::vector<int32_t>& state_of_node, double* d, int32_t* iv) {
     int m1 = l1.n, m = l1.n + l2.n;
     double px[16], py[16], w[16], P[16][6];
+    // cell-local scaled frame (optional): x' = (x - xc)/hx, box [-1/2, 1/2]^2
+    const double hxc = t.xf * t.h / pow2i(t.lvl[cell]), hyc = t.h / pow2i(t.lvl[cell]);
+    if (t.wls_local) {
+        xx = (xx - t.x[cell]) / hxc;
+        yy = (yy - t.y[cell]) / hyc;
+    }
     for (int i = 0; i < m; i++) {
         int q = i < m1 ? l1.a[i] : l2.a[i - m1];
         px[i] = t.x[q]; py[i] = t.y[q];
+        if (t.wls_local) { px[i] = (px[i] - t.x[cell]) / hxc; py[i] = (py[i] - t.y[cell]) / hyc; }
         double dd = std::sqrt((px[i] - xx) * (px[i] - xx) + (py[i] - yy) * (py[i] - yy));
         w[i] = 1.0 / (dd + 1e-6);
         iv[WLSI_IDS + i] = state_of_node[q];
@@ -133,6 +140,7 @@ void wls_geometry(const Tree& t, int cell, double xx, double yy, const List& l1,
     double hy = t.h / std::ldexp(1.0, t.lvl[cell] + 1);
     double x_min = t.x[cell] - hx, x_max = t.x[cell] + hx;
     double y_min = t.y[cell] - hy, y_max = t.y[cell] + hy;
+    if (t.wls_local) { x_min = -0.5; x_max = 0.5; y_min = -0.5; y_max = 0.5; }
     d[WLS_BOX + 0] = x_max - x_min;
     d[WLS_BOX + 1] = y_max - y_min;
     d[WLS_BOX + 2] = x_max * x_max - x_min * x_min;

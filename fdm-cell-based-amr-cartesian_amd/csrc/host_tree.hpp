@@ -22,6 +22,7 @@ struct Tree {
     double pxmin = 0, pxmax = 0, pymin = 0, pymax = 0;  // physical domain (MAIN.F90:35-36)
     double gamma = 1.4, Ma = 1.0, cv = 0;
     int bc[4] = {6, 6, 6, 6};                       // xm, xp, ym, yp
+    int wls_local = 0;                              // WLS-WENO frame (see fdmamr_params)
     double inflow[4] = {0, 0, 0, 0};
     int m = 0, n = 0, total = 0;
 

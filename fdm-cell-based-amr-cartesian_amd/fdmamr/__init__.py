@@ -42,7 +42,7 @@ class Params(C.Structure):
         ("bc_xm", C.c_int32), ("bc_xp", C.c_int32), ("bc_ym", C.c_int32), ("bc_yp", C.c_int32),
         ("inflow", C.c_double * 4),
         ("flux_precision", C.c_int32), ("selector", C.c_int32), ("device", C.c_int32),
-        ("reserved", C.c_int32),
+        ("wls_local", C.c_int32),
     ]
 
 
@@ -140,6 +140,8 @@ class Config:
     coarse_goal_div: float = float(np.float32(0.4))
     amr_sensor: int = 0
     pre_adapt: int = 0
+    # WLS-WENO frame: 0 absolute (the reference), 1 cell-local scaled (fine meshes)
+    wls_local: int = 0
 
 
 class Solver:
@@ -155,6 +157,7 @@ class Solver:
         for i, v in enumerate(cfg.inflow):
             p.inflow[i] = v
         p.flux_precision, p.selector, p.device = cfg.flux_precision, cfg.selector, cfg.device
+        p.wls_local = cfg.wls_local
         self.ctx = C.c_void_p()
         rc = self.L.fdmamr_create(C.byref(p), C.byref(self.ctx))
         if rc != 0:

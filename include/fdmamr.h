@@ -62,7 +62,12 @@ typedef struct fdmamr_params {
     int32_t selector;         /* 0 off; 1 learned troubled-cell selector (FP32 mode only) */
     int32_t device;           /* HIP device ordinal, or FDMAMR_DEVICE_HOST_ONLY for a context
                                  that only builds meshes and tables (no device calls) */
-    int32_t reserved;
+    /* WLS-WENO fitting frame: 0 = absolute coordinates, as the reference
+     * (XuanGua.F90:224-229); 1 = coordinates about the WLS cell centre scaled by its size
+     * (new: the absolute frame's normal equations are numerically singular on fine meshes
+     * far from the origin; the fitted values are the same in exact arithmetic, the
+     * smoothness indicators are taken in the scaled frame) */
+    int32_t wls_local;
 } fdmamr_params;
 
 /* ---- lifecycle ---- */

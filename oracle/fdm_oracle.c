@@ -480,7 +480,17 @@ static void wls_nodes(const Orc* o, const Node* c, double xx, double yy, const N
         px[i] = q->x; py[i] = q->y;
         val[i][0] = q->U[0]; val[i][1] = q->U[1]; val[i][2] = q->U[2]; val[i][3] = q->U[4];
     }
-    Box b = make_box(c->x, c->y, c->lvl, o->h, o->x_factor);
+    Box b;
+    if (o->p.wls_local) {
+        /* new frame: (x - xc)/hx, (y - yc)/hy about the WLS cell; its box is [-1/2, 1/2]^2 */
+        double hx = o->x_factor * o->h / pow2i(c->lvl), hy = o->h / pow2i(c->lvl);
+        for (int i = 0; i < m; i++) { px[i] = (px[i] - c->x) / hx; py[i] = (py[i] - c->y) / hy; }
+        xx = (xx - c->x) / hx;
+        yy = (yy - c->y) / hy;
+        b = make_box(0.0, 0.0, 0, 1.0, 1.0);
+    } else {
+        b = make_box(c->x, c->y, c->lvl, o->h, o->x_factor);
+    }
     double v4[4];
     wls_core(m, m1, px, py, (const double(*)[4])val, xx, yy, &b, v4);
     var[0] = v4[0]; var[1] = v4[1]; var[2] = v4[2]; var[3] = 0.0; var[4] = v4[3];
@@ -1241,6 +1251,7 @@ int32_t orc_get_bc_leaves(const Orc* o, int32_t* node, double* prim5) {
 int32_t orc_set_leaves(Orc* o, const double* prim5) {
     for (int i = 0; i < o->nResgrid; i++)
         for (int k = 0; k < 5; k++) o->cell_NS[i]->U[k] = prim5[5 * i + k];
+    for (int i = 0; i < o->nBCgrid; i++) body_boundary(o, o->cell_BC[i]);
     return 0;
 }
 int32_t orc_get_scalars(const Orc* o, double* out8) {
@@ -1279,4 +1290,43 @@ int32_t orc_sensors(Orc* o, double* rot, double* div, double* s2) {
     }
     if (s2) { s2[0] = o->Srot; s2[1] = o->Sdiv; }
     return o->Ne;
+}
+
+double orc_smoothness(int32_t order, const double* coeff, double cx, double cy, int32_t lvl,
+                      double h, double x_factor) {
+    Box b = make_box(cx, cy, lvl, h, x_factor);
+    return order == 2 ? smooth2(coeff, &b) : smooth1(coeff, &b);
+}
+
+static const Node* find_canon(const Node* c, int id) {
+    if (c->canon == id) return c;
+    if (!c->son1) return NULL;
+    const Node* s[4] = {c->son1, c->son2, c->son3, c->son4};
+    for (int k = 0; k < 4; k++) {
+        const Node* r = find_canon(s[k], id);
+        if (r) return r;
+    }
+    return NULL;
+}
+
+int32_t orc_neighbors(const Orc* o, int32_t node, int32_t* out8) {
+    canon_assign((Orc*)o);
+    const Node* c = NULL;
+    for (int i = 0; i < o->total && !c; i++) c = find_canon(&o->cell[i], node);
+    if (!c) return -1;
+    const Node* nb[8] = {c->upnei, c->downnei, c->rightnei, c->leftnei,
+                         c->upleftnei, c->uprightnei, c->downleftnei, c->downrightnei};
+    for (int k = 0; k < 8; k++) out8[k] = nb[k] ? nb[k]->canon : -1;
+    return 0;
+}
+
+/* one forward-Euler substep (LXL for every interior leaf, nnd_center_RK.F90:15-44) from the
+ * current state; writes Ut (rho,u,v,T,p) in cell_NS order and leaves U unchanged */
+int32_t orc_substep(Orc* o, double dt, double lam_x, double lam_y, double* prim5) {
+    o->tstep = dt; o->lambda_x = lam_x; o->lambda_y = lam_y;
+    for (int i = 0; i < 6; i++) { o->Res_max[i] = 0.0; o->Res_rms[i] = 0.0; }
+    for (int i = 0; i < o->nResgrid; i++) solution_cell(o, o->cell_NS[i]);
+    for (int i = 0; i < o->nResgrid; i++)
+        for (int k = 0; k < 5; k++) prim5[5 * i + k] = o->cell_NS[i]->Ut[k];
+    return 0;
 }

@@ -47,6 +47,10 @@ typedef struct OrcParams {
     uint64_t seed;
     double ic_param[8];    /* IC-specific (see fdm_oracle.c: orc_ic_*) */
     int32_t pre_adapt;     /* refine the initial condition this many times before step 1 */
+    /* WLS-WENO frame: 0 = absolute coordinates (XuanGua.F90:224-229, the reference);
+     * 1 = coordinates relative to the WLS cell centre, scaled by its size (new; the absolute
+     * frame is numerically singular on fine meshes far from the origin) */
+    int32_t wls_local;
 } OrcParams;
 
 typedef struct Orc Orc;
@@ -68,8 +72,11 @@ int32_t orc_base_dims(const Orc* o, int32_t* m, int32_t* n);
 int32_t orc_get_leaves(const Orc* o, double* x, double* y, int32_t* lvl, int32_t* node, double* prim5);
 /* ghost leaves in cell_BC order */
 int32_t orc_get_bc_leaves(const Orc* o, int32_t* node, double* prim5);
-/* overwrite the primitive state of interior leaves (cell_NS order) */
+/* overwrite the primitive state of interior leaves (cell_NS order), then refill the ghosts */
 int32_t orc_set_leaves(Orc* o, const double* prim5);
+/* one forward-Euler substep from the current state: Ut of every interior leaf (cell_NS
+ * order); the state is not changed */
+int32_t orc_substep(Orc* o, double dt, double lam_x, double lam_y, double* prim5);
 int32_t orc_get_scalars(const Orc* o, double* out8); /* tt, tstep, lam_x, lam_y, l_max, it, 0, 0 */
 int32_t orc_get_residual(const Orc* o, double* res_max4, double* res_rms4);
 
@@ -109,6 +116,14 @@ int32_t orc_get_direction(double xx, double yy, double x, double y); /* XuanGua.
  * (x, y, rho, u, v, p); first m1 rows are list 1. cell box from (cx, cy, lvl). out4 = rho,u,v,p */
 void    orc_wls_weno(int32_t m, int32_t m1, const double* pts, double xx, double yy, double cx,
                      double cy, int32_t lvl, double h, double x_factor, double* out4);
+
+/* smoothness indicator of compute_smoothness (XuanGua.F90:309-355): order 2 takes 6
+ * coefficients, order 1 takes 3; box of the cell (cx, cy, lvl) */
+double  orc_smoothness(int32_t order, const double* coeff, double cx, double cy, int32_t lvl,
+                       double h, double x_factor);
+/* the 8 cached neighbours (idnei, neighbor.F90:412-449) of canonical node `node`:
+ * up, down, right, left, upleft, upright, downleft, downright; -1 = null */
+int32_t orc_neighbors(const Orc* o, int32_t node, int32_t* out8);
 
 uint64_t orc_splitmix64(uint64_t* state);
 

@@ -36,6 +36,7 @@ class OrcParams(C.Structure):
         ("seed", C.c_uint64),
         ("ic_param", C.c_double * 8),
         ("pre_adapt", C.c_int32),
+        ("wls_local", C.c_int32),
     ]
 
 
@@ -90,6 +91,13 @@ def lib():
                                    C.c_double, C.c_int32, C.c_double, C.c_double, dp]
         L.orc_splitmix64.argtypes = [C.POINTER(C.c_uint64)]
         L.orc_splitmix64.restype = C.c_uint64
+        L.orc_smoothness.argtypes = [C.c_int32, dp, C.c_double, C.c_double, C.c_int32,
+                                     C.c_double, C.c_double]
+        L.orc_smoothness.restype = C.c_double
+        L.orc_neighbors.argtypes = [P, C.c_int32, ip]
+        L.orc_neighbors.restype = C.c_int32
+        L.orc_substep.argtypes = [P, C.c_double, C.c_double, C.c_double, dp]
+        L.orc_substep.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -103,7 +111,7 @@ def make_params(**kw) -> OrcParams:
              refine_goal_rot=GAMMA_REF, refine_goal_div=GAMMA_REF,
              coarse_goal_rot=float(np.float32(0.4)), coarse_goal_div=float(np.float32(0.4)),
              amr_sensor=0, bc_xm=6, bc_xp=6, bc_ym=6, bc_yp=6, inflow=(0, 0, 0, 0),
-             ic_kind=IC_VORTEX, seed=0, ic_param=(0,) * 8, pre_adapt=0)
+             ic_kind=IC_VORTEX, seed=0, ic_param=(0,) * 8, pre_adapt=0, wls_local=0)
     d.update(kw)
     for k, v in d.items():
         if k in ("inflow", "ic_param"):
@@ -141,6 +149,11 @@ class Oracle:
 
     def advance_rk3(self, dt, lam_x, lam_y):
         self.L.orc_advance_rk3(self.h, dt, lam_x, lam_y)
+
+    def substep(self, dt, lam_x, lam_y):
+        out = np.zeros((self.n_leaf, 5))
+        self.L.orc_substep(self.h, dt, lam_x, lam_y, out.reshape(-1))
+        return out
 
     # ---- accessors ----
     @property
@@ -209,6 +222,11 @@ class Oracle:
         self.L.orc_stencil_value(self.h, i, k, v)
         return v
 
+    def neighbors(self, node):
+        out = np.zeros(8, np.int32)
+        self.L.orc_neighbors(self.h, int(node), out)
+        return out
+
     def sensors(self):
         N = self.n_leaf
         rot = np.zeros(N); div = np.zeros(N); s = np.zeros(2)
@@ -258,6 +276,10 @@ def wls_weno(pts, m1, xx, yy, cx, cy, lvl, h, x_factor=1.0):
     out = np.zeros(4)
     lib().orc_wls_weno(pts.shape[0], m1, pts.reshape(-1), xx, yy, cx, cy, lvl, h, x_factor, out)
     return out
+
+
+def smoothness(order, coeff, cx, cy, lvl, h, x_factor=1.0):
+    return lib().orc_smoothness(order, np.ascontiguousarray(coeff, float), cx, cy, lvl, h, x_factor)
 
 
 def splitmix64(state: int, count: int):

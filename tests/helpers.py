@@ -17,7 +17,7 @@ def oracle_kwargs_from_config(cfg: F.Config) -> dict:
                 amr_sensor=cfg.amr_sensor, bc_xm=cfg.bc[0], bc_xp=cfg.bc[1], bc_ym=cfg.bc[2],
                 bc_yp=cfg.bc[3], inflow=cfg.inflow, ic_kind=cfg.ic_kind, seed=cfg.seed,
                 ic_param=tuple(cfg.ic_param) + (0.0,) * (8 - len(cfg.ic_param)),
-                pre_adapt=cfg.pre_adapt)
+                pre_adapt=cfg.pre_adapt, wls_local=cfg.wls_local)
 
 
 def oracle_for(cfg: F.Config) -> O.Oracle:
@@ -43,6 +43,6 @@ def riemann_c2(h=1.0 / 256, **kw) -> F.Config:
     d = dict(xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, h=h, gamma=1.4,
              bc=(F.BC_TRANSMISSIVE,) * 4, ic_kind=F.IC_RIEMANN4, seed=1,
              ic_param=(0.5, 0.5), localdt=1, cfl=0.5, t_end=0.2, amr=1, lvl_amr=4,
-             pstep_adapt=10, amr_sensor=1, pre_adapt=4)
+             pstep_adapt=10, amr_sensor=1, pre_adapt=4, wls_local=1)
     d.update(kw)
     return F.Config(**d)
