@@ -17,8 +17,11 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libfdmamr.so")
 
-SOURCES = ["fdmamr_kernels.hip", "fdmamr_selector.hip", "fdmamr_api.cpp", "host_tree.cpp",
-           "host_tables.cpp"]
+SOURCES = ["fdmamr_kernels.hip", "fdmamr_fast.hip", "fdmamr_selector.hip", "fdmamr_api.cpp",
+           "host_tree.cpp", "host_tables.cpp"]
+# FP32 fast-path and selector kernels may contract multiply-adds into FMAs
+PER_FILE_FLAGS = {"fdmamr_fast.hip": ["-ffp-contract=fast"],
+                  "fdmamr_selector.hip": ["-ffp-contract=fast"]}
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -49,7 +52,7 @@ def build(verbose: bool = False) -> str:
         if not os.path.exists(path):
             continue
         obj = os.path.join(LIBDIR, os.path.splitext(src)[0] + ".o")
-        cmd = [hipcc(), *HIPCC_FLAGS, "-c", path, "-o", obj]
+        cmd = [hipcc(), *HIPCC_FLAGS, *PER_FILE_FLAGS.get(src, []), "-c", path, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)

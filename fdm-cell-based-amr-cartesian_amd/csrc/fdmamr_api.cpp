@@ -35,6 +35,11 @@ struct fdmamr_ctx {
 
     DBuf<double4> Q[3];
     DBuf<double> T[3];
+    DBuf<float4> Qf[3];      // FP32 shadow of the stage buffers (fast path gathers)
+    DBuf<float> wls_f;       // FP32 slot records (fast path)
+    DBuf<uint8_t> flags;     // selector flags per interior leaf (bit 0: troubled)
+    DBuf<float> label;       // per-leaf WCNS weight deviation (labels / judge)
+    bool use_flags = false;  // fast path consults the flags
     DBuf<int32_t> stencil, wls_i, rst, bcp, son, state;
     DBuf<int8_t> lvl;
     DBuf<double> wls_d, partials, lampart, rot, div, tmp;
@@ -99,7 +104,11 @@ int upload_tables(fdmamr_ctx* c) {
     for (int k = 0; k < 3; k++) {
         HIPCHK(c, ensure(c->Q[k], ext));
         HIPCHK(c, ensure(c->T[k], ext));
+        HIPCHK(c, ensure(c->Qf[k], ext));
     }
+    HIPCHK(c, ensure(c->wls_f, tb.wls_f.size()));
+    HIPCHK(c, ensure(c->flags, tb.N));
+    HIPCHK(c, ensure(c->label, tb.N));
     HIPCHK(c, ensure(c->stencil, tb.stencil.size()));
     HIPCHK(c, ensure(c->lvl, tb.lvl_leaf.size()));
     HIPCHK(c, ensure(c->wls_d, tb.wls_d.size()));
@@ -139,7 +148,11 @@ int upload_tables(fdmamr_ctx* c) {
     for (int k = 0; k < 3; k++) {
         HIPCHK(c, up(c->Q[k].p, q.data(), q.size() * sizeof(double4)));
         HIPCHK(c, up(c->T[k].p, tv.data(), tv.size() * 8));
+        launch_shadow(c->Q[k].p, c->Qf[k].p, tb.L, s);
     }
+    HIPCHK(c, up(c->wls_f.p, tb.wls_f.data(), tb.wls_f.size() * 4));
+    // every leaf troubled until the selector says otherwise
+    if (tb.N > 0) HIPCHK(c, hipMemsetAsync(c->flags.p, 1, tb.N, s));
     DevTables& d = c->dt;
     d.N = tb.N; d.G = tb.G; d.P = tb.P; d.L = tb.L; d.S_w = tb.S_w; d.S_r = tb.S_r;
     d.gamma = t.gamma; d.cv = t.cv; d.h = t.h; d.xf = t.xf; d.Ma = t.Ma;
@@ -161,9 +174,13 @@ int pull_state(fdmamr_ctx* c) {
         HIPCHK(c, hipMemcpyAsync(tv.data(), c->T[0].p, n * 8, hipMemcpyDeviceToHost, c->stream));
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    const bool fast = c->p.flux_precision == FDMAMR_FLUX_FP32;
+    const double gm = c->tree.gamma * c->tree.Ma * c->tree.Ma;
     for (int i = 0; i < n; i++) {
         auto& u = c->tree.U[tb.leaf_node[i]];
-        u[0] = q[i].x; u[1] = q[i].y; u[2] = q[i].z; u[3] = tv[i]; u[4] = q[i].w;
+        u[0] = q[i].x; u[1] = q[i].y; u[2] = q[i].z; u[4] = q[i].w;
+        // the fast path does not carry the passive T: recompute it from the equation of state
+        u[3] = fast ? q[i].w * gm / q[i].x : tv[i];
     }
     return 0;
 }
@@ -191,17 +208,25 @@ void enqueue_rk3(fdmamr_ctx* c, int advance_time) {
     static const int in[3] = {0, 1, 2}, out[3] = {1, 2, 0};
     const DevTables& d = c->dt;
     hipStream_t s = c->stream;
+    const bool fast = c->p.flux_precision == FDMAMR_FLUX_FP32;
     for (int k = 0; k < 3; k++) {
-        launch_slots(d, c->Q[in[k]].p, c->T[in[k]].p, s);
+        if (fast) launch_slots_fast(d, c->wls_f.p, c->Q[in[k]].p, c->Qf[in[k]].p, s);
+        else launch_slots(d, c->Q[in[k]].p, c->T[in[k]].p, s);
         int e0 = -1, e1 = -1;
         if (c->profiling && (e0 = next_event(c)) >= 0) (void)hipEventRecord(c->ev_pool[e0], s);
-        launch_stage(d, c->p.flux_precision, c->Q[in[k]].p, c->Q[0].p, c->T[0].p, c->Q[out[k]].p,
-                     c->T[out[k]].p, c->sc, ra[k], rb[k], k == 2 ? 1 : 0, c->partials.p, s);
+        if (fast)
+            launch_stage_fast(d, c->Q[in[k]].p, c->Qf[in[k]].p, c->Q[0].p, c->Q[out[k]].p, c->Qf[out[k]].p,
+                              c->sc, ra[k], rb[k], k == 2 ? 1 : 0, c->partials.p,
+                              c->use_flags ? c->flags.p : nullptr, nullptr, s);
+        else
+            launch_stage(d, 0, c->Q[in[k]].p, c->Q[0].p, c->T[0].p, c->Q[out[k]].p, c->T[out[k]].p,
+                         c->sc, ra[k], rb[k], k == 2 ? 1 : 0, c->partials.p, s);
         if (e0 >= 0 && (e1 = next_event(c)) >= 0) {
             (void)hipEventRecord(c->ev_pool[e1], s);
             c->ev_stage.emplace_back(e0, e1);
         }
-        launch_bc(d, c->Q[out[k]].p, c->T[out[k]].p, s);
+        if (fast) launch_bc_fast(d, c->Q[out[k]].p, c->Qf[out[k]].p, s);
+        else launch_bc(d, c->Q[out[k]].p, c->T[out[k]].p, s);
     }
     launch_step_end(c->partials.p, nblocks(d.N, 256), d.N, advance_time, c->sc, s);
 }
@@ -396,6 +421,14 @@ int32_t fdmamr_set_state(fdmamr_ctx* c, const double* prim5) {
     }
     HIPCHK(c, hipMemcpyAsync(c->Q[0].p, q.data(), N * sizeof(double4), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->T[0].p, tv.data(), N * 8, hipMemcpyHostToDevice, c->stream));
+    // ghosts follow the new interior state (as after the reference's AMR, solution.F90:224-229)
+    if (c->p.flux_precision == FDMAMR_FLUX_FP32) {
+        launch_shadow(c->Q[0].p, c->Qf[0].p, N, c->stream);
+        launch_bc_fast(c->dt, c->Q[0].p, c->Qf[0].p, c->stream);
+    } else {
+        launch_bc(c->dt, c->Q[0].p, c->T[0].p, c->stream);
+    }
+    HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -408,9 +441,12 @@ static int get_range(fdmamr_ctx* c, int off, int n, double* prim5) {
         HIPCHK(c, hipMemcpyAsync(tv.data(), c->T[0].p + off, n * 8, hipMemcpyDeviceToHost, c->stream));
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    const bool fast = c->p.flux_precision == FDMAMR_FLUX_FP32;
+    const double gm = c->tree.gamma * c->tree.Ma * c->tree.Ma;
     for (int i = 0; i < n; i++) {
         double* u = prim5 + 5 * (size_t)i;
-        u[0] = q[i].x; u[1] = q[i].y; u[2] = q[i].z; u[3] = tv[i]; u[4] = q[i].w;
+        u[0] = q[i].x; u[1] = q[i].y; u[2] = q[i].z; u[4] = q[i].w;
+        u[3] = fast ? q[i].w * gm / q[i].x : tv[i];
     }
     return 0;
 }
@@ -606,6 +642,47 @@ int32_t fdmamr_stencil_info(fdmamr_ctx* c, int32_t i, int32_t* kind8, int32_t* n
                 nb8x9[k * 9 + j] = q >= 0 ? id[q] : -1;
             }
     }
+    return 0;
+}
+
+int32_t fdmamr_selector_set_flags(fdmamr_ctx* c, const uint8_t* flags, int32_t use) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (c->p.flux_precision != FDMAMR_FLUX_FP32) return fail(c, -1, "selector needs the FP32 flux mode");
+    if (flags && c->tb.N > 0)
+        HIPCHK(c, hipMemcpyAsync(c->flags.p, flags, c->tb.N, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->use_flags = use != 0;
+    return 0;
+}
+
+int32_t fdmamr_selector_get_flags(fdmamr_ctx* c, uint8_t* flags) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!flags) return fail(c, -1, "null flags");
+    if (c->tb.N > 0)
+        HIPCHK(c, hipMemcpyAsync(flags, c->flags.p, c->tb.N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int32_t fdmamr_labels(fdmamr_ctx* c, float* dev) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!dev) return fail(c, -1, "null label buffer");
+    if (c->tb.N == 0) return 0;
+    // one full-path FP32 substep into the scratch buffer 2 with label output; the state in
+    // buffer 0 is untouched
+    DevScalars s{};
+    s.dt = 0.0; s.h = c->tree.h; s.xf = c->tree.xf;
+    HIPCHK(c, hipMemcpyAsync(c->sc, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
+    launch_shadow(c->Q[0].p, c->Qf[0].p, c->tb.L, c->stream);
+    launch_slots_fast(c->dt, c->wls_f.p, c->Q[0].p, c->Qf[0].p, c->stream);
+    launch_stage_fast(c->dt, c->Q[0].p, c->Qf[0].p, c->Q[0].p, c->Q[2].p, c->Qf[2].p, c->sc, 0.0, 1.0,
+                      0, c->partials.p, nullptr, c->label.p, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(dev, c->label.p, c->tb.N * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 

@@ -46,4 +46,12 @@ void launch_sensors(const DevTables& t, const double4* Q, double* rot, double* d
 void launch_positivity(const DevTables& t, const double4* Q, int* nbad, hipStream_t st);
 void launch_stencil_values(const DevTables& t, const double4* Q, int i, double* out, hipStream_t st);
 
+// fast path (fdmamr_fast.hip)
+void launch_slots_fast(const DevTables& t, const float* wf, const double4* Q, float4* Qf, hipStream_t st);
+void launch_stage_fast(const DevTables& t, const double4* Qin, const float4* Qf_in, const double4* Qn,
+                       double4* Qout, float4* Qf_out, const DevScalars* sc, double alpha, double beta,
+                       int reduce, double* partials, const uint8_t* flags, float* label, hipStream_t st);
+void launch_bc_fast(const DevTables& t, double4* Q, float4* Qf, hipStream_t st);
+void launch_shadow(const double4* Q, float4* Qf, int n, hipStream_t st);
+
 }  // namespace fdm

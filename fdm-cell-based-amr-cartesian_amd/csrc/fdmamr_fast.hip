@@ -1,0 +1,436 @@
+// fdmamr_fast.hip -- FP32-flux fast path of the hot path (gfx950), built with FMA
+// contraction allowed.
+//
+// Precision split: stencil gathers, WCNS reconstruction, characteristic projection and the
+// Lax-Friedrichs flux run in FP32 from an FP32 shadow of the state (16 bytes per gather
+// instead of 32); the forward-Euler substep and the RK blend accumulate in FP64 on the
+// leaf's own FP64 state. The passive temperature is not carried (it is recomputed from
+// rho and p when the state is read back).
+//
+// Leaves flagged "cleared" by the troubled-cell selector take the linear-weight WCNS3
+// reconstruction, for which R * interp(R^-1 u) = interp(u): no eigenvectors, no projection,
+// no nonlinear weights (nnd_center_RK.F90:66-112 with w = d). Flagged ("troubled") leaves take
+// the full characteristic WCNS3 of the reference (nnd_center_RK.F90:66-353) in FP32 with
+// hardware reciprocals.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fdmamr_device.hpp"
+
+namespace fdm {
+
+#define FD1 (float)0x1.745d18p-4   // REAL(4) 1./11.
+#define FD2 (float)0x1.d1745ep-1   // REAL(4) 10./11.
+// linear-weight reconstruction (-u1 + 6 u2 + 3 u3) / 8 scaled by 1/(d1+d2), as the reference's
+// weights give when tau3 = 0
+#define LIN_C (0.125f / ((float)0x1.745d18p-4 + (float)0x1.d1745ep-1))
+#define W1_LIN ((float)0x1.745d18p-4 / ((float)0x1.745d18p-4 + (float)0x1.d1745ep-1))
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+__device__ __forceinline__ double wmax(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ double wsum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o);
+    return v;
+}
+
+struct PtF { float U[4], F[4]; };
+
+template <int DIR>
+__device__ __forceinline__ void cons_flux_f(float4 q, float gm1inv, PtF& o) {
+    const float r = q.x, u = q.y, v = q.z, p = q.w;
+    const float E = p * gm1inv + 0.5f * r * (u * u + v * v);
+    o.U[0] = r; o.U[1] = r * u; o.U[2] = r * v; o.U[3] = E;
+    const float vn = DIR == 0 ? u : v;
+    o.F[0] = r * vn;
+    o.F[1] = r * u * vn + (DIR == 0 ? p : 0.f);
+    o.F[2] = r * v * vn + (DIR == 1 ? p : 0.f);
+    o.F[3] = vn * (E + p);
+}
+
+// eigenvectors at the face average (REcalc / RFcalc), FP32 with hardware reciprocals
+template <int DIR>
+__device__ __forceinline__ void eigen_f(const float* U, float gama, float R[4][4], float Ri[4][4]) {
+    const float ir = frcp(U[0]);
+    const float r = U[0], u = U[1] * ir, v = U[2] * ir;
+    const float g1 = gama - 1.f, mg = 1.f - gama;
+    const float p = fmaxf(g1 * (U[3] - 0.5f * r * (u * u + v * v)), 1e-30f);
+    const float a2 = gama * p * ir;
+    const float a = __builtin_sqrtf(a2);
+    const float ia2 = frcp(a2), h2 = 0.5f * ia2;
+    const float q2 = 0.5f * (u * u + v * v);
+    const float vn = DIR == 0 ? u : v;
+    const float hq = a2 / g1 + q2;
+    R[0][0] = 1.f; R[0][1] = 1.f; R[0][2] = 1.f; R[0][3] = 0.f;
+    R[3][0] = hq - vn * a; R[3][1] = q2; R[3][2] = hq + vn * a;
+    if (DIR == 0) {
+        R[1][0] = u - a; R[1][1] = u; R[1][2] = u + a; R[1][3] = 0.f;
+        R[2][0] = v; R[2][1] = v; R[2][2] = v; R[2][3] = -1.f;
+        R[3][3] = -v;
+        Ri[0][0] = (g1 * q2 + a * u) * h2; Ri[1][0] = (a2 - g1 * q2) * ia2; Ri[2][0] = (g1 * q2 - a * u) * h2; Ri[3][0] = v;
+        Ri[0][1] = (mg * u - a) * h2; Ri[1][1] = g1 * u * ia2; Ri[2][1] = (mg * u + a) * h2; Ri[3][1] = 0.f;
+        Ri[0][2] = mg * v * h2; Ri[1][2] = g1 * v * ia2; Ri[2][2] = mg * v * h2; Ri[3][2] = -1.f;
+    } else {
+        R[1][0] = u; R[1][1] = u; R[1][2] = u; R[1][3] = 1.f;
+        R[2][0] = v - a; R[2][1] = v; R[2][2] = v + a; R[2][3] = 0.f;
+        R[3][3] = u;
+        Ri[0][0] = (g1 * q2 + a * v) * h2; Ri[1][0] = (a2 - g1 * q2) * ia2; Ri[2][0] = (g1 * q2 - a * v) * h2; Ri[3][0] = -u;
+        Ri[0][1] = mg * u * h2; Ri[1][1] = g1 * u * ia2; Ri[2][1] = mg * u * h2; Ri[3][1] = 1.f;
+        Ri[0][2] = (mg * v - a) * h2; Ri[1][2] = g1 * v * ia2; Ri[2][2] = (mg * v + a) * h2; Ri[3][2] = 0.f;
+    }
+    Ri[0][3] = g1 * h2; Ri[1][3] = mg * ia2; Ri[2][3] = g1 * h2; Ri[3][3] = 0.f;
+}
+
+// one WCNS_Z3 component; returns the reconstruction and accumulates |w1 - w1_lin| into dev.
+// The deviation uses smoothness indicators with a relative noise floor (1e-6 of the mean
+// square), so exactly flat data (where every candidate is the same value) is not reported.
+__device__ __forceinline__ float wcns_f(float u1, float u2, float u3, float& dev) {
+    const float IS1a = (u2 - u1) * (u2 - u1), IS1b = (u3 - u2) * (u3 - u2);
+    const float t3 = u1 - 2.f * u2 + u3;
+    const float IS1 = fminf(IS1a, IS1b);
+    const float IS2 = t3 * t3 + u1 * u1 + u2 * u2 + u3 * u3 - u1 * u2 - u2 * u3 - u1 * u3;
+    const float tq = 10.f * fabsf(IS2 - 3.f * IS1);
+    const float tau3 = tq * tq;
+    const float r1 = fminf(tau3 * frcp(1.0e-30f + IS1), 1.0e30f);
+    const float r2 = fminf(tau3 * frcp(1.0e-30f + IS2), 1.0e30f);
+    const float a1 = FD1 * (1.f + r1), a2 = FD2 * (1.f + r2);
+    const float ia = frcp(a1 + a2);
+    const float w1 = a1 * ia, w2 = a2 * ia;
+    const float p1 = u2;
+    const float p2 = ((-u1 + 6.f * u2 + 3.f * u3) * 0.125f - FD1 * p1) * (1.f / FD2);
+    {
+        const float fl = 1e-6f * (u1 * u1 + u2 * u2 + u3 * u3) * (1.f / 3.f) + 1e-30f;
+        const float s1 = fminf(tau3 * frcp(fl + IS1), 1.0e30f);
+        const float s2 = fminf(tau3 * frcp(fl + IS2), 1.0e30f);
+        const float b1 = FD1 * (1.f + s1), b2 = FD2 * (1.f + s2);
+        dev = fmaxf(dev, fabsf(b1 * frcp(b1 + b2) - W1_LIN));
+    }
+    return w1 * p1 + w2 * p2;
+}
+
+// full characteristic WCNS3 flux derivative in FP32 (troubled leaves)
+template <int DIR>
+__device__ __forceinline__ void flux_full_f(const float4* q, float d, float lam, float gama, float* out,
+                                            float& dev) {
+    PtF P[5];
+    const float gm1inv = 1.f / (gama - 1.f);
+#pragma unroll
+    for (int j = 0; j < 5; j++) cons_flux_f<DIR>(q[j], gm1inv, P[j]);
+    float flux[2][4];
+#pragma unroll
+    for (int f = 0; f < 2; f++) {
+        float avg[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) avg[k] = 0.5f * (P[f + 1].U[k] + P[f + 2].U[k]);
+        float R[4][4], Ri[4][4];
+        eigen_f<DIR>(avg, gama, R, Ri);
+        float w[4][4], g[4][4];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                float s = 0.f, t = 0.f;
+#pragma unroll
+                for (int c = 0; c < 4; c++) { s += Ri[r][c] * P[f + j].U[c]; t += Ri[r][c] * P[f + j].F[c]; }
+                w[j][r] = s; g[j][r] = t;
+            }
+        float cL[4], cR[4], hL[4], hR[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            cL[r] = wcns_f(w[0][r], w[1][r], w[2][r], dev);
+            cR[r] = wcns_f(w[3][r], w[2][r], w[1][r], dev);
+            hL[r] = wcns_f(g[0][r], g[1][r], g[2][r], dev);
+            hR[r] = wcns_f(g[3][r], g[2][r], g[1][r], dev);
+        }
+        // flux in characteristic space, one back-projection: R (gL + gR - lam (wR - wL)) / 2
+        float fc[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) fc[r] = 0.5f * (hL[r] + hR[r] - lam * (cR[r] - cL[r]));
+#pragma unroll
+        for (int r = 0; r < 4; r++) flux[f][r] = R[r][0] * fc[0] + R[r][1] * fc[1] + R[r][2] * fc[2] + R[r][3] * fc[3];
+    }
+    const float id = 1.f / d;
+#pragma unroll
+    for (int n = 0; n < 4; n++)
+        out[n] = (4.f / 3.f) * id * (flux[1][n] - flux[0][n]) - (1.f / 6.f) * id * (P[3].F[n] - P[1].F[n]);
+}
+
+// linear-weight WCNS3 flux derivative (cleared leaves): no projection, no weights
+template <int DIR>
+__device__ __forceinline__ void flux_lin_f(const float4* q, float d, float lam, float gama, float* out) {
+    PtF P[5];
+    const float gm1inv = 1.f / (gama - 1.f);
+#pragma unroll
+    for (int j = 0; j < 5; j++) cons_flux_f<DIR>(q[j], gm1inv, P[j]);
+    const float id = 1.f / d;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        float fx[2];
+#pragma unroll
+        for (int f = 0; f < 2; f++) {
+            const int a = f, b = f + 1, c = f + 2, e = f + 3;
+            const float wL = LIN_C * (-P[a].U[n] + 6.f * P[b].U[n] + 3.f * P[c].U[n]);
+            const float wR = LIN_C * (-P[e].U[n] + 6.f * P[c].U[n] + 3.f * P[b].U[n]);
+            const float gL = LIN_C * (-P[a].F[n] + 6.f * P[b].F[n] + 3.f * P[c].F[n]);
+            const float gR = LIN_C * (-P[e].F[n] + 6.f * P[c].F[n] + 3.f * P[b].F[n]);
+            fx[f] = 0.5f * (gL + gR - lam * (wR - wL));
+        }
+        out[n] = (4.f / 3.f) * id * (fx[1] - fx[0]) - (1.f / 6.f) * id * (P[3].F[n] - P[1].F[n]);
+    }
+}
+
+// ---------------------------------------------------------------- stage kernel (fast path)
+// flags: nullptr = every leaf takes the full path; otherwise bit 0 set = troubled.
+// label: when non-null, receives the leaf's max weight deviation (full path only).
+// Qn/Qout may alias (in-place final stage), so they carry no __restrict__.
+__global__ __launch_bounds__(256) void k_stage_fast(DevTables t, const double4* __restrict__ Qin,
+                                                    const float4* __restrict__ Qf_in,
+                                                    const double4* Qn, double4* Qout,
+                                                    float4* __restrict__ Qf_out,
+                                                    const DevScalars* __restrict__ sc, double alpha,
+                                                    double beta, int reduce,
+                                                    double* __restrict__ partials,
+                                                    const uint8_t* __restrict__ flags,
+                                                    float* __restrict__ label) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double rmax[4] = {0, 0, 0, 0}, rsum[4] = {0, 0, 0, 0};
+    if (i < t.N) {
+        const double gama = t.gamma;
+        const double dt = sc->dt;
+        int s[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) s[k] = t.stencil[(size_t)k * t.N + i];
+        const double4 qc = Qin[i];
+        const float4 qcf = Qf_in[i];
+        const int lv = t.lvl[i];
+        const double dy = t.h / (double)(1 << lv);
+        const double dx = t.xf * dy;
+        const bool full = flags == nullptr || (flags[i] & 1);
+        float dE[4], dF[4];
+        float dev = 0.f;
+        {
+            const float4 col[5] = {Qf_in[s[5]], Qf_in[s[1]], qcf, Qf_in[s[0]], Qf_in[s[4]]};
+            if (full) flux_full_f<0>(col, (float)dx, (float)sc->lam_x, (float)gama, dE, dev);
+            else flux_lin_f<0>(col, (float)dx, (float)sc->lam_x, (float)gama, dE);
+        }
+        {
+            const float4 col[5] = {Qf_in[s[7]], Qf_in[s[3]], qcf, Qf_in[s[2]], Qf_in[s[6]]};
+            if (full) flux_full_f<1>(col, (float)dy, (float)sc->lam_y, (float)gama, dF, dev);
+            else flux_lin_f<1>(col, (float)dy, (float)sc->lam_y, (float)gama, dF);
+        }
+        if (label) label[i] = dev;
+        // FP64 forward-Euler substep and primitive RK blend (nnd_center_RK.F90:22-42,
+        // solution.F90:398-400)
+        double U[4], Ut[4];
+        U[0] = qc.x;
+        U[1] = qc.x * qc.y;
+        U[2] = qc.x * qc.z;
+        U[3] = qc.w / (gama - 1.0) + 0.5 * qc.x * (qc.y * qc.y + qc.z * qc.z);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const double fu = -(double)dE[k] - (double)dF[k];
+            Ut[k] = U[k] + fu * dt;
+            const double fr = fu * dx * dy;
+            rmax[k] = fabs(fr);
+            rsum[k] = fr * fr;
+        }
+        const double r = Ut[0];
+        const double ir = 1.0 / r;
+        const double u = Ut[1] * ir, v = Ut[2] * ir;
+        const double p = (gama - 1.0) * (Ut[3] - 0.5 * r * (u * u + v * v));
+        const double4 qn = Qn[i];
+        double4 o;
+        o.x = alpha * qn.x + beta * r;
+        o.y = alpha * qn.y + beta * u;
+        o.z = alpha * qn.z + beta * v;
+        o.w = alpha * qn.w + beta * p;
+        Qout[i] = o;
+        Qf_out[i] = make_float4((float)o.x, (float)o.y, (float)o.z, (float)o.w);
+    }
+    if (reduce) {
+        __shared__ double red[4][8];
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const double a = wmax(rmax[k]);
+            const double b = wsum(rsum[k]);
+            if (lane == 0) { red[wv][k] = a; red[wv][4 + k] = b; }
+        }
+        __syncthreads();
+        if (threadIdx.x < 8) {
+            const int k = threadIdx.x;
+            const int nw = blockDim.x >> 6;
+            double acc = red[0][k];
+            for (int w = 1; w < nw; w++) acc = k < 4 ? fmax(acc, red[w][k]) : acc + red[w][k];
+            partials[(size_t)blockIdx.x * 8 + k] = acc;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- FP32 WLS-WENO slots
+__device__ __forceinline__ float isq_f(float k1, float k2, float k3, const float* b) {
+    return k1 * k1 * b[0] * b[1] + k1 * k2 * b[2] * b[1] + k1 * k3 * b[0] * b[3] +
+           k2 * k2 * b[4] * (1.f / 3.f) * b[1] + 0.5f * k2 * k3 * b[2] * b[3] + k3 * k3 * b[0] * b[5] * (1.f / 3.f);
+}
+
+__global__ __launch_bounds__(128) void k_slots_fast(DevTables t, const float* __restrict__ wf,
+                                                    float4* __restrict__ Qf) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= t.S_w) return;
+    const float* __restrict__ f = wf + (size_t)s * WF_STRIDE;
+    const int32_t* __restrict__ iv = t.wls_i + (size_t)s * WLSI_STRIDE;
+    const int m = iv[WLSI_M], m1 = iv[WLSI_M1];
+    float4 val[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) val[i] = i < m ? Qf[iv[WLSI_IDS + i]] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float* box = f + WF_BOX;
+    const float area = box[0] * box[1];
+    float out[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        float phi[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) phi[i] = k == 0 ? val[i].x : k == 1 ? val[i].y : k == 2 ? val[i].z : val[i].w;
+        float al[6], vl = 0.f;
+#pragma unroll
+        for (int r = 0; r < 6; r++) {
+            float a = 0.f;
+#pragma unroll
+            for (int i = 0; i < 9; i++) a += f[WF_C6 + r * 9 + i] * phi[i];
+            al[r] = a;
+        }
+#pragma unroll
+        for (int i = 0; i < 9; i++) vl += f[WF_CL + i] * phi[i];
+        float as1[3], as2[3], vs1 = 0.f, vs2 = 0.f;
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            float a = 0.f, b = 0.f;
+#pragma unroll
+            for (int j = 0; j < 5; j++) a += f[WF_C3A + r * 5 + j] * (j < m1 ? phi[j] : 0.f);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int ii = m1 + j;
+                float ph = 0.f;
+#pragma unroll
+                for (int q = 0; q < 9; q++) ph = q == ii ? phi[q] : ph;
+                b += f[WF_C3B + r * 4 + j] * (ii < m ? ph : 0.f);
+            }
+            as1[r] = a; as2[r] = b;
+        }
+#pragma unroll
+        for (int j = 0; j < 5; j++) vs1 += f[WF_CS1 + j] * (j < m1 ? phi[j] : 0.f);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int ii = m1 + j;
+            float ph = 0.f;
+#pragma unroll
+            for (int q = 0; q < 9; q++) ph = q == ii ? phi[q] : ph;
+            vs2 += f[WF_CS2 + j] * (ii < m ? ph : 0.f);
+        }
+        // smoothness indicators (XuanGua.F90:325-339) and WENO combination (:296-306)
+        const float sl = isq_f(al[1], 2.f * al[3], al[4], box) + isq_f(al[2], al[4], 2.f * al[5], box) +
+                         (4.f * al[3] * al[3] + al[4] * al[4] + 4.f * al[5] * al[5]) * area;
+        const float b1 = (as1[1] * as1[1] + as1[2] * as1[2]) * area;
+        const float b2 = (as2[1] * as2[1] + as2[2] * as2[2]) * area;
+        float tau = 0.5f * (fabsf(sl - b1) + fabsf(sl - b2));
+        tau = tau * tau;
+        const float a0 = 0.98f * (1.f + fminf(tau * frcp(1e-6f + sl), 1e30f));
+        const float a1 = 0.01f * (1.f + fminf(tau * frcp(1e-6f + b1), 1e30f));
+        const float a2 = 0.01f * (1.f + fminf(tau * frcp(1e-6f + b2), 1e30f));
+        const float ia = frcp(a0 + a1 + a2);
+        out[k] = a0 * ia * ((1.f / 0.98f) * vl - (0.01f / 0.98f) * (vs1 + vs2)) + (a1 * ia) * vs1 + (a2 * ia) * vs2;
+    }
+    Qf[t.L + s] = make_float4(out[0], out[1], out[2], out[3]);
+}
+
+// restriction-mean slots and ghost fill for the fast path: FP64 state in, both copies out
+__device__ void restrict_mean_q(const DevTables& t, const double4* Q, int node, double4& out) {
+    struct Frame { int node, k; double4 acc; };
+    Frame st[MAX_TREE_DEPTH];
+    int top = 0;
+    st[0].node = node; st[0].k = 0;
+    for (;;) {
+        Frame& fr = st[top];
+        if (fr.k == 4) {
+            const double4 r = make_double4(0.25 * fr.acc.x, 0.25 * fr.acc.y, 0.25 * fr.acc.z, 0.25 * fr.acc.w);
+            if (top == 0) { out = r; return; }
+            top--;
+            Frame& g = st[top];
+            if (g.k == 0) g.acc = r;
+            else { g.acc.x += r.x; g.acc.y += r.y; g.acc.z += r.z; g.acc.w += r.w; }
+            g.k++;
+            continue;
+        }
+        const int c = t.node_son[fr.node] + fr.k;
+        if (t.node_son[c] < 0) {
+            const double4 q = Q[t.node_state[c]];
+            if (fr.k == 0) fr.acc = q;
+            else { fr.acc.x += q.x; fr.acc.y += q.y; fr.acc.z += q.z; fr.acc.w += q.w; }
+            fr.k++;
+        } else {
+            if (top + 1 >= MAX_TREE_DEPTH) { fr.k = 4; continue; }
+            top++;
+            st[top].node = c; st[top].k = 0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(128) void k_slots_restrict_fast(DevTables t, const double4* __restrict__ Q,
+                                                             float4* __restrict__ Qf) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= t.S_r) return;
+    double4 o;
+    restrict_mean_q(t, Q, t.rst_node[s], o);
+    Qf[t.L + t.S_w + s] = make_float4((float)o.x, (float)o.y, (float)o.z, (float)o.w);
+}
+
+__global__ __launch_bounds__(128) void k_bc_fill_fast(DevTables t, double4* __restrict__ Q,
+                                                      float4* __restrict__ Qf) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= t.G) return;
+    const int p = t.bc_partner[g];
+    double4 q;
+    if (p < 0) q = make_double4(t.inflow[0], t.inflow[1], t.inflow[2], t.inflow[3]);
+    else if (t.node_son[p] < 0) q = Q[t.node_state[p]];
+    else restrict_mean_q(t, Q, p, q);
+    Q[t.N + g] = q;
+    Qf[t.N + g] = make_float4((float)q.x, (float)q.y, (float)q.z, (float)q.w);
+}
+
+// FP32 shadow of every leaf state (after an upload)
+__global__ void k_shadow(const double4* __restrict__ Q, float4* __restrict__ Qf, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const double4 q = Q[i];
+        Qf[i] = make_float4((float)q.x, (float)q.y, (float)q.z, (float)q.w);
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+static inline int nb_(int n, int b) { return (n + b - 1) / b; }
+
+void launch_slots_fast(const DevTables& t, const float* wf, const double4* Q, float4* Qf, hipStream_t st) {
+    if (t.S_w > 0) hipLaunchKernelGGL(k_slots_fast, dim3(nb_(t.S_w, 128)), dim3(128), 0, st, t, wf, Qf);
+    if (t.S_r > 0) hipLaunchKernelGGL(k_slots_restrict_fast, dim3(nb_(t.S_r, 128)), dim3(128), 0, st, t, Q, Qf);
+}
+
+void launch_stage_fast(const DevTables& t, const double4* Qin, const float4* Qf_in, const double4* Qn,
+                       double4* Qout, float4* Qf_out, const DevScalars* sc, double alpha, double beta,
+                       int reduce, double* partials, const uint8_t* flags, float* label, hipStream_t st) {
+    if (t.N <= 0) return;
+    hipLaunchKernelGGL(k_stage_fast, dim3(nb_(t.N, 256)), dim3(256), 0, st, t, Qin, Qf_in, Qn, Qout,
+                       Qf_out, sc, alpha, beta, reduce, partials, flags, label);
+}
+
+void launch_bc_fast(const DevTables& t, double4* Q, float4* Qf, hipStream_t st) {
+    if (t.G > 0) hipLaunchKernelGGL(k_bc_fill_fast, dim3(nb_(t.G, 128)), dim3(128), 0, st, t, Q, Qf);
+}
+
+void launch_shadow(const double4* Q, float4* Qf, int n, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_shadow, dim3(nb_(n, 256)), dim3(256), 0, st, Q, Qf, n);
+}
+
+}  // namespace fdm

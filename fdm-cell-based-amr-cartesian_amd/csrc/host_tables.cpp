@@ -149,6 +149,54 @@ void wls_geometry(const Tree& t, int cell, double xx, double yy, const List& l1,
     d[WLS_BOX + 5] = y_max * y_max * y_max - y_min * y_min * y_min;
 }
 
+// Fold a slot's normal-equation solves into FP32 coefficient matrices (all products in
+// FP64, rounded once): arr = C phi with C = (P^T W P)^-1 P^T W, and the fitted value
+// basis(xx,yy)^T arr = c^T phi.
+void fast_record(const double* d, const int32_t* iv, float* f) {
+    const int m = iv[WLSI_M], m1 = iv[WLSI_M1], m2 = m - m1;
+    const double xx = d[WLS_XX], yy = d[WLS_YY];
+    double P[9][6];
+    for (int i = 0; i < m; i++) {
+        const double px = d[WLS_PX + i], py = d[WLS_PY + i];
+        P[i][0] = 1.0; P[i][1] = px; P[i][2] = py; P[i][3] = px * py; P[i][4] = px * px; P[i][5] = py * py;
+    }
+    const double bl[6] = {1.0, xx, yy, xx * yy, xx * xx, yy * yy};
+    for (int i = 0; i < m; i++) {
+        double cl = 0.0;
+        for (int r = 0; r < 6; r++) {
+            double s = 0.0;
+            for (int c = 0; c < 6; c++) s += d[WLS_INV6 + c * 6 + r] * P[i][c];
+            s *= d[WLS_W + i];
+            f[WF_C6 + r * 9 + i] = (float)s;
+            cl += bl[r] * s;
+        }
+        f[WF_CL + i] = (float)cl;
+    }
+    for (int j = 0; j < m1 && j < 5; j++) {
+        double cs = 0.0;
+        for (int r = 0; r < 3; r++) {
+            double s = 0.0;
+            for (int c = 0; c < 3; c++) s += d[WLS_INV3A + c * 3 + r] * P[j][c];
+            s *= d[WLS_W + j];
+            f[WF_C3A + r * 5 + j] = (float)s;
+            cs += bl[r] * s;
+        }
+        f[WF_CS1 + j] = (float)cs;
+    }
+    for (int j = 0; j < m2 && j < 4; j++) {
+        double cs = 0.0;
+        for (int r = 0; r < 3; r++) {
+            double s = 0.0;
+            for (int c = 0; c < 3; c++) s += d[WLS_INV3B + c * 3 + r] * P[m1 + j][c];
+            s *= d[WLS_W + m1 + j];
+            f[WF_C3B + r * 4 + j] = (float)s;
+            cs += bl[r] * s;
+        }
+        f[WF_CS2 + j] = (float)cs;
+    }
+    for (int k = 0; k < 6; k++) f[WF_BOX + k] = (float)d[WLS_BOX + k];
+}
+
 struct Key {
     int kind, node;
     uint64_t xb, yb;
@@ -292,9 +340,11 @@ void build_tables(const Tree& t, Tables& tb) {
     }
     tb.wls_d.resize((size_t)tb.S_w * WLS_STRIDE);
     tb.wls_i.resize((size_t)tb.S_w * WLSI_STRIDE);
+    tb.wls_f.assign((size_t)tb.S_w * WF_STRIDE, 0.0f);
     for (int s = 0; s < tb.S_w; s++) {
         std::memcpy(&tb.wls_d[(size_t)s * WLS_STRIDE], wd[s].data(), sizeof(double) * WLS_STRIDE);
         std::memcpy(&tb.wls_i[(size_t)s * WLSI_STRIDE], wi[s].data(), sizeof(int32_t) * WLSI_STRIDE);
+        fast_record(wd[s].data(), wi[s].data(), &tb.wls_f[(size_t)s * WF_STRIDE]);
     }
     tb.bc_partner.resize(tb.G);
     for (int g = 0; g < tb.G; g++) tb.bc_partner[g] = t.bc_partner(t.bcl[g]);

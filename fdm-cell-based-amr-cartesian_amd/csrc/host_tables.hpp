@@ -28,6 +28,20 @@ enum WlsLayout {
 };
 enum WlsIntLayout { WLSI_M = 0, WLSI_M1 = 1, WLSI_IDS = 2, WLSI_STRIDE = 12 };
 
+// FP32 slot record of the fast path (floats): the least-squares solves folded into
+// coefficient matrices on the host in FP64 (C6 = (P^T W P)^-1 P^T W etc.), plus the value
+// weights c = basis(xx,yy)^T C, so a slot costs a few small mat-vecs per variable.
+enum WlsFastLayout {
+    WF_C6 = 0,      // 6 x 9, row-major [r*9 + i]
+    WF_C3A = 54,    // 3 x 5 (list 1, up to 5 points)
+    WF_C3B = 69,    // 3 x 4 (list 2, up to 4 points)
+    WF_CL = 81,     // 9 value weights of the quadratic fit
+    WF_CS1 = 90,    // 5 value weights of linear fit 1
+    WF_CS2 = 95,    // 4 value weights of linear fit 2
+    WF_BOX = 99,    // 6 box terms (dxb, dyb, x2, y2, x3, y3)
+    WF_STRIDE = 112
+};
+
 // stencil classification, identical to the reference's gathers
 enum StKind { ST_COPY = 0, ST_PARENT_WLS = 1, ST_PARENT_RESTRICT = 2, ST_CHILD_WLS = 3, ST_CHILD_COPY = 4 };
 
@@ -48,6 +62,7 @@ struct Tables {
     // slots
     std::vector<double> wls_d;           // [S_w * WLS_STRIDE]
     std::vector<int32_t> wls_i;          // [S_w * WLSI_STRIDE]
+    std::vector<float> wls_f;            // [S_w * WF_STRIDE] fast-path records
     std::vector<int32_t> rst_node;       // [S_r]
     // ghost fill: partner node per ghost (-1: inflow state)
     std::vector<int32_t> bc_partner;     // [G]

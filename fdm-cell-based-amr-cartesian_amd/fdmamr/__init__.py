@@ -31,7 +31,8 @@ EXPORTS = (
     "fdmamr_get_ghost_state", "fdmamr_leaf_geometry", "fdmamr_max_wavespeed",
     "fdmamr_advance_rk3", "fdmamr_run_steps", "fdmamr_check_positivity", "fdmamr_sensors",
     "fdmamr_regrid", "fdmamr_stencil_info", "fdmamr_stencil_values", "fdmamr_profile",
-    "fdmamr_profile_read",
+    "fdmamr_profile_read", "fdmamr_selector_set_flags", "fdmamr_selector_get_flags",
+    "fdmamr_labels",
 )
 
 
@@ -89,6 +90,9 @@ def lib():
         "fdmamr_stencil_values": ([P, C.c_int32, dp], C.c_int32),
         "fdmamr_profile": ([P, C.c_int32], C.c_int32),
         "fdmamr_profile_read": ([P, dp, ip], C.c_int32),
+        "fdmamr_selector_set_flags": ([P, C.POINTER(C.c_uint8), C.c_int32], C.c_int32),
+        "fdmamr_selector_get_flags": ([P, C.POINTER(C.c_uint8)], C.c_int32),
+        "fdmamr_labels": ([P, C.POINTER(C.c_float)], C.c_int32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -327,6 +331,23 @@ class Solver:
         if rc not in (0, -4):
             self._check(rc, "check_positivity")
         return nb.value
+
+    # -- selector --
+    def set_flags(self, flags, use=True):
+        f = None if flags is None else np.ascontiguousarray(flags, np.uint8)
+        ptr = None if f is None else f.ctypes.data_as(C.POINTER(C.c_uint8))
+        self._check(self.L.fdmamr_selector_set_flags(self.ctx, ptr, 1 if use else 0), "set_flags")
+
+    def get_flags(self):
+        f = np.zeros(self.n_leaf, np.uint8)
+        self._check(self.L.fdmamr_selector_get_flags(self.ctx, f.ctypes.data_as(C.POINTER(C.c_uint8))),
+                    "get_flags")
+        return f
+
+    def labels(self):
+        d = np.zeros(self.n_leaf, np.float32)
+        self._check(self.L.fdmamr_labels(self.ctx, d.ctypes.data_as(C.POINTER(C.c_float))), "labels")
+        return d
 
     # -- measurement --
     def profile(self, enable=True):

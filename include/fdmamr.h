@@ -154,6 +154,19 @@ int32_t fdmamr_stencil_info(fdmamr_ctx* ctx, int32_t i, int32_t* kind8, int32_t*
 /* device-evaluated stencil values (rho,u,v,p) of interior leaf i, 8 positions */
 int32_t fdmamr_stencil_values(fdmamr_ctx* ctx, int32_t i, double* val8x4);
 
+/* ---- troubled-cell selector (no reference counterpart: the reference applies the nonlinear
+ * WCNS weights everywhere, nnd_center_RK.F90:310-353). FP32 mode only. A leaf is "troubled"
+ * when its nonlinear WCNS3 weights deviate from the linear ones; troubled leaves take the full
+ * characteristic reconstruction, cleared leaves the linear-weight one. ---- */
+/* set / read the per-leaf flags (bit 0 = troubled), interior leaves in cell_NS order, and
+ * switch flag use on (use != 0) or off (every leaf takes the full path) */
+int32_t fdmamr_selector_set_flags(fdmamr_ctx* ctx, const uint8_t* flags, int32_t use);
+int32_t fdmamr_selector_get_flags(fdmamr_ctx* ctx, uint8_t* flags);
+/* label of every interior leaf on the current state: the largest deviation |w1 - w1_lin|
+ * over the leaf's 32 WCNS3 reconstructions (FP32, smoothness floored at 1e-6 of the local
+ * mean square so flat data reports 0) */
+int32_t fdmamr_labels(fdmamr_ctx* ctx, float* dev);
+
 /* ---- measurement (no reference counterpart; the reference times only block refinement,
  * mesh.F90:55,76-77) ---- */
 /* bracket every fused stage-kernel launch with HIP events on the context's stream */

@@ -1,0 +1,77 @@
+"""GPU tests of the FP32-flux fast path and the selector flags (through the C-ABI).
+
+Tolerances: the fast path computes the WCNS reconstruction and flux in FP32 with hardware
+reciprocals, so it is compared with the FP64 path by the north-star accuracy bar,
+L1(rho) <= 1e-4 (area-weighted, output_ADT.F90:43-61), on identical meshes.
+"""
+import numpy as np
+import pytest
+
+import fdmamr as F
+from helpers import riemann_c2
+
+pytestmark = pytest.mark.gpu
+
+
+def l1_rho(s, a, b):
+    lvl = s.geometry()["lvl"]
+    area = (1.0 / 2.0 ** lvl) ** 2
+    return float(np.sum(np.abs(a[:, 0] - b[:, 0]) * area) / np.sum(area))
+
+
+def pair(cfg_kw, steps):
+    a = F.Solver(F.Config(**cfg_kw)).setup()
+    b = F.Solver(F.Config(flux_precision=F.FLUX_FP32, **cfg_kw)).setup()
+    return a, b
+
+
+def test_all_troubled_flags_equal_unflagged_path():
+    kw = dict(h=10.0 / 64, localdt=1, cfl=0.5, t_end=1e9)
+    b = F.Solver(F.Config(flux_precision=F.FLUX_FP32, **kw)).setup()
+    c = F.Solver(F.Config(flux_precision=F.FLUX_FP32, **kw)).setup()
+    c.set_flags(np.ones(c.n_leaf, np.uint8), use=True)
+    b.run_steps(5)
+    c.run_steps(5)
+    assert np.array_equal(b.get_state(), c.get_state())
+
+
+def test_cleared_leaves_on_smooth_vortex():
+    # linear weights everywhere on a smooth flow stay within the accuracy bar
+    kw = dict(h=10.0 / 64, localdt=1, cfl=0.5, t_end=1e9)
+    a = F.Solver(F.Config(**kw)).setup()
+    c = F.Solver(F.Config(flux_precision=F.FLUX_FP32, **kw)).setup()
+    c.set_flags(np.zeros(c.n_leaf, np.uint8), use=True)
+    a.run_steps(20)
+    c.run_steps(20)
+    err = l1_rho(a, a.get_state(), c.get_state())
+    print("L1(rho) cleared-FP32 vs FP64", err)
+    assert err < 1e-4
+
+
+def test_labels_find_discontinuities_only():
+    s = F.Solver(F.Config(h=10.0 / 64, flux_precision=F.FLUX_FP32)).setup()
+    lab = s.labels()
+    assert lab.shape == (s.n_leaf,)
+    assert np.mean(lab > 0.05) < 0.05          # smooth vortex: almost nothing troubled
+    r = F.Solver(riemann_c2(h=1.0 / 64, lvl_amr=2, pre_adapt=2, flux_precision=F.FLUX_FP32)).setup()
+    lab = r.labels()
+    frac = np.mean(lab > 0.05)
+    print("troubled fraction riemann", frac)
+    assert 0.005 < frac < 0.6
+
+
+def test_fast_path_riemann_accuracy_and_positivity():
+    cfg = riemann_c2(h=1.0 / 128, lvl_amr=3, pre_adapt=3)
+    a = F.Solver(cfg).setup()
+    t = a.tree()
+    m, n = int(t["num"][:, 1].max()), int(t["num"][:, 0].max())
+    b = F.Solver(riemann_c2(h=1.0 / 128, lvl_amr=3, pre_adapt=3, flux_precision=F.FLUX_FP32))
+    b.import_tree(m, n, t)
+    b.build_tables()
+    b.set_state(a.get_state())
+    a.run_steps(10)
+    b.run_steps(10)
+    err = l1_rho(a, a.get_state(), b.get_state())
+    print("L1(rho) FP32 vs FP64 riemann", err)
+    assert err < 1e-4
+    assert b.check_positivity() == 0
