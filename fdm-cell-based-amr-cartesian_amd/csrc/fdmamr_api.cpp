@@ -13,6 +13,7 @@
 
 #include "../../include/fdmamr.h"
 #include "fdmamr_device.hpp"
+#include "fdmamr_selector.hpp"
 #include "host_tables.hpp"
 #include "host_tree.hpp"
 
@@ -40,6 +41,20 @@ struct fdmamr_ctx {
     DBuf<uint8_t> flags;     // selector flags per interior leaf (bit 0: troubled)
     DBuf<float> label;       // per-leaf WCNS weight deviation (labels / judge)
     bool use_flags = false;  // fast path consults the flags
+
+    // learned selector
+    DBuf<int32_t> off16;
+    DBuf<float> sel_w, sel_m, sel_v, sel_grad, sel_gpart, sel_lpart, sel_loss, sel_prob;
+    DBuf<uint16_t> sel_pk;
+    DBuf<int> sel_cnt;
+    DBuf<uint16_t> ds_x;     // training rows (BF16 features)
+    DBuf<uint8_t> ds_y;
+    DBuf<int32_t> ds_idx;
+    int ds_rows = 0;
+    bool sel_ready = false;
+    float sel_threshold = 0.5f, sel_judge = 0.02f, sel_bound = 0.05f;
+    uint32_t sel_step = 0;
+    int adam_t = 0;
     DBuf<int32_t> stencil, wls_i, rst, bcp, son, state;
     DBuf<int8_t> lvl;
     DBuf<double> wls_d, partials, lampart, rot, div, tmp;
@@ -151,6 +166,9 @@ int upload_tables(fdmamr_ctx* c) {
         launch_shadow(c->Q[k].p, c->Qf[k].p, tb.L, s);
     }
     HIPCHK(c, up(c->wls_f.p, tb.wls_f.data(), tb.wls_f.size() * 4));
+    HIPCHK(c, ensure(c->off16, tb.off16.size()));
+    HIPCHK(c, ensure(c->sel_prob, tb.N));
+    HIPCHK(c, up(c->off16.p, tb.off16.data(), tb.off16.size() * 4));
     // every leaf troubled until the selector says otherwise
     if (tb.N > 0) HIPCHK(c, hipMemsetAsync(c->flags.p, 1, tb.N, s));
     DevTables& d = c->dt;
@@ -209,15 +227,24 @@ void enqueue_rk3(fdmamr_ctx* c, int advance_time) {
     const DevTables& d = c->dt;
     hipStream_t s = c->stream;
     const bool fast = c->p.flux_precision == FDMAMR_FLUX_FP32;
+    // the learned selector runs once per step, on the step's input state, before stage 1;
+    // its judge sample takes the full path in stage 1 and is checked right after it
+    const bool select = fast && c->p.selector && c->sel_ready;
     for (int k = 0; k < 3; k++) {
         if (fast) launch_slots_fast(d, c->wls_f.p, c->Q[in[k]].p, c->Qf[in[k]].p, s);
         else launch_slots(d, c->Q[in[k]].p, c->T[in[k]].p, s);
+        if (select && k == 0) {
+            SelectParams sp{c->sel_threshold, (uint32_t)(c->sel_judge * 65536.0f), c->sel_step++, 8};
+            launch_select(d, c->Qf[0].p, c->off16.p, c->sel_pk.p, c->sel_w.p, sp, c->flags.p, nullptr,
+                          c->sel_cnt.p, s);
+        }
         int e0 = -1, e1 = -1;
         if (c->profiling && (e0 = next_event(c)) >= 0) (void)hipEventRecord(c->ev_pool[e0], s);
         if (fast)
             launch_stage_fast(d, c->Q[in[k]].p, c->Qf[in[k]].p, c->Q[0].p, c->Q[out[k]].p, c->Qf[out[k]].p,
                               c->sc, ra[k], rb[k], k == 2 ? 1 : 0, c->partials.p,
-                              c->use_flags ? c->flags.p : nullptr, nullptr, s);
+                              (c->use_flags || select) ? c->flags.p : nullptr,
+                              (select && k == 0) ? c->label.p : nullptr, s);
         else
             launch_stage(d, 0, c->Q[in[k]].p, c->Q[0].p, c->T[0].p, c->Q[out[k]].p, c->T[out[k]].p,
                          c->sc, ra[k], rb[k], k == 2 ? 1 : 0, c->partials.p, s);
@@ -225,6 +252,7 @@ void enqueue_rk3(fdmamr_ctx* c, int advance_time) {
             (void)hipEventRecord(c->ev_pool[e1], s);
             c->ev_stage.emplace_back(e0, e1);
         }
+        if (select && k == 0) launch_judge(d.N, c->label.p, c->sel_bound, c->flags.p, c->sel_cnt.p, s);
         if (fast) launch_bc_fast(d, c->Q[out[k]].p, c->Qf[out[k]].p, s);
         else launch_bc(d, c->Q[out[k]].p, c->T[out[k]].p, s);
     }
@@ -274,6 +302,11 @@ void fdmamr_destroy(fdmamr_ctx* c) {
     release(c->stencil); release(c->wls_i); release(c->rst); release(c->bcp); release(c->son);
     release(c->state); release(c->lvl); release(c->wls_d); release(c->partials); release(c->lampart);
     release(c->rot); release(c->div); release(c->tmp); release(c->nbad);
+    release(c->Qf[0]); release(c->Qf[1]); release(c->Qf[2]); release(c->wls_f); release(c->flags);
+    release(c->label); release(c->off16); release(c->sel_w); release(c->sel_m); release(c->sel_v);
+    release(c->sel_grad); release(c->sel_gpart); release(c->sel_lpart); release(c->sel_loss);
+    release(c->sel_prob); release(c->sel_pk); release(c->sel_cnt); release(c->ds_x); release(c->ds_y);
+    release(c->ds_idx);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->sc) (void)hipFree(c->sc);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -683,6 +716,253 @@ int32_t fdmamr_labels(fdmamr_ctx* c, float* dev) {
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(dev, c->label.p, c->tb.N * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------- learned selector
+static int sel_alloc(fdmamr_ctx* c) {
+    HIPCHK(c, ensure(c->sel_w, P_TOTAL));
+    HIPCHK(c, ensure(c->sel_m, P_TOTAL));
+    HIPCHK(c, ensure(c->sel_v, P_TOTAL));
+    HIPCHK(c, ensure(c->sel_grad, P_TOTAL));
+    HIPCHK(c, ensure(c->sel_pk, KW_TOTAL));
+    HIPCHK(c, ensure(c->sel_cnt, CNT_N));
+    HIPCHK(c, ensure(c->sel_loss, 4));
+    return 0;
+}
+
+static int sel_upload(fdmamr_ctx* c, const std::vector<float>& w) {
+    int r = sel_alloc(c);
+    if (r) return r;
+    std::vector<float> z(P_TOTAL, 0.0f);
+    HIPCHK(c, hipMemcpyAsync(c->sel_w.p, w.data(), P_TOTAL * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->sel_m.p, z.data(), P_TOTAL * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->sel_v.p, z.data(), P_TOTAL * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->sel_cnt.p, z.data(), CNT_N * 4, hipMemcpyHostToDevice, c->stream));
+    launch_pack(c->sel_w.p, c->sel_pk.p, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->adam_t = 0;
+    c->sel_ready = true;
+    return 0;
+}
+
+int32_t fdmamr_selector_init(fdmamr_ctx* c, uint64_t seed) {
+    if (!c || c->host_only) return fail(c, -5, "no device");
+    std::vector<float> w(P_TOTAL, 0.0f);
+    uint64_t s = seed;
+    auto nrm = [&]() {  // Box-Muller from SplitMix64
+        double u1 = ((splitmix64(s) >> 11) + 1.0) * 0x1.0p-53, u2 = (splitmix64(s) >> 11) * 0x1.0p-53;
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    };
+    for (int i = 0; i < 75; i++)
+        for (int o = 0; o < MLP_H; o++) w[P_W1 + i * MLP_H + o] = (float)(nrm() * std::sqrt(2.0 / 75));
+    for (int k = 0; k < MLP_H * MLP_H; k++) {
+        w[P_W2 + k] = (float)(nrm() * std::sqrt(2.0 / MLP_H));
+        w[P_W3 + k] = (float)(nrm() * std::sqrt(2.0 / MLP_H));
+    }
+    for (int k = 0; k < MLP_H; k++) w[P_W4 + k] = (float)(nrm() * std::sqrt(1.0 / MLP_H));
+    return sel_upload(c, w);
+}
+
+int32_t fdmamr_selector_set_weights(fdmamr_ctx* c, const float* w, int32_t n) {
+    if (!c || c->host_only) return fail(c, -5, "no device");
+    if (!w || n != P_TOTAL) return fail(c, -1, "selector weights: wrong size");
+    std::vector<float> v(w, w + n);
+    for (int i = 75; i < MLP_IN; i++)
+        for (int o = 0; o < MLP_H; o++) v[P_W1 + i * MLP_H + o] = 0.0f;
+    return sel_upload(c, v);
+}
+
+int32_t fdmamr_selector_get_weights(fdmamr_ctx* c, float* w, int32_t n) {
+    if (!c || !c->sel_ready || !w || n != P_TOTAL) return fail(c, -1, "selector weights unavailable");
+    HIPCHK(c, hipMemcpyAsync(w, c->sel_w.p, P_TOTAL * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int32_t fdmamr_selector_config(fdmamr_ctx* c, float threshold, float judge_fraction, float label_bound) {
+    if (!c || !(threshold > 0.f && threshold <= 1.f) || !(judge_fraction >= 0.f && judge_fraction <= 1.f) ||
+        !(label_bound > 0.f))
+        return fail(c, -1, "selector config out of range");
+    c->sel_threshold = threshold;
+    c->sel_judge = judge_fraction;
+    c->sel_bound = label_bound;
+    return 0;
+}
+
+int32_t fdmamr_selector_infer(fdmamr_ctx* c, float* prob, int32_t* n_troubled) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!c->sel_ready) return fail(c, -1, "selector has no weights");
+    const int N = c->tb.N;
+    launch_shadow(c->Q[0].p, c->Qf[0].p, c->tb.L, c->stream);
+    launch_slots_fast(c->dt, c->wls_f.p, c->Q[0].p, c->Qf[0].p, c->stream);
+    std::vector<int> cnt0(CNT_N, 0);
+    HIPCHK(c, hipMemcpyAsync(c->sel_cnt.p, cnt0.data(), CNT_N * 4, hipMemcpyHostToDevice, c->stream));
+    SelectParams sp{c->sel_threshold, 0u, c->sel_step, 8};  // no judge sample here
+    launch_select(c->dt, c->Qf[0].p, c->off16.p, c->sel_pk.p, c->sel_w.p, sp, c->flags.p, c->sel_prob.p,
+                  c->sel_cnt.p, c->stream);
+    HIPCHK(c, hipGetLastError());
+    if (prob && N > 0) HIPCHK(c, hipMemcpyAsync(prob, c->sel_prob.p, N * 4, hipMemcpyDeviceToHost, c->stream));
+    int cnt[CNT_N];
+    HIPCHK(c, hipMemcpyAsync(cnt, c->sel_cnt.p, CNT_N * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (n_troubled) *n_troubled = cnt[CNT_TROUBLED];
+    c->use_flags = true;
+    return 0;
+}
+
+int32_t fdmamr_selector_features(fdmamr_ctx* c, uint16_t* x) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!x) return fail(c, -1, "null feature buffer");
+    const int N = c->tb.N;
+    DBuf<uint16_t> tmp;
+    HIPCHK(c, ensure(tmp, (size_t)N * MLP_IN));
+    launch_shadow(c->Q[0].p, c->Qf[0].p, c->tb.L, c->stream);
+    launch_slots_fast(c->dt, c->wls_f.p, c->Q[0].p, c->Qf[0].p, c->stream);
+    launch_features(c->dt, c->Qf[0].p, c->off16.p, tmp.p, 0, c->stream);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && N > 0)
+        e = hipMemcpyAsync(x, tmp.p, (size_t)N * MLP_IN * 2, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    release(tmp);
+    if (e != hipSuccess) return fail(c, -2, hipGetErrorString(e));
+    return 0;
+}
+
+int32_t fdmamr_selector_clear_dataset(fdmamr_ctx* c) {
+    if (!c) return -1;
+    c->ds_rows = 0;
+    return 0;
+}
+
+int32_t fdmamr_selector_collect(fdmamr_ctx* c, int32_t* n_rows, int32_t* n_positive) {
+    int r = check_ready(c);
+    if (r) return r;
+    const int N = c->tb.N;
+    const size_t need = (size_t)c->ds_rows + N;
+    if (need > 0x7fffffff / MLP_IN) return fail(c, -1, "training set too large");
+    if (c->ds_x.cap < need * MLP_IN) {
+        // grow, keeping the rows collected so far
+        DBuf<uint16_t> nx;
+        DBuf<uint8_t> ny;
+        HIPCHK(c, ensure(nx, need * MLP_IN * 2));
+        HIPCHK(c, ensure(ny, need * 2));
+        if (c->ds_rows) {
+            HIPCHK(c, hipMemcpyAsync(nx.p, c->ds_x.p, (size_t)c->ds_rows * MLP_IN * 2, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(ny.p, c->ds_y.p, (size_t)c->ds_rows, hipMemcpyDeviceToDevice, c->stream));
+        }
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        release(c->ds_x);
+        release(c->ds_y);
+        c->ds_x = nx;
+        c->ds_y = ny;
+    }
+    // labels: one full-path FP32 substep with label output (state unchanged)
+    std::vector<float> dummy;
+    DevScalars s{};
+    s.h = c->tree.h; s.xf = c->tree.xf;
+    HIPCHK(c, hipMemcpyAsync(c->sc, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
+    launch_shadow(c->Q[0].p, c->Qf[0].p, c->tb.L, c->stream);
+    launch_slots_fast(c->dt, c->wls_f.p, c->Q[0].p, c->Qf[0].p, c->stream);
+    launch_stage_fast(c->dt, c->Q[0].p, c->Qf[0].p, c->Q[0].p, c->Q[2].p, c->Qf[2].p, c->sc, 0.0, 1.0, 0,
+                      c->partials.p, nullptr, c->label.p, c->stream);
+    launch_features(c->dt, c->Qf[0].p, c->off16.p, c->ds_x.p, c->ds_rows, c->stream);
+    launch_labels_to_y(N, c->label.p, c->sel_bound, c->ds_y.p, c->ds_rows, c->stream);
+    HIPCHK(c, hipGetLastError());
+    std::vector<uint8_t> yy(N);
+    if (N > 0) HIPCHK(c, hipMemcpyAsync(yy.data(), c->ds_y.p + c->ds_rows, N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->ds_rows += N;
+    int pos = 0;
+    for (uint8_t v : yy) pos += v;
+    if (n_rows) *n_rows = c->ds_rows;
+    if (n_positive) *n_positive = pos;
+    return 0;
+}
+
+static int sel_grad_rows(fdmamr_ctx* c, const uint16_t* X, const uint8_t* y, const int32_t* rows, int n,
+                         float pos_weight) {
+    const int ntiles = (n + 15) / 16;
+    const int nb = std::max(1, std::min(ntiles, 512));
+    HIPCHK(c, ensure(c->sel_gpart, (size_t)nb * P_TOTAL));
+    HIPCHK(c, ensure(c->sel_lpart, nb));
+    launch_mlp_grad(c->sel_pk.p, c->sel_w.p, X, y, rows, n, pos_weight, 1.0f / (float)std::max(n, 1),
+                    c->sel_gpart.p, c->sel_lpart.p, nb, c->sel_grad.p, c->sel_loss.p, c->stream);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+int32_t fdmamr_selector_train(fdmamr_ctx* c, int32_t iters, int32_t batch, float lr, float pos_weight,
+                              uint64_t seed, float* loss) {
+    if (!c || c->host_only) return fail(c, -5, "no device");
+    if (!c->sel_ready) return fail(c, -1, "selector has no weights");
+    if (c->ds_rows == 0 || iters < 0 || batch <= 0) return fail(c, -1, "nothing to train on");
+    HIPCHK(c, ensure(c->ds_idx, batch));
+    std::vector<int32_t> idx(batch);
+    uint64_t s = seed;
+    float last = 0.f;
+    for (int it = 0; it < iters; it++) {
+        for (int b = 0; b < batch; b++) idx[b] = (int32_t)(splitmix64(s) % (uint64_t)c->ds_rows);
+        HIPCHK(c, hipMemcpyAsync(c->ds_idx.p, idx.data(), batch * 4, hipMemcpyHostToDevice, c->stream));
+        int r = sel_grad_rows(c, c->ds_x.p, c->ds_y.p, c->ds_idx.p, batch, pos_weight);
+        if (r) return r;
+        c->adam_t++;
+        const float b1 = 0.9f, b2 = 0.999f;
+        launch_adam(c->sel_w.p, c->sel_grad.p, c->sel_m.p, c->sel_v.p, lr, b1, b2, 1e-8f,
+                    1.0f - std::pow(b1, (float)c->adam_t), 1.0f - std::pow(b2, (float)c->adam_t), c->stream);
+        launch_pack(c->sel_w.p, c->sel_pk.p, c->stream);
+        HIPCHK(c, hipGetLastError());
+        // the host-side index buffer is rewritten next iteration: wait for this one's copy
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    if (iters > 0) {
+        HIPCHK(c, hipMemcpyAsync(&last, c->sel_loss.p, 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    if (loss) *loss = last / (float)batch;
+    return 0;
+}
+
+int32_t fdmamr_selector_grad(fdmamr_ctx* c, const uint16_t* x, const uint8_t* y, int32_t n, float pos_weight,
+                             float* grad, float* loss) {
+    if (!c || c->host_only) return fail(c, -5, "no device");
+    if (!c->sel_ready || !x || !y || n <= 0) return fail(c, -1, "selector grad: bad arguments");
+    DBuf<uint16_t> dx;
+    DBuf<uint8_t> dy;
+    HIPCHK(c, ensure(dx, (size_t)n * MLP_IN));
+    HIPCHK(c, ensure(dy, n));
+    HIPCHK(c, hipMemcpyAsync(dx.p, x, (size_t)n * MLP_IN * 2, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dy.p, y, n, hipMemcpyHostToDevice, c->stream));
+    int r = sel_grad_rows(c, dx.p, dy.p, nullptr, n, pos_weight);
+    float l = 0.f;
+    if (!r) {
+        if (grad) HIPCHK(c, hipMemcpyAsync(grad, c->sel_grad.p, P_TOTAL * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(&l, c->sel_loss.p, 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    release(dx);
+    release(dy);
+    if (loss) *loss = l / (float)n;
+    return r;
+}
+
+int32_t fdmamr_selector_stats(fdmamr_ctx* c, int32_t* troubled, int32_t* judged, int32_t* rejected,
+                              int32_t reset) {
+    if (!c || !c->sel_ready) return fail(c, -1, "selector has no weights");
+    int cnt[CNT_N];
+    HIPCHK(c, hipMemcpyAsync(cnt, c->sel_cnt.p, CNT_N * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (troubled) *troubled = cnt[CNT_TROUBLED];
+    if (judged) *judged = cnt[CNT_JUDGED];
+    if (rejected) *rejected = cnt[CNT_REJECTED];
+    if (reset) {
+        std::vector<int> z(CNT_N, 0);
+        HIPCHK(c, hipMemcpyAsync(c->sel_cnt.p, z.data(), CNT_N * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
     return 0;
 }
 

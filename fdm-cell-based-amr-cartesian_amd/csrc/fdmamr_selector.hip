@@ -1,0 +1,520 @@
+// fdmamr_selector.hip -- learned troubled-cell selector (no reference counterpart).
+//
+// A 4-layer perceptron (75 -> 64 -> 64 -> 64 -> 1, GELU, sigmoid) reads each interior leaf's
+// 5x5 stencil of (rho, p, |u|), each channel centred on the leaf and scaled by its largest
+// deviation on the stencil (plus 1e-3 of |centre|, so flat data maps to ~0), and predicts
+// whether the leaf's nonlinear WCNS3 weights deviate from the linear ones.
+//
+// Matrix cores: every layer is a BF16 MFMA (v_mfma_f32_16x16x16_bf16, FP32 accumulate)
+// computed transposed, H_next^T = W^T H^T, so the 16x16 accumulator tile of one layer is
+// exactly the B operand of the next (no LDS transposes on the forward path). A wave handles
+// 16 leaves per tile; weights live in LDS as BF16, transposed for the A operand.
+//
+// Training (forward + backward) uses the same MFMA tiles; weight gradients are MFMA products
+// with the leaf dimension as K, operands staged through LDS in [unit][leaf] layout.
+// FP32 master weights, Adam, BF16 copies repacked after every update.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fdmamr_device.hpp"
+#include "fdmamr_selector.hpp"
+
+namespace fdm {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// GELU, tanh form, and its derivative
+__device__ __forceinline__ float gelu(float x) {
+    const float k = 0.7978845608028654f;
+    return 0.5f * x * (1.f + tanhf(k * (x + 0.044715f * x * x * x)));
+}
+__device__ __forceinline__ float gelu_d(float x) {
+    const float k = 0.7978845608028654f;
+    const float t = tanhf(k * (x + 0.044715f * x * x * x));
+    return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * k * (1.f + 3.f * 0.044715f * x * x);
+}
+
+__device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
+    bf16x4 r;
+    r[0] = (short)f2bf(a); r[1] = (short)f2bf(b); r[2] = (short)f2bf(c); r[3] = (short)f2bf(d);
+    return r;
+}
+__device__ __forceinline__ bf16x4 lds4(const uint16_t* p) {
+    bf16x4 r;
+    r[0] = (short)p[0]; r[1] = (short)p[1]; r[2] = (short)p[2]; r[3] = (short)p[3];
+    return r;
+}
+
+// ---------------------------------------------------------------- features
+// Lane l of a wave computes for leaf (l % 16) of the tile the stencil points p = j, j+4, ...
+// (j = l / 16) of the 5x5 grid (p = (b+2)*5 + (a+2)); the four lanes of a leaf combine their
+// maxima by shuffles. Writes 80 BF16 features per leaf (75 used) to X (LDS or global).
+__device__ __forceinline__ void leaf_features(const DevTables& t, const float4* __restrict__ Qf,
+                                              const int32_t* __restrict__ off16, int leaf, bool valid,
+                                              uint16_t* xrow) {
+    const int lane = threadIdx.x & 63, part = lane >> 4;
+    float c[3] = {0.f, 0.f, 0.f};
+    float vals[7][3];
+    int np = 0;
+    if (valid) {
+        const float4 q = Qf[leaf];
+        c[0] = q.x; c[1] = q.w; c[2] = sqrtf(q.y * q.y + q.z * q.z);
+    }
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+        const int p = part + 4 * k;
+        vals[k][0] = vals[k][1] = vals[k][2] = 0.f;
+        if (p < 25 && valid) {
+            const int a = p % 5 - 2, b = p / 5 - 2;
+            int idx;
+            if (a == 0 && b == 0) idx = leaf;
+            else if (b == 0) idx = t.stencil[(size_t)(a == 1 ? 0 : a == -1 ? 1 : a == 2 ? 4 : 5) * t.N + leaf];
+            else if (a == 0) idx = t.stencil[(size_t)(b == 1 ? 2 : b == -1 ? 3 : b == 2 ? 6 : 7) * t.N + leaf];
+            else {
+                const int q16 = (b + 2 - (b > 0 ? 1 : 0)) * 4 + (a + 2 - (a > 0 ? 1 : 0));
+                idx = off16[(size_t)q16 * t.N + leaf];
+            }
+            const float4 q = Qf[idx];
+            vals[k][0] = q.x; vals[k][1] = q.w; vals[k][2] = sqrtf(q.y * q.y + q.z * q.z);
+            np = k + 1;
+        }
+    }
+    float s[3];
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+        float mx = 0.f;
+#pragma unroll
+        for (int k = 0; k < 7; k++) mx = fmaxf(mx, fabsf(vals[k][f] - c[f]));
+        mx = fmaxf(mx, __shfl_xor(mx, 16));
+        mx = fmaxf(mx, __shfl_xor(mx, 32));
+        s[f] = 1.f / (mx + 1e-3f * fabsf(c[f]) + 1e-30f);
+    }
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+        const int p = part + 4 * k;
+        if (p < 25)
+#pragma unroll
+            for (int f = 0; f < 3; f++) xrow[p * 3 + f] = f2bf(k < np ? (vals[k][f] - c[f]) * s[f] : 0.f);
+    }
+    if (part == 0)
+#pragma unroll
+        for (int f = 75; f < MLP_IN; f++) xrow[f] = 0;
+}
+
+// ---------------------------------------------------------------- forward (one 16-leaf tile)
+struct FwdOut {
+    f32x4 z[3][4];     // pre-activations per layer, 4 unit tiles (C layout: unit 16mt+4(l/16)+j, leaf l%16)
+    f32x4 h3[4];       // last hidden layer (FP32)
+    float z4;          // logit of leaf l%16 (valid in every lane of the leaf)
+};
+
+// sW: LDS weights (W1T [64][80], W2T [64][64], W3T [64][64] as BF16); sB: b1,b2,b3,w4 (FP32) + b4
+__device__ __forceinline__ void mlp_forward(const uint16_t* sW, const float* sB, const uint16_t* xt,
+                                            FwdOut& o, uint16_t* sH1, uint16_t* sH2) {
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    bf16x4 hb[4];
+    // layer 1: K = 80 (5 k-steps); B operand from the X tile [leaf][80]
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+        f32x4 acc;
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[j] = sB[SB_B1 + 16 * mt + 4 * g + j];
+#pragma unroll
+        for (int s = 0; s < 5; s++) {
+            const bf16x4 a = lds4(sW + KW_W1T + (16 * mt + r) * MLP_IN + 16 * s + 4 * g);
+            const bf16x4 b = lds4(xt + r * MLP_IN + 16 * s + 4 * g);
+            acc = mfma16(a, b, acc);
+        }
+        o.z[0][mt] = acc;
+    }
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+        float h[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) h[j] = gelu(o.z[0][mt][j]);
+        hb[mt] = pack4(h[0], h[1], h[2], h[3]);
+        if (sH1)
+#pragma unroll
+            for (int j = 0; j < 4; j++) sH1[(16 * mt + 4 * g + j) * 16 + r] = (uint16_t)hb[mt][j];
+    }
+    // layers 2 and 3: K = 64 (4 k-steps); B operand = previous accumulator tiles
+#pragma unroll
+    for (int L = 1; L < 3; L++) {
+        const int woff = L == 1 ? KW_W2T : KW_W3T;
+        const int boff = L == 1 ? SB_B2 : SB_B3;
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) {
+            f32x4 acc;
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[j] = sB[boff + 16 * mt + 4 * g + j];
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const bf16x4 a = lds4(sW + woff + (16 * mt + r) * MLP_H + 16 * s + 4 * g);
+                acc = mfma16(a, hb[s], acc);
+            }
+            o.z[L][mt] = acc;
+        }
+        if (L == 1) {
+#pragma unroll
+            for (int mt = 0; mt < 4; mt++) {
+                float h[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) h[j] = gelu(o.z[1][mt][j]);
+                hb[mt] = pack4(h[0], h[1], h[2], h[3]);
+                if (sH2)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) sH2[(16 * mt + 4 * g + j) * 16 + r] = (uint16_t)hb[mt][j];
+            }
+        }
+    }
+    // output unit: dot(w4, h3) over the 64 units (4 lanes of a leaf hold 16 each)
+    float part = 0.f;
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const float h = gelu(o.z[2][mt][j]);
+            o.h3[mt][j] = h;
+            part += sB[SB_W4 + 16 * mt + 4 * g + j] * h;
+        }
+    part += __shfl_xor(part, 16);
+    part += __shfl_xor(part, 32);
+    o.z4 = part + sB[SB_B4];
+}
+
+__device__ __forceinline__ void load_weights(const uint16_t* __restrict__ wpk, const float* __restrict__ w32,
+                                             uint16_t* sW, float* sB, int n_bf) {
+    for (int k = threadIdx.x; k < n_bf; k += blockDim.x) sW[k] = wpk[k];
+    for (int k = threadIdx.x; k < MLP_H; k += blockDim.x) {
+        sB[SB_B1 + k] = w32[P_B1 + k];
+        sB[SB_B2 + k] = w32[P_B2 + k];
+        sB[SB_B3 + k] = w32[P_B3 + k];
+        sB[SB_W4 + k] = w32[P_W4 + k];
+    }
+    if (threadIdx.x == 0) sB[SB_B4] = w32[P_B4];
+}
+
+__device__ __forceinline__ uint32_t hash32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+    return x;
+}
+
+// ---------------------------------------------------------------- inference + judge sampling
+// flags bit 0: troubled (full path); bit 1: judged this step (cleared by the network but
+// sent through the full path with a label); bit 2: forced troubled by an earlier judge
+// rejection (sticky until the tables are rebuilt).
+__global__ __launch_bounds__(256) void k_select(DevTables t, const float4* __restrict__ Qf,
+                                                const int32_t* __restrict__ off16,
+                                                const uint16_t* __restrict__ wpk,
+                                                const float* __restrict__ w32, SelectParams sp,
+                                                uint8_t* __restrict__ flags, float* __restrict__ prob,
+                                                int* __restrict__ counters) {
+    __shared__ uint16_t sW[KW_FWD];
+    __shared__ float sB[SB_TOTAL];
+    __shared__ uint16_t sX[4][16 * MLP_IN];
+    load_weights(wpk, w32, sW, sB, KW_FWD);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int ntiles = (t.N + 15) / 16;
+    int n_tr = 0;
+    for (int it = 0; it < sp.tiles_per_wave; it++) {
+        const int tile = (blockIdx.x * sp.tiles_per_wave + it) * 4 + wv;
+        const int leaf = tile * 16 + (lane & 15);
+        const bool valid = tile < ntiles && leaf < t.N;
+        leaf_features(t, Qf, off16, leaf, valid, sX[wv] + (lane & 15) * MLP_IN);
+        __syncthreads();
+        FwdOut o;
+        mlp_forward(sW, sB, sX[wv], o, nullptr, nullptr);
+        __syncthreads();
+        if (valid && lane < 16) {
+            const float pr = 1.f / (1.f + __expf(-o.z4));
+            if (prob) prob[leaf] = pr;
+            uint8_t f = flags[leaf] & 4;
+            if (pr >= sp.threshold || f) f |= 1;
+            else if ((hash32((uint32_t)leaf * 2654435761u ^ sp.step) & 0xFFFFu) < sp.judge_q16) f |= 3;
+            flags[leaf] = f;
+            n_tr += f & 1;
+        }
+    }
+    if (counters) {
+        for (int o = 32; o > 0; o >>= 1) n_tr += __shfl_xor(n_tr, o);
+        if (lane == 0 && n_tr) atomicAdd(&counters[CNT_TROUBLED], n_tr);
+    }
+}
+
+// judge: a judged leaf whose label exceeds the bound is forced back to the full path
+__global__ __launch_bounds__(256) void k_judge(int n, const float* __restrict__ label, float bound,
+                                               uint8_t* __restrict__ flags, int* __restrict__ counters) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int judged = 0, rejected = 0;
+    if (i < n) {
+        const uint8_t f = flags[i];
+        if (f & 2) {
+            judged = 1;
+            if (label[i] > bound) { rejected = 1; flags[i] = 5; }
+            else flags[i] = f & 4;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) { judged += __shfl_xor(judged, o); rejected += __shfl_xor(rejected, o); }
+    if ((threadIdx.x & 63) == 0) {
+        if (judged) atomicAdd(&counters[CNT_JUDGED], judged);
+        if (rejected) atomicAdd(&counters[CNT_REJECTED], rejected);
+    }
+}
+
+// features of every leaf to global (training data / tests)
+__global__ __launch_bounds__(256) void k_features(DevTables t, const float4* __restrict__ Qf,
+                                                  const int32_t* __restrict__ off16, uint16_t* __restrict__ X,
+                                                  int row0) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int tile = blockIdx.x * 4 + wv;
+    const int leaf = tile * 16 + (lane & 15);
+    const bool valid = leaf < t.N;
+    __shared__ uint16_t sX[4][16 * MLP_IN];
+    leaf_features(t, Qf, off16, leaf, valid, sX[wv] + (lane & 15) * MLP_IN);
+    __syncthreads();
+    // lanes copy the tile out (16 rows of 80)
+    for (int k = lane; k < 16 * MLP_IN; k += 64) {
+        const int rr = k / MLP_IN;
+        const int lf = tile * 16 + rr;
+        if (lf < t.N) X[(size_t)(row0 + lf) * MLP_IN + (k % MLP_IN)] = sX[wv][k];
+    }
+}
+
+__global__ void k_labels_to_y(int n, const float* __restrict__ label, float bound, uint8_t* __restrict__ y,
+                              int row0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[row0 + i] = label[i] > bound ? 1 : 0;
+}
+
+// ---------------------------------------------------------------- training
+// One wave per block; the block accumulates its gradient privately in gpart[blockIdx.x] (no
+// atomics: every gradient element of a tile is owned by one lane), reduced afterwards.
+__global__ __launch_bounds__(64) void k_mlp_grad(const uint16_t* __restrict__ wpk, const float* __restrict__ w32,
+                                                 const uint16_t* __restrict__ X, const uint8_t* __restrict__ y,
+                                                 const int32_t* __restrict__ rows, int n, float pos_weight,
+                                                 float scale, float* __restrict__ gpart, float* __restrict__ lpart) {
+    __shared__ uint16_t sW[KW_TOTAL];
+    __shared__ float sB[SB_TOTAL];
+    __shared__ uint16_t sX[16 * MLP_IN];
+    __shared__ uint16_t sH1[MLP_H * 16], sH2[MLP_H * 16], sDZ[MLP_H * 16];
+    load_weights(wpk, w32, sW, sB, KW_TOTAL);
+    float* gp = gpart + (size_t)blockIdx.x * P_TOTAL;
+    for (int k = threadIdx.x; k < P_TOTAL; k += 64) gp[k] = 0.f;
+    __syncthreads();
+    const int lane = threadIdx.x, r = lane & 15, g = lane >> 4;
+    const int ntiles = (n + 15) / 16;
+    float loss = 0.f;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int idx = tile * 16 + r;
+        const bool valid = idx < n;
+        const int row = valid ? (rows ? rows[idx] : idx) : 0;
+        for (int k = g; k < MLP_IN / 4; k += 4) {
+            const uint16_t* src = X + (size_t)row * MLP_IN + 4 * k;
+#pragma unroll
+            for (int j = 0; j < 4; j++) sX[r * MLP_IN + 4 * k + j] = valid ? src[j] : 0;
+        }
+        __syncthreads();
+        FwdOut o;
+        mlp_forward(sW, sB, sX, o, sH1, sH2);
+        const float yv = valid ? (float)y[row] : 0.f;
+        const float p = 1.f / (1.f + expf(-o.z4));
+        const float pc = fminf(fmaxf(p, 1e-7f), 1.f - 1e-7f);
+        if (valid && g == 0) loss += -(pos_weight * yv * logf(pc) + (1.f - yv) * logf(1.f - pc));
+        const float dz4 = valid ? scale * (p * (pos_weight * yv + 1.f - yv) - pos_weight * yv) : 0.f;
+        // output layer gradients
+        {
+            float gb = g == 0 ? dz4 : 0.f;
+            for (int o2 = 8; o2 > 0; o2 >>= 1) gb += __shfl_xor(gb, o2);
+            if (lane == 0) gp[P_B4] += gb;
+#pragma unroll
+            for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    float v = o.h3[mt][j] * dz4;
+                    for (int o2 = 8; o2 > 0; o2 >>= 1) v += __shfl_xor(v, o2);
+                    if (r == 0) gp[P_W4 + 16 * mt + 4 * g + j] += v;
+                }
+        }
+        // dz3 = w4 dz4 gelu'(z3)
+        f32x4 dz[4];
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                dz[mt][j] = sB[SB_W4 + 16 * mt + 4 * g + j] * dz4 * gelu_d(o.z[2][mt][j]);
+        // layers 3, 2, 1 (L = 2, 1, 0)
+#pragma unroll
+        for (int L = 2; L >= 0; L--) {
+            const int pw = L == 2 ? P_W3 : L == 1 ? P_W2 : P_W1;
+            const int pb = L == 2 ? P_B3 : L == 1 ? P_B2 : P_B1;
+            // bias gradient: sum over the tile's leaves
+#pragma unroll
+            for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    float v = dz[mt][j];
+                    for (int o2 = 8; o2 > 0; o2 >>= 1) v += __shfl_xor(v, o2);
+                    if (r == 0) gp[pb + 16 * mt + 4 * g + j] += v;
+                }
+            // stage dz (BF16) in [unit][leaf] layout
+            bf16x4 dzb[4];
+#pragma unroll
+            for (int mt = 0; mt < 4; mt++) {
+                dzb[mt] = pack4(dz[mt][0], dz[mt][1], dz[mt][2], dz[mt][3]);
+#pragma unroll
+                for (int j = 0; j < 4; j++) sDZ[(16 * mt + 4 * g + j) * 16 + r] = (uint16_t)dzb[mt][j];
+            }
+            __syncthreads();
+            // weight gradient dW[in][out] = sum_leaf act[in][leaf] dz[out][leaf] (K = 16 leaves)
+            const int nin_t = L == 0 ? 5 : 4;
+            for (int mi = 0; mi < nin_t; mi++) {
+                bf16x4 a;
+                if (L == 0) {
+#pragma unroll
+                    for (int j = 0; j < 4; j++) a[j] = (short)sX[(4 * g + j) * MLP_IN + 16 * mi + r];
+                } else {
+                    const uint16_t* act = L == 2 ? sH2 : sH1;
+                    a = lds4(act + (16 * mi + r) * 16 + 4 * g);
+                }
+#pragma unroll
+                for (int no = 0; no < 4; no++) {
+                    const bf16x4 b = lds4(sDZ + (16 * no + r) * 16 + 4 * g);
+                    f32x4 c = {0.f, 0.f, 0.f, 0.f};
+                    c = mfma16(a, b, c);
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int in = 16 * mi + 4 * g + j, out = 16 * no + r;
+                        if (in < 75) gp[pw + in * MLP_H + out] += c[j];
+                    }
+                }
+            }
+            if (L > 0) {
+                // dh_{L-1}[in][leaf] = W_L[in][out] dz[out][leaf]; then dz_{L-1} = dh gelu'(z_{L-1})
+                const int wn = L == 2 ? KW_W3N : KW_W2N;
+                f32x4 nd[4];
+#pragma unroll
+                for (int mt = 0; mt < 4; mt++) {
+                    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int s = 0; s < 4; s++) {
+                        const bf16x4 a = lds4(sW + wn + (16 * mt + r) * MLP_H + 16 * s + 4 * g);
+                        acc = mfma16(a, dzb[s], acc);
+                    }
+                    nd[mt] = acc;
+                }
+#pragma unroll
+                for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) dz[mt][j] = nd[mt][j] * gelu_d(o.z[L - 1][mt][j]);
+            }
+            __syncthreads();
+        }
+    }
+    for (int o2 = 32; o2 > 0; o2 >>= 1) loss += __shfl_xor(loss, o2);
+    if (lane == 0) lpart[blockIdx.x] = loss;
+}
+
+__global__ void k_reduce_grad(const float* __restrict__ gpart, int nb, const float* __restrict__ lpart,
+                              float* __restrict__ grad, float* __restrict__ loss) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < P_TOTAL) {
+        float s = 0.f;
+        for (int b = 0; b < nb; b++) s += gpart[(size_t)b * P_TOTAL + k];
+        grad[k] = s;
+    }
+    if (k == 0) {
+        float s = 0.f;
+        for (int b = 0; b < nb; b++) s += lpart[b];
+        *loss = s;
+    }
+}
+
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ grad, float* __restrict__ m,
+                       float* __restrict__ v, float lr, float b1, float b2, float eps, float bc1, float bc2) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P_TOTAL) return;
+    if (k < P_B1 && (k / MLP_H) >= 75) return;  // padded input rows stay zero
+    const float gk = grad[k];
+    const float mk = b1 * m[k] + (1.f - b1) * gk;
+    const float vk = b2 * v[k] + (1.f - b2) * gk * gk;
+    m[k] = mk;
+    v[k] = vk;
+    w[k] -= lr * (mk / bc1) / (sqrtf(vk / bc2) + eps);
+}
+
+// BF16 copies for the MFMA operands: W1T [64][80], W2T, W3T [64][64] (transposed: A operand of
+// the forward pass) and W2N, W3N [64][64] (as stored: A operand of the backward data path)
+__global__ void k_pack(const float* __restrict__ w, uint16_t* __restrict__ pk) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < MLP_H * MLP_IN) {
+        const int o = k / MLP_IN, i = k % MLP_IN;
+        pk[KW_W1T + k] = f2bf(w[P_W1 + i * MLP_H + o]);
+    }
+    if (k < MLP_H * MLP_H) {
+        const int o = k / MLP_H, i = k % MLP_H;
+        pk[KW_W2T + k] = f2bf(w[P_W2 + i * MLP_H + o]);
+        pk[KW_W3T + k] = f2bf(w[P_W3 + i * MLP_H + o]);
+        pk[KW_W2N + k] = f2bf(w[P_W2 + k]);
+        pk[KW_W3N + k] = f2bf(w[P_W3 + k]);
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+void launch_select(const DevTables& t, const float4* Qf, const int32_t* off16, const uint16_t* wpk,
+                   const float* w32, const SelectParams& sp, uint8_t* flags, float* prob, int* counters,
+                   hipStream_t st) {
+    if (t.N <= 0) return;
+    const int ntiles = (t.N + 15) / 16;
+    const int per_block = 4 * sp.tiles_per_wave;
+    const int nb = (ntiles + per_block - 1) / per_block;
+    hipLaunchKernelGGL(k_select, dim3(nb), dim3(256), 0, st, t, Qf, off16, wpk, w32, sp, flags, prob, counters);
+}
+
+void launch_judge(int n, const float* label, float bound, uint8_t* flags, int* counters, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_judge, dim3((n + 255) / 256), dim3(256), 0, st, n, label, bound, flags, counters);
+}
+
+void launch_features(const DevTables& t, const float4* Qf, const int32_t* off16, uint16_t* X, int row0,
+                     hipStream_t st) {
+    if (t.N <= 0) return;
+    const int ntiles = (t.N + 15) / 16;
+    hipLaunchKernelGGL(k_features, dim3((ntiles + 3) / 4), dim3(256), 0, st, t, Qf, off16, X, row0);
+}
+
+void launch_labels_to_y(int n, const float* label, float bound, uint8_t* y, int row0, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_labels_to_y, dim3((n + 255) / 256), dim3(256), 0, st, n, label, bound, y, row0);
+}
+
+void launch_mlp_grad(const uint16_t* wpk, const float* w32, const uint16_t* X, const uint8_t* y,
+                     const int32_t* rows, int n, float pos_weight, float scale, float* gpart, float* lpart,
+                     int nblocks, float* grad, float* loss, hipStream_t st) {
+    hipLaunchKernelGGL(k_mlp_grad, dim3(nblocks), dim3(64), 0, st, wpk, w32, X, y, rows, n, pos_weight,
+                       scale, gpart, lpart);
+    hipLaunchKernelGGL(k_reduce_grad, dim3((P_TOTAL + 255) / 256), dim3(256), 0, st, gpart, nblocks, lpart,
+                       grad, loss);
+}
+
+void launch_adam(float* w, const float* grad, float* m, float* v, float lr, float b1, float b2, float eps,
+                 float bc1, float bc2, hipStream_t st) {
+    hipLaunchKernelGGL(k_adam, dim3((P_TOTAL + 255) / 256), dim3(256), 0, st, w, grad, m, v, lr, b1, b2, eps,
+                       bc1, bc2);
+}
+
+void launch_pack(const float* w, uint16_t* pk, hipStream_t st) {
+    hipLaunchKernelGGL(k_pack, dim3((MLP_H * MLP_IN + 255) / 256), dim3(256), 0, st, w, pk);
+}
+
+}  // namespace fdm

@@ -1,6 +1,7 @@
 // host_tables.cpp -- regrid-time flattening of the quadtree (see host_tables.hpp).
 #include "host_tables.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -350,10 +351,36 @@ void build_tables(const Tree& t, Tables& tb) {
     for (int g = 0; g < tb.G; g++) tb.bc_partner[g] = t.bc_partner(t.bcl[g]);
     tb.l_max = lmax;
 
+    // off-axis points of the 5x5 selector stencil
+    tb.off16.assign(16 * (size_t)N, 0);
+    for (int i = 0; i < N; i++) {
+        const int c = t.ns[i];
+        const double dy = t.h / pow2i(t.lvl[c]), dx = t.xf * dy;
+        int q = 0;
+        for (int b = -2; b <= 2; b++)
+            for (int a = -2; a <= 2; a++) {
+                if (a == 0 || b == 0) continue;
+                const double px = t.x[c] + a * dx, py = t.y[c] + b * dy;
+                int nj1 = (int)std::floor((px - t.Xmin) / (t.xf * t.h)) + 1;
+                int ni1 = (int)std::floor((py - t.Ymin) / t.h) + 1;
+                nj1 = std::min(std::max(nj1, 1), t.m);
+                ni1 = std::min(std::max(ni1, 1), t.n);
+                int node = (ni1 - 1) * t.m + nj1 - 1;
+                while (t.son[node] >= 0) {
+                    const bool rt = px > t.x[node], upq = py > t.y[node];
+                    node = t.son[node] + (rt && upq ? 2 : rt ? 1 : upq ? 3 : 0);
+                }
+                tb.off16[(size_t)q * N + i] = tb.state_of_node[node];
+                q++;
+            }
+    }
+
     // every index the kernels dereference is checked here, on the host
     const long ext = (long)tb.L + tb.S_w + tb.S_r;
     for (int32_t r : tb.stencil)
         if (r < 0 || r >= ext) throw std::runtime_error("stencil index out of range");
+    for (int32_t r : tb.off16)
+        if (r < 0 || r >= tb.L) throw std::runtime_error("selector stencil index out of range");
     for (int s = 0; s < tb.S_w; s++) {
         const int32_t* iv = &tb.wls_i[(size_t)s * WLSI_STRIDE];
         if (iv[WLSI_M] < 1 || iv[WLSI_M] > 9 || iv[WLSI_M1] < 0 || iv[WLSI_M1] > iv[WLSI_M])

@@ -66,6 +66,11 @@ struct Tables {
     std::vector<int32_t> rst_node;       // [S_r]
     // ghost fill: partner node per ghost (-1: inflow state)
     std::vector<int32_t> bc_partner;     // [G]
+    // selector input: state index of the leaf containing each of the 16 off-axis points of
+    // the leaf's 5x5 stencil (x + a dx, y + b dy), a,b in -2..2, a != 0, b != 0, row-major
+    // in b then a. New rule (the reference has no 5x5 stencil): descend from the base cell
+    // into the son whose quadrant contains the point, ties to the west/south son.
+    std::vector<int32_t> off16;          // [16][N]
     // device topology for restriction means
     std::vector<int32_t> node_son;       // [n_alloc]
 };

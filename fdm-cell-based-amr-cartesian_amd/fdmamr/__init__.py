@@ -32,8 +32,13 @@ EXPORTS = (
     "fdmamr_advance_rk3", "fdmamr_run_steps", "fdmamr_check_positivity", "fdmamr_sensors",
     "fdmamr_regrid", "fdmamr_stencil_info", "fdmamr_stencil_values", "fdmamr_profile",
     "fdmamr_profile_read", "fdmamr_selector_set_flags", "fdmamr_selector_get_flags",
-    "fdmamr_labels",
+    "fdmamr_labels", "fdmamr_selector_init", "fdmamr_selector_set_weights",
+    "fdmamr_selector_get_weights", "fdmamr_selector_config", "fdmamr_selector_infer",
+    "fdmamr_selector_features", "fdmamr_selector_collect", "fdmamr_selector_clear_dataset",
+    "fdmamr_selector_train", "fdmamr_selector_grad", "fdmamr_selector_stats",
 )
+SELECTOR_NPARAM = 13569
+SELECTOR_NFEAT = 80
 
 
 class Params(C.Structure):
@@ -93,6 +98,19 @@ def lib():
         "fdmamr_selector_set_flags": ([P, C.POINTER(C.c_uint8), C.c_int32], C.c_int32),
         "fdmamr_selector_get_flags": ([P, C.POINTER(C.c_uint8)], C.c_int32),
         "fdmamr_labels": ([P, C.POINTER(C.c_float)], C.c_int32),
+        "fdmamr_selector_init": ([P, C.c_uint64], C.c_int32),
+        "fdmamr_selector_set_weights": ([P, C.POINTER(C.c_float), C.c_int32], C.c_int32),
+        "fdmamr_selector_get_weights": ([P, C.POINTER(C.c_float), C.c_int32], C.c_int32),
+        "fdmamr_selector_config": ([P, C.c_float, C.c_float, C.c_float], C.c_int32),
+        "fdmamr_selector_infer": ([P, C.POINTER(C.c_float), ip], C.c_int32),
+        "fdmamr_selector_features": ([P, C.POINTER(C.c_uint16)], C.c_int32),
+        "fdmamr_selector_collect": ([P, ip, ip], C.c_int32),
+        "fdmamr_selector_clear_dataset": ([P], C.c_int32),
+        "fdmamr_selector_train": ([P, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_uint64,
+                                   C.POINTER(C.c_float)], C.c_int32),
+        "fdmamr_selector_grad": ([P, C.POINTER(C.c_uint16), C.POINTER(C.c_uint8), C.c_int32,
+                                  C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)], C.c_int32),
+        "fdmamr_selector_stats": ([P, ip, ip, ip, C.c_int32], C.c_int32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -348,6 +366,68 @@ class Solver:
         d = np.zeros(self.n_leaf, np.float32)
         self._check(self.L.fdmamr_labels(self.ctx, d.ctypes.data_as(C.POINTER(C.c_float))), "labels")
         return d
+
+    def selector_init(self, seed=0):
+        self._check(self.L.fdmamr_selector_init(self.ctx, seed), "selector_init")
+
+    def selector_set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        self._check(self.L.fdmamr_selector_set_weights(self.ctx, w.ctypes.data_as(C.POINTER(C.c_float)),
+                                                       len(w)), "selector_set_weights")
+
+    def selector_get_weights(self):
+        w = np.zeros(SELECTOR_NPARAM, np.float32)
+        self._check(self.L.fdmamr_selector_get_weights(self.ctx, w.ctypes.data_as(C.POINTER(C.c_float)),
+                                                       len(w)), "selector_get_weights")
+        return w
+
+    def selector_config(self, threshold=0.5, judge_fraction=0.02, label_bound=0.05):
+        self._check(self.L.fdmamr_selector_config(self.ctx, threshold, judge_fraction, label_bound),
+                    "selector_config")
+
+    def selector_infer(self):
+        prob = np.zeros(self.n_leaf, np.float32)
+        nt = C.c_int32()
+        self._check(self.L.fdmamr_selector_infer(self.ctx, prob.ctypes.data_as(C.POINTER(C.c_float)),
+                                                 C.byref(nt)), "selector_infer")
+        return prob, nt.value
+
+    def selector_features(self):
+        x = np.zeros((self.n_leaf, SELECTOR_NFEAT), np.uint16)
+        self._check(self.L.fdmamr_selector_features(self.ctx, x.ctypes.data_as(C.POINTER(C.c_uint16))),
+                    "selector_features")
+        return x
+
+    def selector_collect(self):
+        a, b = C.c_int32(), C.c_int32()
+        self._check(self.L.fdmamr_selector_collect(self.ctx, C.byref(a), C.byref(b)), "selector_collect")
+        return a.value, b.value
+
+    def selector_clear_dataset(self):
+        self._check(self.L.fdmamr_selector_clear_dataset(self.ctx), "selector_clear_dataset")
+
+    def selector_train(self, iters, batch=4096, lr=2e-3, pos_weight=4.0, seed=0):
+        loss = C.c_float()
+        self._check(self.L.fdmamr_selector_train(self.ctx, iters, batch, lr, pos_weight, seed,
+                                                 C.byref(loss)), "selector_train")
+        return loss.value
+
+    def selector_grad(self, x_bf16, y, pos_weight=1.0):
+        x = np.ascontiguousarray(x_bf16, np.uint16)
+        yy = np.ascontiguousarray(y, np.uint8)
+        g = np.zeros(SELECTOR_NPARAM, np.float32)
+        loss = C.c_float()
+        self._check(self.L.fdmamr_selector_grad(self.ctx, x.ctypes.data_as(C.POINTER(C.c_uint16)),
+                                                yy.ctypes.data_as(C.POINTER(C.c_uint8)), len(yy),
+                                                pos_weight, g.ctypes.data_as(C.POINTER(C.c_float)),
+                                                C.byref(loss)), "selector_grad")
+        return loss.value, g
+
+    def selector_stats(self, reset=False):
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(self.L.fdmamr_selector_stats(self.ctx, C.byref(a), C.byref(b), C.byref(c),
+                                                 1 if reset else 0), "selector_stats")
+        return dict(troubled=a.value, judged=b.value, rejected=c.value)
 
     # -- measurement --
     def profile(self, enable=True):

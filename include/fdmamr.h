@@ -167,6 +167,38 @@ int32_t fdmamr_selector_get_flags(fdmamr_ctx* ctx, uint8_t* flags);
  * mean square so flat data reports 0) */
 int32_t fdmamr_labels(fdmamr_ctx* ctx, float* dev);
 
+/* The network: 75 -> 64 -> 64 -> 64 -> 1, GELU, sigmoid; BF16 MFMA with FP32 accumulation.
+ * Parameters (FP32 master copy, FDMAMR_SELECTOR_NPARAM floats): W1 [80][64] (input rows 75..79
+ * are padding and stay zero), b1[64], W2 [64][64], b2, W3 [64][64], b3, w4[64], b4. */
+#define FDMAMR_SELECTOR_NPARAM 13569
+#define FDMAMR_SELECTOR_NFEAT 80
+int32_t fdmamr_selector_init(fdmamr_ctx* ctx, uint64_t seed);                 /* random init */
+int32_t fdmamr_selector_set_weights(fdmamr_ctx* ctx, const float* w, int32_t n);
+int32_t fdmamr_selector_get_weights(fdmamr_ctx* ctx, float* w, int32_t n);
+/* threshold: probability >= threshold is troubled (default 0.5); judge_fraction: share of
+ * cleared leaves re-checked per step (default 0.02); label_bound: |w - w_lin| above which a
+ * leaf is troubled (default 0.05) */
+int32_t fdmamr_selector_config(fdmamr_ctx* ctx, float threshold, float judge_fraction,
+                               float label_bound);
+/* run the network on the current state and set the flags (and switch flag use on) */
+int32_t fdmamr_selector_infer(fdmamr_ctx* ctx, float* prob, int32_t* n_troubled);
+/* BF16 feature rows (FDMAMR_SELECTOR_NFEAT per interior leaf) of the current state */
+int32_t fdmamr_selector_features(fdmamr_ctx* ctx, uint16_t* x_bf16);
+/* append features + labels of every interior leaf of the current state to the device-resident
+ * training set; clear it with fdmamr_selector_clear_dataset */
+int32_t fdmamr_selector_collect(fdmamr_ctx* ctx, int32_t* n_rows, int32_t* n_positive);
+int32_t fdmamr_selector_clear_dataset(fdmamr_ctx* ctx);
+/* Adam on the training set: iters minibatches of `batch` rows drawn with `seed`; weighted
+ * binary cross-entropy (pos_weight on troubled rows); loss of the last minibatch */
+int32_t fdmamr_selector_train(fdmamr_ctx* ctx, int32_t iters, int32_t batch, float lr,
+                              float pos_weight, uint64_t seed, float* loss);
+/* gradient of the mean weighted loss over n given rows (for gradient checks) */
+int32_t fdmamr_selector_grad(fdmamr_ctx* ctx, const uint16_t* x_bf16, const uint8_t* y, int32_t n,
+                             float pos_weight, float* grad, float* loss);
+/* counters since the last reset: leaves flagged troubled, judged, rejected by the judge */
+int32_t fdmamr_selector_stats(fdmamr_ctx* ctx, int32_t* troubled, int32_t* judged,
+                              int32_t* rejected, int32_t reset);
+
 /* ---- measurement (no reference counterpart; the reference times only block refinement,
  * mesh.F90:55,76-77) ---- */
 /* bracket every fused stage-kernel launch with HIP events on the context's stream */
