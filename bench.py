@@ -1,10 +1,20 @@
-"""Benchmark: leaf-cell RK3 updates/s of the hot path on one MI355X.
+"""Benchmark: leaf-cell RK3 updates/s, learned selector + FP32 flux vs plain FP64, one MI355X.
 
-A step is one pass of the reference's solute loop body without regrid (solution.F90:183-215):
-lambda (running max), dt from CFL, three SSP-RK3 stages (level-jump slots, fused WCNS3 flux +
-update, ghost fill) and the residual, all on the device. `value` is the throughput of the
-fast path (FP32 WCNS reconstruction / flux, FP64 RK update); the plain FP64 path is timed on
-the same mesh and state beside it and the L1(rho) difference between the two is reported.
+A step is one pass of the reference's solute loop body (solution.F90:183-215): λ (running
+max), dt from CFL, three SSP-RK3 stages (level-jump slot values, fused flux + update, ghost
+fill) and the residual, all on the device. Regrid (every 10 steps in the workload's spec) is
+timed separately and reported amortised.
+
+`value` = updates/s of the learned path: per step the troubled-cell network runs on the step's
+state (BF16 MFMA), cleared leaves take the linear-weight FP32 reconstruction, troubled ones
+the full characteristic WCNS3 in FP32, the substep and RK blend are FP64, and a 2% sample of
+cleared leaves is re-checked by the full method. The plain FP64 path (the reference's
+arithmetic, bit-exact with the CPU oracle) runs on the same mesh and state beside it;
+L1/L∞(ρ) between the two after the timed steps are reported.
+
+Selector training follows the workload spec: trained on labels that plain runs of the
+vortex and 4-quadrant Riemann configurations produce, then fine-tuned online on the target
+workload when the judge's rejection rate exceeds 1%.
 """
 from __future__ import annotations
 
@@ -25,20 +35,28 @@ for _p in (ROOT, PKG):
 HBM_PEAK_GBS = 8000.0
 
 
-def workload_config(name: str, precision: int):
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def workload_config(name: str, precision: int, selector: int = 0, refine_goal: float = 1.4):
     import fdmamr as F
 
+    if name == "vortex":
+        # isentropic vortex, periodic [0,10]^2, base 64x64, lvlAMR 2 (the reference's criterion)
+        return F.Config(h=10.0 / 64, localdt=1, cfl=0.5, amr=1, lvl_amr=2, pstep_adapt=30,
+                        t_end=1e9, pre_adapt=2, flux_precision=precision, selector=selector)
     if name == "riemann4":
         # seeded 4-quadrant Riemann problem, [0,1]^2, base 256x256, lvl <= 4 (density gradient)
         return F.Config(xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, h=1.0 / 256, gamma=1.4,
                         bc=(F.BC_TRANSMISSIVE,) * 4, ic_kind=F.IC_RIEMANN4, seed=1,
                         ic_param=(0.5, 0.5), localdt=1, cfl=0.5, t_end=0.2, amr=1, lvl_amr=4,
                         pstep_adapt=10, amr_sensor=1, pre_adapt=4, wls_local=1,
-                        flux_precision=precision)
+                        refine_goal_rot=refine_goal, refine_goal_div=refine_goal,
+                        flux_precision=precision, selector=selector)
     if name == "shock4bubbles":
         # Mach 3 shock through 4 seeded bubbles, [0,4]x[0,2], base 1024x512, lvl <= 4
-        g = 1.4
-        M = 3.0
+        g, M = 1.4, 3.0
         p2 = 1 + 2 * g / (g + 1) * (M * M - 1)
         r2 = (g + 1) * M * M / ((g - 1) * M * M + 2)
         u2 = M * np.sqrt(g) * (1 - 1 / r2)
@@ -46,14 +64,14 @@ def workload_config(name: str, precision: int):
                         bc=(F.BC_INFLOW, F.BC_TRANSMISSIVE, F.BC_TRANSMISSIVE, F.BC_TRANSMISSIVE),
                         inflow=(r2, u2, 0.0, p2), ic_kind=F.IC_SHOCK_BUBBLES, seed=2,
                         ic_param=(0.5, M, 4, 0.05, 0.15, 0.1, 5.0), localdt=1, cfl=0.5,
-                        t_end=1e9, amr=1, lvl_amr=4, pstep_adapt=10, amr_sensor=1, pre_adapt=4, wls_local=1,
-                        flux_precision=precision)
+                        t_end=1e9, amr=1, lvl_amr=4, pstep_adapt=10, amr_sensor=1, pre_adapt=4,
+                        wls_local=1, refine_goal_rot=refine_goal, refine_goal_div=refine_goal,
+                        flux_precision=precision, selector=selector)
     raise SystemExit(f"unknown workload {name}")
 
 
 def cpu_baseline(budget_s: float = 12.0):
-    """The CPU oracle (scalar FP64 restatement) on a bounded sample: vortex, base 64x64,
-    uniform, fixed dt, steps until the budget is used."""
+    """The CPU oracle (scalar FP64 C restatement of the reference) on a bounded sample."""
     from oracle import oracle as O
 
     o = O.Oracle(h=10.0 / 64, tstep=1e-3)
@@ -70,14 +88,57 @@ def cpu_baseline(budget_s: float = 12.0):
                        f"{steps} steps in {dt:.1f}s")
 
 
+def clone(cfg, tree, prim):
+    import fdmamr as F
+    s = F.Solver(cfg)
+    m, n = int(tree["num"][:, 1].max()), int(tree["num"][:, 0].max())
+    s.import_tree(m, n, tree)
+    s.build_tables()
+    s.set_state(prim)
+    return s
+
+
+def pretrain_selector(seed, rounds, iters):
+    """Train on labels from plain FP32-flux runs of the vortex and Riemann configurations."""
+    import fdmamr as F
+    t0 = time.perf_counter()
+    trainer = F.Solver(workload_config("riemann4", F.FLUX_FP32)).setup()
+    trainer.selector_init(seed)
+    rows = pos = 0
+    for _ in range(rounds):
+        rows, p = trainer.selector_collect()
+        pos += p
+        trainer.run_steps(4)
+    vort = F.Solver(workload_config("vortex", F.FLUX_FP32)).setup()
+    w0 = trainer.selector_get_weights()
+    vort.selector_set_weights(w0)
+    for _ in range(2):
+        vort.selector_collect()
+        vort.run_steps(5)
+    # move the vortex rows into the trainer's set by training the vortex context as well:
+    # both contexts train the same weights in turn
+    loss = trainer.selector_train(iters, batch=4096, lr=3e-3, pos_weight=4.0, seed=seed)
+    vort.selector_set_weights(trainer.selector_get_weights())
+    loss_v = vort.selector_train(max(iters // 4, 1), batch=4096, lr=1e-3, pos_weight=4.0, seed=seed + 1)
+    trainer.selector_set_weights(vort.selector_get_weights())
+    loss = trainer.selector_train(max(iters // 4, 1), batch=4096, lr=1e-3, pos_weight=4.0, seed=seed + 2)
+    w = trainer.selector_get_weights()
+    log(f"selector pre-trained: {rows} Riemann rows ({pos} troubled) + vortex rows, "
+        f"loss {loss:.4f}/{loss_v:.4f}, {time.perf_counter() - t0:.1f}s")
+    trainer.close()
+    vort.close()
+    return w
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="riemann4")
+    ap.add_argument("--workload", default="shock4bubbles")
+    ap.add_argument("--refine-goal", type=float, default=0.3)
+    ap.add_argument("--train-iters", type=int, default=600)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--fp64-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -85,16 +146,50 @@ def main():
 
     import fdmamr as F
 
+    w = pretrain_selector(7, rounds=6, iters=args.train_iters)
+
     t0 = time.perf_counter()
-    s64 = F.Solver(workload_config(args.workload, F.FLUX_FP64)).setup()
+    s64 = F.Solver(workload_config(args.workload, F.FLUX_FP64, refine_goal=args.refine_goal)).setup()
     t_setup = time.perf_counter() - t0
-    tree = s64.tree()
-    m, n = int(tree["num"][:, 1].max()), int(tree["num"][:, 0].max())
     cnt = s64.counts()
     N = cnt["n_leaf"]
-    print(f"[bench] workload {args.workload}: {N} leaves, {cnt['n_slot_wls']} WLS slots, "
-          f"{cnt['n_slot_restrict']} restriction slots, l_max {cnt['l_max']}, setup {t_setup:.1f}s",
-          file=sys.stderr, flush=True)
+    log(f"workload {args.workload}: {N} leaves, {cnt['n_slot_wls']} WLS slots, "
+        f"{cnt['n_slot_restrict']} restriction slots, l_max {cnt['l_max']}, setup {t_setup:.1f}s")
+
+    # regrid cost (host refine/coarsen/balance + tables + upload), timed on its own
+    t0 = time.perf_counter()
+    s64.regrid()
+    t_regrid = time.perf_counter() - t0
+    cnt = s64.counts()
+    N = cnt["n_leaf"]
+    tree = s64.tree()
+    prim0 = s64.get_state()
+    log(f"regrid {t_regrid:.2f}s -> {N} leaves")
+
+    # learned path: online fine-tune on this workload while the judge rejects > 1%
+    sel = clone(workload_config(args.workload, F.FLUX_FP32, selector=1, refine_goal=args.refine_goal),
+                tree, prim0)
+    sel.selector_set_weights(w)
+    sel.run_steps(args.warmup)
+    st = sel.selector_stats(reset=True)
+    rate = st["rejected"] / max(st["judged"], 1)
+    tuned = 0
+    while rate > 0.01 and tuned < 3:
+        sel.selector_collect()
+        sel.selector_train(200, batch=4096, lr=1e-3, pos_weight=4.0, seed=11 + tuned)
+        sel.selector_clear_dataset()
+        sel.set_state(prim0)
+        sel.build_tables()
+        sel.set_state(prim0)
+        sel.run_steps(args.warmup)
+        st = sel.selector_stats(reset=True)
+        rate = st["rejected"] / max(st["judged"], 1)
+        tuned += 1
+    log(f"judge rejection rate {rate:.4f} after {tuned} online fine-tune round(s)")
+    # restart both paths from the same state on the same mesh
+    sel.build_tables()
+    sel.set_state(prim0)
+    s64.set_state(prim0)
 
     def timed(s, label):
         s.run_steps(args.warmup)
@@ -104,36 +199,30 @@ def main():
         wall = time.perf_counter() - t0
         ms, nl = s.profile_read()
         s.profile(False)
-        print(f"[bench] {label}: {args.steps} steps {wall*1e3:.1f} ms, stage kernel "
-              f"{ms/nl:.3f} ms avg over {nl} launches", file=sys.stderr, flush=True)
+        log(f"{label}: {args.steps} steps {wall * 1e3:.1f} ms, stage kernel {ms / nl:.3f} ms avg "
+            f"over {nl} launches")
         return wall, ms / max(nl, 1)
 
-    prim0 = s64.get_state()
+    sel.selector_stats(reset=True)
+    wall_l, k_l = timed(sel, "learned+fp32")
+    st = sel.selector_stats(reset=True)
     wall64, k64 = timed(s64, "fp64")
-    res = {}
-    if not args.fp64_only:
-        s32 = F.Solver(workload_config(args.workload, F.FLUX_FP32))
-        s32.import_tree(m, n, tree)
-        s32.build_tables()
-        s32.set_state(prim0)
-        wall32, k32 = timed(s32, "fp32")
-        # accuracy on identical meshes: same start, same number of steps
-        a = s64.get_state()[:, 0]
-        b = s32.get_state()[:, 0]
-        geo = s64.geometry()
-        area = (1.0 / 2.0 ** geo["lvl"]) ** 2
-        l1 = float(np.sum(np.abs(a - b) * area) / np.sum(area))
-        linf = float(np.max(np.abs(a - b)))
-        res = dict(wall=wall32, kernel_ms=k32, l1=l1, linf=linf)
+    troubled_frac = st["troubled"] / max(N * (args.steps + args.warmup), 1)
 
-    value_wall = res.get("wall", wall64)
-    kernel_ms = res.get("kernel_ms", k64)
-    ups = args.steps * N / value_wall
+    a = s64.get_state()[:, 0]
+    b = sel.get_state()[:, 0]
+    lvl = s64.geometry()["lvl"]
+    area = (0.5 ** lvl) ** 2
+    l1 = float(np.sum(np.abs(a - b) * area) / np.sum(area))
+    linf = float(np.max(np.abs(a - b)))
+
+    ups = args.steps * N / wall_l
     ups64 = args.steps * N / wall64
-    # algorithmic HBM bytes of one fused stage launch, averaged over the 3 stages:
-    # stencil ids 32 + level 1 + own state 32 + state out 32 + T out 8 + T_n 8 + U_n 32 (stages 2,3)
-    bytes_per_leaf = 32 + 1 + 32 + 32 + 8 + 8 + 32 * 2.0 / 3.0
-    achieved = N * bytes_per_leaf / (kernel_ms * 1e-3) / 1e9
+    # per-leaf algorithmic HBM bytes of one fused stage launch (fast path), averaged over the
+    # 3 stages: ids 32 + lvl 1 + flag 1 + own double4 32 + own float4 16 + out double4 32 +
+    # out float4 16 + U_n double4 32 in stages 2-3
+    bytes_per_leaf = 32 + 1 + 1 + 32 + 16 + 32 + 16 + 32 * 2.0 / 3.0
+    achieved = N * bytes_per_leaf / (k_l * 1e-3) / 1e9
     out = {
         "metric": "leaf-cell RK3 updates/s: learned+FP32 vs plain FP64 at L1(rho) err <=1e-4",
         "value": ups,
@@ -141,22 +230,27 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": value_wall / args.steps * 1e3,
+        "ms_per_step": wall_l / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32-flux/fp64-update" if res else "f64",
+        "dtype": "fp32-flux/fp64-update",
         "data": "synthetic",
         "config": {"workload": args.workload, "n_leaf": N, "l_max": cnt["l_max"],
-                   "wls_slots": cnt["n_slot_wls"], "path": "fp32-flux (selector not yet built)"},
+                   "wls_slots": cnt["n_slot_wls"], "refine_goal": args.refine_goal,
+                   "base": "1024x512" if args.workload == "shock4bubbles" else None},
         "fp64_value": ups64,
         "speedup_vs_fp64": ups / ups64,
-        "l1_rho_vs_fp64": res.get("l1"),
-        "linf_rho_vs_fp64": res.get("linf"),
+        "l1_rho_vs_fp64": l1,
+        "linf_rho_vs_fp64": linf,
+        "troubled_fraction": troubled_frac,
+        "judge": st,
+        "regrid_s": t_regrid,
+        "value_regrid_amortised": args.steps * N / (wall_l + t_regrid * args.steps / 10.0),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_stage", "kernel_ms": kernel_ms,
-                     "bytes_per_leaf_launch": bytes_per_leaf},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_stage_fast",
+                     "kernel_ms": k_l, "bytes_per_leaf_launch": bytes_per_leaf,
+                     "fp64_kernel_ms": k64},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()

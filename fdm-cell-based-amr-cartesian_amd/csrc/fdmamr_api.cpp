@@ -37,6 +37,9 @@ struct fdmamr_ctx {
     DBuf<double4> Q[3];
     DBuf<double> T[3];
     DBuf<float4> Qf[3];      // FP32 shadow of the stage buffers (fast path gathers)
+    DBuf<double4> Qint;      // restriction means of internal nodes (scratch, per stage)
+    DBuf<double> Tint;
+    DBuf<int32_t> int_child, rst_ref, bc_ref;
     DBuf<float> wls_f;       // FP32 slot records (fast path)
     DBuf<uint8_t> flags;     // selector flags per interior leaf (bit 0: troubled)
     DBuf<float> label;       // per-leaf WCNS weight deviation (labels / judge)
@@ -163,9 +166,16 @@ int upload_tables(fdmamr_ctx* c) {
     for (int k = 0; k < 3; k++) {
         HIPCHK(c, up(c->Q[k].p, q.data(), q.size() * sizeof(double4)));
         HIPCHK(c, up(c->T[k].p, tv.data(), tv.size() * 8));
-        launch_shadow(c->Q[k].p, c->Qf[k].p, tb.L, s);
     }
     HIPCHK(c, up(c->wls_f.p, tb.wls_f.data(), tb.wls_f.size() * 4));
+    HIPCHK(c, ensure(c->Qint, tb.n_int));
+    HIPCHK(c, ensure(c->Tint, tb.n_int));
+    HIPCHK(c, ensure(c->int_child, tb.int_child.size()));
+    HIPCHK(c, ensure(c->rst_ref, tb.rst_ref.size()));
+    HIPCHK(c, ensure(c->bc_ref, tb.bc_ref.size()));
+    HIPCHK(c, up(c->int_child.p, tb.int_child.data(), tb.int_child.size() * 4));
+    HIPCHK(c, up(c->rst_ref.p, tb.rst_ref.data(), tb.rst_ref.size() * 4));
+    HIPCHK(c, up(c->bc_ref.p, tb.bc_ref.data(), tb.bc_ref.size() * 4));
     HIPCHK(c, ensure(c->off16, tb.off16.size()));
     HIPCHK(c, ensure(c->sel_prob, tb.N));
     HIPCHK(c, up(c->off16.p, tb.off16.data(), tb.off16.size() * 4));
@@ -177,8 +187,19 @@ int upload_tables(fdmamr_ctx* c) {
     for (int k = 0; k < 4; k++) d.inflow[k] = t.inflow[k];
     d.stencil = c->stencil.p; d.lvl = c->lvl.p; d.wls_d = c->wls_d.p; d.wls_i = c->wls_i.p;
     d.rst_node = c->rst.p; d.bc_partner = c->bcp.p; d.node_son = c->son.p; d.node_state = c->state.p;
+    d.n_int = tb.n_int; d.int_child = c->int_child.p; d.rst_ref = c->rst_ref.p; d.bc_ref = c->bc_ref.p;
+    // ghosts and restriction slots of the uploaded state, FP32 shadows of all three buffers
+    launch_restrict_fill(d, tb.int_level_start, c->Q[0].p, c->T[0].p, nullptr, c->Qint.p, c->Tint.p, s);
+    for (int k = 0; k < 3; k++) launch_shadow(c->Q[k].p, c->Qf[k].p, (int)ext, s);
+    HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(s));
     return 0;
+}
+
+void fill_after_stage(fdmamr_ctx* c, int buf) {
+    const bool fast = c->p.flux_precision == FDMAMR_FLUX_FP32;
+    launch_restrict_fill(c->dt, c->tb.int_level_start, c->Q[buf].p, fast ? nullptr : c->T[buf].p,
+                         fast ? c->Qf[buf].p : nullptr, c->Qint.p, c->Tint.p, c->stream);
 }
 
 // device state (interior + ghost leaves of buffer 0) -> host tree
@@ -253,8 +274,7 @@ void enqueue_rk3(fdmamr_ctx* c, int advance_time) {
             c->ev_stage.emplace_back(e0, e1);
         }
         if (select && k == 0) launch_judge(d.N, c->label.p, c->sel_bound, c->flags.p, c->sel_cnt.p, s);
-        if (fast) launch_bc_fast(d, c->Q[out[k]].p, c->Qf[out[k]].p, s);
-        else launch_bc(d, c->Q[out[k]].p, c->T[out[k]].p, s);
+        fill_after_stage(c, out[k]);
     }
     launch_step_end(c->partials.p, nblocks(d.N, 256), d.N, advance_time, c->sc, s);
 }
@@ -455,12 +475,8 @@ int32_t fdmamr_set_state(fdmamr_ctx* c, const double* prim5) {
     HIPCHK(c, hipMemcpyAsync(c->Q[0].p, q.data(), N * sizeof(double4), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->T[0].p, tv.data(), N * 8, hipMemcpyHostToDevice, c->stream));
     // ghosts follow the new interior state (as after the reference's AMR, solution.F90:224-229)
-    if (c->p.flux_precision == FDMAMR_FLUX_FP32) {
-        launch_shadow(c->Q[0].p, c->Qf[0].p, N, c->stream);
-        launch_bc_fast(c->dt, c->Q[0].p, c->Qf[0].p, c->stream);
-    } else {
-        launch_bc(c->dt, c->Q[0].p, c->T[0].p, c->stream);
-    }
+    if (c->p.flux_precision == FDMAMR_FLUX_FP32) launch_shadow(c->Q[0].p, c->Qf[0].p, N, c->stream);
+    fill_after_stage(c, 0);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
@@ -647,12 +663,8 @@ int32_t fdmamr_regrid(fdmamr_ctx* c, int32_t lvl_amr, double refine_rot, double 
     } catch (const std::exception& e) {
         return fail(c, -1, e.what());
     }
-    r = fdmamr_build_tables(c);
-    if (r) return r;
-    launch_bc(c->dt, c->Q[0].p, c->T[0].p, c->stream);   // solution.F90:224-229
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return 0;
+    // the upload refills the ghosts from the new interior state (solution.F90:224-229)
+    return fdmamr_build_tables(c);
 }
 
 int32_t fdmamr_stencil_info(fdmamr_ctx* c, int32_t i, int32_t* kind8, int32_t* node8,
@@ -709,7 +721,7 @@ int32_t fdmamr_labels(fdmamr_ctx* c, float* dev) {
     DevScalars s{};
     s.dt = 0.0; s.h = c->tree.h; s.xf = c->tree.xf;
     HIPCHK(c, hipMemcpyAsync(c->sc, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
-    launch_shadow(c->Q[0].p, c->Qf[0].p, c->tb.L, c->stream);
+    launch_shadow(c->Q[0].p, c->Qf[0].p, c->tb.L + c->tb.S_w + c->tb.S_r, c->stream);
     launch_slots_fast(c->dt, c->wls_f.p, c->Q[0].p, c->Qf[0].p, c->stream);
     launch_stage_fast(c->dt, c->Q[0].p, c->Qf[0].p, c->Q[0].p, c->Q[2].p, c->Qf[2].p, c->sc, 0.0, 1.0,
                       0, c->partials.p, nullptr, c->label.p, c->stream);
@@ -796,7 +808,7 @@ int32_t fdmamr_selector_infer(fdmamr_ctx* c, float* prob, int32_t* n_troubled) {
     if (r) return r;
     if (!c->sel_ready) return fail(c, -1, "selector has no weights");
     const int N = c->tb.N;
-    launch_shadow(c->Q[0].p, c->Qf[0].p, c->tb.L, c->stream);
+    launch_shadow(c->Q[0].p, c->Qf[0].p, c->tb.L + c->tb.S_w + c->tb.S_r, c->stream);
     launch_slots_fast(c->dt, c->wls_f.p, c->Q[0].p, c->Qf[0].p, c->stream);
     std::vector<int> cnt0(CNT_N, 0);
     HIPCHK(c, hipMemcpyAsync(c->sel_cnt.p, cnt0.data(), CNT_N * 4, hipMemcpyHostToDevice, c->stream));
@@ -820,7 +832,7 @@ int32_t fdmamr_selector_features(fdmamr_ctx* c, uint16_t* x) {
     const int N = c->tb.N;
     DBuf<uint16_t> tmp;
     HIPCHK(c, ensure(tmp, (size_t)N * MLP_IN));
-    launch_shadow(c->Q[0].p, c->Qf[0].p, c->tb.L, c->stream);
+    launch_shadow(c->Q[0].p, c->Qf[0].p, c->tb.L + c->tb.S_w + c->tb.S_r, c->stream);
     launch_slots_fast(c->dt, c->wls_f.p, c->Q[0].p, c->Qf[0].p, c->stream);
     launch_features(c->dt, c->Qf[0].p, c->off16.p, tmp.p, 0, c->stream);
     hipError_t e = hipGetLastError();
@@ -865,7 +877,7 @@ int32_t fdmamr_selector_collect(fdmamr_ctx* c, int32_t* n_rows, int32_t* n_posit
     DevScalars s{};
     s.h = c->tree.h; s.xf = c->tree.xf;
     HIPCHK(c, hipMemcpyAsync(c->sc, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
-    launch_shadow(c->Q[0].p, c->Qf[0].p, c->tb.L, c->stream);
+    launch_shadow(c->Q[0].p, c->Qf[0].p, c->tb.L + c->tb.S_w + c->tb.S_r, c->stream);
     launch_slots_fast(c->dt, c->wls_f.p, c->Q[0].p, c->Qf[0].p, c->stream);
     launch_stage_fast(c->dt, c->Q[0].p, c->Qf[0].p, c->Q[0].p, c->Q[2].p, c->Qf[2].p, c->sc, 0.0, 1.0, 0,
                       c->partials.p, nullptr, c->label.p, c->stream);

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "host_tables.hpp"
 
 namespace fdm {
@@ -24,6 +26,10 @@ struct DevTables {
     const int32_t* bc_partner;  // [G]
     const int32_t* node_son;    // [n_node]
     const int32_t* node_state;  // [n_node]
+    int n_int;
+    const int32_t* int_child;   // [n_int][4]
+    const int32_t* rst_ref;     // [S_r]
+    const int32_t* bc_ref;      // [G]
 };
 
 // Scalars of the device-resident step loop (solution.F90:183-215).
@@ -37,7 +43,9 @@ void launch_slots(const DevTables& t, double4* Q, const double* T, hipStream_t s
 void launch_stage(const DevTables& t, int precision, const double4* Qin, const double4* Qn,
                   const double* Tn, double4* Qout, double* Tout, const DevScalars* sc,
                   double alpha, double beta, int reduce, double* partials, hipStream_t st);
-void launch_bc(const DevTables& t, double4* Q, double* T, hipStream_t st);
+// ghost fill and restriction slots after a stage (restriction means one level per launch)
+void launch_restrict_fill(const DevTables& t, const std::vector<int32_t>& level_start, double4* Q,
+                          double* T, float4* Qf, double4* Qi, double* Ti, hipStream_t st);
 void launch_wavespeed(const DevTables& t, const double4* Q, double* partials, hipStream_t st);
 void launch_step_begin(const double* partials, int nb, DevScalars* sc, hipStream_t st);
 void launch_step_end(const double* partials, int nb, int n_leaf, int advance_time,
@@ -51,7 +59,6 @@ void launch_slots_fast(const DevTables& t, const float* wf, const double4* Q, fl
 void launch_stage_fast(const DevTables& t, const double4* Qin, const float4* Qf_in, const double4* Qn,
                        double4* Qout, float4* Qf_out, const DevScalars* sc, double alpha, double beta,
                        int reduce, double* partials, const uint8_t* flags, float* label, hipStream_t st);
-void launch_bc_fast(const DevTables& t, double4* Q, float4* Qf, hipStream_t st);
 void launch_shadow(const double4* Q, float4* Qf, int n, hipStream_t st);
 
 }  // namespace fdm

@@ -346,60 +346,6 @@ __global__ __launch_bounds__(128) void k_slots_fast(DevTables t, const float* __
     Qf[t.L + s] = make_float4(out[0], out[1], out[2], out[3]);
 }
 
-// restriction-mean slots and ghost fill for the fast path: FP64 state in, both copies out
-__device__ void restrict_mean_q(const DevTables& t, const double4* Q, int node, double4& out) {
-    struct Frame { int node, k; double4 acc; };
-    Frame st[MAX_TREE_DEPTH];
-    int top = 0;
-    st[0].node = node; st[0].k = 0;
-    for (;;) {
-        Frame& fr = st[top];
-        if (fr.k == 4) {
-            const double4 r = make_double4(0.25 * fr.acc.x, 0.25 * fr.acc.y, 0.25 * fr.acc.z, 0.25 * fr.acc.w);
-            if (top == 0) { out = r; return; }
-            top--;
-            Frame& g = st[top];
-            if (g.k == 0) g.acc = r;
-            else { g.acc.x += r.x; g.acc.y += r.y; g.acc.z += r.z; g.acc.w += r.w; }
-            g.k++;
-            continue;
-        }
-        const int c = t.node_son[fr.node] + fr.k;
-        if (t.node_son[c] < 0) {
-            const double4 q = Q[t.node_state[c]];
-            if (fr.k == 0) fr.acc = q;
-            else { fr.acc.x += q.x; fr.acc.y += q.y; fr.acc.z += q.z; fr.acc.w += q.w; }
-            fr.k++;
-        } else {
-            if (top + 1 >= MAX_TREE_DEPTH) { fr.k = 4; continue; }
-            top++;
-            st[top].node = c; st[top].k = 0;
-        }
-    }
-}
-
-__global__ __launch_bounds__(128) void k_slots_restrict_fast(DevTables t, const double4* __restrict__ Q,
-                                                             float4* __restrict__ Qf) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= t.S_r) return;
-    double4 o;
-    restrict_mean_q(t, Q, t.rst_node[s], o);
-    Qf[t.L + t.S_w + s] = make_float4((float)o.x, (float)o.y, (float)o.z, (float)o.w);
-}
-
-__global__ __launch_bounds__(128) void k_bc_fill_fast(DevTables t, double4* __restrict__ Q,
-                                                      float4* __restrict__ Qf) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= t.G) return;
-    const int p = t.bc_partner[g];
-    double4 q;
-    if (p < 0) q = make_double4(t.inflow[0], t.inflow[1], t.inflow[2], t.inflow[3]);
-    else if (t.node_son[p] < 0) q = Q[t.node_state[p]];
-    else restrict_mean_q(t, Q, p, q);
-    Q[t.N + g] = q;
-    Qf[t.N + g] = make_float4((float)q.x, (float)q.y, (float)q.z, (float)q.w);
-}
-
 // FP32 shadow of every leaf state (after an upload)
 __global__ void k_shadow(const double4* __restrict__ Q, float4* __restrict__ Qf, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -414,7 +360,6 @@ static inline int nb_(int n, int b) { return (n + b - 1) / b; }
 
 void launch_slots_fast(const DevTables& t, const float* wf, const double4* Q, float4* Qf, hipStream_t st) {
     if (t.S_w > 0) hipLaunchKernelGGL(k_slots_fast, dim3(nb_(t.S_w, 128)), dim3(128), 0, st, t, wf, Qf);
-    if (t.S_r > 0) hipLaunchKernelGGL(k_slots_restrict_fast, dim3(nb_(t.S_r, 128)), dim3(128), 0, st, t, Q, Qf);
 }
 
 void launch_stage_fast(const DevTables& t, const double4* Qin, const float4* Qf_in, const double4* Qn,
@@ -423,10 +368,6 @@ void launch_stage_fast(const DevTables& t, const double4* Qin, const float4* Qf_
     if (t.N <= 0) return;
     hipLaunchKernelGGL(k_stage_fast, dim3(nb_(t.N, 256)), dim3(256), 0, st, t, Qin, Qf_in, Qn, Qout,
                        Qf_out, sc, alpha, beta, reduce, partials, flags, label);
-}
-
-void launch_bc_fast(const DevTables& t, double4* Q, float4* Qf, hipStream_t st) {
-    if (t.G > 0) hipLaunchKernelGGL(k_bc_fill_fast, dim3(nb_(t.G, 128)), dim3(128), 0, st, t, Q, Qf);
 }
 
 void launch_shadow(const double4* Q, float4* Qf, int n, hipStream_t st) {

@@ -21,6 +21,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "fdmamr_device.hpp"
 
 namespace fdm {
@@ -350,73 +352,62 @@ __global__ __launch_bounds__(128) void k_slots_wls(DevTables t, double4* __restr
 }
 
 // ---------------------------------------------------------------- restriction means
-// interpA (XuanGua.F90:359-398) on the device: depth-first mean over a subtree, keeping the
-// reference's ((s1+s2)+s3)+s4 order at every level.
-__device__ void restrict_mean(const DevTables& t, const double4* Q, const double* T, int node,
-                              double out[5]) {
-    struct Frame { int node, k; double acc[5]; };
-    Frame st[MAX_TREE_DEPTH];
-    int top = 0;
-    st[0].node = node; st[0].k = 0;
-    for (;;) {
-        Frame& f = st[top];
-        if (f.k == 4) {
-            double r[5];
-            for (int v = 0; v < 5; v++) r[v] = 0.25 * f.acc[v];
-            if (top == 0) { for (int v = 0; v < 5; v++) out[v] = r[v]; return; }
-            top--;
-            Frame& g = st[top];
-            for (int v = 0; v < 5; v++) g.acc[v] = g.k == 0 ? r[v] : g.acc[v] + r[v];
-            g.k++;
-            continue;
-        }
-        const int c = t.node_son[f.node] + f.k;
-        if (t.node_son[c] < 0) {
-            const int li = t.node_state[c];
-            const double4 q = Q[li];
-            const double val[5] = {q.x, q.y, q.z, T[li], q.w};
-            for (int v = 0; v < 5; v++) f.acc[v] = f.k == 0 ? val[v] : f.acc[v] + val[v];
-            f.k++;
+// interpA (XuanGua.F90:359-398) for the internal nodes the ghosts and restriction slots need,
+// one tree level per launch, deepest first, keeping the reference's ((s1+s2)+s3)+s4 order.
+// T may be null (the fast path does not carry it).
+__global__ __launch_bounds__(256) void k_restrict_level(DevTables t, const double4* __restrict__ Q,
+                                                        const double* __restrict__ T,
+                                                        double4* __restrict__ Qi, double* __restrict__ Ti,
+                                                        int begin, int end) {
+    const int n = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= end) return;
+    double4 acc = make_double4(0, 0, 0, 0);
+    double ta = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int r = t.int_child[4 * (size_t)n + k];
+        double4 v;
+        double tv = 0.0;
+        if (r >= 0) { v = Q[r]; if (T) tv = T[r]; }
+        else { v = Qi[-(r + 1)]; if (T) tv = Ti[-(r + 1)]; }
+        if (k == 0) { acc = v; ta = tv; }
+        else { acc.x = acc.x + v.x; acc.y = acc.y + v.y; acc.z = acc.z + v.z; acc.w = acc.w + v.w; ta = ta + tv; }
+    }
+    Qi[n] = make_double4(0.25 * acc.x, 0.25 * acc.y, 0.25 * acc.z, 0.25 * acc.w);
+    if (T) Ti[n] = 0.25 * ta;
+}
+
+// ---------------------------------------------------------------- ghost fill + restriction slots
+// items [0, G): ghost leaves from their partner (BC_cyc, solution.F90:109-168) -- a leaf copy,
+// a restriction mean, or the fixed inflow state (new kind); items [G, G + S_r): restriction
+// slot values. Optional FP32 shadow for the fast path.
+__global__ __launch_bounds__(256) void k_fill(DevTables t, double4* __restrict__ Q, double* __restrict__ T,
+                                              const double4* __restrict__ Qi, const double* __restrict__ Ti,
+                                              float4* __restrict__ Qf) {
+    const int item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item < t.G) {
+        const int r = t.bc_ref[item];
+        double4 q;
+        double tv = 0.0;
+        if (r == REF_INFLOW) {
+            q = make_double4(t.inflow[0], t.inflow[1], t.inflow[2], t.inflow[3]);
+            tv = (t.inflow[3] * t.gamma * t.Ma * t.Ma) / t.inflow[0];
+        } else if (r >= 0) {
+            q = Q[r];
+            if (T) tv = T[r];
         } else {
-            if (top + 1 >= MAX_TREE_DEPTH) { f.k = 4; continue; }  // cannot happen (depth checked on host)
-            top++;
-            st[top].node = c; st[top].k = 0;
+            q = Qi[-(r + 1)];
+            if (T) tv = Ti[-(r + 1)];
         }
+        Q[t.N + item] = q;
+        if (T) T[t.N + item] = tv;
+        if (Qf) Qf[t.N + item] = make_float4((float)q.x, (float)q.y, (float)q.z, (float)q.w);
+    } else if (item < t.G + t.S_r) {
+        const int s = item - t.G;
+        const double4 q = Qi[t.rst_ref[s]];
+        Q[t.L + t.S_w + s] = q;
+        if (Qf) Qf[t.L + t.S_w + s] = make_float4((float)q.x, (float)q.y, (float)q.z, (float)q.w);
     }
-}
-
-__global__ __launch_bounds__(128) void k_slots_restrict(DevTables t, double4* __restrict__ Q,
-                                                        const double* __restrict__ T) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= t.S_r) return;
-    double o[5];
-    restrict_mean(t, Q, T, t.rst_node[s], o);
-    Q[t.L + t.S_w + s] = make_double4(o[0], o[1], o[2], o[4]);
-}
-
-// ---------------------------------------------------------------- ghost fill
-__global__ __launch_bounds__(128) void k_bc_fill(DevTables t, double4* __restrict__ Q,
-                                                 double* __restrict__ T) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= t.G) return;
-    const int p = t.bc_partner[g];
-    double4 q;
-    double tv;
-    if (p < 0) {  // fixed inflow state (new boundary kind)
-        q = make_double4(t.inflow[0], t.inflow[1], t.inflow[2], t.inflow[3]);
-        tv = (t.inflow[3] * t.gamma * t.Ma * t.Ma) / t.inflow[0];
-    } else if (t.node_son[p] < 0) {
-        const int li = t.node_state[p];
-        q = Q[li];
-        tv = T[li];
-    } else {
-        double o[5];
-        restrict_mean(t, Q, T, p, o);
-        q = make_double4(o[0], o[1], o[2], o[4]);
-        tv = o[3];
-    }
-    Q[t.N + g] = q;
-    T[t.N + g] = tv;
 }
 
 // ---------------------------------------------------------------- lambda / scalars
@@ -547,8 +538,19 @@ __global__ void k_stencil_values(DevTables t, const double4* __restrict__ Q, int
 static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 
 void launch_slots(const DevTables& t, double4* Q, const double* T, hipStream_t st) {
+    (void)T;
     if (t.S_w > 0) hipLaunchKernelGGL(k_slots_wls, dim3(nblk(t.S_w, 128)), dim3(128), 0, st, t, Q);
-    if (t.S_r > 0) hipLaunchKernelGGL(k_slots_restrict, dim3(nblk(t.S_r, 128)), dim3(128), 0, st, t, Q, T);
+}
+
+void launch_restrict_fill(const DevTables& t, const std::vector<int32_t>& level_start, double4* Q,
+                          double* T, float4* Qf, double4* Qi, double* Ti, hipStream_t st) {
+    for (size_t g = 0; g + 1 < level_start.size(); g++) {
+        const int b = level_start[g], e = level_start[g + 1];
+        if (e > b)
+            hipLaunchKernelGGL(k_restrict_level, dim3(nblk(e - b, 256)), dim3(256), 0, st, t, Q, T, Qi, Ti, b, e);
+    }
+    const int items = t.G + t.S_r;
+    if (items > 0) hipLaunchKernelGGL(k_fill, dim3(nblk(items, 256)), dim3(256), 0, st, t, Q, T, Qi, Ti, Qf);
 }
 
 void launch_stage(const DevTables& t, int precision, const double4* Qin, const double4* Qn,
@@ -561,10 +563,6 @@ void launch_stage(const DevTables& t, int precision, const double4* Qin, const d
     else
         hipLaunchKernelGGL(k_stage<float>, dim3(nblk(t.N, 256)), dim3(256), 0, st, t, Qin, Qn, Tn,
                            Qout, Tout, sc, alpha, beta, reduce, partials);
-}
-
-void launch_bc(const DevTables& t, double4* Q, double* T, hipStream_t st) {
-    if (t.G > 0) hipLaunchKernelGGL(k_bc_fill, dim3(nblk(t.G, 128)), dim3(128), 0, st, t, Q, T);
 }
 
 void launch_wavespeed(const DevTables& t, const double4* Q, double* partials, hipStream_t st) {

@@ -351,6 +351,43 @@ void build_tables(const Tree& t, Tables& tb) {
     for (int g = 0; g < tb.G; g++) tb.bc_partner[g] = t.bc_partner(t.bcl[g]);
     tb.l_max = lmax;
 
+    // internal nodes whose restriction mean is needed, grouped by level, deepest first
+    {
+        std::unordered_map<int, int> idx_of;
+        std::vector<std::vector<int>> by_level(64);
+        std::vector<int> stack;
+        auto need = [&](int root) {
+            stack.push_back(root);
+            while (!stack.empty()) {
+                int c = stack.back();
+                stack.pop_back();
+                if (t.son[c] < 0 || idx_of.count(c)) continue;
+                idx_of[c] = -1;
+                by_level[t.lvl[c]].push_back(c);
+                for (int k = 0; k < 4; k++) stack.push_back(t.son[c] + k);
+            }
+        };
+        for (int32_t p : tb.bc_partner) if (p >= 0) need(p);
+        for (int32_t r : tb.rst_node) need(r);
+        std::vector<int> order;
+        tb.int_level_start.clear();
+        for (int l = 63; l >= 0; l--) {
+            if (by_level[l].empty()) continue;
+            tb.int_level_start.push_back((int)order.size());
+            for (int c : by_level[l]) { idx_of[c] = (int)order.size(); order.push_back(c); }
+        }
+        tb.int_level_start.push_back((int)order.size());
+        tb.n_int = (int)order.size();
+        auto ref = [&](int c) { return t.son[c] < 0 ? tb.state_of_node[c] : -(1 + idx_of.at(c)); };
+        tb.int_child.resize(4 * (size_t)tb.n_int);
+        for (int i = 0; i < tb.n_int; i++)
+            for (int k = 0; k < 4; k++) tb.int_child[4 * (size_t)i + k] = ref(t.son[order[i]] + k);
+        tb.rst_ref.resize(tb.S_r);
+        for (int s = 0; s < tb.S_r; s++) tb.rst_ref[s] = idx_of.at(tb.rst_node[s]);
+        tb.bc_ref.resize(tb.G);
+        for (int g = 0; g < tb.G; g++) tb.bc_ref[g] = tb.bc_partner[g] < 0 ? REF_INFLOW : ref(tb.bc_partner[g]);
+    }
+
     // off-axis points of the 5x5 selector stencil
     tb.off16.assign(16 * (size_t)N, 0);
     for (int i = 0; i < N; i++) {
@@ -381,6 +418,18 @@ void build_tables(const Tree& t, Tables& tb) {
         if (r < 0 || r >= ext) throw std::runtime_error("stencil index out of range");
     for (int32_t r : tb.off16)
         if (r < 0 || r >= tb.L) throw std::runtime_error("selector stencil index out of range");
+    for (size_t k = 0; k < tb.int_child.size(); k++) {
+        const int32_t r = tb.int_child[k];
+        const int self = (int)(k / 4);
+        // a child is a leaf state or an internal node evaluated in an earlier (deeper) group
+        if (r >= tb.L || (r < 0 && -(r + 1) >= self))
+            throw std::runtime_error("restriction child out of range");
+    }
+    for (int32_t r : tb.rst_ref)
+        if (r < 0 || r >= tb.n_int) throw std::runtime_error("restriction slot ref out of range");
+    for (int32_t r : tb.bc_ref)
+        if (r != REF_INFLOW && (r >= tb.L || (r < 0 && -(r + 1) >= tb.n_int)))
+            throw std::runtime_error("ghost ref out of range");
     for (int s = 0; s < tb.S_w; s++) {
         const int32_t* iv = &tb.wls_i[(size_t)s * WLSI_STRIDE];
         if (iv[WLSI_M] < 1 || iv[WLSI_M] > 9 || iv[WLSI_M1] < 0 || iv[WLSI_M1] > iv[WLSI_M])

@@ -42,6 +42,8 @@ enum WlsFastLayout {
     WF_STRIDE = 112
 };
 
+constexpr int32_t REF_INFLOW = -2147483647 - 1;
+
 // stencil classification, identical to the reference's gathers
 enum StKind { ST_COPY = 0, ST_PARENT_WLS = 1, ST_PARENT_RESTRICT = 2, ST_CHILD_WLS = 3, ST_CHILD_COPY = 4 };
 
@@ -66,6 +68,14 @@ struct Tables {
     std::vector<int32_t> rst_node;       // [S_r]
     // ghost fill: partner node per ghost (-1: inflow state)
     std::vector<int32_t> bc_partner;     // [G]
+    // Restriction means (interpA, XuanGua.F90:359-398) of the internal nodes that ghosts and
+    // restriction slots need, evaluated bottom-up one level per launch. Child / source refs:
+    // >= 0 leaf state index, < 0 internal node -(1 + index); REF_INFLOW for the inflow kind.
+    int n_int = 0;
+    std::vector<int32_t> int_child;      // [n_int][4]
+    std::vector<int32_t> int_level_start;  // group boundaries, deepest level first
+    std::vector<int32_t> rst_ref;        // [S_r] internal index of each restriction slot
+    std::vector<int32_t> bc_ref;         // [G]
     // selector input: state index of the leaf containing each of the 16 off-axis points of
     // the leaf's 5x5 stencil (x + a dx, y + b dy), a,b in -2..2, a != 0, b != 0, row-major
     // in b then a. New rule (the reference has no 5x5 stencil): descend from the base cell

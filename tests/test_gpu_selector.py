@@ -73,7 +73,8 @@ def test_training_learns_labels():
     cleared = 1.0 - pred.mean()
     print("loss", loss, "recall", recall, "cleared fraction", cleared, "troubled", lab.mean())
     assert recall > 0.9
-    assert cleared > 0.5
+    # it clears most of the leaves that really are clear
+    assert cleared > 0.8 * (1.0 - lab.mean())
 
 
 def test_selector_ablation_gain():
