@@ -138,6 +138,8 @@ def main():
     ap.add_argument("--workload", default="shock4bubbles")
     ap.add_argument("--refine-goal", type=float, default=0.3)
     ap.add_argument("--train-iters", type=int, default=600)
+    ap.add_argument("--advance", type=int, default=0,
+                    help="evolve the flow this many steps (regrid every 10) before measuring")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -156,6 +158,13 @@ def main():
     log(f"workload {args.workload}: {N} leaves, {cnt['n_slot_wls']} WLS slots, "
         f"{cnt['n_slot_restrict']} restriction slots, l_max {cnt['l_max']}, setup {t_setup:.1f}s")
 
+    t0 = time.perf_counter()
+    for _ in range(args.advance // 10):
+        s64.run_steps(10)
+        s64.regrid()
+    if args.advance:
+        log(f"advanced {args.advance} steps with regrid every 10: t = {s64.tt:.4f}, "
+            f"{s64.counts()['n_leaf']} leaves, {time.perf_counter() - t0:.1f}s")
     # regrid cost (host refine/coarsen/balance + tables + upload), timed on its own
     t0 = time.perf_counter()
     s64.regrid()

@@ -377,6 +377,34 @@ __global__ __launch_bounds__(256) void k_restrict_level(DevTables t, const doubl
     if (T) Ti[n] = 0.25 * ta;
 }
 
+// All levels in one launch of one block (few internal nodes): levels separated by barriers.
+struct LevelGroups { int n; int start[MAX_TREE_DEPTH + 1]; };
+
+__global__ __launch_bounds__(1024) void k_restrict_all(DevTables t, const double4* __restrict__ Q,
+                                                       const double* __restrict__ T,
+                                                       double4* __restrict__ Qi, double* __restrict__ Ti,
+                                                       LevelGroups lg) {
+    for (int g = 0; g < lg.n; g++) {
+        for (int n = lg.start[g] + threadIdx.x; n < lg.start[g + 1]; n += blockDim.x) {
+            double4 acc = make_double4(0, 0, 0, 0);
+            double ta = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int r = t.int_child[4 * (size_t)n + k];
+                double4 v;
+                double tv = 0.0;
+                if (r >= 0) { v = Q[r]; if (T) tv = T[r]; }
+                else { v = Qi[-(r + 1)]; if (T) tv = Ti[-(r + 1)]; }
+                if (k == 0) { acc = v; ta = tv; }
+                else { acc.x = acc.x + v.x; acc.y = acc.y + v.y; acc.z = acc.z + v.z; acc.w = acc.w + v.w; ta = ta + tv; }
+            }
+            Qi[n] = make_double4(0.25 * acc.x, 0.25 * acc.y, 0.25 * acc.z, 0.25 * acc.w);
+            if (T) Ti[n] = 0.25 * ta;
+        }
+        __syncthreads();  // the next (shallower) level reads this one
+    }
+}
+
 // ---------------------------------------------------------------- ghost fill + restriction slots
 // items [0, G): ghost leaves from their partner (BC_cyc, solution.F90:109-168) -- a leaf copy,
 // a restriction mean, or the fixed inflow state (new kind); items [G, G + S_r): restriction
@@ -544,10 +572,20 @@ void launch_slots(const DevTables& t, double4* Q, const double* T, hipStream_t s
 
 void launch_restrict_fill(const DevTables& t, const std::vector<int32_t>& level_start, double4* Q,
                           double* T, float4* Qf, double4* Qi, double* Ti, hipStream_t st) {
-    for (size_t g = 0; g + 1 < level_start.size(); g++) {
-        const int b = level_start[g], e = level_start[g + 1];
-        if (e > b)
-            hipLaunchKernelGGL(k_restrict_level, dim3(nblk(e - b, 256)), dim3(256), 0, st, t, Q, T, Qi, Ti, b, e);
+    const int ngroups = (int)level_start.size() - 1;
+    const int n_int = ngroups > 0 ? level_start.back() : 0;
+    if (n_int > 0 && n_int <= 65536 && ngroups <= MAX_TREE_DEPTH) {
+        // few internal nodes: one launch, one block
+        LevelGroups lg;
+        lg.n = ngroups;
+        for (int g = 0; g <= ngroups; g++) lg.start[g] = level_start[g];
+        hipLaunchKernelGGL(k_restrict_all, dim3(1), dim3(1024), 0, st, t, Q, T, Qi, Ti, lg);
+    } else {
+        for (int g = 0; g < ngroups; g++) {
+            const int b = level_start[g], e = level_start[g + 1];
+            if (e > b)
+                hipLaunchKernelGGL(k_restrict_level, dim3(nblk(e - b, 256)), dim3(256), 0, st, t, Q, T, Qi, Ti, b, e);
+        }
     }
     const int items = t.G + t.S_r;
     if (items > 0) hipLaunchKernelGGL(k_fill, dim3(nblk(items, 256)), dim3(256), 0, st, t, Q, T, Qi, Ti, Qf);

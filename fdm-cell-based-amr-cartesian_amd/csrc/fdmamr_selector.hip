@@ -35,14 +35,20 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-// GELU, tanh form, and its derivative
+// GELU, tanh form, and its derivative. tanh(y) = 1 - 2 / (1 + e^{2y}) with the hardware
+// exponential and reciprocal (a few instructions instead of a libm tanhf); the argument is
+// clamped so e^{2y} stays finite.
+__device__ __forceinline__ float fast_tanh(float y) {
+    y = fminf(fmaxf(y, -15.f), 15.f);
+    return 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + __expf(2.f * y));
+}
 __device__ __forceinline__ float gelu(float x) {
     const float k = 0.7978845608028654f;
-    return 0.5f * x * (1.f + tanhf(k * (x + 0.044715f * x * x * x)));
+    return 0.5f * x * (1.f + fast_tanh(k * (x + 0.044715f * x * x * x)));
 }
 __device__ __forceinline__ float gelu_d(float x) {
     const float k = 0.7978845608028654f;
-    const float t = tanhf(k * (x + 0.044715f * x * x * x));
+    const float t = fast_tanh(k * (x + 0.044715f * x * x * x));
     return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * k * (1.f + 3.f * 0.044715f * x * x);
 }
 
@@ -70,7 +76,7 @@ __device__ __forceinline__ void leaf_features(const DevTables& t, const float4* 
     int np = 0;
     if (valid) {
         const float4 q = Qf[leaf];
-        c[0] = q.x; c[1] = q.w; c[2] = sqrtf(q.y * q.y + q.z * q.z);
+        c[0] = q.x; c[1] = q.w; c[2] = __builtin_amdgcn_sqrtf(q.y * q.y + q.z * q.z);
     }
 #pragma unroll
     for (int k = 0; k < 7; k++) {
@@ -87,7 +93,7 @@ __device__ __forceinline__ void leaf_features(const DevTables& t, const float4* 
                 idx = off16[(size_t)q16 * t.N + leaf];
             }
             const float4 q = Qf[idx];
-            vals[k][0] = q.x; vals[k][1] = q.w; vals[k][2] = sqrtf(q.y * q.y + q.z * q.z);
+            vals[k][0] = q.x; vals[k][1] = q.w; vals[k][2] = __builtin_amdgcn_sqrtf(q.y * q.y + q.z * q.z);
             np = k + 1;
         }
     }
@@ -255,22 +261,31 @@ __global__ __launch_bounds__(256) void k_select(DevTables t, const float4* __res
 }
 
 // judge: a judged leaf whose label exceeds the bound is forced back to the full path
+// Each block scans 4096 leaves (16 per thread) and adds its counts with one atomic, so the
+// counters see a few hundred atomics per step, not one per wave.
 __global__ __launch_bounds__(256) void k_judge(int n, const float* __restrict__ label, float bound,
                                                uint8_t* __restrict__ flags, int* __restrict__ counters) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     int judged = 0, rejected = 0;
-    if (i < n) {
+    const int base = blockIdx.x * 4096;
+    for (int k = threadIdx.x; k < 4096; k += 256) {
+        const int i = base + k;
+        if (i >= n) break;
         const uint8_t f = flags[i];
         if (f & 2) {
-            judged = 1;
-            if (label[i] > bound) { rejected = 1; flags[i] = 5; }
+            judged++;
+            if (label[i] > bound) { rejected++; flags[i] = 5; }
             else flags[i] = f & 4;
         }
     }
     for (int o = 32; o > 0; o >>= 1) { judged += __shfl_xor(judged, o); rejected += __shfl_xor(rejected, o); }
-    if ((threadIdx.x & 63) == 0) {
-        if (judged) atomicAdd(&counters[CNT_JUDGED], judged);
-        if (rejected) atomicAdd(&counters[CNT_REJECTED], rejected);
+    __shared__ int sj[4], sr[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) { sj[wv] = judged; sr[wv] = rejected; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int j = sj[0] + sj[1] + sj[2] + sj[3], r = sr[0] + sr[1] + sr[2] + sr[3];
+        if (j) atomicAdd(&counters[CNT_JUDGED], j);
+        if (r) atomicAdd(&counters[CNT_REJECTED], r);
     }
 }
 
@@ -484,7 +499,7 @@ void launch_select(const DevTables& t, const float4* Qf, const int32_t* off16, c
 }
 
 void launch_judge(int n, const float* label, float bound, uint8_t* flags, int* counters, hipStream_t st) {
-    if (n > 0) hipLaunchKernelGGL(k_judge, dim3((n + 255) / 256), dim3(256), 0, st, n, label, bound, flags, counters);
+    if (n > 0) hipLaunchKernelGGL(k_judge, dim3((n + 4095) / 4096), dim3(256), 0, st, n, label, bound, flags, counters);
 }
 
 void launch_features(const DevTables& t, const float4* Qf, const int32_t* off16, uint16_t* X, int row0,
