@@ -679,13 +679,12 @@ int32_t fdmamr_stencil_info(fdmamr_ctx* c, int32_t i, int32_t* kind8, int32_t* n
         if (node8) node8[k] = id[tb.st_node[d]];
         if (px8) px8[k] = tb.st_px[d];
         if (py8) py8[k] = tb.st_py[d];
-        if (m1_8) m1_8[k] = tb.st_m1[d];
-        if (m2_8) m2_8[k] = tb.st_m2[d];
+        int32_t nb9[9], a = 0, b = 0;
+        stencil_lists(c->tree, tb.st_node[d], tb.st_kind[d], nb9, &a, &b);
+        if (m1_8) m1_8[k] = a;
+        if (m2_8) m2_8[k] = b;
         if (nb8x9)
-            for (int j = 0; j < 9; j++) {
-                int q = tb.st_nb[d * 9 + j];
-                nb8x9[k * 9 + j] = q >= 0 ? id[q] : -1;
-            }
+            for (int j = 0; j < 9; j++) nb8x9[k * 9 + j] = nb9[j] >= 0 ? id[nb9[j]] : -1;
     }
     return 0;
 }

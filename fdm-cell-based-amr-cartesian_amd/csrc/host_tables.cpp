@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
+#include <atomic>
 #include <unordered_map>
 
 namespace fdm {
@@ -215,6 +216,18 @@ uint64_t bits(double v) { uint64_t b; std::memcpy(&b, &v, 8); return b; }
 
 }  // namespace
 
+int stencil_lists(const Tree& t, int tg, int kind, int32_t* nb9, int32_t* m1, int32_t* m2) {
+    List l1, l2;
+    if (kind == ST_PARENT_WLS || kind == ST_PARENT_RESTRICT) parent_lists(t, tg, l1, l2);
+    else if (kind == ST_CHILD_WLS || kind == ST_CHILD_COPY) child_lists(t, tg, l1, l2);
+    *m1 = l1.n;
+    *m2 = l2.n;
+    for (int j = 0; j < 9; j++) nb9[j] = -1;
+    for (int j = 0; j < l1.n && j < 9; j++) nb9[j] = l1.a[j];
+    for (int j = 0; j < l2.n && l1.n + j < 9; j++) nb9[l1.n + j] = l2.a[j];
+    return 0;
+}
+
 void build_tables(const Tree& t, Tables& tb) {
     const size_t NA = t.n_alloc();
     tb = Tables();
@@ -241,112 +254,123 @@ void build_tables(const Tree& t, Tables& tb) {
     tb.st_node.assign(8 * (size_t)N, -1);
     tb.st_px.assign(8 * (size_t)N, 0.0);
     tb.st_py.assign(8 * (size_t)N, 0.0);
-    tb.st_m1.assign(8 * (size_t)N, 0);
-    tb.st_m2.assign(8 * (size_t)N, 0);
-    tb.st_nb.assign(8 * 9 * (size_t)N, -1);
 
-    std::unordered_map<Key, int, KeyHash> wls_map, rst_map;
-    std::vector<std::vector<double>> wd;
-    std::vector<std::vector<int32_t>> wi;
-    int lmax = 0;
-
-    for (int i = 0; i < N; i++) {
-        const int c = t.ns[i];
-        tb.lvl_leaf[i] = (int8_t)t.lvl[c];
-        if (t.lvl[c] > lmax) lmax = t.lvl[c];
-        const int32_t* q = &t.nb[8 * (size_t)c];
-        for (int k = 0; k < 8; k++) {
-            // Tip/Tim/Tjp/Tjm (k<4) and Tipp/Timm/Tjpp/Tjmm (k>=4), XuanGua.F90:401-590
-            const int dir = k & 3;
-            const bool two = k >= 4;
-            const int nbk = dir == 0 ? NB_RIGHT : dir == 1 ? NB_LEFT : dir == 2 ? NB_UP : NB_DOWN;
-            const int cn = q[nbk];
-            if (cn < 0) throw std::runtime_error("interior leaf without a face neighbour");
-            double off = two ? (dir < 2 ? 2.0 * t.xf * t.h : 2.0 * t.h) / pow2i(t.lvl[c])
-                             : (dir < 2 ? t.xf * t.h : t.h) / pow2i(t.lvl[c]);
-            double xx = t.x[c], yy = t.y[c];
-            if (dir == 0) xx = t.x[c] + off;
-            else if (dir == 1) xx = t.x[c] - off;
-            else if (dir == 2) yy = t.y[c] + off;
-            else yy = t.y[c] - off;
-            int tg;
-            bool same;
-            if (!two) { tg = cn; same = t.lvl[cn] == t.lvl[c]; }
-            else if (t.lvl[cn] == t.lvl[c]) {
-                int cnn = t.nb[8 * (size_t)cn + nbk];
-                if (cnn < 0) throw std::runtime_error("interior leaf without a second neighbour");
-                tg = cnn; same = t.lvl[cnn] == t.lvl[cn];
-            } else { tg = cn; same = false; }
-
-            List l1, l2;
-            int kind, ref;
-            if (same) {
-                if (t.son[tg] < 0) { kind = ST_COPY; ref = tb.state_of_node[tg]; }
-                else {
-                    parent_lists(t, tg, l1, l2);
-                    xx = t.x[tg]; yy = t.y[tg];
-                    if (l1.n + l2.n < 7) {
-                        kind = ST_PARENT_RESTRICT;
-                        Key key{2, tg, 0, 0};
-                        auto it = rst_map.find(key);
-                        int s;
-                        if (it == rst_map.end()) { s = (int)tb.rst_node.size(); rst_map.emplace(key, s); tb.rst_node.push_back(tg); }
-                        else s = it->second;
-                        ref = -2 - s;  // patched to L + S_w + s below
-                    } else {
-                        kind = ST_PARENT_WLS;
-                        Key key{1, tg, 0, 0};
-                        auto it = wls_map.find(key);
-                        int s;
-                        if (it == wls_map.end()) {
-                            s = (int)wd.size(); wls_map.emplace(key, s);
-                            wd.emplace_back(WLS_STRIDE, 0.0); wi.emplace_back(WLSI_STRIDE, -1);
-                            wls_geometry(t, tg, t.x[tg], t.y[tg], l1, l2, tb.state_of_node, wd.back().data(), wi.back().data());
-                        } else s = it->second;
-                        ref = 1000000000 + s;  // patched to L + s below
+    // phase 1 (parallel): classify every stencil entry as the reference's gathers do
+    // (Tip/Tim/Tjp/Tjm for k < 4, Tipp/Timm/Tjpp/Tjmm for k >= 4, XuanGua.F90:401-590)
+    std::atomic<int> lmax_at{0}, missing{0};
+    parallel_for((size_t)N, [&](size_t b, size_t e) {
+        int lm = 0;
+        for (size_t ii = b; ii < e; ii++) {
+            const int i = (int)ii;
+            const int c = t.ns[i];
+            tb.lvl_leaf[i] = (int8_t)t.lvl[c];
+            lm = std::max(lm, t.lvl[c]);
+            const int32_t* q = &t.nb[8 * (size_t)c];
+            for (int k = 0; k < 8; k++) {
+                const int dir = k & 3;
+                const bool two = k >= 4;
+                const int nbk = dir == 0 ? NB_RIGHT : dir == 1 ? NB_LEFT : dir == 2 ? NB_UP : NB_DOWN;
+                const int cn = q[nbk];
+                if (cn < 0) { missing = 1; continue; }
+                double off = two ? (dir < 2 ? 2.0 * t.xf * t.h : 2.0 * t.h) / pow2i(t.lvl[c])
+                                 : (dir < 2 ? t.xf * t.h : t.h) / pow2i(t.lvl[c]);
+                double xx = t.x[c], yy = t.y[c];
+                if (dir == 0) xx = t.x[c] + off;
+                else if (dir == 1) xx = t.x[c] - off;
+                else if (dir == 2) yy = t.y[c] + off;
+                else yy = t.y[c] - off;
+                int tg;
+                bool same;
+                if (!two) { tg = cn; same = t.lvl[cn] == t.lvl[c]; }
+                else if (t.lvl[cn] == t.lvl[c]) {
+                    int cnn = t.nb[8 * (size_t)cn + nbk];
+                    if (cnn < 0) { missing = 1; continue; }
+                    tg = cnn; same = t.lvl[cnn] == t.lvl[cn];
+                } else { tg = cn; same = false; }
+                int kind;
+                if (same) {
+                    if (t.son[tg] < 0) kind = ST_COPY;
+                    else {
+                        List l1, l2;
+                        parent_lists(t, tg, l1, l2);
+                        xx = t.x[tg]; yy = t.y[tg];
+                        kind = l1.n + l2.n < 7 ? ST_PARENT_RESTRICT : ST_PARENT_WLS;
                     }
+                } else {
+                    List l1, l2;
+                    child_lists(t, tg, l1, l2);
+                    kind = l1.n + l2.n < 7 ? ST_CHILD_COPY : ST_CHILD_WLS;
                 }
-            } else {
-                child_lists(t, tg, l1, l2);
-                if (l1.n + l2.n < 7) { kind = ST_CHILD_COPY; ref = tb.state_of_node[tg]; }
-                else {
-                    kind = ST_CHILD_WLS;
-                    Key key{3, tg, bits(xx), bits(yy)};
-                    auto it = wls_map.find(key);
-                    int s;
-                    if (it == wls_map.end()) {
-                        s = (int)wd.size(); wls_map.emplace(key, s);
-                        wd.emplace_back(WLS_STRIDE, 0.0); wi.emplace_back(WLSI_STRIDE, -1);
-                        wls_geometry(t, tg, xx, yy, l1, l2, tb.state_of_node, wd.back().data(), wi.back().data());
-                    } else s = it->second;
-                    ref = 1000000000 + s;
-                }
+                const size_t d = (size_t)i * 8 + k;
+                tb.st_kind[d] = (int8_t)kind;
+                tb.st_node[d] = tg;
+                tb.st_px[d] = xx; tb.st_py[d] = yy;
             }
-            size_t e = (size_t)k * N + i;
-            tb.stencil[e] = ref;
-            size_t d = (size_t)i * 8 + k;
-            tb.st_kind[d] = (int8_t)kind;
-            tb.st_node[d] = tg;
-            tb.st_px[d] = xx; tb.st_py[d] = yy;
-            tb.st_m1[d] = l1.n; tb.st_m2[d] = l2.n;
-            for (int j = 0; j < l1.n; j++) tb.st_nb[d * 9 + j] = l1.a[j];
-            for (int j = 0; j < l2.n && l1.n + j < 9; j++) tb.st_nb[d * 9 + l1.n + j] = l2.a[j];
         }
-    }
-    tb.S_w = (int)wd.size();
+        int cur = lmax_at.load();
+        while (lm > cur && !lmax_at.compare_exchange_weak(cur, lm)) {}
+    });
+    if (missing) throw std::runtime_error("interior leaf without its stencil neighbours");
+    const int lmax = lmax_at.load();
+
+    // phase 2 (serial, deterministic order): one slot per distinct level-jump value
+    std::unordered_map<Key, int, KeyHash> wls_map, rst_map;
+    std::vector<int32_t> wls_node;      // per WLS slot: the WLS cell
+    std::vector<int8_t> wls_parent;     // per WLS slot: parent (1) or child (0) stencil
+    std::vector<double> wls_xx, wls_yy;
+    for (int i = 0; i < N; i++)
+        for (int k = 0; k < 8; k++) {
+            const size_t d = (size_t)i * 8 + k;
+            const int kind = tb.st_kind[d], tg = tb.st_node[d];
+            int ref;
+            if (kind == ST_COPY || kind == ST_CHILD_COPY) ref = tb.state_of_node[tg];
+            else if (kind == ST_PARENT_RESTRICT) {
+                Key key{2, tg, 0, 0};
+                auto it = rst_map.find(key);
+                int s;
+                if (it == rst_map.end()) { s = (int)tb.rst_node.size(); rst_map.emplace(key, s); tb.rst_node.push_back(tg); }
+                else s = it->second;
+                ref = -2 - s;  // patched to L + S_w + s below
+            } else {
+                const bool parent = kind == ST_PARENT_WLS;
+                Key key{parent ? 1 : 3, tg, parent ? 0 : bits(tb.st_px[d]), parent ? 0 : bits(tb.st_py[d])};
+                auto it = wls_map.find(key);
+                int s;
+                if (it == wls_map.end()) {
+                    s = (int)wls_node.size();
+                    wls_map.emplace(key, s);
+                    wls_node.push_back(tg);
+                    wls_parent.push_back(parent ? 1 : 0);
+                    wls_xx.push_back(tb.st_px[d]);
+                    wls_yy.push_back(tb.st_py[d]);
+                } else s = it->second;
+                ref = 1000000000 + s;  // patched to L + s below
+            }
+            tb.stencil[(size_t)k * N + i] = ref;
+        }
+    tb.S_w = (int)wls_node.size();
     tb.S_r = (int)tb.rst_node.size();
     for (auto& r : tb.stencil) {
         if (r >= 1000000000) r = tb.L + (r - 1000000000);
         else if (r <= -2) r = tb.L + tb.S_w + (-2 - r);
     }
-    tb.wls_d.resize((size_t)tb.S_w * WLS_STRIDE);
-    tb.wls_i.resize((size_t)tb.S_w * WLSI_STRIDE);
+
+    // phase 3 (parallel): stencil lists and FP64 geometry of every WLS slot
+    tb.wls_d.assign((size_t)tb.S_w * WLS_STRIDE, 0.0);
+    tb.wls_i.assign((size_t)tb.S_w * WLSI_STRIDE, -1);
     tb.wls_f.assign((size_t)tb.S_w * WF_STRIDE, 0.0f);
-    for (int s = 0; s < tb.S_w; s++) {
-        std::memcpy(&tb.wls_d[(size_t)s * WLS_STRIDE], wd[s].data(), sizeof(double) * WLS_STRIDE);
-        std::memcpy(&tb.wls_i[(size_t)s * WLSI_STRIDE], wi[s].data(), sizeof(int32_t) * WLSI_STRIDE);
-        fast_record(wd[s].data(), wi[s].data(), &tb.wls_f[(size_t)s * WF_STRIDE]);
-    }
+    parallel_for((size_t)tb.S_w, [&](size_t b, size_t e) {
+        for (size_t s = b; s < e; s++) {
+            List l1, l2;
+            const int tg = wls_node[s];
+            if (wls_parent[s]) parent_lists(t, tg, l1, l2);
+            else child_lists(t, tg, l1, l2);
+            double* dd = &tb.wls_d[s * WLS_STRIDE];
+            int32_t* iv = &tb.wls_i[s * WLSI_STRIDE];
+            wls_geometry(t, tg, wls_xx[s], wls_yy[s], l1, l2, tb.state_of_node, dd, iv);
+            fast_record(dd, iv, &tb.wls_f[s * WF_STRIDE]);
+        }
+    });
     tb.bc_partner.resize(tb.G);
     for (int g = 0; g < tb.G; g++) tb.bc_partner[g] = t.bc_partner(t.bcl[g]);
     tb.l_max = lmax;

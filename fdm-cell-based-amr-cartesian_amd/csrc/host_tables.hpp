@@ -59,8 +59,6 @@ struct Tables {
     std::vector<int8_t> st_kind;         // [N*8]
     std::vector<int32_t> st_node;        // [N*8]
     std::vector<double> st_px, st_py;    // [N*8]
-    std::vector<int32_t> st_m1, st_m2;   // [N*8]
-    std::vector<int32_t> st_nb;          // [N*8*9] node ids
     // slots
     std::vector<double> wls_d;           // [S_w * WLS_STRIDE]
     std::vector<int32_t> wls_i;          // [S_w * WLSI_STRIDE]
@@ -86,6 +84,8 @@ struct Tables {
 };
 
 void build_tables(const Tree& t, Tables& tb);
+// WLS-WENO stencil lists of one classified entry (diagnostics): node ids, list sizes
+int stencil_lists(const Tree& t, int tg, int kind, int32_t* nb9, int32_t* m1, int32_t* m2);
 
 // reference helpers reused by diagnostics
 int brinv(int n, const double* a, double* r);                   // XuanGua.F90:593-660

@@ -4,9 +4,13 @@
 // the reference's case analysis exactly, including its asymmetric diagonal null rules.
 #include "host_tree.hpp"
 
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
+#include <exception>
+#include <thread>
 
 namespace fdm {
 
@@ -18,6 +22,26 @@ uint64_t splitmix64(uint64_t& s) {
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
     return z ^ (z >> 31);
 }
+void parallel_for(size_t n, const std::function<void(size_t, size_t)>& f) {
+    unsigned nt = std::thread::hardware_concurrency();
+    if (const char* e = std::getenv("FDMAMR_HOST_THREADS")) nt = (unsigned)std::max(1, std::atoi(e));
+    nt = std::min<unsigned>(std::max(nt, 1u), 16u);
+    if (n < 4096 || nt == 1) { f(0, n); return; }
+    std::vector<std::thread> th;
+    std::vector<std::exception_ptr> errs(nt);
+    const size_t chunk = (n + nt - 1) / nt;
+    for (unsigned k = 0; k < nt; k++) {
+        const size_t b = k * chunk, e = std::min(n, b + chunk);
+        if (b < e)
+            th.emplace_back([&f, &errs, k, b, e] {
+                try { f(b, e); } catch (...) { errs[k] = std::current_exception(); }
+            });
+    }
+    for (auto& t : th) t.join();
+    for (auto& x : errs)
+        if (x) std::rethrow_exception(x);
+}
+
 static double u01(uint64_t& s) { return (double)(splitmix64(s) >> 11) * 0x1.0p-53; }
 static double uab(uint64_t& s, double a, double b) { return a + (b - a) * u01(s); }
 
@@ -236,7 +260,10 @@ void Tree::idnei(int c) {                           // neighbor.F90:412-449
 }
 void Tree::idneighbor() {
     nb.assign(8 * n_alloc(), -1);
-    for (int i = 0; i < total; i++) idnei(i);
+    // the finders only read the tree; each base cell's subtree writes its own entries
+    parallel_for((size_t)total, [&](size_t b, size_t e) {
+        for (size_t i = b; i < e; i++) idnei((int)i);
+    });
 }
 
 // ---------------------------------------------------------------- leaf lists
@@ -419,11 +446,12 @@ void Tree::gridmodify4(int c) {                     // mesh.F90:355-395
     }
 }
 void Tree::solute_gridmodify(int lvl_amr) {         // solution.F90:537-556
+    // The reference refreshes the cached neighbour pointers after each pass (idneighbor),
+    // but gridmodify1 and gridmodify4 search neighbours afresh through father/son links and
+    // never read the cache, so only the final refresh (by the caller) can be observed.
     for (int k = 1; k <= lvl_amr; k++) {
         for (int i = 0; i < total; i++) gridmodify1(i);
-        idneighbor();
         for (int i = 0; i < total; i++) gridmodify4(i);
-        idneighbor();
     }
 }
 

@@ -8,6 +8,7 @@
 #pragma once
 #include <array>
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -97,5 +98,8 @@ struct Tree {
 
 // SplitMix64 shared by the IC generators
 uint64_t splitmix64(uint64_t& s);
+
+// Run f(begin, end) over [0, n) split across host threads (regrid-time work only).
+void parallel_for(size_t n, const std::function<void(size_t, size_t)>& f);
 
 }  // namespace fdm
