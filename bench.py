@@ -141,10 +141,24 @@ def main():
     ap.add_argument("--advance", type=int, default=0,
                     help="evolve the flow this many steps (regrid every 10) before measuring")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of this many MiB (scale for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         raise SystemExit(2)
+
+    if args.copy_calibration:
+        # torch (when used) is loaded before the solver library, as in the tests
+        import torch
+        n = args.copy_calibration << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 3 device copies of {n} bytes")
+        del src, dst
 
     import fdmamr as F
 

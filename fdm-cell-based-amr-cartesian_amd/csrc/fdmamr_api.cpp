@@ -44,6 +44,8 @@ struct fdmamr_ctx {
     DBuf<uint8_t> flags;     // selector flags per interior leaf (bit 0: troubled)
     DBuf<float> label;       // per-leaf WCNS weight deviation (labels / judge)
     bool use_flags = false;  // fast path consults the flags
+    DBuf<int32_t> lists;     // cleared leaves, then troubled leaves (from the flags)
+    DBuf<int> list_cnt, list_meta;
 
     // learned selector
     DBuf<int32_t> off16;
@@ -126,6 +128,9 @@ int upload_tables(fdmamr_ctx* c) {
     }
     HIPCHK(c, ensure(c->wls_f, tb.wls_f.size()));
     HIPCHK(c, ensure(c->flags, tb.N));
+    HIPCHK(c, ensure(c->lists, tb.N));
+    HIPCHK(c, ensure(c->list_cnt, list_blocks(tb.N)));
+    HIPCHK(c, ensure(c->list_meta, 2));
     HIPCHK(c, ensure(c->label, tb.N));
     HIPCHK(c, ensure(c->stencil, tb.stencil.size()));
     HIPCHK(c, ensure(c->lvl, tb.lvl_leaf.size()));
@@ -136,7 +141,7 @@ int upload_tables(fdmamr_ctx* c) {
     HIPCHK(c, ensure(c->son, tb.node_son.size()));
     HIPCHK(c, ensure(c->state, tb.state_of_node.size()));
     const int nb = std::max(1, nblocks(tb.N, 256));
-    HIPCHK(c, ensure(c->partials, (size_t)nb * 8));
+    HIPCHK(c, ensure(c->partials, (size_t)stage_fast_partial_blocks(tb.N, true) * 8));
     HIPCHK(c, ensure(c->lampart, (size_t)nb * 2));
     HIPCHK(c, ensure(c->rot, tb.N));
     HIPCHK(c, ensure(c->div, tb.N));
@@ -181,6 +186,7 @@ int upload_tables(fdmamr_ctx* c) {
     HIPCHK(c, up(c->off16.p, tb.off16.data(), tb.off16.size() * 4));
     // every leaf troubled until the selector says otherwise
     if (tb.N > 0) HIPCHK(c, hipMemsetAsync(c->flags.p, 1, tb.N, s));
+    launch_build_lists(tb.N, c->flags.p, c->list_cnt.p, c->lists.p, c->list_meta.p, s);
     DevTables& d = c->dt;
     d.N = tb.N; d.G = tb.G; d.P = tb.P; d.L = tb.L; d.S_w = tb.S_w; d.S_r = tb.S_r;
     d.gamma = t.gamma; d.cv = t.cv; d.h = t.h; d.xf = t.xf; d.Ma = t.Ma;
@@ -251,21 +257,24 @@ void enqueue_rk3(fdmamr_ctx* c, int advance_time) {
     // the learned selector runs once per step, on the step's input state, before stage 1;
     // its judge sample takes the full path in stage 1 and is checked right after it
     const bool select = fast && c->p.selector && c->sel_ready;
+    // flagged fast path: cleared and troubled leaves run as separate launches over the lists
+    // built from the flags; the judge's flag changes after stage 1 are not folded back into
+    // the lists, so judged leaves stay on the full path for the whole step
+    const int32_t* lists = fast && (c->use_flags || select) ? c->lists.p : nullptr;
     for (int k = 0; k < 3; k++) {
         if (fast) launch_slots_fast(d, c->wls_f.p, c->Q[in[k]].p, c->Qf[in[k]].p, s);
         else launch_slots(d, c->Q[in[k]].p, c->T[in[k]].p, s);
         if (select && k == 0) {
             SelectParams sp{c->sel_threshold, (uint32_t)(c->sel_judge * 65536.0f), c->sel_step++, 8};
-            launch_select(d, c->Qf[0].p, c->off16.p, c->sel_pk.p, c->sel_w.p, sp, c->flags.p, nullptr,
-                          c->sel_cnt.p, s);
+            launch_select(d, c->Qf[0].p, c->off16.p, c->sel_pk.p, c->sel_w.p, sp, c->flags.p, nullptr, s);
+            launch_build_lists(d.N, c->flags.p, c->list_cnt.p, c->lists.p, c->list_meta.p, s);
         }
         int e0 = -1, e1 = -1;
         if (c->profiling && (e0 = next_event(c)) >= 0) (void)hipEventRecord(c->ev_pool[e0], s);
         if (fast)
-            launch_stage_fast(d, c->Q[in[k]].p, c->Qf[in[k]].p, c->Q[0].p, c->Q[out[k]].p, c->Qf[out[k]].p,
-                              c->sc, ra[k], rb[k], k == 2 ? 1 : 0, c->partials.p,
-                              (c->use_flags || select) ? c->flags.p : nullptr,
-                              (select && k == 0) ? c->label.p : nullptr, s);
+            launch_stage_fast(d, lists, c->list_meta.p, c->Q[in[k]].p, c->Qf[in[k]].p, c->Q[0].p,
+                              c->Q[out[k]].p, c->Qf[out[k]].p, c->sc, ra[k], rb[k], k == 2 ? 1 : 0,
+                              c->partials.p, (select && k == 0) ? c->label.p : nullptr, s);
         else
             launch_stage(d, 0, c->Q[in[k]].p, c->Q[0].p, c->T[0].p, c->Q[out[k]].p, c->T[out[k]].p,
                          c->sc, ra[k], rb[k], k == 2 ? 1 : 0, c->partials.p, s);
@@ -276,7 +285,8 @@ void enqueue_rk3(fdmamr_ctx* c, int advance_time) {
         if (select && k == 0) launch_judge(d.N, c->label.p, c->sel_bound, c->flags.p, c->sel_cnt.p, s);
         fill_after_stage(c, out[k]);
     }
-    launch_step_end(c->partials.p, nblocks(d.N, 256), d.N, advance_time, c->sc, s);
+    launch_step_end(c->partials.p, fast ? stage_fast_partial_blocks(d.N, lists != nullptr) : nblocks(d.N, 256),
+                    d.N, advance_time, c->sc, s);
 }
 
 }  // namespace
@@ -326,7 +336,7 @@ void fdmamr_destroy(fdmamr_ctx* c) {
     release(c->label); release(c->off16); release(c->sel_w); release(c->sel_m); release(c->sel_v);
     release(c->sel_grad); release(c->sel_gpart); release(c->sel_lpart); release(c->sel_loss);
     release(c->sel_prob); release(c->sel_pk); release(c->sel_cnt); release(c->ds_x); release(c->ds_y);
-    release(c->ds_idx);
+    release(c->ds_idx); release(c->lists); release(c->list_cnt); release(c->list_meta);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->sc) (void)hipFree(c->sc);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -695,6 +705,8 @@ int32_t fdmamr_selector_set_flags(fdmamr_ctx* c, const uint8_t* flags, int32_t u
     if (c->p.flux_precision != FDMAMR_FLUX_FP32) return fail(c, -1, "selector needs the FP32 flux mode");
     if (flags && c->tb.N > 0)
         HIPCHK(c, hipMemcpyAsync(c->flags.p, flags, c->tb.N, hipMemcpyHostToDevice, c->stream));
+    launch_build_lists(c->tb.N, c->flags.p, c->list_cnt.p, c->lists.p, c->list_meta.p, c->stream);
+    HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->use_flags = use != 0;
     return 0;
@@ -722,8 +734,8 @@ int32_t fdmamr_labels(fdmamr_ctx* c, float* dev) {
     HIPCHK(c, hipMemcpyAsync(c->sc, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
     launch_shadow(c->Q[0].p, c->Qf[0].p, c->tb.L + c->tb.S_w + c->tb.S_r, c->stream);
     launch_slots_fast(c->dt, c->wls_f.p, c->Q[0].p, c->Qf[0].p, c->stream);
-    launch_stage_fast(c->dt, c->Q[0].p, c->Qf[0].p, c->Q[0].p, c->Q[2].p, c->Qf[2].p, c->sc, 0.0, 1.0,
-                      0, c->partials.p, nullptr, c->label.p, c->stream);
+    launch_stage_fast(c->dt, nullptr, nullptr, c->Q[0].p, c->Qf[0].p, c->Q[0].p, c->Q[2].p, c->Qf[2].p,
+                      c->sc, 0.0, 1.0, 0, c->partials.p, c->label.p, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(dev, c->label.p, c->tb.N * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -813,7 +825,10 @@ int32_t fdmamr_selector_infer(fdmamr_ctx* c, float* prob, int32_t* n_troubled) {
     HIPCHK(c, hipMemcpyAsync(c->sel_cnt.p, cnt0.data(), CNT_N * 4, hipMemcpyHostToDevice, c->stream));
     SelectParams sp{c->sel_threshold, 0u, c->sel_step, 8};  // no judge sample here
     launch_select(c->dt, c->Qf[0].p, c->off16.p, c->sel_pk.p, c->sel_w.p, sp, c->flags.p, c->sel_prob.p,
-                  c->sel_cnt.p, c->stream);
+                  c->stream);
+    // no judge sample was drawn: the judge pass only counts the troubled leaves
+    launch_judge(N, c->label.p, c->sel_bound, c->flags.p, c->sel_cnt.p, c->stream);
+    launch_build_lists(N, c->flags.p, c->list_cnt.p, c->lists.p, c->list_meta.p, c->stream);
     HIPCHK(c, hipGetLastError());
     if (prob && N > 0) HIPCHK(c, hipMemcpyAsync(prob, c->sel_prob.p, N * 4, hipMemcpyDeviceToHost, c->stream));
     int cnt[CNT_N];
@@ -878,8 +893,8 @@ int32_t fdmamr_selector_collect(fdmamr_ctx* c, int32_t* n_rows, int32_t* n_posit
     HIPCHK(c, hipMemcpyAsync(c->sc, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
     launch_shadow(c->Q[0].p, c->Qf[0].p, c->tb.L + c->tb.S_w + c->tb.S_r, c->stream);
     launch_slots_fast(c->dt, c->wls_f.p, c->Q[0].p, c->Qf[0].p, c->stream);
-    launch_stage_fast(c->dt, c->Q[0].p, c->Qf[0].p, c->Q[0].p, c->Q[2].p, c->Qf[2].p, c->sc, 0.0, 1.0, 0,
-                      c->partials.p, nullptr, c->label.p, c->stream);
+    launch_stage_fast(c->dt, nullptr, nullptr, c->Q[0].p, c->Qf[0].p, c->Q[0].p, c->Q[2].p, c->Qf[2].p,
+                      c->sc, 0.0, 1.0, 0, c->partials.p, c->label.p, c->stream);
     launch_features(c->dt, c->Qf[0].p, c->off16.p, c->ds_x.p, c->ds_rows, c->stream);
     launch_labels_to_y(N, c->label.p, c->sel_bound, c->ds_y.p, c->ds_rows, c->stream);
     HIPCHK(c, hipGetLastError());

@@ -13,6 +13,17 @@ namespace fdm {
 // maximum quadtree depth handled by the on-device restriction walk
 constexpr int MAX_TREE_DEPTH = 24;
 
+// Workgroups are dispatched round-robin over the 8 XCDs (block b to XCD b % 8). This renumbers
+// them so that each XCD processes one contiguous range of logical blocks: neighbouring leaves
+// share stencil data, and the range then stays in that XCD's L2. A bijection on [0, gridDim.x).
+// xcd_block_n applies the same renumbering to the first nb blocks of a larger grid (the
+// active prefix of a launch whose size is only known on the device).
+__device__ __forceinline__ int xcd_block_n(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+__device__ __forceinline__ int xcd_block() { return xcd_block_n(blockIdx.x, gridDim.x); }
+
 // Device views of the flattened tables (all pointers are device memory).
 struct DevTables {
     int N, G, P, L, S_w, S_r;
@@ -56,9 +67,16 @@ void launch_stencil_values(const DevTables& t, const double4* Q, int i, double* 
 
 // fast path (fdmamr_fast.hip)
 void launch_slots_fast(const DevTables& t, const float* wf, const double4* Q, float4* Qf, hipStream_t st);
-void launch_stage_fast(const DevTables& t, const double4* Qin, const float4* Qf_in, const double4* Qn,
-                       double4* Qout, float4* Qf_out, const DevScalars* sc, double alpha, double beta,
-                       int reduce, double* partials, const uint8_t* flags, float* label, hipStream_t st);
+// lists: cleared leaves then troubled leaves (meta[0] = number cleared), or nullptr for the
+// full path on every leaf; residual partials occupy stage_fast_partial_blocks() blocks
+void launch_stage_fast(const DevTables& t, const int32_t* lists, const int* meta, const double4* Qin,
+                       const float4* Qf_in, const double4* Qn, double4* Qout, float4* Qf_out,
+                       const DevScalars* sc, double alpha, double beta, int reduce, double* partials,
+                       float* label, hipStream_t st);
+int stage_fast_partial_blocks(int n, bool lists);
+// leaf lists from selector flags (bit 0: troubled); blk_cnt holds list_blocks(n) ints
+void launch_build_lists(int n, const uint8_t* flags, int* blk_cnt, int32_t* lists, int* meta, hipStream_t st);
+int list_blocks(int n);
 void launch_shadow(const double4* Q, float4* Qf, int n, hipStream_t st);
 
 }  // namespace fdm

@@ -87,6 +87,7 @@ __device__ __forceinline__ void eigen_f(const float* U, float gama, float R[4][4
 // one WCNS_Z3 component; returns the reconstruction and accumulates |w1 - w1_lin| into dev.
 // The deviation uses smoothness indicators with a relative noise floor (1e-6 of the mean
 // square), so exactly flat data (where every candidate is the same value) is not reported.
+template <bool LABEL>
 __device__ __forceinline__ float wcns_f(float u1, float u2, float u3, float& dev) {
     const float IS1a = (u2 - u1) * (u2 - u1), IS1b = (u3 - u2) * (u3 - u2);
     const float t3 = u1 - 2.f * u2 + u3;
@@ -101,7 +102,7 @@ __device__ __forceinline__ float wcns_f(float u1, float u2, float u3, float& dev
     const float w1 = a1 * ia, w2 = a2 * ia;
     const float p1 = u2;
     const float p2 = ((-u1 + 6.f * u2 + 3.f * u3) * 0.125f - FD1 * p1) * (1.f / FD2);
-    {
+    if (LABEL) {
         const float fl = 1e-6f * (u1 * u1 + u2 * u2 + u3 * u3) * (1.f / 3.f) + 1e-30f;
         const float s1 = fminf(tau3 * frcp(fl + IS1), 1.0e30f);
         const float s2 = fminf(tau3 * frcp(fl + IS2), 1.0e30f);
@@ -111,51 +112,59 @@ __device__ __forceinline__ float wcns_f(float u1, float u2, float u3, float& dev
     return w1 * p1 + w2 * p2;
 }
 
-// full characteristic WCNS3 flux derivative in FP32 (troubled leaves)
-template <int DIR>
+// full characteristic WCNS3 flux derivative in FP32 (troubled leaves), x-direction form (the
+// y-direction calls it on states with u and v exchanged). The two faces share one loop body
+// (not unrolled) to keep the register footprint of one face.
+template <bool LABEL>
 __device__ __forceinline__ void flux_full_f(const float4* q, float d, float lam, float gama, float* out,
                                             float& dev) {
     PtF P[5];
     const float gm1inv = 1.f / (gama - 1.f);
 #pragma unroll
-    for (int j = 0; j < 5; j++) cons_flux_f<DIR>(q[j], gm1inv, P[j]);
-    float flux[2][4];
-#pragma unroll
+    for (int j = 0; j < 5; j++) cons_flux_f<0>(q[j], gm1inv, P[j]);
+    float fdiff[4] = {0.f, 0.f, 0.f, 0.f};   // flux(i+1/2) - flux(i-1/2)
+#pragma nounroll
     for (int f = 0; f < 2; f++) {
-        float avg[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) avg[k] = 0.5f * (P[f + 1].U[k] + P[f + 2].U[k]);
-        float R[4][4], Ri[4][4];
-        eigen_f<DIR>(avg, gama, R, Ri);
-        float w[4][4], g[4][4];
+        PtF S[4];   // points f .. f+3
 #pragma unroll
         for (int j = 0; j < 4; j++)
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                float s = 0.f, t = 0.f;
-#pragma unroll
-                for (int c = 0; c < 4; c++) { s += Ri[r][c] * P[f + j].U[c]; t += Ri[r][c] * P[f + j].F[c]; }
-                w[j][r] = s; g[j][r] = t;
+            for (int k = 0; k < 4; k++) {
+                S[j].U[k] = f ? P[j + 1].U[k] : P[j].U[k];
+                S[j].F[k] = f ? P[j + 1].F[k] : P[j].F[k];
             }
-        float cL[4], cR[4], hL[4], hR[4];
+        float avg[4];
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            cL[r] = wcns_f(w[0][r], w[1][r], w[2][r], dev);
-            cR[r] = wcns_f(w[3][r], w[2][r], w[1][r], dev);
-            hL[r] = wcns_f(g[0][r], g[1][r], g[2][r], dev);
-            hR[r] = wcns_f(g[3][r], g[2][r], g[1][r], dev);
-        }
-        // flux in characteristic space, one back-projection: R (gL + gR - lam (wR - wL)) / 2
+        for (int k = 0; k < 4; k++) avg[k] = 0.5f * (S[1].U[k] + S[2].U[k]);
+        float R[4][4], Ri[4][4];
+        eigen_f<0>(avg, gama, R, Ri);
         float fc[4];
 #pragma unroll
-        for (int r = 0; r < 4; r++) fc[r] = 0.5f * (hL[r] + hR[r] - lam * (cR[r] - cL[r]));
+        for (int r = 0; r < 4; r++) {
+            float w[4], g[4];
 #pragma unroll
-        for (int r = 0; r < 4; r++) flux[f][r] = R[r][0] * fc[0] + R[r][1] * fc[1] + R[r][2] * fc[2] + R[r][3] * fc[3];
+            for (int j = 0; j < 4; j++) {
+                float a = 0.f, b = 0.f;
+#pragma unroll
+                for (int c = 0; c < 4; c++) { a += Ri[r][c] * S[j].U[c]; b += Ri[r][c] * S[j].F[c]; }
+                w[j] = a; g[j] = b;
+            }
+            const float cL = wcns_f<LABEL>(w[0], w[1], w[2], dev);
+            const float cR = wcns_f<LABEL>(w[3], w[2], w[1], dev);
+            const float hL = wcns_f<LABEL>(g[0], g[1], g[2], dev);
+            const float hR = wcns_f<LABEL>(g[3], g[2], g[1], dev);
+            // flux in characteristic space; one back-projection R (gL + gR - lam (wR - wL)) / 2
+            fc[r] = 0.5f * (hL + hR - lam * (cR - cL));
+        }
+        const float sg = f ? 1.f : -1.f;
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+            fdiff[r] += sg * (R[r][0] * fc[0] + R[r][1] * fc[1] + R[r][2] * fc[2] + R[r][3] * fc[3]);
     }
     const float id = 1.f / d;
 #pragma unroll
     for (int n = 0; n < 4; n++)
-        out[n] = (4.f / 3.f) * id * (flux[1][n] - flux[0][n]) - (1.f / 6.f) * id * (P[3].F[n] - P[1].F[n]);
+        out[n] = (4.f / 3.f) * id * fdiff[n] - (1.f / 6.f) * id * (P[3].F[n] - P[1].F[n]);
 }
 
 // linear-weight WCNS3 flux derivative (cleared leaves): no projection, no weights
@@ -182,73 +191,119 @@ __device__ __forceinline__ void flux_lin_f(const float4* q, float d, float lam, 
     }
 }
 
-// ---------------------------------------------------------------- stage kernel (fast path)
-// flags: nullptr = every leaf takes the full path; otherwise bit 0 set = troubled.
-// label: when non-null, receives the leaf's max weight deviation (full path only).
+// ---------------------------------------------------------------- stage kernels (fast path)
+// One leaf of a fast-path stage: FP32 flux derivatives from the FP32 shadow, FP64 substep and
+// RK blend on the FP64 state. FULL selects the characteristic WCNS3 (troubled leaves) or the
+// linear-weight reconstruction (cleared leaves).
+template <bool FULL, bool LABEL>
+__device__ __forceinline__ void stage_leaf(const DevTables& t, int i, const double4* __restrict__ Qin,
+                                           const float4* __restrict__ Qf_in, const double4* Qn,
+                                           double4* Qout, float4* __restrict__ Qf_out,
+                                           const DevScalars* __restrict__ sc, double alpha, double beta,
+                                           float* __restrict__ label, double* rmax, double* rsum) {
+    const double gama = t.gamma;
+    const double dt = sc->dt;
+    int s[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) s[k] = t.stencil[(size_t)k * t.N + i];
+    const double4 qc = Qin[i];
+    const float4 qcf = Qf_in[i];
+    const int lv = t.lvl[i];
+    const double dy = t.h / (double)(1 << lv);
+    const double dx = t.xf * dy;
+    float dE[4], dF[4];
+    float dev = 0.f;
+    if (FULL) {
+        // one loop body for both directions: the y-direction is the x-direction flux of the
+        // states with u and v exchanged, its momentum components exchanged back
+#pragma nounroll
+        for (int dir = 0; dir < 2; dir++) {
+            float4 col[5] = {Qf_in[dir ? s[7] : s[5]], Qf_in[dir ? s[3] : s[1]], qcf, Qf_in[dir ? s[2] : s[0]],
+                             Qf_in[dir ? s[6] : s[4]]};
+#pragma unroll
+            for (int j = 0; j < 5; j++) {
+                const float a = col[j].y, b = col[j].z;
+                col[j].y = dir ? b : a;
+                col[j].z = dir ? a : b;
+            }
+            float o[4];
+            flux_full_f<LABEL>(col, dir ? (float)dy : (float)dx, dir ? (float)sc->lam_y : (float)sc->lam_x,
+                               (float)gama, o, dev);
+            const float o1 = dir ? o[2] : o[1], o2 = dir ? o[1] : o[2];
+            if (dir) { dF[0] = o[0]; dF[1] = o1; dF[2] = o2; dF[3] = o[3]; }
+            else { dE[0] = o[0]; dE[1] = o1; dE[2] = o2; dE[3] = o[3]; }
+        }
+    } else {
+        {
+            const float4 col[5] = {Qf_in[s[5]], Qf_in[s[1]], qcf, Qf_in[s[0]], Qf_in[s[4]]};
+            flux_lin_f<0>(col, (float)dx, (float)sc->lam_x, (float)gama, dE);
+        }
+        {
+            const float4 col[5] = {Qf_in[s[7]], Qf_in[s[3]], qcf, Qf_in[s[2]], Qf_in[s[6]]};
+            flux_lin_f<1>(col, (float)dy, (float)sc->lam_y, (float)gama, dF);
+        }
+    }
+    if (label) label[i] = dev;
+    // FP64 forward-Euler substep and primitive RK blend (nnd_center_RK.F90:22-42,
+    // solution.F90:398-400)
+    double U[4], Ut[4];
+    U[0] = qc.x;
+    U[1] = qc.x * qc.y;
+    U[2] = qc.x * qc.z;
+    U[3] = qc.w / (gama - 1.0) + 0.5 * qc.x * (qc.y * qc.y + qc.z * qc.z);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double fu = -(double)dE[k] - (double)dF[k];
+        Ut[k] = U[k] + fu * dt;
+        const double fr = fu * dx * dy;
+        rmax[k] = fabs(fr);
+        rsum[k] = fr * fr;
+    }
+    const double r = Ut[0];
+    const double ir = 1.0 / r;
+    const double u = Ut[1] * ir, v = Ut[2] * ir;
+    const double p = (gama - 1.0) * (Ut[3] - 0.5 * r * (u * u + v * v));
+    const double4 qn = Qn[i];
+    double4 o;
+    o.x = alpha * qn.x + beta * r;
+    o.y = alpha * qn.y + beta * u;
+    o.z = alpha * qn.z + beta * v;
+    o.w = alpha * qn.w + beta * p;
+    Qout[i] = o;
+    Qf_out[i] = make_float4((float)o.x, (float)o.y, (float)o.z, (float)o.w);
+}
+
+// Stage over one part of the leaf list: lists[0, n_clear) holds the cleared leaves and
+// lists[n_clear, N) the troubled ones (meta[0] = n_clear), both in ascending leaf order, so
+// no wave mixes the two reconstructions. lists == nullptr: every leaf, full path.
+// Residual partials of logical block b go to partials[(poff + b) * 8 ...].
 // Qn/Qout may alias (in-place final stage), so they carry no __restrict__.
-__global__ __launch_bounds__(256) void k_stage_fast(DevTables t, const double4* __restrict__ Qin,
+template <bool FULL, bool LABEL>
+__global__ __launch_bounds__(256) void k_stage_fast(DevTables t, const int32_t* __restrict__ lists,
+                                                    const int* __restrict__ meta,
+                                                    const double4* __restrict__ Qin,
                                                     const float4* __restrict__ Qf_in,
                                                     const double4* Qn, double4* Qout,
                                                     float4* __restrict__ Qf_out,
                                                     const DevScalars* __restrict__ sc, double alpha,
-                                                    double beta, int reduce,
-                                                    double* __restrict__ partials,
-                                                    const uint8_t* __restrict__ flags,
-                                                    float* __restrict__ label) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+                                                    double beta, int reduce, double* __restrict__ partials,
+                                                    int poff, float* __restrict__ label) {
+    int n = t.N, base = 0;
+    if (lists) {
+        const int nc = meta[0];
+        n = FULL ? t.N - nc : nc;
+        base = FULL ? nc : 0;
+    }
+    // the active prefix of the grid is spread over all XCDs, then renumbered within itself
+    const int nact = (n + 255) >> 8;
+    const bool active = (int)blockIdx.x < nact;
+    if (!active && !reduce) return;
+    const int lb = active ? xcd_block_n(blockIdx.x, nact) : (int)blockIdx.x;
+    const int p = lb * 256 + (int)threadIdx.x;
     double rmax[4] = {0, 0, 0, 0}, rsum[4] = {0, 0, 0, 0};
-    if (i < t.N) {
-        const double gama = t.gamma;
-        const double dt = sc->dt;
-        int s[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++) s[k] = t.stencil[(size_t)k * t.N + i];
-        const double4 qc = Qin[i];
-        const float4 qcf = Qf_in[i];
-        const int lv = t.lvl[i];
-        const double dy = t.h / (double)(1 << lv);
-        const double dx = t.xf * dy;
-        const bool full = flags == nullptr || (flags[i] & 1);
-        float dE[4], dF[4];
-        float dev = 0.f;
-        {
-            const float4 col[5] = {Qf_in[s[5]], Qf_in[s[1]], qcf, Qf_in[s[0]], Qf_in[s[4]]};
-            if (full) flux_full_f<0>(col, (float)dx, (float)sc->lam_x, (float)gama, dE, dev);
-            else flux_lin_f<0>(col, (float)dx, (float)sc->lam_x, (float)gama, dE);
-        }
-        {
-            const float4 col[5] = {Qf_in[s[7]], Qf_in[s[3]], qcf, Qf_in[s[2]], Qf_in[s[6]]};
-            if (full) flux_full_f<1>(col, (float)dy, (float)sc->lam_y, (float)gama, dF, dev);
-            else flux_lin_f<1>(col, (float)dy, (float)sc->lam_y, (float)gama, dF);
-        }
-        if (label) label[i] = dev;
-        // FP64 forward-Euler substep and primitive RK blend (nnd_center_RK.F90:22-42,
-        // solution.F90:398-400)
-        double U[4], Ut[4];
-        U[0] = qc.x;
-        U[1] = qc.x * qc.y;
-        U[2] = qc.x * qc.z;
-        U[3] = qc.w / (gama - 1.0) + 0.5 * qc.x * (qc.y * qc.y + qc.z * qc.z);
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const double fu = -(double)dE[k] - (double)dF[k];
-            Ut[k] = U[k] + fu * dt;
-            const double fr = fu * dx * dy;
-            rmax[k] = fabs(fr);
-            rsum[k] = fr * fr;
-        }
-        const double r = Ut[0];
-        const double ir = 1.0 / r;
-        const double u = Ut[1] * ir, v = Ut[2] * ir;
-        const double p = (gama - 1.0) * (Ut[3] - 0.5 * r * (u * u + v * v));
-        const double4 qn = Qn[i];
-        double4 o;
-        o.x = alpha * qn.x + beta * r;
-        o.y = alpha * qn.y + beta * u;
-        o.z = alpha * qn.z + beta * v;
-        o.w = alpha * qn.w + beta * p;
-        Qout[i] = o;
-        Qf_out[i] = make_float4((float)o.x, (float)o.y, (float)o.z, (float)o.w);
+    if (active && p < n) {
+        const int i = lists ? lists[base + p] : p;
+        stage_leaf<FULL, LABEL>(t, i, Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, label, rmax, rsum);
     }
     if (reduce) {
         __shared__ double red[4][8];
@@ -262,12 +317,82 @@ __global__ __launch_bounds__(256) void k_stage_fast(DevTables t, const double4* 
         __syncthreads();
         if (threadIdx.x < 8) {
             const int k = threadIdx.x;
-            const int nw = blockDim.x >> 6;
             double acc = red[0][k];
-            for (int w = 1; w < nw; w++) acc = k < 4 ? fmax(acc, red[w][k]) : acc + red[w][k];
-            partials[(size_t)blockIdx.x * 8 + k] = acc;
+            for (int w = 1; w < 4; w++) acc = k < 4 ? fmax(acc, red[w][k]) : acc + red[w][k];
+            partials[(size_t)(poff + lb) * 8 + k] = acc;
         }
     }
+}
+
+// ---------------------------------------------------------------- leaf lists from flags
+// Deterministic compaction in two passes over blocks of LIST_BLK leaves: troubled counts per
+// block, then each block places its leaves after the counts of the blocks before it.
+constexpr int LIST_BLK = 1024;
+
+__device__ __forceinline__ int block_sum256(int v, int* sh) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wv] = v;
+    __syncthreads();
+    return sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+__global__ __launch_bounds__(256) void k_list_count(int n, const uint8_t* __restrict__ flags,
+                                                    int* __restrict__ cnt) {
+    __shared__ int sh[4];
+    const int base = blockIdx.x * LIST_BLK;
+    int c = 0;
+    for (int k = threadIdx.x; k < LIST_BLK; k += 256) {
+        const int i = base + k;
+        if (i < n) c += flags[i] & 1;
+    }
+    c = block_sum256(c, sh);
+    if (threadIdx.x == 0) cnt[blockIdx.x] = c;
+}
+
+__global__ __launch_bounds__(256) void k_list_write(int n, const uint8_t* __restrict__ flags,
+                                                    const int* __restrict__ cnt, int nblk,
+                                                    int32_t* __restrict__ lists, int* __restrict__ meta) {
+    __shared__ int sh[4], ws[4];
+    int pre = 0, tot = 0;
+    for (int j = threadIdx.x; j < nblk; j += 256) {
+        const int v = cnt[j];
+        tot += v;
+        if (j < (int)blockIdx.x) pre += v;
+    }
+    pre = block_sum256(pre, sh);
+    tot = block_sum256(tot, sh);
+    const int n_clear = n - tot;
+    // four consecutive leaves per thread; exclusive scan of the troubled counts over the block
+    const int i0 = blockIdx.x * LIST_BLK + 4 * threadIdx.x;
+    int tr[4], mine = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        tr[k] = (i0 + k < n) ? (flags[i0 + k] & 1) : 0;
+        mine += tr[k];
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int inc = mine;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(inc, o);
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) ws[wv] = inc;
+    __syncthreads();
+    int woff = 0;
+    for (int w = 0; w < wv; w++) woff += ws[w];
+    int t_pos = pre + woff + inc - mine;                       // troubled before this thread
+    int c_pos = blockIdx.x * LIST_BLK - pre + 4 * threadIdx.x - (woff + inc - mine);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int i = i0 + k;
+        if (i < n) {
+            if (tr[k]) lists[n_clear + t_pos++] = i;
+            else lists[c_pos++] = i;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) { meta[0] = n_clear; meta[1] = tot; }
 }
 
 // ---------------------------------------------------------------- FP32 WLS-WENO slots
@@ -278,7 +403,7 @@ __device__ __forceinline__ float isq_f(float k1, float k2, float k3, const float
 
 __global__ __launch_bounds__(128) void k_slots_fast(DevTables t, const float* __restrict__ wf,
                                                     float4* __restrict__ Qf) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= t.S_w) return;
     const float* __restrict__ f = wf + (size_t)s * WF_STRIDE;
     const int32_t* __restrict__ iv = t.wls_i + (size_t)s * WLSI_STRIDE;
@@ -362,13 +487,33 @@ void launch_slots_fast(const DevTables& t, const float* wf, const double4* Q, fl
     if (t.S_w > 0) hipLaunchKernelGGL(k_slots_fast, dim3(nb_(t.S_w, 128)), dim3(128), 0, st, t, wf, Qf);
 }
 
-void launch_stage_fast(const DevTables& t, const double4* Qin, const float4* Qf_in, const double4* Qn,
-                       double4* Qout, float4* Qf_out, const DevScalars* sc, double alpha, double beta,
-                       int reduce, double* partials, const uint8_t* flags, float* label, hipStream_t st) {
+void launch_stage_fast(const DevTables& t, const int32_t* lists, const int* meta, const double4* Qin,
+                       const float4* Qf_in, const double4* Qn, double4* Qout, float4* Qf_out,
+                       const DevScalars* sc, double alpha, double beta, int reduce, double* partials,
+                       float* label, hipStream_t st) {
     if (t.N <= 0) return;
-    hipLaunchKernelGGL(k_stage_fast, dim3(nb_(t.N, 256)), dim3(256), 0, st, t, Qin, Qf_in, Qn, Qout,
-                       Qf_out, sc, alpha, beta, reduce, partials, flags, label);
+    const int nb = nb_(t.N, 256);
+    if (lists)
+        hipLaunchKernelGGL((k_stage_fast<false, false>), dim3(nb), dim3(256), 0, st, t, lists, meta, Qin, Qf_in,
+                           Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials, 0, label);
+    if (label)
+        hipLaunchKernelGGL((k_stage_fast<true, true>), dim3(nb), dim3(256), 0, st, t, lists, meta, Qin, Qf_in,
+                           Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials, lists ? nb : 0, label);
+    else
+        hipLaunchKernelGGL((k_stage_fast<true, false>), dim3(nb), dim3(256), 0, st, t, lists, meta, Qin, Qf_in,
+                           Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials, lists ? nb : 0, label);
 }
+
+int stage_fast_partial_blocks(int n, bool lists) { return (lists ? 2 : 1) * nb_(n, 256); }
+
+void launch_build_lists(int n, const uint8_t* flags, int* blk_cnt, int32_t* lists, int* meta, hipStream_t st) {
+    if (n <= 0) return;
+    const int nblk = nb_(n, LIST_BLK);
+    hipLaunchKernelGGL(k_list_count, dim3(nblk), dim3(256), 0, st, n, flags, blk_cnt);
+    hipLaunchKernelGGL(k_list_write, dim3(nblk), dim3(256), 0, st, n, flags, blk_cnt, nblk, lists, meta);
+}
+
+int list_blocks(int n) { return nb_(n, LIST_BLK); }
 
 void launch_shadow(const double4* Q, float4* Qf, int n, hipStream_t st) {
     if (n > 0) hipLaunchKernelGGL(k_shadow, dim3(nb_(n, 256)), dim3(256), 0, st, Q, Qf, n);

@@ -208,7 +208,8 @@ __global__ __launch_bounds__(256) void k_stage(DevTables t, const double4* __res
                                                double4* Qout, double* Tout,
                                                const DevScalars* __restrict__ sc, double alpha,
                                                double beta, int reduce, double* __restrict__ partials) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lb = xcd_block();
+    const int i = lb * blockDim.x + threadIdx.x;
     double rmax[4] = {0, 0, 0, 0}, rsum[4] = {0, 0, 0, 0};
     if (i < t.N) {
         const double gama = t.gamma, cv = t.cv;
@@ -274,7 +275,7 @@ __global__ __launch_bounds__(256) void k_stage(DevTables t, const double4* __res
             const int nw = blockDim.x >> 6;
             double acc = red[0][k];
             for (int w = 1; w < nw; w++) acc = k < 4 ? fmax(acc, red[w][k]) : acc + red[w][k];
-            partials[(size_t)blockIdx.x * 8 + k] = acc;
+            partials[(size_t)lb * 8 + k] = acc;
         }
     }
 }

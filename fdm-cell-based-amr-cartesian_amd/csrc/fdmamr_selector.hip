@@ -1,6 +1,6 @@
 // fdmamr_selector.hip -- learned troubled-cell selector (no reference counterpart).
 //
-// A 4-layer perceptron (75 -> 64 -> 64 -> 64 -> 1, GELU, sigmoid) reads each interior leaf's
+// A 4-layer perceptron (75 -> 64 -> 64 -> 64 -> 1, ReLU, sigmoid) reads each interior leaf's
 // 5x5 stencil of (rho, p, |u|), each channel centred on the leaf and scaled by its largest
 // deviation on the stencil (plus 1e-3 of |centre|, so flat data maps to ~0), and predicts
 // whether the leaf's nonlinear WCNS3 weights deviate from the linear ones.
@@ -35,22 +35,10 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-// GELU, tanh form, and its derivative. tanh(y) = 1 - 2 / (1 + e^{2y}) with the hardware
-// exponential and reciprocal (a few instructions instead of a libm tanhf); the argument is
-// clamped so e^{2y} stays finite.
-__device__ __forceinline__ float fast_tanh(float y) {
-    y = fminf(fmaxf(y, -15.f), 15.f);
-    return 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + __expf(2.f * y));
-}
-__device__ __forceinline__ float gelu(float x) {
-    const float k = 0.7978845608028654f;
-    return 0.5f * x * (1.f + fast_tanh(k * (x + 0.044715f * x * x * x)));
-}
-__device__ __forceinline__ float gelu_d(float x) {
-    const float k = 0.7978845608028654f;
-    const float t = fast_tanh(k * (x + 0.044715f * x * x * x));
-    return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * k * (1.f + 3.f * 0.044715f * x * x);
-}
+// ReLU and its derivative (taken as 0 at 0). The activation is the selector's largest VALU
+// cost per leaf after the gathers (256 evaluations per leaf), so it is one instruction.
+__device__ __forceinline__ float relu(float x) { return fmaxf(x, 0.f); }
+__device__ __forceinline__ float relu_d(float x) { return x > 0.f ? 1.f : 0.f; }
 
 __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
     bf16x4 r;
@@ -64,57 +52,94 @@ __device__ __forceinline__ bf16x4 lds4(const uint16_t* p) {
 }
 
 // ---------------------------------------------------------------- features
-// Lane l of a wave computes for leaf (l % 16) of the tile the stencil points p = j, j+4, ...
-// (j = l / 16) of the 5x5 grid (p = (b+2)*5 + (a+2)); the four lanes of a leaf combine their
-// maxima by shuffles. Writes 80 BF16 features per leaf (75 used) to X (LDS or global).
+// Index-table row of point p of the 5x5 stencil: 0..7 the leaf's own stencil rows (axis
+// points), 8..23 the 16 off-axis rows, 24 the centre (and p > 24, which has no point).
+__device__ constexpr int point_row(int p) {
+    return p >= 25 ? 24
+         : (p % 5 == 2 && p / 5 == 2) ? 24
+         : (p / 5 == 2) ? (p % 5 == 3 ? 0 : p % 5 == 1 ? 1 : p % 5 == 4 ? 4 : 5)
+         : (p % 5 == 2) ? (p / 5 == 3 ? 2 : p / 5 == 1 ? 3 : p / 5 == 4 ? 6 : 7)
+         : 8 + (p / 5 - (p / 5 > 2 ? 1 : 0)) * 4 + (p % 5 - (p % 5 > 2 ? 1 : 0));
+}
+
+// State indices of the lane's 7 stencil points. Branch-free: every lane issues all 7 loads
+// (the centre, points past the grid and leaves past N read an in-range entry that is then
+// replaced by the leaf itself), so the loads of a tile travel together.
+__device__ __forceinline__ void gather_idx(const DevTables& t, const int32_t* __restrict__ off16, int leaf,
+                                           bool valid, int idx[7]) {
+    const int g = (threadIdx.x & 63) >> 4;
+    const int lc = valid ? leaf : 0;
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+        const int r0 = point_row(4 * k), r1 = point_row(4 * k + 1), r2 = point_row(4 * k + 2),
+                  r3 = point_row(4 * k + 3);
+        const int row = g == 0 ? r0 : g == 1 ? r1 : g == 2 ? r2 : r3;
+        const int rr = row < 24 ? row : 0;
+        const int32_t* arr = rr < 8 ? t.stencil + (size_t)rr * t.N : off16 + (size_t)(rr - 8) * t.N;
+        const int raw = arr[lc];
+        idx[k] = row < 24 ? raw : lc;
+    }
+}
+
+// Lane l of a wave computes for leaf (l % 16) of the tile the stencil points p = g, g+4, ...
+// (g = l / 16) of the 5x5 grid (p = (b+2)*5 + (a+2)); the four lanes of a leaf combine their
+// maxima by shuffles. Channel f of point p becomes (v - c) / (m + 1e-3 |c|), c the centre
+// value and m the larger of |c| and the largest |v - c| on the stencil, so every feature lies
+// in [-1, 1] and smooth data maps near 0. v[k][f] holds point g + 4k (0 where p > 24).
+__device__ __forceinline__ void gather_values(const float4* __restrict__ Qf, int leaf, bool valid,
+                                              const int idx[7], float v[7][3]) {
+    const int g = (threadIdx.x & 63) >> 4;
+    const float4 qc = Qf[valid ? leaf : 0];
+    float4 q[7];
+#pragma unroll
+    for (int k = 0; k < 7; k++) q[k] = Qf[idx[k]];
+    float c[3];
+    c[0] = valid ? qc.x : 0.f;
+    c[1] = valid ? qc.w : 0.f;
+    c[2] = valid ? __builtin_amdgcn_sqrtf(qc.y * qc.y + qc.z * qc.z) : 0.f;
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+        const bool in = valid && g + 4 * k < 25;
+        v[k][0] = in ? q[k].x : c[0];
+        v[k][1] = in ? q[k].w : c[1];
+        v[k][2] = in ? __builtin_amdgcn_sqrtf(q[k].y * q[k].y + q[k].z * q[k].z) : c[2];
+    }
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+        float mx = fabsf(c[f]);
+#pragma unroll
+        for (int k = 0; k < 7; k++) mx = fmaxf(mx, fabsf(v[k][f] - c[f]));
+        mx = fmaxf(mx, __shfl_xor(mx, 16));
+        mx = fmaxf(mx, __shfl_xor(mx, 32));
+        const float sc = __builtin_amdgcn_rcpf(mx + 1e-3f * fabsf(c[f]) + 1e-30f);
+#pragma unroll
+        for (int k = 0; k < 7; k++) v[k][f] = g + 4 * k < 25 ? (v[k][f] - c[f]) * sc : 0.f;
+    }
+}
+
+__device__ __forceinline__ void gather_features(const DevTables& t, const float4* __restrict__ Qf,
+                                                const int32_t* __restrict__ off16, int leaf, bool valid,
+                                                float v[7][3]) {
+    int idx[7];
+    gather_idx(t, off16, leaf, valid, idx);
+    gather_values(Qf, leaf, valid, idx, v);
+}
+
+// Writes the 80 BF16 features of the lane's leaf (75 used, feature p*3 + f) to xrow.
 __device__ __forceinline__ void leaf_features(const DevTables& t, const float4* __restrict__ Qf,
                                               const int32_t* __restrict__ off16, int leaf, bool valid,
                                               uint16_t* xrow) {
-    const int lane = threadIdx.x & 63, part = lane >> 4;
-    float c[3] = {0.f, 0.f, 0.f};
-    float vals[7][3];
-    int np = 0;
-    if (valid) {
-        const float4 q = Qf[leaf];
-        c[0] = q.x; c[1] = q.w; c[2] = __builtin_amdgcn_sqrtf(q.y * q.y + q.z * q.z);
-    }
+    const int g = (threadIdx.x & 63) >> 4;
+    float v[7][3];
+    gather_features(t, Qf, off16, leaf, valid, v);
 #pragma unroll
     for (int k = 0; k < 7; k++) {
-        const int p = part + 4 * k;
-        vals[k][0] = vals[k][1] = vals[k][2] = 0.f;
-        if (p < 25 && valid) {
-            const int a = p % 5 - 2, b = p / 5 - 2;
-            int idx;
-            if (a == 0 && b == 0) idx = leaf;
-            else if (b == 0) idx = t.stencil[(size_t)(a == 1 ? 0 : a == -1 ? 1 : a == 2 ? 4 : 5) * t.N + leaf];
-            else if (a == 0) idx = t.stencil[(size_t)(b == 1 ? 2 : b == -1 ? 3 : b == 2 ? 6 : 7) * t.N + leaf];
-            else {
-                const int q16 = (b + 2 - (b > 0 ? 1 : 0)) * 4 + (a + 2 - (a > 0 ? 1 : 0));
-                idx = off16[(size_t)q16 * t.N + leaf];
-            }
-            const float4 q = Qf[idx];
-            vals[k][0] = q.x; vals[k][1] = q.w; vals[k][2] = __builtin_amdgcn_sqrtf(q.y * q.y + q.z * q.z);
-            np = k + 1;
-        }
-    }
-    float s[3];
-#pragma unroll
-    for (int f = 0; f < 3; f++) {
-        float mx = 0.f;
-#pragma unroll
-        for (int k = 0; k < 7; k++) mx = fmaxf(mx, fabsf(vals[k][f] - c[f]));
-        mx = fmaxf(mx, __shfl_xor(mx, 16));
-        mx = fmaxf(mx, __shfl_xor(mx, 32));
-        s[f] = 1.f / (mx + 1e-3f * fabsf(c[f]) + 1e-30f);
-    }
-#pragma unroll
-    for (int k = 0; k < 7; k++) {
-        const int p = part + 4 * k;
+        const int p = g + 4 * k;
         if (p < 25)
 #pragma unroll
-            for (int f = 0; f < 3; f++) xrow[p * 3 + f] = f2bf(k < np ? (vals[k][f] - c[f]) * s[f] : 0.f);
+            for (int f = 0; f < 3; f++) xrow[p * 3 + f] = f2bf(v[k][f]);
     }
-    if (part == 0)
+    if (g == 0)
 #pragma unroll
         for (int f = 75; f < MLP_IN; f++) xrow[f] = 0;
 }
@@ -149,7 +174,7 @@ __device__ __forceinline__ void mlp_forward(const uint16_t* sW, const float* sB,
     for (int mt = 0; mt < 4; mt++) {
         float h[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) h[j] = gelu(o.z[0][mt][j]);
+        for (int j = 0; j < 4; j++) h[j] = relu(o.z[0][mt][j]);
         hb[mt] = pack4(h[0], h[1], h[2], h[3]);
         if (sH1)
 #pragma unroll
@@ -177,7 +202,7 @@ __device__ __forceinline__ void mlp_forward(const uint16_t* sW, const float* sB,
             for (int mt = 0; mt < 4; mt++) {
                 float h[4];
 #pragma unroll
-                for (int j = 0; j < 4; j++) h[j] = gelu(o.z[1][mt][j]);
+                for (int j = 0; j < 4; j++) h[j] = relu(o.z[1][mt][j]);
                 hb[mt] = pack4(h[0], h[1], h[2], h[3]);
                 if (sH2)
 #pragma unroll
@@ -191,7 +216,7 @@ __device__ __forceinline__ void mlp_forward(const uint16_t* sW, const float* sB,
     for (int mt = 0; mt < 4; mt++)
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            const float h = gelu(o.z[2][mt][j]);
+            const float h = relu(o.z[2][mt][j]);
             o.h3[mt][j] = h;
             part += sB[SB_W4 + 16 * mt + 4 * g + j] * h;
         }
@@ -218,74 +243,152 @@ __device__ __forceinline__ uint32_t hash32(uint32_t x) {
 }
 
 // ---------------------------------------------------------------- inference + judge sampling
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ bf16x8 frag(const uint16_t* sF, int f) {
+    return *reinterpret_cast<const bf16x8*>(sF + (f * 64 + (threadIdx.x & 63)) * 8);
+}
+// ReLU of two 16-unit accumulator tiles packed as the B fragment of the next layer: slots
+// 0-3 hold units 32s + 4g + j, slots 4-7 units 32s + 16 + 4g + j (the weight fragments use
+// the same K order).
+__device__ __forceinline__ bf16x8 relu_pack(const f32x4& lo, const f32x4& hi) {
+    bf16x8 r;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        r[j] = (__bf16)relu(lo[j]);
+        r[4 + j] = (__bf16)relu(hi[j]);
+    }
+    return r;
+}
+
 // flags bit 0: troubled (full path); bit 1: judged this step (cleared by the network but
 // sent through the full path with a label); bit 2: forced troubled by an earlier judge
 // rejection (sticky until the tables are rebuilt).
+//
+// Inference keeps every activation in registers: the feature gather of lane (r, g) yields
+// exactly the 24 K-slots of the layer-1 B fragments of leaf r (feature q = 8s + j of the lane
+// is channel q % 3 of its point g + 4 (q / 3)), and each layer's accumulators are the next
+// layer's B fragments. Weights are read from LDS as lane-contiguous fragments.
 __global__ __launch_bounds__(256) void k_select(DevTables t, const float4* __restrict__ Qf,
                                                 const int32_t* __restrict__ off16,
                                                 const uint16_t* __restrict__ wpk,
                                                 const float* __restrict__ w32, SelectParams sp,
-                                                uint8_t* __restrict__ flags, float* __restrict__ prob,
-                                                int* __restrict__ counters) {
-    __shared__ uint16_t sW[KW_FWD];
-    __shared__ float sB[SB_TOTAL];
-    __shared__ uint16_t sX[4][16 * MLP_IN];
-    load_weights(wpk, w32, sW, sB, KW_FWD);
+                                                uint8_t* __restrict__ flags, float* __restrict__ prob) {
+    __shared__ __align__(16) uint16_t sF[KF_TOTAL];
+    __shared__ __align__(16) float sB[SB_TOTAL];
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(wpk + KW_FRAG);
+        uint4* dst = reinterpret_cast<uint4*>(sF);
+        for (int k = threadIdx.x; k < KF_TOTAL / 8; k += blockDim.x) dst[k] = src[k];
+        for (int k = threadIdx.x; k < MLP_H; k += blockDim.x) {
+            sB[SB_B1 + k] = w32[P_B1 + k];
+            sB[SB_B2 + k] = w32[P_B2 + k];
+            sB[SB_B3 + k] = w32[P_B3 + k];
+            sB[SB_W4 + k] = w32[P_W4 + k];
+        }
+        if (threadIdx.x == 0) sB[SB_B4] = w32[P_B4];
+    }
     __syncthreads();
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
     const int ntiles = (t.N + 15) / 16;
-    int n_tr = 0;
+    const int lb = xcd_block();
+    // the stencil indices of the next tile are loaded while this tile's network runs
+    int idx[7];
+    {
+        const int leaf0 = (lb * sp.tiles_per_wave * 4 + wv) * 16 + r;
+        gather_idx(t, off16, leaf0, leaf0 < t.N, idx);
+    }
     for (int it = 0; it < sp.tiles_per_wave; it++) {
-        const int tile = (blockIdx.x * sp.tiles_per_wave + it) * 4 + wv;
-        const int leaf = tile * 16 + (lane & 15);
-        const bool valid = tile < ntiles && leaf < t.N;
-        leaf_features(t, Qf, off16, leaf, valid, sX[wv] + (lane & 15) * MLP_IN);
-        __syncthreads();
-        FwdOut o;
-        mlp_forward(sW, sB, sX[wv], o, nullptr, nullptr);
-        __syncthreads();
-        if (valid && lane < 16) {
-            const float pr = 1.f / (1.f + __expf(-o.z4));
+        const int tile = (lb * sp.tiles_per_wave + it) * 4 + wv;
+        if (tile >= ntiles) break;
+        const int leaf = tile * 16 + r;
+        const bool valid = leaf < t.N;
+        float v[7][3];
+        gather_values(Qf, leaf, valid, idx, v);
+        if (it + 1 < sp.tiles_per_wave) {
+            const int nleaf = leaf + 64;
+            gather_idx(t, off16, nleaf, nleaf < t.N, idx);
+        }
+        bf16x8 xb[3];
+#pragma unroll
+        for (int q = 0; q < 24; q++) xb[q >> 3][q & 7] = (__bf16)(q < 21 ? v[q / 3][q % 3] : 0.f);
+        f32x4 z[4];
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) {
+            z[mt] = *reinterpret_cast<const f32x4*>(sB + SB_B1 + 16 * mt + 4 * g);
+#pragma unroll
+            for (int s3 = 0; s3 < 3; s3++) z[mt] = mfma32(frag(sF + KF_L1, mt * 3 + s3), xb[s3], z[mt]);
+        }
+#pragma unroll
+        for (int L = 0; L < 2; L++) {
+            const bf16x8 hb0 = relu_pack(z[0], z[1]), hb1 = relu_pack(z[2], z[3]);
+            const uint16_t* W = sF + (L == 0 ? KF_L2 : KF_L3);
+            const int boff = L == 0 ? SB_B2 : SB_B3;
+#pragma unroll
+            for (int mt = 0; mt < 4; mt++) {
+                f32x4 acc = *reinterpret_cast<const f32x4*>(sB + boff + 16 * mt + 4 * g);
+                acc = mfma32(frag(W, mt * 2 + 0), hb0, acc);
+                acc = mfma32(frag(W, mt * 2 + 1), hb1, acc);
+                z[mt] = acc;
+            }
+        }
+        float part = 0.f;
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) {
+            const f32x4 w4 = *reinterpret_cast<const f32x4*>(sB + SB_W4 + 16 * mt + 4 * g);
+#pragma unroll
+            for (int j = 0; j < 4; j++) part += w4[j] * relu(z[mt][j]);
+        }
+        part += __shfl_xor(part, 16);
+        part += __shfl_xor(part, 32);
+        const float z4 = part + sB[SB_B4];
+        if (valid && g == 0) {
+            const float pr = 1.f / (1.f + __expf(-z4));
             if (prob) prob[leaf] = pr;
             uint8_t f = flags[leaf] & 4;
             if (pr >= sp.threshold || f) f |= 1;
             else if ((hash32((uint32_t)leaf * 2654435761u ^ sp.step) & 0xFFFFu) < sp.judge_q16) f |= 3;
             flags[leaf] = f;
-            n_tr += f & 1;
         }
-    }
-    if (counters) {
-        for (int o = 32; o > 0; o >>= 1) n_tr += __shfl_xor(n_tr, o);
-        if (lane == 0 && n_tr) atomicAdd(&counters[CNT_TROUBLED], n_tr);
     }
 }
 
-// judge: a judged leaf whose label exceeds the bound is forced back to the full path
-// Each block scans 4096 leaves (16 per thread) and adds its counts with one atomic, so the
-// counters see a few hundred atomics per step, not one per wave.
+// judge: a judged leaf whose label exceeds the bound is forced back to the full path; also
+// counts the leaves that took the full path (bit 0). Each block scans 4096 leaves (16 per
+// thread) and adds its counts with one atomic per counter, a few hundred atomics per step.
 __global__ __launch_bounds__(256) void k_judge(int n, const float* __restrict__ label, float bound,
                                                uint8_t* __restrict__ flags, int* __restrict__ counters) {
-    int judged = 0, rejected = 0;
+    int judged = 0, rejected = 0, troubled = 0;
     const int base = blockIdx.x * 4096;
     for (int k = threadIdx.x; k < 4096; k += 256) {
         const int i = base + k;
         if (i >= n) break;
         const uint8_t f = flags[i];
+        troubled += f & 1;
         if (f & 2) {
             judged++;
             if (label[i] > bound) { rejected++; flags[i] = 5; }
             else flags[i] = f & 4;
         }
     }
-    for (int o = 32; o > 0; o >>= 1) { judged += __shfl_xor(judged, o); rejected += __shfl_xor(rejected, o); }
-    __shared__ int sj[4], sr[4];
+    for (int o = 32; o > 0; o >>= 1) {
+        judged += __shfl_xor(judged, o);
+        rejected += __shfl_xor(rejected, o);
+        troubled += __shfl_xor(troubled, o);
+    }
+    __shared__ int sj[4], sr[4], st[4];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) { sj[wv] = judged; sr[wv] = rejected; }
+    if (lane == 0) { sj[wv] = judged; sr[wv] = rejected; st[wv] = troubled; }
     __syncthreads();
     if (threadIdx.x == 0) {
         const int j = sj[0] + sj[1] + sj[2] + sj[3], r = sr[0] + sr[1] + sr[2] + sr[3];
+        const int tr = st[0] + st[1] + st[2] + st[3];
         if (j) atomicAdd(&counters[CNT_JUDGED], j);
         if (r) atomicAdd(&counters[CNT_REJECTED], r);
+        if (tr) atomicAdd(&counters[CNT_TROUBLED], tr);
     }
 }
 
@@ -321,11 +424,11 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const uint16_t* __restrict__ wp
                                                  const uint16_t* __restrict__ X, const uint8_t* __restrict__ y,
                                                  const int32_t* __restrict__ rows, int n, float pos_weight,
                                                  float scale, float* __restrict__ gpart, float* __restrict__ lpart) {
-    __shared__ uint16_t sW[KW_TOTAL];
+    __shared__ uint16_t sW[KW_PLAIN];
     __shared__ float sB[SB_TOTAL];
     __shared__ uint16_t sX[16 * MLP_IN];
     __shared__ uint16_t sH1[MLP_H * 16], sH2[MLP_H * 16], sDZ[MLP_H * 16];
-    load_weights(wpk, w32, sW, sB, KW_TOTAL);
+    load_weights(wpk, w32, sW, sB, KW_PLAIN);
     float* gp = gpart + (size_t)blockIdx.x * P_TOTAL;
     for (int k = threadIdx.x; k < P_TOTAL; k += 64) gp[k] = 0.f;
     __syncthreads();
@@ -363,13 +466,13 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const uint16_t* __restrict__ wp
                     if (r == 0) gp[P_W4 + 16 * mt + 4 * g + j] += v;
                 }
         }
-        // dz3 = w4 dz4 gelu'(z3)
+        // dz3 = w4 dz4 relu'(z3)
         f32x4 dz[4];
 #pragma unroll
         for (int mt = 0; mt < 4; mt++)
 #pragma unroll
             for (int j = 0; j < 4; j++)
-                dz[mt][j] = sB[SB_W4 + 16 * mt + 4 * g + j] * dz4 * gelu_d(o.z[2][mt][j]);
+                dz[mt][j] = sB[SB_W4 + 16 * mt + 4 * g + j] * dz4 * relu_d(o.z[2][mt][j]);
         // layers 3, 2, 1 (L = 2, 1, 0)
 #pragma unroll
         for (int L = 2; L >= 0; L--) {
@@ -417,7 +520,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const uint16_t* __restrict__ wp
                 }
             }
             if (L > 0) {
-                // dh_{L-1}[in][leaf] = W_L[in][out] dz[out][leaf]; then dz_{L-1} = dh gelu'(z_{L-1})
+                // dh_{L-1}[in][leaf] = W_L[in][out] dz[out][leaf]; then dz_{L-1} = dh relu'(z_{L-1})
                 const int wn = L == 2 ? KW_W3N : KW_W2N;
                 f32x4 nd[4];
 #pragma unroll
@@ -433,7 +536,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const uint16_t* __restrict__ wp
 #pragma unroll
                 for (int mt = 0; mt < 4; mt++)
 #pragma unroll
-                    for (int j = 0; j < 4; j++) dz[mt][j] = nd[mt][j] * gelu_d(o.z[L - 1][mt][j]);
+                    for (int j = 0; j < 4; j++) dz[mt][j] = nd[mt][j] * relu_d(o.z[L - 1][mt][j]);
             }
             __syncthreads();
         }
@@ -485,17 +588,31 @@ __global__ void k_pack(const float* __restrict__ w, uint16_t* __restrict__ pk) {
         pk[KW_W2N + k] = f2bf(w[P_W2 + k]);
         pk[KW_W3N + k] = f2bf(w[P_W3 + k]);
     }
+    if (k < KF_TOTAL) {
+        // fragment order of the inference kernel: element j of lane (r, g) in fragment f
+        const int j = k & 7, lane = (k >> 3) & 63, r = lane & 15, g = lane >> 4;
+        float val = 0.f;
+        if (k < KF_L2) {
+            const int f = k / KF_FRAG, mt = f / 3, q = (f % 3) * 8 + j, p = g + 4 * (q / 3);
+            if (q < 21 && p < 25) val = w[P_W1 + (p * 3 + q % 3) * MLP_H + 16 * mt + r];
+        } else {
+            const int base = k < KF_L3 ? P_W2 : P_W3;
+            const int f = ((k < KF_L3 ? k - KF_L2 : k - KF_L3)) / KF_FRAG, mt = f / 2, s2 = f % 2;
+            const int unit = 32 * s2 + (j < 4 ? 4 * g + j : 16 + 4 * g + j - 4);
+            val = w[base + unit * MLP_H + 16 * mt + r];
+        }
+        pk[KW_FRAG + k] = f2bf(val);
+    }
 }
 
 // ---------------------------------------------------------------- launchers
 void launch_select(const DevTables& t, const float4* Qf, const int32_t* off16, const uint16_t* wpk,
-                   const float* w32, const SelectParams& sp, uint8_t* flags, float* prob, int* counters,
-                   hipStream_t st) {
+                   const float* w32, const SelectParams& sp, uint8_t* flags, float* prob, hipStream_t st) {
     if (t.N <= 0) return;
     const int ntiles = (t.N + 15) / 16;
     const int per_block = 4 * sp.tiles_per_wave;
     const int nb = (ntiles + per_block - 1) / per_block;
-    hipLaunchKernelGGL(k_select, dim3(nb), dim3(256), 0, st, t, Qf, off16, wpk, w32, sp, flags, prob, counters);
+    hipLaunchKernelGGL(k_select, dim3(nb), dim3(256), 0, st, t, Qf, off16, wpk, w32, sp, flags, prob);
 }
 
 void launch_judge(int n, const float* label, float bound, uint8_t* flags, int* counters, hipStream_t st) {
@@ -529,7 +646,7 @@ void launch_adam(float* w, const float* grad, float* m, float* v, float lr, floa
 }
 
 void launch_pack(const float* w, uint16_t* pk, hipStream_t st) {
-    hipLaunchKernelGGL(k_pack, dim3((MLP_H * MLP_IN + 255) / 256), dim3(256), 0, st, w, pk);
+    hipLaunchKernelGGL(k_pack, dim3((KF_TOTAL + 255) / 256), dim3(256), 0, st, w, pk);
 }
 
 }  // namespace fdm

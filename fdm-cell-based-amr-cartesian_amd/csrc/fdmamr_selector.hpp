@@ -28,7 +28,18 @@ constexpr int KW_W3T = KW_W2T + MLP_H * MLP_H;
 constexpr int KW_FWD = KW_W3T + MLP_H * MLP_H;  // forward needs the first three
 constexpr int KW_W2N = KW_FWD;
 constexpr int KW_W3N = KW_W2N + MLP_H * MLP_H;
-constexpr int KW_TOTAL = KW_W3N + MLP_H * MLP_H;
+constexpr int KW_PLAIN = KW_W3N + MLP_H * MLP_H;
+// Inference operands in MFMA fragment order for v_mfma_f32_16x16x32_bf16: fragment f of a
+// layer is 64 lanes x 8 BF16, lane-contiguous, so one 16-byte LDS read per lane fetches it.
+// Layer 1: f = mt*3 + s (K = 96: the 75 features in the lane order the feature gather
+// produces them); layers 2, 3: f = mt*2 + s.
+constexpr int KF_FRAG = 64 * 8;
+constexpr int KF_L1 = 0;
+constexpr int KF_L2 = KF_L1 + 12 * KF_FRAG;
+constexpr int KF_L3 = KF_L2 + 8 * KF_FRAG;
+constexpr int KF_TOTAL = KF_L3 + 8 * KF_FRAG;   // 14336
+constexpr int KW_FRAG = KW_PLAIN;
+constexpr int KW_TOTAL = KW_FRAG + KF_TOTAL;
 
 // FP32 vectors staged in LDS
 constexpr int SB_B1 = 0, SB_B2 = 64, SB_B3 = 128, SB_W4 = 192, SB_B4 = 256, SB_TOTAL = 260;
@@ -44,8 +55,7 @@ struct SelectParams {
 };
 
 void launch_select(const DevTables& t, const float4* Qf, const int32_t* off16, const uint16_t* wpk,
-                   const float* w32, const SelectParams& sp, uint8_t* flags, float* prob, int* counters,
-                   hipStream_t st);
+                   const float* w32, const SelectParams& sp, uint8_t* flags, float* prob, hipStream_t st);
 void launch_judge(int n, const float* label, float bound, uint8_t* flags, int* counters, hipStream_t st);
 void launch_features(const DevTables& t, const float4* Qf, const int32_t* off16, uint16_t* X, int row0,
                      hipStream_t st);

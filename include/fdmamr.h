@@ -167,7 +167,7 @@ int32_t fdmamr_selector_get_flags(fdmamr_ctx* ctx, uint8_t* flags);
  * mean square so flat data reports 0) */
 int32_t fdmamr_labels(fdmamr_ctx* ctx, float* dev);
 
-/* The network: 75 -> 64 -> 64 -> 64 -> 1, GELU, sigmoid; BF16 MFMA with FP32 accumulation.
+/* The network: 75 -> 64 -> 64 -> 64 -> 1, ReLU, sigmoid; BF16 MFMA with FP32 accumulation.
  * Parameters (FP32 master copy, FDMAMR_SELECTOR_NPARAM floats): W1 [80][64] (input rows 75..79
  * are padding and stay zero), b1[64], W2 [64][64], b2, W3 [64][64], b3, w4[64], b4. */
 #define FDMAMR_SELECTOR_NPARAM 13569

@@ -1,7 +1,7 @@
 """Pins for the selector network's CPU oracle (oracle/mlp_oracle.py).
 
 The FP64 network (bf16=False) is checked against central finite differences of its own loss,
-which fixes every backward formula (a transposed weight, a missing GELU derivative or a
+which fixes every backward formula (a transposed weight, a missing activation derivative or a
 dropped bias term fails it). The BF16-rounding variant must agree with it to BF16 precision.
 """
 import numpy as np
