@@ -19,6 +19,7 @@
 
 using namespace fdm;
 
+
 template <typename T>
 struct DBuf {
     T* p = nullptr;
@@ -45,6 +46,9 @@ struct fdmamr_ctx {
     DBuf<float> label;       // per-leaf WCNS weight deviation (labels / judge)
     bool use_flags = false;  // fast path consults the flags
     DBuf<int32_t> lists;     // cleared leaves, then troubled leaves (from the flags)
+    DBuf<uint16_t> tile_mask;  // selector scratch: leaves of each 16-leaf tile needing the network
+    DBuf<uint4> tile_frag;     // selector scratch: their layer-1 MFMA fragments
+    int n_cu = 256;
     DBuf<int> list_cnt, list_meta;
 
     // learned selector
@@ -142,7 +146,7 @@ int upload_tables(fdmamr_ctx* c) {
     HIPCHK(c, ensure(c->state, tb.state_of_node.size()));
     const int nb = std::max(1, nblocks(tb.N, 256));
     HIPCHK(c, ensure(c->partials, (size_t)stage_fast_partial_blocks(tb.N, true) * 8));
-    HIPCHK(c, ensure(c->lampart, (size_t)nb * 2));
+    HIPCHK(c, ensure(c->lampart, (size_t)stage_fast_partial_blocks(tb.N, true) * 2));
     HIPCHK(c, ensure(c->rot, tb.N));
     HIPCHK(c, ensure(c->div, tb.N));
     HIPCHK(c, ensure(c->tmp, 64));
@@ -183,6 +187,11 @@ int upload_tables(fdmamr_ctx* c) {
     HIPCHK(c, up(c->bc_ref.p, tb.bc_ref.data(), tb.bc_ref.size() * 4));
     HIPCHK(c, ensure(c->off16, tb.off16.size()));
     HIPCHK(c, ensure(c->sel_prob, tb.N));
+    if (c->p.flux_precision == FDMAMR_FLUX_FP32) {
+        const size_t ntiles = ((size_t)tb.N + 15) / 16;
+        HIPCHK(c, ensure(c->tile_mask, ntiles));
+        HIPCHK(c, ensure(c->tile_frag, ntiles * 3 * 64));
+    }
     HIPCHK(c, up(c->off16.p, tb.off16.data(), tb.off16.size() * 4));
     // every leaf troubled until the selector says otherwise
     if (tb.N > 0) HIPCHK(c, hipMemsetAsync(c->flags.p, 1, tb.N, s));
@@ -237,6 +246,18 @@ int check_ready(fdmamr_ctx* c) {
     return 0;
 }
 
+// selector counters summed over their slots (synchronises the stream)
+hipError_t read_counters(fdmamr_ctx* c, int out[CNT_N]) {
+    std::vector<int> all(CNT_N * CNT_SLOTS);
+    hipError_t e = hipMemcpyAsync(all.data(), c->sel_cnt.p, all.size() * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    for (int k = 0; k < CNT_N; k++) {
+        out[k] = 0;
+        for (int sl = 0; sl < CNT_SLOTS; sl++) out[k] += all[sl * CNT_N + k];
+    }
+    return e;
+}
+
 int next_event(fdmamr_ctx* c) {
     if (c->ev_used == (int)c->ev_pool.size()) {
         hipEvent_t e;
@@ -246,8 +267,10 @@ int next_event(fdmamr_ctx* c) {
     return c->ev_used++;
 }
 
-// one SSP-RK3 step from buffer 0 back into buffer 0 (solution.F90:267-298)
-void enqueue_rk3(fdmamr_ctx* c, int advance_time) {
+// one SSP-RK3 step from buffer 0 back into buffer 0 (solution.F90:267-298); with prepare_next
+// the final stage also reduces the wave speeds of its output and the step's end computes the
+// next step's lambda and dt (what wavespeed + step_begin would do at the next step's start)
+void enqueue_rk3(fdmamr_ctx* c, int advance_time, bool prepare_next) {
     static const double ra[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0};   // solution.F90:392-393
     static const double rb[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
     static const int in[3] = {0, 1, 2}, out[3] = {1, 2, 0};
@@ -265,8 +288,9 @@ void enqueue_rk3(fdmamr_ctx* c, int advance_time) {
         if (fast) launch_slots_fast(d, c->wls_f.p, c->Q[in[k]].p, c->Qf[in[k]].p, s);
         else launch_slots(d, c->Q[in[k]].p, c->T[in[k]].p, s);
         if (select && k == 0) {
-            SelectParams sp{c->sel_threshold, (uint32_t)(c->sel_judge * 65536.0f), c->sel_step++, 8};
-            launch_select(d, c->Qf[0].p, c->off16.p, c->sel_pk.p, c->sel_w.p, sp, c->flags.p, nullptr, s);
+            SelectParams sp{c->sel_threshold, (uint32_t)(c->sel_judge * 65536.0f), c->sel_step++};
+            launch_select(d, c->Qf[0].p, c->off16.p, c->sel_pk.p, c->sel_w.p, sp, c->flags.p, nullptr,
+                          c->tile_mask.p, c->tile_frag.p, 4 * c->n_cu, s);
             launch_build_lists(d.N, c->flags.p, c->list_cnt.p, c->lists.p, c->list_meta.p, s);
         }
         int e0 = -1, e1 = -1;
@@ -274,10 +298,12 @@ void enqueue_rk3(fdmamr_ctx* c, int advance_time) {
         if (fast)
             launch_stage_fast(d, lists, c->list_meta.p, c->Q[in[k]].p, c->Qf[in[k]].p, c->Q[0].p,
                               c->Q[out[k]].p, c->Qf[out[k]].p, c->sc, ra[k], rb[k], k == 2 ? 1 : 0,
-                              c->partials.p, (select && k == 0) ? c->label.p : nullptr, s);
+                              c->partials.p, (prepare_next && k == 2) ? c->lampart.p : nullptr,
+                              (select && k == 0) ? c->label.p : nullptr, s);
         else
-            launch_stage(d, 0, c->Q[in[k]].p, c->Q[0].p, c->T[0].p, c->Q[out[k]].p, c->T[out[k]].p,
-                         c->sc, ra[k], rb[k], k == 2 ? 1 : 0, c->partials.p, s);
+            launch_stage(d, c->Q[in[k]].p, c->Q[0].p, c->T[0].p, c->Q[out[k]].p, c->T[out[k]].p, c->sc, ra[k],
+                         rb[k], k == 2 ? 1 : 0, c->partials.p, (prepare_next && k == 2) ? c->lampart.p : nullptr,
+                         s);
         if (e0 >= 0 && (e1 = next_event(c)) >= 0) {
             (void)hipEventRecord(c->ev_pool[e1], s);
             c->ev_stage.emplace_back(e0, e1);
@@ -285,8 +311,9 @@ void enqueue_rk3(fdmamr_ctx* c, int advance_time) {
         if (select && k == 0) launch_judge(d.N, c->label.p, c->sel_bound, c->flags.p, c->sel_cnt.p, s);
         fill_after_stage(c, out[k]);
     }
-    launch_step_end(c->partials.p, fast ? stage_fast_partial_blocks(d.N, lists != nullptr) : nblocks(d.N, 256),
-                    d.N, advance_time, c->sc, s);
+    const int npart = fast ? stage_fast_partial_blocks(d.N, lists != nullptr) : nblocks(d.N, 256);
+    launch_step_end(c->partials.p, npart, d.N, advance_time, c->sc, prepare_next ? c->lampart.p : nullptr, npart,
+                    s);
 }
 
 }  // namespace
@@ -316,6 +343,11 @@ int32_t fdmamr_create(const fdmamr_params* p, fdmamr_ctx** out) {
         delete c;
         return -2;
     }
+    if (!host_only) {
+        int n_cu = 0;
+        if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, p->device) == hipSuccess && n_cu > 0)
+            c->n_cu = n_cu;
+    }
     c->tree.configure(p->xmin, p->xmax, p->ymin, p->ymax, p->h, p->gamma, p->Ma > 0 ? p->Ma : 1.0);
     for (int k = 0; k < 4; k++) { c->tree.bc[k] = kinds[k]; c->tree.inflow[k] = p->inflow[k]; }
     c->tree.wls_local = p->wls_local ? 1 : 0;
@@ -337,6 +369,7 @@ void fdmamr_destroy(fdmamr_ctx* c) {
     release(c->sel_grad); release(c->sel_gpart); release(c->sel_lpart); release(c->sel_loss);
     release(c->sel_prob); release(c->sel_pk); release(c->sel_cnt); release(c->ds_x); release(c->ds_y);
     release(c->ds_idx); release(c->lists); release(c->list_cnt); release(c->list_meta);
+    release(c->tile_mask); release(c->tile_frag);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->sc) (void)hipFree(c->sc);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -565,7 +598,7 @@ int32_t fdmamr_advance_rk3(fdmamr_ctx* c, double dt, double lam_x, double lam_y,
     DevScalars s{};
     s.dt = dt; s.lam_x = lam_x; s.lam_y = lam_y; s.h = c->tree.h; s.xf = c->tree.xf;
     HIPCHK(c, hipMemcpyAsync(c->sc, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
-    enqueue_rk3(c, 0);
+    enqueue_rk3(c, 0, false);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(&s, c->sc, sizeof s, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -593,9 +626,11 @@ int32_t fdmamr_run_steps(fdmamr_ctx* c, int32_t n_steps, int32_t localdt, double
     HIPCHK(c, hipMemcpyAsync(c->sc, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
     const int nb = std::max(1, nblocks(c->tb.N, 256));
     for (int it = 0; it < n_steps && c->tb.N > 0; it++) {
-        launch_wavespeed(c->dt, c->Q[0].p, c->lampart.p, c->stream);
-        launch_step_begin(c->lampart.p, nb, c->sc, c->stream);
-        enqueue_rk3(c, 1);
+        if (it == 0) {
+            launch_wavespeed(c->dt, c->Q[0].p, c->lampart.p, c->stream);
+            launch_step_begin(c->lampart.p, nb, c->sc, c->stream);
+        }
+        enqueue_rk3(c, 1, it + 1 < n_steps);
     }
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(&s, c->sc, sizeof s, hipMemcpyDeviceToHost, c->stream));
@@ -735,7 +770,7 @@ int32_t fdmamr_labels(fdmamr_ctx* c, float* dev) {
     launch_shadow(c->Q[0].p, c->Qf[0].p, c->tb.L + c->tb.S_w + c->tb.S_r, c->stream);
     launch_slots_fast(c->dt, c->wls_f.p, c->Q[0].p, c->Qf[0].p, c->stream);
     launch_stage_fast(c->dt, nullptr, nullptr, c->Q[0].p, c->Qf[0].p, c->Q[0].p, c->Q[2].p, c->Qf[2].p,
-                      c->sc, 0.0, 1.0, 0, c->partials.p, c->label.p, c->stream);
+                      c->sc, 0.0, 1.0, 0, c->partials.p, nullptr, c->label.p, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(dev, c->label.p, c->tb.N * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -749,7 +784,7 @@ static int sel_alloc(fdmamr_ctx* c) {
     HIPCHK(c, ensure(c->sel_v, P_TOTAL));
     HIPCHK(c, ensure(c->sel_grad, P_TOTAL));
     HIPCHK(c, ensure(c->sel_pk, KW_TOTAL));
-    HIPCHK(c, ensure(c->sel_cnt, CNT_N));
+    HIPCHK(c, ensure(c->sel_cnt, CNT_N * CNT_SLOTS));
     HIPCHK(c, ensure(c->sel_loss, 4));
     return 0;
 }
@@ -761,7 +796,8 @@ static int sel_upload(fdmamr_ctx* c, const std::vector<float>& w) {
     HIPCHK(c, hipMemcpyAsync(c->sel_w.p, w.data(), P_TOTAL * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->sel_m.p, z.data(), P_TOTAL * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->sel_v.p, z.data(), P_TOTAL * 4, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->sel_cnt.p, z.data(), CNT_N * 4, hipMemcpyHostToDevice, c->stream));
+    const std::vector<int> zc(CNT_N * CNT_SLOTS, 0);
+    HIPCHK(c, hipMemcpyAsync(c->sel_cnt.p, zc.data(), zc.size() * 4, hipMemcpyHostToDevice, c->stream));
     launch_pack(c->sel_w.p, c->sel_pk.p, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -821,18 +857,18 @@ int32_t fdmamr_selector_infer(fdmamr_ctx* c, float* prob, int32_t* n_troubled) {
     const int N = c->tb.N;
     launch_shadow(c->Q[0].p, c->Qf[0].p, c->tb.L + c->tb.S_w + c->tb.S_r, c->stream);
     launch_slots_fast(c->dt, c->wls_f.p, c->Q[0].p, c->Qf[0].p, c->stream);
-    std::vector<int> cnt0(CNT_N, 0);
-    HIPCHK(c, hipMemcpyAsync(c->sel_cnt.p, cnt0.data(), CNT_N * 4, hipMemcpyHostToDevice, c->stream));
-    SelectParams sp{c->sel_threshold, 0u, c->sel_step, 8};  // no judge sample here
+    std::vector<int> cnt0(CNT_N * CNT_SLOTS, 0);
+    HIPCHK(c, hipMemcpyAsync(c->sel_cnt.p, cnt0.data(), cnt0.size() * 4, hipMemcpyHostToDevice, c->stream));
+    SelectParams sp{c->sel_threshold, 0u, c->sel_step};  // no judge sample here
     launch_select(c->dt, c->Qf[0].p, c->off16.p, c->sel_pk.p, c->sel_w.p, sp, c->flags.p, c->sel_prob.p,
-                  c->stream);
+                  c->tile_mask.p, c->tile_frag.p, 4 * c->n_cu, c->stream);
     // no judge sample was drawn: the judge pass only counts the troubled leaves
     launch_judge(N, c->label.p, c->sel_bound, c->flags.p, c->sel_cnt.p, c->stream);
     launch_build_lists(N, c->flags.p, c->list_cnt.p, c->lists.p, c->list_meta.p, c->stream);
     HIPCHK(c, hipGetLastError());
     if (prob && N > 0) HIPCHK(c, hipMemcpyAsync(prob, c->sel_prob.p, N * 4, hipMemcpyDeviceToHost, c->stream));
     int cnt[CNT_N];
-    HIPCHK(c, hipMemcpyAsync(cnt, c->sel_cnt.p, CNT_N * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, read_counters(c, cnt));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (n_troubled) *n_troubled = cnt[CNT_TROUBLED];
     c->use_flags = true;
@@ -894,7 +930,7 @@ int32_t fdmamr_selector_collect(fdmamr_ctx* c, int32_t* n_rows, int32_t* n_posit
     launch_shadow(c->Q[0].p, c->Qf[0].p, c->tb.L + c->tb.S_w + c->tb.S_r, c->stream);
     launch_slots_fast(c->dt, c->wls_f.p, c->Q[0].p, c->Qf[0].p, c->stream);
     launch_stage_fast(c->dt, nullptr, nullptr, c->Q[0].p, c->Qf[0].p, c->Q[0].p, c->Q[2].p, c->Qf[2].p,
-                      c->sc, 0.0, 1.0, 0, c->partials.p, c->label.p, c->stream);
+                      c->sc, 0.0, 1.0, 0, c->partials.p, nullptr, c->label.p, c->stream);
     launch_features(c->dt, c->Qf[0].p, c->off16.p, c->ds_x.p, c->ds_rows, c->stream);
     launch_labels_to_y(N, c->label.p, c->sel_bound, c->ds_y.p, c->ds_rows, c->stream);
     HIPCHK(c, hipGetLastError());
@@ -979,14 +1015,14 @@ int32_t fdmamr_selector_stats(fdmamr_ctx* c, int32_t* troubled, int32_t* judged,
                               int32_t reset) {
     if (!c || !c->sel_ready) return fail(c, -1, "selector has no weights");
     int cnt[CNT_N];
-    HIPCHK(c, hipMemcpyAsync(cnt, c->sel_cnt.p, CNT_N * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, read_counters(c, cnt));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (troubled) *troubled = cnt[CNT_TROUBLED];
     if (judged) *judged = cnt[CNT_JUDGED];
     if (rejected) *rejected = cnt[CNT_REJECTED];
     if (reset) {
-        std::vector<int> z(CNT_N, 0);
-        HIPCHK(c, hipMemcpyAsync(c->sel_cnt.p, z.data(), CNT_N * 4, hipMemcpyHostToDevice, c->stream));
+        std::vector<int> z(CNT_N * CNT_SLOTS, 0);
+        HIPCHK(c, hipMemcpyAsync(c->sel_cnt.p, z.data(), z.size() * 4, hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     return 0;

@@ -51,16 +51,18 @@ struct DevScalars {
 };
 
 void launch_slots(const DevTables& t, double4* Q, const double* T, hipStream_t st);
-void launch_stage(const DevTables& t, int precision, const double4* Qin, const double4* Qn,
-                  const double* Tn, double4* Qout, double* Tout, const DevScalars* sc,
-                  double alpha, double beta, int reduce, double* partials, hipStream_t st);
+// FP64 stage; with reduce, residual partials per block, and with lampart also the wave-speed
+// partials of the output state (the next step's lambda, 2 doubles per block)
+void launch_stage(const DevTables& t, const double4* Qin, const double4* Qn, const double* Tn,
+                  double4* Qout, double* Tout, const DevScalars* sc, double alpha, double beta,
+                  int reduce, double* partials, double* lampart, hipStream_t st);
 // ghost fill and restriction slots after a stage (restriction means one level per launch)
 void launch_restrict_fill(const DevTables& t, const std::vector<int32_t>& level_start, double4* Q,
                           double* T, float4* Qf, double4* Qi, double* Ti, hipStream_t st);
 void launch_wavespeed(const DevTables& t, const double4* Q, double* partials, hipStream_t st);
 void launch_step_begin(const double* partials, int nb, DevScalars* sc, hipStream_t st);
-void launch_step_end(const double* partials, int nb, int n_leaf, int advance_time,
-                     DevScalars* sc, hipStream_t st);
+void launch_step_end(const double* partials, int nb, int n_leaf, int advance_time, DevScalars* sc,
+                     const double* lampart, int lam_nb, hipStream_t st);
 void launch_sensors(const DevTables& t, const double4* Q, double* rot, double* div, hipStream_t st);
 void launch_positivity(const DevTables& t, const double4* Q, int* nbad, hipStream_t st);
 void launch_stencil_values(const DevTables& t, const double4* Q, int i, double* out, hipStream_t st);
@@ -72,10 +74,15 @@ void launch_slots_fast(const DevTables& t, const float* wf, const double4* Q, fl
 void launch_stage_fast(const DevTables& t, const int32_t* lists, const int* meta, const double4* Qin,
                        const float4* Qf_in, const double4* Qn, double4* Qout, float4* Qf_out,
                        const DevScalars* sc, double alpha, double beta, int reduce, double* partials,
-                       float* label, hipStream_t st);
+                       double* lampart, float* label, hipStream_t st);
 int stage_fast_partial_blocks(int n, bool lists);
-// leaf lists from selector flags (bit 0: troubled); blk_cnt holds list_blocks(n) ints
+// leaf lists from selector flags (bit 0: troubled); blk_cnt holds list_blocks(n) ints, the
+// troubled count of each block of LIST_BLK leaves. launch_list_write is the second pass alone,
+// for counts written by another kernel.
+constexpr int LIST_BLK = 512;
 void launch_build_lists(int n, const uint8_t* flags, int* blk_cnt, int32_t* lists, int* meta, hipStream_t st);
+void launch_list_write(int n, const uint8_t* flags, const int* blk_cnt, int32_t* lists, int* meta,
+                       hipStream_t st);
 int list_blocks(int n);
 void launch_shadow(const double4* Q, float4* Qf, int n, hipStream_t st);
 

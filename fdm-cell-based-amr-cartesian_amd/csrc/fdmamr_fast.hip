@@ -200,7 +200,8 @@ __device__ __forceinline__ void stage_leaf(const DevTables& t, int i, const doub
                                            const float4* __restrict__ Qf_in, const double4* Qn,
                                            double4* Qout, float4* __restrict__ Qf_out,
                                            const DevScalars* __restrict__ sc, double alpha, double beta,
-                                           float* __restrict__ label, double* rmax, double* rsum) {
+                                           float* __restrict__ label, double* rmax, double* rsum,
+                                           bool lam, double& lx, double& ly) {
     const double gama = t.gamma;
     const double dt = sc->dt;
     int s[8];
@@ -271,6 +272,12 @@ __device__ __forceinline__ void stage_leaf(const DevTables& t, int i, const doub
     o.w = alpha * qn.w + beta * p;
     Qout[i] = o;
     Qf_out[i] = make_float4((float)o.x, (float)o.y, (float)o.z, (float)o.w);
+    if (lam) {
+        // the next step's wave speeds on this final state (solution.F90:186-193)
+        const double a = sqrt(gama * o.w / o.x);
+        lx = fabs(o.y) + a;
+        ly = fabs(o.z) + a;
+    }
 }
 
 // Stage over one part of the leaf list: lists[0, n_clear) holds the cleared leaves and
@@ -287,7 +294,8 @@ __global__ __launch_bounds__(256) void k_stage_fast(DevTables t, const int32_t* 
                                                     float4* __restrict__ Qf_out,
                                                     const DevScalars* __restrict__ sc, double alpha,
                                                     double beta, int reduce, double* __restrict__ partials,
-                                                    int poff, float* __restrict__ label) {
+                                                    double* __restrict__ lampart, int poff,
+                                                    float* __restrict__ label) {
     int n = t.N, base = 0;
     if (lists) {
         const int nc = meta[0];
@@ -301,12 +309,14 @@ __global__ __launch_bounds__(256) void k_stage_fast(DevTables t, const int32_t* 
     const int lb = active ? xcd_block_n(blockIdx.x, nact) : (int)blockIdx.x;
     const int p = lb * 256 + (int)threadIdx.x;
     double rmax[4] = {0, 0, 0, 0}, rsum[4] = {0, 0, 0, 0};
+    double lx = 0.0, ly = 0.0;
     if (active && p < n) {
         const int i = lists ? lists[base + p] : p;
-        stage_leaf<FULL, LABEL>(t, i, Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, label, rmax, rsum);
+        stage_leaf<FULL, LABEL>(t, i, Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, label, rmax, rsum,
+                                lampart != nullptr, lx, ly);
     }
     if (reduce) {
-        __shared__ double red[4][8];
+        __shared__ double red[4][10];
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -314,20 +324,31 @@ __global__ __launch_bounds__(256) void k_stage_fast(DevTables t, const int32_t* 
             const double b = wsum(rsum[k]);
             if (lane == 0) { red[wv][k] = a; red[wv][4 + k] = b; }
         }
+        if (lampart) {
+            lx = wmax(lx);
+            ly = wmax(ly);
+            if (lane == 0) { red[wv][8] = lx; red[wv][9] = ly; }
+        }
         __syncthreads();
+        const int blk = poff + lb;
         if (threadIdx.x < 8) {
             const int k = threadIdx.x;
             double acc = red[0][k];
             for (int w = 1; w < 4; w++) acc = k < 4 ? fmax(acc, red[w][k]) : acc + red[w][k];
-            partials[(size_t)(poff + lb) * 8 + k] = acc;
+            partials[(size_t)blk * 8 + k] = acc;
+        } else if (lampart && threadIdx.x < 10) {
+            const int k = threadIdx.x;
+            double acc = red[0][k];
+            for (int w = 1; w < 4; w++) acc = fmax(acc, red[w][k]);
+            lampart[(size_t)blk * 2 + (k - 8)] = acc;
         }
     }
 }
 
 // ---------------------------------------------------------------- leaf lists from flags
 // Deterministic compaction in two passes over blocks of LIST_BLK leaves: troubled counts per
-// block, then each block places its leaves after the counts of the blocks before it.
-constexpr int LIST_BLK = 1024;
+// block (this kernel, or the selector's inference kernel, which counts the same blocks), then
+// each block places its leaves after the counts of the blocks before it.
 
 __device__ __forceinline__ int block_sum256(int v, int* sh) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -364,11 +385,12 @@ __global__ __launch_bounds__(256) void k_list_write(int n, const uint8_t* __rest
     pre = block_sum256(pre, sh);
     tot = block_sum256(tot, sh);
     const int n_clear = n - tot;
-    // four consecutive leaves per thread; exclusive scan of the troubled counts over the block
-    const int i0 = blockIdx.x * LIST_BLK + 4 * threadIdx.x;
-    int tr[4], mine = 0;
+    // LPT consecutive leaves per thread; exclusive scan of the troubled counts over the block
+    constexpr int LPT = LIST_BLK / 256;
+    const int i0 = blockIdx.x * LIST_BLK + LPT * threadIdx.x;
+    int tr[LPT], mine = 0;
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < LPT; k++) {
         tr[k] = (i0 + k < n) ? (flags[i0 + k] & 1) : 0;
         mine += tr[k];
     }
@@ -383,9 +405,9 @@ __global__ __launch_bounds__(256) void k_list_write(int n, const uint8_t* __rest
     int woff = 0;
     for (int w = 0; w < wv; w++) woff += ws[w];
     int t_pos = pre + woff + inc - mine;                       // troubled before this thread
-    int c_pos = blockIdx.x * LIST_BLK - pre + 4 * threadIdx.x - (woff + inc - mine);
+    int c_pos = blockIdx.x * LIST_BLK - pre + LPT * threadIdx.x - (woff + inc - mine);
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < LPT; k++) {
         const int i = i0 + k;
         if (i < n) {
             if (tr[k]) lists[n_clear + t_pos++] = i;
@@ -403,7 +425,7 @@ __device__ __forceinline__ float isq_f(float k1, float k2, float k3, const float
 
 __global__ __launch_bounds__(128) void k_slots_fast(DevTables t, const float* __restrict__ wf,
                                                     float4* __restrict__ Qf) {
-    const int s = xcd_block() * blockDim.x + threadIdx.x;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= t.S_w) return;
     const float* __restrict__ f = wf + (size_t)s * WF_STRIDE;
     const int32_t* __restrict__ iv = t.wls_i + (size_t)s * WLSI_STRIDE;
@@ -419,29 +441,33 @@ __global__ __launch_bounds__(128) void k_slots_fast(DevTables t, const float* __
         float phi[9];
 #pragma unroll
         for (int i = 0; i < 9; i++) phi[i] = k == 0 ? val[i].x : k == 1 ? val[i].y : k == 2 ? val[i].z : val[i].w;
+        // coefficients 1..5 of the quadratic fit (the constant term is not needed)
         float al[6], vl = 0.f;
+        al[0] = 0.f;
 #pragma unroll
-        for (int r = 0; r < 6; r++) {
+        for (int r = 1; r < 6; r++) {
             float a = 0.f;
 #pragma unroll
-            for (int i = 0; i < 9; i++) a += f[WF_C6 + r * 9 + i] * phi[i];
+            for (int i = 0; i < 9; i++) a += f[WF_C6 + (r - 1) * 9 + i] * phi[i];
             al[r] = a;
         }
 #pragma unroll
         for (int i = 0; i < 9; i++) vl += f[WF_CL + i] * phi[i];
+        // slopes of the two linear fits
         float as1[3], as2[3], vs1 = 0.f, vs2 = 0.f;
+        as1[0] = as2[0] = 0.f;
 #pragma unroll
-        for (int r = 0; r < 3; r++) {
+        for (int r = 1; r < 3; r++) {
             float a = 0.f, b = 0.f;
 #pragma unroll
-            for (int j = 0; j < 5; j++) a += f[WF_C3A + r * 5 + j] * (j < m1 ? phi[j] : 0.f);
+            for (int j = 0; j < 5; j++) a += f[WF_C3A + (r - 1) * 5 + j] * (j < m1 ? phi[j] : 0.f);
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 const int ii = m1 + j;
                 float ph = 0.f;
 #pragma unroll
                 for (int q = 0; q < 9; q++) ph = q == ii ? phi[q] : ph;
-                b += f[WF_C3B + r * 4 + j] * (ii < m ? ph : 0.f);
+                b += f[WF_C3B + (r - 1) * 4 + j] * (ii < m ? ph : 0.f);
             }
             as1[r] = a; as2[r] = b;
         }
@@ -490,26 +516,34 @@ void launch_slots_fast(const DevTables& t, const float* wf, const double4* Q, fl
 void launch_stage_fast(const DevTables& t, const int32_t* lists, const int* meta, const double4* Qin,
                        const float4* Qf_in, const double4* Qn, double4* Qout, float4* Qf_out,
                        const DevScalars* sc, double alpha, double beta, int reduce, double* partials,
-                       float* label, hipStream_t st) {
+                       double* lampart, float* label, hipStream_t st) {
     if (t.N <= 0) return;
     const int nb = nb_(t.N, 256);
     if (lists)
         hipLaunchKernelGGL((k_stage_fast<false, false>), dim3(nb), dim3(256), 0, st, t, lists, meta, Qin, Qf_in,
-                           Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials, 0, label);
+                           Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials, lampart, 0, label);
     if (label)
         hipLaunchKernelGGL((k_stage_fast<true, true>), dim3(nb), dim3(256), 0, st, t, lists, meta, Qin, Qf_in,
-                           Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials, lists ? nb : 0, label);
+                           Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials, lampart, lists ? nb : 0,
+                           label);
     else
         hipLaunchKernelGGL((k_stage_fast<true, false>), dim3(nb), dim3(256), 0, st, t, lists, meta, Qin, Qf_in,
-                           Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials, lists ? nb : 0, label);
+                           Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials, lampart, lists ? nb : 0,
+                           label);
 }
 
 int stage_fast_partial_blocks(int n, bool lists) { return (lists ? 2 : 1) * nb_(n, 256); }
 
 void launch_build_lists(int n, const uint8_t* flags, int* blk_cnt, int32_t* lists, int* meta, hipStream_t st) {
     if (n <= 0) return;
+    hipLaunchKernelGGL(k_list_count, dim3(nb_(n, LIST_BLK)), dim3(256), 0, st, n, flags, blk_cnt);
+    launch_list_write(n, flags, blk_cnt, lists, meta, st);
+}
+
+void launch_list_write(int n, const uint8_t* flags, const int* blk_cnt, int32_t* lists, int* meta,
+                       hipStream_t st) {
+    if (n <= 0) return;
     const int nblk = nb_(n, LIST_BLK);
-    hipLaunchKernelGGL(k_list_count, dim3(nblk), dim3(256), 0, st, n, flags, blk_cnt);
     hipLaunchKernelGGL(k_list_write, dim3(nblk), dim3(256), 0, st, n, flags, blk_cnt, nblk, lists, meta);
 }
 

@@ -200,6 +200,39 @@ __device__ __forceinline__ void flux_derivative(const double4* q, double d, S la
     }
 }
 
+// ---------------------------------------------------------------- stage reductions
+// Per-block residual partials (max |r|, sum r^2 of the four equations) and, with lampart, the
+// block's largest |u|+c and |v|+c (256 threads).
+__device__ __forceinline__ void block_partials(const double* rmax, const double* rsum,
+                                               double* __restrict__ partials, int blk,
+                                               double* __restrict__ lampart, double lx, double ly) {
+    __shared__ double red[4][10];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double a = wave_max(rmax[k]);
+        const double b = wave_sum(rsum[k]);
+        if (lane == 0) { red[wv][k] = a; red[wv][4 + k] = b; }
+    }
+    if (lampart) {
+        lx = wave_max(lx);
+        ly = wave_max(ly);
+        if (lane == 0) { red[wv][8] = lx; red[wv][9] = ly; }
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        const int k = threadIdx.x;
+        double acc = red[0][k];
+        for (int w = 1; w < 4; w++) acc = k < 4 ? fmax(acc, red[w][k]) : acc + red[w][k];
+        partials[(size_t)blk * 8 + k] = acc;
+    } else if (lampart && threadIdx.x < 10) {
+        const int k = threadIdx.x;
+        double acc = red[0][k];
+        for (int w = 1; w < 4; w++) acc = fmax(acc, red[w][k]);
+        lampart[(size_t)blk * 2 + (k - 8)] = acc;
+    }
+}
+
 // ---------------------------------------------------------------- stage kernel
 // Qn/Tn alias Qout/Tout in stage 3 (in-place final stage), so they carry no __restrict__.
 template <typename S>
@@ -207,10 +240,12 @@ __global__ __launch_bounds__(256) void k_stage(DevTables t, const double4* __res
                                                const double4* Qn, const double* Tn,
                                                double4* Qout, double* Tout,
                                                const DevScalars* __restrict__ sc, double alpha,
-                                               double beta, int reduce, double* __restrict__ partials) {
+                                               double beta, int reduce, double* __restrict__ partials,
+                                               double* __restrict__ lampart) {
     const int lb = xcd_block();
     const int i = lb * blockDim.x + threadIdx.x;
     double rmax[4] = {0, 0, 0, 0}, rsum[4] = {0, 0, 0, 0};
+    double lx = 0.0, ly = 0.0;
     if (i < t.N) {
         const double gama = t.gamma, cv = t.cv;
         const double dt = sc->dt;
@@ -259,25 +294,14 @@ __global__ __launch_bounds__(256) void k_stage(DevTables t, const double4* __res
         o.w = alpha * qn.w + beta * p;
         Qout[i] = o;
         Tout[i] = alpha * Tn[i] + beta * T;
-    }
-    if (reduce) {
-        __shared__ double red[4][8];
-        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            double a = wave_max(rmax[k]);
-            double b = wave_sum(rsum[k]);
-            if (lane == 0) { red[wv][k] = a; red[wv][4 + k] = b; }
-        }
-        __syncthreads();
-        if (threadIdx.x < 8) {
-            const int k = threadIdx.x;
-            const int nw = blockDim.x >> 6;
-            double acc = red[0][k];
-            for (int w = 1; w < nw; w++) acc = k < 4 ? fmax(acc, red[w][k]) : acc + red[w][k];
-            partials[(size_t)lb * 8 + k] = acc;
+        if (lampart) {
+            // the next step's wave speeds on this final state (solution.F90:186-193)
+            const double a = sqrt(t.gamma * o.w / o.x);
+            lx = fabs(o.y) + a;
+            ly = fabs(o.z) + a;
         }
     }
+    if (reduce) block_partials(rmax, rsum, partials, lb, lampart, lx, ly);
 }
 
 // ---------------------------------------------------------------- WLS-WENO slots
@@ -352,6 +376,44 @@ __global__ __launch_bounds__(128) void k_slots_wls(DevTables t, double4* __restr
     Q[t.L + s] = make_double4(out[0], out[1], out[2], out[3]);
 }
 
+// ---------------------------------------------------------------- ghost fill + restriction slots
+// items [0, G): ghost leaves from their partner (BC_cyc, solution.F90:109-168) -- a leaf copy,
+// a restriction mean, or the fixed inflow state (new kind); items [G, G + S_r): restriction
+// slot values. Optional FP32 shadow for the fast path.
+__device__ __forceinline__ void fill_item(const DevTables& t, int item, double4* __restrict__ Q,
+                                          double* __restrict__ T, const double4* __restrict__ Qi,
+                                          const double* __restrict__ Ti, float4* __restrict__ Qf) {
+    if (item < t.G) {
+        const int r = t.bc_ref[item];
+        double4 q;
+        double tv = 0.0;
+        if (r == REF_INFLOW) {
+            q = make_double4(t.inflow[0], t.inflow[1], t.inflow[2], t.inflow[3]);
+            tv = (t.inflow[3] * t.gamma * t.Ma * t.Ma) / t.inflow[0];
+        } else if (r >= 0) {
+            q = Q[r];
+            if (T) tv = T[r];
+        } else {
+            q = Qi[-(r + 1)];
+            if (T) tv = Ti[-(r + 1)];
+        }
+        Q[t.N + item] = q;
+        if (T) T[t.N + item] = tv;
+        if (Qf) Qf[t.N + item] = make_float4((float)q.x, (float)q.y, (float)q.z, (float)q.w);
+    } else if (item < t.G + t.S_r) {
+        const int s = item - t.G;
+        const double4 q = Qi[t.rst_ref[s]];
+        Q[t.L + t.S_w + s] = q;
+        if (Qf) Qf[t.L + t.S_w + s] = make_float4((float)q.x, (float)q.y, (float)q.z, (float)q.w);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill(DevTables t, double4* __restrict__ Q, double* __restrict__ T,
+                                              const double4* __restrict__ Qi, const double* __restrict__ Ti,
+                                              float4* __restrict__ Qf) {
+    fill_item(t, blockIdx.x * blockDim.x + threadIdx.x, Q, T, Qi, Ti, Qf);
+}
+
 // ---------------------------------------------------------------- restriction means
 // interpA (XuanGua.F90:359-398) for the internal nodes the ghosts and restriction slots need,
 // one tree level per launch, deepest first, keeping the reference's ((s1+s2)+s3)+s4 order.
@@ -378,13 +440,14 @@ __global__ __launch_bounds__(256) void k_restrict_level(DevTables t, const doubl
     if (T) Ti[n] = 0.25 * ta;
 }
 
-// All levels in one launch of one block (few internal nodes): levels separated by barriers.
+// All levels in one launch of one block (few internal nodes): levels separated by barriers;
+// with `fill` set the same block then fills the ghosts and restriction slots.
 struct LevelGroups { int n; int start[MAX_TREE_DEPTH + 1]; };
 
-__global__ __launch_bounds__(1024) void k_restrict_all(DevTables t, const double4* __restrict__ Q,
-                                                       const double* __restrict__ T,
+__global__ __launch_bounds__(1024) void k_restrict_all(DevTables t, double4* __restrict__ Q,
+                                                       double* __restrict__ T,
                                                        double4* __restrict__ Qi, double* __restrict__ Ti,
-                                                       LevelGroups lg) {
+                                                       LevelGroups lg, int fill, float4* __restrict__ Qf) {
     for (int g = 0; g < lg.n; g++) {
         for (int n = lg.start[g] + threadIdx.x; n < lg.start[g + 1]; n += blockDim.x) {
             double4 acc = make_double4(0, 0, 0, 0);
@@ -404,40 +467,10 @@ __global__ __launch_bounds__(1024) void k_restrict_all(DevTables t, const double
         }
         __syncthreads();  // the next (shallower) level reads this one
     }
+    if (fill)
+        for (int item = threadIdx.x; item < t.G + t.S_r; item += blockDim.x) fill_item(t, item, Q, T, Qi, Ti, Qf);
 }
 
-// ---------------------------------------------------------------- ghost fill + restriction slots
-// items [0, G): ghost leaves from their partner (BC_cyc, solution.F90:109-168) -- a leaf copy,
-// a restriction mean, or the fixed inflow state (new kind); items [G, G + S_r): restriction
-// slot values. Optional FP32 shadow for the fast path.
-__global__ __launch_bounds__(256) void k_fill(DevTables t, double4* __restrict__ Q, double* __restrict__ T,
-                                              const double4* __restrict__ Qi, const double* __restrict__ Ti,
-                                              float4* __restrict__ Qf) {
-    const int item = blockIdx.x * blockDim.x + threadIdx.x;
-    if (item < t.G) {
-        const int r = t.bc_ref[item];
-        double4 q;
-        double tv = 0.0;
-        if (r == REF_INFLOW) {
-            q = make_double4(t.inflow[0], t.inflow[1], t.inflow[2], t.inflow[3]);
-            tv = (t.inflow[3] * t.gamma * t.Ma * t.Ma) / t.inflow[0];
-        } else if (r >= 0) {
-            q = Q[r];
-            if (T) tv = T[r];
-        } else {
-            q = Qi[-(r + 1)];
-            if (T) tv = Ti[-(r + 1)];
-        }
-        Q[t.N + item] = q;
-        if (T) T[t.N + item] = tv;
-        if (Qf) Qf[t.N + item] = make_float4((float)q.x, (float)q.y, (float)q.z, (float)q.w);
-    } else if (item < t.G + t.S_r) {
-        const int s = item - t.G;
-        const double4 q = Qi[t.rst_ref[s]];
-        Q[t.L + t.S_w + s] = q;
-        if (Qf) Qf[t.L + t.S_w + s] = make_float4((float)q.x, (float)q.y, (float)q.z, (float)q.w);
-    }
-}
 
 // ---------------------------------------------------------------- lambda / scalars
 __global__ __launch_bounds__(256) void k_wavespeed(DevTables t, const double4* __restrict__ Q,
@@ -464,10 +497,11 @@ __global__ __launch_bounds__(256) void k_wavespeed(DevTables t, const double4* _
     }
 }
 
-// reduce lambda partials; running max; dt from CFL (solution.F90:185-201)
-__global__ __launch_bounds__(256) void k_step_begin(const double* __restrict__ partials, int nblk,
-                                                    DevScalars* sc) {
-    __shared__ double sx[256], sy[256];
+// reduce lambda partials; running max; dt from CFL (solution.F90:185-201); STEP_THREADS threads
+constexpr int STEP_THREADS = 1024;
+
+__device__ void step_begin_block(const double* __restrict__ partials, int nblk, DevScalars* sc) {
+    __shared__ double sx[STEP_THREADS], sy[STEP_THREADS];
     double ax = 0.0, ay = 0.0;
     for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
         ax = fmax(ax, partials[2 * b]);
@@ -497,10 +531,17 @@ __global__ __launch_bounds__(256) void k_step_begin(const double* __restrict__ p
     }
 }
 
-// residual finalize; tt advance; NaN status (solution.F90:205-214)
-__global__ __launch_bounds__(256) void k_step_end(const double* __restrict__ partials, int nblk,
-                                                  int n_leaf, int advance_time, DevScalars* sc) {
-    __shared__ double sh[8][256];
+__global__ __launch_bounds__(STEP_THREADS) void k_step_begin(const double* __restrict__ partials, int nblk,
+                                                             DevScalars* sc) {
+    step_begin_block(partials, nblk, sc);
+}
+
+// residual finalize; tt advance; NaN status (solution.F90:205-214); with lampart, the next
+// step's begin (its wave speeds were reduced per block by the final stage)
+__global__ __launch_bounds__(STEP_THREADS) void k_step_end(const double* __restrict__ partials, int nblk,
+                                                           int n_leaf, int advance_time, DevScalars* sc,
+                                                           const double* __restrict__ lampart, int lam_nblk) {
+    __shared__ double sh[8][STEP_THREADS];
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int b = threadIdx.x; b < nblk; b += blockDim.x)
         for (int k = 0; k < 8; k++) acc[k] = k < 4 ? fmax(acc[k], partials[8 * b + k]) : acc[k] + partials[8 * b + k];
@@ -524,6 +565,10 @@ __global__ __launch_bounds__(256) void k_step_end(const double* __restrict__ par
                 if (r != r) sc->status = -3;
             }
         }
+    }
+    if (lampart) {
+        __syncthreads();
+        step_begin_block(lampart, lam_nblk, sc);
     }
 }
 
@@ -575,12 +620,16 @@ void launch_restrict_fill(const DevTables& t, const std::vector<int32_t>& level_
                           double* T, float4* Qf, double4* Qi, double* Ti, hipStream_t st) {
     const int ngroups = (int)level_start.size() - 1;
     const int n_int = ngroups > 0 ? level_start.back() : 0;
+    const int items = t.G + t.S_r;
     if (n_int > 0 && n_int <= 65536 && ngroups <= MAX_TREE_DEPTH) {
-        // few internal nodes: one launch, one block
+        // few internal nodes: one launch, one block (which also fills when the fill is small:
+        // one block is slower than a separate wide launch beyond a few thousand items)
         LevelGroups lg;
         lg.n = ngroups;
         for (int g = 0; g <= ngroups; g++) lg.start[g] = level_start[g];
-        hipLaunchKernelGGL(k_restrict_all, dim3(1), dim3(1024), 0, st, t, Q, T, Qi, Ti, lg);
+        const int fill = items <= 2048 ? 1 : 0;
+        hipLaunchKernelGGL(k_restrict_all, dim3(1), dim3(1024), 0, st, t, Q, T, Qi, Ti, lg, fill, Qf);
+        if (fill) return;
     } else {
         for (int g = 0; g < ngroups; g++) {
             const int b = level_start[g], e = level_start[g + 1];
@@ -588,20 +637,15 @@ void launch_restrict_fill(const DevTables& t, const std::vector<int32_t>& level_
                 hipLaunchKernelGGL(k_restrict_level, dim3(nblk(e - b, 256)), dim3(256), 0, st, t, Q, T, Qi, Ti, b, e);
         }
     }
-    const int items = t.G + t.S_r;
     if (items > 0) hipLaunchKernelGGL(k_fill, dim3(nblk(items, 256)), dim3(256), 0, st, t, Q, T, Qi, Ti, Qf);
 }
 
-void launch_stage(const DevTables& t, int precision, const double4* Qin, const double4* Qn,
-                  const double* Tn, double4* Qout, double* Tout, const DevScalars* sc,
-                  double alpha, double beta, int reduce, double* partials, hipStream_t st) {
+void launch_stage(const DevTables& t, const double4* Qin, const double4* Qn, const double* Tn,
+                  double4* Qout, double* Tout, const DevScalars* sc, double alpha, double beta,
+                  int reduce, double* partials, double* lampart, hipStream_t st) {
     if (t.N <= 0) return;
-    if (precision == 0)
-        hipLaunchKernelGGL(k_stage<double>, dim3(nblk(t.N, 256)), dim3(256), 0, st, t, Qin, Qn,
-                           Tn, Qout, Tout, sc, alpha, beta, reduce, partials);
-    else
-        hipLaunchKernelGGL(k_stage<float>, dim3(nblk(t.N, 256)), dim3(256), 0, st, t, Qin, Qn, Tn,
-                           Qout, Tout, sc, alpha, beta, reduce, partials);
+    hipLaunchKernelGGL(k_stage<double>, dim3(nblk(t.N, 256)), dim3(256), 0, st, t, Qin, Qn, Tn, Qout, Tout,
+                       sc, alpha, beta, reduce, partials, lampart);
 }
 
 void launch_wavespeed(const DevTables& t, const double4* Q, double* partials, hipStream_t st) {
@@ -609,12 +653,13 @@ void launch_wavespeed(const DevTables& t, const double4* Q, double* partials, hi
 }
 
 void launch_step_begin(const double* partials, int nb, DevScalars* sc, hipStream_t st) {
-    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(256), 0, st, partials, nb, sc);
+    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(STEP_THREADS), 0, st, partials, nb, sc);
 }
 
-void launch_step_end(const double* partials, int nb, int n_leaf, int advance_time,
-                     DevScalars* sc, hipStream_t st) {
-    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(256), 0, st, partials, nb, n_leaf, advance_time, sc);
+void launch_step_end(const double* partials, int nb, int n_leaf, int advance_time, DevScalars* sc,
+                     const double* lampart, int lam_nb, hipStream_t st) {
+    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(STEP_THREADS), 0, st, partials, nb, n_leaf, advance_time, sc,
+                       lampart, lam_nb);
 }
 
 void launch_sensors(const DevTables& t, const double4* Q, double* rot, double* div, hipStream_t st) {

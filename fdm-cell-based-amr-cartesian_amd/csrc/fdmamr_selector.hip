@@ -86,7 +86,11 @@ __device__ __forceinline__ void gather_idx(const DevTables& t, const int32_t* __
 // maxima by shuffles. Channel f of point p becomes (v - c) / (m + 1e-3 |c|), c the centre
 // value and m the larger of |c| and the largest |v - c| on the stencil, so every feature lies
 // in [-1, 1] and smooth data maps near 0. v[k][f] holds point g + 4k (0 where p > 24).
-__device__ __forceinline__ void gather_values(const float4* __restrict__ Qf, int leaf, bool valid,
+// Returns true when the leaf's stencil is near-uniform: in every channel the largest |v - c|
+// is at most UNIFORM_REL |c| (exactly uniform data included).
+constexpr float UNIFORM_REL = 1e-6f;
+
+__device__ __forceinline__ bool gather_values(const float4* __restrict__ Qf, int leaf, bool valid,
                                               const int idx[7], float v[7][3]) {
     const int g = (threadIdx.x & 63) >> 4;
     const float4 qc = Qf[valid ? leaf : 0];
@@ -104,17 +108,20 @@ __device__ __forceinline__ void gather_values(const float4* __restrict__ Qf, int
         v[k][1] = in ? q[k].w : c[1];
         v[k][2] = in ? __builtin_amdgcn_sqrtf(q[k].y * q[k].y + q[k].z * q[k].z) : c[2];
     }
+    bool uniform = true;
 #pragma unroll
     for (int f = 0; f < 3; f++) {
-        float mx = fabsf(c[f]);
+        float md = 0.f;
 #pragma unroll
-        for (int k = 0; k < 7; k++) mx = fmaxf(mx, fabsf(v[k][f] - c[f]));
-        mx = fmaxf(mx, __shfl_xor(mx, 16));
-        mx = fmaxf(mx, __shfl_xor(mx, 32));
-        const float sc = __builtin_amdgcn_rcpf(mx + 1e-3f * fabsf(c[f]) + 1e-30f);
+        for (int k = 0; k < 7; k++) md = fmaxf(md, fabsf(v[k][f] - c[f]));
+        md = fmaxf(md, __shfl_xor(md, 16));
+        md = fmaxf(md, __shfl_xor(md, 32));
+        uniform = uniform && md <= UNIFORM_REL * fabsf(c[f]);
+        const float sc = __builtin_amdgcn_rcpf(fmaxf(md, fabsf(c[f])) + 1e-3f * fabsf(c[f]) + 1e-30f);
 #pragma unroll
         for (int k = 0; k < 7; k++) v[k][f] = g + 4 * k < 25 ? (v[k][f] - c[f]) * sc : 0.f;
     }
+    return uniform;
 }
 
 __device__ __forceinline__ void gather_features(const DevTables& t, const float4* __restrict__ Qf,
@@ -122,7 +129,7 @@ __device__ __forceinline__ void gather_features(const DevTables& t, const float4
                                                 float v[7][3]) {
     int idx[7];
     gather_idx(t, off16, leaf, valid, idx);
-    gather_values(Qf, leaf, valid, idx, v);
+    (void)gather_values(Qf, leaf, valid, idx, v);
 }
 
 // Writes the 80 BF16 features of the lane's leaf (75 used, feature p*3 + f) to xrow.
@@ -267,16 +274,69 @@ __device__ __forceinline__ bf16x8 relu_pack(const f32x4& lo, const f32x4& hi) {
 // flags bit 0: troubled (full path); bit 1: judged this step (cleared by the network but
 // sent through the full path with a label); bit 2: forced troubled by an earlier judge
 // rejection (sticky until the tables are rebuilt).
+__device__ __forceinline__ uint8_t decide(uint8_t old, bool troubled, int leaf, const SelectParams& sp) {
+    uint8_t f = old & 4;
+    if (troubled || f) f |= 1;
+    else if ((hash32((uint32_t)leaf * 2654435761u ^ sp.step) & 0xFFFFu) < sp.judge_q16) f |= 3;
+    return f;
+}
+
+// Inference in two passes.
 //
-// Inference keeps every activation in registers: the feature gather of lane (r, g) yields
-// exactly the 24 K-slots of the layer-1 B fragments of leaf r (feature q = 8s + j of the lane
-// is channel q % 3 of its point g + 4 (q / 3)), and each layer's accumulators are the next
-// layer's B fragments. Weights are read from LDS as lane-contiguous fragments.
-__global__ __launch_bounds__(256) void k_select(DevTables t, const float4* __restrict__ Qf,
-                                                const int32_t* __restrict__ off16,
-                                                const uint16_t* __restrict__ wpk,
-                                                const float* __restrict__ w32, SelectParams sp,
-                                                uint8_t* __restrict__ flags, float* __restrict__ prob) {
+// Pass 1 (k_select_gate), one 16-leaf tile per wave at full occupancy: gathers the 5x5
+// stencils, clears near-uniform leaves without the network (their flags and probability are
+// final here), and for a tile with any other leaf stores its layer-1 B fragments (the feature
+// gather of lane (r, g) yields exactly the 24 K-slots of leaf r: feature q = 8s + j of the lane
+// is channel q % 3 of its point g + 4 (q / 3)) and the mask of leaves that need the network.
+//
+// Pass 2 (k_select_mlp), a persistent grid with the weights in LDS as lane-contiguous
+// fragments: runs the network on the stored tiles (tiles are dealt round-robin, which spreads
+// the spatially clustered non-uniform tiles evenly); every activation stays in registers,
+// each layer's accumulators being the next layer's B fragments.
+__global__ __launch_bounds__(256) void k_select_gate(DevTables t, const float4* __restrict__ Qf,
+                                                     const int32_t* __restrict__ off16, SelectParams sp,
+                                                     uint8_t* __restrict__ flags, float* __restrict__ prob,
+                                                     uint16_t* __restrict__ tile_mask,
+                                                     uint4* __restrict__ tile_frag) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
+    const int tile = xcd_block() * 4 + wv;
+    const int ntiles = (t.N + 15) / 16;
+    if (tile >= ntiles) return;
+    const int leaf = tile * 16 + r;
+    const bool valid = leaf < t.N;
+    int idx[7];
+    gather_idx(t, off16, leaf, valid, idx);
+    const uint8_t old = valid ? flags[leaf] : 0;
+    float v[7][3];
+    const bool uniform = gather_values(Qf, leaf, valid, idx, v);
+    // mask of the tile's leaves that need the network (bit r), the same in every lane
+    const uint64_t need = __ballot(valid && !uniform);
+    const uint16_t mask = (uint16_t)(need & 0xFFFFu);
+    if (lane == 0) tile_mask[tile] = mask;
+    if (valid && g == 0 && uniform) {
+        if (prob) prob[leaf] = 0.f;
+        flags[leaf] = decide(old, false, leaf, sp);
+    }
+    if (mask) {
+        uint4* dst = tile_frag + (size_t)tile * 3 * 64;
+#pragma unroll
+        for (int s3 = 0; s3 < 3; s3++) {
+            bf16x8 x;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int q = 8 * s3 + j;
+                x[j] = (__bf16)(q < 21 ? v[q / 3][q % 3] : 0.f);
+            }
+            dst[s3 * 64 + lane] = *reinterpret_cast<const uint4*>(&x);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_select_mlp(int n_leaf, const uint16_t* __restrict__ wpk,
+                                                    const float* __restrict__ w32, SelectParams sp,
+                                                    const uint16_t* __restrict__ tile_mask,
+                                                    const uint4* __restrict__ tile_frag,
+                                                    uint8_t* __restrict__ flags, float* __restrict__ prob) {
     __shared__ __align__(16) uint16_t sF[KF_TOTAL];
     __shared__ __align__(16) float sB[SB_TOTAL];
     {
@@ -293,28 +353,21 @@ __global__ __launch_bounds__(256) void k_select(DevTables t, const float4* __res
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
-    const int ntiles = (t.N + 15) / 16;
-    const int lb = xcd_block();
-    // the stencil indices of the next tile are loaded while this tile's network runs
-    int idx[7];
-    {
-        const int leaf0 = (lb * sp.tiles_per_wave * 4 + wv) * 16 + r;
-        gather_idx(t, off16, leaf0, leaf0 < t.N, idx);
-    }
-    for (int it = 0; it < sp.tiles_per_wave; it++) {
-        const int tile = (lb * sp.tiles_per_wave + it) * 4 + wv;
-        if (tile >= ntiles) break;
-        const int leaf = tile * 16 + r;
-        const bool valid = leaf < t.N;
-        float v[7][3];
-        gather_values(Qf, leaf, valid, idx, v);
-        if (it + 1 < sp.tiles_per_wave) {
-            const int nleaf = leaf + 64;
-            gather_idx(t, off16, nleaf, nleaf < t.N, idx);
-        }
+    const int ntiles = (n_leaf + 15) / 16;
+    const int nwaves = gridDim.x * 4;
+    for (int tile = blockIdx.x * 4 + wv; tile < ntiles; tile += nwaves) {
+        const uint16_t mask = tile_mask[tile];
+        if (!mask) continue;
+        const uint4* src = tile_frag + (size_t)tile * 3 * 64;
         bf16x8 xb[3];
 #pragma unroll
-        for (int q = 0; q < 24; q++) xb[q >> 3][q & 7] = (__bf16)(q < 21 ? v[q / 3][q % 3] : 0.f);
+        for (int s3 = 0; s3 < 3; s3++) {
+            const uint4 u = src[s3 * 64 + lane];
+            xb[s3] = *reinterpret_cast<const bf16x8*>(&u);
+        }
+        const int leaf = tile * 16 + r;
+        const bool mine = (mask >> r) & 1;
+        const uint8_t old = mine && g == 0 ? flags[leaf] : 0;
         f32x4 z[4];
 #pragma unroll
         for (int mt = 0; mt < 4; mt++) {
@@ -344,26 +397,22 @@ __global__ __launch_bounds__(256) void k_select(DevTables t, const float4* __res
         }
         part += __shfl_xor(part, 16);
         part += __shfl_xor(part, 32);
-        const float z4 = part + sB[SB_B4];
-        if (valid && g == 0) {
-            const float pr = 1.f / (1.f + __expf(-z4));
+        if (mine && g == 0) {
+            const float pr = 1.f / (1.f + __expf(-(part + sB[SB_B4])));
             if (prob) prob[leaf] = pr;
-            uint8_t f = flags[leaf] & 4;
-            if (pr >= sp.threshold || f) f |= 1;
-            else if ((hash32((uint32_t)leaf * 2654435761u ^ sp.step) & 0xFFFFu) < sp.judge_q16) f |= 3;
-            flags[leaf] = f;
+            flags[leaf] = decide(old, pr >= sp.threshold, leaf, sp);
         }
     }
 }
 
 // judge: a judged leaf whose label exceeds the bound is forced back to the full path; also
-// counts the leaves that took the full path (bit 0). Each block scans 4096 leaves (16 per
-// thread) and adds its counts with one atomic per counter, a few hundred atomics per step.
+// counts the leaves that took the full path (bit 0). Each block scans 1024 leaves (4 per
+// thread) and adds its counts with one atomic per counter.
 __global__ __launch_bounds__(256) void k_judge(int n, const float* __restrict__ label, float bound,
                                                uint8_t* __restrict__ flags, int* __restrict__ counters) {
     int judged = 0, rejected = 0, troubled = 0;
-    const int base = blockIdx.x * 4096;
-    for (int k = threadIdx.x; k < 4096; k += 256) {
+    const int base = blockIdx.x * 1024;
+    for (int k = threadIdx.x; k < 1024; k += 256) {
         const int i = base + k;
         if (i >= n) break;
         const uint8_t f = flags[i];
@@ -386,9 +435,10 @@ __global__ __launch_bounds__(256) void k_judge(int n, const float* __restrict__ 
     if (threadIdx.x == 0) {
         const int j = sj[0] + sj[1] + sj[2] + sj[3], r = sr[0] + sr[1] + sr[2] + sr[3];
         const int tr = st[0] + st[1] + st[2] + st[3];
-        if (j) atomicAdd(&counters[CNT_JUDGED], j);
-        if (r) atomicAdd(&counters[CNT_REJECTED], r);
-        if (tr) atomicAdd(&counters[CNT_TROUBLED], tr);
+        int* slot = counters + (blockIdx.x % CNT_SLOTS) * CNT_N;
+        if (j) atomicAdd(&slot[CNT_JUDGED], j);
+        if (r) atomicAdd(&slot[CNT_REJECTED], r);
+        if (tr) atomicAdd(&slot[CNT_TROUBLED], tr);
     }
 }
 
@@ -607,16 +657,18 @@ __global__ void k_pack(const float* __restrict__ w, uint16_t* __restrict__ pk) {
 
 // ---------------------------------------------------------------- launchers
 void launch_select(const DevTables& t, const float4* Qf, const int32_t* off16, const uint16_t* wpk,
-                   const float* w32, const SelectParams& sp, uint8_t* flags, float* prob, hipStream_t st) {
+                   const float* w32, const SelectParams& sp, uint8_t* flags, float* prob, uint16_t* tile_mask,
+                   uint4* tile_frag, int mlp_blocks, hipStream_t st) {
     if (t.N <= 0) return;
     const int ntiles = (t.N + 15) / 16;
-    const int per_block = 4 * sp.tiles_per_wave;
-    const int nb = (ntiles + per_block - 1) / per_block;
-    hipLaunchKernelGGL(k_select, dim3(nb), dim3(256), 0, st, t, Qf, off16, wpk, w32, sp, flags, prob);
+    hipLaunchKernelGGL(k_select_gate, dim3((ntiles + 3) / 4), dim3(256), 0, st, t, Qf, off16, sp, flags, prob,
+                       tile_mask, tile_frag);
+    hipLaunchKernelGGL(k_select_mlp, dim3(mlp_blocks), dim3(256), 0, st, t.N, wpk, w32, sp, tile_mask, tile_frag,
+                       flags, prob);
 }
 
 void launch_judge(int n, const float* label, float bound, uint8_t* flags, int* counters, hipStream_t st) {
-    if (n > 0) hipLaunchKernelGGL(k_judge, dim3((n + 4095) / 4096), dim3(256), 0, st, n, label, bound, flags, counters);
+    if (n > 0) hipLaunchKernelGGL(k_judge, dim3((n + 1023) / 1024), dim3(256), 0, st, n, label, bound, flags, counters);
 }
 
 void launch_features(const DevTables& t, const float4* Qf, const int32_t* off16, uint16_t* X, int row0,

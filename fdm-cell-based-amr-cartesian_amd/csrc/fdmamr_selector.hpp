@@ -45,17 +45,22 @@ constexpr int KW_TOTAL = KW_FRAG + KF_TOTAL;
 constexpr int SB_B1 = 0, SB_B2 = 64, SB_B3 = 128, SB_W4 = 192, SB_B4 = 256, SB_TOTAL = 260;
 
 // device counters
-constexpr int CNT_TROUBLED = 0, CNT_JUDGED = 1, CNT_REJECTED = 2, CNT_N = 4;
+// device counters: CNT_SLOTS copies of CNT_N counters (block b adds into slot b % CNT_SLOTS, so
+// no address takes more than a few dozen atomics per step); readers sum the slots
+constexpr int CNT_TROUBLED = 0, CNT_JUDGED = 1, CNT_REJECTED = 2, CNT_N = 4, CNT_SLOTS = 64;
 
 struct SelectParams {
     float threshold;      // probability at or above which a leaf is troubled
     uint32_t judge_q16;   // judge sample fraction of cleared leaves, in 1/65536
     uint32_t step;        // step counter (varies the judge sample)
-    int tiles_per_wave;   // 16-leaf tiles per wave per block
 };
 
+// inference: flags (and probabilities when prob != nullptr) of every interior leaf. Scratch:
+// tile_mask [ceil(N/16)], tile_frag [ceil(N/16) * 192] (layer-1 fragments of the tiles that
+// need the network); mlp_blocks: persistent grid of the network pass.
 void launch_select(const DevTables& t, const float4* Qf, const int32_t* off16, const uint16_t* wpk,
-                   const float* w32, const SelectParams& sp, uint8_t* flags, float* prob, hipStream_t st);
+                   const float* w32, const SelectParams& sp, uint8_t* flags, float* prob, uint16_t* tile_mask,
+                   uint4* tile_frag, int mlp_blocks, hipStream_t st);
 void launch_judge(int n, const float* label, float bound, uint8_t* flags, int* counters, hipStream_t st);
 void launch_features(const DevTables& t, const float4* Qf, const int32_t* off16, uint16_t* X, int row0,
                      hipStream_t st);

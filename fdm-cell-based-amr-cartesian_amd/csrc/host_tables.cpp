@@ -169,7 +169,7 @@ void fast_record(const double* d, const int32_t* iv, float* f) {
             double s = 0.0;
             for (int c = 0; c < 6; c++) s += d[WLS_INV6 + c * 6 + r] * P[i][c];
             s *= d[WLS_W + i];
-            f[WF_C6 + r * 9 + i] = (float)s;
+            if (r > 0) f[WF_C6 + (r - 1) * 9 + i] = (float)s;
             cl += bl[r] * s;
         }
         f[WF_CL + i] = (float)cl;
@@ -180,7 +180,7 @@ void fast_record(const double* d, const int32_t* iv, float* f) {
             double s = 0.0;
             for (int c = 0; c < 3; c++) s += d[WLS_INV3A + c * 3 + r] * P[j][c];
             s *= d[WLS_W + j];
-            f[WF_C3A + r * 5 + j] = (float)s;
+            if (r > 0) f[WF_C3A + (r - 1) * 5 + j] = (float)s;
             cs += bl[r] * s;
         }
         f[WF_CS1 + j] = (float)cs;
@@ -191,7 +191,7 @@ void fast_record(const double* d, const int32_t* iv, float* f) {
             double s = 0.0;
             for (int c = 0; c < 3; c++) s += d[WLS_INV3B + c * 3 + r] * P[m1 + j][c];
             s *= d[WLS_W + m1 + j];
-            f[WF_C3B + r * 4 + j] = (float)s;
+            if (r > 0) f[WF_C3B + (r - 1) * 4 + j] = (float)s;
             cs += bl[r] * s;
         }
         f[WF_CS2 + j] = (float)cs;

@@ -30,16 +30,18 @@ enum WlsIntLayout { WLSI_M = 0, WLSI_M1 = 1, WLSI_IDS = 2, WLSI_STRIDE = 12 };
 
 // FP32 slot record of the fast path (floats): the least-squares solves folded into
 // coefficient matrices on the host in FP64 (C6 = (P^T W P)^-1 P^T W etc.), plus the value
-// weights c = basis(xx,yy)^T C, so a slot costs a few small mat-vecs per variable.
+// weights c = basis(xx,yy)^T C, so a slot costs a few small mat-vecs per variable. Only the
+// rows the smoothness indicators read are kept (the constant-term rows enter through the
+// value weights alone).
 enum WlsFastLayout {
-    WF_C6 = 0,      // 6 x 9, row-major [r*9 + i]
-    WF_C3A = 54,    // 3 x 5 (list 1, up to 5 points)
-    WF_C3B = 69,    // 3 x 4 (list 2, up to 4 points)
-    WF_CL = 81,     // 9 value weights of the quadratic fit
-    WF_CS1 = 90,    // 5 value weights of linear fit 1
-    WF_CS2 = 95,    // 4 value weights of linear fit 2
-    WF_BOX = 99,    // 6 box terms (dxb, dyb, x2, y2, x3, y3)
-    WF_STRIDE = 112
+    WF_C6 = 0,      // rows 1..5 of 6 x 9, row-major [(r-1)*9 + i]
+    WF_C3A = 45,    // rows 1..2 of 3 x 5 (list 1, up to 5 points)
+    WF_C3B = 55,    // rows 1..2 of 3 x 4 (list 2, up to 4 points)
+    WF_CL = 63,     // 9 value weights of the quadratic fit
+    WF_CS1 = 72,    // 5 value weights of linear fit 1
+    WF_CS2 = 77,    // 4 value weights of linear fit 2
+    WF_BOX = 81,    // 6 box terms (dxb, dyb, x2, y2, x3, y3)
+    WF_STRIDE = 88
 };
 
 constexpr int32_t REF_INFLOW = -2147483647 - 1;

@@ -180,7 +180,9 @@ int32_t fdmamr_selector_get_weights(fdmamr_ctx* ctx, float* w, int32_t n);
  * leaf is troubled (default 0.05) */
 int32_t fdmamr_selector_config(fdmamr_ctx* ctx, float threshold, float judge_fraction,
                                float label_bound);
-/* run the network on the current state and set the flags (and switch flag use on) */
+/* run the network on the current state and set the flags (and switch flag use on). Leaves
+ * whose 5x5 stencil is near-uniform (every channel within 1e-6 of the centre value, relative)
+ * are cleared without the network and report probability 0. */
 int32_t fdmamr_selector_infer(fdmamr_ctx* ctx, float* prob, int32_t* n_troubled);
 /* BF16 feature rows (FDMAMR_SELECTOR_NFEAT per interior leaf) of the current state */
 int32_t fdmamr_selector_features(fdmamr_ctx* ctx, uint16_t* x_bf16);
@@ -201,7 +203,7 @@ int32_t fdmamr_selector_stats(fdmamr_ctx* ctx, int32_t* troubled, int32_t* judge
 
 /* ---- measurement (no reference counterpart; the reference times only block refinement,
  * mesh.F90:55,76-77) ---- */
-/* bracket every fused stage-kernel launch with HIP events on the context's stream */
+/* bracket the flux + update launches of every stage with HIP events on the context's stream */
 int32_t fdmamr_profile(fdmamr_ctx* ctx, int32_t enable);
 /* total milliseconds and number of stage-kernel launches recorded since the last read */
 int32_t fdmamr_profile_read(fdmamr_ctx* ctx, double* stage_ms_total, int32_t* stage_launches);

@@ -35,6 +35,32 @@ def test_all_troubled_flags_equal_unflagged_path():
     assert np.array_equal(b.get_state(), c.get_state())
 
 
+def test_mixed_flags_take_both_paths():
+    # Random flags split the leaves into the cleared and troubled launch lists; every leaf
+    # must be updated exactly once by its own path, so the mixture lies between the two
+    # uniform choices (a leaf dropped from or duplicated in the lists would leave a stale or
+    # wrong state far outside that band).
+    kw = dict(h=10.0 / 64, localdt=1, cfl=0.5, t_end=1e9)
+    runs = {}
+    rng = np.random.default_rng(5)
+    for name in ("troubled", "cleared", "mixed"):
+        c = F.Solver(F.Config(flux_precision=F.FLUX_FP32, **kw)).setup()
+        n = c.n_leaf
+        flags = {"troubled": np.ones(n, np.uint8), "cleared": np.zeros(n, np.uint8),
+                 "mixed": (rng.uniform(size=n) < 0.4).astype(np.uint8)}[name]
+        c.set_flags(flags, use=True)
+        c.run_steps(5)
+        assert np.array_equal(c.get_flags() & 1, flags)
+        runs[name] = c
+    ref = runs["troubled"].get_state()
+    e_clear = l1_rho(runs["troubled"], ref, runs["cleared"].get_state())
+    e_mix = l1_rho(runs["troubled"], ref, runs["mixed"].get_state())
+    print("L1 vs all-troubled: cleared", e_clear, "mixed", e_mix)
+    assert np.all(np.isfinite(runs["mixed"].get_state()))
+    assert 0.0 < e_mix < e_clear
+    assert runs["mixed"].check_positivity() == 0
+
+
 def test_cleared_leaves_on_smooth_vortex():
     # Linear weights everywhere on a smooth flow stay close to the nonlinear scheme: on this
     # 64x64 vortex ~12% of the leaves are labelled troubled (the core is resolved by ~6

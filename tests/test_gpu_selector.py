@@ -31,8 +31,13 @@ def test_forward_matches_oracle():
     assert np.all(X[:, 75:] == 0.0)
     prob, _ = s.selector_infer()
     want, _ = M.forward(w.astype(np.float64), X, bf16=True)
-    err = np.max(np.abs(prob - want))
-    print("max |p_gpu - p_oracle|", err)
+    # near-uniform stencils (features within 1e-6 of 0) are cleared without the network
+    amax = np.max(np.abs(X[:, :75]), axis=1)
+    gated, net = amax < 5e-7, amax > 2e-6
+    assert np.all(prob[gated] == 0.0)
+    assert net.sum() > 100
+    err = np.max(np.abs(prob[net] - want[net]))
+    print("max |p_gpu - p_oracle|", err, "gated", gated.sum(), "network", net.sum())
     assert err < 2e-3
 
 
