@@ -88,6 +88,66 @@ def cpu_baseline(budget_s: float = 12.0):
                        f"{steps} steps in {dt:.1f}s")
 
 
+def measure_traffic(args):
+    """HBM bytes of one stage's flux/update launches of the learned path, from two rocprofv3
+    counter runs of a short copy of this benchmark (FETCH_SIZE, then WRITE_SIZE, in KiB per
+    dispatch). Each counter is scaled by what it reads for a plain 256 MiB device copy in the
+    same run (on MI355X FETCH_SIZE counts a 128-byte read as 64 bytes). Returns None when
+    rocprofv3 is unavailable or a run fails."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        log("traffic: rocprofv3 not found")
+        return None
+    copy_mib = 256
+    child = [sys.executable, os.path.abspath(__file__), "--steps", "2", "--warmup", "1",
+             "--train-iters", "20", "--no-cpu", "--no-traffic", "--traffic-child",
+             "--copy-calibration", str(copy_mib), "--workload", args.workload,
+             "--refine-goal", str(args.refine_goal), "--advance", str(args.advance)]
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="fdmamr_pmc_")
+        cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--"] + child
+        try:
+            r = subprocess.run(cmd, capture_output=True, text=True, timeout=400)
+        except subprocess.TimeoutExpired:
+            log(f"traffic: {counter} run timed out")
+            return None
+        info = [json.loads(l) for l in r.stdout.splitlines() if l.startswith("{")]
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if r.returncode != 0 or not info or not files:
+            log(f"traffic: {counter} run failed (rc {r.returncode})")
+            return None
+        n_leaf = info[-1]["n_leaf"]
+        grid = (n_leaf + 255) // 256 * 256
+        rows = sorted(csv.DictReader(open(files[0])), key=lambda x: int(x["Dispatch_Id"]))
+        shutil.rmtree(d, ignore_errors=True)
+        # the calibration copies run before the solver library's first kernel
+        first = min((i for i, x in enumerate(rows) if "fdm::" in x["Kernel_Name"]), default=0)
+        copy_kib = [float(x["Counter_Value"]) for x in rows[:first] if "copyBuffer" in x["Kernel_Name"]]
+        mark = max(i for i, x in enumerate(rows) if "k_positivity" in x["Kernel_Name"])
+        stage = [x for x in rows[mark:] if "k_stage_fast" in x["Kernel_Name"] and int(x["Grid_Size"]) == grid]
+        n_stage = sum(1 for x in stage if "<true" in x["Kernel_Name"])
+        if not copy_kib or not n_stage:
+            log(f"traffic: {counter} dispatches not found")
+            return None
+        scale = 3 * (copy_mib << 20) / (sum(copy_kib) * 1024.0)
+        res[counter] = dict(kib_per_stage=sum(float(x["Counter_Value"]) for x in stage) / n_stage,
+                            copy_scale=scale, stages=n_stage)
+    fetch = res["FETCH_SIZE"]["kib_per_stage"] * 1024 * res["FETCH_SIZE"]["copy_scale"]
+    write = res["WRITE_SIZE"]["kib_per_stage"] * 1024 * res["WRITE_SIZE"]["copy_scale"]
+    log(f"traffic per stage: read {fetch / 1e6:.1f} MB, write {write / 1e6:.1f} MB "
+        f"(copy scales {res['FETCH_SIZE']['copy_scale']:.3f}, {res['WRITE_SIZE']['copy_scale']:.3f})")
+    return {"bytes_per_stage": fetch + write, "read_bytes": fetch, "write_bytes": write,
+            "fetch_copy_scale": res["FETCH_SIZE"]["copy_scale"],
+            "write_copy_scale": res["WRITE_SIZE"]["copy_scale"], "stages": res["FETCH_SIZE"]["stages"]}
+
+
 def clone(cfg, tree, prim):
     import fdmamr as F
     s = F.Solver(cfg)
@@ -143,10 +203,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of this many MiB (scale for HBM counters)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter runs that measure HBM traffic")
+    ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         raise SystemExit(2)
+
+    # counter runs first, in child processes, before this process touches the GPU
+    traffic = None if (args.no_traffic or args.traffic_child) else measure_traffic(args)
 
     if args.copy_calibration:
         # torch (when used) is loaded before the solver library, as in the tests
@@ -213,6 +279,12 @@ def main():
     sel.build_tables()
     sel.set_state(prim0)
     s64.set_state(prim0)
+    if args.traffic_child:
+        # marker dispatch for the counter parser, then the learned steps it measures
+        sel.check_positivity()
+        sel.run_steps(args.steps)
+        print(json.dumps({"n_leaf": N, "steps": args.steps}))
+        return
 
     def timed(s, label):
         s.run_steps(args.warmup)  # synchronises at the end
@@ -250,10 +322,10 @@ def main():
 
     ups = args.steps * N / wall_l
     ups64 = args.steps * N / wall64
-    # per-leaf algorithmic HBM bytes of one fused stage launch (fast path), averaged over the
-    # 3 stages: ids 32 + lvl 1 + flag 1 + own double4 32 + own float4 16 + out double4 32 +
-    # out float4 16 + U_n double4 32 in stages 2-3
-    bytes_per_leaf = 32 + 1 + 1 + 32 + 16 + 32 + 16 + 32 * 2.0 / 3.0
+    # per-leaf algorithmic HBM bytes of one stage's flux/update launches (fast path), averaged
+    # over the 3 stages: stencil ids 32 + list entry 4 + lvl 1 + own double4 32 + own float4 16
+    # + out double4 32 + out float4 16 + U_n double4 32 in stages 2-3
+    bytes_per_leaf = 32 + 4 + 1 + 32 + 16 + 32 + 16 + 32 * 2.0 / 3.0
     achieved = N * bytes_per_leaf / (k_l * 1e-3) / 1e9
     out = {
         "metric": "leaf-cell RK3 updates/s: learned+FP32 vs plain FP64 at L1(rho) err <=1e-4",
@@ -280,9 +352,11 @@ def main():
         "regrid_s": t_regrid,
         "value_regrid_amortised": args.steps * N / (wall_l + t_regrid * args.steps / 10.0),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_stage_fast",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes_per_stage"] if traffic else None,
+                     "kernel": "k_stage_fast<linear>+<characteristic>, one RK stage",
                      "kernel_ms": k_l, "bytes_per_leaf_launch": bytes_per_leaf,
-                     "fp64_kernel_ms": k64},
+                     "fp64_kernel_ms": k64, "traffic_detail": traffic},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()

@@ -427,8 +427,8 @@ __global__ __launch_bounds__(128) void k_slots_fast(DevTables t, const float* __
                                                     float4* __restrict__ Qf) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= t.S_w) return;
-    const float* __restrict__ f = wf + (size_t)s * WF_STRIDE;
     const int32_t* __restrict__ iv = t.wls_i + (size_t)s * WLSI_STRIDE;
+    const float* __restrict__ f = wf + (size_t)iv[WLSI_PAT] * WF_STRIDE;
     const int m = iv[WLSI_M], m1 = iv[WLSI_M1];
     float4 val[9];
 #pragma unroll

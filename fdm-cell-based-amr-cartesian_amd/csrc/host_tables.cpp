@@ -358,7 +358,7 @@ void build_tables(const Tree& t, Tables& tb) {
     // phase 3 (parallel): stencil lists and FP64 geometry of every WLS slot
     tb.wls_d.assign((size_t)tb.S_w * WLS_STRIDE, 0.0);
     tb.wls_i.assign((size_t)tb.S_w * WLSI_STRIDE, -1);
-    tb.wls_f.assign((size_t)tb.S_w * WF_STRIDE, 0.0f);
+    std::vector<float> rec((size_t)tb.S_w * WF_STRIDE, 0.0f);
     parallel_for((size_t)tb.S_w, [&](size_t b, size_t e) {
         for (size_t s = b; s < e; s++) {
             List l1, l2;
@@ -368,9 +368,45 @@ void build_tables(const Tree& t, Tables& tb) {
             double* dd = &tb.wls_d[s * WLS_STRIDE];
             int32_t* iv = &tb.wls_i[s * WLSI_STRIDE];
             wls_geometry(t, tg, wls_xx[s], wls_yy[s], l1, l2, tb.state_of_node, dd, iv);
-            fast_record(dd, iv, &tb.wls_f[s * WF_STRIDE]);
+            fast_record(dd, iv, &rec[s * WF_STRIDE]);
         }
     });
+    // FP32 records by pattern: in the cell-local frame a record depends only on the shape of
+    // the stencil around the level jump (coordinates in units of the cell size), so slots
+    // with the same shape have bit-identical records and share one, numbered in slot order.
+    // The fast path then reads a few hundred records from cache instead of one per slot.
+    {
+        auto hash_rec = [&](size_t s) {
+            uint64_t h = 1469598103934665603ull;
+            const uint32_t* w = reinterpret_cast<const uint32_t*>(&rec[s * WF_STRIDE]);
+            for (int k = 0; k < WF_STRIDE; k++) { h ^= w[k]; h *= 1099511628211ull; }
+            return h;
+        };
+        std::vector<uint64_t> hs(tb.S_w);
+        parallel_for((size_t)tb.S_w, [&](size_t b, size_t e) {
+            for (size_t s = b; s < e; s++) hs[s] = hash_rec(s);
+        });
+        std::unordered_map<uint64_t, std::vector<int>> buckets;
+        std::vector<int> pat_slot;  // first slot of each pattern
+        for (int s = 0; s < tb.S_w; s++) {
+            auto& bk = buckets[hs[s]];
+            int pid = -1;
+            for (int p : bk)
+                if (std::memcmp(&rec[(size_t)pat_slot[p] * WF_STRIDE], &rec[(size_t)s * WF_STRIDE],
+                                WF_STRIDE * sizeof(float)) == 0) { pid = p; break; }
+            if (pid < 0) {
+                pid = (int)pat_slot.size();
+                pat_slot.push_back(s);
+                bk.push_back(pid);
+            }
+            tb.wls_i[(size_t)s * WLSI_STRIDE + WLSI_PAT] = pid;
+        }
+        tb.n_pat = (int)pat_slot.size();
+        tb.wls_f.assign((size_t)tb.n_pat * WF_STRIDE, 0.0f);
+        for (int p = 0; p < tb.n_pat; p++)
+            std::memcpy(&tb.wls_f[(size_t)p * WF_STRIDE], &rec[(size_t)pat_slot[p] * WF_STRIDE],
+                        WF_STRIDE * sizeof(float));
+    }
     tb.bc_partner.resize(tb.G);
     for (int g = 0; g < tb.G; g++) tb.bc_partner[g] = t.bc_partner(t.bcl[g]);
     tb.l_max = lmax;
@@ -460,6 +496,7 @@ void build_tables(const Tree& t, Tables& tb) {
             throw std::runtime_error("WLS stencil size out of range");
         for (int i = 0; i < iv[WLSI_M]; i++)
             if (iv[WLSI_IDS + i] < 0 || iv[WLSI_IDS + i] >= tb.L) throw std::runtime_error("WLS id out of range");
+        if (iv[WLSI_PAT] < 0 || iv[WLSI_PAT] >= tb.n_pat) throw std::runtime_error("WLS record index out of range");
     }
     for (int32_t q : tb.rst_node)
         if (q < 0 || (size_t)q >= NA || t.son[q] < 0) throw std::runtime_error("restriction node invalid");

@@ -26,7 +26,8 @@ enum WlsLayout {
     WLS_BOX = 83,    // 6: dxb, dyb, x2, y2, x3, y3
     WLS_STRIDE = 96
 };
-enum WlsIntLayout { WLSI_M = 0, WLSI_M1 = 1, WLSI_IDS = 2, WLSI_STRIDE = 12 };
+// WLSI_PAT: index of the slot's FP32 record (slots with bit-identical records share one)
+enum WlsIntLayout { WLSI_M = 0, WLSI_M1 = 1, WLSI_IDS = 2, WLSI_PAT = 11, WLSI_STRIDE = 12 };
 
 // FP32 slot record of the fast path (floats): the least-squares solves folded into
 // coefficient matrices on the host in FP64 (C6 = (P^T W P)^-1 P^T W etc.), plus the value
@@ -64,7 +65,8 @@ struct Tables {
     // slots
     std::vector<double> wls_d;           // [S_w * WLS_STRIDE]
     std::vector<int32_t> wls_i;          // [S_w * WLSI_STRIDE]
-    std::vector<float> wls_f;            // [S_w * WF_STRIDE] fast-path records
+    std::vector<float> wls_f;            // [n_pat * WF_STRIDE] fast-path records, one per pattern
+    int n_pat = 0;
     std::vector<int32_t> rst_node;       // [S_r]
     // ghost fill: partner node per ghost (-1: inflow state)
     std::vector<int32_t> bc_partner;     // [G]
