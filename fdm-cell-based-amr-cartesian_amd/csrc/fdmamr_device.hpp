@@ -24,6 +24,68 @@ __device__ __forceinline__ int xcd_block_n(int b, int nb) {
 }
 __device__ __forceinline__ int xcd_block() { return xcd_block_n(blockIdx.x, gridDim.x); }
 
+// ---- wave64 reductions on DPP lane moves (VALU, no LDS round trips) ----
+// Lanes whose DPP source is outside the pattern or whose row is masked off read 0, the
+// identity of a sum and of a maximum of non-negative values; the result is valid in lane 63.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_move(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffll), CTRL, ROWS, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+// quad_perm [1,0,3,2] and [2,3,0,1] sum quads, row_half_mirror and row_mirror the 16-lane
+// rows; row_bcast:15 (rows 1, 3) and row_bcast:31 (rows 2, 3) carry the rows into lane 63
+template <bool MAX>
+__device__ __forceinline__ double wave_reduce63(double v) {
+    auto op = [](double a, double b) { return MAX ? fmax(a, b) : a + b; };
+    v = op(v, dpp_move<0xB1, 0xF>(v));
+    v = op(v, dpp_move<0x4E, 0xF>(v));
+    v = op(v, dpp_move<0x141, 0xF>(v));
+    v = op(v, dpp_move<0x140, 0xF>(v));
+    v = op(v, dpp_move<0x142, 0xA>(v));
+    v = op(v, dpp_move<0x143, 0xC>(v));
+    return v;
+}
+
+// Per-block stage partials (256 threads, every thread active): max |r| and sum r^2 of the
+// four residual components to partials[blk * 8 ...] and, with lampart, the block's largest
+// |u|+c and |v|+c to lampart[blk * 2 ...].
+__device__ __forceinline__ void block_partials(const double* rmax, const double* rsum, double* partials,
+                                               int blk, double* lampart, double lx, double ly) {
+    __shared__ double red[4][10];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double a = wave_reduce63<true>(rmax[k]);
+        const double b = wave_reduce63<false>(rsum[k]);
+        if (lane == 63) { red[wv][k] = a; red[wv][4 + k] = b; }
+    }
+    if (lampart) {
+        lx = wave_reduce63<true>(lx);
+        ly = wave_reduce63<true>(ly);
+        if (lane == 63) { red[wv][8] = lx; red[wv][9] = ly; }
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        const int k = threadIdx.x;
+        double acc = red[0][k];
+        for (int w = 1; w < 4; w++) acc = k < 4 ? fmax(acc, red[w][k]) : acc + red[w][k];
+        partials[(size_t)blk * 8 + k] = acc;
+    } else if (lampart && threadIdx.x < 10) {
+        const int k = threadIdx.x;
+        double acc = red[0][k];
+        for (int w = 1; w < 4; w++) acc = fmax(acc, red[w][k]);
+        lampart[(size_t)blk * 2 + (k - 8)] = acc;
+    }
+}
+
+// zero partials of a block with no leaves
+__device__ __forceinline__ void block_partials_zero(double* partials, int blk, double* lampart) {
+    if (threadIdx.x < 8) partials[(size_t)blk * 8 + threadIdx.x] = 0.0;
+    else if (lampart && threadIdx.x < 10) lampart[(size_t)blk * 2 + (threadIdx.x - 8)] = 0.0;
+}
+
 // Device views of the flattened tables (all pointers are device memory).
 struct DevTables {
     int N, G, P, L, S_w, S_r;

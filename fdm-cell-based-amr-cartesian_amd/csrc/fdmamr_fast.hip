@@ -28,14 +28,6 @@ namespace fdm {
 
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
-__device__ __forceinline__ double wmax(double v) {
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-    return v;
-}
-__device__ __forceinline__ double wsum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o);
-    return v;
-}
 
 struct PtF { float U[4], F[4]; };
 
@@ -207,9 +199,17 @@ __device__ __forceinline__ void stage_leaf(const DevTables& t, int i, const doub
     int s[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) s[k] = t.stencil[(size_t)k * t.N + i];
+    // the full path loads everything up front (own states, U_n, the eight neighbours), so its
+    // few resident waves wait for memory twice (indices, then values) rather than once per
+    // use; the linear path keeps its register count low and relies on occupancy instead
     const double4 qc = Qin[i];
+    double4 qn;
+    if (FULL) qn = Qn[i];
     const float4 qcf = Qf_in[i];
     const int lv = t.lvl[i];
+    float4 nb[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) nb[k] = FULL ? Qf_in[s[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
     const double dy = t.h / (double)(1 << lv);
     const double dx = t.xf * dy;
     float dE[4], dF[4];
@@ -219,8 +219,7 @@ __device__ __forceinline__ void stage_leaf(const DevTables& t, int i, const doub
         // states with u and v exchanged, its momentum components exchanged back
 #pragma nounroll
         for (int dir = 0; dir < 2; dir++) {
-            float4 col[5] = {Qf_in[dir ? s[7] : s[5]], Qf_in[dir ? s[3] : s[1]], qcf, Qf_in[dir ? s[2] : s[0]],
-                             Qf_in[dir ? s[6] : s[4]]};
+            float4 col[5] = {dir ? nb[7] : nb[5], dir ? nb[3] : nb[1], qcf, dir ? nb[2] : nb[0], dir ? nb[6] : nb[4]};
 #pragma unroll
             for (int j = 0; j < 5; j++) {
                 const float a = col[j].y, b = col[j].z;
@@ -264,7 +263,7 @@ __device__ __forceinline__ void stage_leaf(const DevTables& t, int i, const doub
     const double ir = 1.0 / r;
     const double u = Ut[1] * ir, v = Ut[2] * ir;
     const double p = (gama - 1.0) * (Ut[3] - 0.5 * r * (u * u + v * v));
-    const double4 qn = Qn[i];
+    if (!FULL) qn = Qn[i];
     double4 o;
     o.x = alpha * qn.x + beta * r;
     o.y = alpha * qn.y + beta * u;
@@ -305,44 +304,20 @@ __global__ __launch_bounds__(256) void k_stage_fast(DevTables t, const int32_t* 
     // the active prefix of the grid is spread over all XCDs, then renumbered within itself
     const int nact = (n + 255) >> 8;
     const bool active = (int)blockIdx.x < nact;
-    if (!active && !reduce) return;
-    const int lb = active ? xcd_block_n(blockIdx.x, nact) : (int)blockIdx.x;
+    if (!active) {
+        if (reduce) block_partials_zero(partials, poff + blockIdx.x, lampart);
+        return;
+    }
+    const int lb = xcd_block_n(blockIdx.x, nact);
     const int p = lb * 256 + (int)threadIdx.x;
     double rmax[4] = {0, 0, 0, 0}, rsum[4] = {0, 0, 0, 0};
     double lx = 0.0, ly = 0.0;
-    if (active && p < n) {
+    if (p < n) {
         const int i = lists ? lists[base + p] : p;
         stage_leaf<FULL, LABEL>(t, i, Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, label, rmax, rsum,
                                 lampart != nullptr, lx, ly);
     }
-    if (reduce) {
-        __shared__ double red[4][10];
-        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const double a = wmax(rmax[k]);
-            const double b = wsum(rsum[k]);
-            if (lane == 0) { red[wv][k] = a; red[wv][4 + k] = b; }
-        }
-        if (lampart) {
-            lx = wmax(lx);
-            ly = wmax(ly);
-            if (lane == 0) { red[wv][8] = lx; red[wv][9] = ly; }
-        }
-        __syncthreads();
-        const int blk = poff + lb;
-        if (threadIdx.x < 8) {
-            const int k = threadIdx.x;
-            double acc = red[0][k];
-            for (int w = 1; w < 4; w++) acc = k < 4 ? fmax(acc, red[w][k]) : acc + red[w][k];
-            partials[(size_t)blk * 8 + k] = acc;
-        } else if (lampart && threadIdx.x < 10) {
-            const int k = threadIdx.x;
-            double acc = red[0][k];
-            for (int w = 1; w < 4; w++) acc = fmax(acc, red[w][k]);
-            lampart[(size_t)blk * 2 + (k - 8)] = acc;
-        }
-    }
+    if (reduce) block_partials(rmax, rsum, partials, poff + lb, lampart, lx, ly);
 }
 
 // ---------------------------------------------------------------- leaf lists from flags

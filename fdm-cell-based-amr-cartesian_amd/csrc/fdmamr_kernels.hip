@@ -200,39 +200,6 @@ __device__ __forceinline__ void flux_derivative(const double4* q, double d, S la
     }
 }
 
-// ---------------------------------------------------------------- stage reductions
-// Per-block residual partials (max |r|, sum r^2 of the four equations) and, with lampart, the
-// block's largest |u|+c and |v|+c (256 threads).
-__device__ __forceinline__ void block_partials(const double* rmax, const double* rsum,
-                                               double* __restrict__ partials, int blk,
-                                               double* __restrict__ lampart, double lx, double ly) {
-    __shared__ double red[4][10];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const double a = wave_max(rmax[k]);
-        const double b = wave_sum(rsum[k]);
-        if (lane == 0) { red[wv][k] = a; red[wv][4 + k] = b; }
-    }
-    if (lampart) {
-        lx = wave_max(lx);
-        ly = wave_max(ly);
-        if (lane == 0) { red[wv][8] = lx; red[wv][9] = ly; }
-    }
-    __syncthreads();
-    if (threadIdx.x < 8) {
-        const int k = threadIdx.x;
-        double acc = red[0][k];
-        for (int w = 1; w < 4; w++) acc = k < 4 ? fmax(acc, red[w][k]) : acc + red[w][k];
-        partials[(size_t)blk * 8 + k] = acc;
-    } else if (lampart && threadIdx.x < 10) {
-        const int k = threadIdx.x;
-        double acc = red[0][k];
-        for (int w = 1; w < 4; w++) acc = fmax(acc, red[w][k]);
-        lampart[(size_t)blk * 2 + (k - 8)] = acc;
-    }
-}
-
 // ---------------------------------------------------------------- stage kernel
 // Qn/Tn alias Qout/Tout in stage 3 (in-place final stage), so they carry no __restrict__.
 template <typename S>
