@@ -66,7 +66,7 @@ struct fdmamr_ctx {
     int adam_t = 0;
     DBuf<int32_t> stencil, wls_i, rst, bcp, son, state;
     DBuf<int8_t> lvl;
-    DBuf<double> wls_d, partials, lampart, rot, div, tmp;
+    DBuf<double> wls_d, partials, lampart, rot, div, tmp, parts;
     DBuf<int> nbad;
     DevScalars* sc = nullptr;
     DevTables dt{};
@@ -150,6 +150,7 @@ int upload_tables(fdmamr_ctx* c) {
     HIPCHK(c, ensure(c->rot, tb.N));
     HIPCHK(c, ensure(c->div, tb.N));
     HIPCHK(c, ensure(c->tmp, 64));
+    HIPCHK(c, ensure(c->parts, STEP_PARTS * 10));
     HIPCHK(c, ensure(c->nbad, 1));
     hipStream_t s = c->stream;
     auto up = [&](void* d, const void* h, size_t bytes) -> hipError_t {
@@ -312,8 +313,8 @@ void enqueue_rk3(fdmamr_ctx* c, int advance_time, bool prepare_next) {
         fill_after_stage(c, out[k]);
     }
     const int npart = fast ? stage_fast_partial_blocks(d.N, lists != nullptr) : nblocks(d.N, 256);
-    launch_step_end(c->partials.p, npart, d.N, advance_time, c->sc, prepare_next ? c->lampart.p : nullptr, npart,
-                    s);
+    launch_step_end(c->partials.p, npart, lists ? c->list_meta.p : nullptr, nblocks(4 * d.N, 256), d.N, advance_time,
+                    c->sc, prepare_next ? c->lampart.p : nullptr, c->parts.p, s);
 }
 
 }  // namespace
@@ -363,7 +364,7 @@ void fdmamr_destroy(fdmamr_ctx* c) {
     for (int k = 0; k < 3; k++) { release(c->Q[k]); release(c->T[k]); }
     release(c->stencil); release(c->wls_i); release(c->rst); release(c->bcp); release(c->son);
     release(c->state); release(c->lvl); release(c->wls_d); release(c->partials); release(c->lampart);
-    release(c->rot); release(c->div); release(c->tmp); release(c->nbad);
+    release(c->rot); release(c->div); release(c->tmp); release(c->parts); release(c->nbad);
     release(c->Qf[0]); release(c->Qf[1]); release(c->Qf[2]); release(c->wls_f); release(c->flags);
     release(c->label); release(c->off16); release(c->sel_w); release(c->sel_m); release(c->sel_v);
     release(c->sel_grad); release(c->sel_gpart); release(c->sel_lpart); release(c->sel_loss);

@@ -123,8 +123,12 @@ void launch_restrict_fill(const DevTables& t, const std::vector<int32_t>& level_
                           double* T, float4* Qf, double4* Qi, double* Ti, hipStream_t st);
 void launch_wavespeed(const DevTables& t, const double4* Q, double* partials, hipStream_t st);
 void launch_step_begin(const double* partials, int nb, DevScalars* sc, hipStream_t st);
-void launch_step_end(const double* partials, int nb, int n_leaf, int advance_time, DevScalars* sc,
-                     const double* lampart, int lam_nb, hipStream_t st);
+// End of a step: residual, time advance and (with lampart, the final stage's wave speeds) the
+// next step's lambda and dt. Partials: nb blocks, or with meta the fast path's split (full
+// blocks from 0, linear blocks from lin0). parts: STEP_PARTS * 10 doubles of scratch.
+constexpr int STEP_PARTS = 64;
+void launch_step_end(const double* partials, int nb, const int* meta, int lin0, int n_leaf, int advance_time,
+                     DevScalars* sc, const double* lampart, double* parts, hipStream_t st);
 void launch_sensors(const DevTables& t, const double4* Q, double* rot, double* div, hipStream_t st);
 void launch_positivity(const DevTables& t, const double4* Q, int* nbad, hipStream_t st);
 void launch_stencil_values(const DevTables& t, const double4* Q, int i, double* out, hipStream_t st);

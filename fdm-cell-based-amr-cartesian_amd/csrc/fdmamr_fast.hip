@@ -104,60 +104,46 @@ __device__ __forceinline__ float wcns_f(float u1, float u2, float u3, float& dev
     return w1 * p1 + w2 * p2;
 }
 
-// full characteristic WCNS3 flux derivative in FP32 (troubled leaves), x-direction form (the
-// y-direction calls it on states with u and v exchanged). The two faces share one loop body
-// (not unrolled) to keep the register footprint of one face.
+// Characteristic WCNS3 flux through one face in FP32 (troubled leaves), x-direction form
+// (y-direction faces are computed on states with u and v exchanged): the face lies between
+// S[1] and S[2]; eigenvectors at the arithmetic average of those two (REcalc / RFcalc),
+// U and F projected, four WCNS_Z3 reconstructions per characteristic field, the
+// Lax-Friedrichs flux formed in characteristic space and projected back once
+// (nnd_center_RK.F90:66-353).
 template <bool LABEL>
-__device__ __forceinline__ void flux_full_f(const float4* q, float d, float lam, float gama, float* out,
-                                            float& dev) {
-    PtF P[5];
-    const float gm1inv = 1.f / (gama - 1.f);
+__device__ __forceinline__ void face_flux_f(const PtF* S, float gama, float lam, float* flux, float& dev) {
+    float avg[4];
 #pragma unroll
-    for (int j = 0; j < 5; j++) cons_flux_f<0>(q[j], gm1inv, P[j]);
-    float fdiff[4] = {0.f, 0.f, 0.f, 0.f};   // flux(i+1/2) - flux(i-1/2)
-#pragma nounroll
-    for (int f = 0; f < 2; f++) {
-        PtF S[4];   // points f .. f+3
+    for (int k = 0; k < 4; k++) avg[k] = 0.5f * (S[1].U[k] + S[2].U[k]);
+    float R[4][4], Ri[4][4];
+    eigen_f<0>(avg, gama, R, Ri);
+    float fc[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++)
+    for (int r = 0; r < 4; r++) {
+        float w[4], g[4];
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                S[j].U[k] = f ? P[j + 1].U[k] : P[j].U[k];
-                S[j].F[k] = f ? P[j + 1].F[k] : P[j].F[k];
-            }
-        float avg[4];
+        for (int j = 0; j < 4; j++) {
+            float a = 0.f, b = 0.f;
 #pragma unroll
-        for (int k = 0; k < 4; k++) avg[k] = 0.5f * (S[1].U[k] + S[2].U[k]);
-        float R[4][4], Ri[4][4];
-        eigen_f<0>(avg, gama, R, Ri);
-        float fc[4];
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            float w[4], g[4];
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                float a = 0.f, b = 0.f;
-#pragma unroll
-                for (int c = 0; c < 4; c++) { a += Ri[r][c] * S[j].U[c]; b += Ri[r][c] * S[j].F[c]; }
-                w[j] = a; g[j] = b;
-            }
-            const float cL = wcns_f<LABEL>(w[0], w[1], w[2], dev);
-            const float cR = wcns_f<LABEL>(w[3], w[2], w[1], dev);
-            const float hL = wcns_f<LABEL>(g[0], g[1], g[2], dev);
-            const float hR = wcns_f<LABEL>(g[3], g[2], g[1], dev);
-            // flux in characteristic space; one back-projection R (gL + gR - lam (wR - wL)) / 2
-            fc[r] = 0.5f * (hL + hR - lam * (cR - cL));
+            for (int c = 0; c < 4; c++) { a += Ri[r][c] * S[j].U[c]; b += Ri[r][c] * S[j].F[c]; }
+            w[j] = a; g[j] = b;
         }
-        const float sg = f ? 1.f : -1.f;
-#pragma unroll
-        for (int r = 0; r < 4; r++)
-            fdiff[r] += sg * (R[r][0] * fc[0] + R[r][1] * fc[1] + R[r][2] * fc[2] + R[r][3] * fc[3]);
+        const float cL = wcns_f<LABEL>(w[0], w[1], w[2], dev);
+        const float cR = wcns_f<LABEL>(w[3], w[2], w[1], dev);
+        const float hL = wcns_f<LABEL>(g[0], g[1], g[2], dev);
+        const float hR = wcns_f<LABEL>(g[3], g[2], g[1], dev);
+        fc[r] = 0.5f * (hL + hR - lam * (cR - cL));
     }
-    const float id = 1.f / d;
 #pragma unroll
-    for (int n = 0; n < 4; n++)
-        out[n] = (4.f / 3.f) * id * fdiff[n] - (1.f / 6.f) * id * (P[3].F[n] - P[1].F[n]);
+    for (int r = 0; r < 4; r++) flux[r] = R[r][0] * fc[0] + R[r][1] * fc[1] + R[r][2] * fc[2] + R[r][3] * fc[3];
 }
+
+// value of the lane CTRL names within the lane's quad (DPP quad_perm; no LDS round trip)
+template <int CTRL>
+__device__ __forceinline__ float quad_lane(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+constexpr int QP_XOR1 = 0xB1, QP_XOR2 = 0x4E, QP_XOR3 = 0x1B;  // quad_perm [1,0,3,2], [2,3,0,1], [3,2,1,0]
 
 // linear-weight WCNS3 flux derivative (cleared leaves): no projection, no weights
 template <int DIR>
@@ -187,65 +173,14 @@ __device__ __forceinline__ void flux_lin_f(const float4* q, float d, float lam, 
 // One leaf of a fast-path stage: FP32 flux derivatives from the FP32 shadow, FP64 substep and
 // RK blend on the FP64 state. FULL selects the characteristic WCNS3 (troubled leaves) or the
 // linear-weight reconstruction (cleared leaves).
-template <bool FULL, bool LABEL>
-__device__ __forceinline__ void stage_leaf(const DevTables& t, int i, const double4* __restrict__ Qin,
-                                           const float4* __restrict__ Qf_in, const double4* Qn,
-                                           double4* Qout, float4* __restrict__ Qf_out,
-                                           const DevScalars* __restrict__ sc, double alpha, double beta,
-                                           float* __restrict__ label, double* rmax, double* rsum,
-                                           bool lam, double& lx, double& ly) {
-    const double gama = t.gamma;
-    const double dt = sc->dt;
-    int s[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) s[k] = t.stencil[(size_t)k * t.N + i];
-    // the full path loads everything up front (own states, U_n, the eight neighbours), so its
-    // few resident waves wait for memory twice (indices, then values) rather than once per
-    // use; the linear path keeps its register count low and relies on occupancy instead
-    const double4 qc = Qin[i];
-    double4 qn;
-    if (FULL) qn = Qn[i];
-    const float4 qcf = Qf_in[i];
-    const int lv = t.lvl[i];
-    float4 nb[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) nb[k] = FULL ? Qf_in[s[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const double dy = t.h / (double)(1 << lv);
-    const double dx = t.xf * dy;
-    float dE[4], dF[4];
-    float dev = 0.f;
-    if (FULL) {
-        // one loop body for both directions: the y-direction is the x-direction flux of the
-        // states with u and v exchanged, its momentum components exchanged back
-#pragma nounroll
-        for (int dir = 0; dir < 2; dir++) {
-            float4 col[5] = {dir ? nb[7] : nb[5], dir ? nb[3] : nb[1], qcf, dir ? nb[2] : nb[0], dir ? nb[6] : nb[4]};
-#pragma unroll
-            for (int j = 0; j < 5; j++) {
-                const float a = col[j].y, b = col[j].z;
-                col[j].y = dir ? b : a;
-                col[j].z = dir ? a : b;
-            }
-            float o[4];
-            flux_full_f<LABEL>(col, dir ? (float)dy : (float)dx, dir ? (float)sc->lam_y : (float)sc->lam_x,
-                               (float)gama, o, dev);
-            const float o1 = dir ? o[2] : o[1], o2 = dir ? o[1] : o[2];
-            if (dir) { dF[0] = o[0]; dF[1] = o1; dF[2] = o2; dF[3] = o[3]; }
-            else { dE[0] = o[0]; dE[1] = o1; dE[2] = o2; dE[3] = o[3]; }
-        }
-    } else {
-        {
-            const float4 col[5] = {Qf_in[s[5]], Qf_in[s[1]], qcf, Qf_in[s[0]], Qf_in[s[4]]};
-            flux_lin_f<0>(col, (float)dx, (float)sc->lam_x, (float)gama, dE);
-        }
-        {
-            const float4 col[5] = {Qf_in[s[7]], Qf_in[s[3]], qcf, Qf_in[s[2]], Qf_in[s[6]]};
-            flux_lin_f<1>(col, (float)dy, (float)sc->lam_y, (float)gama, dF);
-        }
-    }
-    if (label) label[i] = dev;
-    // FP64 forward-Euler substep and primitive RK blend (nnd_center_RK.F90:22-42,
-    // solution.F90:398-400)
+// FP64 forward-Euler substep and primitive RK blend of one leaf (nnd_center_RK.F90:22-42,
+// solution.F90:398-400); with lam, the next step's wave speeds of the result
+// (solution.F90:186-193)
+__device__ __forceinline__ void update_leaf(int i, const double4& qc, const double4& qn, const float* dE,
+                                            const float* dF, double dx, double dy, double gama, double dt,
+                                            double alpha, double beta, double4* Qout,
+                                            float4* __restrict__ Qf_out, double* rmax, double* rsum, bool lam,
+                                            double& lx, double& ly) {
     double U[4], Ut[4];
     U[0] = qc.x;
     U[1] = qc.x * qc.y;
@@ -263,7 +198,6 @@ __device__ __forceinline__ void stage_leaf(const DevTables& t, int i, const doub
     const double ir = 1.0 / r;
     const double u = Ut[1] * ir, v = Ut[2] * ir;
     const double p = (gama - 1.0) * (Ut[3] - 0.5 * r * (u * u + v * v));
-    if (!FULL) qn = Qn[i];
     double4 o;
     o.x = alpha * qn.x + beta * r;
     o.y = alpha * qn.y + beta * u;
@@ -272,52 +206,186 @@ __device__ __forceinline__ void stage_leaf(const DevTables& t, int i, const doub
     Qout[i] = o;
     Qf_out[i] = make_float4((float)o.x, (float)o.y, (float)o.z, (float)o.w);
     if (lam) {
-        // the next step's wave speeds on this final state (solution.F90:186-193)
         const double a = sqrt(gama * o.w / o.x);
         lx = fabs(o.y) + a;
         ly = fabs(o.z) + a;
     }
 }
 
-// Stage over one part of the leaf list: lists[0, n_clear) holds the cleared leaves and
-// lists[n_clear, N) the troubled ones (meta[0] = n_clear), both in ascending leaf order, so
-// no wave mixes the two reconstructions. lists == nullptr: every leaf, full path.
+// One cleared leaf of a fast-path stage: linear-weight FP32 flux derivatives from the FP32
+// shadow, FP64 substep and RK blend on the FP64 state.
+__device__ __forceinline__ void stage_leaf_lin(const DevTables& t, int i, const double4* __restrict__ Qin,
+                                               const float4* __restrict__ Qf_in, const double4* Qn,
+                                               double4* Qout, float4* __restrict__ Qf_out,
+                                               const DevScalars* __restrict__ sc, double alpha, double beta,
+                                               float* __restrict__ label, double* rmax, double* rsum,
+                                               bool lam, double& lx, double& ly) {
+    const double gama = t.gamma;
+    int s[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) s[k] = t.stencil[(size_t)k * t.N + i];
+    const double4 qc = Qin[i];
+    const float4 qcf = Qf_in[i];
+    const int lv = t.lvl[i];
+    const double dy = t.h / (double)(1 << lv);
+    const double dx = t.xf * dy;
+    float dE[4], dF[4];
+    {
+        const float4 col[5] = {Qf_in[s[5]], Qf_in[s[1]], qcf, Qf_in[s[0]], Qf_in[s[4]]};
+        flux_lin_f<0>(col, (float)dx, (float)sc->lam_x, (float)gama, dE);
+    }
+    {
+        const float4 col[5] = {Qf_in[s[7]], Qf_in[s[3]], qcf, Qf_in[s[2]], Qf_in[s[6]]};
+        flux_lin_f<1>(col, (float)dy, (float)sc->lam_y, (float)gama, dF);
+    }
+    if (label) label[i] = 0.f;
+    update_leaf(i, qc, Qn[i], dE, dF, dx, dy, gama, sc->dt, alpha, beta, Qout, Qf_out, rmax, rsum, lam, lx, ly);
+}
+
+// Linear (cleared) leaves: lists[0, n_clear) (meta[0] = n_clear), ascending leaf order.
 // Residual partials of logical block b go to partials[(poff + b) * 8 ...].
 // Qn/Qout may alias (in-place final stage), so they carry no __restrict__.
-template <bool FULL, bool LABEL>
-__global__ __launch_bounds__(256) void k_stage_fast(DevTables t, const int32_t* __restrict__ lists,
-                                                    const int* __restrict__ meta,
-                                                    const double4* __restrict__ Qin,
-                                                    const float4* __restrict__ Qf_in,
-                                                    const double4* Qn, double4* Qout,
-                                                    float4* __restrict__ Qf_out,
-                                                    const DevScalars* __restrict__ sc, double alpha,
-                                                    double beta, int reduce, double* __restrict__ partials,
-                                                    double* __restrict__ lampart, int poff,
-                                                    float* __restrict__ label) {
-    int n = t.N, base = 0;
-    if (lists) {
-        const int nc = meta[0];
-        n = FULL ? t.N - nc : nc;
-        base = FULL ? nc : 0;
-    }
+__device__ __forceinline__ void lin_block(int blk, const DevTables& t, const int32_t* __restrict__ lists,
+                                          const int* __restrict__ meta, const double4* __restrict__ Qin,
+                                          const float4* __restrict__ Qf_in, const double4* Qn, double4* Qout,
+                                          float4* __restrict__ Qf_out, const DevScalars* __restrict__ sc,
+                                          double alpha, double beta, int reduce, double* __restrict__ partials,
+                                          double* __restrict__ lampart, int poff, float* __restrict__ label) {
+    const int n = meta[0];
     // the active prefix of the grid is spread over all XCDs, then renumbered within itself
     const int nact = (n + 255) >> 8;
-    const bool active = (int)blockIdx.x < nact;
-    if (!active) {
-        if (reduce) block_partials_zero(partials, poff + blockIdx.x, lampart);
+    if (blk >= nact) {
+        if (reduce) block_partials_zero(partials, poff + blk, lampart);
         return;
     }
-    const int lb = xcd_block_n(blockIdx.x, nact);
+    const int lb = xcd_block_n(blk, nact);
     const int p = lb * 256 + (int)threadIdx.x;
     double rmax[4] = {0, 0, 0, 0}, rsum[4] = {0, 0, 0, 0};
     double lx = 0.0, ly = 0.0;
-    if (p < n) {
-        const int i = lists ? lists[base + p] : p;
-        stage_leaf<FULL, LABEL>(t, i, Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, label, rmax, rsum,
-                                lampart != nullptr, lx, ly);
+    if (p < n)
+        stage_leaf_lin(t, lists[p], Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, label, rmax, rsum,
+                       lampart != nullptr, lx, ly);
+    if (reduce) block_partials(rmax, rsum, partials, poff + lb, lampart, lx, ly);
+}
+
+// Troubled leaves, lists[n_clear, N) (lists == nullptr: every leaf), one face per lane: lanes
+// 4p .. 4p+3 of leaf p compute its faces x-1/2, x+1/2, y-1/2, y+1/2, so the expensive
+// characteristic reconstruction fills four times as many lanes as there are leaves. The
+// x+1/2 and y+1/2 lanes form the flux derivatives with their partner faces, and lane 4p
+// performs the FP64 update. With LABEL the leaf's largest |w1 - w1_lin| goes to label.
+template <bool LABEL>
+__device__ __forceinline__ void full_block(int blk, const DevTables& t, const int32_t* __restrict__ lists,
+                                           const int* __restrict__ meta, const double4* __restrict__ Qin,
+                                           const float4* __restrict__ Qf_in, const double4* Qn, double4* Qout,
+                                           float4* __restrict__ Qf_out, const DevScalars* __restrict__ sc,
+                                           double alpha, double beta, int reduce, double* __restrict__ partials,
+                                           double* __restrict__ lampart, int poff, float* __restrict__ label) {
+    int n = t.N, base = 0;
+    if (lists) {
+        base = meta[0];
+        n = t.N - base;
+    }
+    const int nact = (4 * n + 255) >> 8;
+    if (blk >= nact) {
+        if (reduce) block_partials_zero(partials, poff + blk, lampart);
+        return;
+    }
+    const int lb = xcd_block_n(blk, nact);
+    const int tid = lb * 256 + (int)threadIdx.x;
+    const int p = tid >> 2, q = tid & 3, dir = q >> 1, side = q & 1;
+    const bool valid = p < n;
+    // every lane of an active block runs to the quad exchanges (invalid ones on leaf 0)
+    const int i = valid ? (lists ? lists[base + p] : p) : 0;
+    // stencil rows of i-2, i-1, i+1, i+2: (5, 1, 0, 4) in x, (7, 3, 2, 6) in y. Face i-1/2 takes
+    // points i-2, i-1, i, i+1; face i+1/2 takes i-1, i, i+1, i+2.
+    const int row0 = dir ? (side ? 3 : 7) : (side ? 1 : 5);
+    const int row1 = dir ? (side ? 2 : 3) : (side ? 0 : 1);
+    const int row2 = dir ? (side ? 6 : 2) : (side ? 4 : 0);
+    const int s0 = t.stencil[(size_t)row0 * t.N + i];
+    const int s1 = t.stencil[(size_t)row1 * t.N + i];
+    const int s2 = t.stencil[(size_t)row2 * t.N + i];
+    const int lv = t.lvl[i];
+    const float4 qa = Qf_in[s0], qb = Qf_in[s1], qc = Qf_in[i], qd = Qf_in[s2];
+    float4 col[4] = {qa, side ? qc : qb, side ? qb : qc, qd};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const float a = col[j].y, b = col[j].z;
+        col[j].y = dir ? b : a;
+        col[j].z = dir ? a : b;
+    }
+    const double gama = t.gamma;
+    const double dy = t.h / (double)(1 << lv);
+    const double dx = t.xf * dy;
+    const float gm1inv = 1.f / ((float)gama - 1.f);
+    PtF S[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) cons_flux_f<0>(col[j], gm1inv, S[j]);
+    float fl[4], dev = 0.f;
+    face_flux_f<LABEL>(S, (float)gama, dir ? (float)sc->lam_y : (float)sc->lam_x, fl, dev);
+    // lanes x+1/2 and y+1/2: (4/3)(F(+1/2) - F(-1/2))/d - (1/6)(F(i+1) - F(i-1))/d, the
+    // momentum components exchanged back in y
+    const float id = 1.f / (float)(dir ? dy : dx);
+    float dv[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float other = quad_lane<QP_XOR1>(fl[k]);
+        dv[k] = (4.f / 3.f) * id * (fl[k] - other) - (1.f / 6.f) * id * (S[2].F[k] - S[0].F[k]);
+    }
+    {
+        const float a = dv[1], b = dv[2];
+        dv[1] = dir ? b : a;
+        dv[2] = dir ? a : b;
+    }
+    float dE[4], dF[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        dE[k] = quad_lane<QP_XOR1>(dv[k]);  // lane 0 of the quad reads lane 1 (x+1/2)
+        dF[k] = quad_lane<QP_XOR3>(dv[k]);  // and lane 3 (y+1/2)
+    }
+    if (LABEL) {
+        dev = fmaxf(dev, quad_lane<QP_XOR1>(dev));
+        dev = fmaxf(dev, quad_lane<QP_XOR2>(dev));
+    }
+    double rmax[4] = {0, 0, 0, 0}, rsum[4] = {0, 0, 0, 0};
+    double lx = 0.0, ly = 0.0;
+    if (valid && q == 0) {
+        if (label) label[i] = dev;
+        update_leaf(i, Qin[i], Qn[i], dE, dF, dx, dy, gama, sc->dt, alpha, beta, Qout, Qf_out, rmax, rsum,
+                    lampart != nullptr, lx, ly);
     }
     if (reduce) block_partials(rmax, rsum, partials, poff + lb, lampart, lx, ly);
+}
+
+// One launch for both lists: blocks [0, nb4) run the troubled leaves (four lanes each), blocks
+// [nb4, nb4 + nb) the cleared ones, so the compute-bound characteristic work and the
+// memory-bound linear work share the machine instead of draining it in turn. Partials: the
+// full blocks at [0, nb4), the linear ones after them.
+template <bool LABEL>
+__global__ __launch_bounds__(256) void k_stage_mixed(DevTables t, const int32_t* __restrict__ lists,
+                                                     const int* __restrict__ meta, const double4* __restrict__ Qin,
+                                                     const float4* __restrict__ Qf_in, const double4* Qn,
+                                                     double4* Qout, float4* __restrict__ Qf_out,
+                                                     const DevScalars* __restrict__ sc, double alpha, double beta,
+                                                     int reduce, double* __restrict__ partials,
+                                                     double* __restrict__ lampart, int nb4, float* __restrict__ label) {
+    if ((int)blockIdx.x < nb4)
+        full_block<LABEL>(blockIdx.x, t, lists, meta, Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, reduce,
+                          partials, lampart, 0, label);
+    else
+        lin_block(blockIdx.x - nb4, t, lists, meta, Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, reduce,
+                  partials, lampart, nb4, label);
+}
+
+// every leaf on the full path (no flags)
+template <bool LABEL>
+__global__ __launch_bounds__(256) void k_stage_full(DevTables t, const double4* __restrict__ Qin,
+                                                    const float4* __restrict__ Qf_in, const double4* Qn,
+                                                    double4* Qout, float4* __restrict__ Qf_out,
+                                                    const DevScalars* __restrict__ sc, double alpha, double beta,
+                                                    int reduce, double* __restrict__ partials,
+                                                    double* __restrict__ lampart, float* __restrict__ label) {
+    full_block<LABEL>(blockIdx.x, t, nullptr, nullptr, Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, reduce,
+                      partials, lampart, 0, label);
 }
 
 // ---------------------------------------------------------------- leaf lists from flags
@@ -493,21 +561,25 @@ void launch_stage_fast(const DevTables& t, const int32_t* lists, const int* meta
                        const DevScalars* sc, double alpha, double beta, int reduce, double* partials,
                        double* lampart, float* label, hipStream_t st) {
     if (t.N <= 0) return;
-    const int nb = nb_(t.N, 256);
-    if (lists)
-        hipLaunchKernelGGL((k_stage_fast<false, false>), dim3(nb), dim3(256), 0, st, t, lists, meta, Qin, Qf_in,
-                           Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials, lampart, 0, label);
-    if (label)
-        hipLaunchKernelGGL((k_stage_fast<true, true>), dim3(nb), dim3(256), 0, st, t, lists, meta, Qin, Qf_in,
-                           Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials, lampart, lists ? nb : 0,
-                           label);
-    else
-        hipLaunchKernelGGL((k_stage_fast<true, false>), dim3(nb), dim3(256), 0, st, t, lists, meta, Qin, Qf_in,
-                           Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials, lampart, lists ? nb : 0,
-                           label);
+    const int nb = nb_(t.N, 256), nb4 = nb_(4 * t.N, 256);
+    if (lists) {
+        if (label)
+            hipLaunchKernelGGL(k_stage_mixed<true>, dim3(nb4 + nb), dim3(256), 0, st, t, lists, meta, Qin, Qf_in,
+                               Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials, lampart, nb4, label);
+        else
+            hipLaunchKernelGGL(k_stage_mixed<false>, dim3(nb4 + nb), dim3(256), 0, st, t, lists, meta, Qin, Qf_in,
+                               Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials, lampart, nb4, label);
+    } else if (label) {
+        hipLaunchKernelGGL(k_stage_full<true>, dim3(nb4), dim3(256), 0, st, t, Qin, Qf_in, Qn, Qout, Qf_out, sc,
+                           alpha, beta, reduce, partials, lampart, label);
+    } else {
+        hipLaunchKernelGGL(k_stage_full<false>, dim3(nb4), dim3(256), 0, st, t, Qin, Qf_in, Qn, Qout, Qf_out, sc,
+                           alpha, beta, reduce, partials, lampart, label);
+    }
 }
 
-int stage_fast_partial_blocks(int n, bool lists) { return (lists ? 2 : 1) * nb_(n, 256); }
+// the full blocks (four lanes per leaf), then the linear ones when lists are used
+int stage_fast_partial_blocks(int n, bool lists) { return (lists ? nb_(n, 256) : 0) + nb_(4 * n, 256); }
 
 void launch_build_lists(int n, const uint8_t* flags, int* blk_cnt, int32_t* lists, int* meta, hipStream_t st) {
     if (n <= 0) return;

@@ -464,65 +464,124 @@ __global__ __launch_bounds__(256) void k_wavespeed(DevTables t, const double4* _
     }
 }
 
-// reduce lambda partials; running max; dt from CFL (solution.F90:185-201); STEP_THREADS threads
+// one block finishes a step's reductions, over STEP_PARTS first-level entries
 constexpr int STEP_THREADS = 1024;
 
-__device__ void step_begin_block(const double* __restrict__ partials, int nblk, DevScalars* sc) {
-    __shared__ double sx[STEP_THREADS], sy[STEP_THREADS];
-    double ax = 0.0, ay = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
-        ax = fmax(ax, partials[2 * b]);
-        ay = fmax(ay, partials[2 * b + 1]);
+// lambda running maximum and dt from CFL (solution.F90:185-201), from the reduced wave speeds
+__device__ void step_begin_scalar(double lam_x, double lam_y, DevScalars* sc) {
+    if (sc->status != 0) return;
+    // lambda_x = max(lambda_x, ...) keeps the running maximum (solution.F90:189-190)
+    if (lam_x > sc->lam_x) sc->lam_x = lam_x;
+    if (lam_y > sc->lam_y) sc->lam_y = lam_y;
+    double dt = sc->dt_fixed;
+    if (sc->localdt) {
+        const double p2 = (double)(1 << sc->l_max);
+        dt = sc->cfl / (sc->lam_x / (sc->xf * sc->h / p2) + sc->lam_y / (sc->h / p2));
     }
-    sx[threadIdx.x] = ax; sy[threadIdx.x] = ay;
+    if (sc->tt + dt > sc->t_end) dt = sc->t_end - sc->tt;
+    sc->dt = dt;
+}
+
+// Block-wide reduction (STEP_THREADS threads) of per-block partials: NV values per entry, the
+// first NMAX reduced by maximum (non-negative values), the rest by sum. Every thread's loads
+// are issued four entries at a time; the result is valid in thread 0.
+template <int NV, int NMAX>
+__device__ void reduce_entries(const double* __restrict__ part, int n, double* out) {
+    __shared__ double red[STEP_THREADS / 64][NV];
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) acc[k] = 0.0;
+    for (int b0 = threadIdx.x; b0 < n; b0 += 4 * STEP_THREADS) {
+        double v[4][NV];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int b = b0 + u * STEP_THREADS;
+#pragma unroll
+            for (int k = 0; k < NV; k++) v[u][k] = b < n ? part[(size_t)NV * b + k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int k = 0; k < NV; k++) acc[k] = k < NMAX ? fmax(acc[k], v[u][k]) : acc[k] + v[u][k];
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        const double r = k < NMAX ? wave_reduce63<true>(acc[k]) : wave_reduce63<false>(acc[k]);
+        if (lane == 63) red[wv][k] = r;
+    }
     __syncthreads();
-    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) {
-            sx[threadIdx.x] = fmax(sx[threadIdx.x], sx[threadIdx.x + o]);
-            sy[threadIdx.x] = fmax(sy[threadIdx.x], sy[threadIdx.x + o]);
-        }
-        __syncthreads();
-    }
     if (threadIdx.x == 0) {
-        if (sc->status != 0) return;
-        // lambda_x = max(lambda_x, ...) keeps the running maximum (solution.F90:189-190)
-        if (sx[0] > sc->lam_x) sc->lam_x = sx[0];
-        if (sy[0] > sc->lam_y) sc->lam_y = sy[0];
-        double dt = sc->dt_fixed;
-        if (sc->localdt) {
-            const double p2 = (double)(1 << sc->l_max);
-            dt = sc->cfl / (sc->lam_x / (sc->xf * sc->h / p2) + sc->lam_y / (sc->h / p2));
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            double r = red[0][k];
+            for (int w = 1; w < STEP_THREADS / 64; w++) r = k < NMAX ? fmax(r, red[w][k]) : r + red[w][k];
+            out[k] = r;
         }
-        if (sc->tt + dt > sc->t_end) dt = sc->t_end - sc->tt;
-        sc->dt = dt;
     }
 }
 
 __global__ __launch_bounds__(STEP_THREADS) void k_step_begin(const double* __restrict__ partials, int nblk,
                                                              DevScalars* sc) {
-    step_begin_block(partials, nblk, sc);
+    double lam[2];
+    reduce_entries<2, 2>(partials, nblk, lam);
+    if (threadIdx.x == 0) step_begin_scalar(lam[0], lam[1], sc);
 }
 
-// residual finalize; tt advance; NaN status (solution.F90:205-214); with lampart, the next
-// step's begin (its wave speeds were reduced per block by the final stage)
-__global__ __launch_bounds__(STEP_THREADS) void k_step_end(const double* __restrict__ partials, int nblk,
-                                                           int n_leaf, int advance_time, DevScalars* sc,
-                                                           const double* __restrict__ lampart, int lam_nblk) {
-    __shared__ double sh[8][STEP_THREADS];
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int b = threadIdx.x; b < nblk; b += blockDim.x)
-        for (int k = 0; k < 8; k++) acc[k] = k < 4 ? fmax(acc[k], partials[8 * b + k]) : acc[k] + partials[8 * b + k];
-    for (int k = 0; k < 8; k++) sh[k][threadIdx.x] = acc[k];
-    __syncthreads();
-    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o)
-            for (int k = 0; k < 8; k++)
-                sh[k][threadIdx.x] = k < 4 ? fmax(sh[k][threadIdx.x], sh[k][threadIdx.x + o])
-                                           : sh[k][threadIdx.x] + sh[k][threadIdx.x + o];
-        __syncthreads();
+// First level of the step's reduction: STEP_PARTS blocks each fold a share of the stages'
+// per-block partials (residual max and sum of squares, and the final stage's wave speeds) into
+// one entry [rmax 0..3, lam_x, lam_y, rsum 0..3]. Only blocks that held leaves are read:
+// with meta (the fast path's list split) the full-path blocks [0, 4 n_troubled / 256) and the
+// linear blocks [lin0, lin0 + n_cleared / 256); without it, [0, nblk).
+__global__ __launch_bounds__(256) void k_reduce_parts(const double* __restrict__ partials,
+                                                      const double* __restrict__ lampart, int nblk,
+                                                      const int* __restrict__ meta, int n_leaf, int lin0,
+                                                      double* __restrict__ out) {
+    int n0 = nblk, n1 = 0;
+    if (meta) {
+        n0 = (4 * (n_leaf - meta[0]) + 255) >> 8;
+        n1 = (meta[0] + 255) >> 8;
     }
+    const int total = n0 + n1;
+    double acc[10];
+#pragma unroll
+    for (int k = 0; k < 10; k++) acc[k] = 0.0;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+        const int b = e < n0 ? e : lin0 + (e - n0);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            acc[k] = fmax(acc[k], partials[(size_t)b * 8 + k]);
+            acc[6 + k] = acc[6 + k] + partials[(size_t)b * 8 + 4 + k];
+        }
+        if (lampart) {
+            acc[4] = fmax(acc[4], lampart[(size_t)b * 2]);
+            acc[5] = fmax(acc[5], lampart[(size_t)b * 2 + 1]);
+        }
+    }
+    __shared__ double red[4][10];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 10; k++) {
+        const double r = k < 6 ? wave_reduce63<true>(acc[k]) : wave_reduce63<false>(acc[k]);
+        if (lane == 63) red[wv][k] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < 10) {
+        const int k = threadIdx.x;
+        double r = red[0][k];
+        for (int w = 1; w < 4; w++) r = k < 6 ? fmax(r, red[w][k]) : r + red[w][k];
+        out[blockIdx.x * 10 + k] = r;
+    }
+}
+
+// residual finalize; tt advance; NaN status (solution.F90:205-214); with prepare_next, the
+// next step's begin (its wave speeds came from the final stage)
+__global__ __launch_bounds__(STEP_THREADS) void k_step_end(const double* __restrict__ parts, int n_leaf,
+                                                           int advance_time, int prepare_next, DevScalars* sc) {
+    double v[10];
+    reduce_entries<10, 6>(parts, STEP_PARTS, v);
     if (threadIdx.x == 0) {
-        for (int k = 0; k < 4; k++) { sc->res_max[k] = sh[k][0]; sc->res_sum[k] = sh[4 + k][0]; }
+        for (int k = 0; k < 4; k++) { sc->res_max[k] = v[k]; sc->res_sum[k] = v[6 + k]; }
         if (advance_time && sc->status == 0) {
             sc->tt = sc->tt + sc->dt;
             sc->steps_done = sc->steps_done + 1;
@@ -532,10 +591,7 @@ __global__ __launch_bounds__(STEP_THREADS) void k_step_end(const double* __restr
                 if (r != r) sc->status = -3;
             }
         }
-    }
-    if (lampart) {
-        __syncthreads();
-        step_begin_block(lampart, lam_nblk, sc);
+        if (prepare_next) step_begin_scalar(v[4], v[5], sc);
     }
 }
 
@@ -623,12 +679,13 @@ void launch_step_begin(const double* partials, int nb, DevScalars* sc, hipStream
     hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(STEP_THREADS), 0, st, partials, nb, sc);
 }
 
-void launch_step_end(const double* partials, int nb, int n_leaf, int advance_time, DevScalars* sc,
-                     const double* lampart, int lam_nb, hipStream_t st) {
-    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(STEP_THREADS), 0, st, partials, nb, n_leaf, advance_time, sc,
-                       lampart, lam_nb);
+void launch_step_end(const double* partials, int nb, const int* meta, int lin0, int n_leaf, int advance_time,
+                     DevScalars* sc, const double* lampart, double* parts, hipStream_t st) {
+    hipLaunchKernelGGL(k_reduce_parts, dim3(STEP_PARTS), dim3(256), 0, st, partials, lampart, nb, meta, n_leaf,
+                       lin0, parts);
+    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(STEP_THREADS), 0, st, parts, n_leaf, advance_time,
+                       lampart != nullptr ? 1 : 0, sc);
 }
-
 void launch_sensors(const DevTables& t, const double4* Q, double* rot, double* div, hipStream_t st) {
     hipLaunchKernelGGL(k_sensors, dim3(nblk(t.N, 256)), dim3(256), 0, st, t, Q, rot, div);
 }
