@@ -41,6 +41,8 @@ struct fdmamr_ctx {
     DBuf<double4> Qint;      // restriction means of internal nodes (scratch, per stage)
     DBuf<double> Tint;
     DBuf<int32_t> int_child, rst_ref, bc_ref;
+    DBuf<int32_t> fill_beg, fill_leaf;  // flattened fill (fast path)
+    DBuf<float> fill_w;
     DBuf<float> wls_f;       // FP32 slot records (fast path)
     DBuf<uint8_t> flags;     // selector flags per interior leaf (bit 0: troubled)
     DBuf<float> label;       // per-leaf WCNS weight deviation (labels / judge)
@@ -186,6 +188,12 @@ int upload_tables(fdmamr_ctx* c) {
     HIPCHK(c, up(c->int_child.p, tb.int_child.data(), tb.int_child.size() * 4));
     HIPCHK(c, up(c->rst_ref.p, tb.rst_ref.data(), tb.rst_ref.size() * 4));
     HIPCHK(c, up(c->bc_ref.p, tb.bc_ref.data(), tb.bc_ref.size() * 4));
+    HIPCHK(c, ensure(c->fill_beg, tb.fill_beg.size()));
+    HIPCHK(c, ensure(c->fill_leaf, tb.fill_leaf.size()));
+    HIPCHK(c, ensure(c->fill_w, tb.fill_w.size()));
+    HIPCHK(c, up(c->fill_beg.p, tb.fill_beg.data(), tb.fill_beg.size() * 4));
+    HIPCHK(c, up(c->fill_leaf.p, tb.fill_leaf.data(), tb.fill_leaf.size() * 4));
+    HIPCHK(c, up(c->fill_w.p, tb.fill_w.data(), tb.fill_w.size() * 4));
     HIPCHK(c, ensure(c->off16, tb.off16.size()));
     HIPCHK(c, ensure(c->sel_prob, tb.N));
     if (c->p.flux_precision == FDMAMR_FLUX_FP32) {
@@ -214,6 +222,10 @@ int upload_tables(fdmamr_ctx* c) {
 
 void fill_after_stage(fdmamr_ctx* c, int buf) {
     const bool fast = c->p.flux_precision == FDMAMR_FLUX_FP32;
+    if (fast && c->tb.fill_flat_ok) {
+        launch_fill_flat(c->dt, c->Q[buf].p, c->Qf[buf].p, c->fill_beg.p, c->fill_leaf.p, c->fill_w.p, c->stream);
+        return;
+    }
     launch_restrict_fill(c->dt, c->tb.int_level_start, c->Q[buf].p, fast ? nullptr : c->T[buf].p,
                          fast ? c->Qf[buf].p : nullptr, c->Qint.p, c->Tint.p, c->stream);
 }
@@ -371,6 +383,7 @@ void fdmamr_destroy(fdmamr_ctx* c) {
     release(c->sel_prob); release(c->sel_pk); release(c->sel_cnt); release(c->ds_x); release(c->ds_y);
     release(c->ds_idx); release(c->lists); release(c->list_cnt); release(c->list_meta);
     release(c->tile_mask); release(c->tile_frag);
+    release(c->fill_beg); release(c->fill_leaf); release(c->fill_w);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->sc) (void)hipFree(c->sc);
     if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -151,5 +151,8 @@ void launch_list_write(int n, const uint8_t* flags, const int* blk_cnt, int32_t*
                        hipStream_t st);
 int list_blocks(int n);
 void launch_shadow(const double4* Q, float4* Qf, int n, hipStream_t st);
+// ghosts and restriction slots of the fast path in one pass (flattened weighted sums)
+void launch_fill_flat(const DevTables& t, double4* Q, float4* Qf, const int32_t* beg, const int32_t* leaf,
+                      const float* w, hipStream_t st);
 
 }  // namespace fdm

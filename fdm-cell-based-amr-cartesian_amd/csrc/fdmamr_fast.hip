@@ -540,6 +540,55 @@ __global__ __launch_bounds__(128) void k_slots_fast(DevTables t, const float* __
     Qf[t.L + s] = make_float4(out[0], out[1], out[2], out[3]);
 }
 
+// Ghost leaves and restriction slots of the fast path from the flattened lists: item k is
+// the weighted sum of leaf states in [beg[k], beg[k+1]) (leaf -1: the inflow state), written
+// to the FP64 state and its FP32 shadow. Sixteen lanes (one DPP row) share an item, so the
+// long lists of deeply refined partners do not serialise one thread.
+template <int CTRL>
+__device__ __forceinline__ double row_move(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)(b & 0xffffffffll), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+// sum over the 16 lanes of a DPP row, in every lane: quad xor 1, xor 2, row_ror 4, row_ror 8
+__device__ __forceinline__ double row_sum16(double v) {
+    v += row_move<0xB1>(v);
+    v += row_move<0x4E>(v);
+    v += row_move<0x124>(v);
+    v += row_move<0x128>(v);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_fill_flat(DevTables t, double4* __restrict__ Q, float4* __restrict__ Qf,
+                                                   const int32_t* __restrict__ beg, const int32_t* __restrict__ leaf,
+                                                   const float* __restrict__ w) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = tid >> 4, r = tid & 15;
+    const int items = t.G + t.S_r;
+    // every lane of a row reaches the row sums (rows past the last item sum nothing)
+    const int b = k < items ? beg[k] : 0, e = k < items ? beg[k + 1] : 0;
+    const bool inflow = k < items && leaf[b] < 0;
+    double ax = 0.0, ay = 0.0, az = 0.0, aw = 0.0;
+    if (!inflow)
+        for (int j = b + r; j < e; j += 16) {
+            const double4 q = Q[leaf[j]];
+            const double wj = (double)w[j];
+            ax += wj * q.x; ay += wj * q.y; az += wj * q.z; aw += wj * q.w;
+        }
+    ax = row_sum16(ax);
+    ay = row_sum16(ay);
+    az = row_sum16(az);
+    aw = row_sum16(aw);
+    if (k < items && r == 0) {
+        const double4 acc = inflow ? make_double4(t.inflow[0], t.inflow[1], t.inflow[2], t.inflow[3])
+                                   : make_double4(ax, ay, az, aw);
+        const int dst = k < t.G ? t.N + k : t.L + t.S_w + (k - t.G);
+        Q[dst] = acc;
+        Qf[dst] = make_float4((float)acc.x, (float)acc.y, (float)acc.z, (float)acc.w);
+    }
+}
+
 // FP32 shadow of every leaf state (after an upload)
 __global__ void k_shadow(const double4* __restrict__ Q, float4* __restrict__ Qf, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -595,6 +644,12 @@ void launch_list_write(int n, const uint8_t* flags, const int* blk_cnt, int32_t*
 }
 
 int list_blocks(int n) { return nb_(n, LIST_BLK); }
+
+void launch_fill_flat(const DevTables& t, double4* Q, float4* Qf, const int32_t* beg, const int32_t* leaf,
+                      const float* w, hipStream_t st) {
+    const int items = t.G + t.S_r;
+    if (items > 0) hipLaunchKernelGGL(k_fill_flat, dim3(nb_(16 * items, 256)), dim3(256), 0, st, t, Q, Qf, beg, leaf, w);
+}
 
 void launch_shadow(const double4* Q, float4* Qf, int n, hipStream_t st) {
     if (n > 0) hipLaunchKernelGGL(k_shadow, dim3(nb_(n, 256)), dim3(256), 0, st, Q, Qf, n);

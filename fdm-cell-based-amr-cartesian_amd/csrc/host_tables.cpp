@@ -446,6 +446,37 @@ void build_tables(const Tree& t, Tables& tb) {
         for (int s = 0; s < tb.S_r; s++) tb.rst_ref[s] = idx_of.at(tb.rst_node[s]);
         tb.bc_ref.resize(tb.G);
         for (int g = 0; g < tb.G; g++) tb.bc_ref[g] = tb.bc_partner[g] < 0 ? REF_INFLOW : ref(tb.bc_partner[g]);
+        // flattened fill (fast path)
+        const int items = tb.G + tb.S_r;
+        tb.fill_beg.assign(items + 1, 0);
+        tb.fill_leaf.clear();
+        tb.fill_w.clear();
+        std::vector<std::pair<int32_t, float>> work;
+        for (int k = 0; k < items; k++) {
+            tb.fill_beg[k] = (int)tb.fill_leaf.size();
+            const int32_t r = k < tb.G ? tb.bc_ref[k] : -(1 + tb.rst_ref[k - tb.G]);
+            if (r == REF_INFLOW) {
+                tb.fill_leaf.push_back(-1);
+                tb.fill_w.push_back(0.f);
+                continue;
+            }
+            work.assign(1, {r, 1.0f});
+            while (!work.empty()) {
+                const auto [q, w] = work.back();
+                work.pop_back();
+                if (q >= 0) {
+                    tb.fill_leaf.push_back(q);
+                    tb.fill_w.push_back(w);
+                } else {
+                    const size_t i = (size_t)(-(q + 1));
+                    for (int c = 3; c >= 0; c--) work.push_back({tb.int_child[4 * i + c], 0.25f * w});
+                }
+            }
+        }
+        tb.fill_beg[items] = (int)tb.fill_leaf.size();
+        tb.fill_flat_ok = true;
+        for (int32_t q : tb.fill_leaf)
+            if (q >= tb.N) tb.fill_flat_ok = false;
     }
 
     // off-axis points of the 5x5 selector stencil
@@ -485,6 +516,8 @@ void build_tables(const Tree& t, Tables& tb) {
         if (r >= tb.L || (r < 0 && -(r + 1) >= self))
             throw std::runtime_error("restriction child out of range");
     }
+    for (int32_t q : tb.fill_leaf)
+        if (q < -1 || q >= tb.L) throw std::runtime_error("flattened fill leaf out of range");
     for (int32_t r : tb.rst_ref)
         if (r < 0 || r >= tb.n_int) throw std::runtime_error("restriction slot ref out of range");
     for (int32_t r : tb.bc_ref)

@@ -78,6 +78,14 @@ struct Tables {
     std::vector<int32_t> int_level_start;  // group boundaries, deepest level first
     std::vector<int32_t> rst_ref;        // [S_r] internal index of each restriction slot
     std::vector<int32_t> bc_ref;         // [G]
+    // The same fill flattened for the fast path: item k (ghost k, then restriction slot
+    // k - G) is sum_j fill_w[j] * state[fill_leaf[j]] over j in [fill_beg[k], fill_beg[k+1]),
+    // the weights 4^-depth of the leaves below the referenced node; fill_leaf = -1 marks the
+    // inflow state. One pass instead of one per tree level; the order of the sum differs from
+    // the nested means, so the FP64 path keeps those.
+    std::vector<int32_t> fill_beg, fill_leaf;
+    std::vector<float> fill_w;
+    bool fill_flat_ok = true;  // every summed leaf is interior (no ghost read while written)
     // selector input: state index of the leaf containing each of the 16 off-axis points of
     // the leaf's 5x5 stencil (x + a dx, y + b dy), a,b in -2..2, a != 0, b != 0, row-major
     // in b then a. New rule (the reference has no 5x5 stencil): descend from the base cell
