@@ -283,8 +283,8 @@ __device__ __forceinline__ double isq_poly(double k1, double k2, double k3, cons
 __global__ __launch_bounds__(128) void k_slots_wls(DevTables t, double4* __restrict__ Q) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= t.S_w) return;
-    const double* __restrict__ d = t.wls_d + (size_t)s * WLS_STRIDE;
     const int32_t* __restrict__ iv = t.wls_i + (size_t)s * WLSI_STRIDE;
+    const double* __restrict__ d = t.wls_d + (size_t)iv[WLSI_PAT] * WLS_STRIDE;
     const int m = iv[WLSI_M], m1 = iv[WLSI_M1];
     const double xx = d[WLS_XX], yy = d[WLS_YY];
     const double* box = d + WLS_BOX;

@@ -356,31 +356,38 @@ void build_tables(const Tree& t, Tables& tb) {
     }
 
     // phase 3 (parallel): stencil lists and FP64 geometry of every WLS slot
-    tb.wls_d.assign((size_t)tb.S_w * WLS_STRIDE, 0.0);
+    std::vector<double> recd((size_t)tb.S_w * WLS_STRIDE, 0.0);
     tb.wls_i.assign((size_t)tb.S_w * WLSI_STRIDE, -1);
-    std::vector<float> rec((size_t)tb.S_w * WF_STRIDE, 0.0f);
     parallel_for((size_t)tb.S_w, [&](size_t b, size_t e) {
         for (size_t s = b; s < e; s++) {
             List l1, l2;
             const int tg = wls_node[s];
             if (wls_parent[s]) parent_lists(t, tg, l1, l2);
             else child_lists(t, tg, l1, l2);
-            double* dd = &tb.wls_d[s * WLS_STRIDE];
-            int32_t* iv = &tb.wls_i[s * WLSI_STRIDE];
-            wls_geometry(t, tg, wls_xx[s], wls_yy[s], l1, l2, tb.state_of_node, dd, iv);
-            fast_record(dd, iv, &rec[s * WF_STRIDE]);
+            wls_geometry(t, tg, wls_xx[s], wls_yy[s], l1, l2, tb.state_of_node, &recd[s * WLS_STRIDE],
+                         &tb.wls_i[s * WLSI_STRIDE]);
         }
     });
-    // FP32 records by pattern: in the cell-local frame a record depends only on the shape of
-    // the stencil around the level jump (coordinates in units of the cell size), so slots
-    // with the same shape have bit-identical records and share one, numbered in slot order.
-    // The fast path then reads a few hundred records from cache instead of one per slot.
+    // Records by pattern: in the cell-local frame a slot's geometry (coordinates in units of
+    // the cell size, weights, inverted normal matrices, smoothness box) depends only on the
+    // shape of the stencil around the level jump, so slots with the same shape have
+    // bit-identical records and share one, numbered in slot order; wls_i[.. + WLSI_PAT] names
+    // it. Both the FP64 records and the FP32 ones derived from them are stored once per
+    // pattern. In the reference's absolute frame every record differs and each slot keeps its
+    // own.
     {
+        auto same = [&](size_t a, size_t b2) {
+            const int32_t* ia = &tb.wls_i[a * WLSI_STRIDE];
+            const int32_t* ib = &tb.wls_i[b2 * WLSI_STRIDE];
+            return ia[WLSI_M] == ib[WLSI_M] && ia[WLSI_M1] == ib[WLSI_M1] &&
+                   std::memcmp(&recd[a * WLS_STRIDE], &recd[b2 * WLS_STRIDE], WLS_STRIDE * sizeof(double)) == 0;
+        };
         auto hash_rec = [&](size_t s) {
             uint64_t h = 1469598103934665603ull;
-            const uint32_t* w = reinterpret_cast<const uint32_t*>(&rec[s * WF_STRIDE]);
-            for (int k = 0; k < WF_STRIDE; k++) { h ^= w[k]; h *= 1099511628211ull; }
-            return h;
+            const uint32_t* w = reinterpret_cast<const uint32_t*>(&recd[s * WLS_STRIDE]);
+            for (int k = 0; k < 2 * WLS_STRIDE; k++) { h ^= w[k]; h *= 1099511628211ull; }
+            h ^= (uint64_t)tb.wls_i[s * WLSI_STRIDE + WLSI_M] * 131u + (uint64_t)tb.wls_i[s * WLSI_STRIDE + WLSI_M1];
+            return h * 1099511628211ull;
         };
         std::vector<uint64_t> hs(tb.S_w);
         parallel_for((size_t)tb.S_w, [&](size_t b, size_t e) {
@@ -392,8 +399,7 @@ void build_tables(const Tree& t, Tables& tb) {
             auto& bk = buckets[hs[s]];
             int pid = -1;
             for (int p : bk)
-                if (std::memcmp(&rec[(size_t)pat_slot[p] * WF_STRIDE], &rec[(size_t)s * WF_STRIDE],
-                                WF_STRIDE * sizeof(float)) == 0) { pid = p; break; }
+                if (same((size_t)pat_slot[p], (size_t)s)) { pid = p; break; }
             if (pid < 0) {
                 pid = (int)pat_slot.size();
                 pat_slot.push_back(s);
@@ -402,10 +408,13 @@ void build_tables(const Tree& t, Tables& tb) {
             tb.wls_i[(size_t)s * WLSI_STRIDE + WLSI_PAT] = pid;
         }
         tb.n_pat = (int)pat_slot.size();
+        tb.wls_d.assign((size_t)tb.n_pat * WLS_STRIDE, 0.0);
         tb.wls_f.assign((size_t)tb.n_pat * WF_STRIDE, 0.0f);
-        for (int p = 0; p < tb.n_pat; p++)
-            std::memcpy(&tb.wls_f[(size_t)p * WF_STRIDE], &rec[(size_t)pat_slot[p] * WF_STRIDE],
-                        WF_STRIDE * sizeof(float));
+        for (int p = 0; p < tb.n_pat; p++) {
+            const size_t s0 = (size_t)pat_slot[p];
+            std::memcpy(&tb.wls_d[(size_t)p * WLS_STRIDE], &recd[s0 * WLS_STRIDE], WLS_STRIDE * sizeof(double));
+            fast_record(&tb.wls_d[(size_t)p * WLS_STRIDE], &tb.wls_i[s0 * WLSI_STRIDE], &tb.wls_f[(size_t)p * WF_STRIDE]);
+        }
     }
     tb.bc_partner.resize(tb.G);
     for (int g = 0; g < tb.G; g++) tb.bc_partner[g] = t.bc_partner(t.bcl[g]);

@@ -26,7 +26,8 @@ enum WlsLayout {
     WLS_BOX = 83,    // 6: dxb, dyb, x2, y2, x3, y3
     WLS_STRIDE = 96
 };
-// WLSI_PAT: index of the slot's FP32 record (slots with bit-identical records share one)
+// WLSI_PAT: index of the slot's records in wls_d / wls_f (slots with bit-identical geometry
+// share them)
 enum WlsIntLayout { WLSI_M = 0, WLSI_M1 = 1, WLSI_IDS = 2, WLSI_PAT = 11, WLSI_STRIDE = 12 };
 
 // FP32 slot record of the fast path (floats): the least-squares solves folded into
@@ -63,7 +64,7 @@ struct Tables {
     std::vector<int32_t> st_node;        // [N*8]
     std::vector<double> st_px, st_py;    // [N*8]
     // slots
-    std::vector<double> wls_d;           // [S_w * WLS_STRIDE]
+    std::vector<double> wls_d;           // [n_pat * WLS_STRIDE], one per pattern
     std::vector<int32_t> wls_i;          // [S_w * WLSI_STRIDE]
     std::vector<float> wls_f;            // [n_pat * WF_STRIDE] fast-path records, one per pattern
     int n_pat = 0;
