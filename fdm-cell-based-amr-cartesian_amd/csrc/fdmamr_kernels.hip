@@ -288,18 +288,15 @@ __global__ __launch_bounds__(128) void k_slots_wls(DevTables t, double4* __restr
     const int m = iv[WLSI_M], m1 = iv[WLSI_M1];
     const double xx = d[WLS_XX], yy = d[WLS_YY];
     const double* box = d + WLS_BOX;
-    double val[9][4];
-    for (int i = 0; i < 9; i++) {
-        if (i < m) {
-            double4 q = Q[iv[WLSI_IDS + i]];
-            val[i][0] = q.x; val[i][1] = q.y; val[i][2] = q.z; val[i][3] = q.w;
-        }
-    }
+    const double* __restrict__ qd = reinterpret_cast<const double*>(Q);
     const double lw0 = 0.98, lw1 = 0.01, lw2 = 0.01;
     double out[4];
+    // one component at a time (not unrolled): the register footprint of one fit
+#pragma nounroll
     for (int k = 0; k < 4; k++) {
         double wphi[9];
-        for (int i = 0; i < 9; i++) wphi[i] = i < m ? d[WLS_W + i] * val[i][k] : 0.0;
+#pragma unroll
+        for (int i = 0; i < 9; i++) wphi[i] = i < m ? d[WLS_W + i] * qd[(size_t)iv[WLSI_IDS + i] * 4 + k] : 0.0;
         double b6[6] = {0, 0, 0, 0, 0, 0}, b3a[3] = {0, 0, 0}, b3b[3] = {0, 0, 0};
         for (int i = 0; i < 9; i++) {
             if (i >= m) break;
@@ -338,7 +335,11 @@ __global__ __launch_bounds__(128) void k_slots_wls(DevTables t, double4* __restr
         double a2 = lw2 * (1.0 + tau / (1e-6 + be2));
         double sa = a0 + a1 + a2;
         double o0 = a0 / sa, o1 = a1 / sa, o2 = a2 / sa;
-        out[k] = o0 * (1.0 / lw0 * vl - (lw1 / lw0 * vs1 + lw2 / lw0 * vs2)) + (o1 * vs1 + o2 * vs2);
+        const double o = o0 * (1.0 / lw0 * vl - (lw1 / lw0 * vs1 + lw2 / lw0 * vs2)) + (o1 * vs1 + o2 * vs2);
+        out[0] = k == 0 ? o : out[0];
+        out[1] = k == 1 ? o : out[1];
+        out[2] = k == 2 ? o : out[2];
+        out[3] = k == 3 ? o : out[3];
     }
     Q[t.L + s] = make_double4(out[0], out[1], out[2], out[3]);
 }
