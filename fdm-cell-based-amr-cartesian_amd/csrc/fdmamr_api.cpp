@@ -55,6 +55,8 @@ struct fdmamr_ctx {
 
     // learned selector
     DBuf<int32_t> off16;
+    DBuf<int32_t> sel_nbr_beg, sel_nbr;  // selector tile neighbourhoods
+    DBuf<uint16_t> sel_loc;
     DBuf<float> sel_w, sel_m, sel_v, sel_grad, sel_gpart, sel_lpart, sel_loss, sel_prob;
     DBuf<uint16_t> sel_pk;
     DBuf<int> sel_cnt;
@@ -202,6 +204,12 @@ int upload_tables(fdmamr_ctx* c) {
         HIPCHK(c, ensure(c->tile_frag, ntiles * 3 * 64));
     }
     HIPCHK(c, up(c->off16.p, tb.off16.data(), tb.off16.size() * 4));
+    HIPCHK(c, ensure(c->sel_nbr_beg, tb.sel_nbr_beg.size()));
+    HIPCHK(c, ensure(c->sel_nbr, tb.sel_nbr.size()));
+    HIPCHK(c, ensure(c->sel_loc, tb.sel_loc.size()));
+    HIPCHK(c, up(c->sel_nbr_beg.p, tb.sel_nbr_beg.data(), tb.sel_nbr_beg.size() * 4));
+    HIPCHK(c, up(c->sel_nbr.p, tb.sel_nbr.data(), tb.sel_nbr.size() * 4));
+    HIPCHK(c, up(c->sel_loc.p, tb.sel_loc.data(), tb.sel_loc.size() * 2));
     // every leaf troubled until the selector says otherwise
     if (tb.N > 0) HIPCHK(c, hipMemsetAsync(c->flags.p, 1, tb.N, s));
     launch_build_lists(tb.N, c->flags.p, c->list_cnt.p, c->lists.p, c->list_meta.p, s);
@@ -303,7 +311,8 @@ void enqueue_rk3(fdmamr_ctx* c, int advance_time, bool prepare_next) {
         if (select && k == 0) {
             SelectParams sp{c->sel_threshold, (uint32_t)(c->sel_judge * 65536.0f), c->sel_step++};
             launch_select(d, c->Qf[0].p, c->off16.p, c->sel_pk.p, c->sel_w.p, sp, c->flags.p, nullptr,
-                          c->tile_mask.p, c->tile_frag.p, 4 * c->n_cu, s);
+                          c->tile_mask.p, c->tile_frag.p, 4 * c->n_cu, c->sel_nbr_beg.p, c->sel_nbr.p, c->sel_loc.p,
+                          s);
             launch_build_lists(d.N, c->flags.p, c->list_cnt.p, c->lists.p, c->list_meta.p, s);
         }
         int e0 = -1, e1 = -1;
@@ -384,6 +393,7 @@ void fdmamr_destroy(fdmamr_ctx* c) {
     release(c->ds_idx); release(c->lists); release(c->list_cnt); release(c->list_meta);
     release(c->tile_mask); release(c->tile_frag);
     release(c->fill_beg); release(c->fill_leaf); release(c->fill_w);
+    release(c->sel_nbr_beg); release(c->sel_nbr); release(c->sel_loc);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->sc) (void)hipFree(c->sc);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -875,7 +885,8 @@ int32_t fdmamr_selector_infer(fdmamr_ctx* c, float* prob, int32_t* n_troubled) {
     HIPCHK(c, hipMemcpyAsync(c->sel_cnt.p, cnt0.data(), cnt0.size() * 4, hipMemcpyHostToDevice, c->stream));
     SelectParams sp{c->sel_threshold, 0u, c->sel_step};  // no judge sample here
     launch_select(c->dt, c->Qf[0].p, c->off16.p, c->sel_pk.p, c->sel_w.p, sp, c->flags.p, c->sel_prob.p,
-                  c->tile_mask.p, c->tile_frag.p, 4 * c->n_cu, c->stream);
+                  c->tile_mask.p, c->tile_frag.p, 4 * c->n_cu, c->sel_nbr_beg.p, c->sel_nbr.p, c->sel_loc.p,
+                  c->stream);
     // no judge sample was drawn: the judge pass only counts the troubled leaves
     launch_judge(N, c->label.p, c->sel_bound, c->flags.p, c->sel_cnt.p, c->stream);
     launch_build_lists(N, c->flags.p, c->list_cnt.p, c->lists.p, c->list_meta.p, c->stream);

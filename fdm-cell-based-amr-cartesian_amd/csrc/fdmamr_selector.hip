@@ -90,17 +90,12 @@ __device__ __forceinline__ void gather_idx(const DevTables& t, const int32_t* __
 // is at most UNIFORM_REL |c| (exactly uniform data included).
 constexpr float UNIFORM_REL = 1e-6f;
 
-__device__ __forceinline__ bool gather_values(const float4* __restrict__ Qf, int leaf, bool valid,
-                                              const int idx[7], float v[7][3]) {
+__device__ __forceinline__ bool features_from(const float4& qcr, const float4* q, bool valid, float v[7][3]) {
     const int g = (threadIdx.x & 63) >> 4;
-    const float4 qc = Qf[valid ? leaf : 0];
-    float4 q[7];
-#pragma unroll
-    for (int k = 0; k < 7; k++) q[k] = Qf[idx[k]];
     float c[3];
-    c[0] = valid ? qc.x : 0.f;
-    c[1] = valid ? qc.w : 0.f;
-    c[2] = valid ? __builtin_amdgcn_sqrtf(qc.y * qc.y + qc.z * qc.z) : 0.f;
+    c[0] = valid ? qcr.x : 0.f;
+    c[1] = valid ? qcr.w : 0.f;
+    c[2] = valid ? __builtin_amdgcn_sqrtf(qcr.y * qcr.y + qcr.z * qcr.z) : 0.f;
 #pragma unroll
     for (int k = 0; k < 7; k++) {
         const bool in = valid && g + 4 * k < 25;
@@ -122,6 +117,15 @@ __device__ __forceinline__ bool gather_values(const float4* __restrict__ Qf, int
         for (int k = 0; k < 7; k++) v[k][f] = g + 4 * k < 25 ? (v[k][f] - c[f]) * sc : 0.f;
     }
     return uniform;
+}
+
+__device__ __forceinline__ bool gather_values(const float4* __restrict__ Qf, int leaf, bool valid,
+                                              const int idx[7], float v[7][3]) {
+    const float4 qc = Qf[valid ? leaf : 0];
+    float4 q[7];
+#pragma unroll
+    for (int k = 0; k < 7; k++) q[k] = Qf[idx[k]];
+    return features_from(qc, q, valid, v);
 }
 
 __device__ __forceinline__ void gather_features(const DevTables& t, const float4* __restrict__ Qf,
@@ -297,18 +301,47 @@ __global__ __launch_bounds__(256) void k_select_gate(DevTables t, const float4* 
                                                      const int32_t* __restrict__ off16, SelectParams sp,
                                                      uint8_t* __restrict__ flags, float* __restrict__ prob,
                                                      uint16_t* __restrict__ tile_mask,
-                                                     uint4* __restrict__ tile_frag) {
+                                                     uint4* __restrict__ tile_frag,
+                                                     const int32_t* __restrict__ nbr_beg,
+                                                     const int32_t* __restrict__ nbr,
+                                                     const uint4* __restrict__ loc) {
+    __shared__ float4 sQ[4][SEL_NBR_CAP];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
     const int tile = xcd_block() * 4 + wv;
     const int ntiles = (t.N + 15) / 16;
-    if (tile >= ntiles) return;
+    const bool live = tile < ntiles;
     const int leaf = tile * 16 + r;
-    const bool valid = leaf < t.N;
-    int idx[7];
-    gather_idx(t, off16, leaf, valid, idx);
+    const bool valid = live && leaf < t.N;
+    // the tile's neighbourhood into LDS (every wave reaches the barrier)
+    int nU = SEL_NBR_CAP + 1, b = 0;
+    if (live) {
+        b = nbr_beg[tile];
+        nU = nbr_beg[tile + 1] - b;
+    }
+    const bool local = nU <= SEL_NBR_CAP;
+    const uint4 lc = loc[(size_t)(valid ? leaf : 0) * 4 + g];
     const uint8_t old = valid ? flags[leaf] : 0;
+    if (local)
+        for (int k = lane; k < nU; k += 64) sQ[wv][k] = Qf[nbr[b + k]];
+    __syncthreads();
+    if (!live) return;
     float v[7][3];
-    const bool uniform = gather_values(Qf, leaf, valid, idx, v);
+    bool uniform;
+    if (local) {
+        // positions are below nU (checked when the tables are built); leaves past N use entry 0
+        const uint16_t* pos = reinterpret_cast<const uint16_t*>(&lc);
+        float4 q[7];
+#pragma unroll
+        for (int k = 0; k < 7; k++) q[k] = sQ[wv][valid ? (int)pos[k] : 0];
+        // the centre is point 12 = lane g 0, k 3 of the same leaf
+        const int pc = __shfl(valid ? (int)pos[3] : 0, r);
+        const float4 qc = sQ[wv][pc];
+        uniform = features_from(qc, q, valid, v);
+    } else {
+        int idx[7];
+        gather_idx(t, off16, leaf, valid, idx);
+        uniform = gather_values(Qf, leaf, valid, idx, v);
+    }
     // mask of the tile's leaves that need the network (bit r), the same in every lane
     const uint64_t need = __ballot(valid && !uniform);
     const uint16_t mask = (uint16_t)(need & 0xFFFFu);
@@ -658,11 +691,12 @@ __global__ void k_pack(const float* __restrict__ w, uint16_t* __restrict__ pk) {
 // ---------------------------------------------------------------- launchers
 void launch_select(const DevTables& t, const float4* Qf, const int32_t* off16, const uint16_t* wpk,
                    const float* w32, const SelectParams& sp, uint8_t* flags, float* prob, uint16_t* tile_mask,
-                   uint4* tile_frag, int mlp_blocks, hipStream_t st) {
+                   uint4* tile_frag, int mlp_blocks, const int32_t* nbr_beg, const int32_t* nbr,
+                   const uint16_t* loc, hipStream_t st) {
     if (t.N <= 0) return;
     const int ntiles = (t.N + 15) / 16;
     hipLaunchKernelGGL(k_select_gate, dim3((ntiles + 3) / 4), dim3(256), 0, st, t, Qf, off16, sp, flags, prob,
-                       tile_mask, tile_frag);
+                       tile_mask, tile_frag, nbr_beg, nbr, reinterpret_cast<const uint4*>(loc));
     hipLaunchKernelGGL(k_select_mlp, dim3(mlp_blocks), dim3(256), 0, st, t.N, wpk, w32, sp, tile_mask, tile_frag,
                        flags, prob);
 }

@@ -490,32 +490,88 @@ void build_tables(const Tree& t, Tables& tb) {
 
     // off-axis points of the 5x5 selector stencil
     tb.off16.assign(16 * (size_t)N, 0);
-    for (int i = 0; i < N; i++) {
-        const int c = t.ns[i];
-        const double dy = t.h / pow2i(t.lvl[c]), dx = t.xf * dy;
-        int q = 0;
-        for (int b = -2; b <= 2; b++)
-            for (int a = -2; a <= 2; a++) {
-                if (a == 0 || b == 0) continue;
-                const double px = t.x[c] + a * dx, py = t.y[c] + b * dy;
-                int nj1 = (int)std::floor((px - t.Xmin) / (t.xf * t.h)) + 1;
-                int ni1 = (int)std::floor((py - t.Ymin) / t.h) + 1;
-                nj1 = std::min(std::max(nj1, 1), t.m);
-                ni1 = std::min(std::max(ni1, 1), t.n);
-                int node = (ni1 - 1) * t.m + nj1 - 1;
-                while (t.son[node] >= 0) {
-                    const bool rt = px > t.x[node], upq = py > t.y[node];
-                    node = t.son[node] + (rt && upq ? 2 : rt ? 1 : upq ? 3 : 0);
+    parallel_for((size_t)N, [&](size_t b0, size_t e0) {
+        for (size_t ii = b0; ii < e0; ii++) {
+            const int i = (int)ii;
+            const int c = t.ns[i];
+            const double dy = t.h / pow2i(t.lvl[c]), dx = t.xf * dy;
+            int q = 0;
+            for (int b = -2; b <= 2; b++)
+                for (int a = -2; a <= 2; a++) {
+                    if (a == 0 || b == 0) continue;
+                    const double px = t.x[c] + a * dx, py = t.y[c] + b * dy;
+                    int nj1 = (int)std::floor((px - t.Xmin) / (t.xf * t.h)) + 1;
+                    int ni1 = (int)std::floor((py - t.Ymin) / t.h) + 1;
+                    nj1 = std::min(std::max(nj1, 1), t.m);
+                    ni1 = std::min(std::max(ni1, 1), t.n);
+                    int node = (ni1 - 1) * t.m + nj1 - 1;
+                    while (t.son[node] >= 0) {
+                        const bool rt = px > t.x[node], upq = py > t.y[node];
+                        node = t.son[node] + (rt && upq ? 2 : rt ? 1 : upq ? 3 : 0);
+                    }
+                    tb.off16[(size_t)q * N + i] = tb.state_of_node[node];
+                    q++;
                 }
-                tb.off16[(size_t)q * N + i] = tb.state_of_node[node];
-                q++;
+        }
+    });
+    // selector tile neighbourhoods (see Tables::sel_nbr)
+    {
+        const int ntiles = (N + 15) / 16;
+        auto point_src = [&](int i, int p) -> int32_t {
+            if (p >= 25 || p == 12) return i;
+            const int a = p % 5 - 2, b = p / 5 - 2;
+            if (b == 0) return tb.stencil[(size_t)(a == 1 ? 0 : a == -1 ? 1 : a == 2 ? 4 : 5) * N + i];
+            if (a == 0) return tb.stencil[(size_t)(b == 1 ? 2 : b == -1 ? 3 : b == 2 ? 6 : 7) * N + i];
+            const int q16 = (b + 2 - (b > 0 ? 1 : 0)) * 4 + (a + 2 - (a > 0 ? 1 : 0));
+            return tb.off16[(size_t)q16 * N + i];
+        };
+        std::vector<std::vector<int32_t>> nbr(ntiles);
+        tb.sel_loc.assign((size_t)N * 32, 0);
+        parallel_for((size_t)ntiles, [&](size_t b0, size_t e0) {
+            // open-addressing set of at most 16 x 28 keys (1024 slots, cleared after each tile)
+            std::vector<int32_t> key(1024, -1), val(1024, 0), used;
+            for (size_t T = b0; T < e0; T++) {
+                std::vector<int32_t>& U = nbr[T];
+                U.clear();
+                const int i0 = (int)T * 16, i1 = std::min(N, i0 + 16);
+                for (int i = i0; i < i1; i++)
+                    for (int g = 0; g < 4; g++)
+                        for (int j = 0; j < 7; j++) {
+                            const int32_t src = point_src(i, g + 4 * j);
+                            uint32_t h = ((uint32_t)src * 2654435761u) >> 22;
+                            while (key[h] != -1 && key[h] != src) h = (h + 1) & 1023;
+                            if (key[h] == -1) {
+                                key[h] = src;
+                                val[h] = (int)U.size();
+                                U.push_back(src);
+                                used.push_back((int)h);
+                            }
+                            tb.sel_loc[(size_t)i * 32 + g * 8 + j] = (uint16_t)std::min(val[h], 65535);
+                        }
+                for (int h : used) key[h] = -1;
+                used.clear();
             }
+        });
+        tb.sel_nbr_beg.assign(ntiles + 1, 0);
+        tb.sel_nbr.clear();
+        for (int T = 0; T < ntiles; T++) {
+            tb.sel_nbr_beg[T] = (int)tb.sel_nbr.size();
+            if ((int)nbr[T].size() <= SEL_NBR_CAP) tb.sel_nbr.insert(tb.sel_nbr.end(), nbr[T].begin(), nbr[T].end());
+            else tb.sel_nbr.resize(tb.sel_nbr.size() + SEL_NBR_CAP + 1, 0);  // marker: gathered directly
+        }
+        tb.sel_nbr_beg[ntiles] = (int)tb.sel_nbr.size();
     }
-
-    // every index the kernels dereference is checked here, on the host
-    const long ext = (long)tb.L + tb.S_w + tb.S_r;
-    for (int32_t r : tb.stencil)
-        if (r < 0 || r >= ext) throw std::runtime_error("stencil index out of range");
+    for (int32_t r : tb.sel_nbr)
+        if (r < 0 || r >= tb.L + tb.S_w + tb.S_r) throw std::runtime_error("selector neighbourhood index out of range");
+    for (int T = 0; T + 1 < (int)tb.sel_nbr_beg.size(); T++) {
+        const int nU = tb.sel_nbr_beg[T + 1] - tb.sel_nbr_beg[T];
+        if (nU > SEL_NBR_CAP) continue;
+        for (int i = T * 16; i < std::min(N, T * 16 + 16); i++)
+            for (int g = 0; g < 4; g++)
+                for (int j = 0; j < 7; j++)
+                    if (tb.sel_loc[(size_t)i * 32 + g * 8 + j] >= nU)
+                        throw std::runtime_error("selector neighbourhood position out of range");
+    }
     for (int32_t r : tb.off16)
         if (r < 0 || r >= tb.L) throw std::runtime_error("selector stencil index out of range");
     for (size_t k = 0; k < tb.int_child.size(); k++) {
