@@ -108,7 +108,8 @@ def measure_traffic(args):
     child = [sys.executable, os.path.abspath(__file__), "--steps", "2", "--warmup", "1",
              "--train-iters", "20", "--no-cpu", "--no-traffic", "--traffic-child",
              "--copy-calibration", str(copy_mib), "--workload", args.workload,
-             "--refine-goal", str(args.refine_goal), "--advance", str(args.advance)]
+             "--refine-goal", str(args.refine_goal), "--advance", str(args.advance),
+             "--threshold", str(args.threshold), "--label-bound", str(args.label_bound)]
     res = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         d = tempfile.mkdtemp(prefix="fdmamr_pmc_")
@@ -159,11 +160,12 @@ def clone(cfg, tree, prim):
     return s
 
 
-def pretrain_selector(seed, rounds, iters):
+def pretrain_selector(seed, rounds, iters, threshold, label_bound):
     """Train on labels from plain FP32-flux runs of the vortex and Riemann configurations."""
     import fdmamr as F
     t0 = time.perf_counter()
     trainer = F.Solver(workload_config("riemann4", F.FLUX_FP32)).setup()
+    trainer.selector_config(threshold, 0.02, label_bound)
     trainer.selector_init(seed)
     rows = pos = 0
     for _ in range(rounds):
@@ -171,6 +173,7 @@ def pretrain_selector(seed, rounds, iters):
         pos += p
         trainer.run_steps(4)
     vort = F.Solver(workload_config("vortex", F.FLUX_FP32)).setup()
+    vort.selector_config(threshold, 0.02, label_bound)
     w0 = trainer.selector_get_weights()
     vort.selector_set_weights(w0)
     for _ in range(2):
@@ -202,6 +205,10 @@ def main():
     ap.add_argument("--advance", type=int, default=0,
                     help="evolve the flow this many steps (regrid every 10) before measuring")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--threshold", type=float, default=0.5,
+                    help="selector probability above which a leaf takes the full path")
+    ap.add_argument("--label-bound", type=float, default=0.05,
+                    help="largest |w1 - w1_lin| of a leaf that still counts as cleared")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of this many MiB (scale for HBM counters)")
     ap.add_argument("--no-traffic", action="store_true",
@@ -229,7 +236,8 @@ def main():
 
     import fdmamr as F
 
-    w = pretrain_selector(7, rounds=6, iters=args.train_iters)
+    w = pretrain_selector(7, rounds=6, iters=args.train_iters, threshold=args.threshold,
+                          label_bound=args.label_bound)
 
     t0 = time.perf_counter()
     s64 = F.Solver(workload_config(args.workload, F.FLUX_FP64, refine_goal=args.refine_goal)).setup()
@@ -259,6 +267,7 @@ def main():
     # learned path: online fine-tune on this workload while the judge rejects > 1%
     sel = clone(workload_config(args.workload, F.FLUX_FP32, selector=1, refine_goal=args.refine_goal),
                 tree, prim0)
+    sel.selector_config(args.threshold, 0.02, args.label_bound)
     sel.selector_set_weights(w)
     sel.run_steps(args.warmup)
     st = sel.selector_stats(reset=True)
@@ -343,7 +352,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "n_leaf": N, "l_max": cnt["l_max"],
                    "wls_slots": cnt["n_slot_wls"], "refine_goal": args.refine_goal,
-                   "base": "1024x512" if args.workload == "shock4bubbles" else None},
+                   "base": "1024x512" if args.workload == "shock4bubbles" else None,
+                   "selector_threshold": args.threshold, "label_bound": args.label_bound},
         "fp64_value": ups64,
         "speedup_vs_fp64": ups / ups64,
         "l1_rho_vs_fp64": l1,

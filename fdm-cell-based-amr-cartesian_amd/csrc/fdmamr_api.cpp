@@ -41,7 +41,9 @@ struct fdmamr_ctx {
     DBuf<double4> Qint;      // restriction means of internal nodes (scratch, per stage)
     DBuf<double> Tint;
     DBuf<int32_t> int_child, rst_ref, bc_ref;
-    DBuf<int32_t> fill_beg, fill_leaf;  // flattened fill (fast path)
+    DBuf<int32_t> fill_beg, fill_leaf, fill_fast;  // flattened fill (fast path)
+    DBuf<int32_t> dep_hdr, dep_leaf;                // ghost entries of ghost-reading slots
+    DBuf<float> dep_w;
     DBuf<float> fill_w;
     DBuf<float> wls_f;       // FP32 slot records (fast path)
     DBuf<uint8_t> flags;     // selector flags per interior leaf (bit 0: troubled)
@@ -196,6 +198,16 @@ int upload_tables(fdmamr_ctx* c) {
     HIPCHK(c, up(c->fill_beg.p, tb.fill_beg.data(), tb.fill_beg.size() * 4));
     HIPCHK(c, up(c->fill_leaf.p, tb.fill_leaf.data(), tb.fill_leaf.size() * 4));
     HIPCHK(c, up(c->fill_w.p, tb.fill_w.data(), tb.fill_w.size() * 4));
+    HIPCHK(c, ensure(c->dep_hdr, tb.dep_hdr.size()));
+    HIPCHK(c, ensure(c->dep_leaf, tb.dep_leaf.size()));
+    HIPCHK(c, ensure(c->dep_w, tb.dep_w.size()));
+    if (!tb.dep_hdr.empty()) {
+        HIPCHK(c, up(c->dep_hdr.p, tb.dep_hdr.data(), tb.dep_hdr.size() * 4));
+        HIPCHK(c, up(c->dep_leaf.p, tb.dep_leaf.data(), tb.dep_leaf.size() * 4));
+        HIPCHK(c, up(c->dep_w.p, tb.dep_w.data(), tb.dep_w.size() * 4));
+    }
+    HIPCHK(c, ensure(c->fill_fast, std::max<size_t>(tb.fill_fast.size(), 1)));
+    if (!tb.fill_fast.empty()) HIPCHK(c, up(c->fill_fast.p, tb.fill_fast.data(), tb.fill_fast.size() * 4));
     HIPCHK(c, ensure(c->off16, tb.off16.size()));
     HIPCHK(c, ensure(c->sel_prob, tb.N));
     if (c->p.flux_precision == FDMAMR_FLUX_FP32) {
@@ -214,10 +226,13 @@ int upload_tables(fdmamr_ctx* c) {
     if (tb.N > 0) HIPCHK(c, hipMemsetAsync(c->flags.p, 1, tb.N, s));
     launch_build_lists(tb.N, c->flags.p, c->list_cnt.p, c->lists.p, c->list_meta.p, s);
     DevTables& d = c->dt;
-    d.N = tb.N; d.G = tb.G; d.P = tb.P; d.L = tb.L; d.S_w = tb.S_w; d.S_r = tb.S_r;
+    d.N = tb.N; d.G = tb.G; d.P = tb.P; d.L = tb.L; d.S_w = tb.S_w; d.S_r = tb.S_r; d.S_wp = tb.S_wp;
     d.gamma = t.gamma; d.cv = t.cv; d.h = t.h; d.xf = t.xf; d.Ma = t.Ma;
     for (int k = 0; k < 4; k++) d.inflow[k] = t.inflow[k];
     d.stencil = c->stencil.p; d.lvl = c->lvl.p; d.wls_d = c->wls_d.p; d.wls_i = c->wls_i.p;
+    d.dep_hdr = reinterpret_cast<const int4*>(c->dep_hdr.p);
+    d.dep_leaf = reinterpret_cast<const int4*>(c->dep_leaf.p);
+    d.dep_w = reinterpret_cast<const float4*>(c->dep_w.p);
     d.rst_node = c->rst.p; d.bc_partner = c->bcp.p; d.node_son = c->son.p; d.node_state = c->state.p;
     d.n_int = tb.n_int; d.int_child = c->int_child.p; d.rst_ref = c->rst_ref.p; d.bc_ref = c->bc_ref.p;
     // ghosts and restriction slots of the uploaded state, FP32 shadows of all three buffers
@@ -236,6 +251,14 @@ void fill_after_stage(fdmamr_ctx* c, int buf) {
     }
     launch_restrict_fill(c->dt, c->tb.int_level_start, c->Q[buf].p, fast ? nullptr : c->T[buf].p,
                          fast ? c->Qf[buf].p : nullptr, c->Qint.p, c->Tint.p, c->stream);
+}
+
+// After a run of steps on the fast path with merged fills, every ghost and restriction slot
+// of the final state (buffer 0), so that the state seen from outside is the same as after a
+// fill per stage.
+void finish_steps(fdmamr_ctx* c) {
+    if (c->p.flux_precision == FDMAMR_FLUX_FP32 && c->tb.fill_flat_ok)
+        launch_fill_flat(c->dt, c->Q[0].p, c->Qf[0].p, c->fill_beg.p, c->fill_leaf.p, c->fill_w.p, c->stream);
 }
 
 // device state (interior + ghost leaves of buffer 0) -> host tree
@@ -299,14 +322,21 @@ void enqueue_rk3(fdmamr_ctx* c, int advance_time, bool prepare_next) {
     hipStream_t s = c->stream;
     const bool fast = c->p.flux_precision == FDMAMR_FLUX_FP32;
     // the learned selector runs once per step, on the step's input state, before stage 1;
-    // its judge sample takes the full path in stage 1 and is checked right after it
+    // its judge sample takes the full path in stage 1 and is checked in the same launch
     const bool select = fast && c->p.selector && c->sel_ready;
     // flagged fast path: cleared and troubled leaves run as separate launches over the lists
     // built from the flags; the judge's flag changes after stage 1 are not folded back into
     // the lists, so judged leaves stay on the full path for the whole step
     const int32_t* lists = fast && (c->use_flags || select) ? c->lists.p : nullptr;
+    // fast path with flattened fills: each stage's input gets its unaliased ghosts and
+    // restriction slots in the launch that evaluates its WLS slots (finish_steps completes
+    // the ghosts of the final state)
+    const bool merged = fast && c->tb.fill_flat_ok;
     for (int k = 0; k < 3; k++) {
-        if (fast) launch_slots_fast(d, c->wls_f.p, c->Q[in[k]].p, c->Qf[in[k]].p, s);
+        if (merged)
+            launch_slots_fill(d, c->wls_f.p, c->Q[in[k]].p, c->Qf[in[k]].p, c->fill_fast.p,
+                              (int)c->tb.fill_fast.size(), c->fill_beg.p, c->fill_leaf.p, c->fill_w.p, s);
+        else if (fast) launch_slots_fast(d, c->wls_f.p, c->Q[in[k]].p, c->Qf[in[k]].p, s);
         else launch_slots(d, c->Q[in[k]].p, c->T[in[k]].p, s);
         if (select && k == 0) {
             SelectParams sp{c->sel_threshold, (uint32_t)(c->sel_judge * 65536.0f), c->sel_step++};
@@ -321,7 +351,9 @@ void enqueue_rk3(fdmamr_ctx* c, int advance_time, bool prepare_next) {
             launch_stage_fast(d, lists, c->list_meta.p, c->Q[in[k]].p, c->Qf[in[k]].p, c->Q[0].p,
                               c->Q[out[k]].p, c->Qf[out[k]].p, c->sc, ra[k], rb[k], k == 2 ? 1 : 0,
                               c->partials.p, (prepare_next && k == 2) ? c->lampart.p : nullptr,
-                              (select && k == 0) ? c->label.p : nullptr, s);
+                              (select && k == 0) ? c->label.p : nullptr,
+                              (select && k == 0) ? JudgeArgs{c->flags.p, c->sel_cnt.p, c->sel_bound} : JudgeArgs{},
+                              s);
         else
             launch_stage(d, c->Q[in[k]].p, c->Q[0].p, c->T[0].p, c->Q[out[k]].p, c->T[out[k]].p, c->sc, ra[k],
                          rb[k], k == 2 ? 1 : 0, c->partials.p, (prepare_next && k == 2) ? c->lampart.p : nullptr,
@@ -330,8 +362,7 @@ void enqueue_rk3(fdmamr_ctx* c, int advance_time, bool prepare_next) {
             (void)hipEventRecord(c->ev_pool[e1], s);
             c->ev_stage.emplace_back(e0, e1);
         }
-        if (select && k == 0) launch_judge(d.N, c->label.p, c->sel_bound, c->flags.p, c->sel_cnt.p, s);
-        fill_after_stage(c, out[k]);
+        if (!merged) fill_after_stage(c, out[k]);
     }
     const int npart = fast ? stage_fast_partial_blocks(d.N, lists != nullptr) : nblocks(d.N, 256);
     launch_step_end(c->partials.p, npart, lists ? c->list_meta.p : nullptr, nblocks(4 * d.N, 256), d.N, advance_time,
@@ -392,7 +423,8 @@ void fdmamr_destroy(fdmamr_ctx* c) {
     release(c->sel_prob); release(c->sel_pk); release(c->sel_cnt); release(c->ds_x); release(c->ds_y);
     release(c->ds_idx); release(c->lists); release(c->list_cnt); release(c->list_meta);
     release(c->tile_mask); release(c->tile_frag);
-    release(c->fill_beg); release(c->fill_leaf); release(c->fill_w);
+    release(c->fill_beg); release(c->fill_leaf); release(c->fill_w); release(c->fill_fast);
+    release(c->dep_hdr); release(c->dep_leaf); release(c->dep_w);
     release(c->sel_nbr_beg); release(c->sel_nbr); release(c->sel_loc);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->sc) (void)hipFree(c->sc);
@@ -623,6 +655,7 @@ int32_t fdmamr_advance_rk3(fdmamr_ctx* c, double dt, double lam_x, double lam_y,
     s.dt = dt; s.lam_x = lam_x; s.lam_y = lam_y; s.h = c->tree.h; s.xf = c->tree.xf;
     HIPCHK(c, hipMemcpyAsync(c->sc, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
     enqueue_rk3(c, 0, false);
+    finish_steps(c);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(&s, c->sc, sizeof s, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -656,6 +689,7 @@ int32_t fdmamr_run_steps(fdmamr_ctx* c, int32_t n_steps, int32_t localdt, double
         }
         enqueue_rk3(c, 1, it + 1 < n_steps);
     }
+    if (n_steps > 0 && c->tb.N > 0) finish_steps(c);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(&s, c->sc, sizeof s, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -794,7 +828,7 @@ int32_t fdmamr_labels(fdmamr_ctx* c, float* dev) {
     launch_shadow(c->Q[0].p, c->Qf[0].p, c->tb.L + c->tb.S_w + c->tb.S_r, c->stream);
     launch_slots_fast(c->dt, c->wls_f.p, c->Q[0].p, c->Qf[0].p, c->stream);
     launch_stage_fast(c->dt, nullptr, nullptr, c->Q[0].p, c->Qf[0].p, c->Q[0].p, c->Q[2].p, c->Qf[2].p,
-                      c->sc, 0.0, 1.0, 0, c->partials.p, nullptr, c->label.p, c->stream);
+                      c->sc, 0.0, 1.0, 0, c->partials.p, nullptr, c->label.p, JudgeArgs{}, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(dev, c->label.p, c->tb.N * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -955,7 +989,7 @@ int32_t fdmamr_selector_collect(fdmamr_ctx* c, int32_t* n_rows, int32_t* n_posit
     launch_shadow(c->Q[0].p, c->Qf[0].p, c->tb.L + c->tb.S_w + c->tb.S_r, c->stream);
     launch_slots_fast(c->dt, c->wls_f.p, c->Q[0].p, c->Qf[0].p, c->stream);
     launch_stage_fast(c->dt, nullptr, nullptr, c->Q[0].p, c->Qf[0].p, c->Q[0].p, c->Q[2].p, c->Qf[2].p,
-                      c->sc, 0.0, 1.0, 0, c->partials.p, nullptr, c->label.p, c->stream);
+                      c->sc, 0.0, 1.0, 0, c->partials.p, nullptr, c->label.p, JudgeArgs{}, c->stream);
     launch_features(c->dt, c->Qf[0].p, c->off16.p, c->ds_x.p, c->ds_rows, c->stream);
     launch_labels_to_y(N, c->label.p, c->sel_bound, c->ds_y.p, c->ds_rows, c->stream);
     HIPCHK(c, hipGetLastError());
@@ -1074,6 +1108,36 @@ int32_t fdmamr_profile_read(fdmamr_ctx* c, double* stage_ms_total, int32_t* stag
     if (stage_launches) *stage_launches = (int32_t)c->ev_stage.size();
     c->ev_stage.clear();
     c->ev_used = 0;
+    return 0;
+}
+
+int32_t fdmamr_stencil_values_fast(fdmamr_ctx* c, int32_t i, float* val8x4) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (c->p.flux_precision != FDMAMR_FLUX_FP32) return fail(c, -1, "fdmamr_stencil_values_fast: FP32 contexts only");
+    if (i < 0 || i >= c->tb.N || !val8x4) return fail(c, -1, "fdmamr_stencil_values_fast: bad leaf");
+    // the slot and ghost work that precedes a stage, on state 0 (its values are unchanged)
+    if (c->tb.fill_flat_ok)
+        launch_slots_fill(c->dt, c->wls_f.p, c->Q[0].p, c->Qf[0].p, c->fill_fast.p, (int)c->tb.fill_fast.size(),
+                          c->fill_beg.p, c->fill_leaf.p, c->fill_w.p, c->stream);
+    else
+        launch_slots_fast(c->dt, c->wls_f.p, c->Q[0].p, c->Qf[0].p, c->stream);
+    launch_stencil_values_f(c->dt, c->Qf[0].p, i, reinterpret_cast<float*>(c->tmp.p), c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(val8x4, c->tmp.p, 32 * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int32_t fdmamr_table_info(fdmamr_ctx* c, int32_t* info8) {
+    if (!c || !info8) return -1;
+    if (!c->tables_ready) return fail(c, -2, "fdmamr_table_info: tables not built");
+    const Tables& tb = c->tb;
+    int aliased = 0, generic = 0;
+    for (int32_t a : tb.ghost_alias) aliased += a >= 0;
+    for (size_t d = 0; d * 4 < tb.dep_hdr.size(); d++) generic += tb.dep_hdr[d * 4] == DEP_GENERIC;
+    const int32_t v[8] = {tb.S_wp, tb.S_w, aliased, (int32_t)tb.fill_fast.size(), generic, 0, 0, 0};
+    for (int k = 0; k < 8; k++) info8[k] = v[k];
     return 0;
 }
 

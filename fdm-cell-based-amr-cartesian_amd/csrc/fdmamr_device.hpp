@@ -89,6 +89,7 @@ __device__ __forceinline__ void block_partials_zero(double* partials, int blk, d
 // Device views of the flattened tables (all pointers are device memory).
 struct DevTables {
     int N, G, P, L, S_w, S_r;
+    int S_wp;                   // WLS slots [0, S_wp) read no ghost leaf
     double gamma, cv, h, xf, Ma;
     double inflow[4];
     const int32_t* stencil;     // [8][N]
@@ -103,6 +104,9 @@ struct DevTables {
     const int32_t* int_child;   // [n_int][4]
     const int32_t* rst_ref;     // [S_r]
     const int32_t* bc_ref;      // [G]
+    const int4* dep_hdr;        // [S_w - S_wp]: ghost points of the slots that read ghosts
+    const int4* dep_leaf;       // [S_w - S_wp][2]: DEP_ENT leaves each
+    const float4* dep_w;        // [S_w - S_wp][2]: their weights
 };
 
 // Scalars of the device-resident step loop (solution.F90:183-215).
@@ -135,12 +139,25 @@ void launch_stencil_values(const DevTables& t, const double4* Q, int i, double* 
 
 // fast path (fdmamr_fast.hip)
 void launch_slots_fast(const DevTables& t, const float* wf, const double4* Q, float4* Qf, hipStream_t st);
+// the fill items items[0, nitems) of the flattened lists and the WLS slots of the same state in
+// one launch (points of a slot that are ghost leaves are formed from their fill lists)
+void launch_slots_fill(const DevTables& t, const float* wf, double4* Q, float4* Qf, const int32_t* items,
+                       int nitems, const int32_t* beg, const int32_t* leaf, const float* w, hipStream_t st);
+// Selector judge folded into the labelling stage: flags == nullptr for none. cnt: the selector
+// counters (troubled, judged, rejected), bound: the label bound.
+struct JudgeArgs {
+    uint8_t* flags = nullptr;
+    int* cnt = nullptr;
+    float bound = 0.f;
+};
 // lists: cleared leaves then troubled leaves (meta[0] = number cleared), or nullptr for the
-// full path on every leaf; residual partials occupy stage_fast_partial_blocks() blocks
+// full path on every leaf; residual partials occupy stage_fast_partial_blocks() blocks. With
+// label, each troubled leaf's largest weight deviation goes to label, and with jd.flags (lists
+// only) the judge runs on them in the same launch.
 void launch_stage_fast(const DevTables& t, const int32_t* lists, const int* meta, const double4* Qin,
                        const float4* Qf_in, const double4* Qn, double4* Qout, float4* Qf_out,
                        const DevScalars* sc, double alpha, double beta, int reduce, double* partials,
-                       double* lampart, float* label, hipStream_t st);
+                       double* lampart, float* label, const JudgeArgs& jd, hipStream_t st);
 int stage_fast_partial_blocks(int n, bool lists);
 // leaf lists from selector flags (bit 0: troubled); blk_cnt holds list_blocks(n) ints, the
 // troubled count of each block of LIST_BLK leaves. launch_list_write is the second pass alone,
@@ -151,6 +168,7 @@ void launch_list_write(int n, const uint8_t* flags, const int* blk_cnt, int32_t*
                        hipStream_t st);
 int list_blocks(int n);
 void launch_shadow(const double4* Q, float4* Qf, int n, hipStream_t st);
+void launch_stencil_values_f(const DevTables& t, const float4* Qf, int i, float* out, hipStream_t st);
 // ghosts and restriction slots of the fast path in one pass (flattened weighted sums)
 void launch_fill_flat(const DevTables& t, double4* Q, float4* Qf, const int32_t* beg, const int32_t* leaf,
                       const float* w, hipStream_t st);

@@ -16,6 +16,7 @@
 #include <stdint.h>
 
 #include "fdmamr_device.hpp"
+#include "fdmamr_selector.hpp"
 
 namespace fdm {
 
@@ -279,12 +280,15 @@ __device__ __forceinline__ void full_block(int blk, const DevTables& t, const in
                                            const float4* __restrict__ Qf_in, const double4* Qn, double4* Qout,
                                            float4* __restrict__ Qf_out, const DevScalars* __restrict__ sc,
                                            double alpha, double beta, int reduce, double* __restrict__ partials,
-                                           double* __restrict__ lampart, int poff, float* __restrict__ label) {
+                                           double* __restrict__ lampart, int poff, float* __restrict__ label,
+                                           const JudgeArgs& jd) {
     int n = t.N, base = 0;
     if (lists) {
         base = meta[0];
         n = t.N - base;
     }
+    // every leaf on this path counts as troubled for the selector statistics
+    if (LABEL && jd.flags && blk == 0 && threadIdx.x == 0 && n > 0) atomicAdd(&jd.cnt[CNT_TROUBLED], n);
     const int nact = (4 * n + 255) >> 8;
     if (blk >= nact) {
         if (reduce) block_partials_zero(partials, poff + blk, lampart);
@@ -353,6 +357,25 @@ __device__ __forceinline__ void full_block(int blk, const DevTables& t, const in
         update_leaf(i, Qin[i], Qn[i], dE, dF, dx, dy, gama, sc->dt, alpha, beta, Qout, Qf_out, rmax, rsum,
                     lampart != nullptr, lx, ly);
     }
+    if (LABEL && jd.flags) {
+        // the judge: a judged leaf (cleared by the network, sampled onto this path) whose label
+        // exceeds the bound is forced troubled from the next step on
+        bool judged = false, rej = false;
+        if (valid && q == 0) {
+            const uint8_t f = jd.flags[i];
+            if (f & 2) {
+                judged = true;
+                rej = dev > jd.bound;
+                jd.flags[i] = rej ? (uint8_t)5 : (uint8_t)(f & 4);
+            }
+        }
+        const uint64_t bj = __ballot(judged), br = __ballot(rej);
+        if ((threadIdx.x & 63) == 0 && bj) {
+            int* slot = jd.cnt + (blk % CNT_SLOTS) * CNT_N;
+            atomicAdd(&slot[CNT_JUDGED], (int)__popcll(bj));
+            if (br) atomicAdd(&slot[CNT_REJECTED], (int)__popcll(br));
+        }
+    }
     if (reduce) block_partials(rmax, rsum, partials, poff + lb, lampart, lx, ly);
 }
 
@@ -367,10 +390,11 @@ __global__ __launch_bounds__(256) void k_stage_mixed(DevTables t, const int32_t*
                                                      double4* Qout, float4* __restrict__ Qf_out,
                                                      const DevScalars* __restrict__ sc, double alpha, double beta,
                                                      int reduce, double* __restrict__ partials,
-                                                     double* __restrict__ lampart, int nb4, float* __restrict__ label) {
+                                                     double* __restrict__ lampart, int nb4, float* __restrict__ label,
+                                                     JudgeArgs jd) {
     if ((int)blockIdx.x < nb4)
         full_block<LABEL>(blockIdx.x, t, lists, meta, Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, reduce,
-                          partials, lampart, 0, label);
+                          partials, lampart, 0, label, jd);
     else
         lin_block(blockIdx.x - nb4, t, lists, meta, Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, reduce,
                   partials, lampart, nb4, label);
@@ -385,7 +409,7 @@ __global__ __launch_bounds__(256) void k_stage_full(DevTables t, const double4* 
                                                     int reduce, double* __restrict__ partials,
                                                     double* __restrict__ lampart, float* __restrict__ label) {
     full_block<LABEL>(blockIdx.x, t, nullptr, nullptr, Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, reduce,
-                      partials, lampart, 0, label);
+                      partials, lampart, 0, label, JudgeArgs{});
 }
 
 // ---------------------------------------------------------------- leaf lists from flags
@@ -466,16 +490,86 @@ __device__ __forceinline__ float isq_f(float k1, float k2, float k3, const float
            k2 * k2 * b[4] * (1.f / 3.f) * b[1] + 0.5f * k2 * k3 * b[2] * b[3] + k3 * k3 * b[0] * b[5] * (1.f / 3.f);
 }
 
-__global__ __launch_bounds__(128) void k_slots_fast(DevTables t, const float* __restrict__ wf,
-                                                    float4* __restrict__ Qf) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= t.S_w) return;
+// Value of fill item k (ghost k, or restriction slot k - G) from the flattened fill lists:
+// the weighted sum of its leaves' FP64 states, or the inflow state.
+__device__ __forceinline__ float4 fill_value(const DevTables& t, const double4* __restrict__ Q, int k,
+                                             const int32_t* __restrict__ beg, const int32_t* __restrict__ leaf,
+                                             const float* __restrict__ w) {
+    const int b = beg[k], e = beg[k + 1];
+    if (leaf[b] < 0) return make_float4((float)t.inflow[0], (float)t.inflow[1], (float)t.inflow[2], (float)t.inflow[3]);
+    double ax = 0.0, ay = 0.0, az = 0.0, aw = 0.0;
+    // four entries per round trip (the loads of a chunk are issued together), summed in order
+    for (int j0 = b; j0 < e; j0 += 4) {
+        int id[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) id[u] = j0 + u < e ? leaf[j0 + u] : -1;
+        double4 q[4];
+        double wj[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            q[u] = id[u] >= 0 ? Q[id[u]] : make_double4(0.0, 0.0, 0.0, 0.0);
+            wj[u] = id[u] >= 0 ? (double)w[j0 + u] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (id[u] >= 0) { ax += wj[u] * q[u].x; ay += wj[u] * q[u].y; az += wj[u] * q[u].z; aw += wj[u] * q[u].w; }
+    }
+    return make_float4((float)ax, (float)ay, (float)az, (float)aw);
+}
+
+// One WLS-WENO slot in FP32 (size word mw). GHOSTS (the launch that also writes the ghosts):
+// the points that are ghost leaves (the ghost mask of mw) are formed from their fill lists
+// instead of read. A separate instance, so that the common one issues all its gathers at once.
+template <bool GHOSTS>
+__device__ __forceinline__ void slot_fast(const DevTables& t, const float* __restrict__ wf, const double4* __restrict__ Q,
+                                          float4* __restrict__ Qf, int s, int mw, const int32_t* fl_beg,
+                                          const int32_t* fl_leaf, const float* fl_w) {
     const int32_t* __restrict__ iv = t.wls_i + (size_t)s * WLSI_STRIDE;
     const float* __restrict__ f = wf + (size_t)iv[WLSI_PAT] * WF_STRIDE;
-    const int m = iv[WLSI_M], m1 = iv[WLSI_M1];
+    const int m1 = iv[WLSI_M1];
+    const int m = mw & WLSI_MMASK, gm = mw >> WLSI_GSHIFT;
     float4 val[9];
+    if (!GHOSTS) {
 #pragma unroll
-    for (int i = 0; i < 9; i++) val[i] = i < m ? Qf[iv[WLSI_IDS + i]] : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int i = 0; i < 9; i++) val[i] = i < m ? Qf[iv[WLSI_IDS + i]] : make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+        const int d = s - t.S_wp;
+        const int4 hdr = t.dep_hdr[d];
+        if (hdr.x != DEP_GENERIC) {
+            // up to two ghost points from their entry lists (loads independent of the slot's
+            // own indices, so both gathers go out together)
+            float4 g[2];
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                const int4 l = t.dep_leaf[2 * d + p];
+                const float4 w = t.dep_w[2 * d + p];
+                const int li[4] = {l.x, l.y, l.z, l.w};
+                const float wi[4] = {w.x, w.y, w.z, w.w};
+                float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const float4 q = li[u] >= 0 ? Qf[li[u]]
+                                     : li[u] == DEP_INFLOW ? make_float4((float)t.inflow[0], (float)t.inflow[1],
+                                                                         (float)t.inflow[2], (float)t.inflow[3])
+                                                           : make_float4(0.f, 0.f, 0.f, 0.f);
+                    acc.x += wi[u] * q.x; acc.y += wi[u] * q.y; acc.z += wi[u] * q.z; acc.w += wi[u] * q.w;
+                }
+                g[p] = acc;
+            }
+#pragma unroll
+            for (int i = 0; i < 9; i++)
+                val[i] = hdr.x > 0 && i == hdr.y   ? g[0]
+                         : hdr.x > 1 && i == hdr.z ? g[1]
+                         : i < m && !((gm >> i) & 1) ? Qf[iv[WLSI_IDS + i]]
+                                                     : make_float4(0.f, 0.f, 0.f, 0.f);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 9; i++) {
+                if ((gm >> i) & 1) val[i] = fill_value(t, Q, iv[WLSI_IDS + i] - t.N, fl_beg, fl_leaf, fl_w);
+                else val[i] = i < m ? Qf[iv[WLSI_IDS + i]] : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+    }
     const float* box = f + WF_BOX;
     const float area = box[0] * box[1];
     float out[4];
@@ -540,6 +634,14 @@ __global__ __launch_bounds__(128) void k_slots_fast(DevTables t, const float* __
     Qf[t.L + s] = make_float4(out[0], out[1], out[2], out[3]);
 }
 
+__global__ __launch_bounds__(128) void k_slots_fast(DevTables t, const float* __restrict__ wf,
+                                                    float4* __restrict__ Qf) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    // ghosts are filled before this launch: the ghost mask is not used
+    if (s < t.S_w) slot_fast<false>(t, wf, nullptr, Qf, s, t.wls_i[(size_t)s * WLSI_STRIDE + WLSI_M] & WLSI_MMASK,
+                                    nullptr, nullptr, nullptr);
+}
+
 // Ghost leaves and restriction slots of the fast path from the flattened lists: item k is
 // the weighted sum of leaf states in [beg[k], beg[k+1]) (leaf -1: the inflow state), written
 // to the FP64 state and its FP32 shadow. Sixteen lanes (one DPP row) share an item, so the
@@ -560,15 +662,13 @@ __device__ __forceinline__ double row_sum16(double v) {
     return v;
 }
 
-__global__ __launch_bounds__(256) void k_fill_flat(DevTables t, double4* __restrict__ Q, float4* __restrict__ Qf,
-                                                   const int32_t* __restrict__ beg, const int32_t* __restrict__ leaf,
-                                                   const float* __restrict__ w) {
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = tid >> 4, r = tid & 15;
-    const int items = t.G + t.S_r;
-    // every lane of a row reaches the row sums (rows past the last item sum nothing)
-    const int b = k < items ? beg[k] : 0, e = k < items ? beg[k + 1] : 0;
-    const bool inflow = k < items && leaf[b] < 0;
+// item k on the 16 lanes r of a DPP row; every lane of the row reaches the row sums (rows
+// past the last item, live = false, sum nothing)
+__device__ __forceinline__ void fill_row(const DevTables& t, double4* __restrict__ Q, float4* __restrict__ Qf,
+                                         int k, bool live, int r, const int32_t* __restrict__ beg,
+                                         const int32_t* __restrict__ leaf, const float* __restrict__ w) {
+    const int b = live ? beg[k] : 0, e = live ? beg[k + 1] : 0;
+    const bool inflow = live && leaf[b] < 0;
     double ax = 0.0, ay = 0.0, az = 0.0, aw = 0.0;
     if (!inflow)
         for (int j = b + r; j < e; j += 16) {
@@ -580,12 +680,56 @@ __global__ __launch_bounds__(256) void k_fill_flat(DevTables t, double4* __restr
     ay = row_sum16(ay);
     az = row_sum16(az);
     aw = row_sum16(aw);
-    if (k < items && r == 0) {
+    if (live && r == 0) {
         const double4 acc = inflow ? make_double4(t.inflow[0], t.inflow[1], t.inflow[2], t.inflow[3])
                                    : make_double4(ax, ay, az, aw);
         const int dst = k < t.G ? t.N + k : t.L + t.S_w + (k - t.G);
         Q[dst] = acc;
         Qf[dst] = make_float4((float)acc.x, (float)acc.y, (float)acc.z, (float)acc.w);
+    }
+}
+
+// every item: ghost k for k < G, restriction slot k - G after them
+__global__ __launch_bounds__(256) void k_fill_flat(DevTables t, double4* __restrict__ Q, float4* __restrict__ Qf,
+                                                   const int32_t* __restrict__ beg, const int32_t* __restrict__ leaf,
+                                                   const float* __restrict__ w) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = tid >> 4;
+    fill_row(t, Q, Qf, k, k < t.G + t.S_r, tid & 15, beg, leaf, w);
+}
+
+// One launch before each fast-path stage: the first ngb blocks evaluate the few WLS slots that
+// read a ghost leaf, forming those points from the ghosts' fill lists (their longer chains
+// start first), the next nfb blocks refill the items the stage reads (the unaliased ghosts and
+// the restriction slots, items[0, nitems)), the rest evaluate the slots that read no ghost.
+// No block waits for another.
+__global__ __launch_bounds__(128) void k_slots_fill(
+    DevTables t, const float* __restrict__ wf, double4* Q, float4* Qf, int ngb, int nfb,
+    const int32_t* __restrict__ items, int nitems, const int32_t* __restrict__ beg,
+    const int32_t* __restrict__ leaf, const float* __restrict__ w) {
+    const int b = (int)blockIdx.x;
+    if (b < ngb) {
+        const int s = t.S_wp + b * 128 + (int)threadIdx.x;
+        if (s < t.S_w)
+            slot_fast<true>(t, wf, Q, Qf, s, t.wls_i[(size_t)s * WLSI_STRIDE + WLSI_M], beg, leaf, w);
+    } else if (b < ngb + nfb) {
+        const int tid = (b - ngb) * 128 + (int)threadIdx.x;
+        const int kk = tid >> 4;
+        const bool live = kk < nitems;
+        fill_row(t, Q, Qf, live ? items[kk] : 0, live, tid & 15, beg, leaf, w);
+    } else {
+        const int s = (b - ngb - nfb) * 128 + (int)threadIdx.x;
+        if (s < t.S_wp)
+            slot_fast<false>(t, wf, Q, Qf, s, t.wls_i[(size_t)s * WLSI_STRIDE + WLSI_M], nullptr, nullptr, nullptr);
+    }
+}
+
+// the FP32 values the fast path's stage reads for interior leaf i (8 stencil positions)
+__global__ void k_stencil_values_f(DevTables t, const float4* __restrict__ Qf, int i, float* out) {
+    const int k = threadIdx.x;
+    if (k < 8 && i < t.N) {
+        const float4 q = Qf[t.stencil[(size_t)k * t.N + i]];
+        out[4 * k + 0] = q.x; out[4 * k + 1] = q.y; out[4 * k + 2] = q.z; out[4 * k + 3] = q.w;
     }
 }
 
@@ -605,19 +749,28 @@ void launch_slots_fast(const DevTables& t, const float* wf, const double4* Q, fl
     if (t.S_w > 0) hipLaunchKernelGGL(k_slots_fast, dim3(nb_(t.S_w, 128)), dim3(128), 0, st, t, wf, Qf);
 }
 
+void launch_slots_fill(const DevTables& t, const float* wf, double4* Q, float4* Qf, const int32_t* items,
+                       int nitems, const int32_t* beg, const int32_t* leaf, const float* w, hipStream_t st) {
+    const int nfb = nb_(16 * nitems, 128), nsb = nb_(t.S_wp, 128), ngb = nb_(t.S_w - t.S_wp, 128);
+    if (nfb + nsb + ngb > 0)
+        hipLaunchKernelGGL(k_slots_fill, dim3(nfb + nsb + ngb), dim3(128), 0, st, t, wf, Q, Qf, ngb, nfb, items,
+                           nitems, beg, leaf, w);
+}
+
 void launch_stage_fast(const DevTables& t, const int32_t* lists, const int* meta, const double4* Qin,
                        const float4* Qf_in, const double4* Qn, double4* Qout, float4* Qf_out,
                        const DevScalars* sc, double alpha, double beta, int reduce, double* partials,
-                       double* lampart, float* label, hipStream_t st) {
+                       double* lampart, float* label, const JudgeArgs& jd, hipStream_t st) {
     if (t.N <= 0) return;
     const int nb = nb_(t.N, 256), nb4 = nb_(4 * t.N, 256);
     if (lists) {
         if (label)
             hipLaunchKernelGGL(k_stage_mixed<true>, dim3(nb4 + nb), dim3(256), 0, st, t, lists, meta, Qin, Qf_in,
-                               Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials, lampart, nb4, label);
+                               Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials, lampart, nb4, label, jd);
         else
             hipLaunchKernelGGL(k_stage_mixed<false>, dim3(nb4 + nb), dim3(256), 0, st, t, lists, meta, Qin, Qf_in,
-                               Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials, lampart, nb4, label);
+                               Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials, lampart, nb4, label,
+                               JudgeArgs{});
     } else if (label) {
         hipLaunchKernelGGL(k_stage_full<true>, dim3(nb4), dim3(256), 0, st, t, Qin, Qf_in, Qn, Qout, Qf_out, sc,
                            alpha, beta, reduce, partials, lampart, label);
@@ -644,6 +797,10 @@ void launch_list_write(int n, const uint8_t* flags, const int* blk_cnt, int32_t*
 }
 
 int list_blocks(int n) { return nb_(n, LIST_BLK); }
+
+void launch_stencil_values_f(const DevTables& t, const float4* Qf, int i, float* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_stencil_values_f, dim3(1), dim3(64), 0, st, t, Qf, i, out);
+}
 
 void launch_fill_flat(const DevTables& t, double4* Q, float4* Qf, const int32_t* beg, const int32_t* leaf,
                       const float* w, hipStream_t st) {

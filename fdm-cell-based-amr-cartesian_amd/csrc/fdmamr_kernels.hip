@@ -285,7 +285,8 @@ __global__ __launch_bounds__(128) void k_slots_wls(DevTables t, double4* __restr
     if (s >= t.S_w) return;
     const int32_t* __restrict__ iv = t.wls_i + (size_t)s * WLSI_STRIDE;
     const double* __restrict__ d = t.wls_d + (size_t)iv[WLSI_PAT] * WLS_STRIDE;
-    const int m = iv[WLSI_M], m1 = iv[WLSI_M1];
+    // ghosts are filled before this launch, so the ghost mask above the size byte is not used
+    const int m = iv[WLSI_M] & WLSI_MMASK, m1 = iv[WLSI_M1];
     const double xx = d[WLS_XX], yy = d[WLS_YY];
     const double* box = d + WLS_BOX;
     const double* __restrict__ qd = reinterpret_cast<const double*>(Q);

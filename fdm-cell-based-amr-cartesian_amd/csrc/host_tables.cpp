@@ -488,6 +488,91 @@ void build_tables(const Tree& t, Tables& tb) {
             if (q >= tb.N) tb.fill_flat_ok = false;
     }
 
+    // Ghosts whose boundary partner is an interior leaf hold an exact copy of that leaf's state
+    // after every fill, and all inflow ghosts hold the same inflow state, so every table reads
+    // the leaf itself, or the first inflow ghost, instead. The fills still write every ghost
+    // (the exported state keeps them), but no stage reads the aliased ones, and the fast
+    // path's per-stage fill covers only the rest: the first inflow ghost, restriction means
+    // and restriction slots (fill_fast). A WLS slot that still reads a ghost marks the point
+    // in the ghost mask of its size word, so that the fast path can form the value from the
+    // ghost's fill list in the same launch that writes the ghost.
+    tb.ghost_alias.assign(tb.G, -1);
+    int inflow0 = -1;
+    for (int g = 0; g < tb.G; g++) {
+        if (tb.bc_ref[g] >= 0 && tb.bc_ref[g] < tb.N) tb.ghost_alias[g] = tb.bc_ref[g];
+        else if (tb.bc_ref[g] == REF_INFLOW) {
+            if (inflow0 < 0) inflow0 = g;
+            else tb.ghost_alias[g] = tb.N + inflow0;
+        }
+    }
+    auto redirect = [&](int32_t r) -> int32_t {
+        return (r >= tb.N && r < tb.N + tb.G && tb.ghost_alias[r - tb.N] >= 0) ? tb.ghost_alias[r - tb.N] : r;
+    };
+    parallel_for(tb.stencil.size(), [&](size_t b0, size_t e0) {
+        for (size_t k = b0; k < e0; k++) tb.stencil[k] = redirect(tb.stencil[k]);
+    });
+    for (int s = 0; s < tb.S_w; s++) {
+        int32_t* iv = &tb.wls_i[(size_t)s * WLSI_STRIDE];
+        int gmask = 0;
+        for (int j = 0; j < iv[WLSI_M]; j++) {
+            int32_t& r = iv[WLSI_IDS + j];
+            r = redirect(r);
+            if (r >= tb.N && r < tb.N + tb.G) gmask |= 1 << j;
+        }
+        iv[WLSI_M] |= gmask << WLSI_GSHIFT;
+    }
+    // slots that read a ghost go last (slots [0, S_wp) read none), each part in its old order
+    {
+        std::vector<int32_t> newidx(tb.S_w);
+        int np = 0;
+        for (int s = 0; s < tb.S_w; s++)
+            if ((tb.wls_i[(size_t)s * WLSI_STRIDE + WLSI_M] >> WLSI_GSHIFT) == 0) newidx[s] = np++;
+        tb.S_wp = np;
+        for (int s = 0; s < tb.S_w; s++)
+            if ((tb.wls_i[(size_t)s * WLSI_STRIDE + WLSI_M] >> WLSI_GSHIFT) != 0) newidx[s] = np++;
+        if (tb.S_wp < tb.S_w) {
+            std::vector<int32_t> wi(tb.wls_i.size());
+            for (int s = 0; s < tb.S_w; s++)
+                std::memcpy(&wi[(size_t)newidx[s] * WLSI_STRIDE], &tb.wls_i[(size_t)s * WLSI_STRIDE],
+                            WLSI_STRIDE * sizeof(int32_t));
+            tb.wls_i.swap(wi);
+            for (auto& r : tb.stencil)
+                if (r >= tb.L && r < tb.L + tb.S_w) r = tb.L + newidx[r - tb.L];
+        }
+    }
+    // the ghost points of the slots [S_wp, S_w) as fixed-size entry lists, so that the fast
+    // path gathers them without walking the fill lists
+    {
+        const int D = tb.S_w - tb.S_wp;
+        tb.dep_hdr.assign((size_t)D * 4, 0);
+        tb.dep_leaf.assign((size_t)D * 2 * DEP_ENT, -1);
+        tb.dep_w.assign((size_t)D * 2 * DEP_ENT, 0.f);
+        for (int d = 0; d < D; d++) {
+            const int32_t* iv = &tb.wls_i[(size_t)(tb.S_wp + d) * WLSI_STRIDE];
+            const int gm = iv[WLSI_M] >> WLSI_GSHIFT;
+            int cnt = 0;
+            bool fits = true;
+            for (int j = 0; j < 9 && fits; j++) {
+                if (!((gm >> j) & 1)) continue;
+                const int k = iv[WLSI_IDS + j] - tb.N;
+                const int b0 = tb.fill_beg[k], e0 = tb.fill_beg[k + 1];
+                if (cnt == 2 || e0 - b0 > DEP_ENT) { fits = false; break; }
+                tb.dep_hdr[(size_t)d * 4 + 1 + cnt] = j;
+                for (int u = 0; u < e0 - b0; u++) {
+                    const size_t o = ((size_t)d * 2 + cnt) * DEP_ENT + u;
+                    const int32_t q = tb.fill_leaf[b0 + u];
+                    tb.dep_leaf[o] = q < 0 ? DEP_INFLOW : q;
+                    tb.dep_w[o] = q < 0 ? 1.f : tb.fill_w[b0 + u];
+                }
+                cnt++;
+            }
+            tb.dep_hdr[(size_t)d * 4] = fits ? cnt : DEP_GENERIC;
+        }
+    }
+    tb.fill_fast.clear();
+    for (int k = 0; k < tb.G + tb.S_r; k++)
+        if (k >= tb.G || tb.ghost_alias[k] < 0) tb.fill_fast.push_back(k);
+
     // off-axis points of the 5x5 selector stencil
     tb.off16.assign(16 * (size_t)N, 0);
     parallel_for((size_t)N, [&](size_t b0, size_t e0) {
@@ -509,7 +594,7 @@ void build_tables(const Tree& t, Tables& tb) {
                         const bool rt = px > t.x[node], upq = py > t.y[node];
                         node = t.son[node] + (rt && upq ? 2 : rt ? 1 : upq ? 3 : 0);
                     }
-                    tb.off16[(size_t)q * N + i] = tb.state_of_node[node];
+                    tb.off16[(size_t)q * N + i] = redirect(tb.state_of_node[node]);
                     q++;
                 }
         }
@@ -583,6 +668,23 @@ void build_tables(const Tree& t, Tables& tb) {
     }
     for (int32_t q : tb.fill_leaf)
         if (q < -1 || q >= tb.L) throw std::runtime_error("flattened fill leaf out of range");
+    for (int32_t k : tb.fill_fast)
+        if (k < 0 || k >= tb.G + tb.S_r) throw std::runtime_error("fast fill item out of range");
+    for (size_t d = 0; d * 4 < tb.dep_hdr.size(); d++) {
+        const int cnt = tb.dep_hdr[d * 4];
+        if (cnt == DEP_GENERIC) continue;
+        if (cnt < 1 || cnt > 2) throw std::runtime_error("ghost entry count out of range");
+        for (int p = 0; p < cnt; p++) {
+            if (tb.dep_hdr[d * 4 + 1 + p] < 0 || tb.dep_hdr[d * 4 + 1 + p] > 8) throw std::runtime_error("ghost point out of range");
+            for (int u = 0; u < DEP_ENT; u++) {
+                const int32_t q = tb.dep_leaf[(d * 2 + p) * DEP_ENT + u];
+                if (q != -1 && q != DEP_INFLOW && (q < 0 || q >= tb.N)) throw std::runtime_error("ghost entry out of range");
+            }
+        }
+    }
+    for (int32_t r : tb.stencil)
+        if (r >= tb.N && r < tb.N + tb.G && tb.ghost_alias[r - tb.N] >= 0)
+            throw std::runtime_error("stencil reads an aliased ghost");
     for (int32_t r : tb.rst_ref)
         if (r < 0 || r >= tb.n_int) throw std::runtime_error("restriction slot ref out of range");
     for (int32_t r : tb.bc_ref)
@@ -590,10 +692,16 @@ void build_tables(const Tree& t, Tables& tb) {
             throw std::runtime_error("ghost ref out of range");
     for (int s = 0; s < tb.S_w; s++) {
         const int32_t* iv = &tb.wls_i[(size_t)s * WLSI_STRIDE];
-        if (iv[WLSI_M] < 1 || iv[WLSI_M] > 9 || iv[WLSI_M1] < 0 || iv[WLSI_M1] > iv[WLSI_M])
+        const int m = iv[WLSI_M] & WLSI_MMASK, gmask = iv[WLSI_M] >> WLSI_GSHIFT;
+        if (m < 1 || m > 9 || iv[WLSI_M1] < 0 || iv[WLSI_M1] > m || (gmask >> m) != 0)
             throw std::runtime_error("WLS stencil size out of range");
-        for (int i = 0; i < iv[WLSI_M]; i++)
-            if (iv[WLSI_IDS + i] < 0 || iv[WLSI_IDS + i] >= tb.L) throw std::runtime_error("WLS id out of range");
+        for (int i = 0; i < m; i++) {
+            const int32_t r = iv[WLSI_IDS + i];
+            if (r < 0 || r >= tb.L) throw std::runtime_error("WLS id out of range");
+            // the ghost mask names exactly the points that are ghost leaves
+            if (((gmask >> i) & 1) != (r >= tb.N && r < tb.N + tb.G)) throw std::runtime_error("WLS ghost mask wrong");
+        }
+        if ((gmask != 0) != (s >= tb.S_wp)) throw std::runtime_error("WLS slot order wrong");
         if (iv[WLSI_PAT] < 0 || iv[WLSI_PAT] >= tb.n_pat) throw std::runtime_error("WLS record index out of range");
     }
     for (int32_t q : tb.rst_node)

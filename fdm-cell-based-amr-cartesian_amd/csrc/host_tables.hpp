@@ -26,9 +26,15 @@ enum WlsLayout {
     WLS_BOX = 83,    // 6: dxb, dyb, x2, y2, x3, y3
     WLS_STRIDE = 96
 };
-// WLSI_PAT: index of the slot's records in wls_d / wls_f (slots with bit-identical geometry
-// share them)
+// WLSI_M: stencil size m in the low byte; bit WLSI_GSHIFT + j set when point j is a ghost leaf
+// (read through its fill list by the fast path). WLSI_PAT: index of the slot's records in
+// wls_d / wls_f (slots with bit-identical geometry share them).
 enum WlsIntLayout { WLSI_M = 0, WLSI_M1 = 1, WLSI_IDS = 2, WLSI_PAT = 11, WLSI_STRIDE = 12 };
+constexpr int WLSI_MMASK = 0xFF, WLSI_GSHIFT = 8;
+// ghost entries of the slots that read ghost leaves (Tables::dep_hdr): up to 2 ghost points of
+// up to DEP_ENT entries each; DEP_INFLOW marks the inflow state, DEP_GENERIC a slot that does
+// not fit (its ghost points are formed from the fill lists)
+constexpr int DEP_ENT = 4, DEP_INFLOW = -2, DEP_GENERIC = 255;
 
 // FP32 slot record of the fast path (floats): the least-squares solves folded into
 // coefficient matrices on the host in FP64 (C6 = (P^T W P)^-1 P^T W etc.), plus the value
@@ -55,6 +61,7 @@ enum StKind { ST_COPY = 0, ST_PARENT_WLS = 1, ST_PARENT_RESTRICT = 2, ST_CHILD_W
 struct Tables {
     int N = 0, G = 0, P = 0, L = 0;    // interior, ghost, passive (corner) leaves, L = N+G+P
     int S_w = 0, S_r = 0;              // WLS slots, restriction slots
+    int S_wp = 0;                      // WLS slots [0, S_wp) read no ghost leaf (the rest do)
     int l_max = 0;
     std::vector<int32_t> state_of_node;  // node -> state index (leaves) or -1
     std::vector<int32_t> leaf_node;      // state index -> node
@@ -88,6 +95,17 @@ struct Tables {
     std::vector<int32_t> fill_beg, fill_leaf;
     std::vector<float> fill_w;
     bool fill_flat_ok = true;  // every summed leaf is interior (no ghost read while written)
+    // ghost g -> the state index the tables read instead: the interior leaf it copies, or the
+    // first inflow ghost for the other inflow ghosts (-1: read the ghost itself). The fast path
+    // refills only the items in fill_fast (ghosts without an alias, then restriction slots, as
+    // item ids k of the flattened fill).
+    std::vector<int32_t> ghost_alias;    // [G]
+    std::vector<int32_t> fill_fast;
+    // slot S_wp + d: dep_hdr[4d] = number of ghost points (or DEP_GENERIC), [4d+1], [4d+2]
+    // their point indices; dep_leaf / dep_w [d][2][DEP_ENT]: leaf (-1: none) and weight of
+    // each entry of the ghost's fill list
+    std::vector<int32_t> dep_hdr, dep_leaf;
+    std::vector<float> dep_w;
     // selector input: state index of the leaf containing each of the 16 off-axis points of
     // the leaf's 5x5 stencil (x + a dx, y + b dy), a,b in -2..2, a != 0, b != 0, row-major
     // in b then a. New rule (the reference has no 5x5 stencil): descend from the base cell

@@ -30,7 +30,7 @@ EXPORTS = (
     "fdmamr_build_tables", "fdmamr_counts", "fdmamr_set_state", "fdmamr_get_state",
     "fdmamr_get_ghost_state", "fdmamr_leaf_geometry", "fdmamr_max_wavespeed",
     "fdmamr_advance_rk3", "fdmamr_run_steps", "fdmamr_check_positivity", "fdmamr_sensors",
-    "fdmamr_regrid", "fdmamr_stencil_info", "fdmamr_stencil_values", "fdmamr_profile",
+    "fdmamr_regrid", "fdmamr_stencil_info", "fdmamr_stencil_values", "fdmamr_stencil_values_fast", "fdmamr_table_info", "fdmamr_profile",
     "fdmamr_profile_read", "fdmamr_selector_set_flags", "fdmamr_selector_get_flags",
     "fdmamr_labels", "fdmamr_selector_init", "fdmamr_selector_set_weights",
     "fdmamr_selector_get_weights", "fdmamr_selector_config", "fdmamr_selector_infer",
@@ -93,6 +93,8 @@ def lib():
         "fdmamr_regrid": ([P, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32], C.c_int32),
         "fdmamr_stencil_info": ([P, C.c_int32, ip, ip, dp, dp, ip, ip, ip], C.c_int32),
         "fdmamr_stencil_values": ([P, C.c_int32, dp], C.c_int32),
+        "fdmamr_table_info": ([P, ip], C.c_int32),
+        "fdmamr_stencil_values_fast": ([P, C.c_int32, C.POINTER(C.c_float)], C.c_int32),
         "fdmamr_profile": ([P, C.c_int32], C.c_int32),
         "fdmamr_profile_read": ([P, dp, ip], C.c_int32),
         "fdmamr_selector_set_flags": ([P, C.POINTER(C.c_uint8), C.c_int32], C.c_int32),
@@ -451,6 +453,18 @@ class Solver:
         out = np.zeros((8, 4))
         self._check(self.L.fdmamr_stencil_values(self.ctx, i, _dp(out)), "stencil_values")
         return out
+
+    def stencil_values_fast(self, i):
+        out = np.zeros((8, 4), np.float32)
+        self._check(self.L.fdmamr_stencil_values_fast(self.ctx, i, out.ctypes.data_as(C.POINTER(C.c_float))),
+                    "stencil_values_fast")
+        return out
+
+    def table_info(self):
+        v = np.zeros(8, np.int32)
+        self._check(self.L.fdmamr_table_info(self.ctx, _ip(v)), "table_info")
+        return dict(slots_plain=int(v[0]), slots=int(v[1]), ghosts_aliased=int(v[2]),
+                    fill_items_fast=int(v[3]), slots_generic=int(v[4]))
 
 
 def config_c1(**kw) -> Config:

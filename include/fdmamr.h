@@ -153,6 +153,15 @@ int32_t fdmamr_stencil_info(fdmamr_ctx* ctx, int32_t i, int32_t* kind8, int32_t*
                             int32_t* nb8x9);
 /* device-evaluated stencil values (rho,u,v,p) of interior leaf i, 8 positions */
 int32_t fdmamr_stencil_values(fdmamr_ctx* ctx, int32_t i, double* val8x4);
+/* FP32 contexts: the values (rho,u,v,p) the fast path's stage reads for interior leaf i, 8
+ * positions, after the launch that evaluates the WLS slots and refills the ghosts of the
+ * current state (no reference counterpart; diagnostics) */
+int32_t fdmamr_stencil_values_fast(fdmamr_ctx* ctx, int32_t i, float* val8x4);
+/* regrid-time table facts (no reference counterpart; diagnostics): info8[0] WLS slots that
+ * read no ghost leaf, [1] WLS slots, [2] ghost leaves read through the leaf they copy (or the
+ * first inflow ghost), [3] items the fast path refills per stage (the other ghosts and the
+ * restriction slots), [4] ghost-reading slots without a fixed-size entry list, [5..7] 0 */
+int32_t fdmamr_table_info(fdmamr_ctx* ctx, int32_t* info8);
 
 /* ---- troubled-cell selector (no reference counterpart: the reference applies the nonlinear
  * WCNS weights everywhere, nnd_center_RK.F90:310-353). FP32 mode only. A leaf is "troubled"

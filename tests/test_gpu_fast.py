@@ -92,6 +92,51 @@ def test_flat_ghost_fill_matches_nested_restriction():
     assert worst < 1e-12
 
 
+def test_fast_stencil_inputs_at_the_boundary():
+    # The fast path reads ghosts that copy one interior leaf through that leaf, refills the
+    # other ghosts in the launch that evaluates the WLS slots, and forms the ghost points of the
+    # slots that read them from fixed-size entry lists. The values its stage reads for the
+    # leaves next to the boundary must therefore be the FP64 path's stencil values to FP32
+    # accuracy: copies and restriction means to FP32 rounding, WLS-WENO slots (FP32 fit) to
+    # 1e-4 relative, including every slot that reads a refilled ghost.
+    h = 1.0 / 64
+    cfg = riemann_c2(h=h, lvl_amr=2, pre_adapt=2)
+    a = F.Solver(cfg).setup()
+    t = a.tree()
+    m, n = int(t["num"][:, 1].max()), int(t["num"][:, 0].max())
+    b = F.Solver(riemann_c2(h=h, lvl_amr=2, pre_adapt=2, flux_precision=F.FLUX_FP32))
+    b.import_tree(m, n, t)
+    b.build_tables()
+    b.set_state(a.get_state())
+    ti = b.table_info()
+    print(b.counts(), ti)
+    assert ti["slots_plain"] < ti["slots"] and ti["fill_items_fast"] > 0 and ti["slots_generic"] == 0
+    tb = b.tree()
+    outside = (tb["x"] < 0.0) | (tb["x"] > 1.0) | (tb["y"] < 0.0) | (tb["y"] > 1.0)
+    g = b.geometry()
+    lvl, x, y = g["lvl"], g["x"], g["y"]
+    dy = h / 2.0 ** lvl
+    near = np.where((x - 2.5 * dy < 0.0) | (x + 2.5 * dy > 1.0) | (y - 2.5 * dy < 0.0) | (y + 2.5 * dy > 1.0))[0]
+    worst = {"copy": 0.0, "wls": 0.0, "wls_ghost": 0.0}
+    n_ghost_slots = 0
+    for i in near:
+        info = b.stencil_info(int(i))
+        va, vb = a.stencil_values(int(i)), b.stencil_values_fast(int(i)).astype(np.float64)
+        rel = np.max(np.abs(va - vb) / (np.abs(va) + 1.0), axis=1)
+        for k in range(8):
+            if info["kind"][k] in (1, 3):
+                nb = info["nb"][k][: info["m1"][k] + info["m2"][k]]
+                key = "wls_ghost" if outside[nb].any() else "wls"
+                n_ghost_slots += key == "wls_ghost"
+            else:
+                key = "copy"
+            worst[key] = max(worst[key], float(rel[k]))
+    print("near-boundary leaves", len(near), "ghost-reading slot entries", n_ghost_slots, "worst", worst)
+    assert n_ghost_slots > 0
+    assert worst["copy"] < 2e-7
+    assert worst["wls"] < 1e-4 and worst["wls_ghost"] < 1e-4
+
+
 def test_cleared_leaves_on_smooth_vortex():
     # Linear weights everywhere on a smooth flow stay close to the nonlinear scheme: on this
     # 64x64 vortex ~12% of the leaves are labelled troubled (the core is resolved by ~6

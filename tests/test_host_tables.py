@@ -125,6 +125,30 @@ def test_stencil_tables_deep_transmissive_mesh():
     _compare_stencils(s, o, limit=2000)
 
 
+def test_ghost_aliases_and_fast_fill_items():
+    # Ghosts that copy one interior leaf (and every inflow ghost but the first) are read through
+    # that leaf; the fast path refills only the others per stage. On this transmissive mesh the
+    # refinement reaches the boundary, so some ghosts are restriction means of refined partners
+    # and some WLS slots read them.
+    cfg = riemann_c2(h=1.0 / 24, lvl_amr=3, pre_adapt=0)
+    o = oracle_for(cfg)
+    for _ in range(3):
+        assert o.step() == 0
+    s = host(cfg)
+    m, n = o.dims()
+    s.import_tree(m, n, o.tree())
+    s.build_tables()
+    c, ti = s.counts(), s.table_info()
+    print(c, ti)
+    assert ti["slots"] == c["n_slot_wls"]
+    assert ti["ghosts_aliased"] + ti["fill_items_fast"] - c["n_slot_restrict"] == c["n_ghost"]
+    assert 0 < ti["ghosts_aliased"] < c["n_ghost"]
+    assert 0 < ti["slots_plain"] < ti["slots"]
+    assert ti["slots_generic"] == 0
+    # the classification the tables were built from is unchanged (node ids, not state slots)
+    _compare_stencils(s, o, limit=500)
+
+
 def test_tree_round_trip_and_canonical_ids():
     o = O.Oracle(h=10.0 / 32, iflag_amr=1, lvl_amr=2, pstep_adapt=2, iflag_localdt=1, cfl=0.5)
     for _ in range(3):
