@@ -526,7 +526,6 @@ __device__ __forceinline__ void slot_fast(const DevTables& t, const float* __res
                                           const int32_t* fl_leaf, const float* fl_w) {
     const int32_t* __restrict__ iv = t.wls_i + (size_t)s * WLSI_STRIDE;
     const float* __restrict__ f = wf + (size_t)iv[WLSI_PAT] * WF_STRIDE;
-    const int m1 = iv[WLSI_M1];
     const int m = mw & WLSI_MMASK, gm = mw >> WLSI_GSHIFT;
     float4 val[9];
     if (!GHOSTS) {
@@ -590,34 +589,22 @@ __device__ __forceinline__ void slot_fast(const DevTables& t, const float* __res
         }
 #pragma unroll
         for (int i = 0; i < 9; i++) vl += f[WF_CL + i] * phi[i];
-        // slopes of the two linear fits
+        // slopes of the two linear fits (rows zero outside their lists)
         float as1[3], as2[3], vs1 = 0.f, vs2 = 0.f;
         as1[0] = as2[0] = 0.f;
 #pragma unroll
         for (int r = 1; r < 3; r++) {
             float a = 0.f, b = 0.f;
 #pragma unroll
-            for (int j = 0; j < 5; j++) a += f[WF_C3A + (r - 1) * 5 + j] * (j < m1 ? phi[j] : 0.f);
+            for (int j = 0; j < 5; j++) a += f[WF_C3A + (r - 1) * 5 + j] * phi[j];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int ii = m1 + j;
-                float ph = 0.f;
-#pragma unroll
-                for (int q = 0; q < 9; q++) ph = q == ii ? phi[q] : ph;
-                b += f[WF_C3B + (r - 1) * 4 + j] * (ii < m ? ph : 0.f);
-            }
+            for (int i = 0; i < 9; i++) b += f[WF_C3B + (r - 1) * 9 + i] * phi[i];
             as1[r] = a; as2[r] = b;
         }
 #pragma unroll
-        for (int j = 0; j < 5; j++) vs1 += f[WF_CS1 + j] * (j < m1 ? phi[j] : 0.f);
+        for (int j = 0; j < 5; j++) vs1 += f[WF_CS1 + j] * phi[j];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int ii = m1 + j;
-            float ph = 0.f;
-#pragma unroll
-            for (int q = 0; q < 9; q++) ph = q == ii ? phi[q] : ph;
-            vs2 += f[WF_CS2 + j] * (ii < m ? ph : 0.f);
-        }
+        for (int i = 0; i < 9; i++) vs2 += f[WF_CS2 + i] * phi[i];
         // smoothness indicators (XuanGua.F90:325-339) and WENO combination (:296-306)
         const float sl = isq_f(al[1], 2.f * al[3], al[4], box) + isq_f(al[2], al[4], 2.f * al[5], box) +
                          (4.f * al[3] * al[3] + al[4] * al[4] + 4.f * al[5] * al[5]) * area;

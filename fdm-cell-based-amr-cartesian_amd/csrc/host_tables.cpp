@@ -191,10 +191,10 @@ void fast_record(const double* d, const int32_t* iv, float* f) {
             double s = 0.0;
             for (int c = 0; c < 3; c++) s += d[WLS_INV3B + c * 3 + r] * P[m1 + j][c];
             s *= d[WLS_W + m1 + j];
-            if (r > 0) f[WF_C3B + (r - 1) * 4 + j] = (float)s;
+            if (r > 0) f[WF_C3B + (r - 1) * 9 + m1 + j] = (float)s;
             cs += bl[r] * s;
         }
-        f[WF_CS2 + j] = (float)cs;
+        f[WF_CS2 + m1 + j] = (float)cs;
     }
     for (int k = 0; k < 6; k++) f[WF_BOX + k] = (float)d[WLS_BOX + k];
 }

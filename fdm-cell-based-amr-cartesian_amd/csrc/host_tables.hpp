@@ -40,16 +40,18 @@ constexpr int DEP_ENT = 4, DEP_INFLOW = -2, DEP_GENERIC = 255;
 // coefficient matrices on the host in FP64 (C6 = (P^T W P)^-1 P^T W etc.), plus the value
 // weights c = basis(xx,yy)^T C, so a slot costs a few small mat-vecs per variable. Only the
 // rows the smoothness indicators read are kept (the constant-term rows enter through the
-// value weights alone).
+// value weights alone). Every row is indexed by stencil point and zero outside its list
+// (list 1: points [0, min(m1, 5)); list 2: points [m1, m1 + min(m - m1, 4))), so the kernel
+// needs no data-dependent indexing.
 enum WlsFastLayout {
     WF_C6 = 0,      // rows 1..5 of 6 x 9, row-major [(r-1)*9 + i]
-    WF_C3A = 45,    // rows 1..2 of 3 x 5 (list 1, up to 5 points)
-    WF_C3B = 55,    // rows 1..2 of 3 x 4 (list 2, up to 4 points)
-    WF_CL = 63,     // 9 value weights of the quadratic fit
-    WF_CS1 = 72,    // 5 value weights of linear fit 1
-    WF_CS2 = 77,    // 4 value weights of linear fit 2
-    WF_BOX = 81,    // 6 box terms (dxb, dyb, x2, y2, x3, y3)
-    WF_STRIDE = 88
+    WF_C3A = 45,    // rows 1..2 of 3 x 5 (list 1, points 0..4)
+    WF_C3B = 55,    // rows 1..2 of 3 x 9 (list 2, by point)
+    WF_CL = 73,     // 9 value weights of the quadratic fit
+    WF_CS1 = 82,    // 5 value weights of linear fit 1
+    WF_CS2 = 87,    // 9 value weights of linear fit 2 (by point)
+    WF_BOX = 96,    // 6 box terms (dxb, dyb, x2, y2, x3, y3)
+    WF_STRIDE = 104
 };
 
 constexpr int32_t REF_INFLOW = -2147483647 - 1;
