@@ -58,7 +58,7 @@ struct fdmamr_ctx {
     // learned selector
     DBuf<int32_t> off16;
     DBuf<int32_t> sel_nbr_beg, sel_nbr;  // selector tile neighbourhoods
-    DBuf<uint16_t> sel_loc;
+    DBuf<uint8_t> sel_loc;
     DBuf<float> sel_w, sel_m, sel_v, sel_grad, sel_gpart, sel_lpart, sel_loss, sel_prob;
     DBuf<uint16_t> sel_pk;
     DBuf<int> sel_cnt;
@@ -221,7 +221,7 @@ int upload_tables(fdmamr_ctx* c) {
     HIPCHK(c, ensure(c->sel_loc, tb.sel_loc.size()));
     HIPCHK(c, up(c->sel_nbr_beg.p, tb.sel_nbr_beg.data(), tb.sel_nbr_beg.size() * 4));
     HIPCHK(c, up(c->sel_nbr.p, tb.sel_nbr.data(), tb.sel_nbr.size() * 4));
-    HIPCHK(c, up(c->sel_loc.p, tb.sel_loc.data(), tb.sel_loc.size() * 2));
+    HIPCHK(c, up(c->sel_loc.p, tb.sel_loc.data(), tb.sel_loc.size()));
     // every leaf troubled until the selector says otherwise
     if (tb.N > 0) HIPCHK(c, hipMemsetAsync(c->flags.p, 1, tb.N, s));
     launch_build_lists(tb.N, c->flags.p, c->list_cnt.p, c->lists.p, c->list_meta.p, s);

@@ -219,14 +219,14 @@ __device__ __forceinline__ void stage_leaf_lin(const DevTables& t, int i, const 
                                                const float4* __restrict__ Qf_in, const double4* Qn,
                                                double4* Qout, float4* __restrict__ Qf_out,
                                                const DevScalars* __restrict__ sc, double alpha, double beta,
-                                               float* __restrict__ label, double* rmax, double* rsum,
-                                               bool lam, double& lx, double& ly) {
+                                               double* rmax, double* rsum, bool lam, double& lx, double& ly) {
     const double gama = t.gamma;
     int s[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) s[k] = t.stencil[(size_t)k * t.N + i];
     const double4 qc = Qin[i];
-    const float4 qcf = Qf_in[i];
+    // the leaf's own FP32 value is the rounding of its FP64 state (the shadow holds exactly that)
+    const float4 qcf = make_float4((float)qc.x, (float)qc.y, (float)qc.z, (float)qc.w);
     const int lv = t.lvl[i];
     const double dy = t.h / (double)(1 << lv);
     const double dx = t.xf * dy;
@@ -239,7 +239,6 @@ __device__ __forceinline__ void stage_leaf_lin(const DevTables& t, int i, const 
         const float4 col[5] = {Qf_in[s[7]], Qf_in[s[3]], qcf, Qf_in[s[2]], Qf_in[s[6]]};
         flux_lin_f<1>(col, (float)dy, (float)sc->lam_y, (float)gama, dF);
     }
-    if (label) label[i] = 0.f;
     update_leaf(i, qc, Qn[i], dE, dF, dx, dy, gama, sc->dt, alpha, beta, Qout, Qf_out, rmax, rsum, lam, lx, ly);
 }
 
@@ -251,7 +250,7 @@ __device__ __forceinline__ void lin_block(int blk, const DevTables& t, const int
                                           const float4* __restrict__ Qf_in, const double4* Qn, double4* Qout,
                                           float4* __restrict__ Qf_out, const DevScalars* __restrict__ sc,
                                           double alpha, double beta, int reduce, double* __restrict__ partials,
-                                          double* __restrict__ lampart, int poff, float* __restrict__ label) {
+                                          double* __restrict__ lampart, int poff) {
     const int n = meta[0];
     // the active prefix of the grid is spread over all XCDs, then renumbered within itself
     const int nact = (n + 255) >> 8;
@@ -264,8 +263,8 @@ __device__ __forceinline__ void lin_block(int blk, const DevTables& t, const int
     double rmax[4] = {0, 0, 0, 0}, rsum[4] = {0, 0, 0, 0};
     double lx = 0.0, ly = 0.0;
     if (p < n)
-        stage_leaf_lin(t, lists[p], Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, label, rmax, rsum,
-                       lampart != nullptr, lx, ly);
+        stage_leaf_lin(t, lists[p], Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, rmax, rsum, lampart != nullptr,
+                       lx, ly);
     if (reduce) block_partials(rmax, rsum, partials, poff + lb, lampart, lx, ly);
 }
 
@@ -397,7 +396,7 @@ __global__ __launch_bounds__(256) void k_stage_mixed(DevTables t, const int32_t*
                           partials, lampart, 0, label, jd);
     else
         lin_block(blockIdx.x - nb4, t, lists, meta, Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, reduce,
-                  partials, lampart, nb4, label);
+                  partials, lampart, nb4);
 }
 
 // every leaf on the full path (no flags)

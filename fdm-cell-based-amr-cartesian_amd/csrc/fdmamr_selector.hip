@@ -304,7 +304,7 @@ __global__ __launch_bounds__(256) void k_select_gate(DevTables t, const float4* 
                                                      uint4* __restrict__ tile_frag,
                                                      const int32_t* __restrict__ nbr_beg,
                                                      const int32_t* __restrict__ nbr,
-                                                     const uint4* __restrict__ loc) {
+                                                     const uint2* __restrict__ loc) {
     __shared__ float4 sQ[4][SEL_NBR_CAP];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
     const int tile = xcd_block() * 4 + wv;
@@ -319,7 +319,7 @@ __global__ __launch_bounds__(256) void k_select_gate(DevTables t, const float4* 
         nU = nbr_beg[tile + 1] - b;
     }
     const bool local = nU <= SEL_NBR_CAP;
-    const uint4 lc = loc[(size_t)(valid ? leaf : 0) * 4 + g];
+    const uint2 lc = loc[(size_t)(valid ? leaf : 0) * 4 + g];
     const uint8_t old = valid ? flags[leaf] : 0;
     if (local)
         for (int k = lane; k < nU; k += 64) sQ[wv][k] = Qf[nbr[b + k]];
@@ -329,7 +329,7 @@ __global__ __launch_bounds__(256) void k_select_gate(DevTables t, const float4* 
     bool uniform;
     if (local) {
         // positions are below nU (checked when the tables are built); leaves past N use entry 0
-        const uint16_t* pos = reinterpret_cast<const uint16_t*>(&lc);
+        const uint8_t* pos = reinterpret_cast<const uint8_t*>(&lc);
         float4 q[7];
 #pragma unroll
         for (int k = 0; k < 7; k++) q[k] = sQ[wv][valid ? (int)pos[k] : 0];
@@ -692,11 +692,11 @@ __global__ void k_pack(const float* __restrict__ w, uint16_t* __restrict__ pk) {
 void launch_select(const DevTables& t, const float4* Qf, const int32_t* off16, const uint16_t* wpk,
                    const float* w32, const SelectParams& sp, uint8_t* flags, float* prob, uint16_t* tile_mask,
                    uint4* tile_frag, int mlp_blocks, const int32_t* nbr_beg, const int32_t* nbr,
-                   const uint16_t* loc, hipStream_t st) {
+                   const uint8_t* loc, hipStream_t st) {
     if (t.N <= 0) return;
     const int ntiles = (t.N + 15) / 16;
     hipLaunchKernelGGL(k_select_gate, dim3((ntiles + 3) / 4), dim3(256), 0, st, t, Qf, off16, sp, flags, prob,
-                       tile_mask, tile_frag, nbr_beg, nbr, reinterpret_cast<const uint4*>(loc));
+                       tile_mask, tile_frag, nbr_beg, nbr, reinterpret_cast<const uint2*>(loc));
     hipLaunchKernelGGL(k_select_mlp, dim3(mlp_blocks), dim3(256), 0, st, t.N, wpk, w32, sp, tile_mask, tile_frag,
                        flags, prob);
 }

@@ -62,7 +62,7 @@ struct SelectParams {
 void launch_select(const DevTables& t, const float4* Qf, const int32_t* off16, const uint16_t* wpk,
                    const float* w32, const SelectParams& sp, uint8_t* flags, float* prob, uint16_t* tile_mask,
                    uint4* tile_frag, int mlp_blocks, const int32_t* nbr_beg, const int32_t* nbr,
-                   const uint16_t* loc, hipStream_t st);
+                   const uint8_t* loc, hipStream_t st);
 void launch_judge(int n, const float* label, float bound, uint8_t* flags, int* counters, hipStream_t st);
 void launch_features(const DevTables& t, const float4* Qf, const int32_t* off16, uint16_t* X, int row0,
                      hipStream_t st);

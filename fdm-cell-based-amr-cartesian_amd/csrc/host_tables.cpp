@@ -631,7 +631,7 @@ void build_tables(const Tree& t, Tables& tb) {
                                 U.push_back(src);
                                 used.push_back((int)h);
                             }
-                            tb.sel_loc[(size_t)i * 32 + g * 8 + j] = (uint16_t)std::min(val[h], 65535);
+                            tb.sel_loc[(size_t)i * 32 + g * 8 + j] = (uint8_t)std::min(val[h], 255);
                         }
                 for (int h : used) key[h] = -1;
                 used.clear();

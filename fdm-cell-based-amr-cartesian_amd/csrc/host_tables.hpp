@@ -55,7 +55,7 @@ enum WlsFastLayout {
 };
 
 constexpr int32_t REF_INFLOW = -2147483647 - 1;
-constexpr int SEL_NBR_CAP = 192;
+constexpr int SEL_NBR_CAP = 192;  // positions fit a byte
 
 // stencil classification, identical to the reference's gathers
 enum StKind { ST_COPY = 0, ST_PARENT_WLS = 1, ST_PARENT_RESTRICT = 2, ST_CHILD_WLS = 3, ST_CHILD_COPY = 4 };
@@ -115,12 +115,12 @@ struct Tables {
     std::vector<int32_t> off16;          // [16][N]
     // Selector tiles (16 consecutive interior leaves): the distinct state indices their 5x5
     // stencils read, tile k's in sel_nbr[sel_nbr_beg[k], sel_nbr_beg[k+1]), and for every
-    // leaf the positions of its points in that list: sel_loc[leaf][g][j] (uint16) is point
+    // leaf the positions of its points in that list: sel_loc[leaf][g][j] (uint8) is point
     // g + 4j of the 5x5 grid (p = (b+2)*5 + (a+2); the centre for p > 24), j < 7, for the four
     // lanes g of the leaf. A tile with more than SEL_NBR_CAP distinct indices stores none
     // (count SEL_NBR_CAP + 1) and is gathered directly.
     std::vector<int32_t> sel_nbr_beg, sel_nbr;
-    std::vector<uint16_t> sel_loc;       // [N][4][8]
+    std::vector<uint8_t> sel_loc;        // [N][4][8]
     // device topology for restriction means
     std::vector<int32_t> node_son;       // [n_alloc]
 };
