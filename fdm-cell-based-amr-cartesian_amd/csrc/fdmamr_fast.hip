@@ -26,6 +26,10 @@ namespace fdm {
 // weights give when tau3 = 0
 #define LIN_C (0.125f / ((float)0x1.745d18p-4 + (float)0x1.d1745ep-1))
 #define W1_LIN ((float)0x1.745d18p-4 / ((float)0x1.745d18p-4 + (float)0x1.d1745ep-1))
+// second WCNS_Z3 candidate as one weighted sum: (-u1 / 8 + (3/4 - d1) u2 + 3/8 u3) / d2
+#define P2_C1 (-0.125f / FD2)
+#define P2_C2 ((0.75f - FD1) / FD2)
+#define P2_C3 (0.375f / FD2)
 
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
@@ -80,21 +84,23 @@ __device__ __forceinline__ void eigen_f(const float* U, float gama, float R[4][4
 // one WCNS_Z3 component; returns the reconstruction and accumulates |w1 - w1_lin| into dev.
 // The deviation uses smoothness indicators with a relative noise floor (1e-6 of the mean
 // square), so exactly flat data (where every candidate is the same value) is not reported.
+// In differences d1 = u2 - u1, d2 = u3 - u2 the reference's indicators are IS1 = min(d1^2, d2^2)
+// and IS2 = t3^2 + u1^2 + u2^2 + u3^2 - u1 u2 - u2 u3 - u1 u3 = 2 (d1^2 + d2^2) - d1 d2 (t3 =
+// d2 - d1), which FP32 evaluates without the cancellation of the squares; the second candidate
+// p2 = ((-u1 + 6 u2 + 3 u3) / 8 - d1 u2) / d2 is expanded into one weighted sum.
 template <bool LABEL>
 __device__ __forceinline__ float wcns_f(float u1, float u2, float u3, float& dev) {
-    const float IS1a = (u2 - u1) * (u2 - u1), IS1b = (u3 - u2) * (u3 - u2);
-    const float t3 = u1 - 2.f * u2 + u3;
+    const float d1 = u2 - u1, d2 = u3 - u2;
+    const float IS1a = d1 * d1, IS1b = d2 * d2;
     const float IS1 = fminf(IS1a, IS1b);
-    const float IS2 = t3 * t3 + u1 * u1 + u2 * u2 + u3 * u3 - u1 * u2 - u2 * u3 - u1 * u3;
+    const float IS2 = 2.f * (IS1a + IS1b) - d1 * d2;
     const float tq = 10.f * fabsf(IS2 - 3.f * IS1);
     const float tau3 = tq * tq;
     const float r1 = fminf(tau3 * frcp(1.0e-30f + IS1), 1.0e30f);
     const float r2 = fminf(tau3 * frcp(1.0e-30f + IS2), 1.0e30f);
     const float a1 = FD1 * (1.f + r1), a2 = FD2 * (1.f + r2);
     const float ia = frcp(a1 + a2);
-    const float w1 = a1 * ia, w2 = a2 * ia;
-    const float p1 = u2;
-    const float p2 = ((-u1 + 6.f * u2 + 3.f * u3) * 0.125f - FD1 * p1) * (1.f / FD2);
+    const float p2 = P2_C1 * u1 + P2_C2 * u2 + P2_C3 * u3;
     if (LABEL) {
         const float fl = 1e-6f * (u1 * u1 + u2 * u2 + u3 * u3) * (1.f / 3.f) + 1e-30f;
         const float s1 = fminf(tau3 * frcp(fl + IS1), 1.0e30f);
@@ -102,7 +108,7 @@ __device__ __forceinline__ float wcns_f(float u1, float u2, float u3, float& dev
         const float b1 = FD1 * (1.f + s1), b2 = FD2 * (1.f + s2);
         dev = fmaxf(dev, fabsf(b1 * frcp(b1 + b2) - W1_LIN));
     }
-    return w1 * p1 + w2 * p2;
+    return (a1 * u2 + a2 * p2) * ia;
 }
 
 // Characteristic WCNS3 flux through one face in FP32 (troubled leaves), x-direction form
