@@ -386,10 +386,12 @@ __device__ __forceinline__ void full_block(int blk, const DevTables& t, const in
 
 // One launch for both lists: blocks [0, nb4) run the troubled leaves (four lanes each), blocks
 // [nb4, nb4 + nb) the cleared ones, so the compute-bound characteristic work and the
-// memory-bound linear work share the machine instead of draining it in turn. Partials: the
-// full blocks at [0, nb4), the linear ones after them.
+// memory-bound linear work share the machine instead of draining it in turn (the long
+// characteristic blocks are dispatched first; interleaving the two kinds measured slower).
+// Held to 6 waves per SIMD (76 VGPRs), so the labelling instance keeps the occupancy of the
+// other. Partials: the full blocks at [0, nb4), the linear ones after them.
 template <bool LABEL>
-__global__ __launch_bounds__(256) void k_stage_mixed(DevTables t, const int32_t* __restrict__ lists,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_stage_mixed(DevTables t, const int32_t* __restrict__ lists,
                                                      const int* __restrict__ meta, const double4* __restrict__ Qin,
                                                      const float4* __restrict__ Qf_in, const double4* Qn,
                                                      double4* Qout, float4* __restrict__ Qf_out,
