@@ -387,53 +387,110 @@ __global__ __launch_bounds__(256) void k_select_mlp(int n_leaf, const uint16_t* 
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
     const int ntiles = (n_leaf + 15) / 16;
-    const int nwaves = gridDim.x * 4;
-    for (int tile = blockIdx.x * 4 + wv; tile < ntiles; tile += nwaves) {
-        const uint16_t mask = tile_mask[tile];
-        if (!mask) continue;
-        const uint4* src = tile_frag + (size_t)tile * 3 * 64;
-        bf16x8 xb[3];
-#pragma unroll
-        for (int s3 = 0; s3 < 3; s3++) {
-            const uint4 u = src[s3 * 64 + lane];
-            xb[s3] = *reinterpret_cast<const bf16x8*>(&u);
-        }
-        const int leaf = tile * 16 + r;
-        const bool mine = (mask >> r) & 1;
-        const uint8_t old = mine && g == 0 ? flags[leaf] : 0;
-        f32x4 z[4];
-#pragma unroll
-        for (int mt = 0; mt < 4; mt++) {
-            z[mt] = *reinterpret_cast<const f32x4*>(sB + SB_B1 + 16 * mt + 4 * g);
-#pragma unroll
-            for (int s3 = 0; s3 < 3; s3++) z[mt] = mfma32(frag(sF + KF_L1, mt * 3 + s3), xb[s3], z[mt]);
-        }
-#pragma unroll
-        for (int L = 0; L < 2; L++) {
-            const bf16x8 hb0 = relu_pack(z[0], z[1]), hb1 = relu_pack(z[2], z[3]);
-            const uint16_t* W = sF + (L == 0 ? KF_L2 : KF_L3);
-            const int boff = L == 0 ? SB_B2 : SB_B3;
-#pragma unroll
-            for (int mt = 0; mt < 4; mt++) {
-                f32x4 acc = *reinterpret_cast<const f32x4*>(sB + boff + 16 * mt + 4 * g);
-                acc = mfma32(frag(W, mt * 2 + 0), hb0, acc);
-                acc = mfma32(frag(W, mt * 2 + 1), hb1, acc);
-                z[mt] = acc;
+    // Block b owns the tiles b, b + nblk, ... (spread over the mesh, so every block gets about
+    // the same share of the spatially clustered tiles that need the network). Its four waves
+    // read the same 64 masks at a time (one lane each) and take every fourth tile that needs
+    // the network, so the block's share is split evenly; each wave runs its tiles in pairs, so
+    // every weight fragment read from LDS feeds two MFMAs.
+    const int nblk = gridDim.x;
+    for (int c0 = 0; (int)blockIdx.x + c0 * nblk < ntiles; c0 += 64) {
+        const int my_tile = (int)blockIdx.x + (c0 + lane) * nblk;
+        const uint16_t my_mask = my_tile < ntiles ? tile_mask[my_tile] : 0;
+        const uint64_t need = __ballot(my_mask != 0);
+        uint64_t todo = 0;
+        {
+            uint64_t rest = need;
+            for (int idx = 0; rest; idx++) {
+                const uint64_t low = rest & (~rest + 1);
+                if ((idx & 3) == wv) todo |= low;
+                rest ^= low;
             }
         }
-        float part = 0.f;
+        while (todo) {
+            // weight fragments are re-read from LDS per pair (hoisting them all into
+            // registers would cut the occupancy to two waves)
+            __asm__ volatile("" ::: "memory");
+            int kk[2];
+            kk[0] = __ffsll((long long)todo) - 1;
+            todo &= todo - 1;
+            const bool two = todo != 0;
+            kk[1] = two ? __ffsll((long long)todo) - 1 : kk[0];
+            if (two) todo &= todo - 1;
+            bf16x8 xb[2][3];
+            int leaf[2];
+            bool mine[2];
+            uint8_t old[2];
 #pragma unroll
-        for (int mt = 0; mt < 4; mt++) {
-            const f32x4 w4 = *reinterpret_cast<const f32x4*>(sB + SB_W4 + 16 * mt + 4 * g);
+            for (int h = 0; h < 2; h++) {
+                const int tile = (int)blockIdx.x + (c0 + kk[h]) * nblk;
+                const uint4* src = tile_frag + (size_t)tile * 3 * 64;
 #pragma unroll
-            for (int j = 0; j < 4; j++) part += w4[j] * relu(z[mt][j]);
-        }
-        part += __shfl_xor(part, 16);
-        part += __shfl_xor(part, 32);
-        if (mine && g == 0) {
-            const float pr = 1.f / (1.f + __expf(-(part + sB[SB_B4])));
-            if (prob) prob[leaf] = pr;
-            flags[leaf] = decide(old, pr >= sp.threshold, leaf, sp);
+                for (int s3 = 0; s3 < 3; s3++) {
+                    const uint4 u = src[s3 * 64 + lane];
+                    xb[h][s3] = *reinterpret_cast<const bf16x8*>(&u);
+                }
+                const uint16_t mask = (uint16_t)__shfl((int)my_mask, kk[h]);
+                leaf[h] = tile * 16 + r;
+                mine[h] = ((mask >> r) & 1) && (h == 0 || two);
+                old[h] = mine[h] && g == 0 ? flags[leaf[h]] : 0;
+            }
+            f32x4 z[2][4];
+#pragma unroll
+            for (int mt = 0; mt < 4; mt++) {
+                const f32x4 b1 = *reinterpret_cast<const f32x4*>(sB + SB_B1 + 16 * mt + 4 * g);
+                z[0][mt] = b1;
+                z[1][mt] = b1;
+#pragma unroll
+                for (int s3 = 0; s3 < 3; s3++) {
+                    const bf16x8 w = frag(sF + KF_L1, mt * 3 + s3);
+                    z[0][mt] = mfma32(w, xb[0][s3], z[0][mt]);
+                    z[1][mt] = mfma32(w, xb[1][s3], z[1][mt]);
+                }
+            }
+#pragma unroll
+            for (int L = 0; L < 2; L++) {
+                bf16x8 hb[2][2];
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    hb[h][0] = relu_pack(z[h][0], z[h][1]);
+                    hb[h][1] = relu_pack(z[h][2], z[h][3]);
+                }
+                const uint16_t* W = sF + (L == 0 ? KF_L2 : KF_L3);
+                const int boff = L == 0 ? SB_B2 : SB_B3;
+#pragma unroll
+                for (int mt = 0; mt < 4; mt++) {
+                    const f32x4 bb = *reinterpret_cast<const f32x4*>(sB + boff + 16 * mt + 4 * g);
+                    f32x4 a0 = bb, a1 = bb;
+                    const bf16x8 wa = frag(W, mt * 2 + 0), wb = frag(W, mt * 2 + 1);
+                    a0 = mfma32(wa, hb[0][0], a0);
+                    a1 = mfma32(wa, hb[1][0], a1);
+                    a0 = mfma32(wb, hb[0][1], a0);
+                    a1 = mfma32(wb, hb[1][1], a1);
+                    z[0][mt] = a0;
+                    z[1][mt] = a1;
+                }
+            }
+            float part[2] = {0.f, 0.f};
+#pragma unroll
+            for (int mt = 0; mt < 4; mt++) {
+                const f32x4 w4 = *reinterpret_cast<const f32x4*>(sB + SB_W4 + 16 * mt + 4 * g);
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    part[0] += w4[j] * relu(z[0][mt][j]);
+                    part[1] += w4[j] * relu(z[1][mt][j]);
+                }
+            }
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                float pt = part[h];
+                pt += __shfl_xor(pt, 16);
+                pt += __shfl_xor(pt, 32);
+                if (mine[h] && g == 0) {
+                    const float pr = 1.f / (1.f + __expf(-(pt + sB[SB_B4])));
+                    if (prob) prob[leaf[h]] = pr;
+                    flags[leaf[h]] = decide(old[h], pr >= sp.threshold, leaf[h], sp);
+                }
+            }
         }
     }
 }
