@@ -257,7 +257,7 @@ void fill_after_stage(fdmamr_ctx* c, int buf) {
 // of the final state (buffer 0), so that the state seen from outside is the same as after a
 // fill per stage.
 void finish_steps(fdmamr_ctx* c) {
-    if (c->p.flux_precision == FDMAMR_FLUX_FP32 && c->tb.fill_flat_ok)
+    if (c->p.flux_precision == FDMAMR_FLUX_FP32 && c->tb.merge_ok)
         launch_fill_flat(c->dt, c->Q[0].p, c->Qf[0].p, c->fill_beg.p, c->fill_leaf.p, c->fill_w.p, c->stream);
 }
 
@@ -331,7 +331,7 @@ void enqueue_rk3(fdmamr_ctx* c, int advance_time, bool prepare_next) {
     // fast path with flattened fills: each stage's input gets its unaliased ghosts and
     // restriction slots in the launch that evaluates its WLS slots (finish_steps completes
     // the ghosts of the final state)
-    const bool merged = fast && c->tb.fill_flat_ok;
+    const bool merged = fast && c->tb.merge_ok;
     for (int k = 0; k < 3; k++) {
         if (merged)
             launch_slots_fill(d, c->wls_f.p, c->Q[in[k]].p, c->Qf[in[k]].p, c->fill_fast.p,
@@ -1117,7 +1117,7 @@ int32_t fdmamr_stencil_values_fast(fdmamr_ctx* c, int32_t i, float* val8x4) {
     if (c->p.flux_precision != FDMAMR_FLUX_FP32) return fail(c, -1, "fdmamr_stencil_values_fast: FP32 contexts only");
     if (i < 0 || i >= c->tb.N || !val8x4) return fail(c, -1, "fdmamr_stencil_values_fast: bad leaf");
     // the slot and ghost work that precedes a stage, on state 0 (its values are unchanged)
-    if (c->tb.fill_flat_ok)
+    if (c->tb.merge_ok)
         launch_slots_fill(c->dt, c->wls_f.p, c->Q[0].p, c->Qf[0].p, c->fill_fast.p, (int)c->tb.fill_fast.size(),
                           c->fill_beg.p, c->fill_leaf.p, c->fill_w.p, c->stream);
     else

@@ -497,143 +497,115 @@ __device__ __forceinline__ float isq_f(float k1, float k2, float k3, const float
            k2 * k2 * b[4] * (1.f / 3.f) * b[1] + 0.5f * k2 * k3 * b[2] * b[3] + k3 * k3 * b[0] * b[5] * (1.f / 3.f);
 }
 
-// Value of fill item k (ghost k, or restriction slot k - G) from the flattened fill lists:
-// the weighted sum of its leaves' FP64 states, or the inflow state.
-__device__ __forceinline__ float4 fill_value(const DevTables& t, const double4* __restrict__ Q, int k,
-                                             const int32_t* __restrict__ beg, const int32_t* __restrict__ leaf,
-                                             const float* __restrict__ w) {
-    const int b = beg[k], e = beg[k + 1];
-    if (leaf[b] < 0) return make_float4((float)t.inflow[0], (float)t.inflow[1], (float)t.inflow[2], (float)t.inflow[3]);
-    double ax = 0.0, ay = 0.0, az = 0.0, aw = 0.0;
-    // four entries per round trip (the loads of a chunk are issued together), summed in order
-    for (int j0 = b; j0 < e; j0 += 4) {
-        int id[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) id[u] = j0 + u < e ? leaf[j0 + u] : -1;
-        double4 q[4];
-        double wj[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            q[u] = id[u] >= 0 ? Q[id[u]] : make_double4(0.0, 0.0, 0.0, 0.0);
-            wj[u] = id[u] >= 0 ? (double)w[j0 + u] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-            if (id[u] >= 0) { ax += wj[u] * q[u].x; ay += wj[u] * q[u].y; az += wj[u] * q[u].z; aw += wj[u] * q[u].w; }
-    }
-    return make_float4((float)ax, (float)ay, (float)az, (float)aw);
-}
-
-// One WLS-WENO slot in FP32 (size word mw). GHOSTS (the launch that also writes the ghosts):
-// the points that are ghost leaves (the ghost mask of mw) are formed from their fill lists
-// instead of read. A separate instance, so that the common one issues all its gathers at once.
-template <bool GHOSTS>
-__device__ __forceinline__ void slot_fast(const DevTables& t, const float* __restrict__ wf, const double4* __restrict__ Q,
-                                          float4* __restrict__ Qf, int s, int mw, const int32_t* fl_beg,
-                                          const int32_t* fl_leaf, const float* fl_w) {
-    const int32_t* __restrict__ iv = t.wls_i + (size_t)s * WLSI_STRIDE;
-    const float* __restrict__ f = wf + (size_t)iv[WLSI_PAT] * WF_STRIDE;
-    const int m = mw & WLSI_MMASK, gm = mw >> WLSI_GSHIFT;
-    float4 val[9];
-    if (!GHOSTS) {
-#pragma unroll
-        for (int i = 0; i < 9; i++) val[i] = i < m ? Qf[iv[WLSI_IDS + i]] : make_float4(0.f, 0.f, 0.f, 0.f);
-    } else {
-        const int d = s - t.S_wp;
-        const int4 hdr = t.dep_hdr[d];
-        if (hdr.x != DEP_GENERIC) {
-            // up to two ghost points from their entry lists (loads independent of the slot's
-            // own indices, so both gathers go out together)
-            float4 g[2];
-#pragma unroll
-            for (int p = 0; p < 2; p++) {
-                const int4 l = t.dep_leaf[2 * d + p];
-                const float4 w = t.dep_w[2 * d + p];
-                const int li[4] = {l.x, l.y, l.z, l.w};
-                const float wi[4] = {w.x, w.y, w.z, w.w};
-                float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const float4 q = li[u] >= 0 ? Qf[li[u]]
-                                     : li[u] == DEP_INFLOW ? make_float4((float)t.inflow[0], (float)t.inflow[1],
-                                                                         (float)t.inflow[2], (float)t.inflow[3])
-                                                           : make_float4(0.f, 0.f, 0.f, 0.f);
-                    acc.x += wi[u] * q.x; acc.y += wi[u] * q.y; acc.z += wi[u] * q.z; acc.w += wi[u] * q.w;
-                }
-                g[p] = acc;
-            }
-#pragma unroll
-            for (int i = 0; i < 9; i++)
-                val[i] = hdr.x > 0 && i == hdr.y   ? g[0]
-                         : hdr.x > 1 && i == hdr.z ? g[1]
-                         : i < m && !((gm >> i) & 1) ? Qf[iv[WLSI_IDS + i]]
-                                                     : make_float4(0.f, 0.f, 0.f, 0.f);
-        } else {
-#pragma unroll
-            for (int i = 0; i < 9; i++) {
-                if ((gm >> i) & 1) val[i] = fill_value(t, Q, iv[WLSI_IDS + i] - t.N, fl_beg, fl_leaf, fl_w);
-                else val[i] = i < m ? Qf[iv[WLSI_IDS + i]] : make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-        }
-    }
+// One component of a WLS-WENO slot in FP32 from its nine point values (record f).
+__device__ __forceinline__ float slot_component(const float* __restrict__ f, const float* phi) {
     const float* box = f + WF_BOX;
     const float area = box[0] * box[1];
-    float out[4];
+    // coefficients 1..5 of the quadratic fit (the constant term is not needed)
+    float al[6], vl = 0.f;
+    al[0] = 0.f;
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        float phi[9];
+    for (int r = 1; r < 6; r++) {
+        float a = 0.f;
 #pragma unroll
-        for (int i = 0; i < 9; i++) phi[i] = k == 0 ? val[i].x : k == 1 ? val[i].y : k == 2 ? val[i].z : val[i].w;
-        // coefficients 1..5 of the quadratic fit (the constant term is not needed)
-        float al[6], vl = 0.f;
-        al[0] = 0.f;
-#pragma unroll
-        for (int r = 1; r < 6; r++) {
-            float a = 0.f;
-#pragma unroll
-            for (int i = 0; i < 9; i++) a += f[WF_C6 + (r - 1) * 9 + i] * phi[i];
-            al[r] = a;
-        }
-#pragma unroll
-        for (int i = 0; i < 9; i++) vl += f[WF_CL + i] * phi[i];
-        // slopes of the two linear fits (rows zero outside their lists)
-        float as1[3], as2[3], vs1 = 0.f, vs2 = 0.f;
-        as1[0] = as2[0] = 0.f;
-#pragma unroll
-        for (int r = 1; r < 3; r++) {
-            float a = 0.f, b = 0.f;
-#pragma unroll
-            for (int j = 0; j < 5; j++) a += f[WF_C3A + (r - 1) * 5 + j] * phi[j];
-#pragma unroll
-            for (int i = 0; i < 9; i++) b += f[WF_C3B + (r - 1) * 9 + i] * phi[i];
-            as1[r] = a; as2[r] = b;
-        }
-#pragma unroll
-        for (int j = 0; j < 5; j++) vs1 += f[WF_CS1 + j] * phi[j];
-#pragma unroll
-        for (int i = 0; i < 9; i++) vs2 += f[WF_CS2 + i] * phi[i];
-        // smoothness indicators (XuanGua.F90:325-339) and WENO combination (:296-306)
-        const float sl = isq_f(al[1], 2.f * al[3], al[4], box) + isq_f(al[2], al[4], 2.f * al[5], box) +
-                         (4.f * al[3] * al[3] + al[4] * al[4] + 4.f * al[5] * al[5]) * area;
-        const float b1 = (as1[1] * as1[1] + as1[2] * as1[2]) * area;
-        const float b2 = (as2[1] * as2[1] + as2[2] * as2[2]) * area;
-        float tau = 0.5f * (fabsf(sl - b1) + fabsf(sl - b2));
-        tau = tau * tau;
-        const float a0 = 0.98f * (1.f + fminf(tau * frcp(1e-6f + sl), 1e30f));
-        const float a1 = 0.01f * (1.f + fminf(tau * frcp(1e-6f + b1), 1e30f));
-        const float a2 = 0.01f * (1.f + fminf(tau * frcp(1e-6f + b2), 1e30f));
-        const float ia = frcp(a0 + a1 + a2);
-        out[k] = a0 * ia * ((1.f / 0.98f) * vl - (0.01f / 0.98f) * (vs1 + vs2)) + (a1 * ia) * vs1 + (a2 * ia) * vs2;
+        for (int i = 0; i < 9; i++) a += f[WF_C6 + (r - 1) * 9 + i] * phi[i];
+        al[r] = a;
     }
-    Qf[t.L + s] = make_float4(out[0], out[1], out[2], out[3]);
+#pragma unroll
+    for (int i = 0; i < 9; i++) vl += f[WF_CL + i] * phi[i];
+    // slopes of the two linear fits (rows zero outside their lists)
+    float as1[3], as2[3], vs1 = 0.f, vs2 = 0.f;
+    as1[0] = as2[0] = 0.f;
+#pragma unroll
+    for (int r = 1; r < 3; r++) {
+        float a = 0.f, b = 0.f;
+#pragma unroll
+        for (int j = 0; j < 5; j++) a += f[WF_C3A + (r - 1) * 5 + j] * phi[j];
+#pragma unroll
+        for (int i = 0; i < 9; i++) b += f[WF_C3B + (r - 1) * 9 + i] * phi[i];
+        as1[r] = a; as2[r] = b;
+    }
+#pragma unroll
+    for (int j = 0; j < 5; j++) vs1 += f[WF_CS1 + j] * phi[j];
+#pragma unroll
+    for (int i = 0; i < 9; i++) vs2 += f[WF_CS2 + i] * phi[i];
+    // smoothness indicators (XuanGua.F90:325-339) and WENO combination (:296-306)
+    const float sl = isq_f(al[1], 2.f * al[3], al[4], box) + isq_f(al[2], al[4], 2.f * al[5], box) +
+                     (4.f * al[3] * al[3] + al[4] * al[4] + 4.f * al[5] * al[5]) * area;
+    const float b1 = (as1[1] * as1[1] + as1[2] * as1[2]) * area;
+    const float b2 = (as2[1] * as2[1] + as2[2] * as2[2]) * area;
+    float tau = 0.5f * (fabsf(sl - b1) + fabsf(sl - b2));
+    tau = tau * tau;
+    const float a0 = 0.98f * (1.f + fminf(tau * frcp(1e-6f + sl), 1e30f));
+    const float a1 = 0.01f * (1.f + fminf(tau * frcp(1e-6f + b1), 1e30f));
+    const float a2 = 0.01f * (1.f + fminf(tau * frcp(1e-6f + b2), 1e30f));
+    const float ia = frcp(a0 + a1 + a2);
+    return a0 * ia * ((1.f / 0.98f) * vl - (0.01f / 0.98f) * (vs1 + vs2)) + (a1 * ia) * vs1 + (a2 * ia) * vs2;
 }
 
+// A WLS slot on two lanes: lane h of the pair gathers the (x, y) or (z, w) halves of the point
+// values and evaluates components 2h, 2h + 1, so twice as many waves hide the gather latency
+// and each runs half the arithmetic. GHOSTS (the launch that also writes the ghosts, slots
+// [S_wp, S_w)): the points that are ghost leaves are formed from the slot's fixed-size entry
+// lists (Tables::dep_hdr) instead of read.
+template <bool GHOSTS>
+__device__ __forceinline__ void slot_half(const DevTables& t, const float* __restrict__ wf, float4* __restrict__ Qf,
+                                          int s, int h) {
+    const int32_t* __restrict__ iv = t.wls_i + (size_t)s * WLSI_STRIDE;
+    const float* __restrict__ f = wf + (size_t)iv[WLSI_PAT] * WF_STRIDE;
+    const int mw = iv[WLSI_M];
+    const int m = mw & WLSI_MMASK;
+    const float2* __restrict__ Q2 = reinterpret_cast<const float2*>(Qf);
+    float2 val[9];
+    if (!GHOSTS) {
+#pragma unroll
+        for (int i = 0; i < 9; i++) val[i] = i < m ? Q2[(size_t)2 * iv[WLSI_IDS + i] + h] : make_float2(0.f, 0.f);
+    } else {
+        const int gm = mw >> WLSI_GSHIFT;
+        const int d = s - t.S_wp;
+        const int4 hdr = t.dep_hdr[d];
+        const float2 inflow = h ? make_float2((float)t.inflow[2], (float)t.inflow[3])
+                                : make_float2((float)t.inflow[0], (float)t.inflow[1]);
+        float2 g[2];
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            const int4 l = t.dep_leaf[2 * d + p];
+            const float4 w = t.dep_w[2 * d + p];
+            const int li[4] = {l.x, l.y, l.z, l.w};
+            const float wi[4] = {w.x, w.y, w.z, w.w};
+            float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const float2 q = li[u] >= 0 ? Q2[(size_t)2 * li[u] + h]
+                                 : li[u] == DEP_INFLOW ? inflow : make_float2(0.f, 0.f);
+                acc.x += wi[u] * q.x;
+                acc.y += wi[u] * q.y;
+            }
+            g[p] = acc;
+        }
+#pragma unroll
+        for (int i = 0; i < 9; i++)
+            val[i] = hdr.x > 0 && i == hdr.y   ? g[0]
+                     : hdr.x > 1 && i == hdr.z ? g[1]
+                     : i < m && !((gm >> i) & 1) ? Q2[(size_t)2 * iv[WLSI_IDS + i] + h]
+                                                 : make_float2(0.f, 0.f);
+    }
+    float out[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        float phi[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) phi[i] = k == 0 ? val[i].x : val[i].y;
+        out[k] = slot_component(f, phi);
+    }
+    reinterpret_cast<float2*>(Qf)[(size_t)2 * (t.L + s) + h] = make_float2(out[0], out[1]);
+}
+
+// every slot, two lanes each (ghosts are filled before this launch: the ghost mask is not used)
 __global__ __launch_bounds__(128) void k_slots_fast(DevTables t, const float* __restrict__ wf,
                                                     float4* __restrict__ Qf) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    // ghosts are filled before this launch: the ghost mask is not used
-    if (s < t.S_w) slot_fast<false>(t, wf, nullptr, Qf, s, t.wls_i[(size_t)s * WLSI_STRIDE + WLSI_M] & WLSI_MMASK,
-                                    nullptr, nullptr, nullptr);
+    const int s = blockIdx.x * 64 + ((int)threadIdx.x >> 1);
+    if (s < t.S_w) slot_half<false>(t, wf, Qf, s, threadIdx.x & 1);
 }
 
 // Ghost leaves and restriction slots of the fast path from the flattened lists: item k is
@@ -693,28 +665,26 @@ __global__ __launch_bounds__(256) void k_fill_flat(DevTables t, double4* __restr
 }
 
 // One launch before each fast-path stage: the first ngb blocks evaluate the few WLS slots that
-// read a ghost leaf, forming those points from the ghosts' fill lists (their longer chains
-// start first), the next nfb blocks refill the items the stage reads (the unaliased ghosts and
-// the restriction slots, items[0, nitems)), the rest evaluate the slots that read no ghost.
-// No block waits for another.
+// read a ghost leaf, forming those points from their entry lists (their longer chains start
+// first), the next nfb blocks refill the items the stage reads (the unaliased ghosts and the
+// restriction slots, items[0, nitems)), the rest evaluate the slots that read no ghost. Slots
+// take two lanes each. No block waits for another.
 __global__ __launch_bounds__(128) void k_slots_fill(
     DevTables t, const float* __restrict__ wf, double4* Q, float4* Qf, int ngb, int nfb,
     const int32_t* __restrict__ items, int nitems, const int32_t* __restrict__ beg,
     const int32_t* __restrict__ leaf, const float* __restrict__ w) {
     const int b = (int)blockIdx.x;
     if (b < ngb) {
-        const int s = t.S_wp + b * 128 + (int)threadIdx.x;
-        if (s < t.S_w)
-            slot_fast<true>(t, wf, Q, Qf, s, t.wls_i[(size_t)s * WLSI_STRIDE + WLSI_M], beg, leaf, w);
+        const int s = t.S_wp + b * 64 + ((int)threadIdx.x >> 1);
+        if (s < t.S_w) slot_half<true>(t, wf, Qf, s, threadIdx.x & 1);
     } else if (b < ngb + nfb) {
         const int tid = (b - ngb) * 128 + (int)threadIdx.x;
         const int kk = tid >> 4;
         const bool live = kk < nitems;
         fill_row(t, Q, Qf, live ? items[kk] : 0, live, tid & 15, beg, leaf, w);
     } else {
-        const int s = (b - ngb - nfb) * 128 + (int)threadIdx.x;
-        if (s < t.S_wp)
-            slot_fast<false>(t, wf, Q, Qf, s, t.wls_i[(size_t)s * WLSI_STRIDE + WLSI_M], nullptr, nullptr, nullptr);
+        const int s = (b - ngb - nfb) * 64 + ((int)threadIdx.x >> 1);
+        if (s < t.S_wp) slot_half<false>(t, wf, Qf, s, threadIdx.x & 1);
     }
 }
 
@@ -740,12 +710,12 @@ __global__ void k_shadow(const double4* __restrict__ Q, float4* __restrict__ Qf,
 static inline int nb_(int n, int b) { return (n + b - 1) / b; }
 
 void launch_slots_fast(const DevTables& t, const float* wf, const double4* Q, float4* Qf, hipStream_t st) {
-    if (t.S_w > 0) hipLaunchKernelGGL(k_slots_fast, dim3(nb_(t.S_w, 128)), dim3(128), 0, st, t, wf, Qf);
+    if (t.S_w > 0) hipLaunchKernelGGL(k_slots_fast, dim3(nb_(t.S_w, 64)), dim3(128), 0, st, t, wf, Qf);
 }
 
 void launch_slots_fill(const DevTables& t, const float* wf, double4* Q, float4* Qf, const int32_t* items,
                        int nitems, const int32_t* beg, const int32_t* leaf, const float* w, hipStream_t st) {
-    const int nfb = nb_(16 * nitems, 128), nsb = nb_(t.S_wp, 128), ngb = nb_(t.S_w - t.S_wp, 128);
+    const int nfb = nb_(16 * nitems, 128), nsb = nb_(t.S_wp, 64), ngb = nb_(t.S_w - t.S_wp, 64);
     if (nfb + nsb + ngb > 0)
         hipLaunchKernelGGL(k_slots_fill, dim3(nfb + nsb + ngb), dim3(128), 0, st, t, wf, Q, Qf, ngb, nfb, items,
                            nitems, beg, leaf, w);

@@ -569,6 +569,9 @@ void build_tables(const Tree& t, Tables& tb) {
             tb.dep_hdr[(size_t)d * 4] = fits ? cnt : DEP_GENERIC;
         }
     }
+    tb.merge_ok = tb.fill_flat_ok;
+    for (size_t d = 0; d * 4 < tb.dep_hdr.size(); d++)
+        if (tb.dep_hdr[d * 4] == DEP_GENERIC) tb.merge_ok = false;
     tb.fill_fast.clear();
     for (int k = 0; k < tb.G + tb.S_r; k++)
         if (k >= tb.G || tb.ghost_alias[k] < 0) tb.fill_fast.push_back(k);

@@ -33,7 +33,7 @@ enum WlsIntLayout { WLSI_M = 0, WLSI_M1 = 1, WLSI_IDS = 2, WLSI_PAT = 11, WLSI_S
 constexpr int WLSI_MMASK = 0xFF, WLSI_GSHIFT = 8;
 // ghost entries of the slots that read ghost leaves (Tables::dep_hdr): up to 2 ghost points of
 // up to DEP_ENT entries each; DEP_INFLOW marks the inflow state, DEP_GENERIC a slot that does
-// not fit (its ghost points are formed from the fill lists)
+// not fit (the mesh then takes a fill launch per stage)
 constexpr int DEP_ENT = 4, DEP_INFLOW = -2, DEP_GENERIC = 255;
 
 // FP32 slot record of the fast path (floats): the least-squares solves folded into
@@ -108,6 +108,9 @@ struct Tables {
     // each entry of the ghost's fill list
     std::vector<int32_t> dep_hdr, dep_leaf;
     std::vector<float> dep_w;
+    // the fast path refills ghosts in its slot launch when the flattened fill is usable and
+    // every ghost-reading slot fits its fixed-size entry lists (else a fill launch per stage)
+    bool merge_ok = true;
     // selector input: state index of the leaf containing each of the 16 off-axis points of
     // the leaf's 5x5 stencil (x + a dx, y + b dy), a,b in -2..2, a != 0, b != 0, row-major
     // in b then a. New rule (the reference has no 5x5 stencil): descend from the base cell
