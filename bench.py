@@ -39,8 +39,14 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def workload_config(name: str, precision: int, selector: int = 0, refine_goal: float = 1.4):
+def workload_config(name: str, precision: int, selector: int = 0, refine_goal: float = 1.4,
+                    coarse_goal: float = None):
+    """Solver configuration of a named workload. refine_goal / coarse_goal: the regrid's
+    refinement and coarsening goals (multiples of the sensor's RMS, solution.F90:519-535);
+    coarse_goal None keeps the solver's default."""
     import fdmamr as F
+
+    cg = {} if coarse_goal is None else dict(coarse_goal_rot=coarse_goal, coarse_goal_div=coarse_goal)
 
     if name == "vortex":
         # isentropic vortex, periodic [0,10]^2, base 64x64, lvlAMR 2 (the reference's criterion)
@@ -53,7 +59,7 @@ def workload_config(name: str, precision: int, selector: int = 0, refine_goal: f
                         ic_param=(0.5, 0.5), localdt=1, cfl=0.5, t_end=0.2, amr=1, lvl_amr=4,
                         pstep_adapt=10, amr_sensor=1, pre_adapt=4, wls_local=1,
                         refine_goal_rot=refine_goal, refine_goal_div=refine_goal,
-                        flux_precision=precision, selector=selector)
+                        flux_precision=precision, selector=selector, **cg)
     if name == "shock4bubbles":
         # Mach 3 shock through 4 seeded bubbles, [0,4]x[0,2], base 1024x512, lvl <= 4
         g, M = 1.4, 3.0
@@ -66,7 +72,7 @@ def workload_config(name: str, precision: int, selector: int = 0, refine_goal: f
                         ic_param=(0.5, M, 4, 0.05, 0.15, 0.1, 5.0), localdt=1, cfl=0.5,
                         t_end=1e9, amr=1, lvl_amr=4, pstep_adapt=10, amr_sensor=1, pre_adapt=4,
                         wls_local=1, refine_goal_rot=refine_goal, refine_goal_div=refine_goal,
-                        flux_precision=precision, selector=selector)
+                        flux_precision=precision, selector=selector, **cg)
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -88,12 +94,13 @@ def cpu_baseline(budget_s: float = 12.0):
                        f"{steps} steps in {dt:.1f}s")
 
 
-def measure_traffic(args):
+def measure_traffic(args, mesh_file):
     """HBM bytes of one stage's flux/update launches of the learned path, from two rocprofv3
     counter runs of a short copy of this benchmark (FETCH_SIZE, then WRITE_SIZE, in KiB per
     dispatch). Each counter is scaled by what it reads for a plain 256 MiB device copy in the
     same run (on MI355X FETCH_SIZE counts a 128-byte read as 64 bytes). Returns None when
-    rocprofv3 is unavailable or a run fails."""
+    rocprofv3 is unavailable or a run fails. The runs load the measured mesh and state from
+    mesh_file instead of building them again."""
     import csv
     import glob
     import shutil
@@ -109,16 +116,22 @@ def measure_traffic(args):
              "--train-iters", "20", "--no-cpu", "--no-traffic", "--traffic-child",
              "--copy-calibration", str(copy_mib), "--workload", args.workload,
              "--refine-goal", str(args.refine_goal), "--advance", str(args.advance),
+             "--develop-t", str(args.develop_t), "--develop-lvl", str(args.develop_lvl),
+             "--refine-passes", str(args.refine_passes), "--relax-steps", str(args.relax_steps),
+             "--mesh-file", mesh_file,
              "--threshold", str(args.threshold), "--label-bound", str(args.label_bound)]
     res = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         d = tempfile.mkdtemp(prefix="fdmamr_pmc_")
         cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--"] + child
+        log(f"traffic: {counter} counter run")
+        t0 = time.perf_counter()
         try:
             r = subprocess.run(cmd, capture_output=True, text=True, timeout=400)
         except subprocess.TimeoutExpired:
             log(f"traffic: {counter} run timed out")
             return None
+        log(f"traffic: {counter} run done in {time.perf_counter() - t0:.1f}s")
         info = [json.loads(l) for l in r.stdout.splitlines() if l.startswith("{")]
         files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
         if r.returncode != 0 or not info or not files:
@@ -194,14 +207,79 @@ def pretrain_selector(seed, rounds, iters, threshold, label_bound):
     return w
 
 
+def measured_config(args, precision, selector=0):
+    import fdmamr as F
+    dev = args.workload == "shock4bubbles" and args.develop_t > 0
+    return workload_config(args.workload, precision, selector=selector, refine_goal=args.refine_goal,
+                           coarse_goal=args.refine_goal / 3.0 if dev else None)
+
+
+def build_mesh(args, F):
+    """The measured mesh and state (FP64 context) and the time of one regrid on it.
+
+    shock4bubbles with --develop-t: the flow is evolved at a coarse maximum level (the
+    standard goals, regrid every 40 steps) to a developed state (the shock through the
+    bubbles, reflected waves and vortices), regridded --refine-passes times at level 4 with
+    the measured goals, evolved --relax-steps on that mesh (new leaves start from their
+    parent's value, as in the reference's refinement), and regridded once more (timed).
+    Otherwise: the workload's initial adaptation, --advance steps, then one timed regrid."""
+    t0 = time.perf_counter()
+    if args.workload == "shock4bubbles" and args.develop_t > 0:
+        dev = workload_config(args.workload, F.FLUX_FP64, refine_goal=0.3)
+        dev.lvl_amr = dev.pre_adapt = args.develop_lvl
+        s = F.Solver(dev).setup()
+        steps = 0
+        while s.tt < args.develop_t:
+            s.run_steps(40)
+            s.regrid()
+            steps += 40
+        log(f"developed at level {args.develop_lvl}: {steps} steps to t = {s.tt:.4f}, "
+            f"{s.counts()['n_leaf']} leaves, {time.perf_counter() - t0:.1f}s")
+        s.cfg = measured_config(args, F.FLUX_FP64)
+        for _ in range(args.refine_passes):
+            s.regrid()
+        if args.relax_steps > 0:
+            s.run_steps(args.relax_steps)
+    else:
+        s = F.Solver(measured_config(args, F.FLUX_FP64)).setup()
+        for _ in range(args.advance // 10):
+            s.run_steps(10)
+            s.regrid()
+        if args.advance:
+            log(f"advanced {args.advance} steps with regrid every 10: t = {s.tt:.4f}")
+    t_setup = time.perf_counter() - t0
+    # regrid cost (host refine/coarsen/balance + tables + upload), timed on its own
+    t0 = time.perf_counter()
+    s.regrid()
+    t_regrid = time.perf_counter() - t0
+    cnt = s.counts()
+    log(f"workload {args.workload}: {cnt['n_leaf']} leaves, {cnt['n_slot_wls']} WLS slots, "
+        f"{cnt['n_slot_restrict']} restriction slots, l_max {cnt['l_max']}, setup {t_setup:.1f}s, "
+        f"regrid {t_regrid:.2f}s")
+    return s, t_regrid
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="shock4bubbles")
-    ap.add_argument("--refine-goal", type=float, default=0.3)
+    ap.add_argument("--refine-goal", type=float, default=0.01,
+                    help="refinement goal of the measured mesh (its coarsening goal is a third)")
+    ap.add_argument("--develop-t", type=float, default=0.4,
+                    help="shock4bubbles: evolve the flow to this time at --develop-lvl (regrid "
+                         "every 40 steps) before refining to level 4; 0 measures on the initial "
+                         "condition's mesh")
+    ap.add_argument("--develop-lvl", type=int, default=1)
+    ap.add_argument("--refine-passes", type=int, default=3,
+                    help="regrids at level 4 after the development (the timed regrid follows)")
+    ap.add_argument("--relax-steps", type=int, default=120,
+                    help="steps on the refined mesh before its timed regrid (the refinement copies "
+                         "coarse values, and the fine solution needs these steps to lose the steps)")
     ap.add_argument("--train-iters", type=int, default=600)
+    ap.add_argument("--tune-iters", type=int, default=1000,
+                    help="minibatches of each online fine-tune round on the workload")
     ap.add_argument("--advance", type=int, default=0,
                     help="evolve the flow this many steps (regrid every 10) before measuring")
     ap.add_argument("--no-cpu", action="store_true")
@@ -214,13 +292,11 @@ def main():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter runs that measure HBM traffic")
     ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--mesh-file", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         raise SystemExit(2)
-
-    # counter runs first, in child processes, before this process touches the GPU
-    traffic = None if (args.no_traffic or args.traffic_child) else measure_traffic(args)
 
     if args.copy_calibration:
         # torch (when used) is loaded before the solver library, as in the tests
@@ -239,43 +315,31 @@ def main():
     w = pretrain_selector(7, rounds=6, iters=args.train_iters, threshold=args.threshold,
                           label_bound=args.label_bound)
 
-    t0 = time.perf_counter()
-    s64 = F.Solver(workload_config(args.workload, F.FLUX_FP64, refine_goal=args.refine_goal)).setup()
-    t_setup = time.perf_counter() - t0
-    cnt = s64.counts()
-    N = cnt["n_leaf"]
-    log(f"workload {args.workload}: {N} leaves, {cnt['n_slot_wls']} WLS slots, "
-        f"{cnt['n_slot_restrict']} restriction slots, l_max {cnt['l_max']}, setup {t_setup:.1f}s")
-
-    t0 = time.perf_counter()
-    for _ in range(args.advance // 10):
-        s64.run_steps(10)
-        s64.regrid()
-    if args.advance:
-        log(f"advanced {args.advance} steps with regrid every 10: t = {s64.tt:.4f}, "
-            f"{s64.counts()['n_leaf']} leaves, {time.perf_counter() - t0:.1f}s")
-    # regrid cost (host refine/coarsen/balance + tables + upload), timed on its own
-    t0 = time.perf_counter()
-    s64.regrid()
-    t_regrid = time.perf_counter() - t0
-    cnt = s64.counts()
-    N = cnt["n_leaf"]
-    tree = s64.tree()
-    prim0 = s64.get_state()
-    log(f"regrid {t_regrid:.2f}s -> {N} leaves")
+    if args.mesh_file:
+        # a counter run: the parent's mesh and state
+        z = np.load(args.mesh_file)
+        tree = {k: z[k] for k in z.files if k != "prim"}
+        prim0 = z["prim"]
+        s64, t_regrid = None, 0.0
+    else:
+        s64, t_regrid = build_mesh(args, F)
+        tree = s64.tree()
+        prim0 = s64.get_state()
 
     # learned path: online fine-tune on this workload while the judge rejects > 1%
-    sel = clone(workload_config(args.workload, F.FLUX_FP32, selector=1, refine_goal=args.refine_goal),
-                tree, prim0)
+    sel = clone(measured_config(args, F.FLUX_FP32, selector=1), tree, prim0)
+    cnt = sel.counts()
+    N = cnt["n_leaf"]
     sel.selector_config(args.threshold, 0.02, args.label_bound)
     sel.selector_set_weights(w)
     sel.run_steps(args.warmup)
     st = sel.selector_stats(reset=True)
     rate = st["rejected"] / max(st["judged"], 1)
     tuned = 0
-    while rate > 0.01 and tuned < 3:
+    # (the counter runs' copy fine-tunes at most once: it only needs a representative stage)
+    while rate > 0.01 and tuned < (1 if args.traffic_child else 3):
         sel.selector_collect()
-        sel.selector_train(200, batch=4096, lr=1e-3, pos_weight=4.0, seed=11 + tuned)
+        sel.selector_train(args.tune_iters, batch=4096, lr=1e-3, pos_weight=4.0, seed=11 + tuned)
         sel.selector_clear_dataset()
         sel.set_state(prim0)
         sel.build_tables()
@@ -288,13 +352,13 @@ def main():
     # restart both paths from the same state on the same mesh
     sel.build_tables()
     sel.set_state(prim0)
-    s64.set_state(prim0)
     if args.traffic_child:
         # marker dispatch for the counter parser, then the learned steps it measures
         sel.check_positivity()
         sel.run_steps(args.steps)
         print(json.dumps({"n_leaf": N, "steps": args.steps}))
         return
+    s64.set_state(prim0)
 
     def timed(s, label):
         s.run_steps(args.warmup)  # synchronises at the end
@@ -338,6 +402,21 @@ def main():
     # leaf's own FP32 value is the rounding of its FP64 state, so it is not counted
     bytes_per_leaf = 32 + 4 + 1 + 32 + 32 + 16 + 32 * 2.0 / 3.0
     achieved = N * bytes_per_leaf / (k_l * 1e-3) / 1e9
+
+    # HBM traffic of the same stage launch: two counter runs in child processes on this mesh
+    # and state (the measured contexts are released first)
+    traffic = None
+    if not args.no_traffic:
+        import tempfile
+        sel.close()
+        s64.close()
+        fd, mesh_file = tempfile.mkstemp(prefix="fdmamr_mesh_", suffix=".npz")
+        os.close(fd)
+        try:
+            np.savez(mesh_file, prim=prim0, **tree)
+            traffic = measure_traffic(args, mesh_file)
+        finally:
+            os.remove(mesh_file)
     out = {
         "metric": "leaf-cell RK3 updates/s: learned+FP32 vs plain FP64 at L1(rho) err <=1e-4",
         "value": ups,
@@ -353,6 +432,11 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "n_leaf": N, "l_max": cnt["l_max"],
                    "wls_slots": cnt["n_slot_wls"], "refine_goal": args.refine_goal,
+                   "coarse_goal": (args.refine_goal / 3.0 if args.workload == "shock4bubbles"
+                                   and args.develop_t > 0 else None),
+                   "developed": ({"t": args.develop_t, "max_level": args.develop_lvl,
+                                  "refine_passes": args.refine_passes, "relax_steps": args.relax_steps}
+                                 if args.workload == "shock4bubbles" and args.develop_t > 0 else None),
                    "base": "1024x512" if args.workload == "shock4bubbles" else None,
                    "selector_threshold": args.threshold, "label_bound": args.label_bound},
         "fp64_value": ups64,
