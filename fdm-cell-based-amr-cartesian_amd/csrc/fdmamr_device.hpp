@@ -160,11 +160,12 @@ void launch_stage_fast(const DevTables& t, const int32_t* lists, const int* meta
                        double* lampart, float* label, const JudgeArgs& jd, hipStream_t st);
 int stage_fast_partial_blocks(int n, bool lists);
 // leaf lists from selector flags (bit 0: troubled); blk_cnt holds list_blocks(n) ints, the
-// troubled count of each block of LIST_BLK leaves. launch_list_write is the second pass alone,
-// for counts written by another kernel.
+// troubled count of each block of LIST_BLK leaves (turned into the blocks' prefix offsets).
+// launch_list_write is the prefix scan and second pass alone, for counts written by another
+// kernel.
 constexpr int LIST_BLK = 512;
 void launch_build_lists(int n, const uint8_t* flags, int* blk_cnt, int32_t* lists, int* meta, hipStream_t st);
-void launch_list_write(int n, const uint8_t* flags, const int* blk_cnt, int32_t* lists, int* meta,
+void launch_list_write(int n, const uint8_t* flags, int* blk_cnt, int32_t* lists, int* meta,
                        hipStream_t st);
 int list_blocks(int n);
 void launch_shadow(const double4* Q, float4* Qf, int n, hipStream_t st);

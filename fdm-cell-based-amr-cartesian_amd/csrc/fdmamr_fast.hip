@@ -420,9 +420,9 @@ __global__ __launch_bounds__(256) void k_stage_full(DevTables t, const double4* 
 }
 
 // ---------------------------------------------------------------- leaf lists from flags
-// Deterministic compaction in two passes over blocks of LIST_BLK leaves: troubled counts per
-// block (this kernel, or the selector's inference kernel, which counts the same blocks), then
-// each block places its leaves after the counts of the blocks before it.
+// Deterministic compaction over blocks of LIST_BLK leaves: troubled counts per block, their
+// exclusive prefix (one block), then each block places its leaves after those of the blocks
+// before it.
 
 __device__ __forceinline__ int block_sum256(int v, int* sh) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -446,19 +446,44 @@ __global__ __launch_bounds__(256) void k_list_count(int n, const uint8_t* __rest
     if (threadIdx.x == 0) cnt[blockIdx.x] = c;
 }
 
-__global__ __launch_bounds__(256) void k_list_write(int n, const uint8_t* __restrict__ flags,
-                                                    const int* __restrict__ cnt, int nblk,
-                                                    int32_t* __restrict__ lists, int* __restrict__ meta) {
-    __shared__ int sh[4], ws[4];
-    int pre = 0, tot = 0;
-    for (int j = threadIdx.x; j < nblk; j += 256) {
-        const int v = cnt[j];
-        tot += v;
-        if (j < (int)blockIdx.x) pre += v;
+// Exclusive prefix of the block counts, in place (one block of 1024 threads, each a contiguous
+// chunk), and the list split: meta[0] = cleared leaves, meta[1] = troubled leaves.
+__global__ __launch_bounds__(1024) void k_list_scan(int n, int* __restrict__ cnt, int nblk, int* __restrict__ meta) {
+    __shared__ int ws[16];
+    const int per = (nblk + 1023) / 1024;
+    const int j0 = threadIdx.x * per, j1 = min(nblk, j0 + per);
+    int mine = 0;
+    for (int j = j0; j < j1; j++) mine += cnt[j];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int inc = mine;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(inc, o);
+        if (lane >= o) inc += y;
     }
-    pre = block_sum256(pre, sh);
-    tot = block_sum256(tot, sh);
-    const int n_clear = n - tot;
+    if (lane == 63) ws[wv] = inc;
+    __syncthreads();
+    int woff = 0, tot = 0;
+    for (int w = 0; w < 16; w++) {
+        woff += w < wv ? ws[w] : 0;
+        tot += ws[w];
+    }
+    int run = woff + inc - mine;
+    for (int j = j0; j < j1; j++) {
+        const int v = cnt[j];
+        cnt[j] = run;
+        run += v;
+    }
+    if (threadIdx.x == 0) { meta[0] = n - tot; meta[1] = tot; }
+}
+
+// each block places its leaves after the troubled / cleared leaves of the blocks before it
+// (pre: the block's exclusive prefix from k_list_scan)
+__global__ __launch_bounds__(256) void k_list_write(int n, const uint8_t* __restrict__ flags,
+                                                    const int* __restrict__ cnt,
+                                                    int32_t* __restrict__ lists, const int* __restrict__ meta) {
+    __shared__ int ws[4];
+    const int pre = cnt[blockIdx.x];
+    const int n_clear = meta[0];
     // LPT consecutive leaves per thread; exclusive scan of the troubled counts over the block
     constexpr int LPT = LIST_BLK / 256;
     const int i0 = blockIdx.x * LIST_BLK + LPT * threadIdx.x;
@@ -488,7 +513,6 @@ __global__ __launch_bounds__(256) void k_list_write(int n, const uint8_t* __rest
             else lists[c_pos++] = i;
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) { meta[0] = n_clear; meta[1] = tot; }
 }
 
 // ---------------------------------------------------------------- FP32 WLS-WENO slots
@@ -753,11 +777,12 @@ void launch_build_lists(int n, const uint8_t* flags, int* blk_cnt, int32_t* list
     launch_list_write(n, flags, blk_cnt, lists, meta, st);
 }
 
-void launch_list_write(int n, const uint8_t* flags, const int* blk_cnt, int32_t* lists, int* meta,
+void launch_list_write(int n, const uint8_t* flags, int* blk_cnt, int32_t* lists, int* meta,
                        hipStream_t st) {
     if (n <= 0) return;
     const int nblk = nb_(n, LIST_BLK);
-    hipLaunchKernelGGL(k_list_write, dim3(nblk), dim3(256), 0, st, n, flags, blk_cnt, nblk, lists, meta);
+    hipLaunchKernelGGL(k_list_scan, dim3(1), dim3(1024), 0, st, n, blk_cnt, nblk, meta);
+    hipLaunchKernelGGL(k_list_write, dim3(nblk), dim3(256), 0, st, n, flags, blk_cnt, lists, meta);
 }
 
 int list_blocks(int n) { return nb_(n, LIST_BLK); }
