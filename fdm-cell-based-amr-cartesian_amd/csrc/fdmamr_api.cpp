@@ -50,8 +50,6 @@ struct fdmamr_ctx {
     DBuf<float> label;       // per-leaf WCNS weight deviation (labels / judge)
     bool use_flags = false;  // fast path consults the flags
     DBuf<int32_t> lists;     // cleared leaves, then troubled leaves (from the flags)
-    DBuf<uint16_t> tile_mask;  // selector scratch: leaves of each 16-leaf tile needing the network
-    DBuf<uint4> tile_frag;     // selector scratch: their layer-1 MFMA fragments
     int n_cu = 256;
     DBuf<int> list_cnt, list_meta;
 
@@ -210,11 +208,6 @@ int upload_tables(fdmamr_ctx* c) {
     if (!tb.fill_fast.empty()) HIPCHK(c, up(c->fill_fast.p, tb.fill_fast.data(), tb.fill_fast.size() * 4));
     HIPCHK(c, ensure(c->off16, tb.off16.size()));
     HIPCHK(c, ensure(c->sel_prob, tb.N));
-    if (c->p.flux_precision == FDMAMR_FLUX_FP32) {
-        const size_t ntiles = ((size_t)tb.N + 15) / 16;
-        HIPCHK(c, ensure(c->tile_mask, ntiles));
-        HIPCHK(c, ensure(c->tile_frag, ntiles * 3 * 64));
-    }
     HIPCHK(c, up(c->off16.p, tb.off16.data(), tb.off16.size() * 4));
     HIPCHK(c, ensure(c->sel_nbr_beg, tb.sel_nbr_beg.size()));
     HIPCHK(c, ensure(c->sel_nbr, tb.sel_nbr.size()));
@@ -340,9 +333,8 @@ void enqueue_rk3(fdmamr_ctx* c, int advance_time, bool prepare_next) {
         else launch_slots(d, c->Q[in[k]].p, c->T[in[k]].p, s);
         if (select && k == 0) {
             SelectParams sp{c->sel_threshold, (uint32_t)(c->sel_judge * 65536.0f), c->sel_step++};
-            launch_select(d, c->Qf[0].p, c->off16.p, c->sel_pk.p, c->sel_w.p, sp, c->flags.p, nullptr,
-                          c->tile_mask.p, c->tile_frag.p, 4 * c->n_cu, c->sel_nbr_beg.p, c->sel_nbr.p, c->sel_loc.p,
-                          s);
+            launch_select(d, c->Qf[0].p, c->off16.p, c->sel_pk.p, c->sel_w.p, sp, c->flags.p, nullptr, c->n_cu,
+                          c->sel_nbr_beg.p, c->sel_nbr.p, c->sel_loc.p, s);
             launch_build_lists(d.N, c->flags.p, c->list_cnt.p, c->lists.p, c->list_meta.p, s);
         }
         int e0 = -1, e1 = -1;
@@ -422,7 +414,7 @@ void fdmamr_destroy(fdmamr_ctx* c) {
     release(c->sel_grad); release(c->sel_gpart); release(c->sel_lpart); release(c->sel_loss);
     release(c->sel_prob); release(c->sel_pk); release(c->sel_cnt); release(c->ds_x); release(c->ds_y);
     release(c->ds_idx); release(c->lists); release(c->list_cnt); release(c->list_meta);
-    release(c->tile_mask); release(c->tile_frag);
+
     release(c->fill_beg); release(c->fill_leaf); release(c->fill_w); release(c->fill_fast);
     release(c->dep_hdr); release(c->dep_leaf); release(c->dep_w);
     release(c->sel_nbr_beg); release(c->sel_nbr); release(c->sel_loc);
@@ -918,9 +910,8 @@ int32_t fdmamr_selector_infer(fdmamr_ctx* c, float* prob, int32_t* n_troubled) {
     std::vector<int> cnt0(CNT_N * CNT_SLOTS, 0);
     HIPCHK(c, hipMemcpyAsync(c->sel_cnt.p, cnt0.data(), cnt0.size() * 4, hipMemcpyHostToDevice, c->stream));
     SelectParams sp{c->sel_threshold, 0u, c->sel_step};  // no judge sample here
-    launch_select(c->dt, c->Qf[0].p, c->off16.p, c->sel_pk.p, c->sel_w.p, sp, c->flags.p, c->sel_prob.p,
-                  c->tile_mask.p, c->tile_frag.p, 4 * c->n_cu, c->sel_nbr_beg.p, c->sel_nbr.p, c->sel_loc.p,
-                  c->stream);
+    launch_select(c->dt, c->Qf[0].p, c->off16.p, c->sel_pk.p, c->sel_w.p, sp, c->flags.p, c->sel_prob.p, c->n_cu,
+                  c->sel_nbr_beg.p, c->sel_nbr.p, c->sel_loc.p, c->stream);
     // no judge sample was drawn: the judge pass only counts the troubled leaves
     launch_judge(N, c->label.p, c->sel_bound, c->flags.p, c->sel_cnt.p, c->stream);
     launch_build_lists(N, c->flags.p, c->list_cnt.p, c->lists.p, c->list_meta.p, c->stream);

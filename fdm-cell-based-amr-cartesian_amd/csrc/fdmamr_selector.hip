@@ -285,93 +285,32 @@ __device__ __forceinline__ uint8_t decide(uint8_t old, bool troubled, int leaf, 
     return f;
 }
 
-// Inference in two passes.
-//
-// Pass 1 (k_select_gate), one 16-leaf tile per wave at full occupancy: gathers the 5x5
-// stencils, clears near-uniform leaves without the network (their flags and probability are
-// final here), and for a tile with any other leaf stores its layer-1 B fragments (the feature
-// gather of lane (r, g) yields exactly the 24 K-slots of leaf r: feature q = 8s + j of the lane
-// is channel q % 3 of its point g + 4 (q / 3)) and the mask of leaves that need the network.
-//
-// Pass 2 (k_select_mlp), a persistent grid with the weights in LDS as lane-contiguous
-// fragments: runs the network on the stored tiles (tiles are dealt round-robin, which spreads
-// the spatially clustered non-uniform tiles evenly); every activation stays in registers,
-// each layer's accumulators being the next layer's B fragments.
-__global__ __launch_bounds__(256) void k_select_gate(DevTables t, const float4* __restrict__ Qf,
-                                                     const int32_t* __restrict__ off16, SelectParams sp,
-                                                     uint8_t* __restrict__ flags, float* __restrict__ prob,
-                                                     uint16_t* __restrict__ tile_mask,
-                                                     uint4* __restrict__ tile_frag,
-                                                     const int32_t* __restrict__ nbr_beg,
-                                                     const int32_t* __restrict__ nbr,
-                                                     const uint2* __restrict__ loc) {
-    __shared__ float4 sQ[4][SEL_NBR_CAP];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
-    const int tile = xcd_block() * 4 + wv;
-    const int ntiles = (t.N + 15) / 16;
-    const bool live = tile < ntiles;
-    const int leaf = tile * 16 + r;
-    const bool valid = live && leaf < t.N;
-    // the tile's neighbourhood into LDS (every wave reaches the barrier)
-    int nU = SEL_NBR_CAP + 1, b = 0;
-    if (live) {
-        b = nbr_beg[tile];
-        nU = nbr_beg[tile + 1] - b;
-    }
-    const bool local = nU <= SEL_NBR_CAP;
-    const uint2 lc = loc[(size_t)(valid ? leaf : 0) * 4 + g];
-    const uint8_t old = valid ? flags[leaf] : 0;
-    if (local)
-        for (int k = lane; k < nU; k += 64) sQ[wv][k] = Qf[nbr[b + k]];
-    __syncthreads();
-    if (!live) return;
-    float v[7][3];
-    bool uniform;
-    if (local) {
-        // positions are below nU (checked when the tables are built); leaves past N use entry 0
-        const uint8_t* pos = reinterpret_cast<const uint8_t*>(&lc);
-        float4 q[7];
-#pragma unroll
-        for (int k = 0; k < 7; k++) q[k] = sQ[wv][valid ? (int)pos[k] : 0];
-        // the centre is point 12 = lane g 0, k 3 of the same leaf
-        const int pc = __shfl(valid ? (int)pos[3] : 0, r);
-        const float4 qc = sQ[wv][pc];
-        uniform = features_from(qc, q, valid, v);
-    } else {
-        int idx[7];
-        gather_idx(t, off16, leaf, valid, idx);
-        uniform = gather_values(Qf, leaf, valid, idx, v);
-    }
-    // mask of the tile's leaves that need the network (bit r), the same in every lane
-    const uint64_t need = __ballot(valid && !uniform);
-    const uint16_t mask = (uint16_t)(need & 0xFFFFu);
-    if (lane == 0) tile_mask[tile] = mask;
-    if (valid && g == 0 && uniform) {
-        if (prob) prob[leaf] = 0.f;
-        flags[leaf] = decide(old, false, leaf, sp);
-    }
-    if (mask) {
-        uint4* dst = tile_frag + (size_t)tile * 3 * 64;
-#pragma unroll
-        for (int s3 = 0; s3 < 3; s3++) {
-            bf16x8 x;
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const int q = 8 * s3 + j;
-                x[j] = (__bf16)(q < 21 ? v[q / 3][q % 3] : 0.f);
-            }
-            dst[s3 * 64 + lane] = *reinterpret_cast<const uint4*>(&x);
-        }
-    }
+// Inference in one pass: a persistent grid whose blocks hold the network's weights in LDS (as
+// lane-contiguous fragments) and whose waves each take one 16-leaf tile at a time: the tile's
+// neighbourhood of the FP32 state goes to the wave's LDS region, the 5x5 features come from
+// there, near-uniform leaves are cleared without the network, and for the others the gathered
+// features are already the layer-1 B fragments (feature q = 8s + j of lane (r, g) is channel
+// q % 3 of point g + 4 (q / 3)), so the network runs on them in registers: every layer's
+// accumulators are the next layer's B fragments. Each XCD takes a contiguous eighth of the
+// tiles (its waves stride over it), so neighbouring tiles share that XCD's L2.
+constexpr int SF_WAVES = 10;  // waves per block: weights + neighbourhoods fit 64 KB of LDS
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__global__ __launch_bounds__(256) void k_select_mlp(int n_leaf, const uint16_t* __restrict__ wpk,
-                                                    const float* __restrict__ w32, SelectParams sp,
-                                                    const uint16_t* __restrict__ tile_mask,
-                                                    const uint4* __restrict__ tile_frag,
-                                                    uint8_t* __restrict__ flags, float* __restrict__ prob) {
+__global__ __launch_bounds__(SF_WAVES * 64) void k_select(DevTables t, const float4* __restrict__ Qf,
+                                                          const int32_t* __restrict__ off16, SelectParams sp,
+                                                          uint8_t* __restrict__ flags, float* __restrict__ prob,
+                                                          const int32_t* __restrict__ nbr_beg,
+                                                          const int32_t* __restrict__ nbr,
+                                                          const uint2* __restrict__ loc,
+                                                          const uint16_t* __restrict__ wpk,
+                                                          const float* __restrict__ w32) {
     __shared__ __align__(16) uint16_t sF[KF_TOTAL];
     __shared__ __align__(16) float sB[SB_TOTAL];
+    __shared__ float4 sQ[SF_WAVES][SEL_NBR_CAP];
     {
         const uint4* src = reinterpret_cast<const uint4*>(wpk + KW_FRAG);
         uint4* dst = reinterpret_cast<uint4*>(sF);
@@ -386,111 +325,89 @@ __global__ __launch_bounds__(256) void k_select_mlp(int n_leaf, const uint16_t* 
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
-    const int ntiles = (n_leaf + 15) / 16;
-    // Block b owns the tiles b, b + nblk, ... (spread over the mesh, so every block gets about
-    // the same share of the spatially clustered tiles that need the network). Its four waves
-    // read the same 64 masks at a time (one lane each) and take every fourth tile that needs
-    // the network, so the block's share is split evenly; each wave runs its tiles in pairs, so
-    // every weight fragment read from LDS feeds two MFMAs.
-    const int nblk = gridDim.x;
-    for (int c0 = 0; (int)blockIdx.x + c0 * nblk < ntiles; c0 += 64) {
-        const int my_tile = (int)blockIdx.x + (c0 + lane) * nblk;
-        const uint16_t my_mask = my_tile < ntiles ? tile_mask[my_tile] : 0;
-        const uint64_t need = __ballot(my_mask != 0);
-        uint64_t todo = 0;
-        {
-            uint64_t rest = need;
-            for (int idx = 0; rest; idx++) {
-                const uint64_t low = rest & (~rest + 1);
-                if ((idx & 3) == wv) todo |= low;
-                rest ^= low;
+    const int ntiles = (t.N + 15) / 16;
+    // this block's XCD range of tiles (the grid is a multiple of 8 blocks)
+    const int x = blockIdx.x & 7, xb = blockIdx.x >> 3, nbx = gridDim.x >> 3;
+    const int lo = (int)(((long long)ntiles * x) >> 3), hi = (int)(((long long)ntiles * (x + 1)) >> 3);
+    for (int tile = lo + xb * SF_WAVES + wv; tile < hi; tile += nbx * SF_WAVES) {
+        const int leaf = tile * 16 + r;
+        const bool valid = leaf < t.N;
+        const int b = nbr_beg[tile];
+        const int nU = nbr_beg[tile + 1] - b;
+        const bool local = nU <= SEL_NBR_CAP;
+        const uint2 lc = loc[(size_t)(valid ? leaf : 0) * 4 + g];
+        const uint8_t old = valid && g == 0 ? flags[leaf] : 0;
+        if (local)
+            for (int k = lane; k < nU; k += 64) sQ[wv][k] = Qf[nbr[b + k]];
+        wave_lds_sync();
+        float v[7][3];
+        bool uniform;
+        if (local) {
+            // positions are below nU (checked when the tables are built); leaves past N use entry 0
+            const uint8_t* pos = reinterpret_cast<const uint8_t*>(&lc);
+            float4 q[7];
+#pragma unroll
+            for (int k = 0; k < 7; k++) q[k] = sQ[wv][valid ? (int)pos[k] : 0];
+            // the centre is point 12 = lane g 0, k 3 of the same leaf
+            const int pc = __shfl(valid ? (int)pos[3] : 0, r);
+            const float4 qc = sQ[wv][pc];
+            uniform = features_from(qc, q, valid, v);
+        } else {
+            int idx[7];
+            gather_idx(t, off16, leaf, valid, idx);
+            uniform = gather_values(Qf, leaf, valid, idx, v);
+        }
+        // the next tile's neighbourhood overwrites this one only after every lane read it
+        wave_lds_sync();
+        const uint64_t need = __ballot(valid && !uniform);
+        const uint16_t mask = (uint16_t)(need & 0xFFFFu);
+        if (valid && g == 0 && uniform) {
+            if (prob) prob[leaf] = 0.f;
+            flags[leaf] = decide(old, false, leaf, sp);
+        }
+        if (!mask) continue;
+        bf16x8 xb3[3];
+#pragma unroll
+        for (int s3 = 0; s3 < 3; s3++) {
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int q = 8 * s3 + j;
+                xb3[s3][j] = (__bf16)(q < 21 ? v[q / 3][q % 3] : 0.f);
             }
         }
-        while (todo) {
-            // weight fragments are re-read from LDS per pair (hoisting them all into
-            // registers would cut the occupancy to two waves)
-            __asm__ volatile("" ::: "memory");
-            int kk[2];
-            kk[0] = __ffsll((long long)todo) - 1;
-            todo &= todo - 1;
-            const bool two = todo != 0;
-            kk[1] = two ? __ffsll((long long)todo) - 1 : kk[0];
-            if (two) todo &= todo - 1;
-            bf16x8 xb[2][3];
-            int leaf[2];
-            bool mine[2];
-            uint8_t old[2];
+        f32x4 z[4];
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const int tile = (int)blockIdx.x + (c0 + kk[h]) * nblk;
-                const uint4* src = tile_frag + (size_t)tile * 3 * 64;
+        for (int mt = 0; mt < 4; mt++) {
+            z[mt] = *reinterpret_cast<const f32x4*>(sB + SB_B1 + 16 * mt + 4 * g);
 #pragma unroll
-                for (int s3 = 0; s3 < 3; s3++) {
-                    const uint4 u = src[s3 * 64 + lane];
-                    xb[h][s3] = *reinterpret_cast<const bf16x8*>(&u);
-                }
-                const uint16_t mask = (uint16_t)__shfl((int)my_mask, kk[h]);
-                leaf[h] = tile * 16 + r;
-                mine[h] = ((mask >> r) & 1) && (h == 0 || two);
-                old[h] = mine[h] && g == 0 ? flags[leaf[h]] : 0;
-            }
-            f32x4 z[2][4];
+            for (int s3 = 0; s3 < 3; s3++) z[mt] = mfma32(frag(sF + KF_L1, mt * 3 + s3), xb3[s3], z[mt]);
+        }
+#pragma unroll
+        for (int L = 0; L < 2; L++) {
+            const bf16x8 hb0 = relu_pack(z[0], z[1]), hb1 = relu_pack(z[2], z[3]);
+            const uint16_t* W = sF + (L == 0 ? KF_L2 : KF_L3);
+            const int boff = L == 0 ? SB_B2 : SB_B3;
 #pragma unroll
             for (int mt = 0; mt < 4; mt++) {
-                const f32x4 b1 = *reinterpret_cast<const f32x4*>(sB + SB_B1 + 16 * mt + 4 * g);
-                z[0][mt] = b1;
-                z[1][mt] = b1;
-#pragma unroll
-                for (int s3 = 0; s3 < 3; s3++) {
-                    const bf16x8 w = frag(sF + KF_L1, mt * 3 + s3);
-                    z[0][mt] = mfma32(w, xb[0][s3], z[0][mt]);
-                    z[1][mt] = mfma32(w, xb[1][s3], z[1][mt]);
-                }
+                f32x4 acc = *reinterpret_cast<const f32x4*>(sB + boff + 16 * mt + 4 * g);
+                acc = mfma32(frag(W, mt * 2 + 0), hb0, acc);
+                acc = mfma32(frag(W, mt * 2 + 1), hb1, acc);
+                z[mt] = acc;
             }
+        }
+        float part = 0.f;
 #pragma unroll
-            for (int L = 0; L < 2; L++) {
-                bf16x8 hb[2][2];
+        for (int mt = 0; mt < 4; mt++) {
+            const f32x4 w4 = *reinterpret_cast<const f32x4*>(sB + SB_W4 + 16 * mt + 4 * g);
 #pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    hb[h][0] = relu_pack(z[h][0], z[h][1]);
-                    hb[h][1] = relu_pack(z[h][2], z[h][3]);
-                }
-                const uint16_t* W = sF + (L == 0 ? KF_L2 : KF_L3);
-                const int boff = L == 0 ? SB_B2 : SB_B3;
-#pragma unroll
-                for (int mt = 0; mt < 4; mt++) {
-                    const f32x4 bb = *reinterpret_cast<const f32x4*>(sB + boff + 16 * mt + 4 * g);
-                    f32x4 a0 = bb, a1 = bb;
-                    const bf16x8 wa = frag(W, mt * 2 + 0), wb = frag(W, mt * 2 + 1);
-                    a0 = mfma32(wa, hb[0][0], a0);
-                    a1 = mfma32(wa, hb[1][0], a1);
-                    a0 = mfma32(wb, hb[0][1], a0);
-                    a1 = mfma32(wb, hb[1][1], a1);
-                    z[0][mt] = a0;
-                    z[1][mt] = a1;
-                }
-            }
-            float part[2] = {0.f, 0.f};
-#pragma unroll
-            for (int mt = 0; mt < 4; mt++) {
-                const f32x4 w4 = *reinterpret_cast<const f32x4*>(sB + SB_W4 + 16 * mt + 4 * g);
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    part[0] += w4[j] * relu(z[0][mt][j]);
-                    part[1] += w4[j] * relu(z[1][mt][j]);
-                }
-            }
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                float pt = part[h];
-                pt += __shfl_xor(pt, 16);
-                pt += __shfl_xor(pt, 32);
-                if (mine[h] && g == 0) {
-                    const float pr = 1.f / (1.f + __expf(-(pt + sB[SB_B4])));
-                    if (prob) prob[leaf[h]] = pr;
-                    flags[leaf[h]] = decide(old[h], pr >= sp.threshold, leaf[h], sp);
-                }
-            }
+            for (int j = 0; j < 4; j++) part += w4[j] * relu(z[mt][j]);
+        }
+        part += __shfl_xor(part, 16);
+        part += __shfl_xor(part, 32);
+        if (valid && g == 0 && ((mask >> r) & 1)) {
+            const float pr = 1.f / (1.f + __expf(-(part + sB[SB_B4])));
+            if (prob) prob[leaf] = pr;
+            flags[leaf] = decide(old, pr >= sp.threshold, leaf, sp);
         }
     }
 }
@@ -747,15 +664,13 @@ __global__ void k_pack(const float* __restrict__ w, uint16_t* __restrict__ pk) {
 
 // ---------------------------------------------------------------- launchers
 void launch_select(const DevTables& t, const float4* Qf, const int32_t* off16, const uint16_t* wpk,
-                   const float* w32, const SelectParams& sp, uint8_t* flags, float* prob, uint16_t* tile_mask,
-                   uint4* tile_frag, int mlp_blocks, const int32_t* nbr_beg, const int32_t* nbr,
-                   const uint8_t* loc, hipStream_t st) {
+                   const float* w32, const SelectParams& sp, uint8_t* flags, float* prob, int n_cu,
+                   const int32_t* nbr_beg, const int32_t* nbr, const uint8_t* loc, hipStream_t st) {
     if (t.N <= 0) return;
-    const int ntiles = (t.N + 15) / 16;
-    hipLaunchKernelGGL(k_select_gate, dim3((ntiles + 3) / 4), dim3(256), 0, st, t, Qf, off16, sp, flags, prob,
-                       tile_mask, tile_frag, nbr_beg, nbr, reinterpret_cast<const uint2*>(loc));
-    hipLaunchKernelGGL(k_select_mlp, dim3(mlp_blocks), dim3(256), 0, st, t.N, wpk, w32, sp, tile_mask, tile_frag,
-                       flags, prob);
+    // two blocks per CU (the LDS of a block is 60 KB), a multiple of 8 (one tile range per XCD)
+    const int blocks = ((2 * n_cu + 7) / 8) * 8;
+    hipLaunchKernelGGL(k_select, dim3(blocks), dim3(SF_WAVES * 64), 0, st, t, Qf, off16, sp, flags, prob, nbr_beg,
+                       nbr, reinterpret_cast<const uint2*>(loc), wpk, w32);
 }
 
 void launch_judge(int n, const float* label, float bound, uint8_t* flags, int* counters, hipStream_t st) {

@@ -55,14 +55,12 @@ struct SelectParams {
     uint32_t step;        // step counter (varies the judge sample)
 };
 
-// inference: flags (and probabilities when prob != nullptr) of every interior leaf. Scratch:
-// tile_mask [ceil(N/16)], tile_frag [ceil(N/16) * 192] (layer-1 fragments of the tiles that
-// need the network); mlp_blocks: persistent grid of the network pass; nbr_beg, nbr, loc: the
-// tile neighbourhoods of the tables (Tables::sel_nbr).
+// inference: flags (and probabilities when prob != nullptr) of every interior leaf, in one
+// persistent launch of two blocks per CU (n_cu: the device's compute units); nbr_beg, nbr, loc:
+// the tile neighbourhoods of the tables (Tables::sel_nbr).
 void launch_select(const DevTables& t, const float4* Qf, const int32_t* off16, const uint16_t* wpk,
-                   const float* w32, const SelectParams& sp, uint8_t* flags, float* prob, uint16_t* tile_mask,
-                   uint4* tile_frag, int mlp_blocks, const int32_t* nbr_beg, const int32_t* nbr,
-                   const uint8_t* loc, hipStream_t st);
+                   const float* w32, const SelectParams& sp, uint8_t* flags, float* prob, int n_cu,
+                   const int32_t* nbr_beg, const int32_t* nbr, const uint8_t* loc, hipStream_t st);
 void launch_judge(int n, const float* label, float bound, uint8_t* flags, int* counters, hipStream_t st);
 void launch_features(const DevTables& t, const float4* Qf, const int32_t* off16, uint16_t* X, int row0,
                      hipStream_t st);
