@@ -1127,7 +1127,16 @@ int32_t fdmamr_table_info(fdmamr_ctx* c, int32_t* info8) {
     int aliased = 0, generic = 0;
     for (int32_t a : tb.ghost_alias) aliased += a >= 0;
     for (size_t d = 0; d * 4 < tb.dep_hdr.size(); d++) generic += tb.dep_hdr[d * 4] == DEP_GENERIC;
-    const int32_t v[8] = {tb.S_wp, tb.S_w, aliased, (int32_t)tb.fill_fast.size(), generic, 0, 0, 0};
+    // selector tiles gathered directly (too many distinct values), tiles with more than 64,
+    // and the largest count of a tile staged in LDS
+    int direct = 0, over64 = 0, max_nu = 0;
+    for (size_t k = 0; k + 1 < tb.sel_nbr_beg.size(); k++) {
+        const int nu = tb.sel_nbr_beg[k + 1] - tb.sel_nbr_beg[k];
+        if (nu > SEL_NBR_CAP) direct++;
+        else max_nu = std::max(max_nu, nu);
+        over64 += nu > 64;
+    }
+    const int32_t v[8] = {tb.S_wp, tb.S_w, aliased, (int32_t)tb.fill_fast.size(), generic, direct, over64, max_nu};
     for (int k = 0; k < 8; k++) info8[k] = v[k];
     return 0;
 }

@@ -36,8 +36,10 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
 }
 
 // ReLU and its derivative (taken as 0 at 0). The activation is the selector's largest VALU
-// cost per leaf after the gathers (256 evaluations per leaf), so it is one instruction.
-__device__ __forceinline__ float relu(float x) { return fmaxf(x, 0.f); }
+// cost per leaf after the gathers (256 evaluations per leaf), so it is one instruction: an
+// integer max of the bit pattern with 0 keeps every non-negative float and maps every
+// negative one to +0 (fmaxf would first quiet a possible signalling NaN, a second instruction).
+__device__ __forceinline__ float relu(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
 __device__ __forceinline__ float relu_d(float x) { return x > 0.f ? 1.f : 0.f; }
 
 __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
@@ -90,18 +92,21 @@ __device__ __forceinline__ void gather_idx(const DevTables& t, const int32_t* __
 // is at most UNIFORM_REL |c| (exactly uniform data included).
 constexpr float UNIFORM_REL = 1e-6f;
 
-__device__ __forceinline__ bool features_from(const float4& qcr, const float4* q, bool valid, float v[7][3]) {
-    const int g = (threadIdx.x & 63) >> 4;
-    float c[3];
-    c[0] = valid ? qcr.x : 0.f;
-    c[1] = valid ? qcr.w : 0.f;
-    c[2] = valid ? __builtin_amdgcn_sqrtf(qcr.y * qcr.y + qcr.z * qcr.z) : 0.f;
+// The channels (rho, p, |u|) of an FP32 state (fourth component unused).
+__device__ __forceinline__ float4 channels(const float4& q) {
+    return make_float4(q.x, q.w, __builtin_amdgcn_sqrtf(__builtin_fmaf(q.y, q.y, q.z * q.z)), 0.f);
+}
+
+// cc: channels of the centre, q: channels of the lane's 7 points. The index tables give points
+// past the grid (p > 24) the centre itself, so their features are exactly 0; a lane past N reads
+// in-range entries, and its features (discarded) only enter its own leaf's network column.
+__device__ __forceinline__ bool features_from(const float4& cc, const float4* q, float v[7][3]) {
+    const float c[3] = {cc.x, cc.y, cc.z};
 #pragma unroll
     for (int k = 0; k < 7; k++) {
-        const bool in = valid && g + 4 * k < 25;
-        v[k][0] = in ? q[k].x : c[0];
-        v[k][1] = in ? q[k].w : c[1];
-        v[k][2] = in ? __builtin_amdgcn_sqrtf(q[k].y * q[k].y + q[k].z * q[k].z) : c[2];
+        v[k][0] = q[k].x;
+        v[k][1] = q[k].y;
+        v[k][2] = q[k].z;
     }
     bool uniform = true;
 #pragma unroll
@@ -109,23 +114,24 @@ __device__ __forceinline__ bool features_from(const float4& qcr, const float4* q
         float md = 0.f;
 #pragma unroll
         for (int k = 0; k < 7; k++) md = fmaxf(md, fabsf(v[k][f] - c[f]));
-        md = fmaxf(md, __shfl_xor(md, 16));
-        md = fmaxf(md, __shfl_xor(md, 32));
+        // md >= 0: the integer max of the bit patterns is the float max (no NaN quieting)
+        md = __int_as_float(max(__float_as_int(md), __float_as_int(__shfl_xor(md, 16))));
+        md = __int_as_float(max(__float_as_int(md), __float_as_int(__shfl_xor(md, 32))));
         uniform = uniform && md <= UNIFORM_REL * fabsf(c[f]);
         const float sc = __builtin_amdgcn_rcpf(fmaxf(md, fabsf(c[f])) + 1e-3f * fabsf(c[f]) + 1e-30f);
 #pragma unroll
-        for (int k = 0; k < 7; k++) v[k][f] = g + 4 * k < 25 ? (v[k][f] - c[f]) * sc : 0.f;
+        for (int k = 0; k < 7; k++) v[k][f] = (v[k][f] - c[f]) * sc;
     }
     return uniform;
 }
 
 __device__ __forceinline__ bool gather_values(const float4* __restrict__ Qf, int leaf, bool valid,
                                               const int idx[7], float v[7][3]) {
-    const float4 qc = Qf[valid ? leaf : 0];
+    const float4 qc = channels(Qf[valid ? leaf : 0]);
     float4 q[7];
 #pragma unroll
-    for (int k = 0; k < 7; k++) q[k] = Qf[idx[k]];
-    return features_from(qc, q, valid, v);
+    for (int k = 0; k < 7; k++) q[k] = channels(Qf[idx[k]]);
+    return features_from(qc, q, v);
 }
 
 __device__ __forceinline__ void gather_features(const DevTables& t, const float4* __restrict__ Qf,
@@ -286,14 +292,17 @@ __device__ __forceinline__ uint8_t decide(uint8_t old, bool troubled, int leaf, 
 }
 
 // Inference in one pass: a persistent grid whose blocks hold the network's weights in LDS (as
-// lane-contiguous fragments) and whose waves each take one 16-leaf tile at a time: the tile's
-// neighbourhood of the FP32 state goes to the wave's LDS region, the 5x5 features come from
-// there, near-uniform leaves are cleared without the network, and for the others the gathered
-// features are already the layer-1 B fragments (feature q = 8s + j of lane (r, g) is channel
-// q % 3 of point g + 4 (q / 3)), so the network runs on them in registers: every layer's
-// accumulators are the next layer's B fragments. Each XCD takes a contiguous eighth of the
+// lane-contiguous fragments) and whose waves each take one 16-leaf tile at a time: the channels
+// of the tile's distinct neighbourhood states go to the wave's LDS region (each converted
+// once), the 5x5 features come from there, near-uniform leaves are cleared without the
+// network, and for the others the gathered features are already the layer-1 B fragments
+// (feature q = 8s + j of lane (r, g) is channel q % 3 of point g + 4 (q / 3)), so the network
+// runs on them in registers: every layer's accumulators are the next layer's B fragments. Each XCD takes a contiguous eighth of the
 // tiles (its waves stride over it), so neighbouring tiles share that XCD's L2.
-constexpr int SF_WAVES = 10;  // waves per block: weights + neighbourhoods fit 64 KB of LDS
+// Waves per block: a multiple of 4, so the two blocks on a CU put the same number of waves
+// on each SIMD (10-wave blocks left two SIMDs with 6 waves and two with 4, and ran 1.5x
+// longer); the LDS of a block (weights + 12 neighbourhoods) is 66.6 KB.
+constexpr int SF_WAVES = 12;
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -338,7 +347,7 @@ __global__ __launch_bounds__(SF_WAVES * 64) void k_select(DevTables t, const flo
         const uint2 lc = loc[(size_t)(valid ? leaf : 0) * 4 + g];
         const uint8_t old = valid && g == 0 ? flags[leaf] : 0;
         if (local)
-            for (int k = lane; k < nU; k += 64) sQ[wv][k] = Qf[nbr[b + k]];
+            for (int k = lane; k < nU; k += 64) sQ[wv][k] = channels(Qf[nbr[b + k]]);
         wave_lds_sync();
         float v[7][3];
         bool uniform;
@@ -351,7 +360,7 @@ __global__ __launch_bounds__(SF_WAVES * 64) void k_select(DevTables t, const flo
             // the centre is point 12 = lane g 0, k 3 of the same leaf
             const int pc = __shfl(valid ? (int)pos[3] : 0, r);
             const float4 qc = sQ[wv][pc];
-            uniform = features_from(qc, q, valid, v);
+            uniform = features_from(qc, q, v);
         } else {
             int idx[7];
             gather_idx(t, off16, leaf, valid, idx);
@@ -667,7 +676,7 @@ void launch_select(const DevTables& t, const float4* Qf, const int32_t* off16, c
                    const float* w32, const SelectParams& sp, uint8_t* flags, float* prob, int n_cu,
                    const int32_t* nbr_beg, const int32_t* nbr, const uint8_t* loc, hipStream_t st) {
     if (t.N <= 0) return;
-    // two blocks per CU (the LDS of a block is 60 KB), a multiple of 8 (one tile range per XCD)
+    // two blocks per CU (the LDS limit), a multiple of 8 (one tile range per XCD)
     const int blocks = ((2 * n_cu + 7) / 8) * 8;
     hipLaunchKernelGGL(k_select, dim3(blocks), dim3(SF_WAVES * 64), 0, st, t, Qf, off16, sp, flags, prob, nbr_beg,
                        nbr, reinterpret_cast<const uint2*>(loc), wpk, w32);

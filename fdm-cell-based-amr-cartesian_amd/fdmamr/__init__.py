@@ -464,7 +464,8 @@ class Solver:
         v = np.zeros(8, np.int32)
         self._check(self.L.fdmamr_table_info(self.ctx, _ip(v)), "table_info")
         return dict(slots_plain=int(v[0]), slots=int(v[1]), ghosts_aliased=int(v[2]),
-                    fill_items_fast=int(v[3]), slots_generic=int(v[4]))
+                    fill_items_fast=int(v[3]), slots_generic=int(v[4]), select_tiles_direct=int(v[5]),
+                    select_tiles_over64=int(v[6]), select_tile_max=int(v[7]))
 
 
 def config_c1(**kw) -> Config:

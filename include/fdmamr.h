@@ -160,7 +160,9 @@ int32_t fdmamr_stencil_values_fast(fdmamr_ctx* ctx, int32_t i, float* val8x4);
 /* regrid-time table facts (no reference counterpart; diagnostics): info8[0] WLS slots that
  * read no ghost leaf, [1] WLS slots, [2] ghost leaves read through the leaf they copy (or the
  * first inflow ghost), [3] items the fast path refills per stage (the other ghosts and the
- * restriction slots), [4] ghost-reading slots without a fixed-size entry list, [5..7] 0 */
+ * restriction slots), [4] ghost-reading slots without a fixed-size entry list, [5] selector
+ * tiles gathered without LDS staging, [6] selector tiles reading more than 64 distinct values,
+ * [7] the most distinct values of a staged selector tile */
 int32_t fdmamr_table_info(fdmamr_ctx* ctx, int32_t* info8);
 
 /* ---- troubled-cell selector (no reference counterpart: the reference applies the nonlinear
