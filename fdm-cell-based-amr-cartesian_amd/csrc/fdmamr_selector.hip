@@ -53,6 +53,24 @@ __device__ __forceinline__ bf16x4 lds4(const uint16_t* p) {
     return r;
 }
 
+// Cross-row combinations of the four lanes l, l ^ 16, l ^ 32, l ^ 48 with gfx950's lane swaps
+// instead of LDS permutes: swapping a register with itself across rows leaves the pair
+// {value of l, value of l ^ 16} (or l ^ 32) in the two outputs of every lane, and a
+// commutative combination of the pair is the same in both lanes.
+__device__ __forceinline__ float sum4_rows(float v) {
+    const auto a = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    v = __uint_as_float(a[0]) + __uint_as_float(a[1]);
+    const auto b = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(b[0]) + __uint_as_float(b[1]);
+}
+// the same for a maximum of non-negative floats (integer max of the bit patterns)
+__device__ __forceinline__ float max4_rows_nonneg(float v) {
+    const auto a = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    const int m = max((int)a[0], (int)a[1]);
+    const auto b = __builtin_amdgcn_permlane32_swap((unsigned)m, (unsigned)m, false, false);
+    return __int_as_float(max((int)b[0], (int)b[1]));
+}
+
 // ---------------------------------------------------------------- features
 // Index-table row of point p of the 5x5 stencil: 0..7 the leaf's own stencil rows (axis
 // points), 8..23 the 16 off-axis rows, 24 the centre (and p > 24, which has no point).
@@ -114,9 +132,7 @@ __device__ __forceinline__ bool features_from(const float4& cc, const float4* q,
         float md = 0.f;
 #pragma unroll
         for (int k = 0; k < 7; k++) md = fmaxf(md, fabsf(v[k][f] - c[f]));
-        // md >= 0: the integer max of the bit patterns is the float max (no NaN quieting)
-        md = __int_as_float(max(__float_as_int(md), __float_as_int(__shfl_xor(md, 16))));
-        md = __int_as_float(max(__float_as_int(md), __float_as_int(__shfl_xor(md, 32))));
+        md = max4_rows_nonneg(md);
         uniform = uniform && md <= UNIFORM_REL * fabsf(c[f]);
         const float sc = __builtin_amdgcn_rcpf(fmaxf(md, fabsf(c[f])) + 1e-3f * fabsf(c[f]) + 1e-30f);
 #pragma unroll
@@ -237,8 +253,7 @@ __device__ __forceinline__ void mlp_forward(const uint16_t* sW, const float* sB,
             o.h3[mt][j] = h;
             part += sB[SB_W4 + 16 * mt + 4 * g + j] * h;
         }
-    part += __shfl_xor(part, 16);
-    part += __shfl_xor(part, 32);
+    part = sum4_rows(part);
     o.z4 = part + sB[SB_B4];
 }
 
@@ -357,9 +372,8 @@ __global__ __launch_bounds__(SF_WAVES * 64) void k_select(DevTables t, const flo
             float4 q[7];
 #pragma unroll
             for (int k = 0; k < 7; k++) q[k] = sQ[wv][valid ? (int)pos[k] : 0];
-            // the centre is point 12 = lane g 0, k 3 of the same leaf
-            const int pc = __shfl(valid ? (int)pos[3] : 0, r);
-            const float4 qc = sQ[wv][pc];
+            // byte 7: the centre's position
+            const float4 qc = sQ[wv][valid ? (int)pos[7] : 0];
             uniform = features_from(qc, q, v);
         } else {
             int idx[7];
@@ -411,8 +425,7 @@ __global__ __launch_bounds__(SF_WAVES * 64) void k_select(DevTables t, const flo
 #pragma unroll
             for (int j = 0; j < 4; j++) part += w4[j] * relu(z[mt][j]);
         }
-        part += __shfl_xor(part, 16);
-        part += __shfl_xor(part, 32);
+        part = sum4_rows(part);
         if (valid && g == 0 && ((mask >> r) & 1)) {
             const float pr = 1.f / (1.f + __expf(-(part + sB[SB_B4])));
             if (prob) prob[leaf] = pr;

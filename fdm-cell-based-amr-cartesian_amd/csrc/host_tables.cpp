@@ -636,6 +636,9 @@ void build_tables(const Tree& t, Tables& tb) {
                             }
                             tb.sel_loc[(size_t)i * 32 + g * 8 + j] = (uint8_t)std::min(val[h], 255);
                         }
+                // byte 7 of every lane: the centre (point 12 = lane 0, entry 3)
+                for (int i = i0; i < i1; i++)
+                    for (int g = 0; g < 4; g++) tb.sel_loc[(size_t)i * 32 + g * 8 + 7] = tb.sel_loc[(size_t)i * 32 + 3];
                 for (int h : used) key[h] = -1;
                 used.clear();
             }
@@ -656,7 +659,7 @@ void build_tables(const Tree& t, Tables& tb) {
         if (nU > SEL_NBR_CAP) continue;
         for (int i = T * 16; i < std::min(N, T * 16 + 16); i++)
             for (int g = 0; g < 4; g++)
-                for (int j = 0; j < 7; j++)
+                for (int j = 0; j < 8; j++)
                     if (tb.sel_loc[(size_t)i * 32 + g * 8 + j] >= nU)
                         throw std::runtime_error("selector neighbourhood position out of range");
     }

@@ -120,8 +120,8 @@ struct Tables {
     // stencils read, tile k's in sel_nbr[sel_nbr_beg[k], sel_nbr_beg[k+1]), and for every
     // leaf the positions of its points in that list: sel_loc[leaf][g][j] (uint8) is point
     // g + 4j of the 5x5 grid (p = (b+2)*5 + (a+2); the centre for p > 24), j < 7, for the four
-    // lanes g of the leaf. A tile with more than SEL_NBR_CAP distinct indices stores none
-    // (count SEL_NBR_CAP + 1) and is gathered directly.
+    // lanes g of the leaf, and sel_loc[leaf][g][7] the centre's position. A tile with more than
+    // SEL_NBR_CAP distinct indices stores none (count SEL_NBR_CAP + 1) and is gathered directly.
     std::vector<int32_t> sel_nbr_beg, sel_nbr;
     std::vector<uint8_t> sel_loc;        // [N][4][8]
     // device topology for restriction means
