@@ -163,7 +163,7 @@ int stage_fast_partial_blocks(int n, bool lists);
 // troubled count of each block of LIST_BLK leaves (turned into the blocks' prefix offsets).
 // launch_list_write is the prefix scan and second pass alone, for counts written by another
 // kernel.
-constexpr int LIST_BLK = 512;
+constexpr int LIST_BLK = 2048;
 void launch_build_lists(int n, const uint8_t* flags, int* blk_cnt, int32_t* lists, int* meta, hipStream_t st);
 void launch_list_write(int n, const uint8_t* flags, int* blk_cnt, int32_t* lists, int* meta,
                        hipStream_t st);

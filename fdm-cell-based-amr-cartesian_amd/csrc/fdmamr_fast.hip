@@ -433,66 +433,104 @@ __device__ __forceinline__ int block_sum256(int v, int* sh) {
     return sh[0] + sh[1] + sh[2] + sh[3];
 }
 
+// the troubled bits (bit 0) of the LPT = 8 flags of leaves [i0, i0 + 8) as one byte
+constexpr int LPT = LIST_BLK / 256;
+static_assert(LPT == 8, "one 8-byte flag load per thread");
+__device__ __forceinline__ uint32_t troubled8(int n, const uint8_t* __restrict__ flags, int i0) {
+    uint32_t m = 0;
+    if (i0 + LPT <= n) {
+        const uint2 w = *reinterpret_cast<const uint2*>(flags + i0);  // i0 is a multiple of 8
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            m |= ((w.x >> (8 * k)) & 1u) << k;
+            m |= ((w.y >> (8 * k)) & 1u) << (4 + k);
+        }
+    } else {
+        for (int k = 0; k < LPT; k++)
+            if (i0 + k < n) m |= (uint32_t)(flags[i0 + k] & 1) << k;
+    }
+    return m;
+}
+
 __global__ __launch_bounds__(256) void k_list_count(int n, const uint8_t* __restrict__ flags,
                                                     int* __restrict__ cnt) {
     __shared__ int sh[4];
-    const int base = blockIdx.x * LIST_BLK;
-    int c = 0;
-    for (int k = threadIdx.x; k < LIST_BLK; k += 256) {
-        const int i = base + k;
-        if (i < n) c += flags[i] & 1;
-    }
-    c = block_sum256(c, sh);
+    const int c = block_sum256(__popc(troubled8(n, flags, blockIdx.x * LIST_BLK + LPT * threadIdx.x)), sh);
     if (threadIdx.x == 0) cnt[blockIdx.x] = c;
 }
 
-// Exclusive prefix of the block counts, in place (one block of 1024 threads, each a contiguous
-// chunk), and the list split: meta[0] = cleared leaves, meta[1] = troubled leaves.
+// Exclusive prefix of the block counts, in place, by one block of 1024 threads over segments of
+// 8192 counts: coalesced loads into LDS, eight consecutive counts per thread, a block scan,
+// coalesced stores, and the running total carried to the next segment. Also the list split:
+// meta[0] = cleared leaves, meta[1] = troubled leaves.
+constexpr int SCAN_SEG = 8192;
 __global__ __launch_bounds__(1024) void k_list_scan(int n, int* __restrict__ cnt, int nblk, int* __restrict__ meta) {
+    __shared__ __align__(16) int seg[SCAN_SEG];
     __shared__ int ws[16];
-    const int per = (nblk + 1023) / 1024;
-    const int j0 = threadIdx.x * per, j1 = min(nblk, j0 + per);
-    int mine = 0;
-    for (int j = j0; j < j1; j++) mine += cnt[j];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    int inc = mine;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(inc, o);
-        if (lane >= o) inc += y;
+    int carry = 0;
+    for (int s0 = 0; s0 < nblk; s0 += SCAN_SEG) {
+        const int len = min(SCAN_SEG, nblk - s0);
+#pragma unroll
+        for (int k = 0; k < SCAN_SEG / 1024; k++) {
+            const int j = (int)threadIdx.x + 1024 * k;
+            seg[j] = j < len ? cnt[s0 + j] : 0;
+        }
+        __syncthreads();
+        const int4 a = reinterpret_cast<const int4*>(seg)[2 * threadIdx.x];
+        const int4 b = reinterpret_cast<const int4*>(seg)[2 * threadIdx.x + 1];
+        const int v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        int mine = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) mine += v[k];
+        int inc = mine;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(inc, o);
+            if (lane >= o) inc += y;
+        }
+        if (lane == 63) ws[wv] = inc;
+        __syncthreads();
+        int woff = 0, tot = 0;
+        for (int w = 0; w < 16; w++) {
+            woff += w < wv ? ws[w] : 0;
+            tot += ws[w];
+        }
+        int run = carry + woff + inc - mine, o8[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            o8[k] = run;
+            run += v[k];
+        }
+        reinterpret_cast<int4*>(seg)[2 * threadIdx.x] = make_int4(o8[0], o8[1], o8[2], o8[3]);
+        reinterpret_cast<int4*>(seg)[2 * threadIdx.x + 1] = make_int4(o8[4], o8[5], o8[6], o8[7]);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < SCAN_SEG / 1024; k++) {
+            const int j = (int)threadIdx.x + 1024 * k;
+            if (j < len) cnt[s0 + j] = seg[j];
+        }
+        carry += tot;
+        __syncthreads();  // seg and ws are rewritten by the next segment
     }
-    if (lane == 63) ws[wv] = inc;
-    __syncthreads();
-    int woff = 0, tot = 0;
-    for (int w = 0; w < 16; w++) {
-        woff += w < wv ? ws[w] : 0;
-        tot += ws[w];
-    }
-    int run = woff + inc - mine;
-    for (int j = j0; j < j1; j++) {
-        const int v = cnt[j];
-        cnt[j] = run;
-        run += v;
-    }
-    if (threadIdx.x == 0) { meta[0] = n - tot; meta[1] = tot; }
+    if (threadIdx.x == 0) { meta[0] = n - carry; meta[1] = carry; }
 }
 
 // each block places its leaves after the troubled / cleared leaves of the blocks before it
-// (pre: the block's exclusive prefix from k_list_scan)
+// (pre: the block's exclusive prefix from k_list_scan): both of its runs are ordered in LDS and
+// then stored contiguously
 __global__ __launch_bounds__(256) void k_list_write(int n, const uint8_t* __restrict__ flags,
                                                     const int* __restrict__ cnt,
                                                     int32_t* __restrict__ lists, const int* __restrict__ meta) {
     __shared__ int ws[4];
+    __shared__ int32_t sl[LIST_BLK];
     const int pre = cnt[blockIdx.x];
     const int n_clear = meta[0];
+    const int base = blockIdx.x * LIST_BLK;
+    const int nv = min(LIST_BLK, n - base);
     // LPT consecutive leaves per thread; exclusive scan of the troubled counts over the block
-    constexpr int LPT = LIST_BLK / 256;
-    const int i0 = blockIdx.x * LIST_BLK + LPT * threadIdx.x;
-    int tr[LPT], mine = 0;
-#pragma unroll
-    for (int k = 0; k < LPT; k++) {
-        tr[k] = (i0 + k < n) ? (flags[i0 + k] & 1) : 0;
-        mine += tr[k];
-    }
+    const int i0 = base + LPT * threadIdx.x;
+    const uint32_t tm = troubled8(n, flags, i0);
+    const int mine = __popc(tm);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int inc = mine;
     for (int o = 1; o < 64; o <<= 1) {
@@ -503,16 +541,22 @@ __global__ __launch_bounds__(256) void k_list_write(int n, const uint8_t* __rest
     __syncthreads();
     int woff = 0;
     for (int w = 0; w < wv; w++) woff += ws[w];
-    int t_pos = pre + woff + inc - mine;                       // troubled before this thread
-    int c_pos = blockIdx.x * LIST_BLK - pre + LPT * threadIdx.x - (woff + inc - mine);
+    const int tot = ws[0] + ws[1] + ws[2] + ws[3];
+    const int ncl = nv - tot;  // cleared leaves of the block: sl[0, ncl), then its troubled ones
+    int t_pos = ncl + woff + inc - mine;
+    int c_pos = LPT * threadIdx.x - (woff + inc - mine);
 #pragma unroll
     for (int k = 0; k < LPT; k++) {
         const int i = i0 + k;
         if (i < n) {
-            if (tr[k]) lists[n_clear + t_pos++] = i;
-            else lists[c_pos++] = i;
+            if ((tm >> k) & 1) sl[t_pos++] = i;
+            else sl[c_pos++] = i;
         }
     }
+    __syncthreads();
+    int32_t* __restrict__ cl = lists + (base - pre);
+    int32_t* __restrict__ tr = lists + n_clear + pre - ncl;
+    for (int k = threadIdx.x; k < nv; k += 256) (k < ncl ? cl : tr)[k] = sl[k];
 }
 
 // ---------------------------------------------------------------- FP32 WLS-WENO slots
