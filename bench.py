@@ -318,7 +318,7 @@ def main():
     if args.mesh_file:
         # a counter run: the parent's mesh and state
         z = np.load(args.mesh_file)
-        tree = {k: z[k] for k in z.files if k != "prim"}
+        tree = {k: z[k] for k in z.files if k not in ("prim", "weights")}
         prim0 = z["prim"]
         s64, t_regrid = None, 0.0
     else:
@@ -331,13 +331,15 @@ def main():
     cnt = sel.counts()
     N = cnt["n_leaf"]
     sel.selector_config(args.threshold, 0.02, args.label_bound)
+    if args.mesh_file:
+        # the parent's fine-tuned weights: the counter runs measure the same selection
+        w = z["weights"]
     sel.selector_set_weights(w)
     sel.run_steps(args.warmup)
     st = sel.selector_stats(reset=True)
     rate = st["rejected"] / max(st["judged"], 1)
     tuned = 0
-    # (the counter runs' copy fine-tunes at most once: it only needs a representative stage)
-    while rate > 0.01 and tuned < (1 if args.traffic_child else 3):
+    while rate > 0.01 and tuned < 3 and not args.mesh_file:
         sel.selector_collect()
         sel.selector_train(args.tune_iters, batch=4096, lr=1e-3, pos_weight=4.0, seed=11 + tuned)
         sel.selector_clear_dataset()
@@ -349,6 +351,7 @@ def main():
         rate = st["rejected"] / max(st["judged"], 1)
         tuned += 1
     log(f"judge rejection rate {rate:.4f} after {tuned} online fine-tune round(s)")
+    w_tuned = sel.selector_get_weights()
     # restart both paths from the same state on the same mesh
     sel.build_tables()
     sel.set_state(prim0)
@@ -413,7 +416,7 @@ def main():
         fd, mesh_file = tempfile.mkstemp(prefix="fdmamr_mesh_", suffix=".npz")
         os.close(fd)
         try:
-            np.savez(mesh_file, prim=prim0, **tree)
+            np.savez(mesh_file, prim=prim0, weights=w_tuned, **tree)
             traffic = measure_traffic(args, mesh_file)
         finally:
             os.remove(mesh_file)

@@ -332,9 +332,13 @@ __global__ __launch_bounds__(SF_WAVES * 64) void k_select(DevTables t, const flo
                                                           const uint2* __restrict__ loc,
                                                           const uint16_t* __restrict__ wpk,
                                                           const float* __restrict__ w32) {
-    __shared__ __align__(16) uint16_t sF[KF_TOTAL];
-    __shared__ __align__(16) float sB[SB_TOTAL];
-    __shared__ float4 sQ[SF_WAVES][SEL_NBR_CAP];
+    // one LDS array with the weight fragments at its start, so every fragment read encodes
+    // its offset in the instruction (the offset field reaches 64 KB; the neighbourhoods come last)
+    constexpr int SB_BYTES = (SB_TOTAL * 4 + 15) / 16 * 16;
+    __shared__ __align__(16) unsigned char lds[KF_TOTAL * 2 + SB_BYTES + SF_WAVES * SEL_NBR_CAP * 16];
+    uint16_t* sF = reinterpret_cast<uint16_t*>(lds);
+    float* sB = reinterpret_cast<float*>(lds + KF_TOTAL * 2);
+    float4(*sQ)[SEL_NBR_CAP] = reinterpret_cast<float4(*)[SEL_NBR_CAP]>(lds + KF_TOTAL * 2 + SB_BYTES);
     {
         const uint4* src = reinterpret_cast<const uint4*>(wpk + KW_FRAG);
         uint4* dst = reinterpret_cast<uint4*>(sF);
