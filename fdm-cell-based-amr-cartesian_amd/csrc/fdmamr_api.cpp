@@ -807,6 +807,21 @@ int32_t fdmamr_selector_get_flags(fdmamr_ctx* c, uint8_t* flags) {
     return 0;
 }
 
+int32_t fdmamr_leaf_lists(fdmamr_ctx* c, int32_t* lists, int32_t* n_clear) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!lists || !n_clear) return fail(c, -1, "null list buffer");
+    // the lists as the last step built them, or from the current flags for a fresh context
+    if (c->tb.N > 0) {
+        HIPCHK(c, hipMemcpyAsync(lists, c->lists.p, (size_t)c->tb.N * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(n_clear, c->list_meta.p, 4, hipMemcpyDeviceToHost, c->stream));
+    } else {
+        *n_clear = 0;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 int32_t fdmamr_labels(fdmamr_ctx* c, float* dev) {
     int r = check_ready(c);
     if (r) return r;

@@ -460,30 +460,16 @@ __global__ __launch_bounds__(256) void k_list_count(int n, const uint8_t* __rest
 }
 
 // Exclusive prefix of the block counts, in place, by one block of 1024 threads over segments of
-// 8192 counts: coalesced loads into LDS, eight consecutive counts per thread, a block scan,
-// coalesced stores, and the running total carried to the next segment. Also the list split:
-// meta[0] = cleared leaves, meta[1] = troubled leaves.
-constexpr int SCAN_SEG = 8192;
+// 1024 counts (one coalesced load and store per thread each), the running total carried from
+// segment to segment. Also the list split: meta[0] = cleared leaves, meta[1] = troubled leaves.
 __global__ __launch_bounds__(1024) void k_list_scan(int n, int* __restrict__ cnt, int nblk, int* __restrict__ meta) {
-    __shared__ __align__(16) int seg[SCAN_SEG];
     __shared__ int ws[16];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int carry = 0;
-    for (int s0 = 0; s0 < nblk; s0 += SCAN_SEG) {
-        const int len = min(SCAN_SEG, nblk - s0);
-#pragma unroll
-        for (int k = 0; k < SCAN_SEG / 1024; k++) {
-            const int j = (int)threadIdx.x + 1024 * k;
-            seg[j] = j < len ? cnt[s0 + j] : 0;
-        }
-        __syncthreads();
-        const int4 a = reinterpret_cast<const int4*>(seg)[2 * threadIdx.x];
-        const int4 b = reinterpret_cast<const int4*>(seg)[2 * threadIdx.x + 1];
-        const int v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-        int mine = 0;
-#pragma unroll
-        for (int k = 0; k < 8; k++) mine += v[k];
-        int inc = mine;
+    for (int s0 = 0; s0 < nblk; s0 += 1024) {
+        const int j = s0 + (int)threadIdx.x;
+        const int v = j < nblk ? cnt[j] : 0;
+        int inc = v;
         for (int o = 1; o < 64; o <<= 1) {
             const int y = __shfl_up(inc, o);
             if (lane >= o) inc += y;
@@ -495,22 +481,9 @@ __global__ __launch_bounds__(1024) void k_list_scan(int n, int* __restrict__ cnt
             woff += w < wv ? ws[w] : 0;
             tot += ws[w];
         }
-        int run = carry + woff + inc - mine, o8[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            o8[k] = run;
-            run += v[k];
-        }
-        reinterpret_cast<int4*>(seg)[2 * threadIdx.x] = make_int4(o8[0], o8[1], o8[2], o8[3]);
-        reinterpret_cast<int4*>(seg)[2 * threadIdx.x + 1] = make_int4(o8[4], o8[5], o8[6], o8[7]);
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < SCAN_SEG / 1024; k++) {
-            const int j = (int)threadIdx.x + 1024 * k;
-            if (j < len) cnt[s0 + j] = seg[j];
-        }
+        if (j < nblk) cnt[j] = carry + woff + inc - v;
         carry += tot;
-        __syncthreads();  // seg and ws are rewritten by the next segment
+        __syncthreads();  // ws is rewritten by the next segment
     }
     if (threadIdx.x == 0) { meta[0] = n - carry; meta[1] = carry; }
 }

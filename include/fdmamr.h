@@ -164,6 +164,10 @@ int32_t fdmamr_stencil_values_fast(fdmamr_ctx* ctx, int32_t i, float* val8x4);
  * tiles gathered without LDS staging, [6] selector tiles reading more than 64 distinct values,
  * [7] the most distinct values of a staged selector tile */
 int32_t fdmamr_table_info(fdmamr_ctx* ctx, int32_t* info8);
+/* the fast path's leaf lists as the last flagged step built them (no reference counterpart;
+ * diagnostics): lists[0, *n_clear) the cleared interior leaves, then the troubled ones, each
+ * in ascending order (N ints) */
+int32_t fdmamr_leaf_lists(fdmamr_ctx* ctx, int32_t* lists, int32_t* n_clear);
 
 /* ---- troubled-cell selector (no reference counterpart: the reference applies the nonlinear
  * WCNS weights everywhere, nnd_center_RK.F90:310-353). FP32 mode only. A leaf is "troubled"

@@ -61,6 +61,25 @@ def test_mixed_flags_take_both_paths():
     assert runs["mixed"].check_positivity() == 0
 
 
+@pytest.mark.parametrize("base", [64, 1536])
+def test_leaf_lists_from_flags(base):
+    # The flags are compacted into the cleared and the troubled leaf list by per-block troubled
+    # counts (blocks of 2048 leaves), their prefix scan (segments of 1024 block counts: the
+    # 1536 x 1536 mesh has 2 359 296 leaves, 1 152 blocks, so the scan carries a total from one
+    # segment to the next) and a per-block placement; each list must hold exactly its leaves,
+    # in ascending order.
+    kw = dict(h=10.0 / base, localdt=1, cfl=0.5, t_end=1e9)
+    c = F.Solver(F.Config(flux_precision=F.FLUX_FP32, **kw)).setup()
+    n = c.n_leaf
+    assert n == base * base
+    flags = (np.random.default_rng(3).uniform(size=n) < 0.3).astype(np.uint8)
+    c.set_flags(flags, use=True)
+    c.run_steps(1)
+    cleared, troubled = c.leaf_lists()
+    assert np.array_equal(cleared, np.flatnonzero(flags == 0))
+    assert np.array_equal(troubled, np.flatnonzero(flags == 1))
+
+
 def test_flat_ghost_fill_matches_nested_restriction():
     # The fast path fills ghost leaves in one pass from flattened weighted leaf sums; the FP64
     # path takes the reference's nested restriction means (interpA). With dt = 0 the stages
