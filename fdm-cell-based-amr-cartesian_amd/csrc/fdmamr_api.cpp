@@ -50,6 +50,9 @@ struct fdmamr_ctx {
     DBuf<float> label;       // per-leaf WCNS weight deviation (labels / judge)
     bool use_flags = false;  // fast path consults the flags
     DBuf<int32_t> lists;     // cleared leaves, then troubled leaves (from the flags)
+    DBuf<int32_t> slot_blk_pat;  // distinct slot records of each slot block
+    DBuf<uint8_t> slot_lpat;     // each slot's position in its block's list
+    DBuf<int32_t> slot_uncov;    // slots whose record is not staged
     int n_cu = 256;
     DBuf<int> list_cnt, list_meta;
 
@@ -215,6 +218,12 @@ int upload_tables(fdmamr_ctx* c) {
     HIPCHK(c, up(c->sel_nbr_beg.p, tb.sel_nbr_beg.data(), tb.sel_nbr_beg.size() * 4));
     HIPCHK(c, up(c->sel_nbr.p, tb.sel_nbr.data(), tb.sel_nbr.size() * 4));
     HIPCHK(c, up(c->sel_loc.p, tb.sel_loc.data(), tb.sel_loc.size()));
+    HIPCHK(c, ensure(c->slot_blk_pat, std::max<size_t>(tb.slot_blk_pat.size(), 1)));
+    HIPCHK(c, ensure(c->slot_lpat, std::max<size_t>(tb.slot_lpat.size(), 1)));
+    if (!tb.slot_blk_pat.empty()) HIPCHK(c, up(c->slot_blk_pat.p, tb.slot_blk_pat.data(), tb.slot_blk_pat.size() * 4));
+    if (!tb.slot_lpat.empty()) HIPCHK(c, up(c->slot_lpat.p, tb.slot_lpat.data(), tb.slot_lpat.size()));
+    HIPCHK(c, ensure(c->slot_uncov, std::max<size_t>(tb.slot_uncov.size(), 1)));
+    if (!tb.slot_uncov.empty()) HIPCHK(c, up(c->slot_uncov.p, tb.slot_uncov.data(), tb.slot_uncov.size() * 4));
     // every leaf troubled until the selector says otherwise
     if (tb.N > 0) HIPCHK(c, hipMemsetAsync(c->flags.p, 1, tb.N, s));
     launch_build_lists(tb.N, c->flags.p, c->list_cnt.p, c->lists.p, c->list_meta.p, s);
@@ -226,6 +235,10 @@ int upload_tables(fdmamr_ctx* c) {
     d.dep_hdr = reinterpret_cast<const int4*>(c->dep_hdr.p);
     d.dep_leaf = reinterpret_cast<const int4*>(c->dep_leaf.p);
     d.dep_w = reinterpret_cast<const float4*>(c->dep_w.p);
+    d.slot_blk_pat = c->slot_blk_pat.p;
+    d.slot_lpat = c->slot_lpat.p;
+    d.slot_uncov = c->slot_uncov.p;
+    d.S_unc = (int)tb.slot_uncov.size();
     d.rst_node = c->rst.p; d.bc_partner = c->bcp.p; d.node_son = c->son.p; d.node_state = c->state.p;
     d.n_int = tb.n_int; d.int_child = c->int_child.p; d.rst_ref = c->rst_ref.p; d.bc_ref = c->bc_ref.p;
     // ghosts and restriction slots of the uploaded state, FP32 shadows of all three buffers
@@ -413,7 +426,7 @@ void fdmamr_destroy(fdmamr_ctx* c) {
     release(c->label); release(c->off16); release(c->sel_w); release(c->sel_m); release(c->sel_v);
     release(c->sel_grad); release(c->sel_gpart); release(c->sel_lpart); release(c->sel_loss);
     release(c->sel_prob); release(c->sel_pk); release(c->sel_cnt); release(c->ds_x); release(c->ds_y);
-    release(c->ds_idx); release(c->lists); release(c->list_cnt); release(c->list_meta);
+    release(c->ds_idx); release(c->lists); release(c->slot_blk_pat); release(c->slot_lpat); release(c->slot_uncov); release(c->list_cnt); release(c->list_meta);
 
     release(c->fill_beg); release(c->fill_leaf); release(c->fill_w); release(c->fill_fast);
     release(c->dep_hdr); release(c->dep_leaf); release(c->dep_w);

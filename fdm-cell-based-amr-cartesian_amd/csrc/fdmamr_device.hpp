@@ -107,6 +107,10 @@ struct DevTables {
     const int4* dep_hdr;        // [S_w - S_wp]: ghost points of the slots that read ghosts
     const int4* dep_leaf;       // [S_w - S_wp][2]: DEP_ENT leaves each
     const float4* dep_w;        // [S_w - S_wp][2]: their weights
+    const int32_t* slot_blk_pat;  // [ceil(S_w / SLOT_BLK)][SLOT_BLK_PATS] records staged per block
+    const uint8_t* slot_lpat;     // [S_w] position of the slot's record in its block's list
+    const int32_t* slot_uncov;    // [S_unc] the slots whose record is not in their block's list
+    int S_unc;
 };
 
 // Scalars of the device-resident step loop (solution.F90:183-215).

@@ -533,14 +533,18 @@ __global__ __launch_bounds__(256) void k_list_write(int n, const uint8_t* __rest
 }
 
 // ---------------------------------------------------------------- FP32 WLS-WENO slots
-__device__ __forceinline__ float isq_f(float k1, float k2, float k3, const float* b) {
+template <typename FP>
+__device__ __forceinline__ float isq_f(float k1, float k2, float k3, FP b) {
     return k1 * k1 * b[0] * b[1] + k1 * k2 * b[2] * b[1] + k1 * k3 * b[0] * b[3] +
            k2 * k2 * b[4] * (1.f / 3.f) * b[1] + 0.5f * k2 * k3 * b[2] * b[3] + k3 * k3 * b[0] * b[5] * (1.f / 3.f);
 }
 
-// One component of a WLS-WENO slot in FP32 from its nine point values (record f).
-__device__ __forceinline__ float slot_component(const float* __restrict__ f, const float* phi) {
-    const float* box = f + WF_BOX;
+// One component of a WLS-WENO slot in FP32 from its nine point values (record f: a global
+// pointer, or an LDS one when the block staged the record).
+typedef const __attribute__((address_space(3))) float* lds_fptr;
+template <typename FP>
+__device__ __forceinline__ float slot_component(FP f, const float* phi) {
+    const FP box = f + WF_BOX;
     const float area = box[0] * box[1];
     // coefficients 1..5 of the quadratic fit (the constant term is not needed)
     float al[6], vl = 0.f;
@@ -589,9 +593,9 @@ __device__ __forceinline__ float slot_component(const float* __restrict__ f, con
 // and each runs half the arithmetic. GHOSTS (the launch that also writes the ghosts, slots
 // [S_wp, S_w)): the points that are ghost leaves are formed from the slot's fixed-size entry
 // lists (Tables::dep_hdr) instead of read.
-template <bool GHOSTS>
+template <bool GHOSTS, bool STAGED>
 __device__ __forceinline__ void slot_half(const DevTables& t, const float* __restrict__ wf, float4* __restrict__ Qf,
-                                          int s, int h) {
+                                          int s, int h, const float* srec = nullptr) {
     const int32_t* __restrict__ iv = t.wls_i + (size_t)s * WLSI_STRIDE;
     const float* __restrict__ f = wf + (size_t)iv[WLSI_PAT] * WF_STRIDE;
     const int mw = iv[WLSI_M];
@@ -632,21 +636,53 @@ __device__ __forceinline__ void slot_half(const DevTables& t, const float* __res
                                                  : make_float2(0.f, 0.f);
     }
     float out[2];
+    if (STAGED) {
+        // the caller checked that the record is staged
+        const lds_fptr fl = (lds_fptr)(srec + (int)t.slot_lpat[s] * WF_STRIDE);
 #pragma unroll
-    for (int k = 0; k < 2; k++) {
-        float phi[9];
+        for (int k = 0; k < 2; k++) {
+            float phi[9];
 #pragma unroll
-        for (int i = 0; i < 9; i++) phi[i] = k == 0 ? val[i].x : val[i].y;
-        out[k] = slot_component(f, phi);
+            for (int i = 0; i < 9; i++) phi[i] = k == 0 ? val[i].x : val[i].y;
+            out[k] = slot_component(fl, phi);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            float phi[9];
+#pragma unroll
+            for (int i = 0; i < 9; i++) phi[i] = k == 0 ? val[i].x : val[i].y;
+            out[k] = slot_component(f, phi);
+        }
     }
     reinterpret_cast<float2*>(Qf)[(size_t)2 * (t.L + s) + h] = make_float2(out[0], out[1]);
 }
 
-// every slot, two lanes each (ghosts are filled before this launch: the ghost mask is not used)
+// every slot, two lanes each (ghosts are filled before this launch: the ghost mask is not used).
+// Block b < nbs first stages the distinct records of its SLOT_BLK slots in LDS, so the two lanes
+// of a slot and the slots sharing a pattern read them there instead of each loading the 104
+// floats; the slots whose record did not fit the block's list (slot_uncov) are evaluated by the
+// blocks past nbs, with the record read from global memory (a block-uniform choice, so the
+// two variants do not share registers).
 __global__ __launch_bounds__(128) void k_slots_fast(DevTables t, const float* __restrict__ wf,
                                                     float4* __restrict__ Qf) {
-    const int s = blockIdx.x * 64 + ((int)threadIdx.x >> 1);
-    if (s < t.S_w) slot_half<false>(t, wf, Qf, s, threadIdx.x & 1);
+    const int nbs = (t.S_w + SLOT_BLK - 1) / SLOT_BLK;
+    if ((int)blockIdx.x >= nbs) {
+        const int k = ((int)blockIdx.x - nbs) * SLOT_BLK + ((int)threadIdx.x >> 1);
+        if (k < t.S_unc) slot_half<false, false>(t, wf, Qf, t.slot_uncov[k], threadIdx.x & 1);
+        return;
+    }
+    __shared__ __align__(16) float srec[SLOT_BLK_PATS * WF_STRIDE];
+    const int32_t* __restrict__ bp = t.slot_blk_pat + (size_t)blockIdx.x * SLOT_BLK_PATS;
+    constexpr int Q4 = WF_STRIDE / 4;
+    for (int k = threadIdx.x; k < SLOT_BLK_PATS * Q4; k += 128) {
+        const int q = k / Q4, c = k - q * Q4;
+        const int p = bp[q];
+        if (p >= 0) reinterpret_cast<float4*>(srec)[k] = reinterpret_cast<const float4*>(wf + (size_t)p * WF_STRIDE)[c];
+    }
+    __syncthreads();
+    const int s = blockIdx.x * SLOT_BLK + ((int)threadIdx.x >> 1);
+    if (s < t.S_w && t.slot_lpat[s] < SLOT_BLK_PATS) slot_half<false, true>(t, wf, Qf, s, threadIdx.x & 1, srec);
 }
 
 // Ghost leaves and restriction slots of the fast path from the flattened lists: item k is
@@ -717,7 +753,7 @@ __global__ __launch_bounds__(128) void k_slots_fill(
     const int b = (int)blockIdx.x;
     if (b < ngb) {
         const int s = t.S_wp + b * 64 + ((int)threadIdx.x >> 1);
-        if (s < t.S_w) slot_half<true>(t, wf, Qf, s, threadIdx.x & 1);
+        if (s < t.S_w) slot_half<true, false>(t, wf, Qf, s, threadIdx.x & 1);
     } else if (b < ngb + nfb) {
         const int tid = (b - ngb) * 128 + (int)threadIdx.x;
         const int kk = tid >> 4;
@@ -725,7 +761,7 @@ __global__ __launch_bounds__(128) void k_slots_fill(
         fill_row(t, Q, Qf, live ? items[kk] : 0, live, tid & 15, beg, leaf, w);
     } else {
         const int s = (b - ngb - nfb) * 64 + ((int)threadIdx.x >> 1);
-        if (s < t.S_wp) slot_half<false>(t, wf, Qf, s, threadIdx.x & 1);
+        if (s < t.S_wp) slot_half<false, false>(t, wf, Qf, s, threadIdx.x & 1);
     }
 }
 
@@ -751,7 +787,9 @@ __global__ void k_shadow(const double4* __restrict__ Q, float4* __restrict__ Qf,
 static inline int nb_(int n, int b) { return (n + b - 1) / b; }
 
 void launch_slots_fast(const DevTables& t, const float* wf, const double4* Q, float4* Qf, hipStream_t st) {
-    if (t.S_w > 0) hipLaunchKernelGGL(k_slots_fast, dim3(nb_(t.S_w, 64)), dim3(128), 0, st, t, wf, Qf);
+    if (t.S_w > 0)
+        hipLaunchKernelGGL(k_slots_fast, dim3(nb_(t.S_w, SLOT_BLK) + nb_(t.S_unc, SLOT_BLK)), dim3(128), 0, st, t, wf,
+                           Qf);
 }
 
 void launch_slots_fill(const DevTables& t, const float* wf, double4* Q, float4* Qf, const int32_t* items,

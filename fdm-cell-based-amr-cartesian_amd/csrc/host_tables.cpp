@@ -569,6 +569,25 @@ void build_tables(const Tree& t, Tables& tb) {
             tb.dep_hdr[(size_t)d * 4] = fits ? cnt : DEP_GENERIC;
         }
     }
+    // each slot block's distinct records, in first-use order
+    {
+        const int nbs = (tb.S_w + SLOT_BLK - 1) / SLOT_BLK;
+        tb.slot_blk_pat.assign((size_t)nbs * SLOT_BLK_PATS, -1);
+        tb.slot_lpat.assign(tb.S_w, 255);
+        tb.slot_uncov.clear();
+        for (int b = 0; b < nbs; b++) {
+            int32_t* bp = &tb.slot_blk_pat[(size_t)b * SLOT_BLK_PATS];
+            int np = 0;
+            for (int s = b * SLOT_BLK; s < std::min(tb.S_w, (b + 1) * SLOT_BLK); s++) {
+                const int32_t pat = tb.wls_i[(size_t)s * WLSI_STRIDE + WLSI_PAT];
+                int q = 0;
+                while (q < np && bp[q] != pat) q++;
+                if (q == np && np < SLOT_BLK_PATS) bp[np++] = pat;
+                if (q < np) tb.slot_lpat[s] = (uint8_t)q;
+                else tb.slot_uncov.push_back(s);
+            }
+        }
+    }
     tb.merge_ok = tb.fill_flat_ok;
     for (size_t d = 0; d * 4 < tb.dep_hdr.size(); d++)
         if (tb.dep_hdr[d * 4] == DEP_GENERIC) tb.merge_ok = false;
@@ -676,6 +695,17 @@ void build_tables(const Tree& t, Tables& tb) {
         if (q < -1 || q >= tb.L) throw std::runtime_error("flattened fill leaf out of range");
     for (int32_t k : tb.fill_fast)
         if (k < 0 || k >= tb.G + tb.S_r) throw std::runtime_error("fast fill item out of range");
+    for (int32_t p : tb.slot_blk_pat)
+        if (p < -1 || p >= tb.n_pat) throw std::runtime_error("staged slot record out of range");
+    for (int s = 0; s < tb.S_w; s++) {
+        const int lp = tb.slot_lpat[s];
+        if (lp != 255 && (lp >= SLOT_BLK_PATS ||
+                          tb.slot_blk_pat[(size_t)(s / SLOT_BLK) * SLOT_BLK_PATS + lp] !=
+                              tb.wls_i[(size_t)s * WLSI_STRIDE + WLSI_PAT]))
+            throw std::runtime_error("staged slot record wrong");
+    }
+    for (int32_t s : tb.slot_uncov)
+        if (s < 0 || s >= tb.S_w || tb.slot_lpat[s] != 255) throw std::runtime_error("unstaged slot list wrong");
     for (size_t d = 0; d * 4 < tb.dep_hdr.size(); d++) {
         const int cnt = tb.dep_hdr[d * 4];
         if (cnt == DEP_GENERIC) continue;

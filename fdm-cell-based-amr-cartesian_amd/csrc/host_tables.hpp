@@ -56,6 +56,8 @@ enum WlsFastLayout {
 
 constexpr int32_t REF_INFLOW = -2147483647 - 1;
 constexpr int SEL_NBR_CAP = 192;  // positions fit a byte
+// fast-path slot launch: slots per block and the most distinct FP32 records a block stages
+constexpr int SLOT_BLK = 64, SLOT_BLK_PATS = 24;
 
 // stencil classification, identical to the reference's gathers
 enum StKind { ST_COPY = 0, ST_PARENT_WLS = 1, ST_PARENT_RESTRICT = 2, ST_CHILD_WLS = 3, ST_CHILD_COPY = 4 };
@@ -108,6 +110,12 @@ struct Tables {
     // each entry of the ghost's fill list
     std::vector<int32_t> dep_hdr, dep_leaf;
     std::vector<float> dep_w;
+    // the distinct FP32 slot records of each block of SLOT_BLK slots (the first SLOT_BLK_PATS in
+    // slot order; -1 pads), and per slot its record's position in that list (255: not listed,
+    // read from wls_f directly)
+    std::vector<int32_t> slot_blk_pat;   // [ceil(S_w / SLOT_BLK)][SLOT_BLK_PATS]
+    std::vector<uint8_t> slot_lpat;      // [S_w]
+    std::vector<int32_t> slot_uncov;     // the slots with slot_lpat 255
     // the fast path refills ghosts in its slot launch when the flattened fill is usable and
     // every ghost-reading slot fits its fixed-size entry lists (else a fill launch per stage)
     bool merge_ok = true;
