@@ -664,8 +664,8 @@ __device__ __forceinline__ void slot_half(const DevTables& t, const float* __res
 // floats; the slots whose record did not fit the block's list (slot_uncov) are evaluated by the
 // blocks past nbs, with the record read from global memory (a block-uniform choice, so the
 // two variants do not share registers).
-__global__ __launch_bounds__(128) void k_slots_fast(DevTables t, const float* __restrict__ wf,
-                                                    float4* __restrict__ Qf) {
+__global__ __launch_bounds__(2 * SLOT_BLK) void k_slots_fast(DevTables t, const float* __restrict__ wf,
+                                                             float4* __restrict__ Qf) {
     const int nbs = (t.S_w + SLOT_BLK - 1) / SLOT_BLK;
     if ((int)blockIdx.x >= nbs) {
         const int k = ((int)blockIdx.x - nbs) * SLOT_BLK + ((int)threadIdx.x >> 1);
@@ -675,7 +675,7 @@ __global__ __launch_bounds__(128) void k_slots_fast(DevTables t, const float* __
     __shared__ __align__(16) float srec[SLOT_BLK_PATS * WF_STRIDE];
     const int32_t* __restrict__ bp = t.slot_blk_pat + (size_t)blockIdx.x * SLOT_BLK_PATS;
     constexpr int Q4 = WF_STRIDE / 4;
-    for (int k = threadIdx.x; k < SLOT_BLK_PATS * Q4; k += 128) {
+    for (int k = threadIdx.x; k < SLOT_BLK_PATS * Q4; k += 2 * SLOT_BLK) {
         const int q = k / Q4, c = k - q * Q4;
         const int p = bp[q];
         if (p >= 0) reinterpret_cast<float4*>(srec)[k] = reinterpret_cast<const float4*>(wf + (size_t)p * WF_STRIDE)[c];
@@ -788,8 +788,8 @@ static inline int nb_(int n, int b) { return (n + b - 1) / b; }
 
 void launch_slots_fast(const DevTables& t, const float* wf, const double4* Q, float4* Qf, hipStream_t st) {
     if (t.S_w > 0)
-        hipLaunchKernelGGL(k_slots_fast, dim3(nb_(t.S_w, SLOT_BLK) + nb_(t.S_unc, SLOT_BLK)), dim3(128), 0, st, t, wf,
-                           Qf);
+        hipLaunchKernelGGL(k_slots_fast, dim3(nb_(t.S_w, SLOT_BLK) + nb_(t.S_unc, SLOT_BLK)), dim3(2 * SLOT_BLK), 0,
+                           st, t, wf, Qf);
 }
 
 void launch_slots_fill(const DevTables& t, const float* wf, double4* Q, float4* Qf, const int32_t* items,

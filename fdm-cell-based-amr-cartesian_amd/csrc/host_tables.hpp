@@ -57,7 +57,7 @@ enum WlsFastLayout {
 constexpr int32_t REF_INFLOW = -2147483647 - 1;
 constexpr int SEL_NBR_CAP = 192;  // positions fit a byte
 // fast-path slot launch: slots per block and the most distinct FP32 records a block stages
-constexpr int SLOT_BLK = 64, SLOT_BLK_PATS = 24;
+constexpr int SLOT_BLK = 128, SLOT_BLK_PATS = 48;
 
 // stencil classification, identical to the reference's gathers
 enum StKind { ST_COPY = 0, ST_PARENT_WLS = 1, ST_PARENT_RESTRICT = 2, ST_CHILD_WLS = 3, ST_CHILD_COPY = 4 };
