@@ -56,7 +56,8 @@ enum WlsFastLayout {
 
 constexpr int32_t REF_INFLOW = -2147483647 - 1;
 constexpr int SEL_NBR_CAP = 192;  // positions fit a byte
-// fast-path slot launch: slots per block and the most distinct FP32 records a block stages
+// fast-path slot launch: slots per block (two lanes each) and the most distinct FP32 records a
+// block stages in LDS
 constexpr int SLOT_BLK = 128, SLOT_BLK_PATS = 48;
 
 // stencil classification, identical to the reference's gathers
