@@ -295,7 +295,7 @@ def main():
     ap.add_argument("--mesh-file", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        print(json.dumps({"error": "the benchmark runs on one GPU only; --gpus must be 1"}))
         raise SystemExit(2)
 
     if args.copy_calibration:

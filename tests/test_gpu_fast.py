@@ -1,7 +1,7 @@
 """GPU tests of the FP32-flux fast path and the selector flags (through the C-ABI).
 
 Tolerances: the fast path computes the WCNS reconstruction and flux in FP32 with hardware
-reciprocals, so it is compared with the FP64 path by the north-star accuracy bar,
+reciprocals, so it is compared with the FP64 path by the fast path's accuracy bar,
 L1(rho) <= 1e-4 (area-weighted, output_ADT.F90:43-61), on identical meshes.
 """
 import numpy as np

@@ -2,7 +2,8 @@
 
 The FP64 kernels are built without FMA contraction and evaluate in the reference's order, so
 the expected result is bit-for-bit equality; the stated tolerance (1e-12 relative, the
-north-star bar) is what the assertions enforce, and the exact-match count is reported.
+parity bar of the FP64 path) is what the assertions enforce, and the exact-match count is
+reported.
 """
 import numpy as np
 import pytest
