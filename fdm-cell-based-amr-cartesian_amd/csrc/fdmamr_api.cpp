@@ -49,6 +49,7 @@ struct fdmamr_ctx {
     DBuf<uint8_t> flags;     // selector flags per interior leaf (bit 0: troubled)
     DBuf<float> label;       // per-leaf WCNS weight deviation (labels / judge)
     bool use_flags = false;  // fast path consults the flags
+    bool merge_off = false;  // keep the slot launch and the ghost fill apart even where they merge
     DBuf<int32_t> lists;     // cleared leaves, then troubled leaves (from the flags)
     DBuf<int32_t> slot_blk_pat;  // distinct slot records of each slot block
     DBuf<uint8_t> slot_lpat;     // each slot's position in its block's list
@@ -259,11 +260,14 @@ void fill_after_stage(fdmamr_ctx* c, int buf) {
                          fast ? c->Qf[buf].p : nullptr, c->Qint.p, c->Tint.p, c->stream);
 }
 
+// the fast path refills the ghosts inside its slot launch
+bool slots_merged(const fdmamr_ctx* c) { return c->tb.merge_ok && !c->merge_off; }
+
 // After a run of steps on the fast path with merged fills, every ghost and restriction slot
 // of the final state (buffer 0), so that the state seen from outside is the same as after a
 // fill per stage.
 void finish_steps(fdmamr_ctx* c) {
-    if (c->p.flux_precision == FDMAMR_FLUX_FP32 && c->tb.merge_ok)
+    if (c->p.flux_precision == FDMAMR_FLUX_FP32 && slots_merged(c))
         launch_fill_flat(c->dt, c->Q[0].p, c->Qf[0].p, c->fill_beg.p, c->fill_leaf.p, c->fill_w.p, c->stream);
 }
 
@@ -337,7 +341,7 @@ void enqueue_rk3(fdmamr_ctx* c, int advance_time, bool prepare_next) {
     // fast path with flattened fills: each stage's input gets its unaliased ghosts and
     // restriction slots in the launch that evaluates its WLS slots (finish_steps completes
     // the ghosts of the final state)
-    const bool merged = fast && c->tb.merge_ok;
+    const bool merged = fast && slots_merged(c);
     for (int k = 0; k < 3; k++) {
         if (merged)
             launch_slots_fill(d, c->wls_f.p, c->Q[in[k]].p, c->Qf[in[k]].p, c->fill_fast.p,
@@ -1136,7 +1140,7 @@ int32_t fdmamr_stencil_values_fast(fdmamr_ctx* c, int32_t i, float* val8x4) {
     if (c->p.flux_precision != FDMAMR_FLUX_FP32) return fail(c, -1, "fdmamr_stencil_values_fast: FP32 contexts only");
     if (i < 0 || i >= c->tb.N || !val8x4) return fail(c, -1, "fdmamr_stencil_values_fast: bad leaf");
     // the slot and ghost work that precedes a stage, on state 0 (its values are unchanged)
-    if (c->tb.merge_ok)
+    if (slots_merged(c))
         launch_slots_fill(c->dt, c->wls_f.p, c->Q[0].p, c->Qf[0].p, c->fill_fast.p, (int)c->tb.fill_fast.size(),
                           c->fill_beg.p, c->fill_leaf.p, c->fill_w.p, c->stream);
     else
@@ -1145,6 +1149,12 @@ int32_t fdmamr_stencil_values_fast(fdmamr_ctx* c, int32_t i, float* val8x4) {
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(val8x4, c->tmp.p, 32 * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int32_t fdmamr_set_slot_merge(fdmamr_ctx* c, int32_t enable) {
+    if (!c) return -1;
+    c->merge_off = enable == 0;
     return 0;
 }
 

@@ -30,7 +30,7 @@ EXPORTS = (
     "fdmamr_build_tables", "fdmamr_counts", "fdmamr_set_state", "fdmamr_get_state",
     "fdmamr_get_ghost_state", "fdmamr_leaf_geometry", "fdmamr_max_wavespeed",
     "fdmamr_advance_rk3", "fdmamr_run_steps", "fdmamr_check_positivity", "fdmamr_sensors",
-    "fdmamr_regrid", "fdmamr_stencil_info", "fdmamr_stencil_values", "fdmamr_stencil_values_fast", "fdmamr_table_info", "fdmamr_leaf_lists", "fdmamr_profile",
+    "fdmamr_regrid", "fdmamr_stencil_info", "fdmamr_stencil_values", "fdmamr_stencil_values_fast", "fdmamr_table_info", "fdmamr_leaf_lists", "fdmamr_set_slot_merge", "fdmamr_profile",
     "fdmamr_profile_read", "fdmamr_selector_set_flags", "fdmamr_selector_get_flags",
     "fdmamr_labels", "fdmamr_selector_init", "fdmamr_selector_set_weights",
     "fdmamr_selector_get_weights", "fdmamr_selector_config", "fdmamr_selector_infer",
@@ -95,6 +95,7 @@ def lib():
         "fdmamr_stencil_values": ([P, C.c_int32, dp], C.c_int32),
         "fdmamr_table_info": ([P, ip], C.c_int32),
         "fdmamr_leaf_lists": ([P, ip, ip], C.c_int32),
+        "fdmamr_set_slot_merge": ([P, C.c_int32], C.c_int32),
         "fdmamr_stencil_values_fast": ([P, C.c_int32, C.POINTER(C.c_float)], C.c_int32),
         "fdmamr_profile": ([P, C.c_int32], C.c_int32),
         "fdmamr_profile_read": ([P, dp, ip], C.c_int32),
@@ -460,6 +461,10 @@ class Solver:
         self._check(self.L.fdmamr_stencil_values_fast(self.ctx, i, out.ctypes.data_as(C.POINTER(C.c_float))),
                     "stencil_values_fast")
         return out
+
+    def set_slot_merge(self, enable):
+        """False: WLS-slot launch and ghost refill as separate launches on every mesh."""
+        self._check(self.L.fdmamr_set_slot_merge(self.ctx, 1 if enable else 0), "set_slot_merge")
 
     def leaf_lists(self):
         """The fast path's (cleared, troubled) leaf lists as the last flagged step built them."""

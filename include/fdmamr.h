@@ -168,6 +168,10 @@ int32_t fdmamr_table_info(fdmamr_ctx* ctx, int32_t* info8);
  * diagnostics): lists[0, *n_clear) the cleared interior leaves, then the troubled ones, each
  * in ascending order (N ints) */
 int32_t fdmamr_leaf_lists(fdmamr_ctx* ctx, int32_t* lists, int32_t* n_clear);
+/* FP32 contexts: enable = 0 keeps the WLS-slot launch and the ghost refill as separate launches
+ * even on meshes where the refill can ride along in the slot launch (no reference counterpart;
+ * the default, 1, merges them where possible) */
+int32_t fdmamr_set_slot_merge(fdmamr_ctx* ctx, int32_t enable);
 
 /* ---- troubled-cell selector (no reference counterpart: the reference applies the nonlinear
  * WCNS weights everywhere, nnd_center_RK.F90:310-353). FP32 mode only. A leaf is "troubled"

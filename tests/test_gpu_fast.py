@@ -80,6 +80,26 @@ def test_leaf_lists_from_flags(base):
     assert np.array_equal(troubled, np.flatnonzero(flags == 1))
 
 
+def test_unmerged_slot_launch_matches_merged():
+    # Where the ghost refill rides along in the slot launch (this mesh), the alternative — the
+    # slot launch that stages each 128-slot block's distinct records in LDS, then a separate
+    # refill — must give the same steps: same arithmetic per slot; only the FP32 sums that form
+    # a ghost-reading slot's ghost points from their entry lists may round differently from the
+    # refill's FP64 sums, so states agree to FP32 round-off.
+    cfg = riemann_c2(h=1.0 / 64, lvl_amr=2, pre_adapt=2, flux_precision=F.FLUX_FP32)
+    a = F.Solver(cfg).setup()
+    b = F.Solver(cfg).setup()
+    b.set_slot_merge(False)
+    info = a.table_info()
+    assert info["slots"] > 1000 and info["slots_generic"] == 0
+    a.run_steps(4)
+    b.run_steps(4)
+    qa, qb = a.get_state(), b.get_state()
+    rel = float(np.max(np.abs(qa - qb) / (np.abs(qa) + 1.0)))
+    print("slots", info["slots"], "max rel diff merged / unmerged", rel)
+    assert rel < 1e-5
+
+
 def test_flat_ghost_fill_matches_nested_restriction():
     # The fast path fills ghost leaves in one pass from flattened weighted leaf sums; the FP64
     # path takes the reference's nested restriction means (interpA). With dt = 0 the stages
