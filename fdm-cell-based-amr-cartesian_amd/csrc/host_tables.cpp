@@ -315,6 +315,13 @@ void build_tables(const Tree& t, Tables& tb) {
 
     // phase 2 (serial, deterministic order): one slot per distinct level-jump value
     std::unordered_map<Key, int, KeyHash> wls_map, rst_map;
+    {
+        // size the slot map once (a level-jump value per stencil entry at most)
+        size_t jumps = 0;
+        for (size_t d = 0; d < tb.st_kind.size(); d++)
+            jumps += tb.st_kind[d] == ST_PARENT_WLS || tb.st_kind[d] == ST_CHILD_WLS;
+        wls_map.reserve(jumps);
+    }
     std::vector<int32_t> wls_node;      // per WLS slot: the WLS cell
     std::vector<int8_t> wls_parent;     // per WLS slot: parent (1) or child (0) stencil
     std::vector<double> wls_xx, wls_yy;
