@@ -680,17 +680,22 @@ void build_tables(const Tree& t, Tables& tb) {
     }
     for (int32_t r : tb.sel_nbr)
         if (r < 0 || r >= tb.L + tb.S_w + tb.S_r) throw std::runtime_error("selector neighbourhood index out of range");
-    for (int T = 0; T + 1 < (int)tb.sel_nbr_beg.size(); T++) {
-        const int nU = tb.sel_nbr_beg[T + 1] - tb.sel_nbr_beg[T];
-        if (nU > SEL_NBR_CAP) continue;
-        for (int i = T * 16; i < std::min(N, T * 16 + 16); i++)
-            for (int g = 0; g < 4; g++)
-                for (int j = 0; j < 8; j++)
-                    if (tb.sel_loc[(size_t)i * 32 + g * 8 + j] >= nU)
-                        throw std::runtime_error("selector neighbourhood position out of range");
-    }
-    for (int32_t r : tb.off16)
-        if (r < 0 || r >= tb.L) throw std::runtime_error("selector stencil index out of range");
+    // the largest tables are checked on all host threads
+    std::atomic<int> bad_loc{0}, bad_off{0}, bad_alias{0};
+    parallel_for(tb.sel_nbr_beg.empty() ? 0 : tb.sel_nbr_beg.size() - 1, [&](size_t b0, size_t e0) {
+        for (size_t T = b0; T < e0; T++) {
+            const int nU = tb.sel_nbr_beg[T + 1] - tb.sel_nbr_beg[T];
+            if (nU > SEL_NBR_CAP) continue;
+            for (size_t k = T * 16 * 32; k < std::min((size_t)N, T * 16 + 16) * 32; k++)
+                if (tb.sel_loc[k] >= nU) bad_loc = 1;
+        }
+    });
+    if (bad_loc) throw std::runtime_error("selector neighbourhood position out of range");
+    parallel_for(tb.off16.size(), [&](size_t b0, size_t e0) {
+        for (size_t k = b0; k < e0; k++)
+            if (tb.off16[k] < 0 || tb.off16[k] >= tb.L) bad_off = 1;
+    });
+    if (bad_off) throw std::runtime_error("selector stencil index out of range");
     for (size_t k = 0; k < tb.int_child.size(); k++) {
         const int32_t r = tb.int_child[k];
         const int self = (int)(k / 4);
@@ -725,9 +730,13 @@ void build_tables(const Tree& t, Tables& tb) {
             }
         }
     }
-    for (int32_t r : tb.stencil)
-        if (r >= tb.N && r < tb.N + tb.G && tb.ghost_alias[r - tb.N] >= 0)
-            throw std::runtime_error("stencil reads an aliased ghost");
+    parallel_for(tb.stencil.size(), [&](size_t b0, size_t e0) {
+        for (size_t k = b0; k < e0; k++) {
+            const int32_t r = tb.stencil[k];
+            if (r >= tb.N && r < tb.N + tb.G && tb.ghost_alias[r - tb.N] >= 0) bad_alias = 1;
+        }
+    });
+    if (bad_alias) throw std::runtime_error("stencil reads an aliased ghost");
     for (int32_t r : tb.rst_ref)
         if (r < 0 || r >= tb.n_int) throw std::runtime_error("restriction slot ref out of range");
     for (int32_t r : tb.bc_ref)
