@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -355,6 +356,8 @@ void enqueue_rk3(fdmamr_ctx* c, int advance_time, bool prepare_next) {
             launch_build_lists(d.N, c->flags.p, c->list_cnt.p, c->lists.p, c->list_meta.p, s);
         }
         int e0 = -1, e1 = -1;
+        // every stage flags a non-positive FP64 density or pressure of its result (no host sync)
+        int* posflag = &c->sc->pos_bad;
         if (c->profiling && (e0 = next_event(c)) >= 0) (void)hipEventRecord(c->ev_pool[e0], s);
         if (fast)
             launch_stage_fast(d, lists, c->list_meta.p, c->Q[in[k]].p, c->Qf[in[k]].p, c->Q[0].p,
@@ -362,11 +365,11 @@ void enqueue_rk3(fdmamr_ctx* c, int advance_time, bool prepare_next) {
                               c->partials.p, (prepare_next && k == 2) ? c->lampart.p : nullptr,
                               (select && k == 0) ? c->label.p : nullptr,
                               (select && k == 0) ? JudgeArgs{c->flags.p, c->sel_cnt.p, c->sel_bound} : JudgeArgs{},
-                              s);
+                              posflag, s);
         else
             launch_stage(d, c->Q[in[k]].p, c->Q[0].p, c->T[0].p, c->Q[out[k]].p, c->T[out[k]].p, c->sc, ra[k],
                          rb[k], k == 2 ? 1 : 0, c->partials.p, (prepare_next && k == 2) ? c->lampart.p : nullptr,
-                         s);
+                         posflag, s);
         if (e0 >= 0 && (e1 = next_event(c)) >= 0) {
             (void)hipEventRecord(c->ev_pool[e1], s);
             c->ev_stage.emplace_back(e0, e1);
@@ -475,33 +478,62 @@ int32_t fdmamr_tree_import(fdmamr_ctx* c, int32_t m, int32_t n, int32_t n_node,
                            const int32_t* father, const int32_t* son, const int32_t* lvl,
                            const int32_t* sp, const int32_t* num2, const int32_t* cross,
                            const double* x, const double* y, const double* prim7) {
-    if (!c || m < 7 || n < 7 || n_node < m * n || !father || !son || !lvl || !sp || !num2 ||
-        !cross || !x || !y || !prim7)
+    if (!c) return -1;
+    // the current tables describe the old tree: nothing may use them once the import starts
+    c->tables_ready = false;
+    if (m < 7 || n < 7 || (int64_t)m * n > n_node || !father || !son || !lvl || !sp || !num2 || !cross || !x ||
+        !y || !prim7)
         return fail(c, -1, "fdmamr_tree_import: invalid argument");
+    // Validate everything before the context's tree is touched: base cells in the reference's
+    // cell(i) order (mesh.F90:19-26), every other node the son sp of its father, son blocks of
+    // four consecutive ids that name their father, levels one below the father's.
+    const int total = m * n;
+    auto bad = [&](int k, const char* what) {
+        return fail(c, -1, std::string("fdmamr_tree_import: node ") + std::to_string(k) + ": " + what);
+    };
+    for (int k = 0; k < n_node; k++) {
+        if (cross[k] < -5 || cross[k] > 2) return bad(k, "cross out of range");
+        if (!std::isfinite(x[k]) || !std::isfinite(y[k])) return bad(k, "centre not finite");
+        if (k < total) {
+            if (father[k] != -1 || lvl[k] != 0 || sp[k] != 0) return bad(k, "base cell with a father, level or position");
+            if (num2[2 * k] != k / m + 1 || num2[2 * k + 1] != k % m + 1) return bad(k, "base cell index (num) wrong");
+        } else {
+            const int f = father[k];
+            if (f < 0 || f >= n_node) return bad(k, "father out of range");
+            if (son[f] < total || k < son[f] || k > son[f] + 3) return bad(k, "not a son of its father");
+            if (sp[k] != k - son[f] + 1) return bad(k, "son position (sp) wrong");
+            if (lvl[k] != lvl[f] + 1) return bad(k, "level is not the father's plus one");
+            if (num2[2 * k] != num2[2 * f] || num2[2 * k + 1] != num2[2 * f + 1]) return bad(k, "base cell index differs from the father's");
+        }
+        if (lvl[k] < 0 || lvl[k] >= MAX_TREE_DEPTH - 1) return bad(k, "level out of range");
+        if (son[k] != -1) {
+            if (son[k] < total || son[k] > n_node - 4) return bad(k, "son index out of range");
+            for (int j = 0; j < 4; j++)
+                if (father[son[k] + j] != k) return bad(k, "son block does not name this node as father");
+        }
+    }
     Tree& t = c->tree;
-    t.m = m; t.n = n; t.total = m * n;
+    std::vector<int32_t> vni(n_node), vnj(n_node);
+    for (int k = 0; k < n_node; k++) { vni[k] = num2[2 * k]; vnj[k] = num2[2 * k + 1]; }
+    std::vector<std::array<double, 7>> U(n_node);
+    for (int k = 0; k < n_node; k++)
+        for (int i = 0; i < 7; i++) U[k][i] = prim7[7 * (size_t)k + i];
+    t.m = m; t.n = n; t.total = total;
     t.father.assign(father, father + n_node);
     t.son.assign(son, son + n_node);
     t.lvl.assign(lvl, lvl + n_node);
     t.sp.assign(sp, sp + n_node);
     t.cross.assign(cross, cross + n_node);
-    t.ni.resize(n_node); t.nj.resize(n_node);
-    for (int k = 0; k < n_node; k++) { t.ni[k] = num2[2 * k]; t.nj[k] = num2[2 * k + 1]; }
+    t.ni.swap(vni);
+    t.nj.swap(vnj);
     t.x.assign(x, x + n_node);
     t.y.assign(y, y + n_node);
-    t.U.resize(n_node);
-    for (int k = 0; k < n_node; k++)
-        for (int i = 0; i < 7; i++) t.U[k][i] = prim7[7 * (size_t)k + i];
+    t.U.swap(U);
     t.rot.assign(n_node, 0.0);
     t.div.assign(n_node, 0.0);
     t.free_blocks.clear();
-    for (int k = 0; k < n_node; k++) {
-        if (son[k] >= 0 && (son[k] + 3 >= n_node || son[k] < t.total))
-            return fail(c, -1, "fdmamr_tree_import: son index out of range");
-    }
     t.idneighbor();
     t.idgrid();
-    c->tables_ready = false;
     return 0;
 }
 
@@ -672,6 +704,7 @@ int32_t fdmamr_advance_rk3(fdmamr_ctx* c, double dt, double lam_x, double lam_y,
         if (res_max4) res_max4[k] = s.res_max[k];
         if (res_rms_sum4) res_rms_sum4[k] = s.res_sum[k];
     }
+    if (s.pos_bad) return fail(c, -4, "non-positive density or pressure in a stage of the step");
     for (int k = 0; k < 4; k++)
         if (s.res_sum[k] != s.res_sum[k]) return fail(c, -3, "NaN residual");
     return 0;
@@ -708,6 +741,7 @@ int32_t fdmamr_run_steps(fdmamr_ctx* c, int32_t n_steps, int32_t localdt, double
         scalars8[0] = s.tt; scalars8[1] = s.dt; scalars8[2] = s.lam_x; scalars8[3] = s.lam_y;
         for (int k = 0; k < 4; k++) scalars8[4 + k] = s.res_rms[k];
     }
+    if (s.status == -4) return fail(c, -4, "non-positive density or pressure in a stage of a step");
     if (s.status != 0) return fail(c, -3, "NaN residual");
     return 0;
 }
@@ -852,7 +886,7 @@ int32_t fdmamr_labels(fdmamr_ctx* c, float* dev) {
     launch_shadow(c->Q[0].p, c->Qf[0].p, c->tb.L + c->tb.S_w + c->tb.S_r, c->stream);
     launch_slots_fast(c->dt, c->wls_f.p, c->Q[0].p, c->Qf[0].p, c->stream);
     launch_stage_fast(c->dt, nullptr, nullptr, c->Q[0].p, c->Qf[0].p, c->Q[0].p, c->Q[2].p, c->Qf[2].p,
-                      c->sc, 0.0, 1.0, 0, c->partials.p, nullptr, c->label.p, JudgeArgs{}, c->stream);
+                      c->sc, 0.0, 1.0, 0, c->partials.p, nullptr, c->label.p, JudgeArgs{}, nullptr, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(dev, c->label.p, c->tb.N * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1012,7 +1046,7 @@ int32_t fdmamr_selector_collect(fdmamr_ctx* c, int32_t* n_rows, int32_t* n_posit
     launch_shadow(c->Q[0].p, c->Qf[0].p, c->tb.L + c->tb.S_w + c->tb.S_r, c->stream);
     launch_slots_fast(c->dt, c->wls_f.p, c->Q[0].p, c->Qf[0].p, c->stream);
     launch_stage_fast(c->dt, nullptr, nullptr, c->Q[0].p, c->Qf[0].p, c->Q[0].p, c->Q[2].p, c->Qf[2].p,
-                      c->sc, 0.0, 1.0, 0, c->partials.p, nullptr, c->label.p, JudgeArgs{}, c->stream);
+                      c->sc, 0.0, 1.0, 0, c->partials.p, nullptr, c->label.p, JudgeArgs{}, nullptr, c->stream);
     launch_features(c->dt, c->Qf[0].p, c->off16.p, c->ds_x.p, c->ds_rows, c->stream);
     launch_labels_to_y(N, c->label.p, c->sel_bound, c->ds_y.p, c->ds_rows, c->stream);
     HIPCHK(c, hipGetLastError());

@@ -117,7 +117,9 @@ struct DevTables {
 struct DevScalars {
     double dt, lam_x, lam_y, tt, t_end, cfl, dt_fixed, h, xf;
     double res_max[4], res_sum[4], res_rms[4];
-    int32_t localdt, l_max, steps_done, status;
+    // status: 0, -3 NaN residual (solution.F90:209-214), -4 non-positive rho or p in a stage;
+    // pos_bad: set by a stage when any leaf's FP64 rho or p of its result is <= 0
+    int32_t localdt, l_max, steps_done, status, pos_bad;
 };
 
 void launch_slots(const DevTables& t, double4* Q, const double* T, hipStream_t st);
@@ -125,7 +127,7 @@ void launch_slots(const DevTables& t, double4* Q, const double* T, hipStream_t s
 // partials of the output state (the next step's lambda, 2 doubles per block)
 void launch_stage(const DevTables& t, const double4* Qin, const double4* Qn, const double* Tn,
                   double4* Qout, double* Tout, const DevScalars* sc, double alpha, double beta,
-                  int reduce, double* partials, double* lampart, hipStream_t st);
+                  int reduce, double* partials, double* lampart, int* posflag, hipStream_t st);
 // ghost fill and restriction slots after a stage (restriction means one level per launch)
 void launch_restrict_fill(const DevTables& t, const std::vector<int32_t>& level_start, double4* Q,
                           double* T, float4* Qf, double4* Qi, double* Ti, hipStream_t st);
@@ -157,11 +159,12 @@ struct JudgeArgs {
 // lists: cleared leaves then troubled leaves (meta[0] = number cleared), or nullptr for the
 // full path on every leaf; residual partials occupy stage_fast_partial_blocks() blocks. With
 // label, each troubled leaf's largest weight deviation goes to label, and with jd.flags (lists
-// only) the judge runs on them in the same launch.
+// only) the judge runs on them in the same launch. posflag (or nullptr) is set to 1 when a
+// leaf's FP64 result has rho <= 0 or p <= 0.
 void launch_stage_fast(const DevTables& t, const int32_t* lists, const int* meta, const double4* Qin,
                        const float4* Qf_in, const double4* Qn, double4* Qout, float4* Qf_out,
                        const DevScalars* sc, double alpha, double beta, int reduce, double* partials,
-                       double* lampart, float* label, const JudgeArgs& jd, hipStream_t st);
+                       double* lampart, float* label, const JudgeArgs& jd, int* posflag, hipStream_t st);
 int stage_fast_partial_blocks(int n, bool lists);
 // leaf lists from selector flags (bit 0: troubled); blk_cnt holds list_blocks(n) ints, the
 // troubled count of each block of LIST_BLK leaves (turned into the blocks' prefix offsets).

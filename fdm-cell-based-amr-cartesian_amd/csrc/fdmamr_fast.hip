@@ -187,7 +187,7 @@ __device__ __forceinline__ void update_leaf(int i, const double4& qc, const doub
                                             const float* dF, double dx, double dy, double gama, double dt,
                                             double alpha, double beta, double4* Qout,
                                             float4* __restrict__ Qf_out, double* rmax, double* rsum, bool lam,
-                                            double& lx, double& ly) {
+                                            double& lx, double& ly, int* posflag) {
     double U[4], Ut[4];
     U[0] = qc.x;
     U[1] = qc.x * qc.y;
@@ -212,6 +212,9 @@ __device__ __forceinline__ void update_leaf(int i, const double4& qc, const doub
     o.w = alpha * qn.w + beta * p;
     Qout[i] = o;
     Qf_out[i] = make_float4((float)o.x, (float)o.y, (float)o.z, (float)o.w);
+    // FP64 positivity of the stage result: rho <= 0 or p <= 0 sets the flag (a NaN does not: it
+    // reaches the residual and the NaN stop)
+    if (posflag && (o.x <= 0.0 || o.w <= 0.0)) atomicOr(posflag, 1);
     if (lam) {
         const double a = sqrt(gama * o.w / o.x);
         lx = fabs(o.y) + a;
@@ -225,7 +228,8 @@ __device__ __forceinline__ void stage_leaf_lin(const DevTables& t, int i, const 
                                                const float4* __restrict__ Qf_in, const double4* Qn,
                                                double4* Qout, float4* __restrict__ Qf_out,
                                                const DevScalars* __restrict__ sc, double alpha, double beta,
-                                               double* rmax, double* rsum, bool lam, double& lx, double& ly) {
+                                               double* rmax, double* rsum, bool lam, double& lx, double& ly,
+                                               int* posflag) {
     const double gama = t.gamma;
     int s[8];
 #pragma unroll
@@ -245,7 +249,8 @@ __device__ __forceinline__ void stage_leaf_lin(const DevTables& t, int i, const 
         const float4 col[5] = {Qf_in[s[7]], Qf_in[s[3]], qcf, Qf_in[s[2]], Qf_in[s[6]]};
         flux_lin_f<1>(col, (float)dy, (float)sc->lam_y, (float)gama, dF);
     }
-    update_leaf(i, qc, Qn[i], dE, dF, dx, dy, gama, sc->dt, alpha, beta, Qout, Qf_out, rmax, rsum, lam, lx, ly);
+    update_leaf(i, qc, Qn[i], dE, dF, dx, dy, gama, sc->dt, alpha, beta, Qout, Qf_out, rmax, rsum, lam, lx, ly,
+                posflag);
 }
 
 // Linear (cleared) leaves: lists[0, n_clear) (meta[0] = n_clear), ascending leaf order.
@@ -256,7 +261,7 @@ __device__ __forceinline__ void lin_block(int blk, const DevTables& t, const int
                                           const float4* __restrict__ Qf_in, const double4* Qn, double4* Qout,
                                           float4* __restrict__ Qf_out, const DevScalars* __restrict__ sc,
                                           double alpha, double beta, int reduce, double* __restrict__ partials,
-                                          double* __restrict__ lampart, int poff) {
+                                          double* __restrict__ lampart, int poff, int* posflag) {
     const int n = meta[0];
     // the active prefix of the grid is spread over all XCDs, then renumbered within itself
     const int nact = (n + 255) >> 8;
@@ -270,7 +275,7 @@ __device__ __forceinline__ void lin_block(int blk, const DevTables& t, const int
     double lx = 0.0, ly = 0.0;
     if (p < n)
         stage_leaf_lin(t, lists[p], Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, rmax, rsum, lampart != nullptr,
-                       lx, ly);
+                       lx, ly, posflag);
     if (reduce) block_partials(rmax, rsum, partials, poff + lb, lampart, lx, ly);
 }
 
@@ -286,7 +291,7 @@ __device__ __forceinline__ void full_block(int blk, const DevTables& t, const in
                                            float4* __restrict__ Qf_out, const DevScalars* __restrict__ sc,
                                            double alpha, double beta, int reduce, double* __restrict__ partials,
                                            double* __restrict__ lampart, int poff, float* __restrict__ label,
-                                           const JudgeArgs& jd) {
+                                           const JudgeArgs& jd, int* posflag) {
     int n = t.N, base = 0;
     if (lists) {
         base = meta[0];
@@ -360,7 +365,7 @@ __device__ __forceinline__ void full_block(int blk, const DevTables& t, const in
     if (valid && q == 0) {
         if (label) label[i] = dev;
         update_leaf(i, Qin[i], Qn[i], dE, dF, dx, dy, gama, sc->dt, alpha, beta, Qout, Qf_out, rmax, rsum,
-                    lampart != nullptr, lx, ly);
+                    lampart != nullptr, lx, ly, posflag);
     }
     if (LABEL && jd.flags) {
         // the judge: a judged leaf (cleared by the network, sampled onto this path) whose label
@@ -398,13 +403,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                                                      const DevScalars* __restrict__ sc, double alpha, double beta,
                                                      int reduce, double* __restrict__ partials,
                                                      double* __restrict__ lampart, int nb4, float* __restrict__ label,
-                                                     JudgeArgs jd) {
+                                                     JudgeArgs jd, int* posflag) {
     if ((int)blockIdx.x < nb4)
         full_block<LABEL>(blockIdx.x, t, lists, meta, Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, reduce,
-                          partials, lampart, 0, label, jd);
+                          partials, lampart, 0, label, jd, posflag);
     else
         lin_block(blockIdx.x - nb4, t, lists, meta, Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, reduce,
-                  partials, lampart, nb4);
+                  partials, lampart, nb4, posflag);
 }
 
 // every leaf on the full path (no flags)
@@ -414,9 +419,10 @@ __global__ __launch_bounds__(256) void k_stage_full(DevTables t, const double4* 
                                                     double4* Qout, float4* __restrict__ Qf_out,
                                                     const DevScalars* __restrict__ sc, double alpha, double beta,
                                                     int reduce, double* __restrict__ partials,
-                                                    double* __restrict__ lampart, float* __restrict__ label) {
+                                                    double* __restrict__ lampart, float* __restrict__ label,
+                                                    int* posflag) {
     full_block<LABEL>(blockIdx.x, t, nullptr, nullptr, Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, reduce,
-                      partials, lampart, 0, label, JudgeArgs{});
+                      partials, lampart, 0, label, JudgeArgs{}, posflag);
 }
 
 // ---------------------------------------------------------------- leaf lists from flags
@@ -803,23 +809,23 @@ void launch_slots_fill(const DevTables& t, const float* wf, double4* Q, float4* 
 void launch_stage_fast(const DevTables& t, const int32_t* lists, const int* meta, const double4* Qin,
                        const float4* Qf_in, const double4* Qn, double4* Qout, float4* Qf_out,
                        const DevScalars* sc, double alpha, double beta, int reduce, double* partials,
-                       double* lampart, float* label, const JudgeArgs& jd, hipStream_t st) {
+                       double* lampart, float* label, const JudgeArgs& jd, int* posflag, hipStream_t st) {
     if (t.N <= 0) return;
     const int nb = nb_(t.N, 256), nb4 = nb_(4 * t.N, 256);
     if (lists) {
         if (label)
             hipLaunchKernelGGL(k_stage_mixed<true>, dim3(nb4 + nb), dim3(256), 0, st, t, lists, meta, Qin, Qf_in,
-                               Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials, lampart, nb4, label, jd);
+                               Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials, lampart, nb4, label, jd, posflag);
         else
             hipLaunchKernelGGL(k_stage_mixed<false>, dim3(nb4 + nb), dim3(256), 0, st, t, lists, meta, Qin, Qf_in,
                                Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials, lampart, nb4, label,
-                               JudgeArgs{});
+                               JudgeArgs{}, posflag);
     } else if (label) {
         hipLaunchKernelGGL(k_stage_full<true>, dim3(nb4), dim3(256), 0, st, t, Qin, Qf_in, Qn, Qout, Qf_out, sc,
-                           alpha, beta, reduce, partials, lampart, label);
+                           alpha, beta, reduce, partials, lampart, label, posflag);
     } else {
         hipLaunchKernelGGL(k_stage_full<false>, dim3(nb4), dim3(256), 0, st, t, Qin, Qf_in, Qn, Qout, Qf_out, sc,
-                           alpha, beta, reduce, partials, lampart, label);
+                           alpha, beta, reduce, partials, lampart, label, posflag);
     }
 }
 

@@ -208,7 +208,7 @@ __global__ __launch_bounds__(256) void k_stage(DevTables t, const double4* __res
                                                double4* Qout, double* Tout,
                                                const DevScalars* __restrict__ sc, double alpha,
                                                double beta, int reduce, double* __restrict__ partials,
-                                               double* __restrict__ lampart) {
+                                               double* __restrict__ lampart, int* posflag) {
     const int lb = xcd_block();
     const int i = lb * blockDim.x + threadIdx.x;
     double rmax[4] = {0, 0, 0, 0}, rsum[4] = {0, 0, 0, 0};
@@ -261,6 +261,9 @@ __global__ __launch_bounds__(256) void k_stage(DevTables t, const double4* __res
         o.w = alpha * qn.w + beta * p;
         Qout[i] = o;
         Tout[i] = alpha * Tn[i] + beta * T;
+        // FP64 positivity of the stage result: rho <= 0 or p <= 0 sets the flag (a NaN does not: it
+        // reaches the residual and the NaN stop)
+        if (posflag && (o.x <= 0.0 || o.w <= 0.0)) atomicOr(posflag, 1);
         if (lampart) {
             // the next step's wave speeds on this final state (solution.F90:186-193)
             const double a = sqrt(t.gamma * o.w / o.x);
@@ -592,6 +595,8 @@ __global__ __launch_bounds__(STEP_THREADS) void k_step_end(const double* __restr
                 sc->res_rms[k] = r;
                 if (r != r) sc->status = -3;
             }
+            // positivity lost in one of the step's stages (it precedes the NaN it usually causes)
+            if (sc->pos_bad != 0) sc->status = -4;
         }
         if (prepare_next) step_begin_scalar(v[4], v[5], sc);
     }
@@ -667,10 +672,10 @@ void launch_restrict_fill(const DevTables& t, const std::vector<int32_t>& level_
 
 void launch_stage(const DevTables& t, const double4* Qin, const double4* Qn, const double* Tn,
                   double4* Qout, double* Tout, const DevScalars* sc, double alpha, double beta,
-                  int reduce, double* partials, double* lampart, hipStream_t st) {
+                  int reduce, double* partials, double* lampart, int* posflag, hipStream_t st) {
     if (t.N <= 0) return;
     hipLaunchKernelGGL(k_stage<double>, dim3(nblk(t.N, 256)), dim3(256), 0, st, t, Qin, Qn, Tn, Qout, Tout,
-                       sc, alpha, beta, reduce, partials, lampart);
+                       sc, alpha, beta, reduce, partials, lampart, posflag);
 }
 
 void launch_wavespeed(const DevTables& t, const double4* Q, double* partials, hipStream_t st) {

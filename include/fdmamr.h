@@ -86,7 +86,9 @@ int32_t fdmamr_initial(fdmamr_ctx* ctx, int32_t ic_kind, uint64_t seed, const do
 /* Replace the whole quadtree with a caller-flattened one (for drivers that own their tree):
  * n_node nodes; base cells are nodes 0..m*n-1 in the reference's cell(i) order; son[k] is the
  * id of son1 of node k (son1..son4 consecutive) or -1; lvl/sp/num/cross as type(grid)
- * (module.F90:24-35); prim7 = U(1:7) of every node. Rebuilds neighbour caches. */
+ * (module.F90:24-35); prim7 = U(1:7) of every node. Rebuilds neighbour caches. The whole input
+ * is validated first (father/son/sp/lvl/num consistency, ranges); on -1 the context keeps its
+ * old tree. Either way the tables must be rebuilt (fdmamr_build_tables) before the next step. */
 int32_t fdmamr_tree_import(fdmamr_ctx* ctx, int32_t m, int32_t n, int32_t n_node,
                            const int32_t* father, const int32_t* son, const int32_t* lvl,
                            const int32_t* sp, const int32_t* num2, const int32_t* cross,
@@ -120,13 +122,17 @@ int32_t fdmamr_max_wavespeed(fdmamr_ctx* ctx, double* lam_x, double* lam_y, int3
  * values (WLS-WENO / restriction), fused WCNS3 x+y flux derivative + forward-Euler substep +
  * primitive RK blend (nnd_center_RK.F90:15-112, solution.F90:386-404), then the ghost fill
  * (solution.F90:288-293). res_max/res_rms_sum receive Res_max(1:4) and Res_rms(1:4) of the
- * last stage before the sqrt (nnd_center_RK.F90:31-36). */
+ * last stage before the sqrt (nnd_center_RK.F90:31-36). Returns -4 when a stage's FP64 result
+ * has rho <= 0 or p <= 0 at any leaf (new check, folded into the stage kernels), else -3 on a
+ * NaN residual. */
 int32_t fdmamr_advance_rk3(fdmamr_ctx* ctx, double dt, double lam_x, double lam_y,
                            double* res_max4, double* res_rms_sum4);
 /* n passes of the solute loop body without AMR (solution.F90:183-215) kept on the device:
  * running-max lambda, dt = cfl/(lx/dx_min + ly/dy_min) when localdt != 0 (else fixed dt),
  * clamp to t_end, RK3. No host synchronisation inside; scalars[8] (may be NULL) receive
- * tt, dt, lam_x, lam_y, Res_rms(1:4) after the last step (sqrt applied). */
+ * tt, dt, lam_x, lam_y, Res_rms(1:4) after the last step (sqrt applied). A step whose stages
+ * leave rho <= 0 or p <= 0 at a leaf stops the clock (tt no longer advances) and the call
+ * returns -4; a NaN residual does the same with -3 (the reference's stop, solution.F90:209-214). */
 int32_t fdmamr_run_steps(fdmamr_ctx* ctx, int32_t n_steps, int32_t localdt, double cfl,
                          double dt_fixed, double t_end, double* tt_inout, double* lam_inout2,
                          double* scalars8);
