@@ -20,6 +20,8 @@
 
 using namespace fdm;
 
+static_assert(FDMAMR_NOALLOC_MAX_LEAVES == NOALLOC_BLOCKS * 256, "no-allocation scratch size");
+
 
 template <typename T>
 struct DBuf {
@@ -404,7 +406,10 @@ int32_t fdmamr_create(const fdmamr_params* p, fdmamr_ctx** out) {
     c->p = *p;
     c->host_only = host_only;
     if (!host_only && (hipSetDevice(p->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
-                       hipMalloc((void**)&c->sc, sizeof(DevScalars)) != hipSuccess)) {
+                       hipMalloc((void**)&c->sc, sizeof(DevScalars)) != hipSuccess ||
+                       noalloc_scratch_init(p->device) != hipSuccess)) {
+        if (c->sc) (void)hipFree(c->sc);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return -2;
     }
@@ -683,6 +688,24 @@ int32_t fdmamr_max_wavespeed(fdmamr_ctx* c, double* lam_x, double* lam_y, int32_
     for (int b = 0; b < nb; b++) { lx = std::max(lx, part[2 * b]); ly = std::max(ly, part[2 * b + 1]); }
     if (lam_x) *lam_x = lx;
     if (lam_y) *lam_y = ly;
+    if (l_max) *l_max = c->tb.l_max;
+    return 0;
+}
+
+int32_t fdmamr_max_wavespeed_noalloc(fdmamr_ctx* c, double* lam_x, double* lam_y, int32_t* l_max) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (c->tb.N > FDMAMR_NOALLOC_MAX_LEAVES)
+        return fail(c, -1, "fdmamr_max_wavespeed_noalloc: " + std::to_string(c->tb.N) +
+                               " leaves exceed its scratch (" + std::to_string(FDMAMR_NOALLOC_MAX_LEAVES) +
+                               " at most); use fdmamr_max_wavespeed");
+    double out[2] = {0.0, 0.0};
+    if (c->tb.N > 0) {
+        HIPCHK(c, launch_wavespeed_noalloc(c->dt, c->Q[0].p, c->p.device, out, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    if (lam_x) *lam_x = out[0];
+    if (lam_y) *lam_y = out[1];
     if (l_max) *l_max = c->tb.l_max;
     return 0;
 }

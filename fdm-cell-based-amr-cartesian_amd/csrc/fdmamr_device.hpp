@@ -141,6 +141,15 @@ void launch_step_end(const double* partials, int nb, const int* meta, int lin0, 
                      DevScalars* sc, const double* lampart, double* parts, hipStream_t st);
 void launch_sensors(const DevTables& t, const double4* Q, double* rot, double* div, hipStream_t st);
 void launch_positivity(const DevTables& t, const double4* Q, int* nbad, hipStream_t st);
+// fdmamr_max_wavespeed_noalloc: the wave-speed reduction through a static device scratch array
+// (NOALLOC_BLOCKS blocks of 256 leaves at most). noalloc_scratch_init takes the array's address
+// on `device`, which must be current; launch_wavespeed_noalloc writes the two maxima to the
+// host array out2 (asynchronously on st).
+constexpr int NOALLOC_BLOCKS = 131072;
+constexpr int NOALLOC_DEVICES = 64;
+hipError_t noalloc_scratch_init(int device);
+hipError_t launch_wavespeed_noalloc(const DevTables& t, const double4* Q, int device, double* out2,
+                                    hipStream_t st);
 void launch_stencil_values(const DevTables& t, const double4* Q, int i, double* out, hipStream_t st);
 
 // fast path (fdmamr_fast.hip)

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <vector>
 
 #include "fdmamr_device.hpp"
@@ -636,6 +637,52 @@ __global__ void k_stencil_values(DevTables t, const double4* __restrict__ Q, int
         const double4 q = Q[t.stencil[(size_t)k * t.N + i]];
         out[4 * k + 0] = q.x; out[4 * k + 1] = q.y; out[4 * k + 2] = q.z; out[4 * k + 3] = q.w;
     }
+}
+
+// ---------------------------------------------------------------- no-allocation wave speeds
+// Scratch of fdmamr_max_wavespeed_noalloc: 2 doubles of wave-speed partials per 256-leaf block,
+// then the two reduced maxima. It is a file-scope device array, so that entry point allocates
+// nothing; every GPU the library runs on holds its own copy until the process ends, and calls
+// that overlap on one GPU share it (they race). Host code uses only the address
+// hipGetSymbolAddress returned for the current device, never the array's name.
+static __device__ __align__(256) double scratch[2 * NOALLOC_BLOCKS + 8];
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[NOALLOC_DEVICES] = {};
+
+hipError_t noalloc_scratch_init(int device) {
+    if (device < 0 || device >= NOALLOC_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e == hipSuccess) g_scratch_ptr[device] = static_cast<double*>(ptr);
+    return e;
+}
+
+static double* noalloc_scratch(int device) {
+    if (device < 0 || device >= NOALLOC_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
+__global__ __launch_bounds__(STEP_THREADS) void k_reduce_lam(const double* __restrict__ partials, int nblk,
+                                                             double* __restrict__ out) {
+    double lam[2];
+    reduce_entries<2, 2>(partials, nblk, lam);
+    if (threadIdx.x == 0) { out[0] = lam[0]; out[1] = lam[1]; }
+}
+
+hipError_t launch_wavespeed_noalloc(const DevTables& t, const double4* Q, int device, double* out2,
+                                    hipStream_t st) {
+    double* sp = noalloc_scratch(device);
+    if (!sp) return hipErrorNotInitialized;
+    const int nb = (t.N + 255) / 256;
+    if (nb <= 0 || nb > NOALLOC_BLOCKS) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_wavespeed, dim3(nb), dim3(256), 0, st, t, Q, sp);
+    hipLaunchKernelGGL(k_reduce_lam, dim3(1), dim3(STEP_THREADS), 0, st, sp, nb, sp + 2 * NOALLOC_BLOCKS);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return hipMemcpyAsync(out2, sp + 2 * NOALLOC_BLOCKS, 2 * sizeof(double), hipMemcpyDeviceToHost, st);
 }
 
 // ---------------------------------------------------------------- launchers

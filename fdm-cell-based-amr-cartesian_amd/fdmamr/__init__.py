@@ -29,6 +29,7 @@ EXPORTS = (
     "fdmamr_initial", "fdmamr_tree_import", "fdmamr_tree_export", "fdmamr_tree_size",
     "fdmamr_build_tables", "fdmamr_counts", "fdmamr_set_state", "fdmamr_get_state",
     "fdmamr_get_ghost_state", "fdmamr_leaf_geometry", "fdmamr_max_wavespeed",
+    "fdmamr_max_wavespeed_noalloc",
     "fdmamr_advance_rk3", "fdmamr_run_steps", "fdmamr_check_positivity", "fdmamr_sensors",
     "fdmamr_regrid", "fdmamr_stencil_info", "fdmamr_stencil_values", "fdmamr_stencil_values_fast", "fdmamr_table_info", "fdmamr_leaf_lists", "fdmamr_set_slot_merge", "fdmamr_profile",
     "fdmamr_profile_read", "fdmamr_selector_set_flags", "fdmamr_selector_get_flags",
@@ -86,6 +87,7 @@ def lib():
         "fdmamr_get_ghost_state": ([P, dp], C.c_int32),
         "fdmamr_leaf_geometry": ([P, dp, dp, ip, ip], C.c_int32),
         "fdmamr_max_wavespeed": ([P, dp, dp, ip], C.c_int32),
+        "fdmamr_max_wavespeed_noalloc": ([P, dp, dp, ip], C.c_int32),
         "fdmamr_advance_rk3": ([P, C.c_double, C.c_double, C.c_double, dp, dp], C.c_int32),
         "fdmamr_run_steps": ([P, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, dp, dp, dp], C.c_int32),
         "fdmamr_check_positivity": ([P, ip], C.c_int32),
@@ -295,6 +297,13 @@ class Solver:
         a, b, l = C.c_double(), C.c_double(), C.c_int32()
         self._check(self.L.fdmamr_max_wavespeed(self.ctx, C.byref(a), C.byref(b), C.byref(l)),
                     "max_wavespeed")
+        return a.value, b.value, l.value
+
+    def max_wavespeed_noalloc(self):
+        """max_wavespeed through the library's static scratch (allocates no device memory)."""
+        a, b, l = C.c_double(), C.c_double(), C.c_int32()
+        self._check(self.L.fdmamr_max_wavespeed_noalloc(self.ctx, C.byref(a), C.byref(b), C.byref(l)),
+                    "max_wavespeed_noalloc")
         return a.value, b.value, l.value
 
     def advance_rk3(self, dt, lam_x, lam_y):

@@ -118,6 +118,14 @@ int32_t fdmamr_leaf_geometry(fdmamr_ctx* ctx, double* x, double* y, int32_t* lvl
 /* lambda/l_max loop of solute (solution.F90:185-193): max over interior leaves of |u|+c and
  * |v|+c. The caller keeps the running maximum, as the reference does. */
 int32_t fdmamr_max_wavespeed(fdmamr_ctx* ctx, double* lam_x, double* lam_y, int32_t* l_max);
+/* No counterpart in the reference: fdmamr_max_wavespeed for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same result, bit for bit; its partials go to a static device scratch array of 2 MiB that each
+ * GPU holds until the process ends, so it calls no hipMalloc/hipFree. Meshes of more than
+ * FDMAMR_NOALLOC_MAX_LEAVES interior leaves are rejected (-1). Calls that overlap on one GPU
+ * share the scratch and race: serialise them (one context's calls are serial already). */
+#define FDMAMR_NOALLOC_MAX_LEAVES 33554432
+int32_t fdmamr_max_wavespeed_noalloc(fdmamr_ctx* ctx, double* lam_x, double* lam_y, int32_t* l_max);
 /* Time_advance_RK (solution.F90:267-298): three SSP-RK3 stages, each = level-jump slot
  * values (WLS-WENO / restriction), fused WCNS3 x+y flux derivative + forward-Euler substep +
  * primitive RK blend (nnd_center_RK.F90:15-112, solution.F90:386-404), then the ghost fill
