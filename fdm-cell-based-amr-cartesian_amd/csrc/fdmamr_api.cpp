@@ -807,14 +807,10 @@ int32_t fdmamr_sensors(fdmamr_ctx* c, double* rot, double* div) {
     return 0;
 }
 
-int32_t fdmamr_regrid(fdmamr_ctx* c, int32_t lvl_amr, double refine_rot, double refine_div,
-                      double coarse_rot, double coarse_div, int32_t sensor_mode) {
-    int r = check_ready(c);
-    if (r) return r;
-    r = pull_state(c);
-    if (r) return r;
-    r = fdmamr_sensors(c, nullptr, nullptr);
-    if (r) return r;
+// AMR body after rot_div (solution.F90:413-419): RMS of the sensors, solute_ref,
+// solute_gridmodify, idneighbor, IDgrid + tables. The tree's rot/div hold the sensors.
+static int regrid_from_sensors(fdmamr_ctx* c, int32_t lvl_amr, double refine_rot, double refine_div,
+                               double coarse_rot, double coarse_div, int32_t sensor_mode) {
     Tree& t = c->tree;
     if (sensor_mode == 1)
         for (int k : t.ns) t.rot[k] = t.div[k];
@@ -834,6 +830,35 @@ int32_t fdmamr_regrid(fdmamr_ctx* c, int32_t lvl_amr, double refine_rot, double 
     }
     // the upload refills the ghosts from the new interior state (solution.F90:224-229)
     return fdmamr_build_tables(c);
+}
+
+int32_t fdmamr_regrid(fdmamr_ctx* c, int32_t lvl_amr, double refine_rot, double refine_div,
+                      double coarse_rot, double coarse_div, int32_t sensor_mode) {
+    int r = check_ready(c);
+    if (r) return r;
+    r = pull_state(c);
+    if (r) return r;
+    r = fdmamr_sensors(c, nullptr, nullptr);
+    if (r) return r;
+    return regrid_from_sensors(c, lvl_amr, refine_rot, refine_div, coarse_rot, coarse_div, sensor_mode);
+}
+
+int32_t fdmamr_regrid_sensors(fdmamr_ctx* c, int32_t lvl_amr, double refine_rot, double refine_div,
+                              double coarse_rot, double coarse_div, int32_t sensor_mode, const double* rot,
+                              const double* div) {
+    int r = check_ready(c);
+    if (r) return r;
+    if (!rot || !div) return fail(c, -1, "fdmamr_regrid_sensors: null sensor array");
+    r = pull_state(c);
+    if (r) return r;
+    Tree& t = c->tree;
+    for (size_t i = 0; i < t.ns.size(); i++) {
+        if (!std::isfinite(rot[i]) || !std::isfinite(div[i]))
+            return fail(c, -1, "fdmamr_regrid_sensors: sensor value not finite");
+        t.rot[t.ns[i]] = rot[i];
+        t.div[t.ns[i]] = div[i];
+    }
+    return regrid_from_sensors(c, lvl_amr, refine_rot, refine_div, coarse_rot, coarse_div, sensor_mode);
 }
 
 int32_t fdmamr_stencil_info(fdmamr_ctx* c, int32_t i, int32_t* kind8, int32_t* node8,

@@ -31,7 +31,7 @@ EXPORTS = (
     "fdmamr_get_ghost_state", "fdmamr_leaf_geometry", "fdmamr_max_wavespeed",
     "fdmamr_max_wavespeed_noalloc",
     "fdmamr_advance_rk3", "fdmamr_run_steps", "fdmamr_check_positivity", "fdmamr_sensors",
-    "fdmamr_regrid", "fdmamr_stencil_info", "fdmamr_stencil_values", "fdmamr_stencil_values_fast", "fdmamr_table_info", "fdmamr_leaf_lists", "fdmamr_set_slot_merge", "fdmamr_profile",
+    "fdmamr_regrid", "fdmamr_regrid_sensors", "fdmamr_stencil_info", "fdmamr_stencil_values", "fdmamr_stencil_values_fast", "fdmamr_table_info", "fdmamr_leaf_lists", "fdmamr_set_slot_merge", "fdmamr_profile",
     "fdmamr_profile_read", "fdmamr_selector_set_flags", "fdmamr_selector_get_flags",
     "fdmamr_labels", "fdmamr_selector_init", "fdmamr_selector_set_weights",
     "fdmamr_selector_get_weights", "fdmamr_selector_config", "fdmamr_selector_infer",
@@ -93,6 +93,8 @@ def lib():
         "fdmamr_check_positivity": ([P, ip], C.c_int32),
         "fdmamr_sensors": ([P, dp, dp], C.c_int32),
         "fdmamr_regrid": ([P, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32], C.c_int32),
+        "fdmamr_regrid_sensors": ([P, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32,
+                                   dp, dp], C.c_int32),
         "fdmamr_stencil_info": ([P, C.c_int32, ip, ip, dp, dp, ip, ip, ip], C.c_int32),
         "fdmamr_stencil_values": ([P, C.c_int32, dp], C.c_int32),
         "fdmamr_table_info": ([P, ip], C.c_int32),
@@ -349,6 +351,16 @@ class Solver:
         self._check(self.L.fdmamr_regrid(self.ctx, cfg.lvl_amr, cfg.refine_goal_rot,
                                          cfg.refine_goal_div, cfg.coarse_goal_rot,
                                          cfg.coarse_goal_div, cfg.amr_sensor), "regrid")
+
+    def regrid_with_sensors(self, rot, div):
+        """regrid with the given per-leaf sensors (e.g. another context's on the same mesh)."""
+        cfg = self.cfg
+        rot = np.ascontiguousarray(rot, np.float64)
+        div = np.ascontiguousarray(div, np.float64)
+        assert len(rot) == len(div) == self.n_leaf
+        self._check(self.L.fdmamr_regrid_sensors(self.ctx, cfg.lvl_amr, cfg.refine_goal_rot, cfg.refine_goal_div,
+                                                 cfg.coarse_goal_rot, cfg.coarse_goal_div, cfg.amr_sensor,
+                                                 _dp(rot), _dp(div)), "regrid_sensors")
 
     def sensors(self):
         n = self.n_leaf

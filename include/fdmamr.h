@@ -157,6 +157,13 @@ int32_t fdmamr_sensors(fdmamr_ctx* ctx, double* rot, double* div);
  * ghost fill. sensor_mode 0: rot OR div (reference); 1: density gradient only. */
 int32_t fdmamr_regrid(fdmamr_ctx* ctx, int32_t lvl_amr, double refine_rot, double refine_div,
                       double coarse_rot, double coarse_div, int32_t sensor_mode);
+/* The same AMR body with caller-supplied sensors instead of rot_div's: rot/div per interior
+ * leaf in cell_NS order, already scaled by dy**1.5 as fdmamr_sensors returns them (the
+ * reference's c%rot, c%div read by solute_ref, solution.F90:468-535). A driver that computes
+ * its own sensors, or that replays another run's regrid decisions on the same mesh, uses it. */
+int32_t fdmamr_regrid_sensors(fdmamr_ctx* ctx, int32_t lvl_amr, double refine_rot, double refine_div,
+                              double coarse_rot, double coarse_div, int32_t sensor_mode, const double* rot,
+                              const double* div);
 
 /* ---- diagnostics for parity tests ---- */
 /* stencil table of interior leaf i: kind[8] (0 copy, 1 parent WLS, 2 parent restriction,

@@ -17,6 +17,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* REAL(4) literals widened to REAL(8) by the reference (exact values). */
 static const double WCNS_D1 = 0x1.745d18p-4;   /* 1./11.  nnd_center_RK.F90:331 */
@@ -57,6 +60,10 @@ struct Orc {
     /* every son block ever allocated (freed at orc_free; the reference leaks them) */
     Node** blocks; int nblocks, cap_blocks;
     int n_canon;
+    /* per-leaf residual terms fu*dx*dy of the last stage (cell_NS order), reduced serially in
+     * leaf order after the (OpenMP) leaf loop, so the sums do not depend on the thread count */
+    double (*fu_leaf)[4];
+    int fu_cap;
 };
 
 /* ------------------------------------------------------------------------------------ */
@@ -736,8 +743,10 @@ void orc_flux_derivative(int32_t dir, const double* prim5x5, double d, double la
     }
 }
 
-/* solution_cell + LXL + Res_LXL (solution.F90:302-312, nnd_center_RK.F90:15-64) */
-static void solution_cell(Orc* o, Node* c) {
+/* solution_cell + LXL + Res_LXL (solution.F90:302-312, nnd_center_RK.F90:15-64). The leaf's
+ * residual terms fu*dx*dy (nnd_center_RK.F90:31-36) go to res4; the caller folds them into
+ * Res_max / Res_rms in leaf order. */
+static void solution_cell(const Orc* o, Node* c, double* res4) {
     double v[8][7];
     for (int k = 0; k < 8; k++) stencil(o, c, k, v[k], NULL, NULL, NULL, NULL, NULL);
     double dy = o->h / pow2i(c->lvl);
@@ -758,14 +767,35 @@ static void solution_cell(Orc* o, Node* c) {
     for (int i = 0; i < 4; i++) {
         Ut[i] = U[i] + fu[i] * o->tstep;
         fu[i] = fu[i] * dx * dy;
-        if (fabs(fu[i]) > o->Res_max[i]) o->Res_max[i] = fabs(fu[i]);
-        o->Res_rms[i] = o->Res_rms[i] + fu[i] * fu[i];
+        res4[i] = fu[i];
     }
     c->Ut[0] = Ut[0];
     c->Ut[1] = Ut[1] / Ut[0];
     c->Ut[2] = Ut[2] / Ut[0];
     c->Ut[3] = (Ut[3] - 0.5 * Ut[0] * (c->Ut[1] * c->Ut[1] + c->Ut[2] * c->Ut[2])) / (o->cv * Ut[0]);
     c->Ut[4] = (o->gama - 1.0) * c->Ut[0] * (Ut[3] / c->Ut[0] - 0.5 * (c->Ut[1] * c->Ut[1] + c->Ut[2] * c->Ut[2]));
+}
+
+/* LXL for interior leaves [i0, i1): the OpenMP loop of Time_advance_RK (solution.F90:279-284,
+ * PARALLEL DO SCHEDULE(DYNAMIC)) with the residual reduction (the reference's REDUCTION
+ * clause) done afterwards in leaf order, so results do not depend on the thread count. */
+static void fu_reserve(Orc* o) {
+    if (o->fu_cap < o->nResgrid) {
+        free(o->fu_leaf);
+        o->fu_cap = o->nResgrid + 1;
+        o->fu_leaf = (double(*)[4])malloc(sizeof(double[4]) * (size_t)o->fu_cap);
+    }
+}
+static void leaves_lxl(Orc* o, int i0, int i1) {
+    fu_reserve(o);
+    double(*fu)[4] = o->fu_leaf;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = i0; i < i1; i++) solution_cell(o, o->cell_NS[i], fu[i]);
+    for (int i = i0; i < i1; i++)
+        for (int k = 0; k < 4; k++) {
+            if (fabs(fu[i][k]) > o->Res_max[k]) o->Res_max[k] = fabs(fu[i][k]);
+            o->Res_rms[k] = o->Res_rms[k] + fu[i][k] * fu[i][k];
+        }
 }
 
 /* ------------------------------------------------------------------------------------ */
@@ -1086,7 +1116,7 @@ Orc* orc_new(const OrcParams* p) {
 void orc_free(Orc* o) {
     if (!o) return;
     for (int i = 0; i < o->nblocks; i++) free(o->blocks[i]);
-    free(o->blocks); free(o->cell); free(o->cell_NS); free(o->cell_BC);
+    free(o->blocks); free(o->cell); free(o->cell_NS); free(o->cell_BC); free(o->fu_leaf);
     free(o);
 }
 
@@ -1131,7 +1161,7 @@ static void time_advance_rk(Orc* o) {                               /* solution.
     static const double ra[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0}, rb[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
     for (int krk = 0; krk < 3; krk++) {
         for (int i = 0; i < 6; i++) { o->Res_max[i] = 0.0; o->Res_rms[i] = 0.0; }
-        for (int i = 0; i < o->nResgrid; i++) solution_cell(o, o->cell_NS[i]);
+        leaves_lxl(o, 0, o->nResgrid);
         for (int i = 0; i < o->nResgrid; i++) {                     /* updateRK, solution.F90:386-404 */
             Node* c = o->cell_NS[i];
             for (int j = 0; j < o->Neq + 1; j++) c->U[j] = ra[krk] * c->Un[j] + rb[krk] * c->Ut[j];
@@ -1325,8 +1355,156 @@ int32_t orc_neighbors(const Orc* o, int32_t node, int32_t* out8) {
 int32_t orc_substep(Orc* o, double dt, double lam_x, double lam_y, double* prim5) {
     o->tstep = dt; o->lambda_x = lam_x; o->lambda_y = lam_y;
     for (int i = 0; i < 6; i++) { o->Res_max[i] = 0.0; o->Res_rms[i] = 0.0; }
-    for (int i = 0; i < o->nResgrid; i++) solution_cell(o, o->cell_NS[i]);
+    leaves_lxl(o, 0, o->nResgrid);
     for (int i = 0; i < o->nResgrid; i++)
         for (int k = 0; k < 5; k++) prim5[5 * i + k] = o->cell_NS[i]->Ut[k];
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* test-harness entry points: hand-built trees, whole-tree state, threads, timing        */
+/* ------------------------------------------------------------------------------------ */
+int32_t orc_set_threads(int32_t n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
+static Node* node_by_canon(const Orc* o, int32_t id) {
+    canon_assign((Orc*)o);
+    const Node* c = NULL;
+    for (int i = 0; i < o->total && !c; i++) c = find_canon(&o->cell[i], id);
+    return (Node*)c;
+}
+
+/* Refine_cell (mesh.F90:199-260) on the node with canonical id `node` (sons copy its state).
+ * Canonical ids change afterwards; call orc_rebuild before using the tree. */
+int32_t orc_refine(Orc* o, int32_t node) {
+    Node* c = node_by_canon(o, node);
+    if (!c || c->son1) return -1;
+    refine_cell(o, c);
+    return 0;
+}
+
+/* idneighbor + IDgrid after hand edits; with reinit_ic the initial condition is evaluated
+ * again on every leaf; then the ghosts are filled (initial_boundary) */
+int32_t orc_rebuild(Orc* o, int32_t reinit_ic) {
+    idneighbor(o);
+    if (reinit_ic) {
+        ICData d;
+        ic_prepare(o, &d);
+        for (int i = 0; i < o->total; i++) ic_tree(o, &d, &o->cell[i]);
+    }
+    for (int i = 0; i < o->total; i++) initial_boundary(o, &o->cell[i]);
+    IDgrid(o);
+    return 0;
+}
+
+static void set_node_walk(Node* c, const double* prim7) {
+    for (int i = 0; i < 7; i++) { c->U[i] = prim7[7 * (size_t)c->canon + i]; c->Ut[i] = c->U[i]; c->Un[i] = c->U[i]; }
+    if (c->son1) {
+        set_node_walk(c->son1, prim7); set_node_walk(c->son2, prim7);
+        set_node_walk(c->son3, prim7); set_node_walk(c->son4, prim7);
+    }
+}
+/* U(1:7) of every node, canonical numbering (the ghosts are not refilled) */
+int32_t orc_set_node_values(Orc* o, const double* prim7) {
+    canon_assign(o);
+    for (int i = 0; i < o->total; i++) set_node_walk(&o->cell[i], prim7);
+    return 0;
+}
+
+/* Replace the tree by a flattened one in canonical numbering (as orc_tree_export writes it;
+ * base cells 0..m*n-1 with this oracle's m, n). idneighbor and IDgrid follow; the ghosts keep
+ * the given values. */
+int32_t orc_tree_import(Orc* o, int32_t n_node, const int32_t* father, const int32_t* son,
+                        const int32_t* lvl, const int32_t* sp, const int32_t* num2, const int32_t* cross,
+                        const double* x, const double* y, const double* prim7) {
+    if (n_node < o->total) return -1;
+    for (int i = 0; i < o->nblocks; i++) free(o->blocks[i]);
+    o->nblocks = 0;
+    Node** at = (Node**)calloc((size_t)n_node, sizeof(Node*));
+    for (int k = 0; k < o->total; k++) at[k] = &o->cell[k];
+    for (int k = 0; k < n_node; k++) {
+        if (son[k] < 0) continue;
+        if (son[k] < o->total || son[k] + 3 >= n_node) { free(at); return -1; }
+        Node* b = alloc_sons(o);
+        for (int j = 0; j < 4; j++) at[son[k] + j] = &b[j];
+    }
+    for (int k = 0; k < n_node; k++) {
+        Node* c = at[k];
+        if (!c) { free(at); return -1; }
+        c->num[0] = num2[2 * k]; c->num[1] = num2[2 * k + 1];
+        c->lvl = lvl[k]; c->sp = sp[k]; c->cross = cross[k]; c->sort = 0;
+        c->x = x[k]; c->y = y[k];
+        c->father = father[k] >= 0 ? at[father[k]] : NULL;
+        if (son[k] >= 0) {
+            c->son1 = at[son[k]]; c->son2 = at[son[k] + 1]; c->son3 = at[son[k] + 2]; c->son4 = at[son[k] + 3];
+        } else {
+            c->son1 = c->son2 = c->son3 = c->son4 = NULL;
+        }
+        for (int i = 0; i < 7; i++) { c->U[i] = prim7[7 * (size_t)k + i]; c->Ut[i] = c->U[i]; c->Un[i] = c->U[i]; }
+        c->rot = c->div = 0.0;
+    }
+    free(at);
+    idneighbor(o);
+    IDgrid(o);
+    return 0;
+}
+
+/* Time_advance_RK restricted to interior leaves [i0, i1) (no ghost fill): the CPU baseline's
+ * timed unit on a slice of a large mesh */
+int32_t orc_rk3_range(Orc* o, int32_t i0, int32_t i1, double dt, double lam_x, double lam_y) {
+    if (i0 < 0 || i1 > o->nResgrid || i0 > i1) return -1;
+    o->tstep = dt; o->lambda_x = lam_x; o->lambda_y = lam_y;
+    for (int i = i0; i < i1; i++) memcpy(o->cell_NS[i]->Un, o->cell_NS[i]->U, sizeof(double) * 7);
+    static const double ra[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0}, rb[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
+    for (int krk = 0; krk < 3; krk++) {
+        for (int i = 0; i < 6; i++) { o->Res_max[i] = 0.0; o->Res_rms[i] = 0.0; }
+        leaves_lxl(o, i0, i1);
+        for (int i = i0; i < i1; i++) {
+            Node* c = o->cell_NS[i];
+            for (int j = 0; j < o->Neq + 1; j++) c->U[j] = ra[krk] * c->Un[j] + rb[krk] * c->Ut[j];
+        }
+    }
+    return 0;
+}
+
+/* orc_neighbors for every node: out8[8 * canonical id + k] */
+static void nb_walk(const Node* c, int32_t* out) {
+    const Node* nb[8] = {c->upnei, c->downnei, c->rightnei, c->leftnei,
+                         c->upleftnei, c->uprightnei, c->downleftnei, c->downrightnei};
+    for (int k = 0; k < 8; k++) out[8 * (size_t)c->canon + k] = nb[k] ? nb[k]->canon : -1;
+    if (c->son1) { nb_walk(c->son1, out); nb_walk(c->son2, out); nb_walk(c->son3, out); nb_walk(c->son4, out); }
+}
+int32_t orc_neighbors_all(const Orc* o, int32_t* out8) {
+    canon_assign((Orc*)o);
+    for (int i = 0; i < o->total; i++) nb_walk(&o->cell[i], out8);
+    return 0;
+}
+
+/* orc_stencil_info and orc_stencil_value for every interior leaf and position: arrays of
+ * N*8 entries (nb: N*8*9, var: N*8*7; either may be NULL) */
+int32_t orc_stencil_all(const Orc* o, int32_t* kind, int32_t* node, double* px, double* py, int32_t* m1,
+                        int32_t* m2, int32_t* nb, double* var) {
+    canon_assign((Orc*)o);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < o->nResgrid; i++)
+        for (int k = 0; k < 8; k++) {
+            const size_t d = (size_t)i * 8 + k;
+            const Node* t; NList l1, l2;
+            kind[d] = stencil(o, o->cell_NS[i], k, var ? var + 7 * d : NULL, &t, px + d, py + d, &l1, &l2);
+            node[d] = t->canon;
+            m1[d] = l1.n; m2[d] = l2.n;
+            if (nb) {
+                for (int j = 0; j < 9; j++) nb[9 * d + j] = -1;
+                for (int j = 0; j < l1.n && j < 9; j++) nb[9 * d + j] = l1.a[j]->canon;
+                for (int j = 0; j < l2.n && l1.n + j < 9; j++) nb[9 * d + l1.n + j] = l2.a[j]->canon;
+            }
+        }
     return 0;
 }

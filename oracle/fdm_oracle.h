@@ -127,6 +127,30 @@ int32_t orc_neighbors(const Orc* o, int32_t node, int32_t* out8);
 
 uint64_t orc_splitmix64(uint64_t* state);
 
+/* ---- test harness ---- */
+/* OpenMP threads of the leaf loop (n <= 0 keeps the current count); returns the count */
+int32_t orc_set_threads(int32_t n);
+/* Refine_cell (mesh.F90:199-260) on canonical node `node`; then orc_rebuild */
+int32_t orc_refine(Orc* o, int32_t node);
+/* idneighbor + IDgrid after hand edits (reinit_ic: evaluate the initial condition again on
+ * every leaf), then the ghost fill */
+int32_t orc_rebuild(Orc* o, int32_t reinit_ic);
+/* U(1:7) of every node in canonical numbering; ghosts are not refilled */
+int32_t orc_set_node_values(Orc* o, const double* prim7);
+/* replace the tree by a flattened one in canonical numbering (base cells 0..m*n-1 of this
+ * oracle's m, n); idneighbor + IDgrid follow, ghosts keep the given values */
+int32_t orc_tree_import(Orc* o, int32_t n_node, const int32_t* father, const int32_t* son,
+                        const int32_t* lvl, const int32_t* sp, const int32_t* num2, const int32_t* cross,
+                        const double* x, const double* y, const double* prim7);
+/* orc_neighbors of every node: out8[8 * canonical id + k] */
+int32_t orc_neighbors_all(const Orc* o, int32_t* out8);
+/* orc_stencil_info / orc_stencil_value of every interior leaf and position (N*8 entries;
+ * nb N*8*9, var N*8*7, either may be NULL) */
+int32_t orc_stencil_all(const Orc* o, int32_t* kind, int32_t* node, double* px, double* py, int32_t* m1,
+                        int32_t* m2, int32_t* nb, double* var);
+/* Time_advance_RK on interior leaves [i0, i1) only, no ghost fill (CPU-baseline timing) */
+int32_t orc_rk3_range(Orc* o, int32_t i0, int32_t i1, double dt, double lam_x, double lam_y);
+
 #ifdef __cplusplus
 }
 #endif

@@ -98,6 +98,22 @@ def lib():
         L.orc_neighbors.restype = C.c_int32
         L.orc_substep.argtypes = [P, C.c_double, C.c_double, C.c_double, dp]
         L.orc_substep.restype = C.c_int32
+        L.orc_set_threads.argtypes = [C.c_int32]
+        L.orc_set_threads.restype = C.c_int32
+        L.orc_refine.argtypes = [P, C.c_int32]
+        L.orc_refine.restype = C.c_int32
+        L.orc_rebuild.argtypes = [P, C.c_int32]
+        L.orc_rebuild.restype = C.c_int32
+        L.orc_set_node_values.argtypes = [P, dp]
+        L.orc_set_node_values.restype = C.c_int32
+        L.orc_tree_import.argtypes = [P, C.c_int32, ip, ip, ip, ip, ip, ip, dp, dp, dp]
+        L.orc_tree_import.restype = C.c_int32
+        L.orc_rk3_range.argtypes = [P, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double]
+        L.orc_rk3_range.restype = C.c_int32
+        L.orc_neighbors_all.argtypes = [P, ip]
+        L.orc_neighbors_all.restype = C.c_int32
+        L.orc_stencil_all.argtypes = [P, ip, ip, dp, dp, ip, ip, ip, C.c_void_p]
+        L.orc_stencil_all.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -227,6 +243,43 @@ class Oracle:
         self.L.orc_neighbors(self.h, int(node), out)
         return out
 
+    def neighbors_all(self):
+        out = np.zeros((self.L.orc_n_node(self.h), 8), np.int32)
+        self.L.orc_neighbors_all(self.h, out.reshape(-1))
+        return out
+
+    def stencil_all(self, values=False):
+        """Classification (and values) of every interior leaf's 8 stencil entries."""
+        N = self.n_leaf
+        r = dict(kind=np.zeros((N, 8), np.int32), node=np.zeros((N, 8), np.int32), px=np.zeros((N, 8)),
+                 py=np.zeros((N, 8)), m1=np.zeros((N, 8), np.int32), m2=np.zeros((N, 8), np.int32),
+                 nb=np.zeros((N, 8, 9), np.int32))
+        var = np.zeros((N, 8, 7)) if values else None
+        self.L.orc_stencil_all(self.h, r["kind"].reshape(-1), r["node"].reshape(-1), r["px"].reshape(-1),
+                               r["py"].reshape(-1), r["m1"].reshape(-1), r["m2"].reshape(-1),
+                               r["nb"].reshape(-1), var.ctypes.data if values else None)
+        if values:
+            r["var"] = var
+        return r
+
+    def refine(self, node):
+        assert self.L.orc_refine(self.h, int(node)) == 0, node
+
+    def rebuild(self, reinit_ic=True):
+        self.L.orc_rebuild(self.h, 1 if reinit_ic else 0)
+
+    def set_node_values(self, prim7):
+        self.L.orc_set_node_values(self.h, np.ascontiguousarray(prim7, np.float64).reshape(-1))
+
+    def import_tree(self, t):
+        rc = self.L.orc_tree_import(self.h, len(t["father"]), *[np.ascontiguousarray(t[k], np.int32).reshape(-1)
+                                                                 for k in ("father", "son", "lvl", "sp", "num", "cross")],
+                                    *[np.ascontiguousarray(t[k], np.float64).reshape(-1) for k in ("x", "y", "prim7")])
+        assert rc == 0
+
+    def rk3_range(self, i0, i1, dt, lam_x, lam_y):
+        assert self.L.orc_rk3_range(self.h, i0, i1, dt, lam_x, lam_y) == 0
+
     def sensors(self):
         N = self.n_leaf
         rot = np.zeros(N); div = np.zeros(N); s = np.zeros(2)
@@ -280,6 +333,11 @@ def wls_weno(pts, m1, xx, yy, cx, cy, lvl, h, x_factor=1.0):
 
 def smoothness(order, coeff, cx, cy, lvl, h, x_factor=1.0):
     return lib().orc_smoothness(order, np.ascontiguousarray(coeff, float), cx, cy, lvl, h, x_factor)
+
+
+def set_threads(n: int) -> int:
+    """OpenMP threads of the oracle's leaf loops (n <= 0: query)."""
+    return lib().orc_set_threads(n)
 
 
 def splitmix64(state: int, count: int):

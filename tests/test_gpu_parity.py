@@ -59,14 +59,15 @@ def test_stencil_values_at_level_jumps():
 
 
 def test_c1_trajectory_with_amr():
-    # vortex, base 64x64, lvlAMR=2, CFL 0.5, AMR every 30 steps: 40 steps cross two regrids
+    # c1: vortex, base 64x64, lvlAMR=2, CFL 0.5, AMR every 30 steps; its full 100 steps cross
+    # four regrids (it = 1, 31, 61, 91)
     cfg = F.config_c1()
     o = oracle_for(cfg)
     s = F.Solver(cfg).setup()
-    for it in range(40):
+    for it in range(100):
         assert o.step() == 0
         s.step()
-        if it % 10 == 9 or it in (0, 1, 31):
+        if it % 10 == 9 or it in (0, 1, 31, 61, 91):
             L = o.leaves()
             g = s.geometry()
             assert len(g["node"]) == len(L["node"]), f"leaf count differs at it={it+1}"
@@ -115,3 +116,49 @@ def test_fp32_flux_close_to_fp64():
     print("L1(rho) fp32 vs fp64", l1)
     assert l1 < 1e-4
     assert b.check_positivity() == 0
+
+
+def _trajectory(cfg, steps, check_every):
+    o = oracle_for(cfg)
+    s = F.Solver(cfg).setup()
+    L = o.leaves(); g = s.geometry()
+    assert (g["node"] == L["node"]).all()
+    assert rel_err(s.get_state(), L["prim"]) <= TOL
+    for it in range(steps):
+        assert o.step() == 0
+        s.step()
+        if it % check_every == check_every - 1 or it == steps - 1:
+            L = o.leaves(); g = s.geometry()
+            assert (g["node"] == L["node"]).all(), f"mesh differs at it={it + 1}"
+            assert rel_err(s.get_state(), L["prim"]) <= TOL, f"state differs at it={it + 1}"
+            assert rel_err(s.get_ghost_state(), o.bc_leaves()["prim"]) <= TOL, f"ghosts differ at it={it + 1}"
+    return o, s
+
+
+def test_periodic_mesh_with_level2_ghosts_against_oracle():
+    # periodic density wave refined to level 3, whose refinement reaches the periodic ghosts
+    # (level 2 there, filled through the reversed son path of BC_cyc, solution.F90:142-159):
+    # FP64 steps and regrids equal the oracle's, ghosts included
+    cfg = F.Config(xmin=0.0, xmax=2.0, ymin=0.0, ymax=2.0, h=2 / 24, ic_kind=F.IC_DENSITY_WAVE, localdt=1,
+                   cfl=0.5, amr=1, lvl_amr=3, pstep_adapt=2, amr_sensor=1, wls_local=1, refine_goal_rot=1.0,
+                   refine_goal_div=1.0)
+    o, s = _trajectory(cfg, 9, 3)
+    t = s.tree()
+    ghost_leaf = (t["son"] < 0) & (t["cross"] < 0)
+    assert (t["lvl"][ghost_leaf] >= 2).sum() > 50
+
+
+def test_shock_bubble_inflow_against_oracle():
+    # small shock-bubble case: Mach 2 inflow boundary (new kind) on the left, transmissive
+    # elsewhere, one seeded bubble, level 2 adaptation every 5 steps
+    g, M = 1.4, 2.0
+    p2 = 1 + 2 * g / (g + 1) * (M * M - 1)
+    r2 = (g + 1) * M * M / ((g - 1) * M * M + 2)
+    u2 = M * np.sqrt(g) * (1 - 1 / r2)
+    cfg = F.Config(xmin=0.0, xmax=2.0, ymin=0.0, ymax=1.0, h=1 / 32, gamma=g,
+                   bc=(F.BC_INFLOW, F.BC_TRANSMISSIVE, F.BC_TRANSMISSIVE, F.BC_TRANSMISSIVE),
+                   inflow=(r2, u2, 0.0, p2), ic_kind=F.IC_SHOCK_BUBBLES, seed=3,
+                   ic_param=(0.4, M, 1, 0.2, 0.2, 0.1, 5.0), localdt=1, cfl=0.5, t_end=1e9, amr=1,
+                   lvl_amr=2, pstep_adapt=5, amr_sensor=1, pre_adapt=2, wls_local=1)
+    o, s = _trajectory(cfg, 12, 4)
+    assert s.counts()["l_max"] == 2

@@ -158,3 +158,41 @@ def test_tree_round_trip_and_canonical_ids():
     m, n = o.dims()
     s.import_tree(m, n, t)
     _compare_trees(s.tree(), t)
+
+
+def _product_tables_equal_oracle(o, cfg, limit=None):
+    s = host(cfg)
+    m, n = o.dims()
+    s.import_tree(m, n, o.tree())
+    s.build_tables()
+    assert s.counts()["n_leaf"] == o.n_leaf
+    assert np.array_equal(s.geometry()["node"], o.leaves()["node"])
+    return _compare_stencils(s, o, limit=limit)
+
+
+def test_stencil_tables_periodic_mesh_refined_at_the_boundary():
+    # the periodic density-wave mesh whose level-2 ghosts take the reversed son path
+    # (pinned in test_oracle_stencil_pins.py); the product's tables equal the oracle's there
+    from test_oracle_stencil_pins import periodic_wave_kwargs
+    kw = periodic_wave_kwargs()
+    o = O.Oracle(**kw)
+    for _ in range(7):
+        assert o.step() == 0
+    cfg = F.Config(xmin=0.0, xmax=2.0, ymin=0.0, ymax=2.0, h=2 / 24, wls_local=1)
+    kinds = _product_tables_equal_oracle(o, cfg, limit=3000)
+    assert kinds[4] > 0 and kinds[3] > 0 and kinds[1] > 0
+
+
+@pytest.mark.parametrize("refine", [
+    [(4.5, 4.5, 0), (4.25, 4.75, 1)],                                       # child m = 7
+    [(4.5, 4.5, 0), (4.25, 4.75, 1), (4.5, 3.5, 0)],                        # child m = 8
+    [(0.5, 4.5, 0), (-0.5, 5.5, 0), (-0.25, 5.25, 1)],                      # child m = 6 (copy)
+    [(4.5, 4.5, 0), (4.25, 4.25, 1), (4.125, 4.125, 2)],                    # parent m = 6 (interpA)
+    [(4.5, 4.5, 0), (4.25, 4.25, 1), (4.125, 4.125, 2), (4.5, 3.5, 0)],     # parent m = 7
+])
+def test_stencil_tables_hand_built_trees(refine):
+    # the hand-built trees whose stencil lists were derived by hand (test_oracle_stencil_pins.py)
+    from test_oracle_stencil_pins import _hand_tree
+    o = _hand_tree(refine)
+    kinds = _product_tables_equal_oracle(o, F.Config(h=1.0))
+    assert kinds[1] + kinds[2] + kinds[3] + kinds[4] > 0
