@@ -613,7 +613,7 @@ int32_t fdmamr_set_state(fdmamr_ctx* c, const double* prim5) {
     std::vector<double4> q(N);
     std::vector<double> tv(N);
     for (int i = 0; i < N; i++) {
-        const double* u = prim5 + 5 * (size_t)i;
+        const double* u = prim5 + 5 * (size_t)c->tb.ns_pos[i];  // device order from cell_NS order
         q[i] = make_double4(u[0], u[1], u[2], u[4]);
         tv[i] = u[3];
     }
@@ -627,6 +627,7 @@ int32_t fdmamr_set_state(fdmamr_ctx* c, const double* prim5) {
     return 0;
 }
 
+// state [off, off + n) of buffer 0 to host AoS (interior leaves: from device to cell_NS order)
 static int get_range(fdmamr_ctx* c, int off, int n, double* prim5) {
     std::vector<double4> q(n);
     std::vector<double> tv(n);
@@ -637,8 +638,9 @@ static int get_range(fdmamr_ctx* c, int off, int n, double* prim5) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const bool fast = c->p.flux_precision == FDMAMR_FLUX_FP32;
     const double gm = c->tree.gamma * c->tree.Ma * c->tree.Ma;
+    const bool interior = off == 0;
     for (int i = 0; i < n; i++) {
-        double* u = prim5 + 5 * (size_t)i;
+        double* u = prim5 + 5 * (size_t)(interior ? c->tb.ns_pos[i] : i);
         u[0] = q[i].x; u[1] = q[i].y; u[2] = q[i].z; u[4] = q[i].w;
         u[3] = fast ? q[i].w * gm / q[i].x : tv[i];
     }
@@ -796,13 +798,14 @@ int32_t fdmamr_sensors(fdmamr_ctx* c, double* rot, double* div) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     Tree& t = c->tree;
     for (int i = 0; i < N; i++) {
-        int k = t.ns[i];
+        const int k = c->tb.leaf_node[i];
         double dy = t.h / (double)(1 << t.lvl[k]);
         double f = std::pow(dy, 1.5);                 // dy**1.5, solution.F90:565,578
         t.rot[k] = a[i] * f;
         t.div[k] = b[i] * f;
-        if (rot) rot[i] = t.rot[k];
-        if (div) div[i] = t.div[k];
+        const int e = c->tb.ns_pos[i];                // outputs in cell_NS order
+        if (rot) rot[e] = t.rot[k];
+        if (div) div[e] = t.div[k];
     }
     return 0;
 }
@@ -867,8 +870,9 @@ int32_t fdmamr_stencil_info(fdmamr_ctx* c, int32_t i, int32_t* kind8, int32_t* n
     if (!c || !c->tables_ready || i < 0 || i >= c->tb.N) return fail(c, -1, "fdmamr_stencil_info: bad leaf");
     std::vector<int32_t> id = c->tree.canonical_ids(nullptr);
     const Tables& tb = c->tb;
+    const int dev = tb.dev_of_ns[i];
     for (int k = 0; k < 8; k++) {
-        size_t d = (size_t)i * 8 + k;
+        size_t d = (size_t)dev * 8 + k;
         if (kind8) kind8[k] = tb.st_kind[d];
         if (node8) node8[k] = id[tb.st_node[d]];
         if (px8) px8[k] = tb.st_px[d];
@@ -887,8 +891,11 @@ int32_t fdmamr_selector_set_flags(fdmamr_ctx* c, const uint8_t* flags, int32_t u
     int r = check_ready(c);
     if (r) return r;
     if (c->p.flux_precision != FDMAMR_FLUX_FP32) return fail(c, -1, "selector needs the FP32 flux mode");
-    if (flags && c->tb.N > 0)
-        HIPCHK(c, hipMemcpyAsync(c->flags.p, flags, c->tb.N, hipMemcpyHostToDevice, c->stream));
+    std::vector<uint8_t> fd(c->tb.N);
+    if (flags && c->tb.N > 0) {
+        for (int i = 0; i < c->tb.N; i++) fd[i] = flags[c->tb.ns_pos[i]];
+        HIPCHK(c, hipMemcpyAsync(c->flags.p, fd.data(), c->tb.N, hipMemcpyHostToDevice, c->stream));
+    }
     launch_build_lists(c->tb.N, c->flags.p, c->list_cnt.p, c->lists.p, c->list_meta.p, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -900,9 +907,11 @@ int32_t fdmamr_selector_get_flags(fdmamr_ctx* c, uint8_t* flags) {
     int r = check_ready(c);
     if (r) return r;
     if (!flags) return fail(c, -1, "null flags");
+    std::vector<uint8_t> fd(c->tb.N);
     if (c->tb.N > 0)
-        HIPCHK(c, hipMemcpyAsync(flags, c->flags.p, c->tb.N, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(fd.data(), c->flags.p, c->tb.N, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < c->tb.N; i++) flags[c->tb.ns_pos[i]] = fd[i];
     return 0;
 }
 
@@ -918,6 +927,14 @@ int32_t fdmamr_leaf_lists(fdmamr_ctx* c, int32_t* lists, int32_t* n_clear) {
         *n_clear = 0;
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    // device leaf indices -> cell_NS positions, each list ascending
+    const int nc = std::min(std::max(*n_clear, 0), c->tb.N);
+    for (int k = 0; k < c->tb.N; k++) {
+        if (lists[k] < 0 || lists[k] >= c->tb.N) return fail(c, -2, "fdmamr_leaf_lists: corrupt list entry");
+        lists[k] = c->tb.ns_pos[lists[k]];
+    }
+    std::sort(lists, lists + nc);
+    std::sort(lists + nc, lists + c->tb.N);
     return 0;
 }
 
@@ -936,8 +953,10 @@ int32_t fdmamr_labels(fdmamr_ctx* c, float* dev) {
     launch_stage_fast(c->dt, nullptr, nullptr, c->Q[0].p, c->Qf[0].p, c->Q[0].p, c->Q[2].p, c->Qf[2].p,
                       c->sc, 0.0, 1.0, 0, c->partials.p, nullptr, c->label.p, JudgeArgs{}, nullptr, c->stream);
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipMemcpyAsync(dev, c->label.p, c->tb.N * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    std::vector<float> lab(c->tb.N);
+    HIPCHK(c, hipMemcpyAsync(lab.data(), c->label.p, c->tb.N * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < c->tb.N; i++) dev[c->tb.ns_pos[i]] = lab[i];
     return 0;
 }
 
@@ -1030,10 +1049,13 @@ int32_t fdmamr_selector_infer(fdmamr_ctx* c, float* prob, int32_t* n_troubled) {
     launch_judge(N, c->label.p, c->sel_bound, c->flags.p, c->sel_cnt.p, c->stream);
     launch_build_lists(N, c->flags.p, c->list_cnt.p, c->lists.p, c->list_meta.p, c->stream);
     HIPCHK(c, hipGetLastError());
-    if (prob && N > 0) HIPCHK(c, hipMemcpyAsync(prob, c->sel_prob.p, N * 4, hipMemcpyDeviceToHost, c->stream));
+    std::vector<float> pd(N);
+    if (prob && N > 0) HIPCHK(c, hipMemcpyAsync(pd.data(), c->sel_prob.p, N * 4, hipMemcpyDeviceToHost, c->stream));
     int cnt[CNT_N];
     HIPCHK(c, read_counters(c, cnt));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (prob)
+        for (int i = 0; i < N; i++) prob[c->tb.ns_pos[i]] = pd[i];
     if (n_troubled) *n_troubled = cnt[CNT_TROUBLED];
     c->use_flags = true;
     return 0;
@@ -1050,11 +1072,14 @@ int32_t fdmamr_selector_features(fdmamr_ctx* c, uint16_t* x) {
     launch_slots_fast(c->dt, c->wls_f.p, c->Q[0].p, c->Qf[0].p, c->stream);
     launch_features(c->dt, c->Qf[0].p, c->off16.p, tmp.p, 0, c->stream);
     hipError_t e = hipGetLastError();
+    std::vector<uint16_t> rows((size_t)N * MLP_IN);
     if (e == hipSuccess && N > 0)
-        e = hipMemcpyAsync(x, tmp.p, (size_t)N * MLP_IN * 2, hipMemcpyDeviceToHost, c->stream);
+        e = hipMemcpyAsync(rows.data(), tmp.p, (size_t)N * MLP_IN * 2, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     release(tmp);
     if (e != hipSuccess) return fail(c, -2, hipGetErrorString(e));
+    for (int i = 0; i < N; i++)
+        std::memcpy(x + (size_t)c->tb.ns_pos[i] * MLP_IN, rows.data() + (size_t)i * MLP_IN, MLP_IN * 2);
     return 0;
 }
 
@@ -1227,7 +1252,7 @@ int32_t fdmamr_stencil_values_fast(fdmamr_ctx* c, int32_t i, float* val8x4) {
                           c->fill_beg.p, c->fill_leaf.p, c->fill_w.p, c->stream);
     else
         launch_slots_fast(c->dt, c->wls_f.p, c->Q[0].p, c->Qf[0].p, c->stream);
-    launch_stencil_values_f(c->dt, c->Qf[0].p, i, reinterpret_cast<float*>(c->tmp.p), c->stream);
+    launch_stencil_values_f(c->dt, c->Qf[0].p, c->tb.dev_of_ns[i], reinterpret_cast<float*>(c->tmp.p), c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(val8x4, c->tmp.p, 32 * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1266,7 +1291,7 @@ int32_t fdmamr_stencil_values(fdmamr_ctx* c, int32_t i, double* val8x4) {
     if (r) return r;
     if (i < 0 || i >= c->tb.N || !val8x4) return fail(c, -1, "fdmamr_stencil_values: bad leaf");
     launch_slots(c->dt, c->Q[0].p, c->T[0].p, c->stream);
-    launch_stencil_values(c->dt, c->Q[0].p, i, c->tmp.p, c->stream);
+    launch_stencil_values(c->dt, c->Q[0].p, c->tb.dev_of_ns[i], c->tmp.p, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(val8x4, c->tmp.p, 32 * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -92,7 +92,7 @@ struct DevTables {
     int S_wp;                   // WLS slots [0, S_wp) read no ghost leaf
     double gamma, cv, h, xf, Ma;
     double inflow[4];
-    const int32_t* stencil;     // [8][N]
+    const int32_t* stencil;     // [N][8]: one 32-byte record per leaf (+1x, -1x, +1y, -1y, +2x, -2x, +2y, -2y)
     const int8_t* lvl;          // [N]
     const double* wls_d;        // [S_w][WLS_STRIDE]
     const int32_t* wls_i;       // [S_w][WLSI_STRIDE]
@@ -112,6 +112,14 @@ struct DevTables {
     const int32_t* slot_uncov;    // [S_unc] the slots whose record is not in their block's list
     int S_unc;
 };
+
+// the eight stencil ids of interior leaf i (two 16-byte loads of its record)
+__device__ __forceinline__ void load_stencil(const DevTables& t, int i, int s[8]) {
+    const int4* r = reinterpret_cast<const int4*>(t.stencil + (size_t)i * 8);
+    const int4 a = r[0], b = r[1];
+    s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w;
+    s[4] = b.x; s[5] = b.y; s[6] = b.z; s[7] = b.w;
+}
 
 // Scalars of the device-resident step loop (solution.F90:183-215).
 struct DevScalars {

@@ -22,9 +22,6 @@ namespace fdm {
 
 #define FD1 (float)0x1.745d18p-4   // REAL(4) 1./11.
 #define FD2 (float)0x1.d1745ep-1   // REAL(4) 10./11.
-// linear-weight reconstruction (-u1 + 6 u2 + 3 u3) / 8 scaled by 1/(d1+d2), as the reference's
-// weights give when tau3 = 0
-#define LIN_C (0.125f / ((float)0x1.745d18p-4 + (float)0x1.d1745ep-1))
 #define W1_LIN ((float)0x1.745d18p-4 / ((float)0x1.745d18p-4 + (float)0x1.d1745ep-1))
 // second WCNS_Z3 candidate as one weighted sum: (-u1 / 8 + (3/4 - d1) u2 + 3/8 u3) / d2
 #define P2_C1 (-0.125f / FD2)
@@ -152,7 +149,17 @@ __device__ __forceinline__ float quad_lane(float v) {
 }
 constexpr int QP_XOR1 = 0xB1, QP_XOR2 = 0x4E, QP_XOR3 = 0x1B;  // quad_perm [1,0,3,2], [2,3,0,1], [3,2,1,0]
 
-// linear-weight WCNS3 flux derivative (cleared leaves): no projection, no weights
+// Linear-weight WCNS3 flux derivative (cleared leaves): no projection, no weights. With the
+// linear weights every face value is a fixed combination L(a, b, c) = (-a + 6 b + 3 c) / (8 s)
+// (s = d1 + d2, the REAL(4) weights' sum), so the whole derivative (nnd_center_RK.F90:95-111)
+// collapses to one 5-point stencil per component: with F+- the Lax-Friedrichs face fluxes,
+//   (4/3)(F+ - F-) - (1/6)(F_{i+1} - F_{i-1})
+//     = cA (F_{i-2} - F_{i+2}) + cB (F_{i+1} - F_{i-1})
+//       + lam cA (U_{i-2} + U_{i+2} - 4 (U_{i-1} + U_{i+1}) + 6 U_i),
+// cA = 1 / (12 s), cB = 10 / (12 s) - 1/6 (for s = 1 a fourth-order central difference plus a
+// fourth-difference dissipation); divided by d.
+#define LIN_CA (1.f / (12.f * ((float)0x1.745d18p-4 + (float)0x1.d1745ep-1)))
+#define LIN_CB (10.f / (12.f * ((float)0x1.745d18p-4 + (float)0x1.d1745ep-1)) - 1.f / 6.f)
 template <int DIR>
 __device__ __forceinline__ void flux_lin_f(const float4* q, float d, float lam, float gama, float* out) {
     PtF P[5];
@@ -160,19 +167,12 @@ __device__ __forceinline__ void flux_lin_f(const float4* q, float d, float lam, 
 #pragma unroll
     for (int j = 0; j < 5; j++) cons_flux_f<DIR>(q[j], gm1inv, P[j]);
     const float id = 1.f / d;
+    const float ld = lam * LIN_CA;
 #pragma unroll
     for (int n = 0; n < 4; n++) {
-        float fx[2];
-#pragma unroll
-        for (int f = 0; f < 2; f++) {
-            const int a = f, b = f + 1, c = f + 2, e = f + 3;
-            const float wL = LIN_C * (-P[a].U[n] + 6.f * P[b].U[n] + 3.f * P[c].U[n]);
-            const float wR = LIN_C * (-P[e].U[n] + 6.f * P[c].U[n] + 3.f * P[b].U[n]);
-            const float gL = LIN_C * (-P[a].F[n] + 6.f * P[b].F[n] + 3.f * P[c].F[n]);
-            const float gR = LIN_C * (-P[e].F[n] + 6.f * P[c].F[n] + 3.f * P[b].F[n]);
-            fx[f] = 0.5f * (gL + gR - lam * (wR - wL));
-        }
-        out[n] = (4.f / 3.f) * id * (fx[1] - fx[0]) - (1.f / 6.f) * id * (P[3].F[n] - P[1].F[n]);
+        const float f = LIN_CA * (P[0].F[n] - P[4].F[n]) + LIN_CB * (P[3].F[n] - P[1].F[n]);
+        const float u = (P[0].U[n] + P[4].U[n]) - 4.f * (P[1].U[n] + P[3].U[n]) + 6.f * P[2].U[n];
+        out[n] = id * (f + ld * u);
     }
 }
 
@@ -232,8 +232,7 @@ __device__ __forceinline__ void stage_leaf_lin(const DevTables& t, int i, const 
                                                int* posflag) {
     const double gama = t.gamma;
     int s[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) s[k] = t.stencil[(size_t)k * t.N + i];
+    load_stencil(t, i, s);
     const double4 qc = Qin[i];
     // the leaf's own FP32 value is the rounding of its FP64 state (the shadow holds exactly that)
     const float4 qcf = make_float4((float)qc.x, (float)qc.y, (float)qc.z, (float)qc.w);
@@ -315,9 +314,11 @@ __device__ __forceinline__ void full_block(int blk, const DevTables& t, const in
     const int row0 = dir ? (side ? 3 : 7) : (side ? 1 : 5);
     const int row1 = dir ? (side ? 2 : 3) : (side ? 0 : 1);
     const int row2 = dir ? (side ? 6 : 2) : (side ? 4 : 0);
-    const int s0 = t.stencil[(size_t)row0 * t.N + i];
-    const int s1 = t.stencil[(size_t)row1 * t.N + i];
-    const int s2 = t.stencil[(size_t)row2 * t.N + i];
+    // the four lanes of a leaf read from the leaf's one 32-byte stencil record
+    const int32_t* rec = t.stencil + (size_t)i * 8;
+    const int s0 = rec[row0];
+    const int s1 = rec[row1];
+    const int s2 = rec[row2];
     const int lv = t.lvl[i];
     const float4 qa = Qf_in[s0], qb = Qf_in[s1], qc = Qf_in[i], qd = Qf_in[s2];
     float4 col[4] = {qa, side ? qc : qb, side ? qb : qc, qd};
@@ -775,7 +776,7 @@ __global__ __launch_bounds__(128) void k_slots_fill(
 __global__ void k_stencil_values_f(DevTables t, const float4* __restrict__ Qf, int i, float* out) {
     const int k = threadIdx.x;
     if (k < 8 && i < t.N) {
-        const float4 q = Qf[t.stencil[(size_t)k * t.N + i]];
+        const float4 q = Qf[t.stencil[(size_t)i * 8 + k]];
         out[4 * k + 0] = q.x; out[4 * k + 1] = q.y; out[4 * k + 2] = q.z; out[4 * k + 3] = q.w;
     }
 }

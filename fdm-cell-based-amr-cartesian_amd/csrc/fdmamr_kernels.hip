@@ -14,7 +14,7 @@
 // Layout in HBM: state as AoS double4 (rho,u,v,p) per leaf -- one 32-byte gather per stencil
 // entry -- plus a separate double array for the passive temperature T. Leaves are in the
 // reference's cell_NS order (Morton inside each base cell) followed by ghost and corner
-// leaves, then the slot values. Stencil ids are int32 [8][N] (coalesced per position).
+// leaves, then the slot values. Stencil ids are one 32-byte int32 record per leaf ([N][8]).
 //
 // FP64 kernels are compiled with -ffp-contract=off and follow the reference's evaluation
 // order, so they reproduce the CPU restatement bit for bit.
@@ -218,8 +218,7 @@ __global__ __launch_bounds__(256) void k_stage(DevTables t, const double4* __res
         const double gama = t.gamma, cv = t.cv;
         const double dt = sc->dt;
         int s[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++) s[k] = t.stencil[(size_t)k * t.N + i];
+        load_stencil(t, i, s);
         const double4 qc = Qin[i];
         const int lv = t.lvl[i];
         const double dy = t.h / (double)(1 << lv);
@@ -613,8 +612,9 @@ __global__ __launch_bounds__(256) void k_sensors(DevTables t, const double4* __r
     const int lv = t.lvl[i];
     const double dy = t.h / (double)(1 << lv);
     const double dx = t.xf * dy;
-    const double4 ip = Q[t.stencil[i]], im = Q[t.stencil[(size_t)t.N + i]];
-    const double4 jp = Q[t.stencil[(size_t)2 * t.N + i]], jm = Q[t.stencil[(size_t)3 * t.N + i]];
+    const int32_t* rec = t.stencil + (size_t)i * 8;
+    const double4 ip = Q[rec[0]], im = Q[rec[1]];
+    const double4 jp = Q[rec[2]], jm = Q[rec[3]];
     const double vx = (ip.z - im.z) / (2.0 * dx), uy = (jp.y - jm.y) / (2.0 * dy);
     rot[i] = fabs(vx - uy);
     const double ux = (ip.x - im.x) / (2.0 * dx), vy = (jp.x - jm.x) / (2.0 * dy);
@@ -634,7 +634,7 @@ __global__ __launch_bounds__(256) void k_positivity(DevTables t, const double4* 
 __global__ void k_stencil_values(DevTables t, const double4* __restrict__ Q, int i, double* out) {
     const int k = threadIdx.x;
     if (k < 8 && i < t.N) {
-        const double4 q = Q[t.stencil[(size_t)k * t.N + i]];
+        const double4 q = Q[t.stencil[(size_t)i * 8 + k]];
         out[4 * k + 0] = q.x; out[4 * k + 1] = q.y; out[4 * k + 2] = q.z; out[4 * k + 3] = q.w;
     }
 }

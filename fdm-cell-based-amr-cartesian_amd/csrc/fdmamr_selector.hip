@@ -95,8 +95,7 @@ __device__ __forceinline__ void gather_idx(const DevTables& t, const int32_t* __
                   r3 = point_row(4 * k + 3);
         const int row = g == 0 ? r0 : g == 1 ? r1 : g == 2 ? r2 : r3;
         const int rr = row < 24 ? row : 0;
-        const int32_t* arr = rr < 8 ? t.stencil + (size_t)rr * t.N : off16 + (size_t)(rr - 8) * t.N;
-        const int raw = arr[lc];
+        const int raw = rr < 8 ? t.stencil[(size_t)lc * 8 + rr] : off16[(size_t)(rr - 8) * t.N + lc];
         idx[k] = row < 24 ? raw : lc;
     }
 }

@@ -235,7 +235,43 @@ void build_tables(const Tree& t, Tables& tb) {
     tb.G = (int)t.bcl.size();
     tb.state_of_node.assign(NA, -1);
     tb.leaf_node.clear();
-    for (int i = 0; i < tb.N; i++) { tb.state_of_node[t.ns[i]] = i; tb.leaf_node.push_back(t.ns[i]); }
+    // Device order of the interior leaves: the base cells along a Morton (Z-order) curve over
+    // the base grid, the leaves of each base cell in the reference's cell_NS order (son1, son2,
+    // son4, son3: Morton again), so leaves that are neighbours in either direction are close in
+    // memory. ns_pos / dev_of_ns translate to and from cell_NS order at the C-ABI.
+    {
+        auto spread = [](uint32_t v) {
+            uint64_t x = v;
+            x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
+            x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
+            x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
+            x = (x | (x << 2)) & 0x3333333333333333ull;
+            x = (x | (x << 1)) & 0x5555555555555555ull;
+            return x;
+        };
+        // runs of cell_NS that share a base cell (cell_NS walks the base cells row-major)
+        std::vector<std::pair<uint64_t, int>> runs;  // (Morton key of the base cell, first position)
+        for (int e = 0; e < tb.N; e++) {
+            const int c = t.ns[e];
+            if (e == 0 || t.ni[c] != t.ni[t.ns[e - 1]] || t.nj[c] != t.nj[t.ns[e - 1]])
+                runs.emplace_back(spread((uint32_t)(t.nj[c] - 1)) | (spread((uint32_t)(t.ni[c] - 1)) << 1), e);
+        }
+        std::vector<int> run_end(runs.size());
+        for (size_t r = 0; r < runs.size(); r++) run_end[r] = r + 1 < runs.size() ? runs[r + 1].second : tb.N;
+        std::vector<int> order(runs.size());
+        for (size_t r = 0; r < runs.size(); r++) order[r] = (int)r;
+        std::sort(order.begin(), order.end(), [&](int a, int b) { return runs[a].first < runs[b].first; });
+        tb.ns_pos.resize(tb.N);
+        tb.dev_of_ns.resize(tb.N);
+        int i = 0;
+        for (int r : order)
+            for (int e = runs[r].second; e < run_end[r]; e++) { tb.ns_pos[i] = e; tb.dev_of_ns[e] = i; i++; }
+    }
+    for (int i = 0; i < tb.N; i++) {
+        const int c = t.ns[tb.ns_pos[i]];
+        tb.state_of_node[c] = i;
+        tb.leaf_node.push_back(c);
+    }
     for (int i = 0; i < tb.G; i++) { tb.state_of_node[t.bcl[i]] = tb.N + i; tb.leaf_node.push_back(t.bcl[i]); }
     // passive leaves: corners (never updated, solution.F90:644,682-686)
     for (size_t c = 0; c < NA; c++)
@@ -262,7 +298,7 @@ void build_tables(const Tree& t, Tables& tb) {
         int lm = 0;
         for (size_t ii = b; ii < e; ii++) {
             const int i = (int)ii;
-            const int c = t.ns[i];
+            const int c = tb.leaf_node[i];
             tb.lvl_leaf[i] = (int8_t)t.lvl[c];
             lm = std::max(lm, t.lvl[c]);
             const int32_t* q = &t.nb[8 * (size_t)c];
@@ -353,7 +389,7 @@ void build_tables(const Tree& t, Tables& tb) {
                 } else s = it->second;
                 ref = 1000000000 + s;  // patched to L + s below
             }
-            tb.stencil[(size_t)k * N + i] = ref;
+            tb.stencil[(size_t)i * 8 + k] = ref;
         }
     tb.S_w = (int)wls_node.size();
     tb.S_r = (int)tb.rst_node.size();
@@ -607,7 +643,7 @@ void build_tables(const Tree& t, Tables& tb) {
     parallel_for((size_t)N, [&](size_t b0, size_t e0) {
         for (size_t ii = b0; ii < e0; ii++) {
             const int i = (int)ii;
-            const int c = t.ns[i];
+            const int c = tb.leaf_node[i];
             const double dy = t.h / pow2i(t.lvl[c]), dx = t.xf * dy;
             int q = 0;
             for (int b = -2; b <= 2; b++)
@@ -634,8 +670,8 @@ void build_tables(const Tree& t, Tables& tb) {
         auto point_src = [&](int i, int p) -> int32_t {
             if (p >= 25 || p == 12) return i;
             const int a = p % 5 - 2, b = p / 5 - 2;
-            if (b == 0) return tb.stencil[(size_t)(a == 1 ? 0 : a == -1 ? 1 : a == 2 ? 4 : 5) * N + i];
-            if (a == 0) return tb.stencil[(size_t)(b == 1 ? 2 : b == -1 ? 3 : b == 2 ? 6 : 7) * N + i];
+            if (b == 0) return tb.stencil[(size_t)i * 8 + (a == 1 ? 0 : a == -1 ? 1 : a == 2 ? 4 : 5)];
+            if (a == 0) return tb.stencil[(size_t)i * 8 + (b == 1 ? 2 : b == -1 ? 3 : b == 2 ? 6 : 7)];
             const int q16 = (b + 2 - (b > 0 ? 1 : 0)) * 4 + (a + 2 - (a > 0 ? 1 : 0));
             return tb.off16[(size_t)q16 * N + i];
         };

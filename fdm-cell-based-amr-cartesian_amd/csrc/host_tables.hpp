@@ -70,7 +70,10 @@ struct Tables {
     int l_max = 0;
     std::vector<int32_t> state_of_node;  // node -> state index (leaves) or -1
     std::vector<int32_t> leaf_node;      // state index -> node
-    std::vector<int32_t> stencil;        // [8][N]: state index, or L + slot index
+    // interior leaves are stored in device order (base cells along a Morton curve); ns_pos[i]
+    // is the cell_NS position of device leaf i, dev_of_ns its inverse
+    std::vector<int32_t> ns_pos, dev_of_ns;
+    std::vector<int32_t> stencil;        // [N][8]: state index, or L + slot index
     std::vector<int8_t> lvl_leaf;        // [N]
     // diagnostics (host only)
     std::vector<int8_t> st_kind;         // [N*8]
