@@ -76,6 +76,11 @@ struct fdmamr_ctx {
     uint32_t sel_step = 0;
     int adam_t = 0;
     DBuf<int32_t> stencil, wls_i, rst, bcp, son, state;
+    DBuf<uint8_t> face_share;  // FP64 path: which lower faces equal the neighbour's upper face
+    DBuf<double4> fxp, fyp;    // FP64 path: every leaf's x+1/2 and y+1/2 face fluxes of a stage
+    DBuf<double4> fxm, fym;    // FP64 path: the lower faces that are not shared
+    DBuf<int32_t> face_lo;
+    bool share_off = false;    // FP64 path: compute every face in the update kernel
     DBuf<int8_t> lvl;
     DBuf<double> wls_d, partials, lampart, rot, div, tmp, parts;
     DBuf<int> nbad;
@@ -169,6 +174,16 @@ int upload_tables(fdmamr_ctx* c) {
         return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s);
     };
     HIPCHK(c, up(c->stencil.p, tb.stencil.data(), tb.stencil.size() * 4));
+    HIPCHK(c, ensure(c->face_share, tb.face_share.size()));
+    HIPCHK(c, up(c->face_share.p, tb.face_share.data(), tb.face_share.size()));
+    if (c->p.flux_precision == FDMAMR_FLUX_FP64) {
+        HIPCHK(c, ensure(c->fxp, tb.N));
+        HIPCHK(c, ensure(c->fyp, tb.N));
+        HIPCHK(c, ensure(c->fxm, tb.N));
+        HIPCHK(c, ensure(c->fym, tb.N));
+        HIPCHK(c, ensure(c->face_lo, tb.face_lo.size()));
+        HIPCHK(c, up(c->face_lo.p, tb.face_lo.data(), tb.face_lo.size() * 4));
+    }
     HIPCHK(c, up(c->lvl.p, tb.lvl_leaf.data(), tb.lvl_leaf.size()));
     HIPCHK(c, up(c->wls_d.p, tb.wls_d.data(), tb.wls_d.size() * 8));
     HIPCHK(c, up(c->wls_i.p, tb.wls_i.data(), tb.wls_i.size() * 4));
@@ -242,6 +257,10 @@ int upload_tables(fdmamr_ctx* c) {
     d.slot_blk_pat = c->slot_blk_pat.p;
     d.slot_lpat = c->slot_lpat.p;
     d.slot_uncov = c->slot_uncov.p;
+    d.face_share = c->face_share.p;
+    d.face_lo = c->face_lo.p;
+    d.n_lo_x = tb.n_lo_x;
+    d.n_lo_y = tb.n_lo_y;
     d.S_unc = (int)tb.slot_uncov.size();
     d.rst_node = c->rst.p; d.bc_partner = c->bcp.p; d.node_son = c->son.p; d.node_state = c->state.p;
     d.n_int = tb.n_int; d.int_child = c->int_child.p; d.rst_ref = c->rst_ref.p; d.bc_ref = c->bc_ref.p;
@@ -368,10 +387,15 @@ void enqueue_rk3(fdmamr_ctx* c, int advance_time, bool prepare_next) {
                               (select && k == 0) ? c->label.p : nullptr,
                               (select && k == 0) ? JudgeArgs{c->flags.p, c->sel_cnt.p, c->sel_bound} : JudgeArgs{},
                               posflag, s);
-        else
+        else if (c->share_off)
             launch_stage(d, c->Q[in[k]].p, c->Q[0].p, c->T[0].p, c->Q[out[k]].p, c->T[out[k]].p, c->sc, ra[k],
                          rb[k], k == 2 ? 1 : 0, c->partials.p, (prepare_next && k == 2) ? c->lampart.p : nullptr,
                          posflag, s);
+        else
+            launch_stage_shared(d, c->Q[in[k]].p, c->Q[0].p, c->T[0].p, c->Q[out[k]].p, c->T[out[k]].p, c->sc,
+                                ra[k], rb[k], k == 2 ? 1 : 0, c->partials.p,
+                                (prepare_next && k == 2) ? c->lampart.p : nullptr, posflag, c->fxp.p, c->fyp.p,
+                                c->fxm.p, c->fym.p, s);
         if (e0 >= 0 && (e1 = next_event(c)) >= 0) {
             (void)hipEventRecord(c->ev_pool[e1], s);
             c->ev_stage.emplace_back(e0, e1);
@@ -431,7 +455,7 @@ void fdmamr_destroy(fdmamr_ctx* c) {
     (void)hipSetDevice(c->p.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int k = 0; k < 3; k++) { release(c->Q[k]); release(c->T[k]); }
-    release(c->stencil); release(c->wls_i); release(c->rst); release(c->bcp); release(c->son);
+    release(c->stencil); release(c->face_share); release(c->fxp); release(c->fyp); release(c->fxm); release(c->fym); release(c->face_lo); release(c->wls_i); release(c->rst); release(c->bcp); release(c->son);
     release(c->state); release(c->lvl); release(c->wls_d); release(c->partials); release(c->lampart);
     release(c->rot); release(c->div); release(c->tmp); release(c->parts); release(c->nbad);
     release(c->Qf[0]); release(c->Qf[1]); release(c->Qf[2]); release(c->wls_f); release(c->flags);
@@ -1259,13 +1283,19 @@ int32_t fdmamr_stencil_values_fast(fdmamr_ctx* c, int32_t i, float* val8x4) {
     return 0;
 }
 
+int32_t fdmamr_set_face_share(fdmamr_ctx* c, int32_t enable) {
+    if (!c) return -1;
+    c->share_off = enable == 0;
+    return 0;
+}
+
 int32_t fdmamr_set_slot_merge(fdmamr_ctx* c, int32_t enable) {
     if (!c) return -1;
     c->merge_off = enable == 0;
     return 0;
 }
 
-int32_t fdmamr_table_info(fdmamr_ctx* c, int32_t* info8) {
+int32_t fdmamr_table_info(fdmamr_ctx* c, int32_t* info8) {  // 10 entries
     if (!c || !info8) return -1;
     if (!c->tables_ready) return fail(c, -2, "fdmamr_table_info: tables not built");
     const Tables& tb = c->tb;
@@ -1281,8 +1311,11 @@ int32_t fdmamr_table_info(fdmamr_ctx* c, int32_t* info8) {
         else max_nu = std::max(max_nu, nu);
         over64 += nu > 64;
     }
-    const int32_t v[8] = {tb.S_wp, tb.S_w, aliased, (int32_t)tb.fill_fast.size(), generic, direct, over64, max_nu};
-    for (int k = 0; k < 8; k++) info8[k] = v[k];
+    int shx = 0, shy = 0;
+    for (uint8_t f : tb.face_share) { shx += f & 1; shy += (f >> 1) & 1; }
+    const int32_t v[10] = {tb.S_wp, tb.S_w, aliased, (int32_t)tb.fill_fast.size(), generic, direct, over64, max_nu,
+                           shx, shy};
+    for (int k = 0; k < 10; k++) info8[k] = v[k];
     return 0;
 }
 

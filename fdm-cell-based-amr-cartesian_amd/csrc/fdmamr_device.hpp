@@ -111,6 +111,9 @@ struct DevTables {
     const uint8_t* slot_lpat;     // [S_w] position of the slot's record in its block's list
     const int32_t* slot_uncov;    // [S_unc] the slots whose record is not in their block's list
     int S_unc;
+    const uint8_t* face_share;    // [N] bit 0: x-1/2 face = left neighbour's x+1/2; bit 1: same in y
+    const int32_t* face_lo;       // leaves whose x-1/2 face is not shared ([0, n_lo_x)), then y-1/2
+    int n_lo_x, n_lo_y;
 };
 
 // the eight stencil ids of interior leaf i (two 16-byte loads of its record)
@@ -136,6 +139,13 @@ void launch_slots(const DevTables& t, double4* Q, const double* T, hipStream_t s
 void launch_stage(const DevTables& t, const double4* Qin, const double4* Qn, const double* Tn,
                   double4* Qout, double* Tout, const DevScalars* sc, double alpha, double beta,
                   int reduce, double* partials, double* lampart, int* posflag, hipStream_t st);
+// the same stage with shared faces: every leaf's x+1/2, y+1/2 faces into fxp, fyp and its
+// unshared lower faces into fxm, fym ([N] each), one face per thread, then the update reading
+// them (bit-identical to launch_stage)
+void launch_stage_shared(const DevTables& t, const double4* Qin, const double4* Qn, const double* Tn,
+                         double4* Qout, double* Tout, const DevScalars* sc, double alpha, double beta, int reduce,
+                         double* partials, double* lampart, int* posflag, double4* fxp, double4* fyp, double4* fxm,
+                         double4* fym, hipStream_t st);
 // ghost fill and restriction slots after a stage (restriction means one level per launch)
 void launch_restrict_fill(const DevTables& t, const std::vector<int32_t>& level_start, double4* Q,
                           double* T, float4* Qf, double4* Qi, double* Ti, hipStream_t st);

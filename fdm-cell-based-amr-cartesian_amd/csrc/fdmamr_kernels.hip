@@ -159,6 +159,49 @@ __device__ __forceinline__ void wcns(const S* u1, const S* u2, const S* u3, cons
     for (int r = 0; r < 4; r++) hp[r] = R[r][0] * uL[0] + R[r][1] * uL[1] + R[r][2] * uL[2] + R[r][3] * uL[3];
 }
 
+// One face of LXL_X_wcns3 / LXL_Y_wcns3 (nnd_center_RK.F90:86-106): the face between P[1] and
+// P[2] of four consecutive points; eigenvectors at the arithmetic mean of the two conservative
+// states, characteristic U and F, four WCNS_Z3 reconstructions, Lax-Friedrichs flux.
+template <typename S, int DIR>
+__device__ __forceinline__ void face_flux(const Pt<S>* P, S lam, S gama, S* flux) {
+    S avg[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) avg[k] = (P[1].U[k] + P[2].U[k]) / (S)2.0;
+    S R[4][4], Ri[4][4];
+    eigen<S, DIR>(avg, gama, R, Ri);
+    S w[4][4], g[4][4];  // characteristic variables at the four points
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            S s = (S)0.0, t = (S)0.0;
+#pragma unroll
+            for (int c = 0; c < 4; c++) { s = s + Ri[r][c] * P[j].U[c]; t = t + Ri[r][c] * P[j].F[c]; }
+            w[j][r] = s; g[j][r] = t;
+        }
+    S wL[4], wR[4], gL[4], gR[4];
+    wcns<S>(w[0], w[1], w[2], R, wL);
+    wcns<S>(w[3], w[2], w[1], R, wR);
+    wcns<S>(g[0], g[1], g[2], R, gL);
+    wcns<S>(g[3], g[2], g[1], R, gR);
+#pragma unroll
+    for (int k = 0; k < 4; k++) flux[k] = (S)0.5 * (gL[k] + gR[k] - lam * (wR[k] - wL[k]));
+}
+
+// the derivative from the two face fluxes and the point fluxes of i-1, i+1
+// (nnd_center_RK.F90:108-111 and :229-232, whose operator orders differ)
+template <typename S, int DIR>
+__device__ __forceinline__ void assemble(const S* flux0, const S* flux1, const Pt<S>& Pm, const Pt<S>& Pp,
+                                         double d, double* dEF) {
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        double df = (double)(flux1[n] - flux0[n]);
+        double de = (double)(Pp.F[n] - Pm.F[n]);
+        if (DIR == 0) dEF[n] = 4.0 / (3.0 * d) * df - 1.0 / (6.0 * d) * de;
+        else dEF[n] = 4.0 / 3.0 / d * df - 1.0 / 6.0 / d * de;
+    }
+}
+
 // LXL_X_wcns3 / LXL_Y_wcns3 (nnd_center_RK.F90:66-112,187-234): q[0..4] = points i-2..i+2
 template <typename S, int DIR>
 __device__ __forceinline__ void flux_derivative(const double4* q, double d, S lam, S gama, double* dEF) {
@@ -168,37 +211,8 @@ __device__ __forceinline__ void flux_derivative(const double4* q, double d, S la
     for (int j = 0; j < 5; j++) cons_flux<S, DIR>(q[j], gm1, P[j]);
     S flux[2][4];
 #pragma unroll
-    for (int f = 0; f < 2; f++) {
-        S avg[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) avg[k] = (P[f + 1].U[k] + P[f + 2].U[k]) / (S)2.0;
-        S R[4][4], Ri[4][4];
-        eigen<S, DIR>(avg, gama, R, Ri);
-        S w[4][4], g[4][4];  // characteristic variables at points f..f+3
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                S s = (S)0.0, t = (S)0.0;
-#pragma unroll
-                for (int c = 0; c < 4; c++) { s = s + Ri[r][c] * P[f + j].U[c]; t = t + Ri[r][c] * P[f + j].F[c]; }
-                w[j][r] = s; g[j][r] = t;
-            }
-        S wL[4], wR[4], gL[4], gR[4];
-        wcns<S>(w[0], w[1], w[2], R, wL);
-        wcns<S>(w[3], w[2], w[1], R, wR);
-        wcns<S>(g[0], g[1], g[2], R, gL);
-        wcns<S>(g[3], g[2], g[1], R, gR);
-#pragma unroll
-        for (int k = 0; k < 4; k++) flux[f][k] = (S)0.5 * (gL[k] + gR[k] - lam * (wR[k] - wL[k]));
-    }
-#pragma unroll
-    for (int n = 0; n < 4; n++) {
-        double df = (double)(flux[1][n] - flux[0][n]);
-        double de = (double)(P[3].F[n] - P[1].F[n]);
-        if (DIR == 0) dEF[n] = 4.0 / (3.0 * d) * df - 1.0 / (6.0 * d) * de;
-        else dEF[n] = 4.0 / 3.0 / d * df - 1.0 / 6.0 / d * de;
-    }
+    for (int f = 0; f < 2; f++) face_flux<S, DIR>(P + f, lam, gama, flux[f]);
+    assemble<S, DIR>(flux[0], flux[1], P[1], P[3], d, dEF);
 }
 
 // ---------------------------------------------------------------- stage kernel
@@ -266,6 +280,137 @@ __global__ __launch_bounds__(256) void k_stage(DevTables t, const double4* __res
         if (posflag && (o.x <= 0.0 || o.w <= 0.0)) atomicOr(posflag, 1);
         if (lampart) {
             // the next step's wave speeds on this final state (solution.F90:186-193)
+            const double a = sqrt(t.gamma * o.w / o.x);
+            lx = fabs(o.y) + a;
+            ly = fabs(o.z) + a;
+        }
+    }
+    if (reduce) block_partials(rmax, rsum, partials, lb, lampart, lx, ly);
+}
+
+// ---------------------------------------------------------------- face-shared FP64 stage
+// The same stage with each face computed once where two leaves read it from the same points.
+// k_faces computes, one face per thread, every leaf's x+1/2 and y+1/2 face (points i-1, i,
+// i+1, i+2) into fxp / fyp, and the x-1/2 / y-1/2 faces that differ from the lower neighbour's
+// upper face (Tables::face_lo) into fxm / fym; k_stage_shared then only reads faces: its lower
+// face from the neighbour's upper one where the tables mark them identical (the neighbour's
+// four stencil ids are the leaf's), else its own. Every face is evaluated with k_stage's
+// arithmetic, so the results are the same bit for bit.
+template <int DIR, bool LOWER>
+__device__ __forceinline__ void face_item(const DevTables& t, const double4* __restrict__ Qin, int i, double lam,
+                                          double4* __restrict__ out) {
+    const int32_t* rec = t.stencil + (size_t)i * 8;
+    // upper face: points i-1, i, i+1, i+2; lower face: i-2, i-1, i, i+1 (stencil rows
+    // +1x 0, -1x 1, +1y 2, -1y 3, +2x 4, -2x 5, +2y 6, -2y 7)
+    const int a = LOWER ? rec[DIR ? 7 : 5] : rec[DIR ? 3 : 1];
+    const int b = LOWER ? rec[DIR ? 3 : 1] : i;
+    const int c = LOWER ? i : rec[DIR ? 2 : 0];
+    const int d = LOWER ? rec[DIR ? 2 : 0] : rec[DIR ? 6 : 4];
+    const double gama = t.gamma, gm1 = gama - 1.0;
+    Pt<double> P[4];
+    cons_flux<double, DIR>(Qin[a], gm1, P[0]);
+    cons_flux<double, DIR>(Qin[b], gm1, P[1]);
+    cons_flux<double, DIR>(Qin[c], gm1, P[2]);
+    cons_flux<double, DIR>(Qin[d], gm1, P[3]);
+    double fl[4];
+    face_flux<double, DIR>(P, lam, gama, fl);
+    out[i] = make_double4(fl[0], fl[1], fl[2], fl[3]);
+}
+
+// Blocks: x+1/2 faces of leaves [0, N), y+1/2, then the listed x-1/2 and y-1/2 faces; each
+// segment starts on a block boundary (nb0 .. nb3 blocks), so a block runs one face kind.
+__global__ __launch_bounds__(256) void k_faces(DevTables t, const double4* __restrict__ Qin,
+                                               const DevScalars* __restrict__ sc, double4* __restrict__ fxp,
+                                               double4* __restrict__ fyp, double4* __restrict__ fxm,
+                                               double4* __restrict__ fym, int nb0, int nb2) {
+    const int b = blockIdx.x;
+    const double lx = sc->lam_x, ly = sc->lam_y;
+    if (b < 2 * nb0) {
+        const int seg = b < nb0 ? 0 : 1;
+        const int lb = xcd_block_n(b - seg * nb0, nb0);
+        const int i = lb * 256 + (int)threadIdx.x;
+        if (i >= t.N) return;
+        if (seg == 0) face_item<0, false>(t, Qin, i, lx, fxp);
+        else face_item<1, false>(t, Qin, i, ly, fyp);
+    } else {
+        const int k = (b - 2 * nb0) * 256 + (int)threadIdx.x;
+        if (k < t.n_lo_x) face_item<0, true>(t, Qin, t.face_lo[k], lx, fxm);
+        else if (k >= nb2 * 256 && k - nb2 * 256 < t.n_lo_y)
+            face_item<1, true>(t, Qin, t.face_lo[t.n_lo_x + (k - nb2 * 256)], ly, fym);
+    }
+}
+
+template <int DIR>
+__device__ __forceinline__ void shared_derivative(const DevTables& t, const double4* __restrict__ Qin,
+                                                  const int* s, int i, bool shared,
+                                                  const double4* __restrict__ fp, const double4* __restrict__ fm,
+                                                  double d, double* dEF) {
+    const double gm1 = t.gamma - 1.0;
+    Pt<double> Pm, Pp;  // points i-1, i+1 (their fluxes enter the derivative directly)
+    cons_flux<double, DIR>(Qin[DIR ? s[3] : s[1]], gm1, Pm);
+    cons_flux<double, DIR>(Qin[DIR ? s[2] : s[0]], gm1, Pp);
+    const double4 q0 = shared ? fp[DIR ? s[3] : s[1]] : fm[i];
+    const double4 q1 = fp[i];
+    const double f0[4] = {q0.x, q0.y, q0.z, q0.w}, f1[4] = {q1.x, q1.y, q1.z, q1.w};
+    assemble<double, DIR>(f0, f1, Pm, Pp, d, dEF);
+}
+
+__global__ __launch_bounds__(256) void k_stage_shared(DevTables t, const double4* __restrict__ Qin,
+                                                      const double4* Qn, const double* Tn, double4* Qout,
+                                                      double* Tout, const DevScalars* __restrict__ sc,
+                                                      double alpha, double beta, int reduce,
+                                                      double* __restrict__ partials, double* __restrict__ lampart,
+                                                      int* posflag, const double4* __restrict__ fxp,
+                                                      const double4* __restrict__ fyp,
+                                                      const double4* __restrict__ fxm,
+                                                      const double4* __restrict__ fym) {
+    const int lb = xcd_block();
+    const int i = lb * blockDim.x + threadIdx.x;
+    double rmax[4] = {0, 0, 0, 0}, rsum[4] = {0, 0, 0, 0};
+    double lx = 0.0, ly = 0.0;
+    if (i < t.N) {
+        const double gama = t.gamma, cv = t.cv;
+        const double dt = sc->dt;
+        int s[8];
+        load_stencil(t, i, s);
+        const double4 qc = Qin[i];
+        const int lv = t.lvl[i];
+        const double dy = t.h / (double)(1 << lv);
+        const double dx = t.xf * dy;
+        const uint8_t share = t.face_share[i];
+        double dE[4], dF[4];
+        shared_derivative<0>(t, Qin, s, i, share & 1, fxp, fxm, dx, dE);
+        shared_derivative<1>(t, Qin, s, i, share & 2, fyp, fym, dy, dF);
+        // LXL (nnd_center_RK.F90:22-42), as k_stage
+        double U[4], Ut[4], fu[4];
+        U[0] = qc.x;
+        U[1] = qc.x * qc.y;
+        U[2] = qc.x * qc.z;
+        U[3] = qc.w / (gama - 1.0) + 0.5 * qc.x * (qc.y * qc.y + qc.z * qc.z);
+#pragma unroll
+        for (int k = 0; k < 4; k++) fu[k] = -dE[k] - dF[k];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            Ut[k] = U[k] + fu[k] * dt;
+            fu[k] = fu[k] * dx * dy;
+            rmax[k] = fabs(fu[k]);
+            rsum[k] = fu[k] * fu[k];
+        }
+        double r = Ut[0];
+        double u = Ut[1] / Ut[0];
+        double v = Ut[2] / Ut[0];
+        double T = (Ut[3] - 0.5 * Ut[0] * (u * u + v * v)) / (cv * Ut[0]);
+        double p = (gama - 1.0) * r * (Ut[3] / r - 0.5 * (u * u + v * v));
+        const double4 qn = Qn[i];
+        double4 o;
+        o.x = alpha * qn.x + beta * r;
+        o.y = alpha * qn.y + beta * u;
+        o.z = alpha * qn.z + beta * v;
+        o.w = alpha * qn.w + beta * p;
+        Qout[i] = o;
+        Tout[i] = alpha * Tn[i] + beta * T;
+        if (posflag && (o.x <= 0.0 || o.w <= 0.0)) atomicOr(posflag, 1);
+        if (lampart) {
             const double a = sqrt(t.gamma * o.w / o.x);
             lx = fabs(o.y) + a;
             ly = fabs(o.z) + a;
@@ -723,6 +868,18 @@ void launch_stage(const DevTables& t, const double4* Qin, const double4* Qn, con
     if (t.N <= 0) return;
     hipLaunchKernelGGL(k_stage<double>, dim3(nblk(t.N, 256)), dim3(256), 0, st, t, Qin, Qn, Tn, Qout, Tout,
                        sc, alpha, beta, reduce, partials, lampart, posflag);
+}
+
+void launch_stage_shared(const DevTables& t, const double4* Qin, const double4* Qn, const double* Tn,
+                         double4* Qout, double* Tout, const DevScalars* sc, double alpha, double beta, int reduce,
+                         double* partials, double* lampart, int* posflag, double4* fxp, double4* fyp, double4* fxm,
+                         double4* fym, hipStream_t st) {
+    if (t.N <= 0) return;
+    const int nb0 = nblk(t.N, 256), nb2 = nblk(t.n_lo_x, 256), nb3 = nblk(t.n_lo_y, 256);
+    hipLaunchKernelGGL(k_faces, dim3(2 * nb0 + nb2 + nb3), dim3(256), 0, st, t, Qin, sc, fxp, fyp, fxm, fym, nb0,
+                       nb2);
+    hipLaunchKernelGGL(k_stage_shared, dim3(nb0), dim3(256), 0, st, t, Qin, Qn, Tn, Qout, Tout, sc, alpha, beta,
+                       reduce, partials, lampart, posflag, fxp, fyp, fxm, fym);
 }
 
 void launch_wavespeed(const DevTables& t, const double4* Q, double* partials, hipStream_t st) {

@@ -564,6 +564,32 @@ void build_tables(const Tree& t, Tables& tb) {
         }
         iv[WLSI_M] |= gmask << WLSI_GSHIFT;
     }
+    // shared faces (the FP64 stage computes such a face once): after every redirect, so the
+    // comparison is on the ids the kernels read
+    tb.face_share.assign(N, 0);
+    parallel_for((size_t)N, [&](size_t b0, size_t e0) {
+        for (size_t ii = b0; ii < e0; ii++) {
+            const int32_t* sv = &tb.stencil[ii * 8];
+            uint8_t f = 0;
+            const int32_t L = sv[1], B = sv[3];
+            if (L >= 0 && L < N) {
+                const int32_t* sl = &tb.stencil[(size_t)L * 8];
+                if (sl[0] == (int32_t)ii && sl[1] == sv[5] && sl[4] == sv[0]) f |= 1;
+            }
+            if (B >= 0 && B < N) {
+                const int32_t* sb = &tb.stencil[(size_t)B * 8];
+                if (sb[2] == (int32_t)ii && sb[3] == sv[7] && sb[6] == sv[2]) f |= 2;
+            }
+            tb.face_share[ii] = f;
+        }
+    });
+    tb.face_lo.clear();
+    for (int i = 0; i < N; i++)
+        if (!(tb.face_share[i] & 1)) tb.face_lo.push_back(i);
+    tb.n_lo_x = (int)tb.face_lo.size();
+    for (int i = 0; i < N; i++)
+        if (!(tb.face_share[i] & 2)) tb.face_lo.push_back(i);
+    tb.n_lo_y = (int)tb.face_lo.size() - tb.n_lo_x;
     // slots that read a ghost go last (slots [0, S_wp) read none), each part in its old order
     {
         std::vector<int32_t> newidx(tb.S_w);

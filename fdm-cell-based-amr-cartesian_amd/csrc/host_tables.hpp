@@ -74,6 +74,12 @@ struct Tables {
     // is the cell_NS position of device leaf i, dev_of_ns its inverse
     std::vector<int32_t> ns_pos, dev_of_ns;
     std::vector<int32_t> stencil;        // [N][8]: state index, or L + slot index
+    // bit 0: the leaf's x-1/2 face reads the same four state indices as its left neighbour's
+    // x+1/2 face (so both compute the same flux, bit for bit); bit 1: the same in y
+    std::vector<uint8_t> face_share;     // [N]
+    // the leaves whose x-1/2 face is not shared, ascending, then those whose y-1/2 face is not
+    std::vector<int32_t> face_lo;
+    int n_lo_x = 0, n_lo_y = 0;
     std::vector<int8_t> lvl_leaf;        // [N]
     // diagnostics (host only)
     std::vector<int8_t> st_kind;         // [N*8]

@@ -31,7 +31,7 @@ EXPORTS = (
     "fdmamr_get_ghost_state", "fdmamr_leaf_geometry", "fdmamr_max_wavespeed",
     "fdmamr_max_wavespeed_noalloc",
     "fdmamr_advance_rk3", "fdmamr_run_steps", "fdmamr_check_positivity", "fdmamr_sensors",
-    "fdmamr_regrid", "fdmamr_regrid_sensors", "fdmamr_stencil_info", "fdmamr_stencil_values", "fdmamr_stencil_values_fast", "fdmamr_table_info", "fdmamr_leaf_lists", "fdmamr_set_slot_merge", "fdmamr_profile",
+    "fdmamr_regrid", "fdmamr_regrid_sensors", "fdmamr_stencil_info", "fdmamr_stencil_values", "fdmamr_stencil_values_fast", "fdmamr_table_info", "fdmamr_leaf_lists", "fdmamr_set_slot_merge", "fdmamr_set_face_share", "fdmamr_profile",
     "fdmamr_profile_read", "fdmamr_selector_set_flags", "fdmamr_selector_get_flags",
     "fdmamr_labels", "fdmamr_selector_init", "fdmamr_selector_set_weights",
     "fdmamr_selector_get_weights", "fdmamr_selector_config", "fdmamr_selector_infer",
@@ -100,6 +100,7 @@ def lib():
         "fdmamr_table_info": ([P, ip], C.c_int32),
         "fdmamr_leaf_lists": ([P, ip, ip], C.c_int32),
         "fdmamr_set_slot_merge": ([P, C.c_int32], C.c_int32),
+        "fdmamr_set_face_share": ([P, C.c_int32], C.c_int32),
         "fdmamr_stencil_values_fast": ([P, C.c_int32, C.POINTER(C.c_float)], C.c_int32),
         "fdmamr_profile": ([P, C.c_int32], C.c_int32),
         "fdmamr_profile_read": ([P, dp, ip], C.c_int32),
@@ -487,6 +488,10 @@ class Solver:
         """False: WLS-slot launch and ghost refill as separate launches on every mesh."""
         self._check(self.L.fdmamr_set_slot_merge(self.ctx, 1 if enable else 0), "set_slot_merge")
 
+    def set_face_share(self, enable):
+        """False: the FP64 update kernel computes every face itself (the reference's way)."""
+        self._check(self.L.fdmamr_set_face_share(self.ctx, 1 if enable else 0), "set_face_share")
+
     def leaf_lists(self):
         """The fast path's (cleared, troubled) leaf lists as the last flagged step built them."""
         lst = np.zeros(self.n_leaf, np.int32)
@@ -495,11 +500,12 @@ class Solver:
         return lst[: int(nc[0])], lst[int(nc[0]):]
 
     def table_info(self):
-        v = np.zeros(8, np.int32)
+        v = np.zeros(10, np.int32)
         self._check(self.L.fdmamr_table_info(self.ctx, _ip(v)), "table_info")
         return dict(slots_plain=int(v[0]), slots=int(v[1]), ghosts_aliased=int(v[2]),
                     fill_items_fast=int(v[3]), slots_generic=int(v[4]), select_tiles_direct=int(v[5]),
-                    select_tiles_over64=int(v[6]), select_tile_max=int(v[7]))
+                    select_tiles_over64=int(v[6]), select_tile_max=int(v[7]), faces_shared_x=int(v[8]),
+                    faces_shared_y=int(v[9]))
 
 
 def config_c1(**kw) -> Config:

@@ -178,12 +178,13 @@ int32_t fdmamr_stencil_values(fdmamr_ctx* ctx, int32_t i, double* val8x4);
  * positions, after the launch that evaluates the WLS slots and refills the ghosts of the
  * current state (no reference counterpart; diagnostics) */
 int32_t fdmamr_stencil_values_fast(fdmamr_ctx* ctx, int32_t i, float* val8x4);
-/* regrid-time table facts (no reference counterpart; diagnostics): info8[0] WLS slots that
- * read no ghost leaf, [1] WLS slots, [2] ghost leaves read through the leaf they copy (or the
- * first inflow ghost), [3] items the fast path refills per stage (the other ghosts and the
- * restriction slots), [4] ghost-reading slots without a fixed-size entry list, [5] selector
+/* regrid-time table facts (no reference counterpart; diagnostics), 10 entries: info8[0] WLS
+ * slots that read no ghost leaf, [1] WLS slots, [2] ghost leaves read through the leaf they copy
+ * (or the first inflow ghost), [3] items the fast path refills per stage (the other ghosts and
+ * the restriction slots), [4] ghost-reading slots without a fixed-size entry list, [5] selector
  * tiles gathered without LDS staging, [6] selector tiles reading more than 64 distinct values,
- * [7] the most distinct values of a staged selector tile */
+ * [7] the most distinct values of a staged selector tile, [8] / [9] interior leaves whose x-1/2
+ * / y-1/2 face the FP64 stage takes from the neighbour (fdmamr_set_face_share) */
 int32_t fdmamr_table_info(fdmamr_ctx* ctx, int32_t* info8);
 /* the fast path's leaf lists as the last flagged step built them (no reference counterpart;
  * diagnostics): lists[0, *n_clear) the cleared interior leaves, then the troubled ones, each
@@ -193,6 +194,12 @@ int32_t fdmamr_leaf_lists(fdmamr_ctx* ctx, int32_t* lists, int32_t* n_clear);
  * even on meshes where the refill can ride along in the slot launch (no reference counterpart;
  * the default, 1, merges them where possible) */
 int32_t fdmamr_set_slot_merge(fdmamr_ctx* ctx, int32_t enable);
+
+/* FP64 contexts: enable = 0 computes both faces of each direction in the update kernel of every
+ * leaf, as the reference does; the default, 1, computes a face once where both adjacent leaves
+ * read it from the same four state indices (same results bit for bit; no reference
+ * counterpart, diagnostics and timing) */
+int32_t fdmamr_set_face_share(fdmamr_ctx* ctx, int32_t enable);
 
 /* ---- troubled-cell selector (no reference counterpart: the reference applies the nonlinear
  * WCNS weights everywhere, nnd_center_RK.F90:310-353). FP32 mode only. A leaf is "troubled"

@@ -162,3 +162,27 @@ def test_shock_bubble_inflow_against_oracle():
                    lvl_amr=2, pstep_adapt=5, amr_sensor=1, pre_adapt=2, wls_local=1)
     o, s = _trajectory(cfg, 12, 4)
     assert s.counts()["l_max"] == 2
+
+
+@pytest.mark.parametrize("case", ["riemann", "periodic"])
+def test_face_shared_fp64_stage_is_bit_identical(case):
+    # the FP64 stage computes a face once where the two adjacent leaves read it from the same
+    # four state indices; the results must equal the reference-structured stage (each leaf
+    # computing its own four faces) bit for bit, ghosts included
+    if case == "riemann":
+        cfg = riemann_c2(h=1.0 / 64, lvl_amr=2, pre_adapt=2)
+    else:
+        cfg = F.Config(xmin=0.0, xmax=2.0, ymin=0.0, ymax=2.0, h=2 / 24, ic_kind=F.IC_DENSITY_WAVE, localdt=1,
+                       cfl=0.5, amr=1, lvl_amr=3, pstep_adapt=2, amr_sensor=1, wls_local=1, refine_goal_rot=1.0,
+                       refine_goal_div=1.0, pre_adapt=3)
+    a = F.Solver(cfg).setup()
+    b = F.Solver(cfg).setup()
+    b.set_face_share(False)
+    for _ in range(3):
+        a.run_steps(4)
+        b.run_steps(4)
+        a.regrid()
+        b.regrid()
+    assert np.array_equal(a.get_state(), b.get_state())
+    assert np.array_equal(a.get_ghost_state(), b.get_ghost_state())
+    assert a.tt == b.tt
