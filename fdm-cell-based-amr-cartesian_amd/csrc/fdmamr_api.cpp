@@ -87,6 +87,11 @@ struct fdmamr_ctx {
     DevScalars* sc = nullptr;
     DevTables dt{};
 
+    // pinned staging for large host-to-device copies (two buffers, alternating)
+    void* stage_buf[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    bool stage_pending[2] = {false, false};
+
     // kernel timing with HIP events on the context's stream (bench / profiling)
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
@@ -136,6 +141,38 @@ int tree_depth(const Tree& t) {
     return d;
 }
 
+// Host-to-device copy of a large host array through two pinned staging buffers: host threads
+// copy one chunk into a buffer while the DMA engine reads the other, so the transfer runs at
+// the pinned rate rather than the runtime's single-threaded pageable path. Enqueued on the
+// context's stream; the source may be reused once the stream has passed the copy.
+constexpr size_t STAGE_BYTES = (size_t)32 << 20;
+hipError_t upload_staged(fdmamr_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return hipSuccess;
+    if (bytes < ((size_t)4 << 20)) return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream);
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 2 && e == hipSuccess; k++) {
+        if (!c->stage_buf[k]) e = hipHostMalloc(&c->stage_buf[k], STAGE_BYTES, hipHostMallocDefault);
+        if (e == hipSuccess && !c->stage_ev[k]) e = hipEventCreateWithFlags(&c->stage_ev[k], hipEventDisableTiming);
+    }
+    if (e != hipSuccess) return e;
+    for (size_t off = 0, k = 0; off < bytes; off += STAGE_BYTES, k ^= 1) {
+        const size_t n = std::min(STAGE_BYTES, bytes - off);
+        if (c->stage_pending[k]) {
+            e = hipEventSynchronize(c->stage_ev[k]);
+            if (e != hipSuccess) return e;
+            c->stage_pending[k] = false;
+        }
+        unsigned char* buf = static_cast<unsigned char*>(c->stage_buf[k]);
+        const unsigned char* from = static_cast<const unsigned char*>(src) + off;
+        parallel_for(n, [&](size_t b, size_t e2) { std::memcpy(buf + b, from + b, e2 - b); }, (size_t)1 << 20);
+        e = hipMemcpyAsync(static_cast<unsigned char*>(dst) + off, buf, n, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipEventRecord(c->stage_ev[k], c->stream);
+        if (e != hipSuccess) return e;
+        c->stage_pending[k] = true;
+    }
+    return hipSuccess;
+}
+
 int upload_tables(fdmamr_ctx* c) {
     Tree& t = c->tree;
     Tables& tb = c->tb;
@@ -169,10 +206,7 @@ int upload_tables(fdmamr_ctx* c) {
     HIPCHK(c, ensure(c->parts, STEP_PARTS * 10));
     HIPCHK(c, ensure(c->nbad, 1));
     hipStream_t s = c->stream;
-    auto up = [&](void* d, const void* h, size_t bytes) -> hipError_t {
-        if (bytes == 0) return hipSuccess;
-        return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s);
-    };
+    auto up = [&](void* d, const void* h, size_t bytes) -> hipError_t { return upload_staged(c, d, h, bytes); };
     HIPCHK(c, up(c->stencil.p, tb.stencil.data(), tb.stencil.size() * 4));
     HIPCHK(c, ensure(c->face_share, tb.face_share.size()));
     HIPCHK(c, up(c->face_share.p, tb.face_share.data(), tb.face_share.size()));
@@ -192,16 +226,20 @@ int upload_tables(fdmamr_ctx* c) {
     HIPCHK(c, up(c->son.p, tb.node_son.data(), tb.node_son.size() * 4));
     HIPCHK(c, up(c->state.p, tb.state_of_node.data(), tb.state_of_node.size() * 4));
     // state of every leaf (interior, ghost, corner) into all three stage buffers
-    std::vector<double4> q(tb.L);
-    std::vector<double> tv(tb.L);
-    for (int i = 0; i < tb.L; i++) {
-        const auto& u = t.U[tb.leaf_node[i]];
-        q[i] = make_double4(u[0], u[1], u[2], u[4]);
-        tv[i] = u[3];
-    }
-    for (int k = 0; k < 3; k++) {
-        HIPCHK(c, up(c->Q[k].p, q.data(), q.size() * sizeof(double4)));
-        HIPCHK(c, up(c->T[k].p, tv.data(), tv.size() * 8));
+    BigVec<double4> q(tb.L);
+    BigVec<double> tv(tb.L);
+    parallel_for((size_t)tb.L, [&](size_t b, size_t e) {
+        for (size_t i = b; i < e; i++) {
+            const auto& u = t.U[tb.leaf_node[i]];
+            q[i] = make_double4(u[0], u[1], u[2], u[4]);
+            tv[i] = u[3];
+        }
+    });
+    HIPCHK(c, up(c->Q[0].p, q.data(), q.size() * sizeof(double4)));
+    HIPCHK(c, up(c->T[0].p, tv.data(), tv.size() * 8));
+    for (int k = 1; k < 3; k++) {
+        HIPCHK(c, hipMemcpyAsync(c->Q[k].p, c->Q[0].p, q.size() * sizeof(double4), hipMemcpyDeviceToDevice, s));
+        HIPCHK(c, hipMemcpyAsync(c->T[k].p, c->T[0].p, tv.size() * 8, hipMemcpyDeviceToDevice, s));
     }
     HIPCHK(c, up(c->wls_f.p, tb.wls_f.data(), tb.wls_f.size() * 4));
     HIPCHK(c, ensure(c->Qint, tb.n_int));
@@ -468,6 +506,10 @@ void fdmamr_destroy(fdmamr_ctx* c) {
     release(c->dep_hdr); release(c->dep_leaf); release(c->dep_w);
     release(c->sel_nbr_beg); release(c->sel_nbr); release(c->sel_loc);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    for (int k = 0; k < 2; k++) {
+        if (c->stage_ev[k]) (void)hipEventDestroy(c->stage_ev[k]);
+        if (c->stage_buf[k]) (void)hipHostFree(c->stage_buf[k]);
+    }
     if (c->sc) (void)hipFree(c->sc);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -542,9 +584,9 @@ int32_t fdmamr_tree_import(fdmamr_ctx* c, int32_t m, int32_t n, int32_t n_node,
         }
     }
     Tree& t = c->tree;
-    std::vector<int32_t> vni(n_node), vnj(n_node);
+    BigVec<int32_t> vni(n_node), vnj(n_node);
     for (int k = 0; k < n_node; k++) { vni[k] = num2[2 * k]; vnj[k] = num2[2 * k + 1]; }
-    std::vector<std::array<double, 7>> U(n_node);
+    BigVec<std::array<double, 7>> U(n_node);
     for (int k = 0; k < n_node; k++)
         for (int i = 0; i < 7; i++) U[k][i] = prim7[7 * (size_t)k + i];
     t.m = m; t.n = n; t.total = total;
@@ -601,10 +643,13 @@ int32_t fdmamr_build_tables(fdmamr_ctx* c) {
     if (!c) return -1;
     if (c->tree.total == 0) return fail(c, -1, "no mesh");
     if (!c->host_only && hipSetDevice(c->p.device) != hipSuccess) return fail(c, -2, "hipSetDevice failed");
+    PhaseTimer pt;
     try {
         c->tree.idgrid();
+        pt.mark("IDgrid");
         if (tree_depth(c->tree) >= MAX_TREE_DEPTH - 1) return fail(c, -1, "quadtree deeper than supported");
         build_tables(c->tree, c->tb);
+        pt.mark("build_tables");
     } catch (const std::exception& e) {
         return fail(c, -1, std::string("table build: ") + e.what());
     }
@@ -614,6 +659,7 @@ int32_t fdmamr_build_tables(fdmamr_ctx* c) {
     }
     int r = upload_tables(c);
     if (r) return r;
+    pt.mark("upload");
     c->tables_ready = true;
     return 0;
 }
@@ -838,6 +884,7 @@ int32_t fdmamr_sensors(fdmamr_ctx* c, double* rot, double* div) {
 // solute_gridmodify, idneighbor, IDgrid + tables. The tree's rot/div hold the sensors.
 static int regrid_from_sensors(fdmamr_ctx* c, int32_t lvl_amr, double refine_rot, double refine_div,
                                double coarse_rot, double coarse_div, int32_t sensor_mode) {
+    PhaseTimer pt;
     Tree& t = c->tree;
     if (sensor_mode == 1)
         for (int k : t.ns) t.rot[k] = t.div[k];
@@ -850,8 +897,11 @@ static int regrid_from_sensors(fdmamr_ctx* c, int32_t lvl_amr, double refine_rot
     double arot = std::sqrt(srot / ne), adiv = std::sqrt(sdiv / ne);
     try {
         t.solute_ref(arot, adiv, lvl_amr, refine_rot, refine_div, coarse_rot, coarse_div);
+        pt.mark("solute_ref");
         t.solute_gridmodify(lvl_amr);
+        pt.mark("solute_gridmodify");
         t.idneighbor();
+        pt.mark("idneighbor");
     } catch (const std::exception& e) {
         return fail(c, -1, e.what());
     }
@@ -863,21 +913,27 @@ int32_t fdmamr_regrid(fdmamr_ctx* c, int32_t lvl_amr, double refine_rot, double 
                       double coarse_rot, double coarse_div, int32_t sensor_mode) {
     int r = check_ready(c);
     if (r) return r;
+    PhaseTimer pt;
     r = pull_state(c);
     if (r) return r;
+    pt.mark("state download");
     r = fdmamr_sensors(c, nullptr, nullptr);
     if (r) return r;
+    pt.mark("sensors");
     return regrid_from_sensors(c, lvl_amr, refine_rot, refine_div, coarse_rot, coarse_div, sensor_mode);
 }
 
 int32_t fdmamr_regrid_sensors(fdmamr_ctx* c, int32_t lvl_amr, double refine_rot, double refine_div,
                               double coarse_rot, double coarse_div, int32_t sensor_mode, const double* rot,
                               const double* div) {
-    int r = check_ready(c);
-    if (r) return r;
+    if (!c) return -1;
+    // a host-only context regrids its host tree (no state to download, no upload)
+    if (!c->tables_ready) return fail(c, -1, "tables not built: call fdmamr_build_tables first");
     if (!rot || !div) return fail(c, -1, "fdmamr_regrid_sensors: null sensor array");
-    r = pull_state(c);
-    if (r) return r;
+    if (!c->host_only) {
+        int r = pull_state(c);
+        if (r) return r;
+    }
     Tree& t = c->tree;
     for (size_t i = 0; i < t.ns.size(); i++) {
         if (!std::isfinite(rot[i]) || !std::isfinite(div[i]))

@@ -45,6 +45,18 @@ int brinv(int n, const double* a, double* r) {     // XuanGua.F90:593-660 (colum
 
 int get_direction(double xx, double yy, double x, double y) {   // XuanGua.F90:33-43
     double dx = xx - x, dy = yy - y;
+    // The octants are bounded by |dy| / |dx| = tan(22.5 deg) and its inverse. Away from those
+    // lines (relative margin 1e-7, far above the arctangent's rounding and the 1e-10 shift) the
+    // octant follows from comparisons; on a stencil the offsets are small multiples of the cell
+    // sizes, whose ratios never come near the irrational bound, so the arctangent below is the
+    // rare fallback.
+    const double ax = std::fabs(dx), ay = std::fabs(dy), t = 0.41421356237309503;  // tan(22.5 deg)
+    const double tol = 1e-7 * (ax + ay);
+    if (ax + ay > 0 && std::fabs(ay - t * ax) > tol && std::fabs(ax - t * ay) > tol) {
+        if (ay < t * ax) return dx > 0 ? 1 : 5;
+        if (ax < t * ay) return dy > 0 ? 3 : 7;
+        return dx > 0 ? (dy > 0 ? 2 : 8) : (dy > 0 ? 4 : 6);
+    }
     double angle = (std::atan2(dy, dx) + DIR_EPS) * 180.0 / PI_REF;
     int fl = (int)std::floor((angle + 22.5) / 45.0 + 8);
     return fl % 8 + 1;
@@ -95,29 +107,56 @@ void child_lists(const Tree& t, int c, List& l1, List& l2) {   // XuanGua.F90:96
         if (q[order[k]] >= 0) search5(t, q[order[k]], t.x[c], t.y[c], l1, l2, used);
 }
 
-// WLS-WENO geometry of one slot (XuanGua.F90:206-255 and the box of :318-323)
-void wls_geometry(const Tree& t, int cell, double xx, double yy, const List& l1, const List& l2,
-                  const std::vector<int32_t>& state_of_node, double* d, int32_t* iv) {
-    int m1 = l1.n, m = l1.n + l2.n;
-    double px[16], py[16], w[16], P[16][6];
-    // cell-local scaled frame (optional): x' = (x - xc)/hx, box [-1/2, 1/2]^2
+// Inputs of a slot's WLS-WENO geometry: the stencil coordinates, the evaluation point and the
+// cell's box, in the frame the record uses (the reference's absolute frame, or the cell-local
+// scaled one: x' = (x - xc)/hx, box [-1/2, 1/2]^2). Unused entries are zero, so two slots have
+// the same geometry record exactly when their inputs are equal bit for bit.
+struct WlsIn {
+    int32_t m, m1;
+    double xx, yy, px[9], py[9], box[4];  // box: x_min, x_max, y_min, y_max
+};
+
+void wls_inputs(const Tree& t, int cell, double xx, double yy, const int32_t* nodes, int m, int m1, WlsIn& in) {
+    std::memset(&in, 0, sizeof(in));
+    in.m = m;
+    in.m1 = m1;
     const double hxc = t.xf * t.h / pow2i(t.lvl[cell]), hyc = t.h / pow2i(t.lvl[cell]);
     if (t.wls_local) {
         xx = (xx - t.x[cell]) / hxc;
         yy = (yy - t.y[cell]) / hyc;
     }
+    in.xx = xx;
+    in.yy = yy;
     for (int i = 0; i < m; i++) {
-        int q = i < m1 ? l1.a[i] : l2.a[i - m1];
-        px[i] = t.x[q]; py[i] = t.y[q];
-        if (t.wls_local) { px[i] = (px[i] - t.x[cell]) / hxc; py[i] = (py[i] - t.y[cell]) / hyc; }
-        double dd = std::sqrt((px[i] - xx) * (px[i] - xx) + (py[i] - yy) * (py[i] - yy));
-        w[i] = 1.0 / (dd + 1e-6);
-        iv[WLSI_IDS + i] = state_of_node[q];
-        d[WLS_PX + i] = px[i]; d[WLS_PY + i] = py[i]; d[WLS_W + i] = w[i];
-        P[i][0] = 1.0; P[i][1] = px[i]; P[i][2] = py[i];
-        P[i][3] = px[i] * py[i]; P[i][4] = px[i] * px[i]; P[i][5] = py[i] * py[i];
+        const int q = nodes[i];
+        double px = t.x[q], py = t.y[q];
+        if (t.wls_local) { px = (px - t.x[cell]) / hxc; py = (py - t.y[cell]) / hyc; }
+        in.px[i] = px;
+        in.py[i] = py;
     }
-    iv[WLSI_M] = m; iv[WLSI_M1] = m1;
+    if (t.wls_local) {
+        in.box[0] = -0.5; in.box[1] = 0.5; in.box[2] = -0.5; in.box[3] = 0.5;
+    } else {
+        const double hx = t.xf * t.h / std::ldexp(1.0, t.lvl[cell] + 1);
+        const double hy = t.h / std::ldexp(1.0, t.lvl[cell] + 1);
+        in.box[0] = t.x[cell] - hx; in.box[1] = t.x[cell] + hx;
+        in.box[2] = t.y[cell] - hy; in.box[3] = t.y[cell] + hy;
+    }
+}
+
+// WLS-WENO geometry record of one slot (XuanGua.F90:206-255 and the box of :318-323)
+void wls_record(const WlsIn& in, double* d) {
+    const int m1 = in.m1, m = in.m;
+    const double xx = in.xx, yy = in.yy;
+    double w[16], P[16][6];
+    for (int i = 0; i < m; i++) {
+        const double px = in.px[i], py = in.py[i];
+        double dd = std::sqrt((px - xx) * (px - xx) + (py - yy) * (py - yy));
+        w[i] = 1.0 / (dd + 1e-6);
+        d[WLS_PX + i] = px; d[WLS_PY + i] = py; d[WLS_W + i] = w[i];
+        P[i][0] = 1.0; P[i][1] = px; P[i][2] = py;
+        P[i][3] = px * py; P[i][4] = px * px; P[i][5] = py * py;
+    }
     double A6[36], A3a[9], A3b[9];
     for (int r = 0; r < 6; r++)
         for (int c = 0; c < 6; c++) {
@@ -138,11 +177,7 @@ void wls_geometry(const Tree& t, int cell, double xx, double yy, const List& l1,
     brinv(3, A3a, d + WLS_INV3A);
     brinv(3, A3b, d + WLS_INV3B);
     d[WLS_XX] = xx; d[WLS_YY] = yy;
-    double hx = t.xf * t.h / std::ldexp(1.0, t.lvl[cell] + 1);
-    double hy = t.h / std::ldexp(1.0, t.lvl[cell] + 1);
-    double x_min = t.x[cell] - hx, x_max = t.x[cell] + hx;
-    double y_min = t.y[cell] - hy, y_max = t.y[cell] + hy;
-    if (t.wls_local) { x_min = -0.5; x_max = 0.5; y_min = -0.5; y_max = 0.5; }
+    const double x_min = in.box[0], x_max = in.box[1], y_min = in.box[2], y_max = in.box[3];
     d[WLS_BOX + 0] = x_max - x_min;
     d[WLS_BOX + 1] = y_max - y_min;
     d[WLS_BOX + 2] = x_max * x_max - x_min * x_min;
@@ -228,13 +263,42 @@ int stencil_lists(const Tree& t, int tg, int kind, int32_t* nb9, int32_t* m1, in
     return 0;
 }
 
+// v resized to n elements; beyond its capacity the old contents are dropped, not copied (the
+// caller writes every element)
+template <class V>
+static void resize_fresh(V& v, size_t n) {
+    if (n > v.capacity()) {
+        v.clear();
+        v.shrink_to_fit();
+        v.reserve(n + n / 8);
+    }
+    v.resize(n);
+}
+// v resized to n with every element set to val, on all host threads
+template <class V, class T>
+static void fill_par(V& v, size_t n, T val) {
+    resize_fresh(v, n);
+    parallel_for(n, [&](size_t b, size_t e) { std::fill(v.begin() + b, v.begin() + e, val); });
+}
+
 void build_tables(const Tree& t, Tables& tb) {
+    PhaseTimer pt;
     const size_t NA = t.n_alloc();
-    tb = Tables();
+    {
+        // The largest arrays keep their storage from the previous build (resized, not cleared:
+        // every element is written below), which spares the time of mapping fresh memory.
+        Tables old = std::move(tb);
+        tb = Tables();
+        tb.stencil.swap(old.stencil); tb.st_kind.swap(old.st_kind); tb.st_node.swap(old.st_node);
+        tb.st_px.swap(old.st_px); tb.st_py.swap(old.st_py); tb.off16.swap(old.off16);
+        tb.sel_loc.swap(old.sel_loc); tb.state_of_node.swap(old.state_of_node);
+        tb.leaf_node.swap(old.leaf_node); tb.node_son.swap(old.node_son); tb.lvl_leaf.swap(old.lvl_leaf);
+        tb.face_share.swap(old.face_share); tb.ns_pos.swap(old.ns_pos); tb.dev_of_ns.swap(old.dev_of_ns);
+        tb.wls_i.swap(old.wls_i);
+    }
     tb.N = (int)t.ns.size();
     tb.G = (int)t.bcl.size();
-    tb.state_of_node.assign(NA, -1);
-    tb.leaf_node.clear();
+    fill_par(tb.state_of_node, NA, -1);
     // Device order of the interior leaves: the base cells along a Morton (Z-order) curve over
     // the base grid, the leaves of each base cell in the reference's cell_NS order (son1, son2,
     // son4, son3: Morton again), so leaves that are neighbours in either direction are close in
@@ -261,38 +325,59 @@ void build_tables(const Tree& t, Tables& tb) {
         std::vector<int> order(runs.size());
         for (size_t r = 0; r < runs.size(); r++) order[r] = (int)r;
         std::sort(order.begin(), order.end(), [&](int a, int b) { return runs[a].first < runs[b].first; });
-        tb.ns_pos.resize(tb.N);
-        tb.dev_of_ns.resize(tb.N);
+        resize_fresh(tb.ns_pos, tb.N);
+        resize_fresh(tb.dev_of_ns, tb.N);
         int i = 0;
         for (int r : order)
             for (int e = runs[r].second; e < run_end[r]; e++) { tb.ns_pos[i] = e; tb.dev_of_ns[e] = i; i++; }
     }
-    for (int i = 0; i < tb.N; i++) {
-        const int c = t.ns[tb.ns_pos[i]];
-        tb.state_of_node[c] = i;
-        tb.leaf_node.push_back(c);
-    }
-    for (int i = 0; i < tb.G; i++) { tb.state_of_node[t.bcl[i]] = tb.N + i; tb.leaf_node.push_back(t.bcl[i]); }
-    // passive leaves: corners (never updated, solution.F90:644,682-686)
-    for (size_t c = 0; c < NA; c++)
-        if (t.father[c] != -2 && t.son[c] < 0 && tb.state_of_node[c] < 0) {
-            tb.state_of_node[c] = (int)tb.leaf_node.size();
-            tb.leaf_node.push_back((int)c);
+    pt.mark("tables: device order");
+    resize_fresh(tb.leaf_node, (size_t)tb.N + tb.G);
+    parallel_for((size_t)tb.N, [&](size_t b, size_t e) {
+        for (size_t i = b; i < e; i++) {
+            const int c = t.ns[tb.ns_pos[i]];
+            tb.state_of_node[c] = (int32_t)i;
+            tb.leaf_node[i] = c;
         }
+    });
+    for (int i = 0; i < tb.G; i++) { tb.state_of_node[t.bcl[i]] = tb.N + i; tb.leaf_node[(size_t)tb.N + i] = t.bcl[i]; }
+    // passive leaves: corners (never updated, solution.F90:644,682-686), in node order
+    {
+        std::vector<std::vector<int32_t>> part(64);
+        const size_t per = (NA + 63) / 64;
+        parallel_for(64, [&](size_t b, size_t e) {
+            for (size_t k = b; k < e; k++)
+                for (size_t c = k * per; c < std::min(NA, (k + 1) * per); c++)
+                    if (t.father[c] != -2 && t.son[c] < 0 && tb.state_of_node[c] < 0) part[k].push_back((int32_t)c);
+        }, 1);
+        for (auto& v : part)
+            for (int32_t c : v) {
+                tb.state_of_node[c] = (int)tb.leaf_node.size();
+                tb.leaf_node.push_back(c);
+            }
+    }
     tb.L = (int)tb.leaf_node.size();
     tb.P = tb.L - tb.N - tb.G;
-    tb.node_son.assign(t.son.begin(), t.son.end());
+    resize_fresh(tb.node_son, NA);
+    parallel_for(NA, [&](size_t b, size_t e) { std::copy(t.son.begin() + b, t.son.begin() + e, tb.node_son.begin() + b); });
 
+    pt.mark("tables: leaf numbering");
     const int N = tb.N;
-    tb.stencil.assign(8 * (size_t)N, -1);
-    tb.lvl_leaf.assign(N, 0);
-    tb.st_kind.assign(8 * (size_t)N, 0);
-    tb.st_node.assign(8 * (size_t)N, -1);
-    tb.st_px.assign(8 * (size_t)N, 0.0);
-    tb.st_py.assign(8 * (size_t)N, 0.0);
+    // every entry of these is written by the classification below
+    resize_fresh(tb.stencil, 8 * (size_t)N);
+    resize_fresh(tb.lvl_leaf, N);
+    resize_fresh(tb.st_kind, 8 * (size_t)N);
+    resize_fresh(tb.st_node, 8 * (size_t)N);
+    resize_fresh(tb.st_px, 8 * (size_t)N);
+    resize_fresh(tb.st_py, 8 * (size_t)N);
 
-    // phase 1 (parallel): classify every stencil entry as the reference's gathers do
-    // (Tip/Tim/Tjp/Tjm for k < 4, Tipp/Timm/Tjpp/Tjmm for k >= 4, XuanGua.F90:401-590)
+    pt.mark("tables: allocation");
+    // phase 1 (parallel): locate every stencil entry as the reference's gathers do
+    // (Tip/Tim/Tjp/Tjm for k < 4, Tipp/Timm/Tjpp/Tjmm for k >= 4, XuanGua.F90:401-590). A
+    // same-level leaf is a copy; a same-level parent and a coarser leaf are level jumps, whose
+    // kind (WLS-WENO, or the fallbacks when the stencil has fewer than 7 points) depends only
+    // on the jump's key: the parent node, or the coarse leaf and the point.
+    constexpr int8_t CAND_PARENT = -1, CAND_CHILD = -2;
     std::atomic<int> lmax_at{0}, missing{0};
     parallel_for((size_t)N, [&](size_t b, size_t e) {
         int lm = 0;
@@ -323,22 +408,18 @@ void build_tables(const Tree& t, Tables& tb) {
                     if (cnn < 0) { missing = 1; continue; }
                     tg = cnn; same = t.lvl[cnn] == t.lvl[cn];
                 } else { tg = cn; same = false; }
-                int kind;
-                if (same) {
-                    if (t.son[tg] < 0) kind = ST_COPY;
-                    else {
-                        List l1, l2;
-                        parent_lists(t, tg, l1, l2);
-                        xx = t.x[tg]; yy = t.y[tg];
-                        kind = l1.n + l2.n < 7 ? ST_PARENT_RESTRICT : ST_PARENT_WLS;
-                    }
-                } else {
-                    List l1, l2;
-                    child_lists(t, tg, l1, l2);
-                    kind = l1.n + l2.n < 7 ? ST_CHILD_COPY : ST_CHILD_WLS;
-                }
+                int8_t kind;
                 const size_t d = (size_t)i * 8 + k;
-                tb.st_kind[d] = (int8_t)kind;
+                if (same && t.son[tg] < 0) {
+                    kind = ST_COPY;
+                    tb.stencil[d] = tb.state_of_node[tg];
+                } else if (same) {
+                    kind = CAND_PARENT;
+                    xx = t.x[tg]; yy = t.y[tg];
+                } else {
+                    kind = CAND_CHILD;
+                }
+                tb.st_kind[d] = kind;
                 tb.st_node[d] = tg;
                 tb.st_px[d] = xx; tb.st_py[d] = yy;
             }
@@ -347,118 +428,212 @@ void build_tables(const Tree& t, Tables& tb) {
         while (lm > cur && !lmax_at.compare_exchange_weak(cur, lm)) {}
     });
     if (missing) throw std::runtime_error("interior leaf without its stencil neighbours");
+    pt.mark("tables: classify");
     const int lmax = lmax_at.load();
 
-    // phase 2 (serial, deterministic order): one slot per distinct level-jump value
-    std::unordered_map<Key, int, KeyHash> wls_map, rst_map;
+    // phase 2: the distinct level-jump keys, numbered in order of first occurrence. The jump
+    // entries are sharded by key hash over the threads; each shard finds the first entry of
+    // each of its keys, and a prefix count over the first entries numbers the keys.
+    std::vector<int64_t> jd;  // entries that are level jumps, ascending
     {
-        // size the slot map once (a level-jump value per stencil entry at most)
-        size_t jumps = 0;
-        for (size_t d = 0; d < tb.st_kind.size(); d++)
-            jumps += tb.st_kind[d] == ST_PARENT_WLS || tb.st_kind[d] == ST_CHILD_WLS;
-        wls_map.reserve(jumps);
+        const unsigned nch = 64;
+        const size_t per = ((size_t)N + nch - 1) / nch;
+        std::vector<std::vector<int64_t>> part(nch);
+        parallel_for(nch, [&](size_t b, size_t e) {
+            for (size_t ch = b; ch < e; ch++)
+                for (size_t d = ch * per * 8; d < std::min((size_t)N, (ch + 1) * per) * 8; d++)
+                    if (tb.st_kind[d] < 0) part[ch].push_back((int64_t)d);
+        }, 1);
+        size_t tot = 0;
+        for (auto& v : part) tot += v.size();
+        jd.reserve(tot);
+        for (auto& v : part) jd.insert(jd.end(), v.begin(), v.end());
     }
-    std::vector<int32_t> wls_node;      // per WLS slot: the WLS cell
-    std::vector<int8_t> wls_parent;     // per WLS slot: parent (1) or child (0) stencil
-    std::vector<double> wls_xx, wls_yy;
-    for (int i = 0; i < N; i++)
-        for (int k = 0; k < 8; k++) {
-            const size_t d = (size_t)i * 8 + k;
-            const int kind = tb.st_kind[d], tg = tb.st_node[d];
-            int ref;
-            if (kind == ST_COPY || kind == ST_CHILD_COPY) ref = tb.state_of_node[tg];
-            else if (kind == ST_PARENT_RESTRICT) {
-                Key key{2, tg, 0, 0};
-                auto it = rst_map.find(key);
-                int s;
-                if (it == rst_map.end()) { s = (int)tb.rst_node.size(); rst_map.emplace(key, s); tb.rst_node.push_back(tg); }
-                else s = it->second;
-                ref = -2 - s;  // patched to L + S_w + s below
-            } else {
-                const bool parent = kind == ST_PARENT_WLS;
-                Key key{parent ? 1 : 3, tg, parent ? 0 : bits(tb.st_px[d]), parent ? 0 : bits(tb.st_py[d])};
-                auto it = wls_map.find(key);
-                int s;
-                if (it == wls_map.end()) {
-                    s = (int)wls_node.size();
-                    wls_map.emplace(key, s);
-                    wls_node.push_back(tg);
-                    wls_parent.push_back(parent ? 1 : 0);
-                    wls_xx.push_back(tb.st_px[d]);
-                    wls_yy.push_back(tb.st_py[d]);
-                } else s = it->second;
-                ref = 1000000000 + s;  // patched to L + s below
+    pt.mark("tables: jump entries");
+    const size_t J = jd.size();
+    auto key_of = [&](size_t j) {
+        const size_t d = (size_t)jd[j];
+        const bool parent = tb.st_kind[d] == CAND_PARENT;
+        return Key{parent ? 1 : 3, tb.st_node[d], parent ? 0 : bits(tb.st_px[d]), parent ? 0 : bits(tb.st_py[d])};
+    };
+    pt.mark("tables: jump keys");
+    std::vector<int32_t> rep(J);  // first jump entry with the same key
+    {
+        auto mix = [](uint64_t z) {
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+            return z ^ (z >> 31);
+        };
+        std::vector<uint64_t> hj(J);
+        parallel_for(J, [&](size_t b, size_t e) {
+            for (size_t j = b; j < e; j++) {
+                const Key k = key_of(j);
+                hj[j] = mix(mix(mix((uint64_t)k.kind << 32 | (uint32_t)k.node) ^ k.xb) ^ k.yb);
             }
-            tb.stencil[(size_t)i * 8 + k] = ref;
-        }
-    tb.S_w = (int)wls_node.size();
-    tb.S_r = (int)tb.rst_node.size();
-    for (auto& r : tb.stencil) {
-        if (r >= 1000000000) r = tb.L + (r - 1000000000);
-        else if (r <= -2) r = tb.L + tb.S_w + (-2 - r);
+        });
+        // 16 shards by the top hash bits, each an open-addressing table of first entries
+        constexpr unsigned ns = 16;
+        parallel_for(ns, [&](size_t b, size_t e) {
+            for (size_t sh = b; sh < e; sh++) {
+                size_t cnt = 0;
+                for (size_t j = 0; j < J; j++) cnt += (hj[j] >> 60) == sh;
+                size_t cap = 16;
+                while (cap < 2 * cnt) cap <<= 1;
+                const size_t mask = cap - 1;
+                std::vector<int32_t> first(cap, -1);
+                for (size_t j = 0; j < J; j++) {
+                    if ((hj[j] >> 60) != sh) continue;
+                    size_t at = hj[j] & mask;
+                    int32_t f;
+                    while ((f = first[at]) >= 0) {
+                        if (hj[f] == hj[j] && key_of((size_t)f) == key_of(j)) break;
+                        at = (at + 1) & mask;
+                    }
+                    if (f < 0) first[at] = f = (int32_t)j;
+                    rep[j] = f;
+                }
+            }
+        }, 1);
     }
-
-    // phase 3 (parallel): stencil lists and FP64 geometry of every WLS slot
-    std::vector<double> recd((size_t)tb.S_w * WLS_STRIDE, 0.0);
-    tb.wls_i.assign((size_t)tb.S_w * WLSI_STRIDE, -1);
-    parallel_for((size_t)tb.S_w, [&](size_t b, size_t e) {
-        for (size_t s = b; s < e; s++) {
+    pt.mark("tables: dedup");
+    // per distinct key (in order): its first entry, then the kind its stencil lists give
+    std::vector<int32_t> uid(J), ufirst;
+    for (size_t j = 0; j < J; j++) {
+        if (rep[j] == (int32_t)j) { uid[j] = (int32_t)ufirst.size(); ufirst.push_back((int32_t)j); }
+        else uid[j] = uid[rep[j]];
+    }
+    pt.mark("tables: slot numbering");
+    const size_t U = ufirst.size();
+    std::vector<int8_t> ukind(U), um(U), um1(U);
+    std::vector<int32_t> unodes(U * 9, -1);
+    parallel_for(U, [&](size_t b, size_t e) {
+        for (size_t u = b; u < e; u++) {
+            const size_t d = (size_t)jd[ufirst[u]];
+            const int tg = tb.st_node[d];
+            const bool parent = tb.st_kind[d] == CAND_PARENT;
             List l1, l2;
-            const int tg = wls_node[s];
-            if (wls_parent[s]) parent_lists(t, tg, l1, l2);
+            if (parent) parent_lists(t, tg, l1, l2);
             else child_lists(t, tg, l1, l2);
-            wls_geometry(t, tg, wls_xx[s], wls_yy[s], l1, l2, tb.state_of_node, &recd[s * WLS_STRIDE],
-                         &tb.wls_i[s * WLSI_STRIDE]);
+            const int m = l1.n + l2.n;
+            ukind[u] = parent ? (m < 7 ? ST_PARENT_RESTRICT : ST_PARENT_WLS) : (m < 7 ? ST_CHILD_COPY : ST_CHILD_WLS);
+            um[u] = (int8_t)m;
+            um1[u] = (int8_t)l1.n;
+            if (m >= 7) {
+                if (m > 9) throw std::runtime_error("WLS-WENO stencil with more than 9 points");
+                for (int i = 0; i < l1.n; i++) unodes[u * 9 + i] = l1.a[i];
+                for (int i = 0; i < l2.n; i++) unodes[u * 9 + l1.n + i] = l2.a[i];
+            }
         }
     });
+    pt.mark("tables: lists");
+    // WLS slots and restriction slots, each in order of first occurrence
+    std::vector<int32_t> uref(U);
+    std::vector<int32_t> wls_u;  // per WLS slot: its key
+    for (size_t u = 0; u < U; u++) {
+        const size_t d = (size_t)jd[ufirst[u]];
+        const int tg = tb.st_node[d];
+        switch (ukind[u]) {
+            case ST_PARENT_WLS:
+            case ST_CHILD_WLS: uref[u] = 1000000000 + (int32_t)wls_u.size(); wls_u.push_back((int32_t)u); break;
+            case ST_PARENT_RESTRICT: uref[u] = -2 - (int32_t)tb.rst_node.size(); tb.rst_node.push_back(tg); break;
+            default: uref[u] = tb.state_of_node[tg];
+        }
+    }
+    tb.S_w = (int)wls_u.size();
+    tb.S_r = (int)tb.rst_node.size();
+    parallel_for(J, [&](size_t b, size_t e) {
+        for (size_t j = b; j < e; j++) {
+            const size_t d = (size_t)jd[j];
+            const int u = uid[j];
+            int32_t r = uref[u];
+            if (r >= 1000000000) r = tb.L + (r - 1000000000);
+            else if (r <= -2) r = tb.L + tb.S_w + (-2 - r);
+            tb.stencil[d] = r;
+            tb.st_kind[d] = ukind[u];
+        }
+    });
+    pt.mark("tables: slot lists");
+
     // Records by pattern: in the cell-local frame a slot's geometry (coordinates in units of
     // the cell size, weights, inverted normal matrices, smoothness box) depends only on the
     // shape of the stencil around the level jump, so slots with the same shape have
     // bit-identical records and share one, numbered in slot order; wls_i[.. + WLSI_PAT] names
-    // it. Both the FP64 records and the FP32 ones derived from them are stored once per
+    // it. The record is a function of the slot's inputs (WlsIn), so equal inputs mean equal
+    // records; both the FP64 records and the FP32 ones derived from them are stored once per
     // pattern. In the reference's absolute frame every record differs and each slot keeps its
-    // own.
+    // own. Each thread finds the distinct inputs of a run of slots; the runs are merged in slot
+    // order and the records computed once per pattern.
+    fill_par(tb.wls_i, (size_t)tb.S_w * WLSI_STRIDE, -1);
     {
-        auto same = [&](size_t a, size_t b2) {
-            const int32_t* ia = &tb.wls_i[a * WLSI_STRIDE];
-            const int32_t* ib = &tb.wls_i[b2 * WLSI_STRIDE];
-            return ia[WLSI_M] == ib[WLSI_M] && ia[WLSI_M1] == ib[WLSI_M1] &&
-                   std::memcmp(&recd[a * WLS_STRIDE], &recd[b2 * WLS_STRIDE], WLS_STRIDE * sizeof(double)) == 0;
-        };
-        auto hash_rec = [&](size_t s) {
-            uint64_t h = 1469598103934665603ull;
-            const uint32_t* w = reinterpret_cast<const uint32_t*>(&recd[s * WLS_STRIDE]);
-            for (int k = 0; k < 2 * WLS_STRIDE; k++) { h ^= w[k]; h *= 1099511628211ull; }
-            h ^= (uint64_t)tb.wls_i[s * WLSI_STRIDE + WLSI_M] * 131u + (uint64_t)tb.wls_i[s * WLSI_STRIDE + WLSI_M1];
-            return h * 1099511628211ull;
-        };
-        std::vector<uint64_t> hs(tb.S_w);
-        parallel_for((size_t)tb.S_w, [&](size_t b, size_t e) {
-            for (size_t s = b; s < e; s++) hs[s] = hash_rec(s);
-        });
-        std::unordered_map<uint64_t, std::vector<int>> buckets;
-        std::vector<int> pat_slot;  // first slot of each pattern
-        for (int s = 0; s < tb.S_w; s++) {
-            auto& bk = buckets[hs[s]];
-            int pid = -1;
-            for (int p : bk)
-                if (same((size_t)pat_slot[p], (size_t)s)) { pid = p; break; }
-            if (pid < 0) {
-                pid = (int)pat_slot.size();
-                pat_slot.push_back(s);
-                bk.push_back(pid);
+        struct InHash {
+            size_t operator()(const WlsIn& in) const {
+                uint64_t h = 1469598103934665603ull;
+                const uint64_t* w = reinterpret_cast<const uint64_t*>(&in);
+                for (size_t k = 0; k < sizeof(WlsIn) / 8; k++) { h ^= w[k]; h *= 1099511628211ull; }
+                return (size_t)(h ^ (h >> 29));
             }
-            tb.wls_i[(size_t)s * WLSI_STRIDE + WLSI_PAT] = pid;
+        };
+        struct InEq {
+            bool operator()(const WlsIn& a, const WlsIn& b) const { return std::memcmp(&a, &b, sizeof(WlsIn)) == 0; }
+        };
+        static_assert(sizeof(WlsIn) % 8 == 0, "WlsIn hashed as 64-bit words");
+        auto inputs = [&](int s, WlsIn& in) {
+            const size_t u = (size_t)wls_u[s];
+            const size_t d = (size_t)jd[ufirst[u]];
+            wls_inputs(t, tb.st_node[d], tb.st_px[d], tb.st_py[d], &unodes[u * 9], um[u], um1[u], in);
+        };
+        const size_t nrun = std::max<size_t>(1, std::min<size_t>(64, (size_t)tb.S_w / 4096));
+        const size_t per = ((size_t)tb.S_w + nrun - 1) / nrun;
+        std::vector<std::vector<int32_t>> run_first(nrun);  // first slot of each local pattern
+        std::vector<int32_t> lpat(tb.S_w);
+        parallel_for(nrun, [&](size_t b, size_t e) {
+            for (size_t r = b; r < e; r++) {
+                std::unordered_map<WlsIn, int32_t, InHash, InEq> loc;
+                WlsIn in;
+                for (size_t s = r * per; s < std::min((size_t)tb.S_w, (r + 1) * per); s++) {
+                    inputs((int)s, in);
+                    auto it = loc.emplace(in, (int32_t)run_first[r].size());
+                    if (it.second) run_first[r].push_back((int32_t)s);
+                    lpat[s] = it.first->second;
+                }
+            }
+        }, 1);
+        std::unordered_map<WlsIn, int32_t, InHash, InEq> glob;
+        std::vector<int32_t> pat_slot;  // first slot of each pattern
+        std::vector<std::vector<int32_t>> run_map(nrun);
+        for (size_t r = 0; r < nrun; r++) {
+            WlsIn in;
+            for (int32_t s0 : run_first[r]) {
+                inputs(s0, in);
+                auto it = glob.emplace(in, (int32_t)pat_slot.size());
+                if (it.second) pat_slot.push_back(s0);
+                run_map[r].push_back(it.first->second);
+            }
         }
         tb.n_pat = (int)pat_slot.size();
         tb.wls_d.assign((size_t)tb.n_pat * WLS_STRIDE, 0.0);
         tb.wls_f.assign((size_t)tb.n_pat * WF_STRIDE, 0.0f);
-        for (int p = 0; p < tb.n_pat; p++) {
-            const size_t s0 = (size_t)pat_slot[p];
-            std::memcpy(&tb.wls_d[(size_t)p * WLS_STRIDE], &recd[s0 * WLS_STRIDE], WLS_STRIDE * sizeof(double));
-            fast_record(&tb.wls_d[(size_t)p * WLS_STRIDE], &tb.wls_i[s0 * WLSI_STRIDE], &tb.wls_f[(size_t)p * WF_STRIDE]);
-        }
+        parallel_for((size_t)tb.S_w, [&](size_t b, size_t e) {
+            for (size_t s = b; s < e; s++) {
+                int32_t* iv = &tb.wls_i[s * WLSI_STRIDE];
+                const size_t u = (size_t)wls_u[s];
+                iv[WLSI_M] = um[u];
+                iv[WLSI_M1] = um1[u];
+                for (int i = 0; i < um[u]; i++) iv[WLSI_IDS + i] = tb.state_of_node[unodes[u * 9 + i]];
+                iv[WLSI_PAT] = run_map[s / per][lpat[s]];
+            }
+        });
+        parallel_for((size_t)tb.n_pat, [&](size_t b, size_t e) {
+            WlsIn in;
+            for (size_t p = b; p < e; p++) {
+                const int s0 = pat_slot[p];
+                inputs(s0, in);
+                wls_record(in, &tb.wls_d[p * WLS_STRIDE]);
+                fast_record(&tb.wls_d[p * WLS_STRIDE], &tb.wls_i[(size_t)s0 * WLSI_STRIDE], &tb.wls_f[p * WF_STRIDE]);
+            }
+        });
     }
+    pt.mark("tables: slot patterns");
     tb.bc_partner.resize(tb.G);
     for (int g = 0; g < tb.G; g++) tb.bc_partner[g] = t.bc_partner(t.bcl[g]);
     tb.l_max = lmax;
@@ -531,6 +706,7 @@ void build_tables(const Tree& t, Tables& tb) {
             if (q >= tb.N) tb.fill_flat_ok = false;
     }
 
+    pt.mark("tables: restriction + fill");
     // Ghosts whose boundary partner is an interior leaf hold an exact copy of that leaf's state
     // after every fill, and all inflow ghosts hold the same inflow state, so every table reads
     // the leaf itself, or the first inflow ghost, instead. The fills still write every ghost
@@ -554,19 +730,21 @@ void build_tables(const Tree& t, Tables& tb) {
     parallel_for(tb.stencil.size(), [&](size_t b0, size_t e0) {
         for (size_t k = b0; k < e0; k++) tb.stencil[k] = redirect(tb.stencil[k]);
     });
-    for (int s = 0; s < tb.S_w; s++) {
-        int32_t* iv = &tb.wls_i[(size_t)s * WLSI_STRIDE];
-        int gmask = 0;
-        for (int j = 0; j < iv[WLSI_M]; j++) {
-            int32_t& r = iv[WLSI_IDS + j];
-            r = redirect(r);
-            if (r >= tb.N && r < tb.N + tb.G) gmask |= 1 << j;
+    parallel_for((size_t)tb.S_w, [&](size_t b0, size_t e0) {
+        for (size_t s = b0; s < e0; s++) {
+            int32_t* iv = &tb.wls_i[s * WLSI_STRIDE];
+            int gmask = 0;
+            for (int j = 0; j < iv[WLSI_M]; j++) {
+                int32_t& r = iv[WLSI_IDS + j];
+                r = redirect(r);
+                if (r >= tb.N && r < tb.N + tb.G) gmask |= 1 << j;
+            }
+            iv[WLSI_M] |= gmask << WLSI_GSHIFT;
         }
-        iv[WLSI_M] |= gmask << WLSI_GSHIFT;
-    }
+    });
     // shared faces (the FP64 stage computes such a face once): after every redirect, so the
     // comparison is on the ids the kernels read
-    tb.face_share.assign(N, 0);
+    resize_fresh(tb.face_share, N);
     parallel_for((size_t)N, [&](size_t b0, size_t e0) {
         for (size_t ii = b0; ii < e0; ii++) {
             const int32_t* sv = &tb.stencil[ii * 8];
@@ -600,13 +778,19 @@ void build_tables(const Tree& t, Tables& tb) {
         for (int s = 0; s < tb.S_w; s++)
             if ((tb.wls_i[(size_t)s * WLSI_STRIDE + WLSI_M] >> WLSI_GSHIFT) != 0) newidx[s] = np++;
         if (tb.S_wp < tb.S_w) {
-            std::vector<int32_t> wi(tb.wls_i.size());
-            for (int s = 0; s < tb.S_w; s++)
-                std::memcpy(&wi[(size_t)newidx[s] * WLSI_STRIDE], &tb.wls_i[(size_t)s * WLSI_STRIDE],
-                            WLSI_STRIDE * sizeof(int32_t));
+            BigVec<int32_t> wi(tb.wls_i.size());
+            parallel_for((size_t)tb.S_w, [&](size_t b0, size_t e0) {
+                for (size_t s = b0; s < e0; s++)
+                    std::memcpy(&wi[(size_t)newidx[s] * WLSI_STRIDE], &tb.wls_i[s * WLSI_STRIDE],
+                                WLSI_STRIDE * sizeof(int32_t));
+            });
             tb.wls_i.swap(wi);
-            for (auto& r : tb.stencil)
-                if (r >= tb.L && r < tb.L + tb.S_w) r = tb.L + newidx[r - tb.L];
+            parallel_for(tb.stencil.size(), [&](size_t b0, size_t e0) {
+                for (size_t k = b0; k < e0; k++) {
+                    int32_t& r = tb.stencil[k];
+                    if (r >= tb.L && r < tb.L + tb.S_w) r = tb.L + newidx[r - tb.L];
+                }
+            });
         }
     }
     // the ghost points of the slots [S_wp, S_w) as fixed-size entry lists, so that the fast
@@ -644,18 +828,25 @@ void build_tables(const Tree& t, Tables& tb) {
         tb.slot_blk_pat.assign((size_t)nbs * SLOT_BLK_PATS, -1);
         tb.slot_lpat.assign(tb.S_w, 255);
         tb.slot_uncov.clear();
-        for (int b = 0; b < nbs; b++) {
-            int32_t* bp = &tb.slot_blk_pat[(size_t)b * SLOT_BLK_PATS];
-            int np = 0;
-            for (int s = b * SLOT_BLK; s < std::min(tb.S_w, (b + 1) * SLOT_BLK); s++) {
-                const int32_t pat = tb.wls_i[(size_t)s * WLSI_STRIDE + WLSI_PAT];
-                int q = 0;
-                while (q < np && bp[q] != pat) q++;
-                if (q == np && np < SLOT_BLK_PATS) bp[np++] = pat;
-                if (q < np) tb.slot_lpat[s] = (uint8_t)q;
-                else tb.slot_uncov.push_back(s);
+        std::vector<uint8_t> blk_uncov(nbs, 0);
+        parallel_for((size_t)nbs, [&](size_t b0, size_t e0) {
+            for (size_t b = b0; b < e0; b++) {
+                int32_t* bp = &tb.slot_blk_pat[b * SLOT_BLK_PATS];
+                int np = 0;
+                for (int s = (int)b * SLOT_BLK; s < std::min(tb.S_w, ((int)b + 1) * SLOT_BLK); s++) {
+                    const int32_t pat = tb.wls_i[(size_t)s * WLSI_STRIDE + WLSI_PAT];
+                    int q = 0;
+                    while (q < np && bp[q] != pat) q++;
+                    if (q == np && np < SLOT_BLK_PATS) bp[np++] = pat;
+                    if (q < np) tb.slot_lpat[s] = (uint8_t)q;
+                    else blk_uncov[b] = 1;
+                }
             }
-        }
+        }, 64);
+        for (int b = 0; b < nbs; b++)
+            if (blk_uncov[b])
+                for (int s = b * SLOT_BLK; s < std::min(tb.S_w, (b + 1) * SLOT_BLK); s++)
+                    if (tb.slot_lpat[s] == 255) tb.slot_uncov.push_back(s);
     }
     tb.merge_ok = tb.fill_flat_ok;
     for (size_t d = 0; d * 4 < tb.dep_hdr.size(); d++)
@@ -664,23 +855,40 @@ void build_tables(const Tree& t, Tables& tb) {
     for (int k = 0; k < tb.G + tb.S_r; k++)
         if (k >= tb.G || tb.ghost_alias[k] < 0) tb.fill_fast.push_back(k);
 
+    pt.mark("tables: aliases, faces, slots");
     // off-axis points of the 5x5 selector stencil
-    tb.off16.assign(16 * (size_t)N, 0);
+    resize_fresh(tb.off16, 16 * (size_t)N);
     parallel_for((size_t)N, [&](size_t b0, size_t e0) {
         for (size_t ii = b0; ii < e0; ii++) {
             const int i = (int)ii;
             const int c = tb.leaf_node[i];
             const double dy = t.h / pow2i(t.lvl[c]), dx = t.xf * dy;
+            // The points are cell centres at the leaf's level, so each lies at least half a
+            // leaf width inside every box of that level or coarser: the nearest ancestor whose
+            // box holds it is on the path from its base cell, and the descent can start there.
+            // The ancestors with their half widths, nearest first:
+            int anc[64], na = 0;
+            double ax[64], ay[64], ahx[64], ahy[64];
+            for (int f = t.father[c], k = 1; f >= 0 && na < 64; f = t.father[f], k++) {
+                anc[na] = f; ax[na] = t.x[f]; ay[na] = t.y[f];
+                ahx[na] = 0.5 * dx * (double)(1 << k); ahy[na] = 0.5 * dy * (double)(1 << k);
+                na++;
+            }
             int q = 0;
             for (int b = -2; b <= 2; b++)
                 for (int a = -2; a <= 2; a++) {
                     if (a == 0 || b == 0) continue;
                     const double px = t.x[c] + a * dx, py = t.y[c] + b * dy;
-                    int nj1 = (int)std::floor((px - t.Xmin) / (t.xf * t.h)) + 1;
-                    int ni1 = (int)std::floor((py - t.Ymin) / t.h) + 1;
-                    nj1 = std::min(std::max(nj1, 1), t.m);
-                    ni1 = std::min(std::max(ni1, 1), t.n);
-                    int node = (ni1 - 1) * t.m + nj1 - 1;
+                    int node = -1;
+                    for (int k = 0; k < na; k++)
+                        if (std::fabs(px - ax[k]) < ahx[k] && std::fabs(py - ay[k]) < ahy[k]) { node = anc[k]; break; }
+                    if (node < 0) {
+                        int nj1 = (int)std::floor((px - t.Xmin) / (t.xf * t.h)) + 1;
+                        int ni1 = (int)std::floor((py - t.Ymin) / t.h) + 1;
+                        nj1 = std::min(std::max(nj1, 1), t.m);
+                        ni1 = std::min(std::max(ni1, 1), t.n);
+                        node = (ni1 - 1) * t.m + nj1 - 1;
+                    }
                     while (t.son[node] >= 0) {
                         const bool rt = px > t.x[node], upq = py > t.y[node];
                         node = t.son[node] + (rt && upq ? 2 : rt ? 1 : upq ? 3 : 0);
@@ -690,6 +898,7 @@ void build_tables(const Tree& t, Tables& tb) {
                 }
         }
     });
+    pt.mark("tables: off16");
     // selector tile neighbourhoods (see Tables::sel_nbr)
     {
         const int ntiles = (N + 15) / 16;
@@ -701,47 +910,64 @@ void build_tables(const Tree& t, Tables& tb) {
             const int q16 = (b + 2 - (b > 0 ? 1 : 0)) * 4 + (a + 2 - (a > 0 ? 1 : 0));
             return tb.off16[(size_t)q16 * N + i];
         };
-        std::vector<std::vector<int32_t>> nbr(ntiles);
-        tb.sel_loc.assign((size_t)N * 32, 0);
-        parallel_for((size_t)ntiles, [&](size_t b0, size_t e0) {
+        // Tiles in runs; each run keeps its tiles' lists in one buffer, and the buffers are
+        // then placed at the prefix sums of the tile counts.
+        const size_t nrun = std::min<size_t>(256, std::max(1, ntiles));
+        const size_t per = ((size_t)ntiles + nrun - 1) / nrun;
+        std::vector<std::vector<int32_t>> runbuf(nrun);
+        std::vector<int32_t> tile_cnt(ntiles);
+        resize_fresh(tb.sel_loc, (size_t)N * 32);
+        parallel_for(nrun, [&](size_t r0, size_t r1) {
             // open-addressing set of at most 16 x 28 keys (1024 slots, cleared after each tile)
-            std::vector<int32_t> key(1024, -1), val(1024, 0), used;
-            for (size_t T = b0; T < e0; T++) {
-                std::vector<int32_t>& U = nbr[T];
-                U.clear();
-                const int i0 = (int)T * 16, i1 = std::min(N, i0 + 16);
-                for (int i = i0; i < i1; i++)
-                    for (int g = 0; g < 4; g++)
-                        for (int j = 0; j < 7; j++) {
-                            const int32_t src = point_src(i, g + 4 * j);
-                            uint32_t h = ((uint32_t)src * 2654435761u) >> 22;
-                            while (key[h] != -1 && key[h] != src) h = (h + 1) & 1023;
-                            if (key[h] == -1) {
-                                key[h] = src;
-                                val[h] = (int)U.size();
-                                U.push_back(src);
-                                used.push_back((int)h);
+            std::vector<int32_t> key(1024, -1), val(1024, 0), used, U;
+            for (size_t r = r0; r < r1; r++)
+                for (size_t T = r * per; T < std::min((size_t)ntiles, (r + 1) * per); T++) {
+                    U.clear();
+                    const int i0 = (int)T * 16, i1 = std::min(N, i0 + 16);
+                    for (int i = i0; i < i1; i++)
+                        for (int g = 0; g < 4; g++)
+                            for (int j = 0; j < 7; j++) {
+                                const int32_t src = point_src(i, g + 4 * j);
+                                uint32_t h = ((uint32_t)src * 2654435761u) >> 22;
+                                while (key[h] != -1 && key[h] != src) h = (h + 1) & 1023;
+                                if (key[h] == -1) {
+                                    key[h] = src;
+                                    val[h] = (int)U.size();
+                                    U.push_back(src);
+                                    used.push_back((int)h);
+                                }
+                                tb.sel_loc[(size_t)i * 32 + g * 8 + j] = (uint8_t)std::min(val[h], 255);
                             }
-                            tb.sel_loc[(size_t)i * 32 + g * 8 + j] = (uint8_t)std::min(val[h], 255);
-                        }
-                // byte 7 of every lane: the centre (point 12 = lane 0, entry 3)
-                for (int i = i0; i < i1; i++)
-                    for (int g = 0; g < 4; g++) tb.sel_loc[(size_t)i * 32 + g * 8 + 7] = tb.sel_loc[(size_t)i * 32 + 3];
-                for (int h : used) key[h] = -1;
-                used.clear();
-            }
-        });
+                    // byte 7 of every lane: the centre (point 12 = lane 0, entry 3)
+                    for (int i = i0; i < i1; i++)
+                        for (int g = 0; g < 4; g++) tb.sel_loc[(size_t)i * 32 + g * 8 + 7] = tb.sel_loc[(size_t)i * 32 + 3];
+                    for (int h : used) key[h] = -1;
+                    used.clear();
+                    if ((int)U.size() <= SEL_NBR_CAP) {
+                        runbuf[r].insert(runbuf[r].end(), U.begin(), U.end());
+                        tile_cnt[T] = (int32_t)U.size();
+                    } else {
+                        runbuf[r].resize(runbuf[r].size() + SEL_NBR_CAP + 1, 0);  // marker: gathered directly
+                        tile_cnt[T] = SEL_NBR_CAP + 1;
+                    }
+                }
+        }, 1);
         tb.sel_nbr_beg.assign(ntiles + 1, 0);
-        tb.sel_nbr.clear();
-        for (int T = 0; T < ntiles; T++) {
-            tb.sel_nbr_beg[T] = (int)tb.sel_nbr.size();
-            if ((int)nbr[T].size() <= SEL_NBR_CAP) tb.sel_nbr.insert(tb.sel_nbr.end(), nbr[T].begin(), nbr[T].end());
-            else tb.sel_nbr.resize(tb.sel_nbr.size() + SEL_NBR_CAP + 1, 0);  // marker: gathered directly
-        }
-        tb.sel_nbr_beg[ntiles] = (int)tb.sel_nbr.size();
+        for (int T = 0; T < ntiles; T++) tb.sel_nbr_beg[T + 1] = tb.sel_nbr_beg[T] + tile_cnt[T];
+        tb.sel_nbr.resize(tb.sel_nbr_beg[ntiles]);
+        std::atomic<int> bad_nbr{0};
+        parallel_for(nrun, [&](size_t r0, size_t r1) {
+            for (size_t r = r0; r < r1; r++) {
+                if (r * per >= (size_t)ntiles) continue;
+                const size_t at = (size_t)tb.sel_nbr_beg[r * per];
+                std::copy(runbuf[r].begin(), runbuf[r].end(), tb.sel_nbr.begin() + at);
+                for (int32_t q : runbuf[r])
+                    if (q < 0 || q >= tb.L + tb.S_w + tb.S_r) bad_nbr = 1;
+            }
+        }, 1);
+        if (bad_nbr) throw std::runtime_error("selector neighbourhood index out of range");
     }
-    for (int32_t r : tb.sel_nbr)
-        if (r < 0 || r >= tb.L + tb.S_w + tb.S_r) throw std::runtime_error("selector neighbourhood index out of range");
+    pt.mark("tables: selector tiles");
     // the largest tables are checked on all host threads
     std::atomic<int> bad_loc{0}, bad_off{0}, bad_alias{0};
     parallel_for(tb.sel_nbr_beg.empty() ? 0 : tb.sel_nbr_beg.size() - 1, [&](size_t b0, size_t e0) {
@@ -771,13 +997,15 @@ void build_tables(const Tree& t, Tables& tb) {
         if (k < 0 || k >= tb.G + tb.S_r) throw std::runtime_error("fast fill item out of range");
     for (int32_t p : tb.slot_blk_pat)
         if (p < -1 || p >= tb.n_pat) throw std::runtime_error("staged slot record out of range");
-    for (int s = 0; s < tb.S_w; s++) {
-        const int lp = tb.slot_lpat[s];
-        if (lp != 255 && (lp >= SLOT_BLK_PATS ||
-                          tb.slot_blk_pat[(size_t)(s / SLOT_BLK) * SLOT_BLK_PATS + lp] !=
-                              tb.wls_i[(size_t)s * WLSI_STRIDE + WLSI_PAT]))
-            throw std::runtime_error("staged slot record wrong");
-    }
+    parallel_for((size_t)tb.S_w, [&](size_t b0, size_t e0) {
+        for (size_t s = b0; s < e0; s++) {
+            const int lp = tb.slot_lpat[s];
+            if (lp != 255 && (lp >= SLOT_BLK_PATS ||
+                              tb.slot_blk_pat[(s / SLOT_BLK) * SLOT_BLK_PATS + lp] !=
+                                  tb.wls_i[s * WLSI_STRIDE + WLSI_PAT]))
+                throw std::runtime_error("staged slot record wrong");
+        }
+    });
     for (int32_t s : tb.slot_uncov)
         if (s < 0 || s >= tb.S_w || tb.slot_lpat[s] != 255) throw std::runtime_error("unstaged slot list wrong");
     for (size_t d = 0; d * 4 < tb.dep_hdr.size(); d++) {
@@ -804,30 +1032,35 @@ void build_tables(const Tree& t, Tables& tb) {
     for (int32_t r : tb.bc_ref)
         if (r != REF_INFLOW && (r >= tb.L || (r < 0 && -(r + 1) >= tb.n_int)))
             throw std::runtime_error("ghost ref out of range");
-    for (int s = 0; s < tb.S_w; s++) {
-        const int32_t* iv = &tb.wls_i[(size_t)s * WLSI_STRIDE];
-        const int m = iv[WLSI_M] & WLSI_MMASK, gmask = iv[WLSI_M] >> WLSI_GSHIFT;
-        if (m < 1 || m > 9 || iv[WLSI_M1] < 0 || iv[WLSI_M1] > m || (gmask >> m) != 0)
-            throw std::runtime_error("WLS stencil size out of range");
-        for (int i = 0; i < m; i++) {
-            const int32_t r = iv[WLSI_IDS + i];
-            if (r < 0 || r >= tb.L) throw std::runtime_error("WLS id out of range");
-            // the ghost mask names exactly the points that are ghost leaves
-            if (((gmask >> i) & 1) != (r >= tb.N && r < tb.N + tb.G)) throw std::runtime_error("WLS ghost mask wrong");
+    parallel_for((size_t)tb.S_w, [&](size_t b0, size_t e0) {
+        for (size_t s = b0; s < e0; s++) {
+            const int32_t* iv = &tb.wls_i[s * WLSI_STRIDE];
+            const int m = iv[WLSI_M] & WLSI_MMASK, gmask = iv[WLSI_M] >> WLSI_GSHIFT;
+            if (m < 1 || m > 9 || iv[WLSI_M1] < 0 || iv[WLSI_M1] > m || (gmask >> m) != 0)
+                throw std::runtime_error("WLS stencil size out of range");
+            for (int i = 0; i < m; i++) {
+                const int32_t r = iv[WLSI_IDS + i];
+                if (r < 0 || r >= tb.L) throw std::runtime_error("WLS id out of range");
+                // the ghost mask names exactly the points that are ghost leaves
+                if (((gmask >> i) & 1) != (r >= tb.N && r < tb.N + tb.G)) throw std::runtime_error("WLS ghost mask wrong");
+            }
+            if ((gmask != 0) != ((int)s >= tb.S_wp)) throw std::runtime_error("WLS slot order wrong");
+            if (iv[WLSI_PAT] < 0 || iv[WLSI_PAT] >= tb.n_pat) throw std::runtime_error("WLS record index out of range");
         }
-        if ((gmask != 0) != (s >= tb.S_wp)) throw std::runtime_error("WLS slot order wrong");
-        if (iv[WLSI_PAT] < 0 || iv[WLSI_PAT] >= tb.n_pat) throw std::runtime_error("WLS record index out of range");
-    }
+    });
     for (int32_t q : tb.rst_node)
         if (q < 0 || (size_t)q >= NA || t.son[q] < 0) throw std::runtime_error("restriction node invalid");
     for (int32_t q : tb.bc_partner)
         if (q >= 0 && ((size_t)q >= NA || t.father[q] == -2 || (t.son[q] < 0 && tb.state_of_node[q] < 0)))
             throw std::runtime_error("boundary partner invalid");
-    for (size_t c = 0; c < NA; c++) {
-        if (t.father[c] == -2) continue;
-        if (t.son[c] >= 0 && (size_t)t.son[c] + 3 >= NA) throw std::runtime_error("son index out of range");
-        if (t.son[c] < 0 && tb.state_of_node[c] < 0) throw std::runtime_error("leaf without state slot");
-    }
+    parallel_for(NA, [&](size_t b0, size_t e0) {
+        for (size_t c = b0; c < e0; c++) {
+            if (t.father[c] == -2) continue;
+            if (t.son[c] >= 0 && (size_t)t.son[c] + 3 >= NA) throw std::runtime_error("son index out of range");
+            if (t.son[c] < 0 && tb.state_of_node[c] < 0) throw std::runtime_error("leaf without state slot");
+        }
+    });
+    pt.mark("tables: checks");
 }
 
 }  // namespace fdm

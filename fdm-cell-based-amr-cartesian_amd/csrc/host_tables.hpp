@@ -68,26 +68,26 @@ struct Tables {
     int S_w = 0, S_r = 0;              // WLS slots, restriction slots
     int S_wp = 0;                      // WLS slots [0, S_wp) read no ghost leaf (the rest do)
     int l_max = 0;
-    std::vector<int32_t> state_of_node;  // node -> state index (leaves) or -1
-    std::vector<int32_t> leaf_node;      // state index -> node
+    BigVec<int32_t> state_of_node;       // node -> state index (leaves) or -1
+    BigVec<int32_t> leaf_node;           // state index -> node
     // interior leaves are stored in device order (base cells along a Morton curve); ns_pos[i]
     // is the cell_NS position of device leaf i, dev_of_ns its inverse
-    std::vector<int32_t> ns_pos, dev_of_ns;
-    std::vector<int32_t> stencil;        // [N][8]: state index, or L + slot index
+    BigVec<int32_t> ns_pos, dev_of_ns;
+    BigVec<int32_t> stencil;             // [N][8]: state index, or L + slot index
     // bit 0: the leaf's x-1/2 face reads the same four state indices as its left neighbour's
     // x+1/2 face (so both compute the same flux, bit for bit); bit 1: the same in y
-    std::vector<uint8_t> face_share;     // [N]
+    BigVec<uint8_t> face_share;          // [N]
     // the leaves whose x-1/2 face is not shared, ascending, then those whose y-1/2 face is not
     std::vector<int32_t> face_lo;
     int n_lo_x = 0, n_lo_y = 0;
-    std::vector<int8_t> lvl_leaf;        // [N]
+    BigVec<int8_t> lvl_leaf;             // [N]
     // diagnostics (host only)
-    std::vector<int8_t> st_kind;         // [N*8]
-    std::vector<int32_t> st_node;        // [N*8]
-    std::vector<double> st_px, st_py;    // [N*8]
+    BigVec<int8_t> st_kind;              // [N*8]
+    BigVec<int32_t> st_node;             // [N*8]
+    BigVec<double> st_px, st_py;         // [N*8]
     // slots
     std::vector<double> wls_d;           // [n_pat * WLS_STRIDE], one per pattern
-    std::vector<int32_t> wls_i;          // [S_w * WLSI_STRIDE]
+    BigVec<int32_t> wls_i;               // [S_w * WLSI_STRIDE]
     std::vector<float> wls_f;            // [n_pat * WF_STRIDE] fast-path records, one per pattern
     int n_pat = 0;
     std::vector<int32_t> rst_node;       // [S_r]
@@ -133,7 +133,7 @@ struct Tables {
     // the leaf's 5x5 stencil (x + a dx, y + b dy), a,b in -2..2, a != 0, b != 0, row-major
     // in b then a. New rule (the reference has no 5x5 stencil): descend from the base cell
     // into the son whose quadrant contains the point, ties to the west/south son.
-    std::vector<int32_t> off16;          // [16][N]
+    BigVec<int32_t> off16;               // [16][N]
     // Selector tiles (16 consecutive interior leaves): the distinct state indices their 5x5
     // stencils read, tile k's in sel_nbr[sel_nbr_beg[k], sel_nbr_beg[k+1]), and for every
     // leaf the positions of its points in that list: sel_loc[leaf][g][j] (uint8) is point
@@ -141,9 +141,9 @@ struct Tables {
     // lanes g of the leaf, and sel_loc[leaf][g][7] the centre's position. A tile with more than
     // SEL_NBR_CAP distinct indices stores none (count SEL_NBR_CAP + 1) and is gathered directly.
     std::vector<int32_t> sel_nbr_beg, sel_nbr;
-    std::vector<uint8_t> sel_loc;        // [N][4][8]
+    BigVec<uint8_t> sel_loc;             // [N][4][8]
     // device topology for restriction means
-    std::vector<int32_t> node_son;       // [n_alloc]
+    BigVec<int32_t> node_son;            // [n_alloc]
 };
 
 void build_tables(const Tree& t, Tables& tb);

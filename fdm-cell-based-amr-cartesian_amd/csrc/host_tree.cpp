@@ -5,9 +5,15 @@
 #include "host_tree.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <climits>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
+#include <shared_mutex>
 #include <stdexcept>
 #include <exception>
 #include <thread>
@@ -22,11 +28,16 @@ uint64_t splitmix64(uint64_t& s) {
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
     return z ^ (z >> 31);
 }
-void parallel_for(size_t n, const std::function<void(size_t, size_t)>& f) {
+// host threads of the regrid-time work: the hardware's, at most 16, or FDMAMR_HOST_THREADS
+static unsigned host_threads() {
     unsigned nt = std::thread::hardware_concurrency();
     if (const char* e = std::getenv("FDMAMR_HOST_THREADS")) nt = (unsigned)std::max(1, std::atoi(e));
-    nt = std::min<unsigned>(std::max(nt, 1u), 16u);
-    if (n < 4096 || nt == 1) { f(0, n); return; }
+    return std::min<unsigned>(std::max(nt, 1u), 16u);
+}
+
+void parallel_for(size_t n, const std::function<void(size_t, size_t)>& f, size_t serial_below) {
+    const unsigned nt = host_threads();
+    if (n < serial_below || n < 2 || nt == 1) { f(0, n); return; }
     std::vector<std::thread> th;
     std::vector<std::exception_ptr> errs(nt);
     const size_t chunk = (n + nt - 1) / nt;
@@ -40,6 +51,17 @@ void parallel_for(size_t n, const std::function<void(size_t, size_t)>& f) {
     for (auto& t : th) t.join();
     for (auto& x : errs)
         if (x) std::rethrow_exception(x);
+}
+
+static double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+PhaseTimer::PhaseTimer() : on(std::getenv("FDMAMR_TIMING") != nullptr), t0(now_ms()) {}
+void PhaseTimer::mark(const char* phase) {
+    if (!on) return;
+    const double t = now_ms();
+    std::fprintf(stderr, "[fdmamr] %-28s %9.1f ms\n", phase, t - t0);
+    t0 = t;
 }
 
 static double u01(uint64_t& s) { return (double)(splitmix64(s) >> 11) * 0x1.0p-53; }
@@ -93,17 +115,82 @@ void Tree::creat_mesh() {                           // mesh.F90:9-47
     idneighbor();
 }
 
+// Node allocation of a parallel pass. The node arrays extend past the nodes in use (`used`);
+// son blocks come from the free list or the tail, under `mu`. Workers hold `grow` shared while
+// they process cells. A worker that finds the tail exhausted releases its share and takes
+// `grow` exclusively, which waits until no other worker is inside a cell, then resizes the
+// arrays; no reference into them is held across an allocation.
+struct PassPool {
+    std::mutex mu;
+    std::shared_mutex grow;
+    size_t used = 0;
+};
+// Each worker takes son blocks from the pool a few at a time into its own cache; a worker
+// returns the blocks it did not use to the free list, marked released, at the end of the pass.
+static thread_local std::vector<int32_t>* tl_blocks = nullptr;
+constexpr int kBlockBatch = 16;
+
+void Tree::resize_nodes(size_t nn) {
+    if (nn > (size_t)INT_MAX) throw std::runtime_error("quadtree node count exceeds 2^31");
+    if (nn > father.capacity()) {
+        // with room to grow, so that further passes do not copy the arrays again
+        const size_t cap = nn + nn / 4;
+        father.reserve(cap); son.reserve(cap); lvl.reserve(cap); sp.reserve(cap); ni.reserve(cap);
+        nj.reserve(cap); cross.reserve(cap); x.reserve(cap); y.reserve(cap); U.reserve(cap);
+        rot.reserve(cap); div.reserve(cap);
+    }
+    father.resize(nn); son.resize(nn); lvl.resize(nn); sp.resize(nn); ni.resize(nn);
+    nj.resize(nn); cross.resize(nn); x.resize(nn); y.resize(nn); U.resize(nn); rot.resize(nn);
+    div.resize(nn);
+}
+
 int Tree::alloc_block() {
+    if (pool) {
+        std::vector<int32_t>& cache = *tl_blocks;
+        for (;;) {
+            if (!cache.empty()) {
+                int b = cache.back();
+                cache.pop_back();
+                return b;
+            }
+            {
+                std::lock_guard<std::mutex> g(pool->mu);
+                while (!free_blocks.empty() && cache.size() < kBlockBatch) {
+                    cache.push_back(free_blocks.back());
+                    free_blocks.pop_back();
+                }
+                while (cache.size() < kBlockBatch && pool->used + 4 <= father.size()) {
+                    cache.push_back((int32_t)pool->used);
+                    pool->used += 4;
+                }
+                if (!cache.empty()) {
+                    // hand out the lowest id first, as a serial allocation would
+                    std::reverse(cache.begin(), cache.end());
+                    continue;
+                }
+            }
+            pool->grow.unlock_shared();
+            std::exception_ptr err;
+            {
+                std::unique_lock<std::shared_mutex> ex(pool->grow);
+                try {
+                    if (pool->used + 4 > father.size())
+                        resize_nodes(father.size() + std::max<size_t>(father.size() / 8, (size_t)1 << 20));
+                } catch (...) {
+                    err = std::current_exception();
+                }
+            }
+            pool->grow.lock_shared();
+            if (err) std::rethrow_exception(err);
+        }
+    }
     if (!free_blocks.empty()) {
         int b = free_blocks.back();
         free_blocks.pop_back();
         return b;
     }
     int b = (int)father.size();
-    size_t nn = father.size() + 4;
-    father.resize(nn); son.resize(nn); lvl.resize(nn); sp.resize(nn); ni.resize(nn);
-    nj.resize(nn); cross.resize(nn); x.resize(nn); y.resize(nn); U.resize(nn); rot.resize(nn);
-    div.resize(nn);
+    resize_nodes(father.size() + 4);
     return b;
 }
 
@@ -112,8 +199,172 @@ void Tree::release_subtree(int c) {
     if (s < 0) return;
     for (int k = 0; k < 4; k++) release_subtree(s + k);
     for (int k = 0; k < 4; k++) { father[s + k] = -2; son[s + k] = -1; }
-    free_blocks.push_back(s);
     son[c] = -1;
+    if (pool) {
+        std::lock_guard<std::mutex> g(pool->mu);
+        free_blocks.push_back(s);
+    } else {
+        free_blocks.push_back(s);
+    }
+}
+
+void Tree::compact() {
+    PhaseTimer pt;
+    const size_t NA = n_alloc();
+    // nodes below each base cell, then their first new id
+    std::vector<int64_t> first((size_t)total + 1, 0);
+    parallel_for((size_t)total, [&](size_t b, size_t e) {
+        std::vector<int32_t> st;
+        for (size_t i = b; i < e; i++) {
+            int64_t cnt = 0;
+            st.assign(1, (int32_t)i);
+            while (!st.empty()) {
+                const int c = st.back();
+                st.pop_back();
+                if (son[c] < 0) continue;
+                cnt += 4;
+                for (int k = 0; k < 4; k++) st.push_back(son[c] + k);
+            }
+            first[i + 1] = cnt;
+        }
+    });
+    pt.mark("compact: count");
+    first[0] = total;
+    for (int i = 0; i < total; i++) first[i + 1] += first[i];
+    const size_t nn = (size_t)first[total];
+    if (nn > (size_t)INT_MAX) throw std::runtime_error("quadtree node count exceeds 2^31");
+    std::vector<int32_t> id(NA, -1);
+    // block pre-order below each base cell: a node's four sons, then each son's subtree
+    parallel_for((size_t)total, [&](size_t b, size_t e) {
+        std::vector<int32_t> st;
+        for (size_t i = b; i < e; i++) {
+            int32_t next = (int32_t)first[i];
+            id[i] = (int32_t)i;
+            st.assign(1, (int32_t)i);
+            while (!st.empty()) {
+                const int c = st.back();
+                st.pop_back();
+                if (son[c] < 0) continue;
+                for (int k = 0; k < 4; k++) id[son[c] + k] = next++;
+                for (int k = 3; k >= 0; k--) st.push_back(son[c] + k);
+            }
+        }
+    });
+    pt.mark("compact: ids");
+    std::vector<int32_t> old(nn);
+    parallel_for(NA, [&](size_t b, size_t e) {
+        for (size_t c = b; c < e; c++)
+            if (id[c] >= 0) old[id[c]] = (int32_t)c;
+    });
+    // in place through a spare buffer kept for the next regrid (the arrays keep their capacity)
+    auto permute = [&](auto& v, auto map) {
+        using T = typename std::decay_t<decltype(v)>::value_type;
+        if (spare_bytes < NA * sizeof(T)) {
+            spare_bytes = NA * sizeof(T) + NA * sizeof(T) / 4;
+            spare.reset(new unsigned char[spare_bytes]);  // not zeroed: first touched in parallel
+        }
+        T* src = reinterpret_cast<T*>(spare.get());
+        parallel_for(NA, [&](size_t b, size_t e) { std::memcpy(src + b, v.data() + b, (e - b) * sizeof(T)); });
+        parallel_for(nn, [&](size_t b, size_t e) {
+            for (size_t k = b; k < e; k++) v[k] = map(src[old[k]]);
+        });
+        v.resize(nn);
+    };
+    auto same = [](auto a) { return a; };
+    auto link = [&](int32_t a) { return a < 0 ? a : id[a]; };
+    permute(father, link);
+    permute(son, link);
+    permute(lvl, same); permute(sp, same); permute(ni, same); permute(nj, same); permute(cross, same);
+    pt.mark("compact: links");
+    permute(x, same); permute(y, same); permute(U, same); permute(rot, same); permute(div, same);
+    pt.mark("compact: values");
+    free_blocks.clear();
+    nb.clear();
+}
+
+// Run f(cell) for the base cells on host threads, front by front: front T holds the cells of
+// row r and column c (cell r*m + c) with c + skew * r == T, and starts when the previous front
+// has finished; skew < 0 makes fronts of consecutive cells. Cells of one front must neither
+// write what another reads nor read what another writes; then every cell sees the tree exactly
+// as in a serial loop over the cells in index order. New son blocks come from a PassPool whose
+// tail starts with room for `room` nodes.
+template <class F>
+static void run_fronts(Tree& t, int skew, unsigned nt, size_t room, const F& f) {
+    const int m = t.m, n = t.n;
+    constexpr int kRun = 4096;  // cells per front when skew < 0
+    const int nfront = skew < 0 ? (t.total + kRun - 1) / kRun : (m - 1) + skew * (n - 1) + 1;
+    const int chunk = skew < 0 ? 32 : 4;
+    PassPool pool;
+    pool.used = t.n_alloc();
+    t.resize_nodes(pool.used + std::max(room, (size_t)1 << 16));
+    t.pool = &pool;
+    std::atomic<int> next{0}, arrived{0}, gen{0};
+    std::atomic<bool> abort{false};
+    std::vector<std::exception_ptr> errs(nt);
+    auto worker = [&](unsigned w) {
+        std::vector<int32_t> cache;
+        tl_blocks = &cache;
+        for (int T = 0; T < nfront; T++) {
+            int r0 = 0, count = std::min(kRun, t.total - T * kRun);
+            if (skew >= 0) {
+                r0 = T > m - 1 ? (T - (m - 1) + skew - 1) / skew : 0;
+                const int r1 = std::min(n - 1, T / skew);
+                count = std::max(0, r1 - r0 + 1);
+            }
+            pool.grow.lock_shared();
+            if (!abort.load(std::memory_order_relaxed)) {
+                try {
+                    for (;;) {
+                        const int k0 = next.fetch_add(chunk, std::memory_order_relaxed);
+                        if (k0 >= count) break;
+                        for (int k = k0; k < std::min(count, k0 + chunk); k++) {
+                            if (skew < 0) f(T * kRun + k);
+                            else {
+                                const int r = r0 + k;
+                                f(r * m + (T - skew * r));
+                            }
+                        }
+                    }
+                } catch (...) {
+                    errs[w] = std::current_exception();
+                    abort = true;
+                }
+            }
+            pool.grow.unlock_shared();
+            // barrier: the last worker to arrive resets the cell counter and opens the next front
+            const int g = gen.load(std::memory_order_acquire);
+            if (arrived.fetch_add(1, std::memory_order_acq_rel) == (int)nt - 1) {
+                arrived.store(0, std::memory_order_relaxed);
+                next.store(0, std::memory_order_relaxed);
+                gen.fetch_add(1, std::memory_order_release);
+            } else {
+                for (int spin = 0; gen.load(std::memory_order_acquire) == g; spin++)
+                    if (spin > 2000) std::this_thread::yield();
+            }
+        }
+        {
+            std::lock_guard<std::mutex> g(pool.mu);
+            for (int32_t b : cache) {
+                for (int k = 0; k < 4; k++) { t.father[b + k] = -2; t.son[b + k] = -1; }
+                t.free_blocks.push_back(b);
+            }
+        }
+        tl_blocks = nullptr;
+    };
+    std::vector<std::thread> th;
+    for (unsigned w = 1; w < nt; w++) th.emplace_back(worker, w);
+    worker(0);
+    for (auto& x : th) x.join();
+    t.pool = nullptr;
+    t.resize_nodes(pool.used);
+    for (auto& e : errs)
+        if (e) std::rethrow_exception(e);
+}
+
+// parallel passes only on meshes with enough base cells to share out
+static unsigned pass_threads(const Tree& t) {
+    const unsigned nt = host_threads();
+    return t.total >= 4096 ? nt : 1;
 }
 
 void Tree::refine(int c) {                          // mesh.F90:199-260
@@ -134,6 +385,10 @@ void Tree::refine(int c) {                          // mesh.F90:199-260
         U[q] = U[c];
         rot[q] = rot[c]; div[q] = div[c];
     }
+    if (track_levels) {
+        int8_t& hi = cell_lmax[(size_t)(ni[c] - 1) * m + nj[c] - 1];
+        hi = std::max<int8_t>(hi, (int8_t)(lvl[c] + 1));
+    }
 }
 
 void Tree::restrict_mean(int c, double* var) const { // XuanGua.F90:359-398
@@ -152,6 +407,30 @@ void Tree::coarse(int c) {                          // mesh.F90:397-408
     restrict_mean(c, v);
     for (int i = 0; i < 7; i++) U[c][i] = v[i];
     release_subtree(c);
+    if (track_levels) {
+        int8_t& lo = cell_lmin[(size_t)(ni[c] - 1) * m + nj[c] - 1];
+        lo = std::min<int8_t>(lo, (int8_t)lvl[c]);
+    }
+}
+
+void Tree::leaf_levels() {
+    cell_lmin.assign(total, 127);
+    cell_lmax.assign(total, 0);
+    parallel_for((size_t)total, [&](size_t b, size_t e) {
+        std::vector<int32_t> st;
+        for (size_t i = b; i < e; i++) {
+            int lo = 127, hi = 0;
+            st.assign(1, (int32_t)i);
+            while (!st.empty()) {
+                const int c = st.back();
+                st.pop_back();
+                if (son[c] < 0) { lo = std::min(lo, lvl[c]); hi = std::max(hi, lvl[c]); continue; }
+                for (int k = 0; k < 4; k++) st.push_back(son[c] + k);
+            }
+            cell_lmin[i] = (int8_t)lo;
+            cell_lmax[i] = (int8_t)hi;
+        }
+    });
 }
 
 // ---------------------------------------------------------------- neighbours
@@ -290,11 +569,27 @@ void Tree::bc_walk(int c, bool fill, int& cnt) {    // solution.F90:634-693
     }
 }
 void Tree::idgrid() {
-    int a = 0, b = 0;
-    for (int i = 0; i < total; i++) { ns_walk(i, false, a); bc_walk(i, false, b); }
-    ns.assign(a, -1); bcl.assign(b, -1);
-    a = b = 0;
-    for (int i = 0; i < total; i++) { ns_walk(i, true, a); bc_walk(i, true, b); }
+    // per base cell counts, their prefix sums, then each base cell's walk writes its own range
+    std::vector<int> na((size_t)total + 1, 0), nbc((size_t)total + 1, 0);
+    parallel_for((size_t)total, [&](size_t b0, size_t e0) {
+        for (size_t i = b0; i < e0; i++) {
+            int a = 0, b = 0;
+            ns_walk((int)i, false, a);
+            bc_walk((int)i, false, b);
+            na[i + 1] = a;
+            nbc[i + 1] = b;
+        }
+    });
+    for (int i = 0; i < total; i++) { na[i + 1] += na[i]; nbc[i + 1] += nbc[i]; }
+    ns.assign(na[total], -1);
+    bcl.assign(nbc[total], -1);
+    parallel_for((size_t)total, [&](size_t b0, size_t e0) {
+        for (size_t i = b0; i < e0; i++) {
+            int a = na[i], b = nbc[i];
+            ns_walk((int)i, true, a);
+            bc_walk((int)i, true, b);
+        }
+    });
 }
 
 // ---------------------------------------------------------------- boundaries
@@ -415,24 +710,42 @@ void Tree::ref_sub(int c, double arot, double adiv, int lvl_amr, double rg_rot,
 }
 void Tree::solute_ref(double arot, double adiv, int lvl_amr, double rg_rot, double rg_div,
                       double cg_rot, double cg_div) { // solution.F90:468-488
-    for (int i = 0; i < total; i++) {
+    auto cell = [&](int i) {
         if (cross[i] == 0 || cross[i] == 1) ref_sub(i, arot, adiv, lvl_amr, rg_rot, rg_div);
         if (son[i] >= 0) coarse_sub(i, arot, adiv, lvl_amr, cg_rot, cg_div);
+    };
+    // each base cell reads and writes only its own subtree: any order gives the same tree
+    const unsigned nt = pass_threads(*this);
+    if (nt > 1) {
+        // ref_sub refines each leaf below lvl_amr above the goals at most once (its new sons
+        // are not visited), and coarse_sub only releases: that bounds the new nodes
+        std::atomic<size_t> refines{0};
+        parallel_for(n_alloc(), [&](size_t b, size_t e) {
+            size_t k = 0;
+            for (size_t c = b; c < e; c++)
+                if (father[c] != -2 && son[c] < 0 && lvl[c] < lvl_amr &&
+                    (rot[c] > rg_rot * arot || div[c] > rg_div * adiv))
+                    k++;
+            refines += k;
+        });
+        run_fronts(*this, -1, nt, 4 * refines.load(), cell);
+    } else {
+        for (int i = 0; i < total; i++) cell(i);
     }
 }
 
 void Tree::gridmodify1(int c) {                     // mesh.F90:305-353
     if (son[c] >= 0) for (int k = 0; k < 4; k++) gridmodify1(son[c] + k);
     if (cross[c] != 0) return;
-    int jp = up(c), jm = down(c), im = left(c), ip = right(c);
-    bool upS = jp >= 0 && son[jp] >= 0, dnS = jm >= 0 && son[jm] >= 0;
-    bool lfS = im >= 0 && son[im] >= 0, rtS = ip >= 0 && son[ip] >= 0;
+    // the finders only read the tree, so each is evaluated only when the outcome needs it
+    auto has_sons = [&](int q) { return q >= 0 && son[q] >= 0; };
     if (son[c] < 0) {
-        if ((lfS && rtS) || (upS && dnS)) refine(c);
+        if ((has_sons(left(c)) && has_sons(right(c))) || (has_sons(up(c)) && has_sons(down(c)))) refine(c);
     } else {
         int s = son[c];
         if (son[s] < 0 && son[s + 1] < 0 && son[s + 2] < 0 && son[s + 3] < 0)
-            if ((!lfS && !rtS) || (!upS && !dnS)) coarse(c);
+            if ((!has_sons(left(c)) && !has_sons(right(c))) || (!has_sons(up(c)) && !has_sons(down(c))))
+                coarse(c);
     }
 }
 void Tree::gridmodify4(int c) {                     // mesh.F90:355-395
@@ -449,10 +762,56 @@ void Tree::solute_gridmodify(int lvl_amr) {         // solution.F90:537-556
     // The reference refreshes the cached neighbour pointers after each pass (idneighbor),
     // but gridmodify1 and gridmodify4 search neighbours afresh through father/son links and
     // never read the cache, so only the final refresh (by the caller) can be observed.
+    // In parallel, the base cells run in fronts. gridmodify1 on a base cell writes its own
+    // subtree and reads those of the four cells beside it: cells that differ in both row and
+    // column do not interact, so fronts c + r = T keep every interaction in the serial order.
+    // gridmodify4 also refines the neighbour's neighbour: it reads and writes the cells up to
+    // two away in a row or a column, and cells (r, c), (r', c') interact when they share a row
+    // or column within four, or |r - r'|, |c - c'| <= 2; fronts c + 3 r = T keep those in order.
+    // The node numbering then depends on the threads' timing, so the tree is renumbered.
+    //
+    // A base cell whose leaves and those of the four cells beside it are all on one level
+    // gives gridmodify1 nothing to do (each leaf's neighbours are leaves of its own level, each
+    // parent of leaves has parents of leaves beside it), and a base cell whose deepest leaf is
+    // at level L, with no leaf below level L - 1 in it or beside it, gives gridmodify4 nothing
+    // (a leaf at level L >= 2 finds its neighbour and the neighbour's neighbour within the cells
+    // beside its own); the parallel passes skip such cells, by the level bounds of each cell.
+    const unsigned nt = pass_threads(*this);
+    PhaseTimer pt;
+    auto span = [&](int i, int& lo, int& hi) {
+        const int r = i / m, c = i % m;
+        lo = cell_lmin[i]; hi = cell_lmax[i];
+        const int nbr[4] = {c > 0 ? i - 1 : -1, c < m - 1 ? i + 1 : -1, r > 0 ? i - m : -1, r < n - 1 ? i + m : -1};
+        for (int j : nbr)
+            if (j >= 0) { lo = std::min<int>(lo, cell_lmin[j]); hi = std::max<int>(hi, cell_lmax[j]); }
+    };
     for (int k = 1; k <= lvl_amr; k++) {
-        for (int i = 0; i < total; i++) gridmodify1(i);
-        for (int i = 0; i < total; i++) gridmodify4(i);
+        if (nt > 1) {
+            track_levels = true;
+            leaf_levels();
+            run_fronts(*this, 1, nt, n_alloc() / 16, [&](int i) {
+                int lo, hi;
+                span(i, lo, hi);
+                if (lo != hi) gridmodify1(i);
+            });
+            pt.mark("gridmodify1");
+            leaf_levels();
+            run_fronts(*this, 3, nt, n_alloc() / 16, [&](int i) {
+                int lo, hi;
+                span(i, lo, hi);
+                if (cell_lmax[i] >= 2 && lo < cell_lmax[i] - 1) gridmodify4(i);
+            });
+            track_levels = false;
+            pt.mark("gridmodify4");
+        } else {
+            for (int i = 0; i < total; i++) gridmodify1(i);
+            pt.mark("gridmodify1");
+            for (int i = 0; i < total; i++) gridmodify4(i);
+            pt.mark("gridmodify4");
+        }
     }
+    compact();
+    pt.mark("compact");
 }
 
 // ---------------------------------------------------------------- initial conditions

@@ -9,10 +9,34 @@
 #include <array>
 #include <cstdint>
 #include <functional>
+#include <memory>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 namespace fdm {
+
+struct PassPool;  // node allocation of a parallel pass over the base cells (host_tree.cpp)
+
+// Allocator whose resize leaves new elements uninitialised, for the node, per-leaf and
+// per-entry arrays whose every element is written before it is read: growing them costs
+// nothing, and fresh memory is first touched by the host threads that fill it.
+template <class T>
+struct NoInit : std::allocator<T> {
+    template <class U>
+    struct rebind { using other = NoInit<U>; };
+    NoInit() = default;
+    template <class U>
+    NoInit(const NoInit<U>&) {}
+    template <class U>
+    void construct(U* p) noexcept(std::is_nothrow_default_constructible<U>::value) { ::new ((void*)p) U; }
+    template <class U, class... A>
+    void construct(U* p, A&&... a) { ::new ((void*)p) U(std::forward<A>(a)...); }
+};
+template <class T>
+using BigVec = std::vector<T, NoInit<T>>;
+
 
 enum Nb { NB_UP = 0, NB_DOWN, NB_RIGHT, NB_LEFT, NB_UPLEFT, NB_UPRIGHT, NB_DOWNLEFT, NB_DOWNRIGHT };
 
@@ -28,10 +52,10 @@ struct Tree {
     int m = 0, n = 0, total = 0;
 
     // ---- node arrays ----
-    std::vector<int32_t> father, son, lvl, sp, ni, nj, cross;
-    std::vector<double> x, y;
-    std::vector<std::array<double, 7>> U;           // U(1:7) of every node
-    std::vector<double> rot, div;
+    BigVec<int32_t> father, son, lvl, sp, ni, nj, cross;
+    BigVec<double> x, y;
+    BigVec<std::array<double, 7>> U;                // U(1:7) of every node
+    BigVec<double> rot, div;
     std::vector<int32_t> nb;                        // 8 cached neighbours per node (idnei)
     std::vector<int32_t> free_blocks;               // first ids of released son blocks
 
@@ -48,6 +72,18 @@ struct Tree {
     void refine(int c);                             // mesh.F90:199-260
     void coarse(int c);                             // mesh.F90:397-408
     void release_subtree(int c);
+    // Renumber the nodes canonically (base cells, then son blocks in block pre-order, as
+    // canonical_ids) and drop released blocks; the cached neighbours must be rebuilt after.
+    void compact();
+    void resize_nodes(size_t nn);
+    PassPool* pool = nullptr;                       // set while a parallel pass runs
+    std::shared_ptr<unsigned char[]> spare;         // compact()'s staging buffer, kept
+    size_t spare_bytes = 0;
+    // per base cell: bounds on the levels of the leaves below it (a lower bound on the least,
+    // an upper bound on the greatest), kept by refine and coarse while track_levels is set
+    std::vector<int8_t> cell_lmin, cell_lmax;
+    bool track_levels = false;
+    void leaf_levels();
 
     // neighbour finders (neighbor.F90:6-396); -1 = null
     int init_up(int c) const;
@@ -99,7 +135,17 @@ struct Tree {
 // SplitMix64 shared by the IC generators
 uint64_t splitmix64(uint64_t& s);
 
-// Run f(begin, end) over [0, n) split across host threads (regrid-time work only).
-void parallel_for(size_t n, const std::function<void(size_t, size_t)>& f);
+// Run f(begin, end) over [0, n) split across host threads (regrid-time work only); ranges
+// shorter than serial_below run on the calling thread.
+void parallel_for(size_t n, const std::function<void(size_t, size_t)>& f, size_t serial_below = 4096);
+
+// Wall time of the phases of a regrid / table build, printed to stderr when the environment
+// variable FDMAMR_TIMING is set (diagnostics).
+struct PhaseTimer {
+    PhaseTimer();
+    void mark(const char* phase);
+    bool on;
+    double t0;
+};
 
 }  // namespace fdm

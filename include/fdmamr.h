@@ -160,7 +160,9 @@ int32_t fdmamr_regrid(fdmamr_ctx* ctx, int32_t lvl_amr, double refine_rot, doubl
 /* The same AMR body with caller-supplied sensors instead of rot_div's: rot/div per interior
  * leaf in cell_NS order, already scaled by dy**1.5 as fdmamr_sensors returns them (the
  * reference's c%rot, c%div read by solute_ref, solution.F90:468-535). A driver that computes
- * its own sensors, or that replays another run's regrid decisions on the same mesh, uses it. */
+ * its own sensors, or that replays another run's regrid decisions on the same mesh, uses it.
+ * On a host-only context (FDMAMR_DEVICE_HOST_ONLY) it regrids the host tree and rebuilds the
+ * host tables: the leaves keep the values refine and coarse give them (no ghost refill). */
 int32_t fdmamr_regrid_sensors(fdmamr_ctx* ctx, int32_t lvl_amr, double refine_rot, double refine_div,
                               double coarse_rot, double coarse_div, int32_t sensor_mode, const double* rot,
                               const double* div);

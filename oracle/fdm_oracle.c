@@ -1312,6 +1312,37 @@ int32_t orc_stencil_value(const Orc* o, int32_t i, int32_t k, double* var7) {
     return 0;
 }
 
+/* amr_regrid with the leaves' sensors given (cell_NS order) instead of computed by rot_div:
+ * the sums Srot, Sdiv, Ne in rot_div's visiting order (solution.F90:424-466), solute_ref,
+ * gridmodify, idneighbor, IDgrid (solution.F90:413-419). amr_sensor 1 tests div only, as
+ * sensors_leaf does. The ghosts are not refilled. */
+static void sum_given(Orc* o, Node* c) {
+    if (!c->son1 && (c->cross == 0 || c->cross == 1)) {
+        if (o->p.amr_sensor == 1) c->rot = c->div;
+        o->Srot = o->Srot + c->rot * c->rot;
+        o->Sdiv = o->Sdiv + c->div * c->div;
+        o->Ne = o->Ne + 1;
+    } else if (c->son1) {
+        sum_given(o, c->son1); sum_given(o, c->son2); sum_given(o, c->son3); sum_given(o, c->son4);
+    }
+}
+int32_t orc_regrid_sensors(Orc* o, const double* rot, const double* div) {
+    for (int i = 0; i < o->nResgrid; i++) {
+        o->cell_NS[i]->rot = rot[i];
+        o->cell_NS[i]->div = div[i];
+    }
+    o->Srot = 0; o->Sdiv = 0; o->Ne = 0;
+    for (int i = 0; i < o->total; i++) {
+        Node* c = &o->cell[i];
+        if (c->cross == 0 || c->cross == 1 || c->cross == 2) sum_given(o, c);
+    }
+    solute_ref(o);
+    gridmodify_all(o);
+    idneighbor(o);
+    IDgrid(o);
+    return 0;
+}
+
 int32_t orc_sensors(Orc* o, double* rot, double* div, double* s2) {
     rot_div(o);
     for (int i = 0; i < o->nResgrid; i++) {

@@ -101,6 +101,8 @@ int32_t orc_stencil_value(const Orc* o, int32_t i, int32_t k, double* var7);
 /* per-leaf vorticity / density-gradient sensors (solution.F90:424-466,558-580), cell_NS
  * order, computed on the current state; returns Ne and writes Srot,Sdiv */
 int32_t orc_sensors(Orc* o, double* rot, double* div, double* s_rot_div);
+/* amr_regrid with the interior leaves' sensors given in cell_NS order (no ghost refill) */
+int32_t orc_regrid_sensors(Orc* o, const double* rot, const double* div);
 
 /* ---- kernel-level restatements (pure functions) ---- */
 void    orc_wcns_z3(const double* u1, const double* u2, const double* u3, const double* R,

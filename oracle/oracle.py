@@ -102,6 +102,8 @@ def lib():
         L.orc_set_threads.restype = C.c_int32
         L.orc_refine.argtypes = [P, C.c_int32]
         L.orc_refine.restype = C.c_int32
+        L.orc_regrid_sensors.argtypes = [P, dp, dp]
+        L.orc_regrid_sensors.restype = C.c_int32
         L.orc_rebuild.argtypes = [P, C.c_int32]
         L.orc_rebuild.restype = C.c_int32
         L.orc_set_node_values.argtypes = [P, dp]
@@ -279,6 +281,13 @@ class Oracle:
 
     def rk3_range(self, i0, i1, dt, lam_x, lam_y):
         assert self.L.orc_rk3_range(self.h, i0, i1, dt, lam_x, lam_y) == 0
+
+    def regrid_sensors(self, rot, div):
+        """amr_regrid with the interior leaves' sensors given (cell_NS order)."""
+        rot = np.ascontiguousarray(rot, np.float64)
+        div = np.ascontiguousarray(div, np.float64)
+        assert rot.shape == div.shape == (self.n_leaf,)
+        assert self.L.orc_regrid_sensors(self.h, rot, div) == 0
 
     def sensors(self):
         N = self.n_leaf
