@@ -173,6 +173,42 @@ hipError_t upload_staged(fdmamr_ctx* c, void* dst, const void* src, size_t bytes
     return hipSuccess;
 }
 
+// Device-to-host copy of a large array through the same staging buffers: the DMA engine fills
+// one buffer while host threads empty the other. Synchronises the context's stream.
+hipError_t download_staged(fdmamr_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return hipStreamSynchronize(c->stream);
+    if (bytes < ((size_t)4 << 20)) {
+        hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream);
+        return e == hipSuccess ? hipStreamSynchronize(c->stream) : e;
+    }
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 2 && e == hipSuccess; k++) {
+        if (!c->stage_buf[k]) e = hipHostMalloc(&c->stage_buf[k], STAGE_BYTES, hipHostMallocDefault);
+        if (e == hipSuccess && !c->stage_ev[k]) e = hipEventCreateWithFlags(&c->stage_ev[k], hipEventDisableTiming);
+        if (e == hipSuccess && c->stage_pending[k]) e = hipEventSynchronize(c->stage_ev[k]);
+        c->stage_pending[k] = false;
+    }
+    if (e != hipSuccess) return e;
+    const size_t nchunk = (bytes + STAGE_BYTES - 1) / STAGE_BYTES;
+    auto issue = [&](size_t ch) -> hipError_t {
+        const size_t off = ch * STAGE_BYTES, n = std::min(STAGE_BYTES, bytes - off);
+        hipError_t r = hipMemcpyAsync(c->stage_buf[ch & 1], static_cast<const unsigned char*>(src) + off, n,
+                                      hipMemcpyDeviceToHost, c->stream);
+        return r == hipSuccess ? hipEventRecord(c->stage_ev[ch & 1], c->stream) : r;
+    };
+    e = issue(0);
+    for (size_t ch = 0; ch < nchunk && e == hipSuccess; ch++) {
+        if (ch + 1 < nchunk) e = issue(ch + 1);
+        if (e == hipSuccess) e = hipEventSynchronize(c->stage_ev[ch & 1]);
+        if (e != hipSuccess) break;
+        const size_t off = ch * STAGE_BYTES, n = std::min(STAGE_BYTES, bytes - off);
+        const unsigned char* buf = static_cast<const unsigned char*>(c->stage_buf[ch & 1]);
+        unsigned char* to = static_cast<unsigned char*>(dst) + off;
+        parallel_for(n, [&](size_t b, size_t e2) { std::memcpy(to + b, buf + b, e2 - b); }, (size_t)1 << 20);
+    }
+    return e == hipSuccess ? hipStreamSynchronize(c->stream) : e;
+}
+
 int upload_tables(fdmamr_ctx* c) {
     Tree& t = c->tree;
     Tables& tb = c->tb;
@@ -335,21 +371,20 @@ void finish_steps(fdmamr_ctx* c) {
 int pull_state(fdmamr_ctx* c) {
     Tables& tb = c->tb;
     const int n = tb.N + tb.G;
-    std::vector<double4> q(n);
-    std::vector<double> tv(n);
-    if (n > 0) {
-        HIPCHK(c, hipMemcpyAsync(q.data(), c->Q[0].p, n * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipMemcpyAsync(tv.data(), c->T[0].p, n * 8, hipMemcpyDeviceToHost, c->stream));
-    }
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    BigVec<double4> q(n);
+    BigVec<double> tv(n);
     const bool fast = c->p.flux_precision == FDMAMR_FLUX_FP32;
+    HIPCHK(c, download_staged(c, q.data(), c->Q[0].p, (size_t)n * sizeof(double4)));
+    if (!fast) HIPCHK(c, download_staged(c, tv.data(), c->T[0].p, (size_t)n * 8));
     const double gm = c->tree.gamma * c->tree.Ma * c->tree.Ma;
-    for (int i = 0; i < n; i++) {
-        auto& u = c->tree.U[tb.leaf_node[i]];
-        u[0] = q[i].x; u[1] = q[i].y; u[2] = q[i].z; u[4] = q[i].w;
-        // the fast path does not carry the passive T: recompute it from the equation of state
-        u[3] = fast ? q[i].w * gm / q[i].x : tv[i];
-    }
+    parallel_for((size_t)n, [&](size_t b, size_t e) {
+        for (size_t i = b; i < e; i++) {
+            auto& u = c->tree.U[tb.leaf_node[i]];
+            u[0] = q[i].x; u[1] = q[i].y; u[2] = q[i].z; u[4] = q[i].w;
+            // the fast path does not carry the passive T: recompute it from the equation of state
+            u[3] = fast ? q[i].w * gm / q[i].x : tv[i];
+        }
+    });
     return 0;
 }
 
@@ -860,23 +895,23 @@ int32_t fdmamr_sensors(fdmamr_ctx* c, double* rot, double* div) {
     launch_slots(c->dt, c->Q[0].p, c->T[0].p, c->stream);
     if (N > 0) launch_sensors(c->dt, c->Q[0].p, c->rot.p, c->div.p, c->stream);
     HIPCHK(c, hipGetLastError());
-    std::vector<double> a(N), b(N);
-    if (N > 0) {
-        HIPCHK(c, hipMemcpyAsync(a.data(), c->rot.p, N * 8, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipMemcpyAsync(b.data(), c->div.p, N * 8, hipMemcpyDeviceToHost, c->stream));
-    }
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    BigVec<double> a(N), b(N);
+    HIPCHK(c, download_staged(c, a.data(), c->rot.p, (size_t)N * 8));
+    HIPCHK(c, download_staged(c, b.data(), c->div.p, (size_t)N * 8));
     Tree& t = c->tree;
-    for (int i = 0; i < N; i++) {
-        const int k = c->tb.leaf_node[i];
-        double dy = t.h / (double)(1 << t.lvl[k]);
-        double f = std::pow(dy, 1.5);                 // dy**1.5, solution.F90:565,578
-        t.rot[k] = a[i] * f;
-        t.div[k] = b[i] * f;
-        const int e = c->tb.ns_pos[i];                // outputs in cell_NS order
-        if (rot) rot[e] = t.rot[k];
-        if (div) div[e] = t.div[k];
-    }
+    double f_of_lvl[64];
+    for (int l = 0; l < 64; l++) f_of_lvl[l] = l < 31 ? std::pow(t.h / (double)(1 << l), 1.5) : 0.0;  // dy**1.5, solution.F90:565,578
+    parallel_for((size_t)N, [&](size_t b0, size_t e0) {
+        for (size_t i = b0; i < e0; i++) {
+            const int k = c->tb.leaf_node[i];
+            const double f = f_of_lvl[t.lvl[k]];
+            t.rot[k] = a[i] * f;
+            t.div[k] = b[i] * f;
+            const int e = c->tb.ns_pos[i];                // outputs in cell_NS order
+            if (rot) rot[e] = t.rot[k];
+            if (div) div[e] = t.div[k];
+        }
+    });
     return 0;
 }
 
@@ -955,8 +990,10 @@ int32_t fdmamr_stencil_info(fdmamr_ctx* c, int32_t i, int32_t* kind8, int32_t* n
         size_t d = (size_t)dev * 8 + k;
         if (kind8) kind8[k] = tb.st_kind[d];
         if (node8) node8[k] = id[tb.st_node[d]];
-        if (px8) px8[k] = tb.st_px[d];
-        if (py8) py8[k] = tb.st_py[d];
+        double px, py;
+        stencil_point(c->tree, tb.leaf_node[dev], k, tb.st_kind[d], tb.st_node[d], &px, &py);
+        if (px8) px8[k] = px;
+        if (py8) py8[k] = py;
         int32_t nb9[9], a = 0, b = 0;
         stencil_lists(c->tree, tb.st_node[d], tb.st_kind[d], nb9, &a, &b);
         if (m1_8) m1_8[k] = a;

@@ -251,6 +251,25 @@ uint64_t bits(double v) { uint64_t b; std::memcpy(&b, &v, 8); return b; }
 
 }  // namespace
 
+void stencil_point(const Tree& t, int c, int k, int kind, int tg, double* px, double* py) {
+    if (kind == ST_PARENT_WLS || kind == ST_PARENT_RESTRICT) {
+        *px = t.x[tg];
+        *py = t.y[tg];
+        return;
+    }
+    const int dir = k & 3;
+    const bool two = k >= 4;
+    const double off = two ? (dir < 2 ? 2.0 * t.xf * t.h : 2.0 * t.h) / pow2i(t.lvl[c])
+                           : (dir < 2 ? t.xf * t.h : t.h) / pow2i(t.lvl[c]);
+    double xx = t.x[c], yy = t.y[c];
+    if (dir == 0) xx = t.x[c] + off;
+    else if (dir == 1) xx = t.x[c] - off;
+    else if (dir == 2) yy = t.y[c] + off;
+    else yy = t.y[c] - off;
+    *px = xx;
+    *py = yy;
+}
+
 int stencil_lists(const Tree& t, int tg, int kind, int32_t* nb9, int32_t* m1, int32_t* m2) {
     List l1, l2;
     if (kind == ST_PARENT_WLS || kind == ST_PARENT_RESTRICT) parent_lists(t, tg, l1, l2);
@@ -290,7 +309,7 @@ void build_tables(const Tree& t, Tables& tb) {
         Tables old = std::move(tb);
         tb = Tables();
         tb.stencil.swap(old.stencil); tb.st_kind.swap(old.st_kind); tb.st_node.swap(old.st_node);
-        tb.st_px.swap(old.st_px); tb.st_py.swap(old.st_py); tb.off16.swap(old.off16);
+        tb.off16.swap(old.off16);
         tb.sel_loc.swap(old.sel_loc); tb.state_of_node.swap(old.state_of_node);
         tb.leaf_node.swap(old.leaf_node); tb.node_son.swap(old.node_son); tb.lvl_leaf.swap(old.lvl_leaf);
         tb.face_share.swap(old.face_share); tb.ns_pos.swap(old.ns_pos); tb.dev_of_ns.swap(old.dev_of_ns);
@@ -313,23 +332,29 @@ void build_tables(const Tree& t, Tables& tb) {
             x = (x | (x << 1)) & 0x5555555555555555ull;
             return x;
         };
-        // runs of cell_NS that share a base cell (cell_NS walks the base cells row-major)
-        std::vector<std::pair<uint64_t, int>> runs;  // (Morton key of the base cell, first position)
-        for (int e = 0; e < tb.N; e++) {
-            const int c = t.ns[e];
-            if (e == 0 || t.ni[c] != t.ni[t.ns[e - 1]] || t.nj[c] != t.nj[t.ns[e - 1]])
-                runs.emplace_back(spread((uint32_t)(t.nj[c] - 1)) | (spread((uint32_t)(t.ni[c] - 1)) << 1), e);
-        }
-        std::vector<int> run_end(runs.size());
-        for (size_t r = 0; r < runs.size(); r++) run_end[r] = r + 1 < runs.size() ? runs[r + 1].second : tb.N;
-        std::vector<int> order(runs.size());
-        for (size_t r = 0; r < runs.size(); r++) order[r] = (int)r;
-        std::sort(order.begin(), order.end(), [&](int a, int b) { return runs[a].first < runs[b].first; });
+        // the base cells that hold interior leaves (cell_NS walks the base cells row-major,
+        // ns_beg marks where each starts), by Morton key
+        if (t.ns_beg.size() != (size_t)t.total + 1 || t.ns_beg[t.total] != tb.N)
+            throw std::runtime_error("leaf lists out of date");
+        std::vector<std::pair<uint64_t, int>> runs;  // (Morton key, base cell)
+        for (int b = 0; b < t.total; b++)
+            if (t.ns_beg[b + 1] > t.ns_beg[b])
+                runs.emplace_back(spread((uint32_t)(b % t.m)) | (spread((uint32_t)(b / t.m)) << 1), b);
+        std::sort(runs.begin(), runs.end());
+        std::vector<int> at(runs.size() + 1, 0);
+        for (size_t r = 0; r < runs.size(); r++)
+            at[r + 1] = at[r] + t.ns_beg[runs[r].second + 1] - t.ns_beg[runs[r].second];
         resize_fresh(tb.ns_pos, tb.N);
         resize_fresh(tb.dev_of_ns, tb.N);
-        int i = 0;
-        for (int r : order)
-            for (int e = runs[r].second; e < run_end[r]; e++) { tb.ns_pos[i] = e; tb.dev_of_ns[e] = i; i++; }
+        parallel_for(runs.size(), [&](size_t r0, size_t r1) {
+            for (size_t r = r0; r < r1; r++) {
+                int i = at[r];
+                for (int e = t.ns_beg[runs[r].second]; e < t.ns_beg[runs[r].second + 1]; e++, i++) {
+                    tb.ns_pos[i] = e;
+                    tb.dev_of_ns[e] = i;
+                }
+            }
+        });
     }
     pt.mark("tables: device order");
     resize_fresh(tb.leaf_node, (size_t)tb.N + tb.G);
@@ -368,8 +393,6 @@ void build_tables(const Tree& t, Tables& tb) {
     resize_fresh(tb.lvl_leaf, N);
     resize_fresh(tb.st_kind, 8 * (size_t)N);
     resize_fresh(tb.st_node, 8 * (size_t)N);
-    resize_fresh(tb.st_px, 8 * (size_t)N);
-    resize_fresh(tb.st_py, 8 * (size_t)N);
 
     pt.mark("tables: allocation");
     // phase 1 (parallel): locate every stencil entry as the reference's gathers do
@@ -393,13 +416,6 @@ void build_tables(const Tree& t, Tables& tb) {
                 const int nbk = dir == 0 ? NB_RIGHT : dir == 1 ? NB_LEFT : dir == 2 ? NB_UP : NB_DOWN;
                 const int cn = q[nbk];
                 if (cn < 0) { missing = 1; continue; }
-                double off = two ? (dir < 2 ? 2.0 * t.xf * t.h : 2.0 * t.h) / pow2i(t.lvl[c])
-                                 : (dir < 2 ? t.xf * t.h : t.h) / pow2i(t.lvl[c]);
-                double xx = t.x[c], yy = t.y[c];
-                if (dir == 0) xx = t.x[c] + off;
-                else if (dir == 1) xx = t.x[c] - off;
-                else if (dir == 2) yy = t.y[c] + off;
-                else yy = t.y[c] - off;
                 int tg;
                 bool same;
                 if (!two) { tg = cn; same = t.lvl[cn] == t.lvl[c]; }
@@ -413,15 +429,11 @@ void build_tables(const Tree& t, Tables& tb) {
                 if (same && t.son[tg] < 0) {
                     kind = ST_COPY;
                     tb.stencil[d] = tb.state_of_node[tg];
-                } else if (same) {
-                    kind = CAND_PARENT;
-                    xx = t.x[tg]; yy = t.y[tg];
                 } else {
-                    kind = CAND_CHILD;
+                    kind = same ? CAND_PARENT : CAND_CHILD;
                 }
                 tb.st_kind[d] = kind;
                 tb.st_node[d] = tg;
-                tb.st_px[d] = xx; tb.st_py[d] = yy;
             }
         }
         int cur = lmax_at.load();
@@ -454,7 +466,10 @@ void build_tables(const Tree& t, Tables& tb) {
     auto key_of = [&](size_t j) {
         const size_t d = (size_t)jd[j];
         const bool parent = tb.st_kind[d] == CAND_PARENT;
-        return Key{parent ? 1 : 3, tb.st_node[d], parent ? 0 : bits(tb.st_px[d]), parent ? 0 : bits(tb.st_py[d])};
+        if (parent) return Key{1, tb.st_node[d], 0, 0};
+        double px, py;
+        stencil_point(t, tb.leaf_node[d / 8], (int)(d % 8), ST_CHILD_WLS, tb.st_node[d], &px, &py);
+        return Key{3, tb.st_node[d], bits(px), bits(py)};
     };
     pt.mark("tables: jump keys");
     std::vector<int32_t> rep(J);  // first jump entry with the same key
@@ -580,7 +595,9 @@ void build_tables(const Tree& t, Tables& tb) {
         auto inputs = [&](int s, WlsIn& in) {
             const size_t u = (size_t)wls_u[s];
             const size_t d = (size_t)jd[ufirst[u]];
-            wls_inputs(t, tb.st_node[d], tb.st_px[d], tb.st_py[d], &unodes[u * 9], um[u], um1[u], in);
+            double px, py;
+            stencil_point(t, tb.leaf_node[d / 8], (int)(d % 8), ukind[u], tb.st_node[d], &px, &py);
+            wls_inputs(t, tb.st_node[d], px, py, &unodes[u * 9], um[u], um1[u], in);
         };
         const size_t nrun = std::max<size_t>(1, std::min<size_t>(64, (size_t)tb.S_w / 4096));
         const size_t per = ((size_t)tb.S_w + nrun - 1) / nrun;

@@ -84,7 +84,6 @@ struct Tables {
     // diagnostics (host only)
     BigVec<int8_t> st_kind;              // [N*8]
     BigVec<int32_t> st_node;             // [N*8]
-    BigVec<double> st_px, st_py;         // [N*8]
     // slots
     std::vector<double> wls_d;           // [n_pat * WLS_STRIDE], one per pattern
     BigVec<int32_t> wls_i;               // [S_w * WLSI_STRIDE]
@@ -149,6 +148,9 @@ struct Tables {
 void build_tables(const Tree& t, Tables& tb);
 // WLS-WENO stencil lists of one classified entry (diagnostics): node ids, list sizes
 int stencil_lists(const Tree& t, int tg, int kind, int32_t* nb9, int32_t* m1, int32_t* m2);
+// the point stencil entry k of leaf node c evaluates (kind, target tg as classified): the
+// parent's centre for a parent entry, else the point at one or two cell widths (Tip..Tjmm)
+void stencil_point(const Tree& t, int c, int k, int kind, int tg, double* px, double* py);
 
 // reference helpers reused by diagnostics
 int brinv(int n, const double* a, double* r);                   // XuanGua.F90:593-660

@@ -13,10 +13,13 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <new>
 #include <shared_mutex>
 #include <stdexcept>
 #include <exception>
 #include <thread>
+
+#include <sys/mman.h>
 
 namespace fdm {
 
@@ -52,6 +55,14 @@ void parallel_for(size_t n, const std::function<void(size_t, size_t)>& f, size_t
     for (auto& x : errs)
         if (x) std::rethrow_exception(x);
 }
+
+void* big_alloc(size_t bytes) {
+    void* p = nullptr;
+    if (posix_memalign(&p, (size_t)2 << 20, bytes) != 0 || !p) throw std::bad_alloc();
+    (void)madvise(p, bytes, MADV_HUGEPAGE);  // advice only: ignored where huge pages are off
+    return p;
+}
+void big_free(void* p) { std::free(p); }
 
 static double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -133,11 +144,19 @@ constexpr int kBlockBatch = 16;
 void Tree::resize_nodes(size_t nn) {
     if (nn > (size_t)INT_MAX) throw std::runtime_error("quadtree node count exceeds 2^31");
     if (nn > father.capacity()) {
-        // with room to grow, so that further passes do not copy the arrays again
+        // with room to grow, so that further passes do not copy the arrays again; the copy
+        // into the new storage runs on all host threads
         const size_t cap = nn + nn / 4;
-        father.reserve(cap); son.reserve(cap); lvl.reserve(cap); sp.reserve(cap); ni.reserve(cap);
-        nj.reserve(cap); cross.reserve(cap); x.reserve(cap); y.reserve(cap); U.reserve(cap);
-        rot.reserve(cap); div.reserve(cap);
+        auto grow = [&](auto& v) {
+            std::decay_t<decltype(v)> w;
+            w.reserve(cap);
+            w.resize(v.size());
+            using T = typename std::decay_t<decltype(v)>::value_type;
+            parallel_for(v.size(), [&](size_t b, size_t e) { std::memcpy(w.data() + b, v.data() + b, (e - b) * sizeof(T)); });
+            v.swap(w);
+        };
+        grow(father); grow(son); grow(lvl); grow(sp); grow(ni); grow(nj); grow(cross);
+        grow(x); grow(y); grow(U); grow(rot); grow(div);
     }
     father.resize(nn); son.resize(nn); lvl.resize(nn); sp.resize(nn); ni.resize(nn);
     nj.resize(nn); cross.resize(nn); x.resize(nn); y.resize(nn); U.resize(nn); rot.resize(nn);
@@ -260,8 +279,10 @@ void Tree::compact() {
     auto permute = [&](auto& v, auto map) {
         using T = typename std::decay_t<decltype(v)>::value_type;
         if (spare_bytes < NA * sizeof(T)) {
-            spare_bytes = NA * sizeof(T) + NA * sizeof(T) / 4;
-            spare.reset(new unsigned char[spare_bytes]);  // not zeroed: first touched in parallel
+            spare.reset();
+            spare_bytes = std::max(NA * sizeof(T) + NA * sizeof(T) / 4, BIG_ALLOC_MIN);
+            // not zeroed: first touched in parallel
+            spare = std::shared_ptr<unsigned char[]>(static_cast<unsigned char*>(big_alloc(spare_bytes)), big_free);
         }
         T* src = reinterpret_cast<T*>(spare.get());
         parallel_for(NA, [&](size_t b, size_t e) { std::memcpy(src + b, v.data() + b, (e - b) * sizeof(T)); });
@@ -296,7 +317,9 @@ static void run_fronts(Tree& t, int skew, unsigned nt, size_t room, const F& f) 
     const int chunk = skew < 0 ? 32 : 4;
     PassPool pool;
     pool.used = t.n_alloc();
+    PhaseTimer pt;
     t.resize_nodes(pool.used + std::max(room, (size_t)1 << 16));
+    pt.mark("  pass: node arrays");
     t.pool = &pool;
     std::atomic<int> next{0}, arrived{0}, gen{0};
     std::atomic<bool> abort{false};
@@ -355,6 +378,7 @@ static void run_fronts(Tree& t, int skew, unsigned nt, size_t room, const F& f) 
     for (unsigned w = 1; w < nt; w++) th.emplace_back(worker, w);
     worker(0);
     for (auto& x : th) x.join();
+    pt.mark("  pass: cells");
     t.pool = nullptr;
     t.resize_nodes(pool.used);
     for (auto& e : errs)
@@ -385,6 +409,7 @@ void Tree::refine(int c) {                          // mesh.F90:199-260
         U[q] = U[c];
         rot[q] = rot[c]; div[q] = div[c];
     }
+    if (mod_count) mod_count->fetch_add(1, std::memory_order_relaxed);
     if (track_levels) {
         int8_t& hi = cell_lmax[(size_t)(ni[c] - 1) * m + nj[c] - 1];
         hi = std::max<int8_t>(hi, (int8_t)(lvl[c] + 1));
@@ -407,6 +432,7 @@ void Tree::coarse(int c) {                          // mesh.F90:397-408
     restrict_mean(c, v);
     for (int i = 0; i < 7; i++) U[c][i] = v[i];
     release_subtree(c);
+    if (mod_count) mod_count->fetch_add(1, std::memory_order_relaxed);
     if (track_levels) {
         int8_t& lo = cell_lmin[(size_t)(ni[c] - 1) * m + nj[c] - 1];
         lo = std::min<int8_t>(lo, (int8_t)lvl[c]);
@@ -506,32 +532,40 @@ static const DiagRule kUR = {1, 2, 3, 0, 2, 4};
 static const DiagRule kDR = {4, 1, 2, 3, 3, 1};
 static const DiagRule kDL = {3, 0, 1, 2, 4, 2};
 
-#define DIAG_BODY(RULE, HFN, VFN)                                                      \
-    int hn = HFN(c), vn = VFN(c), f = father[c];                                         \
-    if (f < 0) return (hn < 0 || vn < 0) ? -1 : VFN(hn);                                 \
-    if (sp[c] == RULE.sib_sp) return son[f] + RULE.sib_son;                              \
-    if (sp[c] == RULE.desc_sp) {                                                         \
-        if (hn < 0 || vn < 0) return -1;                                                 \
-        int r = VFN(hn);                                                                 \
-        while (r >= 0 && lvl[r] < lvl[c] && son[r] >= 0) r = son[r] + RULE.desc_son;     \
-        return r;                                                                        \
-    }                                                                                    \
-    if (sp[c] == RULE.hside_sp) return (hn < 0 || lvl[hn] < lvl[c]) ? -1 : VFN(hn);      \
-    if (sp[c] == RULE.vside_sp) return (vn < 0 || lvl[vn] < lvl[c]) ? -1 : HFN(vn);      \
+// One corner's finder given the node's own H and V neighbours hn, vn (idnei has them already)
+template <int (Tree::*HFN)(int) const, int (Tree::*VFN)(int) const>
+static int diag(const Tree& t, const DiagRule& R, int c, int hn, int vn) {
+    const int f = t.father[c];
+    if (f < 0) return (hn < 0 || vn < 0) ? -1 : (t.*VFN)(hn);
+    if (t.sp[c] == R.sib_sp) return t.son[f] + R.sib_son;
+    if (t.sp[c] == R.desc_sp) {
+        if (hn < 0 || vn < 0) return -1;
+        int r = (t.*VFN)(hn);
+        while (r >= 0 && t.lvl[r] < t.lvl[c] && t.son[r] >= 0) r = t.son[r] + R.desc_son;
+        return r;
+    }
+    if (t.sp[c] == R.hside_sp) return (hn < 0 || t.lvl[hn] < t.lvl[c]) ? -1 : (t.*VFN)(hn);
+    if (t.sp[c] == R.vside_sp) return (vn < 0 || t.lvl[vn] < t.lvl[c]) ? -1 : (t.*HFN)(vn);
     return -1;
+}
+static int diag_ul(const Tree& t, int c, int hn, int vn) { return diag<&Tree::left, &Tree::up>(t, kUL, c, hn, vn); }
+static int diag_ur(const Tree& t, int c, int hn, int vn) { return diag<&Tree::right, &Tree::up>(t, kUR, c, hn, vn); }
+static int diag_dr(const Tree& t, int c, int hn, int vn) { return diag<&Tree::right, &Tree::down>(t, kDR, c, hn, vn); }
+static int diag_dl(const Tree& t, int c, int hn, int vn) { return diag<&Tree::left, &Tree::down>(t, kDL, c, hn, vn); }
 
-int Tree::upleft(int c) const { DIAG_BODY(kUL, left, up) }
-int Tree::upright(int c) const { DIAG_BODY(kUR, right, up) }
-int Tree::downright(int c) const { DIAG_BODY(kDR, right, down) }
-int Tree::downleft(int c) const { DIAG_BODY(kDL, left, down) }
-#undef DIAG_BODY
+int Tree::upleft(int c) const { return diag_ul(*this, c, left(c), up(c)); }
+int Tree::upright(int c) const { return diag_ur(*this, c, right(c), up(c)); }
+int Tree::downright(int c) const { return diag_dr(*this, c, right(c), down(c)); }
+int Tree::downleft(int c) const { return diag_dl(*this, c, left(c), down(c)); }
 
 void Tree::idnei(int c) {                           // neighbor.F90:412-449
     int32_t* q = &nb[8 * (size_t)c];
     q[NB_UP] = up(c); q[NB_DOWN] = down(c); q[NB_RIGHT] = right(c); q[NB_LEFT] = left(c);
     if (cross[c] >= 0) {
-        q[NB_UPLEFT] = upleft(c); q[NB_UPRIGHT] = upright(c);
-        q[NB_DOWNLEFT] = downleft(c); q[NB_DOWNRIGHT] = downright(c);
+        q[NB_UPLEFT] = diag_ul(*this, c, q[NB_LEFT], q[NB_UP]);
+        q[NB_UPRIGHT] = diag_ur(*this, c, q[NB_RIGHT], q[NB_UP]);
+        q[NB_DOWNLEFT] = diag_dl(*this, c, q[NB_LEFT], q[NB_DOWN]);
+        q[NB_DOWNRIGHT] = diag_dr(*this, c, q[NB_RIGHT], q[NB_DOWN]);
     } else {
         q[NB_UPLEFT] = q[NB_UPRIGHT] = q[NB_DOWNLEFT] = q[NB_DOWNRIGHT] = -1;
     }
@@ -581,6 +615,7 @@ void Tree::idgrid() {
         }
     });
     for (int i = 0; i < total; i++) { na[i + 1] += na[i]; nbc[i + 1] += nbc[i]; }
+    ns_beg.assign(na.begin(), na.end());
     ns.assign(na[total], -1);
     bcl.assign(nbc[total], -1);
     parallel_for((size_t)total, [&](size_t b0, size_t e0) {
@@ -728,7 +763,9 @@ void Tree::solute_ref(double arot, double adiv, int lvl_amr, double rg_rot, doub
                     k++;
             refines += k;
         });
+        PhaseTimer pt;
         run_fronts(*this, -1, nt, 4 * refines.load(), cell);
+        if (pt.on) std::fprintf(stderr, "[fdmamr]   refine candidates %zu\n", refines.load());
     } else {
         for (int i = 0; i < total; i++) cell(i);
     }
@@ -785,7 +822,16 @@ void Tree::solute_gridmodify(int lvl_amr) {         // solution.F90:537-556
         for (int j : nbr)
             if (j >= 0) { lo = std::min<int>(lo, cell_lmin[j]); hi = std::max<int>(hi, cell_lmax[j]); }
     };
+    // A pass (gridmodify1 then gridmodify4 over all cells) that changes nothing leaves the tree
+    // as it found it, so every later pass would change nothing either: those are skipped.
+    std::atomic<long> mods{0};
+    mod_count = &mods;
+    struct Reset {
+        Tree& t;
+        ~Reset() { t.mod_count = nullptr; t.track_levels = false; }
+    } reset{*this};
     for (int k = 1; k <= lvl_amr; k++) {
+        mods = 0;
         if (nt > 1) {
             track_levels = true;
             leaf_levels();
@@ -809,6 +855,7 @@ void Tree::solute_gridmodify(int lvl_amr) {         // solution.F90:537-556
             for (int i = 0; i < total; i++) gridmodify4(i);
             pt.mark("gridmodify4");
         }
+        if (mods.load() == 0) break;
     }
     compact();
     pt.mark("compact");

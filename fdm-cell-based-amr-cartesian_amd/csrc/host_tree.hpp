@@ -9,6 +9,7 @@
 #include <array>
 #include <cstdint>
 #include <functional>
+#include <atomic>
 #include <memory>
 #include <string>
 #include <type_traits>
@@ -22,6 +23,11 @@ struct PassPool;  // node allocation of a parallel pass over the base cells (hos
 // Allocator whose resize leaves new elements uninitialised, for the node, per-leaf and
 // per-entry arrays whose every element is written before it is read: growing them costs
 // nothing, and fresh memory is first touched by the host threads that fill it.
+// Large blocks are 2 MiB aligned and advised for transparent huge pages, which cuts the page
+// faults of first touch and the TLB misses of the regrid's scattered reads.
+void* big_alloc(size_t bytes);
+void big_free(void* p);
+constexpr size_t BIG_ALLOC_MIN = (size_t)4 << 20;
 template <class T>
 struct NoInit : std::allocator<T> {
     template <class U>
@@ -29,6 +35,14 @@ struct NoInit : std::allocator<T> {
     NoInit() = default;
     template <class U>
     NoInit(const NoInit<U>&) {}
+    T* allocate(size_t n) {
+        if (n * sizeof(T) < BIG_ALLOC_MIN) return std::allocator<T>::allocate(n);
+        return static_cast<T*>(big_alloc(n * sizeof(T)));
+    }
+    void deallocate(T* p, size_t n) {
+        if (n * sizeof(T) < BIG_ALLOC_MIN) std::allocator<T>::deallocate(p, n);
+        else big_free(p);
+    }
     template <class U>
     void construct(U* p) noexcept(std::is_nothrow_default_constructible<U>::value) { ::new ((void*)p) U; }
     template <class U, class... A>
@@ -61,6 +75,7 @@ struct Tree {
 
     // ---- leaf lists (IDgrid) ----
     std::vector<int32_t> ns, bcl;                   // cell_NS, cell_BC
+    std::vector<int32_t> ns_beg;                    // cell_NS position of each base cell's first leaf
 
     void configure(double xmin, double xmax, double ymin, double ymax, double h_, double gamma_,
                    double Ma_);
@@ -83,6 +98,7 @@ struct Tree {
     // an upper bound on the greatest), kept by refine and coarse while track_levels is set
     std::vector<int8_t> cell_lmin, cell_lmax;
     bool track_levels = false;
+    std::atomic<long>* mod_count = nullptr;         // refines + coarsens, counted when set
     void leaf_levels();
 
     // neighbour finders (neighbor.F90:6-396); -1 = null
