@@ -2,19 +2,24 @@
 
 A step is one pass of the reference's solute loop body (solution.F90:183-215): λ (running
 max), dt from CFL, three SSP-RK3 stages (level-jump slot values, fused flux + update, ghost
-fill) and the residual, all on the device. Regrid (every 10 steps in the workload's spec) is
-timed separately and reported amortised.
+fill) and the residual, all on the device.
 
-`value` = updates/s of the learned path: per step the troubled-cell network runs on the step's
-state (BF16 MFMA), cleared leaves take the linear-weight FP32 reconstruction, troubled ones
-the full characteristic WCNS3 in FP32, the substep and RK blend are FP64, and a 2% sample of
-cleared leaves is re-checked by the full method. The plain FP64 path (the reference's
-arithmetic, bit-exact with the CPU oracle) runs on the same mesh and state beside it;
-L1/L∞(ρ) between the two after the timed steps are reported.
+The workload runs as its spec says: the 4-bubble shock on a 1024x512 base, level 4, with a
+regrid every 10 steps (solution.F90:408-421). The plain FP64 path (the reference's
+arithmetic, bit-exact with the CPU oracle) and the learned path advance side by side in
+cycles of 10 steps; after each cycle the FP64 context regrids on its own sensors and the
+learned context regrids on the same sensors, so both stay on identical meshes and L1/L∞(ρ)
+between them measures the learned path's error alone. `value` = updates/s of the learned
+path over the timed steps (K, in cycles that start right after a regrid, so the steps after
+each regrid, whose copied values make many leaves troubled, are included); regrids are timed
+apart and folded into `value_regrid_amortised`. The learned path: per step the troubled-cell
+network runs on the step's state (BF16 MFMA), cleared leaves take the linear-weight FP32
+reconstruction, troubled ones the full characteristic WCNS3 in FP32, the substep and RK blend
+are FP64, and a 2% sample of cleared leaves is re-checked by the full method (the judge).
 
-Selector training follows the workload spec: trained on labels that plain runs of the
-vortex and 4-quadrant Riemann configurations produce, then fine-tuned online on the target
-workload when the judge's rejection rate exceeds 1%.
+Selector training follows the workload spec: pre-trained on ~20 M labelled leaves dumped from
+plain runs of the vortex and 4-quadrant Riemann configurations (with their regrids), then
+fine-tuned online on the target workload after a cycle whose judge rejection rate exceeds 1%.
 """
 from __future__ import annotations
 
@@ -76,22 +81,67 @@ def workload_config(name: str, precision: int, selector: int = 0, refine_goal: f
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(budget_s: float = 12.0):
-    """The CPU oracle (scalar FP64 C restatement of the reference) on a bounded sample."""
-    from oracle import oracle as O
+def _oracle_kwargs(cfg):
+    """The oracle's parameters for a solver configuration (same mesh, IC and AMR settings)."""
+    return dict(xmin=cfg.xmin, xmax=cfg.xmax, ymin=cfg.ymin, ymax=cfg.ymax, h=cfg.h, gamma=cfg.gamma,
+                Ma=cfg.Ma, tstep=cfg.tstep, cfl=cfg.cfl, t_end=cfg.t_end, iflag_localdt=cfg.localdt,
+                iflag_amr=cfg.amr, lvl_amr=cfg.lvl_amr, pstep_adapt=cfg.pstep_adapt,
+                nstep_adapt=cfg.nstep_adapt, refine_goal_rot=cfg.refine_goal_rot,
+                refine_goal_div=cfg.refine_goal_div, coarse_goal_rot=cfg.coarse_goal_rot,
+                coarse_goal_div=cfg.coarse_goal_div, amr_sensor=cfg.amr_sensor, bc_xm=cfg.bc[0],
+                bc_xp=cfg.bc[1], bc_ym=cfg.bc[2], bc_yp=cfg.bc[3], inflow=cfg.inflow,
+                ic_kind=cfg.ic_kind, seed=cfg.seed,
+                ic_param=tuple(cfg.ic_param) + (0.0,) * (8 - len(cfg.ic_param)),
+                pre_adapt=cfg.pre_adapt, wls_local=cfg.wls_local)
 
-    o = O.Oracle(h=10.0 / 64, tstep=1e-3)
-    n = o.n_leaf
+
+def cpu_baseline(cfg5, tree, lam, budget_s: float = 8.0):
+    """The reference's OpenMP CPU path as the CPU oracle restates it (FP64; the stage's leaf
+    loop is an OpenMP dynamic-schedule loop over cell_NS, as solution.F90:279-284) on all the
+    host's cores (OMP_NUM_THREADS, threads bound close):
+    * the vortex configuration (base 64x64, level 2, regrid every 30 steps): whole steps;
+    * a slice of the measured 4-bubble mesh (imported into the oracle): three RK3 stages over
+      the cell_NS leaves [N/3, N/3 + n), n sized for about budget_s of work. `value` is this
+      slice's rate, the measured workload's."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    from oracle import oracle as O
+    import fdmamr as F
+
+    cores = O.set_threads(0)
+    vcfg = workload_config("vortex", F.FLUX_FP64)
+    o = O.Oracle(**_oracle_kwargs(vcfg))
     o.step()
     t0 = time.perf_counter()
-    steps = 0
-    while time.perf_counter() - t0 < budget_s:
-        o.step()
+    upd = steps = 0
+    while time.perf_counter() - t0 < budget_s and steps < 100:
+        upd += o.n_leaf
+        assert o.step() == 0
         steps += 1
-    dt = time.perf_counter() - t0
-    return dict(value=steps * n / dt, unit="leaf-updates/s", cores=1, kind="port",
-                sample=f"oracle (C, FP64, 1 thread): vortex base 64x64 uniform, {n} leaves, "
-                       f"{steps} steps in {dt:.1f}s")
+    dt_v = time.perf_counter() - t0
+    vort = dict(value=upd / dt_v, unit="leaf-updates/s", leaves=o.n_leaf, steps=steps, seconds=round(dt_v, 2))
+    o.close()
+    t0 = time.perf_counter()
+    o = O.Oracle(**_oracle_kwargs(cfg5))
+    o.import_tree(tree)
+    t_imp = time.perf_counter() - t0
+    N = o.n_leaf
+    i0, n = N // 3, min(N // 3, 100_000)
+    t0 = time.perf_counter()
+    o.rk3_range(i0, i0 + n, 1e-6, float(lam[0]), float(lam[1]))
+    per = time.perf_counter() - t0
+    n = int(min(N // 3, max(n, n * budget_s / max(per, 1e-6) / 2)))
+    t0 = time.perf_counter()
+    reps = 0
+    while time.perf_counter() - t0 < budget_s:
+        o.rk3_range(i0, i0 + n, 1e-6, float(lam[0]), float(lam[1]))
+        reps += 1
+    dt_s = time.perf_counter() - t0
+    o.close()
+    return dict(value=reps * n / dt_s, unit="leaf-updates/s", cores=cores, kind="port",
+                sample=f"oracle (C, FP64, OpenMP dynamic over cell_NS, {cores} threads): 4-bubble mesh "
+                       f"({N} leaves, imported in {t_imp:.0f}s), cell_NS leaves [{i0}, {i0 + n}) x {reps} "
+                       f"RK3 steps in {dt_s:.1f}s",
+                vortex=vort)
 
 
 def measure_traffic(args, mesh_file):
@@ -113,11 +163,9 @@ def measure_traffic(args, mesh_file):
         return None
     copy_mib = 256
     child = [sys.executable, os.path.abspath(__file__), "--steps", "2", "--warmup", "1",
-             "--train-iters", "20", "--no-cpu", "--no-traffic", "--traffic-child",
+             "--no-cpu", "--no-traffic", "--traffic-child",
              "--copy-calibration", str(copy_mib), "--workload", args.workload,
-             "--refine-goal", str(args.refine_goal), "--advance", str(args.advance),
-             "--develop-t", str(args.develop_t), "--develop-lvl", str(args.develop_lvl),
-             "--refine-passes", str(args.refine_passes), "--relax-steps", str(args.relax_steps),
+             "--refine-goal", str(args.refine_goal), "--develop-t", str(args.develop_t),
              "--mesh-file", mesh_file,
              "--threshold", str(args.threshold), "--label-bound", str(args.label_bound)]
     res = {}
@@ -142,9 +190,9 @@ def measure_traffic(args, mesh_file):
         grid = ((4 * n_leaf + 255) // 256 + (n_leaf + 255) // 256) * 256
         rows = sorted(csv.DictReader(open(files[0])), key=lambda x: int(x["Dispatch_Id"]))
         shutil.rmtree(d, ignore_errors=True)
-        # the calibration copies run before the solver library's first kernel
-        first = min((i for i, x in enumerate(rows) if "fdm::" in x["Kernel_Name"]), default=0)
-        copy_kib = [float(x["Counter_Value"]) for x in rows[:first] if "copyBuffer" in x["Kernel_Name"]]
+        # the calibration copies are the run's first three copy dispatches (the solver's own
+        # device-to-device copies come later)
+        copy_kib = [float(x["Counter_Value"]) for x in rows if "copyBuffer" in x["Kernel_Name"]][:3]
         mark = max(i for i, x in enumerate(rows) if "k_positivity" in x["Kernel_Name"])
         stage = [x for x in rows[mark:] if "k_stage_mixed" in x["Kernel_Name"] and int(x["Grid_Size"]) == grid]
         n_stage = len(stage)
@@ -173,38 +221,54 @@ def clone(cfg, tree, prim):
     return s
 
 
-def pretrain_selector(seed, rounds, iters, threshold, label_bound):
-    """Train on labels from plain FP32-flux runs of the vortex and Riemann configurations."""
+def pretrain_selector(seed, rows_target, iters, threshold, label_bound):
+    """Pre-training as the selector's spec has it: labelled leaves dumped from plain FP32-flux
+    runs of the 4-quadrant Riemann configuration (base 256x256, level 4, regrid every 10 steps)
+    and of the vortex (base 64x64, level 2, regrid every 30), about rows_target rows, taken
+    right after each regrid (copied values: many troubled leaves) and every second step after.
+    Trained by the library's own MFMA forward/backward kernels, Adam, three learning rates."""
     import fdmamr as F
     t0 = time.perf_counter()
     trainer = F.Solver(workload_config("riemann4", F.FLUX_FP32)).setup()
     trainer.selector_config(threshold, 0.02, label_bound)
     trainer.selector_init(seed)
-    rows = pos = 0
-    for _ in range(rounds):
-        rows, p = trainer.selector_collect()
-        pos += p
-        trainer.run_steps(4)
     vort = F.Solver(workload_config("vortex", F.FLUX_FP32)).setup()
     vort.selector_config(threshold, 0.02, label_bound)
-    w0 = trainer.selector_get_weights()
-    vort.selector_set_weights(w0)
-    for _ in range(2):
-        vort.selector_collect()
-        vort.run_steps(5)
-    # move the vortex rows into the trainer's set by training the vortex context as well:
-    # both contexts train the same weights in turn
-    loss = trainer.selector_train(iters, batch=4096, lr=3e-3, pos_weight=4.0, seed=seed)
     vort.selector_set_weights(trainer.selector_get_weights())
-    loss_v = vort.selector_train(max(iters // 4, 1), batch=4096, lr=1e-3, pos_weight=4.0, seed=seed + 1)
-    trainer.selector_set_weights(vort.selector_get_weights())
-    loss = trainer.selector_train(max(iters // 4, 1), batch=4096, lr=1e-3, pos_weight=4.0, seed=seed + 2)
+    rows = pos = 0
+    vrows = 0
+    for step in range(100):
+        if step % 30 == 0 and step:
+            vort.regrid()
+        if step % 2 == 0 or step % 30 == 1:
+            vrows, _ = vort.selector_collect()
+        vort.run_steps(1)
+    step = 0
+    while rows < rows_target - vrows and trainer.tt < 0.2:
+        if step % 10 == 0 and step:
+            trainer.regrid()
+        if step % 2 == 0 or step % 10 == 1:
+            rows, p = trainer.selector_collect()
+            pos += p
+        trainer.run_steps(1)
+        step += 1
+    t_data = time.perf_counter() - t0
+    # the two row sets live in their own contexts: each trains the same weights in turn
+    losses = []
+    for k, lr in enumerate((3e-3, 1e-3, 3e-4)):
+        losses.append(trainer.selector_train(iters // 3, batch=8192, lr=lr, pos_weight=4.0, seed=seed + k))
+        vort.selector_set_weights(trainer.selector_get_weights())
+        vort.selector_train(max(iters // 30, 1), batch=8192, lr=lr, pos_weight=4.0, seed=seed + 10 + k)
+        trainer.selector_set_weights(vort.selector_get_weights())
     w = trainer.selector_get_weights()
-    log(f"selector pre-trained: {rows} Riemann rows ({pos} troubled) + vortex rows, "
-        f"loss {loss:.4f}/{loss_v:.4f}, {time.perf_counter() - t0:.1f}s")
+    log(f"selector pre-trained: {rows} Riemann rows ({pos} troubled) from {step} steps to t = {trainer.tt:.4f} "
+        f"+ {vrows} vortex rows, loss {losses[0]:.4f} -> {losses[-1]:.4f}, data {t_data:.1f}s, "
+        f"total {time.perf_counter() - t0:.1f}s")
+    info = dict(rows=rows + vrows, riemann_rows=rows, riemann_troubled=pos, vortex_rows=vrows,
+                riemann_steps=step, iters=iters, batch=8192, loss=losses[-1])
     trainer.close()
     vort.close()
-    return w
+    return w, info
 
 
 def measured_config(args, precision, selector=0):
@@ -215,14 +279,15 @@ def measured_config(args, precision, selector=0):
 
 
 def build_mesh(args, F):
-    """The measured mesh and state (FP64 context) and the time of one regrid on it.
+    """The FP64 context on the measured mesh and state, right after a regrid.
 
     shock4bubbles with --develop-t: the flow is evolved at a coarse maximum level (the
     standard goals, regrid every 40 steps) to a developed state (the shock through the
     bubbles, reflected waves and vortices), regridded --refine-passes times at level 4 with
-    the measured goals, evolved --relax-steps on that mesh (new leaves start from their
-    parent's value, as in the reference's refinement), and regridded once more (timed).
-    Otherwise: the workload's initial adaptation, --advance steps, then one timed regrid."""
+    the measured goals (new leaves start from their parent's value, as the reference's
+    refinement does), then run for --spinup-cycles cycles of --regrid-every steps, each
+    ending in a regrid, until the mesh has settled. Otherwise: the workload's initial
+    adaptation, then the same cycles."""
     t0 = time.perf_counter()
     if args.workload == "shock4bubbles" and args.develop_t > 0:
         dev = workload_config(args.workload, F.FLUX_FP64, refine_goal=0.3)
@@ -238,50 +303,49 @@ def build_mesh(args, F):
         s.cfg = measured_config(args, F.FLUX_FP64)
         for _ in range(args.refine_passes):
             s.regrid()
-        if args.relax_steps > 0:
-            s.run_steps(args.relax_steps)
     else:
         s = F.Solver(measured_config(args, F.FLUX_FP64)).setup()
-        for _ in range(args.advance // 10):
-            s.run_steps(10)
-            s.regrid()
-        if args.advance:
-            log(f"advanced {args.advance} steps with regrid every 10: t = {s.tt:.4f}")
-    t_setup = time.perf_counter() - t0
-    # regrid cost (host refine/coarsen/balance + tables + upload), timed on its own
-    t0 = time.perf_counter()
-    s.regrid()
-    t_regrid = time.perf_counter() - t0
+    counts = []
+    for _ in range(args.spinup_cycles):
+        s.run_steps(args.regrid_every)
+        s.regrid()
+        counts.append(s.n_leaf)
     cnt = s.counts()
-    log(f"workload {args.workload}: {cnt['n_leaf']} leaves, {cnt['n_slot_wls']} WLS slots, "
-        f"{cnt['n_slot_restrict']} restriction slots, l_max {cnt['l_max']}, setup {t_setup:.1f}s, "
-        f"regrid {t_regrid:.2f}s")
-    return s, t_regrid
+    log(f"workload {args.workload}: {cnt['n_leaf']} leaves (after each spin-up cycle: {counts}), "
+        f"{cnt['n_slot_wls']} WLS slots, {cnt['n_slot_restrict']} restriction slots, l_max {cnt['l_max']}, "
+        f"t = {s.tt:.4f}, setup {time.perf_counter() - t0:.1f}s")
+    return s
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=20,
+                    help="timed steps, in cycles of --regrid-every steps that start right after a regrid")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed steps of both paths before the cycles")
     ap.add_argument("--workload", default="shock4bubbles")
-    ap.add_argument("--refine-goal", type=float, default=0.01,
-                    help="refinement goal of the measured mesh (its coarsening goal is a third)")
+    ap.add_argument("--refine-goal", type=float, default=0.03,
+                    help="refinement goal of the measured mesh (its coarsening goal is a third); 0.03 "
+                         "settles the 4-bubble mesh at about 19 M leaves")
     ap.add_argument("--develop-t", type=float, default=0.4,
                     help="shock4bubbles: evolve the flow to this time at --develop-lvl (regrid "
                          "every 40 steps) before refining to level 4; 0 measures on the initial "
                          "condition's mesh")
     ap.add_argument("--develop-lvl", type=int, default=1)
     ap.add_argument("--refine-passes", type=int, default=3,
-                    help="regrids at level 4 after the development (the timed regrid follows)")
-    ap.add_argument("--relax-steps", type=int, default=120,
-                    help="steps on the refined mesh before its timed regrid (the refinement copies "
-                         "coarse values, and the fine solution needs these steps to lose the steps)")
-    ap.add_argument("--train-iters", type=int, default=600)
+                    help="regrids at level 4 after the development")
+    ap.add_argument("--spinup-cycles", type=int, default=4,
+                    help="cycles of --regrid-every FP64 steps and a regrid before the paired run")
+    ap.add_argument("--regrid-every", type=int, default=10)
+    ap.add_argument("--tune-cycles", type=int, default=8,
+                    help="paired cycles before the timed ones; the selector is fine-tuned after "
+                         "each whose judge rejection rate exceeds 1%%")
+    ap.add_argument("--accuracy-cycles", type=int, default=10,
+                    help="paired cycles after the timed ones, before L1/Linf(rho) is taken")
+    ap.add_argument("--pretrain-rows", type=int, default=20_000_000)
+    ap.add_argument("--train-iters", type=int, default=4000)
     ap.add_argument("--tune-iters", type=int, default=1000,
-                    help="minibatches of each online fine-tune round on the workload")
-    ap.add_argument("--advance", type=int, default=0,
-                    help="evolve the flow this many steps (regrid every 10) before measuring")
+                    help="minibatches of each online fine-tune")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--threshold", type=float, default=0.5,
                     help="selector probability above which a leaf takes the full path")
@@ -312,102 +376,160 @@ def main():
 
     import fdmamr as F
 
-    w = pretrain_selector(7, rounds=6, iters=args.train_iters, threshold=args.threshold,
-                          label_bound=args.label_bound)
-
     if args.mesh_file:
-        # a counter run: the parent's mesh and state
+        # a counter run: the parent's mesh, state and fine-tuned weights
         z = np.load(args.mesh_file)
         tree = {k: z[k] for k in z.files if k not in ("prim", "weights")}
-        prim0 = z["prim"]
-        s64, t_regrid = None, 0.0
-    else:
-        s64, t_regrid = build_mesh(args, F)
-        tree = s64.tree()
-        prim0 = s64.get_state()
-
-    # learned path: online fine-tune on this workload while the judge rejects > 1%
-    sel = clone(measured_config(args, F.FLUX_FP32, selector=1), tree, prim0)
-    cnt = sel.counts()
-    N = cnt["n_leaf"]
-    sel.selector_config(args.threshold, 0.02, args.label_bound)
-    if args.mesh_file:
-        # the parent's fine-tuned weights: the counter runs measure the same selection
-        w = z["weights"]
-    sel.selector_set_weights(w)
-    sel.run_steps(args.warmup)
-    st = sel.selector_stats(reset=True)
-    rate = st["rejected"] / max(st["judged"], 1)
-    tuned = 0
-    while rate > 0.01 and tuned < 3 and not args.mesh_file:
-        sel.selector_collect()
-        sel.selector_train(args.tune_iters, batch=4096, lr=1e-3, pos_weight=4.0, seed=11 + tuned)
-        sel.selector_clear_dataset()
-        sel.set_state(prim0)
-        sel.build_tables()
-        sel.set_state(prim0)
+        sel = clone(measured_config(args, F.FLUX_FP32, selector=1), tree, z["prim"])
+        sel.selector_config(args.threshold, 0.02, args.label_bound)
+        sel.selector_set_weights(z["weights"])
         sel.run_steps(args.warmup)
-        st = sel.selector_stats(reset=True)
-        rate = st["rejected"] / max(st["judged"], 1)
-        tuned += 1
-    log(f"judge rejection rate {rate:.4f} after {tuned} online fine-tune round(s)")
-    w_tuned = sel.selector_get_weights()
-    # restart both paths from the same state on the same mesh
-    sel.build_tables()
-    sel.set_state(prim0)
-    if args.traffic_child:
         # marker dispatch for the counter parser, then the learned steps it measures
         sel.check_positivity()
         sel.run_steps(args.steps)
-        print(json.dumps({"n_leaf": N, "steps": args.steps}))
+        print(json.dumps({"n_leaf": sel.n_leaf, "steps": args.steps}))
         return
-    s64.set_state(prim0)
 
-    def timed(s, label):
-        s.run_steps(args.warmup)  # synchronises at the end
+    w, pre = pretrain_selector(7, args.pretrain_rows, args.train_iters, args.threshold, args.label_bound)
+    s64 = build_mesh(args, F)
+    sel = clone(measured_config(args, F.FLUX_FP32, selector=1), s64.tree(), s64.get_state())
+    sel.selector_config(args.threshold, 0.02, args.label_bound)
+    sel.selector_set_weights(w)
+    sel.tt, sel.lam = s64.tt, s64.lam.copy()
+
+    def run(s, n):
         t0 = time.perf_counter()
-        s.run_steps(args.steps)  # synchronises at the end
-        wall = time.perf_counter() - t0
-        log(f"{label}: {args.steps} steps {wall * 1e3:.1f} ms")
-        return wall
+        s.run_steps(n)  # synchronises at the end
+        return time.perf_counter() - t0
 
-    def stage_kernel_ms(s, label):
-        # HIP events around each stage's flux/update launches, in a pass of its own after the
-        # timed one (event records add gaps to the stream, so they stay out of `value`)
-        s.profile(True)
-        s.run_steps(args.steps)
-        ms, nl = s.profile_read()
-        s.profile(False)
-        log(f"{label}: stage kernels {ms / max(nl, 1):.4f} ms avg over {nl} stages")
-        return ms / max(nl, 1)
+    def regrid_both():
+        # the FP64 context on its own sensors, the learned one on the same sensors (same mesh)
+        t0 = time.perf_counter()
+        rot, div = s64.sensors()
+        s64.regrid_with_sensors(rot, div)
+        t64 = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        sel.regrid_with_sensors(rot, div)
+        return t64, time.perf_counter() - t0
 
+    def judge_rate(st):
+        return st["rejected"] / max(st["judged"], 1)
+
+    # warm-up steps, then every cycle starts right after a regrid
+    run(s64, args.warmup)
+    run(sel, args.warmup)
+    regrid_both()
+    E = args.regrid_every
+    tune_log = []
+    tuning = True  # the pre-trained network has not seen this workload
+    rows = 0
+    for cyc in range(args.tune_cycles):
+        sel.selector_stats(reset=True)
+        if tuning:
+            # rows from the cycle's first two states (just after the regrid, and one step on),
+            # added to those of earlier fine-tune cycles (up to about 150 M rows)
+            if rows + 2 * sel.n_leaf > 150_000_000:
+                sel.selector_clear_dataset()
+            sel.selector_collect()
+            run(sel, 1)
+            rows, _ = sel.selector_collect()
+            sel.selector_train(args.tune_iters, batch=8192, lr=5e-4, pos_weight=4.0, seed=100 + cyc)
+            run(sel, E - 1)
+        else:
+            run(sel, E)
+        run(s64, E)
+        st = sel.selector_stats(reset=True)
+        rate = judge_rate(st)
+        tune_log.append(dict(cycle=cyc, tuned=tuning, rejection=round(rate, 5), leaves=sel.n_leaf))
+        tuning = rate > 0.01
+        regrid_both()
+    sel.selector_clear_dataset()
+    log("fine-tune cycles: " + ", ".join(f"{d['rejection']:.4f}{'*' if d['tuned'] else ''}" for d in tune_log)
+        + " (* = tuned in that cycle)")
+    w_tuned = sel.selector_get_weights()
+
+    # timed cycles (regrids between them timed apart)
+    n_timed_cycles = max(1, (args.steps + E - 1) // E)
+    K = n_timed_cycles * E
+    tree_t, prim_t = sel.tree(), sel.get_state()
+    lam_t, tt_t = sel.lam.copy(), sel.tt
+    wall_l = wall64 = reg_l = reg64 = 0.0
+    upd = 0
     sel.selector_stats(reset=True)
-    wall_l = timed(sel, "learned+fp32")
+    for cyc in range(n_timed_cycles):
+        upd += E * sel.n_leaf
+        wall_l += run(sel, E)
+        wall64 += run(s64, E)
+        a, b = regrid_both()
+        reg64 += a
+        reg_l += b
     st = sel.selector_stats(reset=True)
-    wall64 = timed(s64, "fp64")
-    troubled_frac = st["troubled"] / max(N * (args.steps + args.warmup), 1)
+    troubled_frac = st["troubled"] / max(upd, 1)
+    ups = upd / wall_l
+    ups64 = upd / wall64
+    log(f"timed: learned+fp32 {K} steps {wall_l * 1e3:.1f} ms, fp64 {wall64 * 1e3:.1f} ms, "
+        f"judge rejection {judge_rate(st):.4f}, regrid {reg_l / n_timed_cycles:.2f}s (fp64 context "
+        f"{reg64 / n_timed_cycles:.2f}s)")
+    err_timed = l1_linf(s64, sel)
+    # accuracy horizon: more paired cycles, then L1/Linf(rho) on the common mesh
+    for cyc in range(args.accuracy_cycles):
+        run(sel, E)
+        run(s64, E)
+        regrid_both()
+    total_steps = args.warmup + E * (args.tune_cycles + n_timed_cycles + args.accuracy_cycles)
+    err = l1_linf(s64, sel)
+    log(f"L1/Linf(rho) learned vs fp64 after {total_steps} steps and "
+        f"{1 + args.tune_cycles + n_timed_cycles + args.accuracy_cycles} regrids: {err}")
 
-    a = s64.get_state()[:, 0]
-    b = sel.get_state()[:, 0]
-    lvl = s64.geometry()["lvl"]
-    area = (0.5 ** lvl) ** 2
-    l1 = float(np.sum(np.abs(a - b) * area) / np.sum(area))
-    linf = float(np.max(np.abs(a - b)))
+    # the same timed steps on the FP32 path with every leaf troubled (no selector): what the
+    # network adds over FP32 alone
+    s32 = clone(measured_config(args, F.FLUX_FP32, selector=0), tree_t, prim_t)
+    s32.tt, s32.lam = tt_t, lam_t.copy()
+    run(s32, 1)
+    s32.set_state(prim_t)
+    wall32 = run(s32, K)
+    ups32 = K * s32.n_leaf / wall32
+    s32.close()
 
-    k_l = stage_kernel_ms(sel, "learned+fp32")
-    k64 = stage_kernel_ms(s64, "fp64")
+    # kernel timings with HIP events, in passes of their own from the timed cycles' start
+    # (event records add gaps to the stream, so they stay out of `value`)
+    def kernel_ms(s, label):
+        s.profile(True)
+        if s is sel:
+            s.selector_stats(reset=True)
+        s.run_steps(E)
+        ms, nl = s.profile_read()
+        sel_ms = sel_n = tiles = 0
+        if s is sel:
+            sel_ms, sel_n, tiles = s.profile_read_select()
+        s.profile(False)
+        log(f"{label}: stage kernels {ms / max(nl, 1):.4f} ms avg over {nl} stages"
+            + (f"; selector {sel_ms / max(sel_n, 1):.4f} ms avg over {sel_n} launches, "
+               f"{tiles / max(sel_n, 1):.0f} network tiles per launch" if s is sel else ""))
+        return ms / max(nl, 1), sel_ms / max(sel_n, 1), tiles / max(sel_n, 1)
 
-    ups = args.steps * N / wall_l
-    ups64 = args.steps * N / wall64
-    # per-leaf algorithmic HBM bytes of one stage's flux/update launch (fast path), averaged
-    # over the 3 stages: stencil ids 32 + list entry 4 + lvl 1 + own double4 32 + out double4
-    # 32 + out float4 16 (the next stage's FP32 gathers) + U_n double4 32 in stages 2-3; the
-    # leaf's own FP32 value is the rounding of its FP64 state, so it is not counted
-    bytes_per_leaf = 32 + 4 + 1 + 32 + 32 + 16 + 32 * 2.0 / 3.0
+    for s in (sel, s64):
+        s.import_tree(*_dims(tree_t), tree_t)
+        s.build_tables()
+        s.set_state(prim_t)
+        s.tt, s.lam = tt_t, lam_t.copy()
+    N = sel.n_leaf
+    l_max = sel.counts()["l_max"]
+    k_l, k_sel, tiles = kernel_ms(sel, "learned+fp32")
+    k64, _, _ = kernel_ms(s64, "fp64")
+    # per-leaf algorithmic HBM bytes of one stage's flux/update launch, averaged over the 3
+    # stages: stencil ids 32, own double4 state in 32, out 32, U_n 32 in stages 2 and 3
+    bytes_per_leaf = 32 + 32 + 32 + 32 * 2.0 / 3.0
     achieved = N * bytes_per_leaf / (k_l * 1e-3) / 1e9
+    # MFMA work of the selector: 28 16x16x32 BF16 MFMAs per 16-leaf tile (3 + 2 + 2 per
+    # 16-unit output block of the three hidden layers), 16384 flops each; algorithmic
+    # 2 * (75*64 + 64*64 + 64*64 + 64) = 26112 flops per leaf the network runs on
+    mfma_flops = tiles * 28 * 16384
+    alg_flops = tiles * 16 * 26112
+    mfma = dict(kernel="k_select", ms=k_sel, network_tiles=tiles, mfma_tflops=mfma_flops / (k_sel * 1e-3) / 1e12,
+                algorithmic_tflops=alg_flops / (k_sel * 1e-3) / 1e12, peak_tflops=2500.0,
+                frac=mfma_flops / (k_sel * 1e-3) / 2.5e15)
 
-    # HBM traffic of the same stage launch: two counter runs in child processes on this mesh
-    # and state (the measured contexts are released first)
     traffic = None
     if not args.no_traffic:
         import tempfile
@@ -416,7 +538,7 @@ def main():
         fd, mesh_file = tempfile.mkstemp(prefix="fdmamr_mesh_", suffix=".npz")
         os.close(fd)
         try:
-            np.savez(mesh_file, prim=prim0, weights=w_tuned, **tree)
+            np.savez(mesh_file, prim=prim_t, weights=w_tuned, **tree_t)
             traffic = measure_traffic(args, mesh_file)
         finally:
             os.remove(mesh_file)
@@ -425,41 +547,62 @@ def main():
         "value": ups,
         "unit": "leaf-updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": wall_l / args.steps * 1e3,
+        "ms_per_step": wall_l / K * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32-flux/fp64-update",
         "data": "synthetic",
-        "config": {"workload": args.workload, "n_leaf": N, "l_max": cnt["l_max"],
-                   "wls_slots": cnt["n_slot_wls"], "refine_goal": args.refine_goal,
+        "config": {"workload": args.workload, "n_leaf": N, "l_max": l_max,
+                   "regrid_every": E, "refine_goal": args.refine_goal,
                    "coarse_goal": (args.refine_goal / 3.0 if args.workload == "shock4bubbles"
                                    and args.develop_t > 0 else None),
                    "developed": ({"t": args.develop_t, "max_level": args.develop_lvl,
-                                  "refine_passes": args.refine_passes, "relax_steps": args.relax_steps}
+                                  "refine_passes": args.refine_passes, "spinup_cycles": args.spinup_cycles}
                                  if args.workload == "shock4bubbles" and args.develop_t > 0 else None),
                    "base": "1024x512" if args.workload == "shock4bubbles" else None,
-                   "selector_threshold": args.threshold, "label_bound": args.label_bound},
+                   "selector_threshold": args.threshold, "label_bound": args.label_bound,
+                   "pretrain": pre},
         "fp64_value": ups64,
         "speedup_vs_fp64": ups / ups64,
-        "l1_rho_vs_fp64": l1,
-        "linf_rho_vs_fp64": linf,
+        "fp32_all_troubled_value": ups32,
+        "speedup_fp32_all_troubled_vs_fp64": ups32 / ups64,
+        "l1_rho_vs_fp64": err[0],
+        "linf_rho_vs_fp64": err[1],
+        "accuracy_steps": total_steps,
+        "l1_rho_vs_fp64_after_timed": err_timed[0],
         "troubled_fraction": troubled_frac,
-        "judge": st,
-        "regrid_s": t_regrid,
-        "value_regrid_amortised": args.steps * N / (wall_l + t_regrid * args.steps / 10.0),
+        "judge": dict(st, rejection_rate=judge_rate(st)),
+        "fine_tune_cycles": tune_log,
+        "regrid_s": reg_l / n_timed_cycles,
+        "regrid_fp64_s": reg64 / n_timed_cycles,
+        "value_regrid_amortised": upd / (wall_l + reg_l),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes_per_stage"] if traffic else None,
                      "kernel": "k_stage_mixed (cleared + troubled leaves of one RK stage)",
                      "kernel_ms": k_l, "bytes_per_leaf_launch": bytes_per_leaf,
                      "fp64_kernel_ms": k64, "traffic_detail": traffic},
+        "mfma": mfma,
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline(measured_config(args, F.FLUX_FP64), tree_t, lam_t)
     print(json.dumps(out))
+
+
+def _dims(tree):
+    return int(tree["num"][:, 1].max()), int(tree["num"][:, 0].max())
+
+
+def l1_linf(s64, sel):
+    """L1 (area-weighted) and Linf of rho between two contexts on the same mesh."""
+    a = s64.get_state()[:, 0]
+    b = sel.get_state()[:, 0]
+    lvl = s64.geometry()["lvl"]
+    area = (0.5 ** lvl) ** 2
+    return float(np.sum(np.abs(a - b) * area) / np.sum(area)), float(np.max(np.abs(a - b)))
 
 
 if __name__ == "__main__":

@@ -96,6 +96,7 @@ struct fdmamr_ctx {
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, int>> ev_stage;  // (start, stop) indices around k_stage launches
+    std::vector<std::pair<int, int>> ev_select; // (start, stop) indices around k_select launches
     int ev_used = 0;
 };
 
@@ -317,8 +318,31 @@ int upload_tables(fdmamr_ctx* c) {
     if (!tb.slot_lpat.empty()) HIPCHK(c, up(c->slot_lpat.p, tb.slot_lpat.data(), tb.slot_lpat.size()));
     HIPCHK(c, ensure(c->slot_uncov, std::max<size_t>(tb.slot_uncov.size(), 1)));
     if (!tb.slot_uncov.empty()) HIPCHK(c, up(c->slot_uncov.p, tb.slot_uncov.data(), tb.slot_uncov.size() * 4));
-    // every leaf troubled until the selector says otherwise
-    if (tb.N > 0) HIPCHK(c, hipMemsetAsync(c->flags.p, 1, tb.N, s));
+    // every leaf troubled until the selector says otherwise; the leaves the last regrid made
+    // (copied or averaged values) and the leaves whose stencil reads one of them stay
+    // troubled through the selector's first decision
+    {
+        std::vector<uint8_t> f0(tb.N);
+        const bool have_fresh = t.fresh.size() == t.n_alloc();
+        auto fresh_node = [&](int q) {
+            if (q < 0) return false;
+            if (t.fresh[q]) return true;
+            const int sq = t.son[q];  // a parent's level-jump value reads its sons
+            return sq >= 0 && (t.fresh[sq] || t.fresh[sq + 1] || t.fresh[sq + 2] || t.fresh[sq + 3]);
+        };
+        parallel_for((size_t)tb.N, [&](size_t b, size_t e) {
+            for (size_t i = b; i < e; i++) {
+                bool fr = false;
+                if (have_fresh) {
+                    fr = t.fresh[tb.leaf_node[i]] != 0;
+                    for (int k = 0; k < 8 && !fr; k++) fr = fresh_node(tb.st_node[i * 8 + k]);
+                }
+                f0[i] = (uint8_t)(1 | (fr ? 8 : 0));
+            }
+        });
+        if (tb.N > 0) HIPCHK(c, hipMemcpyAsync(c->flags.p, f0.data(), tb.N, hipMemcpyHostToDevice, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+    }
     launch_build_lists(tb.N, c->flags.p, c->list_cnt.p, c->lists.p, c->list_meta.p, s);
     DevTables& d = c->dt;
     d.N = tb.N; d.G = tb.G; d.P = tb.P; d.L = tb.L; d.S_w = tb.S_w; d.S_r = tb.S_r; d.S_wp = tb.S_wp;
@@ -445,8 +469,14 @@ void enqueue_rk3(fdmamr_ctx* c, int advance_time, bool prepare_next) {
         else launch_slots(d, c->Q[in[k]].p, c->T[in[k]].p, s);
         if (select && k == 0) {
             SelectParams sp{c->sel_threshold, (uint32_t)(c->sel_judge * 65536.0f), c->sel_step++};
+            int f0 = -1, f1 = -1;
+            if (c->profiling && (f0 = next_event(c)) >= 0) (void)hipEventRecord(c->ev_pool[f0], s);
             launch_select(d, c->Qf[0].p, c->off16.p, c->sel_pk.p, c->sel_w.p, sp, c->flags.p, nullptr, c->n_cu,
-                          c->sel_nbr_beg.p, c->sel_nbr.p, c->sel_loc.p, s);
+                          c->sel_nbr_beg.p, c->sel_nbr.p, c->sel_loc.p, c->sel_cnt.p, s);
+            if (f0 >= 0 && (f1 = next_event(c)) >= 0) {
+                (void)hipEventRecord(c->ev_pool[f1], s);
+                c->ev_select.emplace_back(f0, f1);
+            }
             launch_build_lists(d.N, c->flags.p, c->list_cnt.p, c->lists.p, c->list_meta.p, s);
         }
         int e0 = -1, e1 = -1;
@@ -637,6 +667,7 @@ int32_t fdmamr_tree_import(fdmamr_ctx* c, int32_t m, int32_t n, int32_t n_node,
     t.U.swap(U);
     t.rot.assign(n_node, 0.0);
     t.div.assign(n_node, 0.0);
+    t.fresh.assign(n_node, 0);
     t.free_blocks.clear();
     t.idneighbor();
     t.idgrid();
@@ -1161,7 +1192,7 @@ int32_t fdmamr_selector_infer(fdmamr_ctx* c, float* prob, int32_t* n_troubled) {
     HIPCHK(c, hipMemcpyAsync(c->sel_cnt.p, cnt0.data(), cnt0.size() * 4, hipMemcpyHostToDevice, c->stream));
     SelectParams sp{c->sel_threshold, 0u, c->sel_step};  // no judge sample here
     launch_select(c->dt, c->Qf[0].p, c->off16.p, c->sel_pk.p, c->sel_w.p, sp, c->flags.p, c->sel_prob.p, c->n_cu,
-                  c->sel_nbr_beg.p, c->sel_nbr.p, c->sel_loc.p, c->stream);
+                  c->sel_nbr_beg.p, c->sel_nbr.p, c->sel_loc.p, c->sel_cnt.p, c->stream);
     // no judge sample was drawn: the judge pass only counts the troubled leaves
     launch_judge(N, c->label.p, c->sel_bound, c->flags.p, c->sel_cnt.p, c->stream);
     launch_build_lists(N, c->flags.p, c->list_cnt.p, c->lists.p, c->list_meta.p, c->stream);
@@ -1211,13 +1242,14 @@ int32_t fdmamr_selector_collect(fdmamr_ctx* c, int32_t* n_rows, int32_t* n_posit
     if (r) return r;
     const int N = c->tb.N;
     const size_t need = (size_t)c->ds_rows + N;
-    if (need > 0x7fffffff / MLP_IN) return fail(c, -1, "training set too large");
+    // rows are numbered in int32 (element offsets are 64-bit)
+    if (need > 0x7fffffff) return fail(c, -1, "training set too large");
     if (c->ds_x.cap < need * MLP_IN) {
-        // grow, keeping the rows collected so far
+        // grow by half again, keeping the rows collected so far
         DBuf<uint16_t> nx;
         DBuf<uint8_t> ny;
-        HIPCHK(c, ensure(nx, need * MLP_IN * 2));
-        HIPCHK(c, ensure(ny, need * 2));
+        HIPCHK(c, ensure(nx, need * MLP_IN + need * MLP_IN / 2));
+        HIPCHK(c, ensure(ny, need + need / 2));
         if (c->ds_rows) {
             HIPCHK(c, hipMemcpyAsync(nx.p, c->ds_x.p, (size_t)c->ds_rows * MLP_IN * 2, hipMemcpyDeviceToDevice, c->stream));
             HIPCHK(c, hipMemcpyAsync(ny.p, c->ds_y.p, (size_t)c->ds_rows, hipMemcpyDeviceToDevice, c->stream));
@@ -1338,6 +1370,7 @@ int32_t fdmamr_profile(fdmamr_ctx* c, int32_t enable) {
     if (!c || c->host_only) return -1;
     c->profiling = enable != 0;
     c->ev_stage.clear();
+    c->ev_select.clear();
     c->ev_used = 0;
     return 0;
 }
@@ -1354,7 +1387,32 @@ int32_t fdmamr_profile_read(fdmamr_ctx* c, double* stage_ms_total, int32_t* stag
     if (stage_ms_total) *stage_ms_total = tot;
     if (stage_launches) *stage_launches = (int32_t)c->ev_stage.size();
     c->ev_stage.clear();
-    c->ev_used = 0;
+    if (c->ev_select.empty()) c->ev_used = 0;
+    return 0;
+}
+
+int32_t fdmamr_profile_read_select(fdmamr_ctx* c, double* ms_total, int32_t* launches, int64_t* network_tiles) {
+    if (!c || c->host_only) return -1;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    for (auto& pr : c->ev_select) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev_pool[pr.first], c->ev_pool[pr.second]));
+        tot += ms;
+    }
+    if (ms_total) *ms_total = tot;
+    if (launches) *launches = (int32_t)c->ev_select.size();
+    if (network_tiles) {
+        *network_tiles = 0;
+        if (c->sel_cnt.p) {
+            int cnt[CNT_N];
+            HIPCHK(c, read_counters(c, cnt));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            *network_tiles = cnt[CNT_TILES];
+        }
+    }
+    c->ev_select.clear();
+    if (c->ev_stage.empty()) c->ev_used = 0;
     return 0;
 }
 

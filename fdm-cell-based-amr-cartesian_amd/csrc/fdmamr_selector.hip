@@ -1,6 +1,6 @@
 // fdmamr_selector.hip -- learned troubled-cell selector (no reference counterpart).
 //
-// A 4-layer perceptron (75 -> 64 -> 64 -> 64 -> 1, ReLU, sigmoid) reads each interior leaf's
+// A 4-layer perceptron (75 -> 64 -> 64 -> 64 -> 1, GELU, sigmoid) reads each interior leaf's
 // 5x5 stencil of (rho, p, |u|), each channel centred on the leaf and scaled by its largest
 // deviation on the stencil (plus 1e-3 of |centre|, so flat data maps to ~0), and predicts
 // whether the leaf's nonlinear WCNS3 weights deviate from the linear ones.
@@ -35,12 +35,27 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-// ReLU and its derivative (taken as 0 at 0). The activation is the selector's largest VALU
-// cost per leaf after the gathers (256 evaluations per leaf), so it is one instruction: an
-// integer max of the bit pattern with 0 keeps every non-negative float and maps every
-// negative one to +0 (fmaxf would first quiet a possible signalling NaN, a second instruction).
-__device__ __forceinline__ float relu(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
-__device__ __forceinline__ float relu_d(float x) { return x > 0.f ? 1.f : 0.f; }
+// GELU in its tanh form and its derivative. With u = k0 (x + k1 x^3),
+// 0.5 x (1 + tanh u) = x s, s = 1 / (1 + exp(-2u)): one hardware exponential and one
+// reciprocal per unit (the activation is the selector's largest VALU cost per leaf after the
+// gathers, 192 evaluations). For large |x| the exponential goes to 0 or +inf and s to 1 or 0,
+// so g(x) tends to x or to -0 without a NaN. g'(x) = s + 2 x s (1 - s) k0 (1 + 3 k1 x^2).
+constexpr float GELU_K0 = 0.7978845608028654f;  // sqrt(2 / pi)
+constexpr float GELU_K1 = 0.044715f;
+// -2u log2(e) = x (GELU_A + GELU_B x^2): three VALU operations, then the hardware 2^x
+constexpr float GELU_A = (float)(-2.0 * 0.7978845608028654 * 1.4426950408889634);
+constexpr float GELU_B = (float)(-2.0 * 0.7978845608028654 * 0.044715 * 1.4426950408889634);
+__device__ __forceinline__ float gelu_s(float x) {
+    const float arg = x * __builtin_fmaf(GELU_B, x * x, GELU_A);
+    return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(arg));
+}
+__device__ __forceinline__ float gelu(float x) { return x * gelu_s(x); }
+__device__ __forceinline__ float gelu_d(float x) {
+    const float s = gelu_s(x), t = s * (1.f - s);
+    // t == 0 once s has saturated (|x| beyond ~6): the second term vanishes, and skipping it
+    // keeps a huge x from making inf * 0
+    return t == 0.f ? s : s + 2.f * x * t * GELU_K0 * (1.f + 3.f * GELU_K1 * x * x);
+}
 
 __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
     bf16x4 r;
@@ -206,7 +221,7 @@ __device__ __forceinline__ void mlp_forward(const uint16_t* sW, const float* sB,
     for (int mt = 0; mt < 4; mt++) {
         float h[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) h[j] = relu(o.z[0][mt][j]);
+        for (int j = 0; j < 4; j++) h[j] = gelu(o.z[0][mt][j]);
         hb[mt] = pack4(h[0], h[1], h[2], h[3]);
         if (sH1)
 #pragma unroll
@@ -234,7 +249,7 @@ __device__ __forceinline__ void mlp_forward(const uint16_t* sW, const float* sB,
             for (int mt = 0; mt < 4; mt++) {
                 float h[4];
 #pragma unroll
-                for (int j = 0; j < 4; j++) h[j] = relu(o.z[1][mt][j]);
+                for (int j = 0; j < 4; j++) h[j] = gelu(o.z[1][mt][j]);
                 hb[mt] = pack4(h[0], h[1], h[2], h[3]);
                 if (sH2)
 #pragma unroll
@@ -248,7 +263,7 @@ __device__ __forceinline__ void mlp_forward(const uint16_t* sW, const float* sB,
     for (int mt = 0; mt < 4; mt++)
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            const float h = relu(o.z[2][mt][j]);
+            const float h = gelu(o.z[2][mt][j]);
             o.h3[mt][j] = h;
             part += sB[SB_W4 + 16 * mt + 4 * g + j] * h;
         }
@@ -282,25 +297,27 @@ __device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
 __device__ __forceinline__ bf16x8 frag(const uint16_t* sF, int f) {
     return *reinterpret_cast<const bf16x8*>(sF + (f * 64 + (threadIdx.x & 63)) * 8);
 }
-// ReLU of two 16-unit accumulator tiles packed as the B fragment of the next layer: slots
+// GELU of two 16-unit accumulator tiles packed as the B fragment of the next layer: slots
 // 0-3 hold units 32s + 4g + j, slots 4-7 units 32s + 16 + 4g + j (the weight fragments use
 // the same K order).
-__device__ __forceinline__ bf16x8 relu_pack(const f32x4& lo, const f32x4& hi) {
+__device__ __forceinline__ bf16x8 act_pack(const f32x4& lo, const f32x4& hi) {
     bf16x8 r;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-        r[j] = (__bf16)relu(lo[j]);
-        r[4 + j] = (__bf16)relu(hi[j]);
+        r[j] = (__bf16)gelu(lo[j]);
+        r[4 + j] = (__bf16)gelu(hi[j]);
     }
     return r;
 }
 
 // flags bit 0: troubled (full path); bit 1: judged this step (cleared by the network but
 // sent through the full path with a label); bit 2: forced troubled by an earlier judge
-// rejection (sticky until the tables are rebuilt).
+// rejection (sticky until the tables are rebuilt); bit 3: made by the last regrid (refined
+// son or coarsened parent: copied or averaged values) or reading such a leaf in its stencil,
+// troubled in the first step after the regrid.
 __device__ __forceinline__ uint8_t decide(uint8_t old, bool troubled, int leaf, const SelectParams& sp) {
     uint8_t f = old & 4;
-    if (troubled || f) f |= 1;
+    if (troubled || f || (old & 8)) f |= 1;
     else if ((hash32((uint32_t)leaf * 2654435761u ^ sp.step) & 0xFFFFu) < sp.judge_q16) f |= 3;
     return f;
 }
@@ -330,7 +347,8 @@ __global__ __launch_bounds__(SF_WAVES * 64) void k_select(DevTables t, const flo
                                                           const int32_t* __restrict__ nbr,
                                                           const uint2* __restrict__ loc,
                                                           const uint16_t* __restrict__ wpk,
-                                                          const float* __restrict__ w32) {
+                                                          const float* __restrict__ w32,
+                                                          int* __restrict__ counters) {
     // one LDS array with the weight fragments at its start, so every fragment read encodes
     // its offset in the instruction (the offset field reaches 64 KB; the neighbourhoods come last)
     constexpr int SB_BYTES = (SB_TOTAL * 4 + 15) / 16 * 16;
@@ -356,6 +374,7 @@ __global__ __launch_bounds__(SF_WAVES * 64) void k_select(DevTables t, const flo
     // this block's XCD range of tiles (the grid is a multiple of 8 blocks)
     const int x = blockIdx.x & 7, xb = blockIdx.x >> 3, nbx = gridDim.x >> 3;
     const int lo = (int)(((long long)ntiles * x) >> 3), hi = (int)(((long long)ntiles * (x + 1)) >> 3);
+    int net_tiles = 0;  // tiles this wave ran the network on
     for (int tile = lo + xb * SF_WAVES + wv; tile < hi; tile += nbx * SF_WAVES) {
         const int leaf = tile * 16 + r;
         const bool valid = leaf < t.N;
@@ -392,6 +411,7 @@ __global__ __launch_bounds__(SF_WAVES * 64) void k_select(DevTables t, const flo
             flags[leaf] = decide(old, false, leaf, sp);
         }
         if (!mask) continue;
+        net_tiles++;
         bf16x8 xb3[3];
 #pragma unroll
         for (int s3 = 0; s3 < 3; s3++) {
@@ -410,7 +430,7 @@ __global__ __launch_bounds__(SF_WAVES * 64) void k_select(DevTables t, const flo
         }
 #pragma unroll
         for (int L = 0; L < 2; L++) {
-            const bf16x8 hb0 = relu_pack(z[0], z[1]), hb1 = relu_pack(z[2], z[3]);
+            const bf16x8 hb0 = act_pack(z[0], z[1]), hb1 = act_pack(z[2], z[3]);
             const uint16_t* W = sF + (L == 0 ? KF_L2 : KF_L3);
             const int boff = L == 0 ? SB_B2 : SB_B3;
 #pragma unroll
@@ -426,7 +446,7 @@ __global__ __launch_bounds__(SF_WAVES * 64) void k_select(DevTables t, const flo
         for (int mt = 0; mt < 4; mt++) {
             const f32x4 w4 = *reinterpret_cast<const f32x4*>(sB + SB_W4 + 16 * mt + 4 * g);
 #pragma unroll
-            for (int j = 0; j < 4; j++) part += w4[j] * relu(z[mt][j]);
+            for (int j = 0; j < 4; j++) part += w4[j] * gelu(z[mt][j]);
         }
         part = sum4_rows(part);
         if (valid && g == 0 && ((mask >> r) & 1)) {
@@ -435,6 +455,8 @@ __global__ __launch_bounds__(SF_WAVES * 64) void k_select(DevTables t, const flo
             flags[leaf] = decide(old, pr >= sp.threshold, leaf, sp);
         }
     }
+    if (counters && lane == 0 && net_tiles)
+        atomicAdd(&counters[(blockIdx.x % CNT_SLOTS) * CNT_N + CNT_TILES], net_tiles);
 }
 
 // judge: a judged leaf whose label exceeds the bound is forced back to the full path; also
@@ -548,13 +570,13 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const uint16_t* __restrict__ wp
                     if (r == 0) gp[P_W4 + 16 * mt + 4 * g + j] += v;
                 }
         }
-        // dz3 = w4 dz4 relu'(z3)
+        // dz3 = w4 dz4 gelu'(z3)
         f32x4 dz[4];
 #pragma unroll
         for (int mt = 0; mt < 4; mt++)
 #pragma unroll
             for (int j = 0; j < 4; j++)
-                dz[mt][j] = sB[SB_W4 + 16 * mt + 4 * g + j] * dz4 * relu_d(o.z[2][mt][j]);
+                dz[mt][j] = sB[SB_W4 + 16 * mt + 4 * g + j] * dz4 * gelu_d(o.z[2][mt][j]);
         // layers 3, 2, 1 (L = 2, 1, 0)
 #pragma unroll
         for (int L = 2; L >= 0; L--) {
@@ -602,7 +624,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const uint16_t* __restrict__ wp
                 }
             }
             if (L > 0) {
-                // dh_{L-1}[in][leaf] = W_L[in][out] dz[out][leaf]; then dz_{L-1} = dh relu'(z_{L-1})
+                // dh_{L-1}[in][leaf] = W_L[in][out] dz[out][leaf]; then dz_{L-1} = dh gelu'(z_{L-1})
                 const int wn = L == 2 ? KW_W3N : KW_W2N;
                 f32x4 nd[4];
 #pragma unroll
@@ -618,7 +640,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const uint16_t* __restrict__ wp
 #pragma unroll
                 for (int mt = 0; mt < 4; mt++)
 #pragma unroll
-                    for (int j = 0; j < 4; j++) dz[mt][j] = nd[mt][j] * relu_d(o.z[L - 1][mt][j]);
+                    for (int j = 0; j < 4; j++) dz[mt][j] = nd[mt][j] * gelu_d(o.z[L - 1][mt][j]);
             }
             __syncthreads();
         }
@@ -690,12 +712,13 @@ __global__ void k_pack(const float* __restrict__ w, uint16_t* __restrict__ pk) {
 // ---------------------------------------------------------------- launchers
 void launch_select(const DevTables& t, const float4* Qf, const int32_t* off16, const uint16_t* wpk,
                    const float* w32, const SelectParams& sp, uint8_t* flags, float* prob, int n_cu,
-                   const int32_t* nbr_beg, const int32_t* nbr, const uint8_t* loc, hipStream_t st) {
+                   const int32_t* nbr_beg, const int32_t* nbr, const uint8_t* loc, int* counters,
+                   hipStream_t st) {
     if (t.N <= 0) return;
     // two blocks per CU (the LDS limit), a multiple of 8 (one tile range per XCD)
     const int blocks = ((2 * n_cu + 7) / 8) * 8;
     hipLaunchKernelGGL(k_select, dim3(blocks), dim3(SF_WAVES * 64), 0, st, t, Qf, off16, sp, flags, prob, nbr_beg,
-                       nbr, reinterpret_cast<const uint2*>(loc), wpk, w32);
+                       nbr, reinterpret_cast<const uint2*>(loc), wpk, w32, counters);
 }
 
 void launch_judge(int n, const float* label, float bound, uint8_t* flags, int* counters, hipStream_t st) {

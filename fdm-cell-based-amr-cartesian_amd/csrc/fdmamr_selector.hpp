@@ -44,10 +44,10 @@ constexpr int KW_TOTAL = KW_FRAG + KF_TOTAL;
 // FP32 vectors staged in LDS
 constexpr int SB_B1 = 0, SB_B2 = 64, SB_B3 = 128, SB_W4 = 192, SB_B4 = 256, SB_TOTAL = 260;
 
-// device counters
 // device counters: CNT_SLOTS copies of CNT_N counters (block b adds into slot b % CNT_SLOTS, so
-// no address takes more than a few dozen atomics per step); readers sum the slots
-constexpr int CNT_TROUBLED = 0, CNT_JUDGED = 1, CNT_REJECTED = 2, CNT_N = 4, CNT_SLOTS = 64;
+// no address takes more than a few dozen atomics per step); readers sum the slots.
+// CNT_TILES: 16-leaf tiles the inference ran the network on (the others were all near-uniform)
+constexpr int CNT_TROUBLED = 0, CNT_JUDGED = 1, CNT_REJECTED = 2, CNT_TILES = 3, CNT_N = 4, CNT_SLOTS = 64;
 
 struct SelectParams {
     float threshold;      // probability at or above which a leaf is troubled
@@ -60,7 +60,8 @@ struct SelectParams {
 // the tile neighbourhoods of the tables (Tables::sel_nbr).
 void launch_select(const DevTables& t, const float4* Qf, const int32_t* off16, const uint16_t* wpk,
                    const float* w32, const SelectParams& sp, uint8_t* flags, float* prob, int n_cu,
-                   const int32_t* nbr_beg, const int32_t* nbr, const uint8_t* loc, hipStream_t st);
+                   const int32_t* nbr_beg, const int32_t* nbr, const uint8_t* loc, int* counters,
+                   hipStream_t st);
 void launch_judge(int n, const float* label, float bound, uint8_t* flags, int* counters, hipStream_t st);
 void launch_features(const DevTables& t, const float4* Qf, const int32_t* off16, uint16_t* X, int row0,
                      hipStream_t st);

@@ -113,6 +113,7 @@ void Tree::creat_mesh() {                           // mesh.F90:9-47
     x.resize(total); y.resize(total);
     U.assign(total, {0, 0, 0, 0, 0, 0, 0});
     rot.assign(total, 0.0); div.assign(total, 0.0);
+    fresh.assign(total, 0);
     free_blocks.clear();
     for (int i = 1; i <= total; i++) {
         int a = (i - 1) / m + 1;
@@ -156,11 +157,11 @@ void Tree::resize_nodes(size_t nn) {
             v.swap(w);
         };
         grow(father); grow(son); grow(lvl); grow(sp); grow(ni); grow(nj); grow(cross);
-        grow(x); grow(y); grow(U); grow(rot); grow(div);
+        grow(x); grow(y); grow(U); grow(rot); grow(div); grow(fresh);
     }
     father.resize(nn); son.resize(nn); lvl.resize(nn); sp.resize(nn); ni.resize(nn);
     nj.resize(nn); cross.resize(nn); x.resize(nn); y.resize(nn); U.resize(nn); rot.resize(nn);
-    div.resize(nn);
+    div.resize(nn); fresh.resize(nn);
 }
 
 int Tree::alloc_block() {
@@ -298,6 +299,7 @@ void Tree::compact() {
     permute(lvl, same); permute(sp, same); permute(ni, same); permute(nj, same); permute(cross, same);
     pt.mark("compact: links");
     permute(x, same); permute(y, same); permute(U, same); permute(rot, same); permute(div, same);
+    permute(fresh, same);
     pt.mark("compact: values");
     free_blocks.clear();
     nb.clear();
@@ -408,6 +410,7 @@ void Tree::refine(int c) {                          // mesh.F90:199-260
         cross[q] = gridcross(ni[q], nj[q]);
         U[q] = U[c];
         rot[q] = rot[c]; div[q] = div[c];
+        fresh[q] = 1;
     }
     if (mod_count) mod_count->fetch_add(1, std::memory_order_relaxed);
     if (track_levels) {
@@ -432,6 +435,7 @@ void Tree::coarse(int c) {                          // mesh.F90:397-408
     restrict_mean(c, v);
     for (int i = 0; i < 7; i++) U[c][i] = v[i];
     release_subtree(c);
+    fresh[c] = 1;
     if (mod_count) mod_count->fetch_add(1, std::memory_order_relaxed);
     if (track_levels) {
         int8_t& lo = cell_lmin[(size_t)(ni[c] - 1) * m + nj[c] - 1];
@@ -745,6 +749,8 @@ void Tree::ref_sub(int c, double arot, double adiv, int lvl_amr, double rg_rot,
 }
 void Tree::solute_ref(double arot, double adiv, int lvl_amr, double rg_rot, double rg_div,
                       double cg_rot, double cg_div) { // solution.F90:468-488
+    fresh.resize(n_alloc());
+    parallel_for(n_alloc(), [&](size_t b, size_t e) { std::fill(fresh.begin() + b, fresh.begin() + e, 0); });
     auto cell = [&](int i) {
         if (cross[i] == 0 || cross[i] == 1) ref_sub(i, arot, adiv, lvl_amr, rg_rot, rg_div);
         if (son[i] >= 0) coarse_sub(i, arot, adiv, lvl_amr, cg_rot, cg_div);

@@ -70,6 +70,9 @@ struct Tree {
     BigVec<double> x, y;
     BigVec<std::array<double, 7>> U;                // U(1:7) of every node
     BigVec<double> rot, div;
+    // 1 for the nodes the current regrid made leaves (refined sons, coarsened parents): their
+    // values are copies or means, not evolved states
+    BigVec<uint8_t> fresh;
     std::vector<int32_t> nb;                        // 8 cached neighbours per node (idnei)
     std::vector<int32_t> free_blocks;               // first ids of released son blocks
 

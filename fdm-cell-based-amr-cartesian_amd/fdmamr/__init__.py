@@ -32,7 +32,7 @@ EXPORTS = (
     "fdmamr_max_wavespeed_noalloc",
     "fdmamr_advance_rk3", "fdmamr_run_steps", "fdmamr_check_positivity", "fdmamr_sensors",
     "fdmamr_regrid", "fdmamr_regrid_sensors", "fdmamr_stencil_info", "fdmamr_stencil_values", "fdmamr_stencil_values_fast", "fdmamr_table_info", "fdmamr_leaf_lists", "fdmamr_set_slot_merge", "fdmamr_set_face_share", "fdmamr_profile",
-    "fdmamr_profile_read", "fdmamr_selector_set_flags", "fdmamr_selector_get_flags",
+    "fdmamr_profile_read", "fdmamr_profile_read_select", "fdmamr_selector_set_flags", "fdmamr_selector_get_flags",
     "fdmamr_labels", "fdmamr_selector_init", "fdmamr_selector_set_weights",
     "fdmamr_selector_get_weights", "fdmamr_selector_config", "fdmamr_selector_infer",
     "fdmamr_selector_features", "fdmamr_selector_collect", "fdmamr_selector_clear_dataset",
@@ -104,6 +104,7 @@ def lib():
         "fdmamr_stencil_values_fast": ([P, C.c_int32, C.POINTER(C.c_float)], C.c_int32),
         "fdmamr_profile": ([P, C.c_int32], C.c_int32),
         "fdmamr_profile_read": ([P, dp, ip], C.c_int32),
+        "fdmamr_profile_read_select": ([P, dp, ip, C.POINTER(C.c_int64)], C.c_int32),
         "fdmamr_selector_set_flags": ([P, C.POINTER(C.c_uint8), C.c_int32], C.c_int32),
         "fdmamr_selector_get_flags": ([P, C.POINTER(C.c_uint8)], C.c_int32),
         "fdmamr_labels": ([P, C.POINTER(C.c_float)], C.c_int32),
@@ -463,6 +464,14 @@ class Solver:
         ms = C.c_double(); n = C.c_int32()
         self._check(self.L.fdmamr_profile_read(self.ctx, C.byref(ms), C.byref(n)), "profile_read")
         return ms.value, n.value
+
+    def profile_read_select(self):
+        """(total ms, launches) of the selector's inference launches since the last read, and
+        the 16-leaf tiles the network ran on since the selector counters were last reset."""
+        ms = C.c_double(); n = C.c_int32(); tiles = C.c_int64()
+        self._check(self.L.fdmamr_profile_read_select(self.ctx, C.byref(ms), C.byref(n), C.byref(tiles)),
+                    "profile_read_select")
+        return ms.value, n.value, tiles.value
 
     # -- diagnostics --
     def stencil_info(self, i):

@@ -207,8 +207,12 @@ int32_t fdmamr_set_face_share(fdmamr_ctx* ctx, int32_t enable);
  * WCNS weights everywhere, nnd_center_RK.F90:310-353). FP32 mode only. A leaf is "troubled"
  * when its nonlinear WCNS3 weights deviate from the linear ones; troubled leaves take the full
  * characteristic reconstruction, cleared leaves the linear-weight one. ---- */
-/* set / read the per-leaf flags (bit 0 = troubled), interior leaves in cell_NS order, and
- * switch flag use on (use != 0) or off (every leaf takes the full path) */
+/* set / read the per-leaf flags (bit 0 = troubled; bit 2 = forced troubled by a judge
+ * rejection until the tables are rebuilt; bit 3 = made by the last regrid, troubled through
+ * the selector's next decision), interior leaves in cell_NS order, and switch flag use on
+ * (use != 0) or off (every leaf takes the full path). After a table build every leaf is
+ * troubled, and the leaves the last regrid made (refined sons and coarsened parents, whose
+ * values are copies or means) carry bit 3, as do the leaves whose stencil reads one of them. */
 int32_t fdmamr_selector_set_flags(fdmamr_ctx* ctx, const uint8_t* flags, int32_t use);
 int32_t fdmamr_selector_get_flags(fdmamr_ctx* ctx, uint8_t* flags);
 /* label of every interior leaf on the current state: the largest deviation |w1 - w1_lin|
@@ -216,7 +220,7 @@ int32_t fdmamr_selector_get_flags(fdmamr_ctx* ctx, uint8_t* flags);
  * mean square so flat data reports 0) */
 int32_t fdmamr_labels(fdmamr_ctx* ctx, float* dev);
 
-/* The network: 75 -> 64 -> 64 -> 64 -> 1, ReLU, sigmoid; BF16 MFMA with FP32 accumulation.
+/* The network: 75 -> 64 -> 64 -> 64 -> 1, GELU (tanh form), sigmoid; BF16 MFMA with FP32 accumulation.
  * Parameters (FP32 master copy, FDMAMR_SELECTOR_NPARAM floats): W1 [80][64] (input rows 75..79
  * are padding and stay zero), b1[64], W2 [64][64], b2, W3 [64][64], b3, w4[64], b4. */
 #define FDMAMR_SELECTOR_NPARAM 13569
@@ -256,6 +260,10 @@ int32_t fdmamr_selector_stats(fdmamr_ctx* ctx, int32_t* troubled, int32_t* judge
 int32_t fdmamr_profile(fdmamr_ctx* ctx, int32_t enable);
 /* total milliseconds and number of stage-kernel launches recorded since the last read */
 int32_t fdmamr_profile_read(fdmamr_ctx* ctx, double* stage_ms_total, int32_t* stage_launches);
+/* the same for the selector's inference launches (k_select), with the number of 16-leaf tiles
+ * the network ran on since the selector counters were last reset (fdmamr_selector_stats with
+ * reset); tiles whose leaves are all near-uniform are cleared without the network */
+int32_t fdmamr_profile_read_select(fdmamr_ctx* ctx, double* ms_total, int32_t* launches, int64_t* network_tiles);
 
 #ifdef __cplusplus
 }

@@ -1,6 +1,6 @@
 """CPU oracle of the troubled-cell selector network (TEST INFRASTRUCTURE ONLY).
 
-FP64 numpy forward/backward of the 75 -> 64 -> 64 -> 64 -> 1 ReLU/sigmoid network with the
+FP64 numpy forward/backward of the 75 -> 64 -> 64 -> 64 -> 1 GELU/sigmoid network with the
 same parameter layout as the library (FDMAMR_SELECTOR_NPARAM floats). With `bf16=True` the
 operands are rounded to BF16 exactly where the GPU kernels feed matrix cores (weights,
 hidden activations of layers 1-2, back-propagated deltas), so the library's gradients can be
@@ -42,12 +42,20 @@ def unpack(w):
                 b4=w[P_B4])
 
 
-def relu(x):
-    return np.maximum(x, 0.0)
+GELU_K0 = float(np.sqrt(2.0 / np.pi))
+GELU_K1 = 0.044715
 
 
-def relu_d(x):
-    return (np.asarray(x) > 0.0).astype(np.float64)
+def gelu(x):
+    """GELU in its tanh form, 0.5 x (1 + tanh(k0 (x + k1 x^3))) = x sigmoid(2 k0 (x + k1 x^3))."""
+    x = np.asarray(x, np.float64)
+    return 0.5 * x * (1.0 + np.tanh(GELU_K0 * (x + GELU_K1 * x ** 3)))
+
+
+def gelu_d(x):
+    x = np.asarray(x, np.float64)
+    s = 0.5 * (1.0 + np.tanh(GELU_K0 * (x + GELU_K1 * x ** 3)))
+    return s + 2.0 * x * s * (1.0 - s) * GELU_K0 * (1.0 + 3.0 * GELU_K1 * x * x)
 
 
 def forward(w, X, bf16=True):
@@ -56,11 +64,11 @@ def forward(w, X, bf16=True):
     r = bf16_round if bf16 else (lambda a: np.asarray(a, np.float64))
     x = r(X)
     z1 = x @ r(p["W1"]) + p["b1"]
-    h1 = r(relu(z1))
+    h1 = r(gelu(z1))
     z2 = h1 @ r(p["W2"]) + p["b2"]
-    h2 = r(relu(z2))
+    h2 = r(gelu(z2))
     z3 = h2 @ r(p["W3"]) + p["b3"]
-    h3 = relu(z3)
+    h3 = gelu(z3)
     z4 = h3 @ p["w4"] + p["b4"]
     prob = 1.0 / (1.0 + np.exp(-z4))
     return prob, dict(x=x, z1=z1, h1=h1, z2=z2, h2=h2, z3=z3, h3=h3, z4=z4, p=p, r=r)
@@ -77,15 +85,15 @@ def loss_and_grad(w, X, y, pos_weight=1.0, bf16=True):
     g = np.zeros(P_TOTAL)
     g[P_W4:P_B4] = c["h3"].T @ dz4
     g[P_B4] = dz4.sum()
-    dz3 = np.outer(dz4, p["w4"]) * relu_d(c["z3"])
+    dz3 = np.outer(dz4, p["w4"]) * gelu_d(c["z3"])
     g[P_B3:P_W4] = dz3.sum(0)
     dz3b = r(dz3)
     g[P_W3:P_B3] = (c["h2"].T @ dz3b).reshape(-1)
-    dz2 = (dz3b @ r(p["W3"]).T) * relu_d(c["z2"])
+    dz2 = (dz3b @ r(p["W3"]).T) * gelu_d(c["z2"])
     g[P_B2:P_W3] = dz2.sum(0)
     dz2b = r(dz2)
     g[P_W2:P_B2] = (c["h1"].T @ dz2b).reshape(-1)
-    dz1 = (dz2b @ r(p["W2"]).T) * relu_d(c["z1"])
+    dz1 = (dz2b @ r(p["W2"]).T) * gelu_d(c["z1"])
     g[P_B1:P_W2] = dz1.sum(0)
     gw1 = c["x"].T @ r(dz1)
     gw1[75:] = 0.0

@@ -100,14 +100,15 @@ def test_c3_shock_bubble_512x256_level5_fp32_vs_fp64_through_regrid():
 
 
 def test_c5_shock4bubbles_learned_vs_fp64():
-    # the benchmark's measured mesh (developed flow, 1024x512 base, level 4, ~19 M leaves),
-    # the selector pre-trained on the vortex and Riemann configurations, 2% judge per step
+    # the benchmark's measured mesh (developed flow, 1024x512 base, level 4, ~19 M leaves, right
+    # after a regrid), the selector pre-trained on the vortex and Riemann configurations, 2%
+    # judge per step
     import bench
 
-    args = argparse.Namespace(workload="shock4bubbles", refine_goal=0.01, develop_t=0.4, develop_lvl=1,
-                              refine_passes=3, relax_steps=120, advance=0)
-    w = bench.pretrain_selector(7, rounds=6, iters=300, threshold=0.5, label_bound=0.05)
-    s64, _ = bench.build_mesh(args, F)
+    args = argparse.Namespace(workload="shock4bubbles", refine_goal=0.03, develop_t=0.4, develop_lvl=1,
+                              refine_passes=3, spinup_cycles=2, regrid_every=10)
+    w, _ = bench.pretrain_selector(7, rows_target=2_000_000, iters=600, threshold=0.5, label_bound=0.05)
+    s64 = bench.build_mesh(args, F)
     cnt = s64.counts()
     print("c5 leaves", cnt["n_leaf"], "slots", cnt["n_slot_wls"])
     assert cnt["n_leaf"] > 15_000_000 and cnt["l_max"] == 4

@@ -54,3 +54,21 @@ def test_bf16_rounding_is_nearest_even():
     assert r[2] == 1.0 + 2 ** -7
     assert r[3] == -2.5
     assert abs(r[4] - 3.140625) < 1e-12
+
+
+def test_gelu_closed_forms():
+    from scipy.special import erf
+    x = np.linspace(-8, 8, 4001)
+    g = M.gelu(x)
+    # the tanh form stays within 1e-3 of the exact GELU x Phi(x)
+    exact = 0.5 * x * (1.0 + erf(x / np.sqrt(2.0)))
+    assert np.max(np.abs(g - exact)) < 1e-3
+    assert M.gelu(0.0) == 0.0 and M.gelu_d(0.0) == 0.5
+    # odd part: g(x) - g(-x) = x
+    assert np.allclose(g - M.gelu(-x), x, atol=1e-14)
+    # tails: identity for large x, zero for large -x
+    assert abs(M.gelu(30.0) - 30.0) < 1e-12 and abs(M.gelu(-30.0)) < 1e-12
+    # the derivative by central differences
+    h = 1e-6
+    fd = (M.gelu(x + h) - M.gelu(x - h)) / (2 * h)
+    assert np.max(np.abs(fd - M.gelu_d(x))) < 1e-8
