@@ -144,6 +144,11 @@ def test_all_clear_selector_is_forced_back_by_the_judge():
     w[-1] = -30.0
     s.selector_set_weights(w)
     bound = 0.05
+    # the leaves the set-up's regrids made stay troubled through the selector's first decision:
+    # one step without a judge sample uses that up
+    s.selector_config(0.5, 0.0, bound)
+    s.run_steps(1)
+    s.selector_stats(reset=True)
     s.selector_config(0.5, 1.0, bound)
     lab = s.labels()
     s.run_steps(1)
