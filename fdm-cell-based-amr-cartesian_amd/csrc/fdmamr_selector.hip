@@ -35,26 +35,41 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-// GELU in its tanh form and its derivative. With u = k0 (x + k1 x^3),
-// 0.5 x (1 + tanh u) = x s, s = 1 / (1 + exp(-2u)): one hardware exponential and one
-// reciprocal per unit (the activation is the selector's largest VALU cost per leaf after the
-// gathers, 192 evaluations). For large |x| the exponential goes to 0 or +inf and s to 1 or 0,
-// so g(x) tends to x or to -0 without a NaN. g'(x) = s + 2 x s (1 - s) k0 (1 + 3 k1 x^2).
-constexpr float GELU_K0 = 0.7978845608028654f;  // sqrt(2 / pi)
-constexpr float GELU_K1 = 0.044715f;
-// -2u log2(e) = x (GELU_A + GELU_B x^2): three VALU operations, then the hardware 2^x
-constexpr float GELU_A = (float)(-2.0 * 0.7978845608028654 * 1.4426950408889634);
-constexpr float GELU_B = (float)(-2.0 * 0.7978845608028654 * 0.044715 * 1.4426950408889634);
-__device__ __forceinline__ float gelu_s(float x) {
-    const float arg = x * __builtin_fmaf(GELU_B, x * x, GELU_A);
-    return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(arg));
+// GELU and its derivative. g(x) = x s(c), c = x clamped to [-GELU_A, GELU_A], with
+// s(c) = 1/2 + c P(c^2) and P of degree 5: the minimax fit (weighted by |x|, with s(GELU_A) = 1)
+// of the tanh form's factor 1 / (1 + exp(-2 k0 (x + k1 x^3))), k0 = sqrt(2/pi), k1 = 0.044715.
+// |g - GELU_tanh| <= 6.2e-4 and |g - x Phi(x)| <= 8.2e-4 everywhere, below the BF16 rounding
+// of the hidden activations (2^-9 relative); g(x) = x for x >= GELU_A and 0 for x <= -GELU_A
+// (up to the coefficients' rounding); g(x) - g(-x) = x. Nine full-rate VALU operations per
+// unit and no transcendentals (the activation is the selector's largest VALU cost per leaf,
+// 192 evaluations). g'(x) = s(c) + x s'(c) inside the clamp and s(c) outside.
+constexpr float GELU_A = 3.5f;
+constexpr float GELU_P0 = 0x1.958016p-2f, GELU_P1 = -0x1.fed952p-5f, GELU_P2 = 0x1.013ab0p-7f,
+                GELU_P3 = -0x1.496018p-11f, GELU_P4 = 0x1.da140ap-16f, GELU_P5 = -0x1.215ff2p-21f;
+__device__ __forceinline__ float gelu_clamp(float x) { return __builtin_amdgcn_fmed3f(x, -GELU_A, GELU_A); }
+__device__ __forceinline__ float gelu_poly(float t) {
+    float p = __builtin_fmaf(GELU_P5, t, GELU_P4);
+    p = __builtin_fmaf(p, t, GELU_P3);
+    p = __builtin_fmaf(p, t, GELU_P2);
+    p = __builtin_fmaf(p, t, GELU_P1);
+    return __builtin_fmaf(p, t, GELU_P0);
 }
-__device__ __forceinline__ float gelu(float x) { return x * gelu_s(x); }
+__device__ __forceinline__ float gelu(float x) {
+    const float c = gelu_clamp(x);
+    return x * __builtin_fmaf(c, gelu_poly(c * c), 0.5f);
+}
 __device__ __forceinline__ float gelu_d(float x) {
-    const float s = gelu_s(x), t = s * (1.f - s);
-    // t == 0 once s has saturated (|x| beyond ~6): the second term vanishes, and skipping it
-    // keeps a huge x from making inf * 0
-    return t == 0.f ? s : s + 2.f * x * t * GELU_K0 * (1.f + 3.f * GELU_K1 * x * x);
+    const float c = gelu_clamp(x), t = c * c;
+    const float s = __builtin_fmaf(c, gelu_poly(t), 0.5f);
+    if (c != x) return s;
+    // s'(c) = P(t) + 2 t P'(t)
+    float q = 5.f * GELU_P5;
+    q = __builtin_fmaf(q, t, 4.f * GELU_P4);
+    q = __builtin_fmaf(q, t, 3.f * GELU_P3);
+    q = __builtin_fmaf(q, t, 2.f * GELU_P2);
+    q = __builtin_fmaf(q, t, GELU_P1);
+    const float ds = __builtin_fmaf(2.f * t, q, gelu_poly(t));
+    return __builtin_fmaf(x, ds, s);
 }
 
 __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {

@@ -42,20 +42,44 @@ def unpack(w):
                 b4=w[P_B4])
 
 
+# The activation the library evaluates (csrc/fdmamr_selector.hip, gelu / gelu_d): GELU as
+# x s(c), c = x clamped to [-GELU_A, GELU_A], s(c) = 1/2 + c P(c^2), P the degree-5 minimax fit
+# of the tanh form's factor 1 / (1 + exp(-2 k0 (x + k1 x^3))) (k0 = sqrt(2/pi), k1 = 0.044715)
+# with s(GELU_A) = 1; the same FP32 coefficients, evaluated in FP64.
 GELU_K0 = float(np.sqrt(2.0 / np.pi))
 GELU_K1 = 0.044715
+GELU_A = 3.5
+GELU_P = [float.fromhex(h) for h in ("0x1.958016p-2", "-0x1.fed952p-5", "0x1.013ab0p-7",
+                                     "-0x1.496018p-11", "0x1.da140ap-16", "-0x1.215ff2p-21")]
 
 
-def gelu(x):
-    """GELU in its tanh form, 0.5 x (1 + tanh(k0 (x + k1 x^3))) = x sigmoid(2 k0 (x + k1 x^3))."""
+def gelu_tanh(x):
+    """GELU in its tanh form, 0.5 x (1 + tanh(k0 (x + k1 x^3)))."""
     x = np.asarray(x, np.float64)
     return 0.5 * x * (1.0 + np.tanh(GELU_K0 * (x + GELU_K1 * x ** 3)))
 
 
+def _poly(t, c):
+    p = np.zeros_like(t) + c[-1]
+    for a in c[-2::-1]:
+        p = p * t + a
+    return p
+
+
+def gelu(x):
+    x = np.asarray(x, np.float64)
+    c = np.clip(x, -GELU_A, GELU_A)
+    return x * (0.5 + c * _poly(c * c, GELU_P))
+
+
 def gelu_d(x):
     x = np.asarray(x, np.float64)
-    s = 0.5 * (1.0 + np.tanh(GELU_K0 * (x + GELU_K1 * x ** 3)))
-    return s + 2.0 * x * s * (1.0 - s) * GELU_K0 * (1.0 + 3.0 * GELU_K1 * x * x)
+    c = np.clip(x, -GELU_A, GELU_A)
+    t = c * c
+    s = 0.5 + c * _poly(t, GELU_P)
+    dp = _poly(t, [k * a for k, a in enumerate(GELU_P)][1:])
+    ds = _poly(t, GELU_P) + 2.0 * t * dp
+    return np.where(np.abs(x) <= GELU_A, s + x * ds, s)
 
 
 def forward(w, X, bf16=True):
