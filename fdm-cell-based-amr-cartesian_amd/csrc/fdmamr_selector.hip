@@ -116,8 +116,11 @@ __device__ constexpr int point_row(int p) {
 // (the centre, points past the grid and leaves past N read an in-range entry that is then
 // replaced by the leaf itself), so the loads of a tile travel together.
 __device__ __forceinline__ void gather_idx(const DevTables& t, const int32_t* __restrict__ off16, int leaf,
-                                           bool valid, int idx[7]) {
-    const int g = (threadIdx.x & 63) >> 4;
+                                           bool valid, int idx[7], bool no_hoist = false) {
+    int g = (threadIdx.x & 63) >> 4;
+    // in a loop's rare branch: the row selections are recomputed there instead of being hoisted
+    // out of the loop (where their registers would be live across the whole loop)
+    if (no_hoist) asm volatile("" : "+v"(g));
     const int lc = valid ? leaf : 0;
 #pragma unroll
     for (int k = 0; k < 7; k++) {
@@ -355,7 +358,53 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__global__ __launch_bounds__(SF_WAVES * 64) void k_select(DevTables t, const float4* __restrict__ Qf,
+// Neighbourhood extent of a tile (wave-uniform: scalar loads) and what the pipeline of
+// k_select fetches ahead for a tile: the lanes' position words, the flag byte, the indices of
+// neighbourhood entries lane and lane + 64; then their FP32 states, straight into the wave's
+// LDS region (global_load_lds: lane l's 16 bytes land at entry l of the region).
+struct TileMeta { int b, n; };
+__device__ __forceinline__ TileMeta tile_meta(const int32_t* __restrict__ nbr_beg, int tile) {
+    const int b = nbr_beg[tile];
+    return TileMeta{b, nbr_beg[tile + 1] - b};
+}
+struct TilePre {
+    uint2 lc;
+    uint32_t old;   // the leaf's flag byte (every lane loads it; lanes g > 0 ignore it)
+    int idx[2];
+};
+// Every load is unconditional (a lane past the extent reads an in-range entry it then ignores),
+// so no branch makes the wave wait for the data before it is used. nbr_end: the length of nbr.
+__device__ __forceinline__ void tile_pre_index(const DevTables& t, const int32_t* __restrict__ nbr, int nbr_end,
+                                               const uint2* __restrict__ loc, const uint8_t* flags, int tile,
+                                               const TileMeta& m, TilePre& p) {
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const int leaf = tile * 16 + r;
+    const int lc = leaf < t.N ? leaf : 0;
+    // 32-bit unsigned offsets from the (scalar) bases: no per-lane 64-bit addresses to keep
+    p.lc = loc[(uint32_t)(lc * 4 + g)];
+    p.old = flags[(uint32_t)lc];
+    p.idx[0] = nbr[(uint32_t)min(m.b + lane, nbr_end - 1)];
+    p.idx[1] = nbr[(uint32_t)min(m.b + lane + 64, nbr_end - 1)];
+}
+typedef __attribute__((address_space(3))) void lds_void;
+__device__ __forceinline__ void tile_pre_states(const float4* __restrict__ Qf, const TilePre& p, float4* region) {
+    __builtin_amdgcn_global_load_lds((const void*)(Qf + p.idx[0]), (lds_void*)region, 16, 0, 0);
+    __builtin_amdgcn_global_load_lds((const void*)(Qf + p.idx[1]), (lds_void*)(region + 64), 16, 0, 0);
+}
+// one hidden layer of the network on a 16-leaf tile: z <- W^T gelu(z) + b (K = 64)
+__device__ __forceinline__ void net_layer(const uint16_t* W, const float* bias, f32x4 z[4]) {
+    const int g = (threadIdx.x & 63) >> 4;
+    const bf16x8 hb0 = act_pack(z[0], z[1]), hb1 = act_pack(z[2], z[3]);
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+        f32x4 acc = *reinterpret_cast<const f32x4*>(bias + 16 * mt + 4 * g);
+        acc = mfma32(frag(W, mt * 2 + 0), hb0, acc);
+        acc = mfma32(frag(W, mt * 2 + 1), hb1, acc);
+        z[mt] = acc;
+    }
+}
+
+__global__ __launch_bounds__(SF_WAVES * 64) __attribute__((amdgpu_waves_per_eu(6))) void k_select(DevTables t, const float4* __restrict__ Qf,
                                                           const int32_t* __restrict__ off16, SelectParams sp,
                                                           uint8_t* __restrict__ flags, float* __restrict__ prob,
                                                           const int32_t* __restrict__ nbr_beg,
@@ -384,22 +433,45 @@ __global__ __launch_bounds__(SF_WAVES * 64) void k_select(DevTables t, const flo
         if (threadIdx.x == 0) sB[SB_B4] = w32[P_B4];
     }
     __syncthreads();
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int ntiles = (t.N + 15) / 16;
     // this block's XCD range of tiles (the grid is a multiple of 8 blocks)
     const int x = blockIdx.x & 7, xb = blockIdx.x >> 3, nbx = gridDim.x >> 3;
     const int lo = (int)(((long long)ntiles * x) >> 3), hi = (int)(((long long)ntiles * (x + 1)) >> 3);
+    const int stride = nbx * SF_WAVES;
     int net_tiles = 0;  // tiles this wave ran the network on
-    for (int tile = lo + xb * SF_WAVES + wv; tile < hi; tile += nbx * SF_WAVES) {
+    // Software pipeline over the wave's tiles T, T + stride, ...: while tile T's features and
+    // network run, the next tile's position words, old flags and neighbourhood indices are
+    // loaded (before layer 1), then its first 128 neighbourhood states (before layer 3), and
+    // the neighbourhood extent of the tile after it; so the dependent loads of a tile
+    // (extent -> indices -> states) overlap the network of the one before.
+    int tile = lo + xb * SF_WAVES + wv;
+    TileMeta mc = tile < hi ? tile_meta(nbr_beg, tile) : TileMeta{0, 0};
+    TileMeta mn = tile + stride < hi ? tile_meta(nbr_beg, tile + stride) : TileMeta{0, 0};
+    // entries of the marker tiles (gathered directly) are 0; every index is a state index
+    const int nbr_end = nbr_beg[ntiles];
+    TilePre pc;
+    if (tile < hi) {
+        tile_pre_index(t, nbr, nbr_end, loc, flags, tile, mc, pc);
+        tile_pre_states(Qf, pc, sQ[wv]);
+    }
+    for (; tile < hi; tile += stride) {
         const int leaf = tile * 16 + r;
         const bool valid = leaf < t.N;
-        const int b = nbr_beg[tile];
-        const int nU = nbr_beg[tile + 1] - b;
-        const bool local = nU <= SEL_NBR_CAP;
-        const uint2 lc = loc[(size_t)(valid ? leaf : 0) * 4 + g];
-        const uint8_t old = valid && g == 0 ? flags[leaf] : 0;
-        if (local)
-            for (int k = lane; k < nU; k += 64) sQ[wv][k] = channels(Qf[nbr[b + k]]);
+        const bool local = mc.n <= SEL_NBR_CAP;
+        const uint2 lc = pc.lc;
+        uint32_t old32 = valid && g == 0 ? pc.old & 0xFFu : 0u;
+        // materialised here, where the wave waits for this tile's data anyway: used only after
+        // the next tile's loads are issued, it would make the wave wait for those too
+        asm volatile("" : "+v"(old32));
+        const uint8_t old = (uint8_t)old32;
+        if (local) {
+            // entries 0..127 arrived as raw states: converted in place (each lane its own two)
+            if (lane < mc.n) sQ[wv][lane] = channels(sQ[wv][lane]);
+            if (lane + 64 < mc.n) sQ[wv][lane + 64] = channels(sQ[wv][lane + 64]);
+            for (int k = lane + 128; k < mc.n; k += 64) sQ[wv][k] = channels(Qf[nbr[mc.b + k]]);
+        }
         wave_lds_sync();
         float v[7][3];
         bool uniform;
@@ -414,61 +486,63 @@ __global__ __launch_bounds__(SF_WAVES * 64) void k_select(DevTables t, const flo
             uniform = features_from(qc, q, v);
         } else {
             int idx[7];
-            gather_idx(t, off16, leaf, valid, idx);
+            gather_idx(t, off16, leaf, valid, idx, true);
             uniform = gather_values(Qf, leaf, valid, idx, v);
         }
         // the next tile's neighbourhood overwrites this one only after every lane read it
         wave_lds_sync();
+        // next tile: position words, old flags, neighbourhood indices; the extent of the one after
+        const int tn = tile + stride;
+        TilePre pn;
+        if (tn < hi) tile_pre_index(t, nbr, nbr_end, loc, flags, tn, mn, pn);
+        const TileMeta mnn = tn + stride < hi ? tile_meta(nbr_beg, tn + stride) : TileMeta{0, 0};
         const uint64_t need = __ballot(valid && !uniform);
         const uint16_t mask = (uint16_t)(need & 0xFFFFu);
         if (valid && g == 0 && uniform) {
             if (prob) prob[leaf] = 0.f;
             flags[leaf] = decide(old, false, leaf, sp);
         }
-        if (!mask) continue;
-        net_tiles++;
-        bf16x8 xb3[3];
-#pragma unroll
-        for (int s3 = 0; s3 < 3; s3++) {
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const int q = 8 * s3 + j;
-                xb3[s3][j] = (__bf16)(q < 21 ? v[q / 3][q % 3] : 0.f);
-            }
-        }
         f32x4 z[4];
+        if (mask) {
+            net_tiles++;
+            bf16x8 xb3[3];
 #pragma unroll
-        for (int mt = 0; mt < 4; mt++) {
-            z[mt] = *reinterpret_cast<const f32x4*>(sB + SB_B1 + 16 * mt + 4 * g);
+            for (int s3 = 0; s3 < 3; s3++) {
 #pragma unroll
-            for (int s3 = 0; s3 < 3; s3++) z[mt] = mfma32(frag(sF + KF_L1, mt * 3 + s3), xb3[s3], z[mt]);
-        }
-#pragma unroll
-        for (int L = 0; L < 2; L++) {
-            const bf16x8 hb0 = act_pack(z[0], z[1]), hb1 = act_pack(z[2], z[3]);
-            const uint16_t* W = sF + (L == 0 ? KF_L2 : KF_L3);
-            const int boff = L == 0 ? SB_B2 : SB_B3;
+                for (int j = 0; j < 8; j++) {
+                    const int q = 8 * s3 + j;
+                    xb3[s3][j] = (__bf16)(q < 21 ? v[q / 3][q % 3] : 0.f);
+                }
+            }
 #pragma unroll
             for (int mt = 0; mt < 4; mt++) {
-                f32x4 acc = *reinterpret_cast<const f32x4*>(sB + boff + 16 * mt + 4 * g);
-                acc = mfma32(frag(W, mt * 2 + 0), hb0, acc);
-                acc = mfma32(frag(W, mt * 2 + 1), hb1, acc);
-                z[mt] = acc;
+                z[mt] = *reinterpret_cast<const f32x4*>(sB + SB_B1 + 16 * mt + 4 * g);
+#pragma unroll
+                for (int s3 = 0; s3 < 3; s3++) z[mt] = mfma32(frag(sF + KF_L1, mt * 3 + s3), xb3[s3], z[mt]);
+            }
+            net_layer(sF + KF_L2, sB + SB_B2, z);
+        }
+        // next tile: its first 128 neighbourhood states
+        if (tn < hi) tile_pre_states(Qf, pn, sQ[wv]);
+        if (mask) {
+            net_layer(sF + KF_L3, sB + SB_B3, z);
+            float part = 0.f;
+#pragma unroll
+            for (int mt = 0; mt < 4; mt++) {
+                const f32x4 w4 = *reinterpret_cast<const f32x4*>(sB + SB_W4 + 16 * mt + 4 * g);
+#pragma unroll
+                for (int j = 0; j < 4; j++) part += w4[j] * gelu(z[mt][j]);
+            }
+            part = sum4_rows(part);
+            if (valid && g == 0 && ((mask >> r) & 1)) {
+                const float pr = 1.f / (1.f + __expf(-(part + sB[SB_B4])));
+                if (prob) prob[leaf] = pr;
+                flags[leaf] = decide(old, pr >= sp.threshold, leaf, sp);
             }
         }
-        float part = 0.f;
-#pragma unroll
-        for (int mt = 0; mt < 4; mt++) {
-            const f32x4 w4 = *reinterpret_cast<const f32x4*>(sB + SB_W4 + 16 * mt + 4 * g);
-#pragma unroll
-            for (int j = 0; j < 4; j++) part += w4[j] * gelu(z[mt][j]);
-        }
-        part = sum4_rows(part);
-        if (valid && g == 0 && ((mask >> r) & 1)) {
-            const float pr = 1.f / (1.f + __expf(-(part + sB[SB_B4])));
-            if (prob) prob[leaf] = pr;
-            flags[leaf] = decide(old, pr >= sp.threshold, leaf, sp);
-        }
+        mc = mn;
+        mn = mnn;
+        pc = pn;
     }
     if (counters && lane == 0 && net_tiles)
         atomicAdd(&counters[(blockIdx.x % CNT_SLOTS) * CNT_N + CNT_TILES], net_tiles);
