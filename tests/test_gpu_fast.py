@@ -35,6 +35,20 @@ def test_all_troubled_flags_equal_unflagged_path():
     assert np.array_equal(b.get_state(), c.get_state())
 
 
+def test_all_troubled_flags_equal_unflagged_path_with_level_jumps():
+    # On an AMR mesh every WLS slot is read by some leaf; with every leaf troubled the first
+    # stage stamps them all, so the later stages evaluate every slot with the nonlinear weights
+    # as the unflagged path does: bit-identical steps.
+    cfg = riemann_c2(h=1.0 / 64, lvl_amr=2, pre_adapt=2, flux_precision=F.FLUX_FP32)
+    b = F.Solver(cfg).setup()
+    c = F.Solver(cfg).setup()
+    assert c.table_info()["slots"] > 1000
+    c.set_flags(np.ones(c.n_leaf, np.uint8), use=True)
+    b.run_steps(3)
+    c.run_steps(3)
+    assert np.array_equal(b.get_state(), c.get_state())
+
+
 def test_mixed_flags_take_both_paths():
     # Random flags split the leaves into the cleared and troubled launch lists; every leaf
     # must be updated exactly once by its own path, so the mixture lies between the two
