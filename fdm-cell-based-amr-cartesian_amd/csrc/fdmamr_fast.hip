@@ -546,6 +546,11 @@ __device__ __forceinline__ float isq_f(float k1, float k2, float k3, FP b) {
            k2 * k2 * b[4] * (1.f / 3.f) * b[1] + 0.5f * k2 * k3 * b[2] * b[3] + k3 * k3 * b[0] * b[5] * (1.f / 3.f);
 }
 
+template <bool STAGED>
+__device__ __forceinline__ void slot_half_eval(const DevTables& t, const float* __restrict__ f,
+                                               float4* __restrict__ Qf, int s, int h, const float2* val,
+                                               const float* srec);
+
 // One component of a WLS-WENO slot in FP32 from its nine point values (record f: a global
 // pointer, or an LDS one when the block staged the record).
 typedef const __attribute__((address_space(3))) float* lds_fptr;
@@ -595,6 +600,26 @@ __device__ __forceinline__ float slot_component(FP f, const float* phi) {
     return a0 * ia * ((1.f / 0.98f) * vl - (0.01f / 0.98f) * (vs1 + vs2)) + (a1 * ia) * vs1 + (a2 * ia) * vs2;
 }
 
+// The id row of slot s (three 16-byte loads: m word, m1, ids 0..8, pattern) and the halves h
+// of its m point values (0 past m). Every point load is issued without a branch (points past m
+// re-read point 0, which every WLS slot has), so the nine gathers are in flight together.
+__device__ __forceinline__ void slot_points(const DevTables& t, const float4* __restrict__ Qf, int s, int h,
+                                            float2 val[9], int& mw, int& pat) {
+    const int4* __restrict__ r = reinterpret_cast<const int4*>(t.wls_i + (size_t)s * WLSI_STRIDE);
+    static_assert(WLSI_STRIDE == 12 && WLSI_M == 0 && WLSI_IDS == 2 && WLSI_PAT == 11, "id row layout");
+    const int4 a = r[0], b = r[1], c = r[2];
+    const int ids[9] = {a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z};
+    mw = a.x;
+    pat = c.w;
+    const int m = mw & WLSI_MMASK;
+    const float2* __restrict__ Q2 = reinterpret_cast<const float2*>(Qf);
+    float2 v[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) v[i] = Q2[(size_t)2 * (i < m ? ids[i] : ids[0]) + h];
+#pragma unroll
+    for (int i = 0; i < 9; i++) val[i] = i < m ? v[i] : make_float2(0.f, 0.f);
+}
+
 // A WLS slot on two lanes: lane h of the pair gathers the (x, y) or (z, w) halves of the point
 // values and evaluates components 2h, 2h + 1, so twice as many waves hide the gather latency
 // and each runs half the arithmetic. GHOSTS (the launch that also writes the ghosts, slots
@@ -604,15 +629,15 @@ template <bool GHOSTS, bool STAGED>
 __device__ __forceinline__ void slot_half(const DevTables& t, const float* __restrict__ wf, float4* __restrict__ Qf,
                                           int s, int h, const float* srec = nullptr) {
     const int32_t* __restrict__ iv = t.wls_i + (size_t)s * WLSI_STRIDE;
-    const float* __restrict__ f = wf + (size_t)iv[WLSI_PAT] * WF_STRIDE;
-    const int mw = iv[WLSI_M];
-    const int m = mw & WLSI_MMASK;
-    const float2* __restrict__ Q2 = reinterpret_cast<const float2*>(Qf);
     float2 val[9];
+    int mw, pat;
     if (!GHOSTS) {
-#pragma unroll
-        for (int i = 0; i < 9; i++) val[i] = i < m ? Q2[(size_t)2 * iv[WLSI_IDS + i] + h] : make_float2(0.f, 0.f);
+        slot_points(t, Qf, s, h, val, mw, pat);
     } else {
+        mw = iv[WLSI_M];
+        pat = iv[WLSI_PAT];
+        const int m = mw & WLSI_MMASK;
+        const float2* __restrict__ Q2 = reinterpret_cast<const float2*>(Qf);
         const int gm = mw >> WLSI_GSHIFT;
         const int d = s - t.S_wp;
         const int4 hdr = t.dep_hdr[d];
@@ -642,6 +667,15 @@ __device__ __forceinline__ void slot_half(const DevTables& t, const float* __res
                      : i < m && !((gm >> i) & 1) ? Q2[(size_t)2 * iv[WLSI_IDS + i] + h]
                                                  : make_float2(0.f, 0.f);
     }
+    slot_half_eval<STAGED>(t, wf + (size_t)pat * WF_STRIDE, Qf, s, h, val, srec);
+}
+
+// The arithmetic half of a slot: components 2h, 2h + 1 from the gathered point halves val,
+// the record f (global) or, with STAGED, the block's staged copy; the result to slot s.
+template <bool STAGED>
+__device__ __forceinline__ void slot_half_eval(const DevTables& t, const float* __restrict__ f,
+                                               float4* __restrict__ Qf, int s, int h, const float2* val,
+                                               const float* srec) {
     float out[2];
     if (STAGED) {
         // the caller checked that the record is staged
@@ -679,17 +713,43 @@ __global__ __launch_bounds__(2 * SLOT_BLK) void k_slots_fast(DevTables t, const 
         if (k < t.S_unc) slot_half<false, false>(t, wf, Qf, t.slot_uncov[k], threadIdx.x & 1);
         return;
     }
-    __shared__ __align__(16) float srec[SLOT_BLK_PATS * WF_STRIDE];
+    constexpr int SREC_PIECES = (SLOT_BLK_PATS * WF_STRIDE / 4 + 2 * SLOT_BLK - 1) / (2 * SLOT_BLK) * (2 * SLOT_BLK);
+    __shared__ __align__(16) float srec[SREC_PIECES * 4];
+    // the two dependent load chains of a block, its pattern list -> its records and each slot's
+    // id row -> its point values, are issued interleaved, so they overlap
+    const int s = blockIdx.x * SLOT_BLK + ((int)threadIdx.x >> 1);
+    const int h = threadIdx.x & 1;
+    const bool in = s < t.S_w;
+    // the block's records: each thread's STAGE_IT float4 pieces, their pattern ids loaded
+    // first, then the pieces, all without branches (an empty list entry and the padding past
+    // the list re-read pattern 0), so the staging costs two round trips
     const int32_t* __restrict__ bp = t.slot_blk_pat + (size_t)blockIdx.x * SLOT_BLK_PATS;
     constexpr int Q4 = WF_STRIDE / 4;
-    for (int k = threadIdx.x; k < SLOT_BLK_PATS * Q4; k += 2 * SLOT_BLK) {
-        const int q = k / Q4, c = k - q * Q4;
-        const int p = bp[q];
-        if (p >= 0) reinterpret_cast<float4*>(srec)[k] = reinterpret_cast<const float4*>(wf + (size_t)p * WF_STRIDE)[c];
+    constexpr int NPIECE = SLOT_BLK_PATS * Q4;
+    constexpr int STAGE_IT = (NPIECE + 2 * SLOT_BLK - 1) / (2 * SLOT_BLK);
+    int pid[STAGE_IT];
+#pragma unroll
+    for (int j = 0; j < STAGE_IT; j++) {
+        const int k = threadIdx.x + j * 2 * SLOT_BLK;
+        pid[j] = bp[(k < NPIECE ? k : 0) / Q4];
     }
+    float2 val[9];
+    int mw, pat;
+    slot_points(t, Qf, in ? s : 0, h, val, mw, pat);
+    const int lpat = t.slot_lpat[in ? s : 0];
+    float4 piece[STAGE_IT];
+#pragma unroll
+    for (int j = 0; j < STAGE_IT; j++) {
+        const int k = threadIdx.x + j * 2 * SLOT_BLK;
+        const int c = (k < NPIECE ? k : 0) % Q4;
+        piece[j] = reinterpret_cast<const float4*>(wf + (size_t)max(pid[j], 0) * WF_STRIDE)[c];
+    }
+    // stored unconditionally (srec is padded to STAGE_IT pieces per thread; entries of the list
+    // that no slot uses may hold pattern 0), so no load is sunk into a branch
+#pragma unroll
+    for (int j = 0; j < STAGE_IT; j++) reinterpret_cast<float4*>(srec)[threadIdx.x + j * 2 * SLOT_BLK] = piece[j];
     __syncthreads();
-    const int s = blockIdx.x * SLOT_BLK + ((int)threadIdx.x >> 1);
-    if (s < t.S_w && t.slot_lpat[s] < SLOT_BLK_PATS) slot_half<false, true>(t, wf, Qf, s, threadIdx.x & 1, srec);
+    if (in && lpat < SLOT_BLK_PATS) slot_half_eval<true>(t, nullptr, Qf, s, h, val, srec);
 }
 
 // Ghost leaves and restriction slots of the fast path from the flattened lists: item k is
