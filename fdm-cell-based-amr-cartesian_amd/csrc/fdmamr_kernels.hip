@@ -431,21 +431,34 @@ __device__ __forceinline__ double isq_poly(double k1, double k2, double k3, cons
 __global__ __launch_bounds__(128) void k_slots_wls(DevTables t, double4* __restrict__ Q) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= t.S_w) return;
-    const int32_t* __restrict__ iv = t.wls_i + (size_t)s * WLSI_STRIDE;
-    const double* __restrict__ d = t.wls_d + (size_t)iv[WLSI_PAT] * WLS_STRIDE;
+    // the id row as three 16-byte loads (size word, m1, ids 0..8, pattern)
+    static_assert(WLSI_STRIDE == 12 && WLSI_M == 0 && WLSI_M1 == 1 && WLSI_IDS == 2 && WLSI_PAT == 11,
+                  "id row layout");
+    const int4* __restrict__ row = reinterpret_cast<const int4*>(t.wls_i + (size_t)s * WLSI_STRIDE);
+    const int4 ra = row[0], rb = row[1], rc = row[2];
+    const int ids[9] = {ra.z, ra.w, rb.x, rb.y, rb.z, rb.w, rc.x, rc.y, rc.z};
+    const double* __restrict__ d = t.wls_d + (size_t)rc.w * WLS_STRIDE;
     // ghosts are filled before this launch, so the ghost mask above the size byte is not used
-    const int m = iv[WLSI_M] & WLSI_MMASK, m1 = iv[WLSI_M1];
+    const int m = ra.x & WLSI_MMASK, m1 = ra.y;
     const double xx = d[WLS_XX], yy = d[WLS_YY];
     const double* box = d + WLS_BOX;
     const double* __restrict__ qd = reinterpret_cast<const double*>(Q);
     const double lw0 = 0.98, lw1 = 0.01, lw2 = 0.01;
+    // the inverse-distance weights, loaded without a branch (the record always holds nine)
+    double wgt[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) wgt[i] = d[WLS_W + i];
     double out[4];
     // one component at a time (not unrolled): the register footprint of one fit
 #pragma nounroll
     for (int k = 0; k < 4; k++) {
-        double wphi[9];
+        // the nine point values issued together (points past m re-read point 0, every WLS
+        // slot has one, and are then dropped)
+        double q[9], wphi[9];
 #pragma unroll
-        for (int i = 0; i < 9; i++) wphi[i] = i < m ? d[WLS_W + i] * qd[(size_t)iv[WLSI_IDS + i] * 4 + k] : 0.0;
+        for (int i = 0; i < 9; i++) q[i] = qd[(size_t)(i < m ? ids[i] : ids[0]) * 4 + k];
+#pragma unroll
+        for (int i = 0; i < 9; i++) wphi[i] = i < m ? wgt[i] * q[i] : 0.0;
         double b6[6] = {0, 0, 0, 0, 0, 0}, b3a[3] = {0, 0, 0}, b3b[3] = {0, 0, 0};
         for (int i = 0; i < 9; i++) {
             if (i >= m) break;
