@@ -484,7 +484,7 @@ void enqueue_rk3(fdmamr_ctx* c, int advance_time, bool prepare_next) {
         int* posflag = &c->sc->pos_bad;
         if (c->profiling && (e0 = next_event(c)) >= 0) (void)hipEventRecord(c->ev_pool[e0], s);
         if (fast)
-            launch_stage_fast(d, lists, c->list_meta.p, c->Q[in[k]].p, c->Qf[in[k]].p, c->Q[0].p,
+            launch_stage_fast(d, lists, c->list_meta.p, c->flags.p, c->Q[in[k]].p, c->Qf[in[k]].p, c->Q[0].p,
                               c->Q[out[k]].p, c->Qf[out[k]].p, c->sc, ra[k], rb[k], k == 2 ? 1 : 0,
                               c->partials.p, (prepare_next && k == 2) ? c->lampart.p : nullptr,
                               (select && k == 0) ? c->label.p : nullptr,
@@ -1098,7 +1098,7 @@ int32_t fdmamr_labels(fdmamr_ctx* c, float* dev) {
     HIPCHK(c, hipMemcpyAsync(c->sc, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
     launch_shadow(c->Q[0].p, c->Qf[0].p, c->tb.L + c->tb.S_w + c->tb.S_r, c->stream);
     launch_slots_fast(c->dt, c->wls_f.p, c->Q[0].p, c->Qf[0].p, c->stream);
-    launch_stage_fast(c->dt, nullptr, nullptr, c->Q[0].p, c->Qf[0].p, c->Q[0].p, c->Q[2].p, c->Qf[2].p,
+    launch_stage_fast(c->dt, nullptr, nullptr, nullptr, c->Q[0].p, c->Qf[0].p, c->Q[0].p, c->Q[2].p, c->Qf[2].p,
                       c->sc, 0.0, 1.0, 0, c->partials.p, nullptr, c->label.p, JudgeArgs{}, nullptr, c->stream);
     HIPCHK(c, hipGetLastError());
     std::vector<float> lab(c->tb.N);
@@ -1267,7 +1267,7 @@ int32_t fdmamr_selector_collect(fdmamr_ctx* c, int32_t* n_rows, int32_t* n_posit
     HIPCHK(c, hipMemcpyAsync(c->sc, &s, sizeof s, hipMemcpyHostToDevice, c->stream));
     launch_shadow(c->Q[0].p, c->Qf[0].p, c->tb.L + c->tb.S_w + c->tb.S_r, c->stream);
     launch_slots_fast(c->dt, c->wls_f.p, c->Q[0].p, c->Qf[0].p, c->stream);
-    launch_stage_fast(c->dt, nullptr, nullptr, c->Q[0].p, c->Qf[0].p, c->Q[0].p, c->Q[2].p, c->Qf[2].p,
+    launch_stage_fast(c->dt, nullptr, nullptr, nullptr, c->Q[0].p, c->Qf[0].p, c->Q[0].p, c->Q[2].p, c->Qf[2].p,
                       c->sc, 0.0, 1.0, 0, c->partials.p, nullptr, c->label.p, JudgeArgs{}, nullptr, c->stream);
     launch_features(c->dt, c->Qf[0].p, c->off16.p, c->ds_x.p, c->ds_rows, c->stream);
     launch_labels_to_y(N, c->label.p, c->sel_bound, c->ds_y.p, c->ds_rows, c->stream);

@@ -183,13 +183,14 @@ struct JudgeArgs {
     int* cnt = nullptr;
     float bound = 0.f;
 };
-// lists: cleared leaves then troubled leaves (meta[0] = number cleared), or nullptr for the
-// full path on every leaf; residual partials occupy stage_fast_partial_blocks() blocks. With
+// lists: cleared leaves then troubled leaves (meta[0] = number cleared; the troubled part feeds
+// the full path, the cleared leaves are taken from flags bit 0), or nullptr for the full path
+// on every leaf; residual partials occupy stage_fast_partial_blocks() blocks. With
 // label, each troubled leaf's largest weight deviation goes to label, and with jd.flags (lists
 // only) the judge runs on them in the same launch. posflag (or nullptr) is set to 1 when a
 // leaf's FP64 result has rho <= 0 or p <= 0.
-void launch_stage_fast(const DevTables& t, const int32_t* lists, const int* meta, const double4* Qin,
-                       const float4* Qf_in, const double4* Qn, double4* Qout, float4* Qf_out,
+void launch_stage_fast(const DevTables& t, const int32_t* lists, const int* meta, const uint8_t* flags,
+                       const double4* Qin, const float4* Qf_in, const double4* Qn, double4* Qout, float4* Qf_out,
                        const DevScalars* sc, double alpha, double beta, int reduce, double* partials,
                        double* lampart, float* label, const JudgeArgs& jd, int* posflag, hipStream_t st);
 int stage_fast_partial_blocks(int n, bool lists);

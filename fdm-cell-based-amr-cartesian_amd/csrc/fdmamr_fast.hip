@@ -252,28 +252,23 @@ __device__ __forceinline__ void stage_leaf_lin(const DevTables& t, int i, const 
                 posflag);
 }
 
-// Linear (cleared) leaves: lists[0, n_clear) (meta[0] = n_clear), ascending leaf order.
+// Linear (cleared) leaves: block b covers leaves [256 lb, 256 lb + 256) (lb its XCD-renumbered
+// index) and skips the troubled ones (flag bit 0, which stays set through the step: the judge
+// only adds bit 2), so the leaf index needs no list load and every stream is contiguous.
 // Residual partials of logical block b go to partials[(poff + b) * 8 ...].
 // Qn/Qout may alias (in-place final stage), so they carry no __restrict__.
-__device__ __forceinline__ void lin_block(int blk, const DevTables& t, const int32_t* __restrict__ lists,
-                                          const int* __restrict__ meta, const double4* __restrict__ Qin,
+__device__ __forceinline__ void lin_block(int blk, const DevTables& t, const uint8_t* flags,
+                                          const double4* __restrict__ Qin,
                                           const float4* __restrict__ Qf_in, const double4* Qn, double4* Qout,
                                           float4* __restrict__ Qf_out, const DevScalars* __restrict__ sc,
                                           double alpha, double beta, int reduce, double* __restrict__ partials,
                                           double* __restrict__ lampart, int poff, int* posflag) {
-    const int n = meta[0];
-    // the active prefix of the grid is spread over all XCDs, then renumbered within itself
-    const int nact = (n + 255) >> 8;
-    if (blk >= nact) {
-        if (reduce) block_partials_zero(partials, poff + blk, lampart);
-        return;
-    }
-    const int lb = xcd_block_n(blk, nact);
-    const int p = lb * 256 + (int)threadIdx.x;
+    const int lb = xcd_block_n(blk, (t.N + 255) >> 8);
+    const int i = lb * 256 + (int)threadIdx.x;
     double rmax[4] = {0, 0, 0, 0}, rsum[4] = {0, 0, 0, 0};
     double lx = 0.0, ly = 0.0;
-    if (p < n)
-        stage_leaf_lin(t, lists[p], Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, rmax, rsum, lampart != nullptr,
+    if (i < t.N && !(flags[i] & 1))
+        stage_leaf_lin(t, i, Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, rmax, rsum, lampart != nullptr,
                        lx, ly, posflag);
     if (reduce) block_partials(rmax, rsum, partials, poff + lb, lampart, lx, ly);
 }
@@ -377,7 +372,8 @@ __device__ __forceinline__ void full_block(int blk, const DevTables& t, const in
             if (f & 2) {
                 judged = true;
                 rej = dev > jd.bound;
-                jd.flags[i] = rej ? (uint8_t)5 : (uint8_t)(f & 4);
+                // bits 0 and 1 stay (the leaf is on the full path for the whole step)
+                if (rej) jd.flags[i] = (uint8_t)(f | 4);
             }
         }
         const uint64_t bj = __ballot(judged), br = __ballot(rej);
@@ -404,13 +400,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                                                      const DevScalars* __restrict__ sc, double alpha, double beta,
                                                      int reduce, double* __restrict__ partials,
                                                      double* __restrict__ lampart, int nb4, float* __restrict__ label,
-                                                     JudgeArgs jd, int* posflag) {
+                                                     JudgeArgs jd, int* posflag, const uint8_t* flags) {
     if ((int)blockIdx.x < nb4)
         full_block<LABEL>(blockIdx.x, t, lists, meta, Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, reduce,
                           partials, lampart, 0, label, jd, posflag);
     else
-        lin_block(blockIdx.x - nb4, t, lists, meta, Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, reduce,
-                  partials, lampart, nb4, posflag);
+        lin_block(blockIdx.x - nb4, t, flags, Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials,
+                  lampart, nb4, posflag);
 }
 
 // every leaf on the full path (no flags)
@@ -867,8 +863,8 @@ void launch_slots_fill(const DevTables& t, const float* wf, double4* Q, float4* 
                            nitems, beg, leaf, w);
 }
 
-void launch_stage_fast(const DevTables& t, const int32_t* lists, const int* meta, const double4* Qin,
-                       const float4* Qf_in, const double4* Qn, double4* Qout, float4* Qf_out,
+void launch_stage_fast(const DevTables& t, const int32_t* lists, const int* meta, const uint8_t* flags,
+                       const double4* Qin, const float4* Qf_in, const double4* Qn, double4* Qout, float4* Qf_out,
                        const DevScalars* sc, double alpha, double beta, int reduce, double* partials,
                        double* lampart, float* label, const JudgeArgs& jd, int* posflag, hipStream_t st) {
     if (t.N <= 0) return;
@@ -876,11 +872,12 @@ void launch_stage_fast(const DevTables& t, const int32_t* lists, const int* meta
     if (lists) {
         if (label)
             hipLaunchKernelGGL(k_stage_mixed<true>, dim3(nb4 + nb), dim3(256), 0, st, t, lists, meta, Qin, Qf_in,
-                               Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials, lampart, nb4, label, jd, posflag);
+                               Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials, lampart, nb4, label, jd, posflag,
+                               flags);
         else
             hipLaunchKernelGGL(k_stage_mixed<false>, dim3(nb4 + nb), dim3(256), 0, st, t, lists, meta, Qin, Qf_in,
                                Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials, lampart, nb4, label,
-                               JudgeArgs{}, posflag);
+                               JudgeArgs{}, posflag, flags);
     } else if (label) {
         hipLaunchKernelGGL(k_stage_full<true>, dim3(nb4), dim3(256), 0, st, t, Qin, Qf_in, Qn, Qout, Qf_out, sc,
                            alpha, beta, reduce, partials, lampart, label, posflag);
@@ -890,7 +887,7 @@ void launch_stage_fast(const DevTables& t, const int32_t* lists, const int* meta
     }
 }
 
-// the full blocks (four lanes per leaf), then the linear ones when lists are used
+// the full blocks (four lanes per leaf), then the linear ones (one per leaf) when lists are used
 int stage_fast_partial_blocks(int n, bool lists) { return (lists ? nb_(n, 256) : 0) + nb_(4 * n, 256); }
 
 void launch_build_lists(int n, const uint8_t* flags, int* blk_cnt, int32_t* lists, int* meta, hipStream_t st) {

@@ -702,8 +702,9 @@ __global__ __launch_bounds__(256) void k_reduce_parts(const double* __restrict__
                                                       double* __restrict__ out) {
     int n0 = nblk, n1 = 0;
     if (meta) {
+        // the full blocks of the troubled leaves, then every linear block
         n0 = (4 * (n_leaf - meta[0]) + 255) >> 8;
-        n1 = (meta[0] + 255) >> 8;
+        n1 = (n_leaf + 255) >> 8;
     }
     const int total = n0 + n1;
     double acc[10];

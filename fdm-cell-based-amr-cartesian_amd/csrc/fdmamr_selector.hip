@@ -332,7 +332,8 @@ __device__ __forceinline__ bf16x8 act_pack(const f32x4& lo, const f32x4& hi) {
 // sent through the full path with a label); bit 2: forced troubled by an earlier judge
 // rejection (sticky until the tables are rebuilt); bit 3: made by the last regrid (refined
 // son or coarsened parent: copied or averaged values) or reading such a leaf in its stencil,
-// troubled in the first step after the regrid.
+// troubled in the first step after the regrid. Bits 0 and 1 hold for the whole step (the
+// judge only adds bit 2), so the stage launches can read the troubled set from the flags.
 __device__ __forceinline__ uint8_t decide(uint8_t old, bool troubled, int leaf, const SelectParams& sp) {
     uint8_t f = old & 4;
     if (troubled || f || (old & 8)) f |= 1;
@@ -562,8 +563,8 @@ __global__ __launch_bounds__(256) void k_judge(int n, const float* __restrict__ 
         troubled += f & 1;
         if (f & 2) {
             judged++;
-            if (label[i] > bound) { rejected++; flags[i] = 5; }
-            else flags[i] = f & 4;
+            // bits 0 and 1 stay (the leaf is on the full path for the whole step)
+            if (label[i] > bound) { rejected++; flags[i] = (uint8_t)(f | 4); }
         }
     }
     for (int o = 32; o > 0; o >>= 1) {
