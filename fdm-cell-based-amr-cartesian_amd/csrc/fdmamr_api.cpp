@@ -76,6 +76,8 @@ struct fdmamr_ctx {
     uint32_t sel_step = 0;
     int adam_t = 0;
     DBuf<int32_t> stencil, wls_i, rst, bcp, son, state;
+    DBuf<int16_t> st16;        // 16-byte stencil records (fast-path stage)
+    DBuf<int32_t> st16_base;
     DBuf<uint8_t> face_share;  // FP64 path: which lower faces equal the neighbour's upper face
     DBuf<double4> fxp, fyp;    // FP64 path: every leaf's x+1/2 and y+1/2 face fluxes of a stage
     DBuf<double4> fxm, fym;    // FP64 path: the lower faces that are not shared
@@ -245,6 +247,10 @@ int upload_tables(fdmamr_ctx* c) {
     hipStream_t s = c->stream;
     auto up = [&](void* d, const void* h, size_t bytes) -> hipError_t { return upload_staged(c, d, h, bytes); };
     HIPCHK(c, up(c->stencil.p, tb.stencil.data(), tb.stencil.size() * 4));
+    HIPCHK(c, ensure(c->st16, std::max<size_t>(tb.st16.size(), 8)));
+    HIPCHK(c, ensure(c->st16_base, std::max<size_t>(tb.st16_base.size(), 1)));
+    if (!tb.st16.empty()) HIPCHK(c, up(c->st16.p, tb.st16.data(), tb.st16.size() * 2));
+    if (!tb.st16_base.empty()) HIPCHK(c, up(c->st16_base.p, tb.st16_base.data(), tb.st16_base.size() * 4));
     HIPCHK(c, ensure(c->face_share, tb.face_share.size()));
     HIPCHK(c, up(c->face_share.p, tb.face_share.data(), tb.face_share.size()));
     if (c->p.flux_precision == FDMAMR_FLUX_FP64) {
@@ -348,7 +354,8 @@ int upload_tables(fdmamr_ctx* c) {
     d.N = tb.N; d.G = tb.G; d.P = tb.P; d.L = tb.L; d.S_w = tb.S_w; d.S_r = tb.S_r; d.S_wp = tb.S_wp;
     d.gamma = t.gamma; d.cv = t.cv; d.h = t.h; d.xf = t.xf; d.Ma = t.Ma;
     for (int k = 0; k < 4; k++) d.inflow[k] = t.inflow[k];
-    d.stencil = c->stencil.p; d.lvl = c->lvl.p; d.wls_d = c->wls_d.p; d.wls_i = c->wls_i.p;
+    d.stencil = c->stencil.p; d.lvl = c->lvl.p;
+    d.st16 = c->st16.p; d.st16_base = c->st16_base.p; d.wls_d = c->wls_d.p; d.wls_i = c->wls_i.p;
     d.dep_hdr = reinterpret_cast<const int4*>(c->dep_hdr.p);
     d.dep_leaf = reinterpret_cast<const int4*>(c->dep_leaf.p);
     d.dep_w = reinterpret_cast<const float4*>(c->dep_w.p);
@@ -558,7 +565,7 @@ void fdmamr_destroy(fdmamr_ctx* c) {
     (void)hipSetDevice(c->p.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int k = 0; k < 3; k++) { release(c->Q[k]); release(c->T[k]); }
-    release(c->stencil); release(c->face_share); release(c->fxp); release(c->fyp); release(c->fxm); release(c->fym); release(c->face_lo); release(c->wls_i); release(c->rst); release(c->bcp); release(c->son);
+    release(c->stencil); release(c->st16); release(c->st16_base); release(c->face_share); release(c->fxp); release(c->fyp); release(c->fxm); release(c->fym); release(c->face_lo); release(c->wls_i); release(c->rst); release(c->bcp); release(c->son);
     release(c->state); release(c->lvl); release(c->wls_d); release(c->partials); release(c->lampart);
     release(c->rot); release(c->div); release(c->tmp); release(c->parts); release(c->nbad);
     release(c->Qf[0]); release(c->Qf[1]); release(c->Qf[2]); release(c->wls_f); release(c->flags);

@@ -93,6 +93,8 @@ struct DevTables {
     double gamma, cv, h, xf, Ma;
     double inflow[4];
     const int32_t* stencil;     // [N][8]: one 32-byte record per leaf (+1x, -1x, +1y, -1y, +2x, -2x, +2y, -2y)
+    const int16_t* st16;        // [N][8]: the same in 16 bytes (Tables::st16)
+    const int32_t* st16_base;   // [ceil(N / ST16_BLK)]
     const int8_t* lvl;          // [N]
     const double* wls_d;        // [S_w][WLS_STRIDE]
     const int32_t* wls_i;       // [S_w][WLSI_STRIDE]
@@ -122,6 +124,26 @@ __device__ __forceinline__ void load_stencil(const DevTables& t, int i, int s[8]
     const int4 a = r[0], b = r[1];
     s[0] = a.x; s[1] = a.y; s[2] = a.z; s[3] = a.w;
     s[4] = b.x; s[5] = b.y; s[6] = b.z; s[7] = b.w;
+}
+
+// the same from the 16-byte record (one 16-byte load and the block's base); a leaf whose
+// record is escaped reads the 32-byte one
+__device__ __forceinline__ void load_stencil16(const DevTables& t, int i, int s[8]) {
+    const int4 r = *reinterpret_cast<const int4*>(t.st16 + (size_t)i * 8);
+    const int base = t.st16_base[i / ST16_BLK];
+    const int w[4] = {r.x, r.y, r.z, r.w};
+    int e[8];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        e[2 * k] = (int)(int16_t)(w[k] & 0xFFFF);
+        e[2 * k + 1] = w[k] >> 16;
+    }
+    if (e[0] == ST16_ESC) {
+        load_stencil(t, i, s);
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) s[k] = e[k] >= ST16_FAR ? base + (e[k] - ST16_FAR) : i + e[k];
 }
 
 // Scalars of the device-resident step loop (solution.F90:183-215).

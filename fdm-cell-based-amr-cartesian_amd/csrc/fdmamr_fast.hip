@@ -232,7 +232,7 @@ __device__ __forceinline__ void stage_leaf_lin(const DevTables& t, int i, const 
                                                int* posflag) {
     const double gama = t.gamma;
     int s[8];
-    load_stencil(t, i, s);
+    load_stencil16(t, i, s);
     const double4 qc = Qin[i];
     // the leaf's own FP32 value is the rounding of its FP64 state (the shadow holds exactly that)
     const float4 qcf = make_float4((float)qc.x, (float)qc.y, (float)qc.z, (float)qc.w);
@@ -306,10 +306,11 @@ __device__ __forceinline__ void full_block(int blk, const DevTables& t, const in
     const int i = valid ? (lists ? lists[base + p] : p) : 0;
     // stencil rows of i-2, i-1, i+1, i+2: (5, 1, 0, 4) in x, (7, 3, 2, 6) in y. Face i-1/2 takes
     // points i-2, i-1, i, i+1; face i+1/2 takes i-1, i, i+1, i+2.
+    // the four lanes of a leaf read from the leaf's one 32-byte stencil record (this path is
+    // bound by its arithmetic; decoding the 16-byte record costs more than it saves)
     const int row0 = dir ? (side ? 3 : 7) : (side ? 1 : 5);
     const int row1 = dir ? (side ? 2 : 3) : (side ? 0 : 1);
     const int row2 = dir ? (side ? 6 : 2) : (side ? 4 : 0);
-    // the four lanes of a leaf read from the leaf's one 32-byte stencil record
     const int32_t* rec = t.stencil + (size_t)i * 8;
     const int s0 = rec[row0];
     const int s1 = rec[row1];

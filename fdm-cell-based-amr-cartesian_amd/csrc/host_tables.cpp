@@ -1078,6 +1078,47 @@ void build_tables(const Tree& t, Tables& tb) {
         }
     });
     pt.mark("tables: checks");
+    // 16-byte stencil records (Tables::st16): per block of ST16_BLK leaves the smallest id of
+    // a ghost or slot its leaves read is the base of the far entries; then every record is
+    // encoded and decoded back, and must give the 32-byte record's ids
+    {
+        const size_t nblk = ((size_t)N + ST16_BLK - 1) / ST16_BLK;
+        tb.st16_base.assign(std::max<size_t>(nblk, 1), 0);
+        resize_fresh(tb.st16, (size_t)N * 8);
+        std::atomic<int> escaped{0}, bad{0};
+        parallel_for(nblk, [&](size_t b0, size_t e0) {
+            int esc = 0;
+            for (size_t b = b0; b < e0; b++) {
+                const size_t i0 = b * ST16_BLK, i1 = std::min((size_t)N, i0 + ST16_BLK);
+                int32_t base = INT32_MAX;
+                for (size_t k = i0 * 8; k < i1 * 8; k++)
+                    if (tb.stencil[k] >= N) base = std::min(base, tb.stencil[k]);
+                if (base == INT32_MAX) base = 0;
+                tb.st16_base[b] = base;
+                for (size_t i = i0; i < i1; i++) {
+                    int16_t e[8];
+                    bool ok = true;
+                    for (int k = 0; k < 8; k++) {
+                        const int64_t r = tb.stencil[i * 8 + k];
+                        const int64_t d = r - (int64_t)i, f = r - base;
+                        if (r < N && d > -ST16_FAR && d < ST16_FAR) e[k] = (int16_t)d;
+                        else if (r >= N && f >= 0 && f < 32768 - ST16_FAR) e[k] = (int16_t)(ST16_FAR + f);
+                        else ok = false;
+                    }
+                    for (int k = 0; k < 8; k++) {
+                        tb.st16[i * 8 + k] = ok ? e[k] : ST16_ESC;
+                        const int32_t back = e[k] >= ST16_FAR ? base + (e[k] - ST16_FAR) : (int32_t)i + e[k];
+                        if (ok && back != tb.stencil[i * 8 + k]) bad = 1;
+                    }
+                    esc += !ok;
+                }
+            }
+            escaped += esc;
+        });
+        if (bad) throw std::runtime_error("compressed stencil record wrong");
+        tb.st16_escaped = escaped;
+    }
+    pt.mark("tables: 16-byte records");
 }
 
 }  // namespace fdm

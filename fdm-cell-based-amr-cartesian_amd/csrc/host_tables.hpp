@@ -56,6 +56,9 @@ enum WlsFastLayout {
 
 constexpr int32_t REF_INFLOW = -2147483647 - 1;
 constexpr int SEL_NBR_CAP = 192;  // positions fit a byte
+// compressed stencil records (Tables::st16)
+constexpr int ST16_BLK = 256, ST16_FAR = 16384;
+constexpr int16_t ST16_ESC = -32768;
 // fast-path slot launch: slots per block (two lanes each) and the most distinct FP32 records a
 // block stages in LDS
 constexpr int SLOT_BLK = 128, SLOT_BLK_PATS = 48;
@@ -74,6 +77,13 @@ struct Tables {
     // is the cell_NS position of device leaf i, dev_of_ns its inverse
     BigVec<int32_t> ns_pos, dev_of_ns;
     BigVec<int32_t> stencil;             // [N][8]: state index, or L + slot index
+    // The same records in 16 bytes for the fast path's stage launch: entry e of leaf i is the
+    // leaf i + e for |e| < ST16_FAR, or the state st16_base[i / ST16_BLK] + (e - ST16_FAR) for
+    // e >= ST16_FAR (ghosts and level-jump slots, numbered in leaf order); a leaf with an entry
+    // neither form holds has every entry ST16_ESC and is read from `stencil`.
+    BigVec<int16_t> st16;                // [N][8]
+    BigVec<int32_t> st16_base;           // [ceil(N / ST16_BLK)]
+    int st16_escaped = 0;                // leaves read from `stencil`
     // bit 0: the leaf's x-1/2 face reads the same four state indices as its left neighbour's
     // x+1/2 face (so both compute the same flux, bit for bit); bit 1: the same in y
     BigVec<uint8_t> face_share;          // [N]
