@@ -234,21 +234,29 @@ __device__ __forceinline__ void stage_leaf_lin(const DevTables& t, int i, const 
     int s[8];
     load_stencil16(t, i, s);
     const double4 qc = Qin[i];
+    // U_n with the stencil record, and all eight neighbour values together (the scheduling
+    // barriers keep the compiler from sinking the loads to their uses): two round trips per leaf
+    const double4 qn = Qn[i];
+    __builtin_amdgcn_sched_barrier(0);
     // the leaf's own FP32 value is the rounding of its FP64 state (the shadow holds exactly that)
     const float4 qcf = make_float4((float)qc.x, (float)qc.y, (float)qc.z, (float)qc.w);
     const int lv = t.lvl[i];
     const double dy = t.h / (double)(1 << lv);
     const double dx = t.xf * dy;
+    float4 nb[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) nb[k] = Qf_in[s[k]];
+    __builtin_amdgcn_sched_barrier(0);
     float dE[4], dF[4];
     {
-        const float4 col[5] = {Qf_in[s[5]], Qf_in[s[1]], qcf, Qf_in[s[0]], Qf_in[s[4]]};
+        const float4 col[5] = {nb[5], nb[1], qcf, nb[0], nb[4]};
         flux_lin_f<0>(col, (float)dx, (float)sc->lam_x, (float)gama, dE);
     }
     {
-        const float4 col[5] = {Qf_in[s[7]], Qf_in[s[3]], qcf, Qf_in[s[2]], Qf_in[s[6]]};
+        const float4 col[5] = {nb[7], nb[3], qcf, nb[2], nb[6]};
         flux_lin_f<1>(col, (float)dy, (float)sc->lam_y, (float)gama, dF);
     }
-    update_leaf(i, qc, Qn[i], dE, dF, dx, dy, gama, sc->dt, alpha, beta, Qout, Qf_out, rmax, rsum, lam, lx, ly,
+    update_leaf(i, qc, qn, dE, dF, dx, dy, gama, sc->dt, alpha, beta, Qout, Qf_out, rmax, rsum, lam, lx, ly,
                 posflag);
 }
 
