@@ -1453,7 +1453,7 @@ int32_t fdmamr_set_slot_merge(fdmamr_ctx* c, int32_t enable) {
     return 0;
 }
 
-int32_t fdmamr_table_info(fdmamr_ctx* c, int32_t* info8) {  // 10 entries
+int32_t fdmamr_table_info(fdmamr_ctx* c, int32_t* info8) {  // 11 entries
     if (!c || !info8) return -1;
     if (!c->tables_ready) return fail(c, -2, "fdmamr_table_info: tables not built");
     const Tables& tb = c->tb;
@@ -1471,9 +1471,9 @@ int32_t fdmamr_table_info(fdmamr_ctx* c, int32_t* info8) {  // 10 entries
     }
     int shx = 0, shy = 0;
     for (uint8_t f : tb.face_share) { shx += f & 1; shy += (f >> 1) & 1; }
-    const int32_t v[10] = {tb.S_wp, tb.S_w, aliased, (int32_t)tb.fill_fast.size(), generic, direct, over64, max_nu,
-                           shx, shy};
-    for (int k = 0; k < 10; k++) info8[k] = v[k];
+    const int32_t v[11] = {tb.S_wp, tb.S_w, aliased, (int32_t)tb.fill_fast.size(), generic, direct, over64, max_nu,
+                           shx, shy, tb.st16_escaped};
+    for (int k = 0; k < 11; k++) info8[k] = v[k];
     return 0;
 }
 

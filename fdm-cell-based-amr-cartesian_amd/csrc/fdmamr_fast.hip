@@ -837,11 +837,14 @@ __global__ __launch_bounds__(128) void k_slots_fill(
     }
 }
 
-// the FP32 values the fast path's stage reads for interior leaf i (8 stencil positions)
+// the FP32 values the fast path's stage reads for interior leaf i (8 stencil positions), the
+// ids decoded from its 16-byte record as the cleared leaves' stage decodes them
 __global__ void k_stencil_values_f(DevTables t, const float4* __restrict__ Qf, int i, float* out) {
     const int k = threadIdx.x;
     if (k < 8 && i < t.N) {
-        const float4 q = Qf[t.stencil[(size_t)i * 8 + k]];
+        int s[8];
+        load_stencil16(t, i, s);
+        const float4 q = Qf[s[k]];
         out[4 * k + 0] = q.x; out[4 * k + 1] = q.y; out[4 * k + 2] = q.z; out[4 * k + 3] = q.w;
     }
 }

@@ -509,12 +509,12 @@ class Solver:
         return lst[: int(nc[0])], lst[int(nc[0]):]
 
     def table_info(self):
-        v = np.zeros(10, np.int32)
+        v = np.zeros(11, np.int32)
         self._check(self.L.fdmamr_table_info(self.ctx, _ip(v)), "table_info")
         return dict(slots_plain=int(v[0]), slots=int(v[1]), ghosts_aliased=int(v[2]),
                     fill_items_fast=int(v[3]), slots_generic=int(v[4]), select_tiles_direct=int(v[5]),
                     select_tiles_over64=int(v[6]), select_tile_max=int(v[7]), faces_shared_x=int(v[8]),
-                    faces_shared_y=int(v[9]))
+                    faces_shared_y=int(v[9]), records16_escaped=int(v[10]))
 
 
 def config_c1(**kw) -> Config:

@@ -180,13 +180,14 @@ int32_t fdmamr_stencil_values(fdmamr_ctx* ctx, int32_t i, double* val8x4);
  * positions, after the launch that evaluates the WLS slots and refills the ghosts of the
  * current state (no reference counterpart; diagnostics) */
 int32_t fdmamr_stencil_values_fast(fdmamr_ctx* ctx, int32_t i, float* val8x4);
-/* regrid-time table facts (no reference counterpart; diagnostics), 10 entries: info8[0] WLS
+/* regrid-time table facts (no reference counterpart; diagnostics), 11 entries: info8[0] WLS
  * slots that read no ghost leaf, [1] WLS slots, [2] ghost leaves read through the leaf they copy
  * (or the first inflow ghost), [3] items the fast path refills per stage (the other ghosts and
  * the restriction slots), [4] ghost-reading slots without a fixed-size entry list, [5] selector
  * tiles gathered without LDS staging, [6] selector tiles reading more than 64 distinct values,
  * [7] the most distinct values of a staged selector tile, [8] / [9] interior leaves whose x-1/2
- * / y-1/2 face the FP64 stage takes from the neighbour (fdmamr_set_face_share) */
+ * / y-1/2 face the FP64 stage takes from the neighbour (fdmamr_set_face_share), [10] interior
+ * leaves whose 16-byte stencil record cannot hold their ids (read from the 32-byte one) */
 int32_t fdmamr_table_info(fdmamr_ctx* ctx, int32_t* info8);
 /* the fast path's leaf lists as the last flagged step built them (no reference counterpart;
  * diagnostics): lists[0, *n_clear) the cleared interior leaves, then the troubled ones, each

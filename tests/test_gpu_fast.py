@@ -49,6 +49,44 @@ def test_all_troubled_flags_equal_unflagged_path_with_level_jumps():
     assert np.array_equal(b.get_state(), c.get_state())
 
 
+@pytest.mark.parametrize("case", ["uniform256", "amr"])
+def test_sixteen_byte_records_decode_on_the_device(case):
+    # The cleared leaves' stage reads its stencil ids from the 16-byte records (offsets, block
+    # bases, escapes to the 32-byte record); the fast path's stencil values decode them the same
+    # way, so at every sampled leaf they must be the FP64 path's values (32-byte records)
+    # rounded to FP32, including the leaves whose records escape (a uniform 256 x 256 mesh
+    # has them along the boundaries of the largest Morton blocks) and level-jump slots.
+    if case == "uniform256":
+        kw = dict(h=1.0 / 256, lvl_amr=0, pre_adapt=0)
+    else:
+        kw = dict(h=1.0 / 64, lvl_amr=3, pre_adapt=3)
+    a = F.Solver(riemann_c2(**kw)).setup()
+    t = a.tree()
+    m, n = int(t["num"][:, 1].max()), int(t["num"][:, 0].max())
+    b = F.Solver(riemann_c2(flux_precision=F.FLUX_FP32, **kw))
+    b.import_tree(m, n, t)
+    b.build_tables()
+    b.set_state(a.get_state())
+    esc = b.table_info()["records16_escaped"]
+    g = b.geometry()
+    x, y = g["x"], g["y"]
+    rng = np.random.default_rng(3)
+    # the leaves next to the base grid's middle lines (largest Morton block boundaries) + a sample
+    mid = np.where((np.abs(x - 0.5) < 3.0 / 256) | (np.abs(y - 0.5) < 3.0 / 256))[0]
+    pick = np.unique(np.concatenate([mid[:: max(1, len(mid) // 1500)], rng.choice(b.n_leaf, 1500, replace=False)]))
+    worst = 0.0
+    for i in pick:
+        va = a.stencil_values(int(i))
+        vb = b.stencil_values_fast(int(i)).astype(np.float64)
+        worst = max(worst, float(np.max(np.abs(va - vb) / (np.abs(va) + 1.0))))
+    print(case, "leaves", b.n_leaf, "escaped", esc, "checked", len(pick), "worst rel diff", worst)
+    if case == "uniform256":
+        assert esc > 0
+        assert worst < 1e-7   # copies: FP32 rounding of the same values
+    else:
+        assert worst < 1e-4   # WLS-WENO slots in FP32
+
+
 def test_mixed_flags_take_both_paths():
     # Random flags split the leaves into the cleared and troubled launch lists; every leaf
     # must be updated exactly once by its own path, so the mixture lies between the two

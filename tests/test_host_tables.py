@@ -149,6 +149,33 @@ def test_ghost_aliases_and_fast_fill_items():
     _compare_stencils(s, o, limit=500)
 
 
+def test_sixteen_byte_records_escape_only_far_entries():
+    # The fast path's 16-byte stencil records hold a neighbour leaf as its offset from the leaf
+    # (below 16 384) and a ghost or slot as an offset from its block's base; on a uniform
+    # 256 x 256 mesh the y-neighbours across the boundaries of the largest Morton blocks of base
+    # cells lie tens of thousands of leaves away, so those leaves (and only a few others) are
+    # read from the 32-byte record. The table build decodes every record and checks it against
+    # the 32-byte one (it raises otherwise).
+    s = host(riemann_c2(h=1.0 / 256, lvl_amr=0, pre_adapt=0))
+    s.setup()
+    n = s.counts()["n_leaf"]
+    esc = s.table_info()["records16_escaped"]
+    print("leaves", n, "escaped 16-byte records", esc)
+    assert n == 256 * 256
+    assert 0 < esc < 0.05 * n
+    # an AMR mesh with level jumps (slot ids encoded from the blocks' bases)
+    cfg = riemann_c2(h=1.0 / 24, lvl_amr=3, pre_adapt=0)
+    o = oracle_for(cfg)
+    for _ in range(3):
+        assert o.step() == 0
+    s2 = host(cfg)
+    m, n2 = o.dims()
+    s2.import_tree(m, n2, o.tree())
+    s2.build_tables()
+    ti = s2.table_info()
+    assert ti["slots"] > 0 and ti["records16_escaped"] < 0.05 * s2.counts()["n_leaf"]
+
+
 def test_tree_round_trip_and_canonical_ids():
     o = O.Oracle(h=10.0 / 32, iflag_amr=1, lvl_amr=2, pstep_adapt=2, iflag_localdt=1, cfl=0.5)
     for _ in range(3):
