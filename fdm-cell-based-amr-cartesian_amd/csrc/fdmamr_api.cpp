@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -960,13 +961,13 @@ static int regrid_from_sensors(fdmamr_ctx* c, int32_t lvl_amr, double refine_rot
     PhaseTimer pt;
     Tree& t = c->tree;
     if (sensor_mode == 1)
-        for (int k : t.ns) t.rot[k] = t.div[k];
+        parallel_for(t.ns.size(), [&](size_t b, size_t e) {
+            for (size_t i = b; i < e; i++) t.rot[t.ns[i]] = t.div[t.ns[i]];
+        });
     // Srot, Sdiv, Ne in rot_div's visiting order (solution.F90:430-465)
-    std::vector<int32_t> order;
-    t.rot_div_order(order);
     double srot = 0, sdiv = 0;
-    for (int k : order) { srot = srot + t.rot[k] * t.rot[k]; sdiv = sdiv + t.div[k] * t.div[k]; }
-    const int ne = (int)order.size();
+    int ne = 0;
+    t.rot_div_sums(srot, sdiv, ne);
     double arot = std::sqrt(srot / ne), adiv = std::sqrt(sdiv / ne);
     try {
         t.solute_ref(arot, adiv, lvl_amr, refine_rot, refine_div, coarse_rot, coarse_div);
@@ -1008,12 +1009,18 @@ int32_t fdmamr_regrid_sensors(fdmamr_ctx* c, int32_t lvl_amr, double refine_rot,
         if (r) return r;
     }
     Tree& t = c->tree;
-    for (size_t i = 0; i < t.ns.size(); i++) {
-        if (!std::isfinite(rot[i]) || !std::isfinite(div[i]))
-            return fail(c, -1, "fdmamr_regrid_sensors: sensor value not finite");
-        t.rot[t.ns[i]] = rot[i];
-        t.div[t.ns[i]] = div[i];
-    }
+    std::atomic<int> bad{0};
+    parallel_for(t.ns.size(), [&](size_t b, size_t e) {
+        for (size_t i = b; i < e; i++)
+            if (!std::isfinite(rot[i]) || !std::isfinite(div[i])) bad = 1;
+    });
+    if (bad) return fail(c, -1, "fdmamr_regrid_sensors: sensor value not finite");
+    parallel_for(t.ns.size(), [&](size_t b, size_t e) {
+        for (size_t i = b; i < e; i++) {
+            t.rot[t.ns[i]] = rot[i];
+            t.div[t.ns[i]] = div[i];
+        }
+    });
     return regrid_from_sensors(c, lvl_amr, refine_rot, refine_div, coarse_rot, coarse_div, sensor_mode);
 }
 

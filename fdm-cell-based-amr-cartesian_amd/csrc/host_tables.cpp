@@ -300,6 +300,28 @@ static void fill_par(V& v, size_t n, T val) {
     parallel_for(n, [&](size_t b, size_t e) { std::fill(v.begin() + b, v.begin() + e, val); });
 }
 
+// out = the i in [0, n) for which pred(i) holds, ascending, appended; on all host threads (a
+// prefix count over chunks, then each chunk writes its own range)
+template <class V, class P>
+static void append_where(size_t n, const P& pred, V& out) {
+    const size_t nch = 256, per = (n + nch - 1) / nch;
+    std::vector<size_t> at(nch + 1, 0);
+    parallel_for(nch, [&](size_t b, size_t e) {
+        for (size_t ch = b; ch < e; ch++)
+            for (size_t i = ch * per; i < std::min(n, (ch + 1) * per); i++) at[ch + 1] += pred(i) ? 1 : 0;
+    }, 1);
+    for (size_t ch = 0; ch < nch; ch++) at[ch + 1] += at[ch];
+    const size_t o = out.size();
+    out.resize(o + at[nch]);
+    parallel_for(nch, [&](size_t b, size_t e) {
+        for (size_t ch = b; ch < e; ch++) {
+            size_t k = o + at[ch];
+            for (size_t i = ch * per; i < std::min(n, (ch + 1) * per); i++)
+                if (pred(i)) out[k++] = (int32_t)i;
+        }
+    }, 1);
+}
+
 void build_tables(const Tree& t, Tables& tb) {
     PhaseTimer pt;
     const size_t NA = t.n_alloc();
@@ -313,7 +335,7 @@ void build_tables(const Tree& t, Tables& tb) {
         tb.sel_loc.swap(old.sel_loc); tb.state_of_node.swap(old.state_of_node);
         tb.leaf_node.swap(old.leaf_node); tb.node_son.swap(old.node_son); tb.lvl_leaf.swap(old.lvl_leaf);
         tb.face_share.swap(old.face_share); tb.ns_pos.swap(old.ns_pos); tb.dev_of_ns.swap(old.dev_of_ns);
-        tb.wls_i.swap(old.wls_i);
+        tb.wls_i.swap(old.wls_i); tb.sel_nbr.swap(old.sel_nbr); tb.face_lo.swap(old.face_lo);
     }
     tb.N = (int)t.ns.size();
     tb.G = (int)t.bcl.size();
@@ -472,57 +494,107 @@ void build_tables(const Tree& t, Tables& tb) {
         return Key{3, tb.st_node[d], bits(px), bits(py)};
     };
     pt.mark("tables: jump keys");
-    std::vector<int32_t> rep(J);  // first jump entry with the same key
+    // (the large temporaries below are not zeroed: every element is written before it is read,
+    // on all host threads)
+    BigVec<int32_t> rep(J);  // first jump entry with the same key
     {
         auto mix = [](uint64_t z) {
             z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
             z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
             return z ^ (z >> 31);
         };
-        std::vector<uint64_t> hj(J);
+        BigVec<uint64_t> hj(J);
+        BigVec<Key> kj(J);  // each entry's key, computed once
         parallel_for(J, [&](size_t b, size_t e) {
             for (size_t j = b; j < e; j++) {
                 const Key k = key_of(j);
+                kj[j] = k;
                 hj[j] = mix(mix(mix((uint64_t)k.kind << 32 | (uint32_t)k.node) ^ k.xb) ^ k.yb);
             }
         });
-        // 16 shards by the top hash bits, each an open-addressing table of first entries
-        constexpr unsigned ns = 16;
+        // 256 shards by the top hash bits, each an open-addressing table of first entries small
+        // enough to stay in a core's cache. The entries are first sorted by shard (stably:
+        // chunks of entries in order, each chunk's entries of a shard at that chunk's offset),
+        // so each shard reads only its own.
+        constexpr unsigned ns = 256, shard_shift = 56;
+        const size_t nch = 64, per = (J + nch - 1) / nch;
+        std::vector<size_t> off(ns * nch + 1, 0);
+        parallel_for(nch, [&](size_t b, size_t e) {
+            for (size_t ch = b; ch < e; ch++)
+                for (size_t j = ch * per; j < std::min(J, (ch + 1) * per); j++) off[1 + (hj[j] >> shard_shift) * nch + ch]++;
+        }, 1);
+        for (size_t k = 0; k < ns * nch; k++) off[k + 1] += off[k];
+        // entry numbers, hashes and keys in shard order
+        BigVec<int32_t> by_shard(J);
+        BigVec<uint64_t> hs(J);
+        BigVec<Key> ks(J);
+        parallel_for(nch, [&](size_t b, size_t e) {
+            for (size_t ch = b; ch < e; ch++) {
+                std::vector<size_t> at(ns);
+                for (unsigned sh = 0; sh < ns; sh++) at[sh] = off[sh * nch + ch];
+                for (size_t j = ch * per; j < std::min(J, (ch + 1) * per); j++) {
+                    const size_t k = at[hj[j] >> shard_shift]++;
+                    by_shard[k] = (int32_t)j;
+                    hs[k] = hj[j];
+                    ks[k] = kj[j];
+                }
+            }
+        }, 1);
         parallel_for(ns, [&](size_t b, size_t e) {
+            std::vector<int32_t> first;  // shard positions of the first entries
             for (size_t sh = b; sh < e; sh++) {
-                size_t cnt = 0;
-                for (size_t j = 0; j < J; j++) cnt += (hj[j] >> 60) == sh;
+                const size_t k0 = off[sh * nch], k1 = off[(sh + 1) * nch];
+                const size_t cnt = k1 - k0;
                 size_t cap = 16;
                 while (cap < 2 * cnt) cap <<= 1;
                 const size_t mask = cap - 1;
-                std::vector<int32_t> first(cap, -1);
-                for (size_t j = 0; j < J; j++) {
-                    if ((hj[j] >> 60) != sh) continue;
-                    size_t at = hj[j] & mask;
+                first.assign(cap, -1);
+                for (size_t k = k0; k < k1; k++) {
+                    size_t at = hs[k] & mask;
                     int32_t f;
                     while ((f = first[at]) >= 0) {
-                        if (hj[f] == hj[j] && key_of((size_t)f) == key_of(j)) break;
+                        if (hs[(size_t)f] == hs[k] && ks[(size_t)f] == ks[k]) break;
                         at = (at + 1) & mask;
                     }
-                    if (f < 0) first[at] = f = (int32_t)j;
-                    rep[j] = f;
+                    if (f < 0) first[at] = f = (int32_t)k;
+                    rep[by_shard[k]] = by_shard[(size_t)f];
                 }
             }
         }, 1);
     }
     pt.mark("tables: dedup");
-    // per distinct key (in order): its first entry, then the kind its stencil lists give
-    std::vector<int32_t> uid(J), ufirst;
-    for (size_t j = 0; j < J; j++) {
-        if (rep[j] == (int32_t)j) { uid[j] = (int32_t)ufirst.size(); ufirst.push_back((int32_t)j); }
-        else uid[j] = uid[rep[j]];
+    // per distinct key (in order): its first entry, then the kind its stencil lists give. The
+    // keys are numbered by a prefix count of the first entries over chunks of entries; every
+    // other entry then takes its first entry's number.
+    BigVec<int32_t> uid(J), ufirst;
+    {
+        const size_t nch = 256, per = (J + nch - 1) / nch;
+        std::vector<size_t> at(nch + 1, 0);
+        parallel_for(nch, [&](size_t b, size_t e) {
+            for (size_t ch = b; ch < e; ch++)
+                for (size_t j = ch * per; j < std::min(J, (ch + 1) * per); j++) at[ch + 1] += rep[j] == (int32_t)j;
+        }, 1);
+        for (size_t ch = 0; ch < nch; ch++) at[ch + 1] += at[ch];
+        ufirst.resize(at[nch]);
+        parallel_for(nch, [&](size_t b, size_t e) {
+            for (size_t ch = b; ch < e; ch++) {
+                size_t u = at[ch];
+                for (size_t j = ch * per; j < std::min(J, (ch + 1) * per); j++)
+                    if (rep[j] == (int32_t)j) { uid[j] = (int32_t)u; ufirst[u++] = (int32_t)j; }
+            }
+        }, 1);
+        parallel_for(J, [&](size_t b, size_t e) {
+            for (size_t j = b; j < e; j++)
+                if (rep[j] != (int32_t)j) uid[j] = uid[rep[j]];
+        });
     }
     pt.mark("tables: slot numbering");
     const size_t U = ufirst.size();
-    std::vector<int8_t> ukind(U), um(U), um1(U);
-    std::vector<int32_t> unodes(U * 9, -1);
+    BigVec<int8_t> ukind(U), um(U), um1(U);
+    BigVec<int32_t> unodes(U * 9);
     parallel_for(U, [&](size_t b, size_t e) {
         for (size_t u = b; u < e; u++) {
+            std::fill(&unodes[u * 9], &unodes[u * 9] + 9, -1);
             const size_t d = (size_t)jd[ufirst[u]];
             const int tg = tb.st_node[d];
             const bool parent = tb.st_kind[d] == CAND_PARENT;
@@ -542,17 +614,36 @@ void build_tables(const Tree& t, Tables& tb) {
     });
     pt.mark("tables: lists");
     // WLS slots and restriction slots, each in order of first occurrence
-    std::vector<int32_t> uref(U);
+    // (numbered by prefix counts over chunks of keys)
+    BigVec<int32_t> uref(U);
     std::vector<int32_t> wls_u;  // per WLS slot: its key
-    for (size_t u = 0; u < U; u++) {
-        const size_t d = (size_t)jd[ufirst[u]];
-        const int tg = tb.st_node[d];
-        switch (ukind[u]) {
-            case ST_PARENT_WLS:
-            case ST_CHILD_WLS: uref[u] = 1000000000 + (int32_t)wls_u.size(); wls_u.push_back((int32_t)u); break;
-            case ST_PARENT_RESTRICT: uref[u] = -2 - (int32_t)tb.rst_node.size(); tb.rst_node.push_back(tg); break;
-            default: uref[u] = tb.state_of_node[tg];
-        }
+    {
+        const size_t nch = 256, per = (U + nch - 1) / nch;
+        std::vector<size_t> nw(nch + 1, 0), nr(nch + 1, 0);
+        parallel_for(nch, [&](size_t b, size_t e) {
+            for (size_t ch = b; ch < e; ch++)
+                for (size_t u = ch * per; u < std::min(U, (ch + 1) * per); u++) {
+                    nw[ch + 1] += ukind[u] == ST_PARENT_WLS || ukind[u] == ST_CHILD_WLS;
+                    nr[ch + 1] += ukind[u] == ST_PARENT_RESTRICT;
+                }
+        }, 1);
+        for (size_t ch = 0; ch < nch; ch++) { nw[ch + 1] += nw[ch]; nr[ch + 1] += nr[ch]; }
+        wls_u.resize(nw[nch]);
+        tb.rst_node.resize(nr[nch]);
+        parallel_for(nch, [&](size_t b, size_t e) {
+            for (size_t ch = b; ch < e; ch++) {
+                size_t w = nw[ch], r = nr[ch];
+                for (size_t u = ch * per; u < std::min(U, (ch + 1) * per); u++) {
+                    const int tg = tb.st_node[(size_t)jd[ufirst[u]]];
+                    switch (ukind[u]) {
+                        case ST_PARENT_WLS:
+                        case ST_CHILD_WLS: uref[u] = 1000000000 + (int32_t)w; wls_u[w++] = (int32_t)u; break;
+                        case ST_PARENT_RESTRICT: uref[u] = -2 - (int32_t)r; tb.rst_node[r++] = tg; break;
+                        default: uref[u] = tb.state_of_node[tg];
+                    }
+                }
+            }
+        }, 1);
     }
     tb.S_w = (int)wls_u.size();
     tb.S_r = (int)tb.rst_node.size();
@@ -602,7 +693,7 @@ void build_tables(const Tree& t, Tables& tb) {
         const size_t nrun = std::max<size_t>(1, std::min<size_t>(64, (size_t)tb.S_w / 4096));
         const size_t per = ((size_t)tb.S_w + nrun - 1) / nrun;
         std::vector<std::vector<int32_t>> run_first(nrun);  // first slot of each local pattern
-        std::vector<int32_t> lpat(tb.S_w);
+        BigVec<int32_t> lpat(tb.S_w);
         parallel_for(nrun, [&](size_t b, size_t e) {
             for (size_t r = b; r < e; r++) {
                 std::unordered_map<WlsIn, int32_t, InHash, InEq> loc;
@@ -779,21 +870,24 @@ void build_tables(const Tree& t, Tables& tb) {
         }
     });
     tb.face_lo.clear();
-    for (int i = 0; i < N; i++)
-        if (!(tb.face_share[i] & 1)) tb.face_lo.push_back(i);
+    append_where((size_t)N, [&](size_t i) { return !(tb.face_share[i] & 1); }, tb.face_lo);
     tb.n_lo_x = (int)tb.face_lo.size();
-    for (int i = 0; i < N; i++)
-        if (!(tb.face_share[i] & 2)) tb.face_lo.push_back(i);
+    append_where((size_t)N, [&](size_t i) { return !(tb.face_share[i] & 2); }, tb.face_lo);
     tb.n_lo_y = (int)tb.face_lo.size() - tb.n_lo_x;
     // slots that read a ghost go last (slots [0, S_wp) read none), each part in its old order
     {
-        std::vector<int32_t> newidx(tb.S_w);
-        int np = 0;
-        for (int s = 0; s < tb.S_w; s++)
-            if ((tb.wls_i[(size_t)s * WLSI_STRIDE + WLSI_M] >> WLSI_GSHIFT) == 0) newidx[s] = np++;
-        tb.S_wp = np;
-        for (int s = 0; s < tb.S_w; s++)
-            if ((tb.wls_i[(size_t)s * WLSI_STRIDE + WLSI_M] >> WLSI_GSHIFT) != 0) newidx[s] = np++;
+        BigVec<int32_t> plain, ghosted;
+        auto reads_ghost = [&](size_t s) { return (tb.wls_i[s * WLSI_STRIDE + WLSI_M] >> WLSI_GSHIFT) != 0; };
+        append_where((size_t)tb.S_w, [&](size_t s) { return !reads_ghost(s); }, plain);
+        append_where((size_t)tb.S_w, reads_ghost, ghosted);
+        tb.S_wp = (int)plain.size();
+        BigVec<int32_t> newidx(tb.S_w);
+        parallel_for(plain.size(), [&](size_t b0, size_t e0) {
+            for (size_t k = b0; k < e0; k++) newidx[plain[k]] = (int32_t)k;
+        });
+        parallel_for(ghosted.size(), [&](size_t b0, size_t e0) {
+            for (size_t k = b0; k < e0; k++) newidx[ghosted[k]] = (int32_t)(plain.size() + k);
+        });
         if (tb.S_wp < tb.S_w) {
             BigVec<int32_t> wi(tb.wls_i.size());
             parallel_for((size_t)tb.S_w, [&](size_t b0, size_t e0) {
@@ -883,18 +977,34 @@ void build_tables(const Tree& t, Tables& tb) {
             // The points are cell centres at the leaf's level, so each lies at least half a
             // leaf width inside every box of that level or coarser: the nearest ancestor whose
             // box holds it is on the path from its base cell, and the descent can start there.
-            // The ancestors with their half widths, nearest first:
-            int anc[64], na = 0;
+            // The ancestors with their half widths, nearest first (listed when first needed):
+            int anc[64], na = -1;
             double ax[64], ay[64], ahx[64], ahy[64];
-            for (int f = t.father[c], k = 1; f >= 0 && na < 64; f = t.father[f], k++) {
-                anc[na] = f; ax[na] = t.x[f]; ay[na] = t.y[f];
-                ahx[na] = 0.5 * dx * (double)(1 << k); ahy[na] = 0.5 * dy * (double)(1 << k);
-                na++;
-            }
             int q = 0;
             for (int b = -2; b <= 2; b++)
                 for (int a = -2; a <= 2; a++) {
                     if (a == 0 || b == 0) continue;
+                    // A same-level leaf at a along x whose stencil has a same-level leaf at b
+                    // along y: that leaf has the point at its centre, so it is the leaf the
+                    // descent below would find.
+                    const size_t ex = (size_t)i * 8 + (a == 1 ? 0 : a == -1 ? 1 : a == 2 ? 4 : 5);
+                    if (tb.st_kind[ex] == ST_COPY) {
+                        const int32_t j = tb.state_of_node[tb.st_node[ex]];
+                        const size_t ey = (size_t)j * 8 + (b == 1 ? 2 : b == -1 ? 3 : b == 2 ? 6 : 7);
+                        if (j >= 0 && j < N && tb.st_kind[ey] == ST_COPY) {
+                            tb.off16[(size_t)q * N + i] = redirect(tb.state_of_node[tb.st_node[ey]]);
+                            q++;
+                            continue;
+                        }
+                    }
+                    if (na < 0) {
+                        na = 0;
+                        for (int f = t.father[c], k = 1; f >= 0 && na < 64; f = t.father[f], k++) {
+                            anc[na] = f; ax[na] = t.x[f]; ay[na] = t.y[f];
+                            ahx[na] = 0.5 * dx * (double)(1 << k); ahy[na] = 0.5 * dy * (double)(1 << k);
+                            na++;
+                        }
+                    }
                     const double px = t.x[c] + a * dx, py = t.y[c] + b * dy;
                     int node = -1;
                     for (int k = 0; k < na; k++)
@@ -971,7 +1081,7 @@ void build_tables(const Tree& t, Tables& tb) {
         }, 1);
         tb.sel_nbr_beg.assign(ntiles + 1, 0);
         for (int T = 0; T < ntiles; T++) tb.sel_nbr_beg[T + 1] = tb.sel_nbr_beg[T] + tile_cnt[T];
-        tb.sel_nbr.resize(tb.sel_nbr_beg[ntiles]);
+        resize_fresh(tb.sel_nbr, (size_t)tb.sel_nbr_beg[ntiles]);  // every element is copied below
         std::atomic<int> bad_nbr{0};
         parallel_for(nrun, [&](size_t r0, size_t r1) {
             for (size_t r = r0; r < r1; r++) {

@@ -88,7 +88,7 @@ struct Tables {
     // x+1/2 face (so both compute the same flux, bit for bit); bit 1: the same in y
     BigVec<uint8_t> face_share;          // [N]
     // the leaves whose x-1/2 face is not shared, ascending, then those whose y-1/2 face is not
-    std::vector<int32_t> face_lo;
+    BigVec<int32_t> face_lo;
     int n_lo_x = 0, n_lo_y = 0;
     BigVec<int8_t> lvl_leaf;             // [N]
     // diagnostics (host only)
@@ -149,7 +149,8 @@ struct Tables {
     // g + 4j of the 5x5 grid (p = (b+2)*5 + (a+2); the centre for p > 24), j < 7, for the four
     // lanes g of the leaf, and sel_loc[leaf][g][7] the centre's position. A tile with more than
     // SEL_NBR_CAP distinct indices stores none (count SEL_NBR_CAP + 1) and is gathered directly.
-    std::vector<int32_t> sel_nbr_beg, sel_nbr;
+    std::vector<int32_t> sel_nbr_beg;
+    BigVec<int32_t> sel_nbr;
     BigVec<uint8_t> sel_loc;             // [N][4][8]
     // device topology for restriction means
     BigVec<int32_t> node_son;            // [n_alloc]

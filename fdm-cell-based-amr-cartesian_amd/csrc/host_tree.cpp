@@ -140,7 +140,9 @@ struct PassPool {
 // Each worker takes son blocks from the pool a few at a time into its own cache; a worker
 // returns the blocks it did not use to the free list, marked released, at the end of the pass.
 static thread_local std::vector<int32_t>* tl_blocks = nullptr;
-constexpr int kBlockBatch = 16;
+// ... and counts its refines and coarsens on its own, adding them to Tree::mod_count at the end
+static thread_local long* tl_mods = nullptr;
+constexpr int kBlockBatch = 64;
 
 void Tree::resize_nodes(size_t nn) {
     if (nn > (size_t)INT_MAX) throw std::runtime_error("quadtree node count exceeds 2^31");
@@ -220,12 +222,10 @@ void Tree::release_subtree(int c) {
     for (int k = 0; k < 4; k++) release_subtree(s + k);
     for (int k = 0; k < 4; k++) { father[s + k] = -2; son[s + k] = -1; }
     son[c] = -1;
-    if (pool) {
-        std::lock_guard<std::mutex> g(pool->mu);
-        free_blocks.push_back(s);
-    } else {
-        free_blocks.push_back(s);
-    }
+    // in a parallel pass the block goes to the worker's own cache, for its next allocations;
+    // what the worker has not used at the end of the pass returns to the free list
+    if (pool) tl_blocks->push_back(s);
+    else free_blocks.push_back(s);
 }
 
 void Tree::compact() {
@@ -253,7 +253,8 @@ void Tree::compact() {
     for (int i = 0; i < total; i++) first[i + 1] += first[i];
     const size_t nn = (size_t)first[total];
     if (nn > (size_t)INT_MAX) throw std::runtime_error("quadtree node count exceeds 2^31");
-    std::vector<int32_t> id(NA, -1);
+    BigVec<int32_t> id(NA);  // -1 for released nodes
+    parallel_for(NA, [&](size_t b, size_t e) { std::fill(id.begin() + b, id.begin() + e, -1); });
     // block pre-order below each base cell: a node's four sons, then each son's subtree
     parallel_for((size_t)total, [&](size_t b, size_t e) {
         std::vector<int32_t> st;
@@ -271,7 +272,7 @@ void Tree::compact() {
         }
     });
     pt.mark("compact: ids");
-    std::vector<int32_t> old(nn);
+    BigVec<int32_t> old(nn);  // every entry is written: each new id has one old node
     parallel_for(NA, [&](size_t b, size_t e) {
         for (size_t c = b; c < e; c++)
             if (id[c] >= 0) old[id[c]] = (int32_t)c;
@@ -328,7 +329,9 @@ static void run_fronts(Tree& t, int skew, unsigned nt, size_t room, const F& f) 
     std::vector<std::exception_ptr> errs(nt);
     auto worker = [&](unsigned w) {
         std::vector<int32_t> cache;
+        long mods = 0;
         tl_blocks = &cache;
+        tl_mods = &mods;
         for (int T = 0; T < nfront; T++) {
             int r0 = 0, count = std::min(kRun, t.total - T * kRun);
             if (skew >= 0) {
@@ -375,6 +378,8 @@ static void run_fronts(Tree& t, int skew, unsigned nt, size_t room, const F& f) 
             }
         }
         tl_blocks = nullptr;
+        tl_mods = nullptr;
+        if (t.mod_count) t.mod_count->fetch_add(mods, std::memory_order_relaxed);
     };
     std::vector<std::thread> th;
     for (unsigned w = 1; w < nt; w++) th.emplace_back(worker, w);
@@ -385,6 +390,84 @@ static void run_fronts(Tree& t, int skew, unsigned nt, size_t room, const F& f) 
     t.resize_nodes(pool.used);
     for (auto& e : errs)
         if (e) std::rethrow_exception(e);
+}
+
+// Run visit(cell) for the base cells that want(cell) selects, on host threads, row by row: a
+// worker takes the next row and walks it by increasing column, and starts cell (r, c) once row
+// r - 1 has finished its columns up to c + lag. With lag = 0, every cell that shares an edge
+// with an earlier cell of the row-major order starts after it; with lag = 2, every cell of an
+// earlier row in the same column within four rows, or within two rows and two columns, does
+// (row r - 1 at c + 2 itself waited for row r - 2 at c + 4, and so on). Cells that interact
+// only in those ways then see the tree exactly as in a serial loop over the cells in index
+// order. want() reads only the per-cell level bounds; visit() runs while the worker holds the
+// pool's growth lock shared, and never waits for another worker. Returns the number of cells
+// visited.
+template <class W, class V>
+static size_t run_rows(Tree& t, int lag, unsigned nt, size_t room, const W& want, const V& visit) {
+    const int m = t.m, n = t.n;
+    constexpr int kPublish = 16;  // columns between progress stores
+    PassPool pool;
+    pool.used = t.n_alloc();
+    t.resize_nodes(pool.used + std::max(room, (size_t)1 << 16));
+    t.pool = &pool;
+    std::unique_ptr<std::atomic<int>[]> done(new std::atomic<int>[(size_t)n]);
+    for (int r = 0; r < n; r++) done[r].store(0, std::memory_order_relaxed);
+    std::atomic<int> next_row{0};
+    std::atomic<bool> abort{false};
+    std::atomic<size_t> visited{0};
+    std::vector<std::exception_ptr> errs(nt);
+    auto worker = [&](unsigned w) {
+        std::vector<int32_t> cache;
+        long mods = 0;
+        size_t visits = 0;
+        tl_blocks = &cache;
+        tl_mods = &mods;
+        for (;;) {
+            const int r = next_row.fetch_add(1, std::memory_order_relaxed);
+            if (r >= n) break;
+            int seen = r == 0 ? m : 0;  // columns of row r - 1 known to be finished
+            for (int c = 0; c < m; c++) {
+                const int need = std::min(m, c + lag + 1);
+                for (int spin = 0; seen < need; spin++) {
+                    seen = done[r - 1].load(std::memory_order_acquire);
+                    if (seen < need && spin > 2000) std::this_thread::yield();
+                }
+                const int i = r * m + c;
+                if (!abort.load(std::memory_order_relaxed) && want(i)) {
+                    pool.grow.lock_shared();
+                    visits++;
+                    try {
+                        visit(i);
+                    } catch (...) {
+                        errs[w] = std::current_exception();
+                        abort = true;
+                    }
+                    pool.grow.unlock_shared();
+                }
+                if ((c + 1) % kPublish == 0 || c + 1 == m) done[r].store(c + 1, std::memory_order_release);
+            }
+        }
+        {
+            std::lock_guard<std::mutex> g(pool.mu);
+            for (int32_t b : cache) {
+                for (int k = 0; k < 4; k++) { t.father[b + k] = -2; t.son[b + k] = -1; }
+                t.free_blocks.push_back(b);
+            }
+        }
+        tl_blocks = nullptr;
+        tl_mods = nullptr;
+        if (t.mod_count) t.mod_count->fetch_add(mods, std::memory_order_relaxed);
+        visited += visits;
+    };
+    std::vector<std::thread> th;
+    for (unsigned w = 1; w < nt; w++) th.emplace_back(worker, w);
+    worker(0);
+    for (auto& x : th) x.join();
+    t.pool = nullptr;
+    t.resize_nodes(pool.used);
+    for (auto& e : errs)
+        if (e) std::rethrow_exception(e);
+    return visited.load();
 }
 
 // parallel passes only on meshes with enough base cells to share out
@@ -412,10 +495,16 @@ void Tree::refine(int c) {                          // mesh.F90:199-260
         rot[q] = rot[c]; div[q] = div[c];
         fresh[q] = 1;
     }
-    if (mod_count) mod_count->fetch_add(1, std::memory_order_relaxed);
+    if (mod_count) {
+        if (tl_mods) ++*tl_mods;
+        else mod_count->fetch_add(1, std::memory_order_relaxed);
+    }
     if (track_levels) {
-        int8_t& hi = cell_lmax[(size_t)(ni[c] - 1) * m + nj[c] - 1];
-        hi = std::max<int8_t>(hi, (int8_t)(lvl[c] + 1));
+        // the new leaves are at lvl + 1; a replaced subtree may have held only deeper ones
+        const size_t cell = (size_t)(ni[c] - 1) * m + nj[c] - 1;
+        cell_lmax[cell] = std::max<int8_t>(cell_lmax[cell], (int8_t)(lvl[c] + 1));
+        cell_lmin[cell] = std::min<int8_t>(cell_lmin[cell], (int8_t)(lvl[c] + 1));
+        cell_dirty[cell] = 1;
     }
 }
 
@@ -436,31 +525,51 @@ void Tree::coarse(int c) {                          // mesh.F90:397-408
     for (int i = 0; i < 7; i++) U[c][i] = v[i];
     release_subtree(c);
     fresh[c] = 1;
-    if (mod_count) mod_count->fetch_add(1, std::memory_order_relaxed);
+    if (mod_count) {
+        if (tl_mods) ++*tl_mods;
+        else mod_count->fetch_add(1, std::memory_order_relaxed);
+    }
     if (track_levels) {
-        int8_t& lo = cell_lmin[(size_t)(ni[c] - 1) * m + nj[c] - 1];
-        lo = std::min<int8_t>(lo, (int8_t)lvl[c]);
+        const size_t cell = (size_t)(ni[c] - 1) * m + nj[c] - 1;
+        cell_lmin[cell] = std::min<int8_t>(cell_lmin[cell], (int8_t)lvl[c]);
+        cell_dirty[cell] = 1;
     }
 }
 
+static void cell_levels(const Tree& t, int i, std::vector<int32_t>& st, int& lo, int& hi) {
+    lo = 127; hi = 0;
+    st.assign(1, (int32_t)i);
+    while (!st.empty()) {
+        const int c = st.back();
+        st.pop_back();
+        if (t.son[c] < 0) { lo = std::min(lo, t.lvl[c]); hi = std::max(hi, t.lvl[c]); continue; }
+        for (int k = 0; k < 4; k++) st.push_back(t.son[c] + k);
+    }
+}
 void Tree::leaf_levels() {
     cell_lmin.assign(total, 127);
     cell_lmax.assign(total, 0);
+    cell_dirty.assign(total, 0);
     parallel_for((size_t)total, [&](size_t b, size_t e) {
         std::vector<int32_t> st;
         for (size_t i = b; i < e; i++) {
-            int lo = 127, hi = 0;
-            st.assign(1, (int32_t)i);
-            while (!st.empty()) {
-                const int c = st.back();
-                st.pop_back();
-                if (son[c] < 0) { lo = std::min(lo, lvl[c]); hi = std::max(hi, lvl[c]); continue; }
-                for (int k = 0; k < 4; k++) st.push_back(son[c] + k);
-            }
+            int lo, hi;
+            cell_levels(*this, (int)i, st, lo, hi);
             cell_lmin[i] = (int8_t)lo;
             cell_lmax[i] = (int8_t)hi;
         }
     });
+}
+void Tree::leaf_levels_dirty() {
+    std::vector<int32_t> st;
+    for (int i = 0; i < total; i++) {
+        if (!cell_dirty[i]) continue;
+        int lo, hi;
+        cell_levels(*this, i, st, lo, hi);
+        cell_lmin[i] = (int8_t)lo;
+        cell_lmax[i] = (int8_t)hi;
+        cell_dirty[i] = 0;
+    }
 }
 
 // ---------------------------------------------------------------- neighbours
@@ -576,7 +685,13 @@ void Tree::idnei(int c) {                           // neighbor.F90:412-449
     if (son[c] >= 0) for (int k = 0; k < 4; k++) idnei(son[c] + k);
 }
 void Tree::idneighbor() {
-    nb.assign(8 * n_alloc(), -1);
+    // idnei writes all eight entries of every node below a base cell; only released nodes
+    // (father -2), which no walk reaches, are cleared here
+    nb.resize(8 * n_alloc());
+    parallel_for(n_alloc(), [&](size_t b, size_t e) {
+        for (size_t c = b; c < e; c++)
+            if (father[c] == -2) std::fill(nb.begin() + 8 * c, nb.begin() + 8 * c + 8, -1);
+    });
     // the finders only read the tree; each base cell's subtree writes its own entries
     parallel_for((size_t)total, [&](size_t b, size_t e) {
         for (size_t i = b; i < e; i++) idnei((int)i);
@@ -720,10 +835,27 @@ void Tree::rot_div_walk(int c, std::vector<int32_t>& out) const { // solution.F9
     if (son[c] < 0 && (cross[c] == 0 || cross[c] == 1)) out.push_back(c);
     else if (son[c] >= 0) for (int k = 0; k < 4; k++) rot_div_walk(son[c] + k, out);
 }
-void Tree::rot_div_order(std::vector<int32_t>& out) const {       // solution.F90:424-440
-    out.clear();
-    for (int i = 0; i < total; i++)
-        if (cross[i] == 0 || cross[i] == 1 || cross[i] == 2) rot_div_walk(i, out);
+void Tree::rot_div_sums(double& srot, double& sdiv, int& ne) const { // solution.F90:424-440
+    // the visiting order is collected on all host threads, in runs of base cells, and the
+    // sums then run through the runs in order on one thread, so the additions are the
+    // reference's, in its order
+    const size_t nrun = std::min<size_t>(1024, std::max(1, total));
+    const size_t per = ((size_t)total + nrun - 1) / nrun;
+    std::vector<std::vector<int32_t>> runs(nrun);
+    parallel_for(nrun, [&](size_t r0, size_t r1) {
+        for (size_t r = r0; r < r1; r++)
+            for (size_t i = r * per; i < std::min((size_t)total, (r + 1) * per); i++)
+                if (cross[i] == 0 || cross[i] == 1 || cross[i] == 2) rot_div_walk((int)i, runs[r]);
+    }, 1);
+    double a = 0, b = 0;
+    size_t cnt = 0;
+    for (const auto& run : runs) {
+        for (int32_t k : run) { a = a + rot[k] * rot[k]; b = b + div[k] * div[k]; }
+        cnt += run.size();
+    }
+    srot = a;
+    sdiv = b;
+    ne = (int)cnt;
 }
 
 void Tree::coarse_sub(int c, double arot, double adiv, int lvl_amr, double cg_rot,
@@ -805,13 +937,13 @@ void Tree::solute_gridmodify(int lvl_amr) {         // solution.F90:537-556
     // The reference refreshes the cached neighbour pointers after each pass (idneighbor),
     // but gridmodify1 and gridmodify4 search neighbours afresh through father/son links and
     // never read the cache, so only the final refresh (by the caller) can be observed.
-    // In parallel, the base cells run in fronts. gridmodify1 on a base cell writes its own
-    // subtree and reads those of the four cells beside it: cells that differ in both row and
-    // column do not interact, so fronts c + r = T keep every interaction in the serial order.
-    // gridmodify4 also refines the neighbour's neighbour: it reads and writes the cells up to
-    // two away in a row or a column, and cells (r, c), (r', c') interact when they share a row
-    // or column within four, or |r - r'|, |c - c'| <= 2; fronts c + 3 r = T keep those in order.
-    // The node numbering then depends on the threads' timing, so the tree is renumbered.
+    // In parallel, the base cells run row by row (run_rows). gridmodify1 on a base cell writes
+    // its own subtree and reads those of the four cells beside it: cells that differ in both
+    // row and column do not interact (lag 0). gridmodify4 also refines the neighbour's
+    // neighbour: it reads and writes the cells up to two away in a row or a column, and cells
+    // (r, c), (r', c') interact when they share a row or column within four, or
+    // |r - r'|, |c - c'| <= 2 (lag 2). The node numbering then depends on the threads' timing,
+    // so the tree is renumbered.
     //
     // A base cell whose leaves and those of the four cells beside it are all on one level
     // gives gridmodify1 nothing to do (each leaf's neighbours are leaves of its own level, each
@@ -819,6 +951,8 @@ void Tree::solute_gridmodify(int lvl_amr) {         // solution.F90:537-556
     // at level L, with no leaf below level L - 1 in it or beside it, gives gridmodify4 nothing
     // (a leaf at level L >= 2 finds its neighbour and the neighbour's neighbour within the cells
     // beside its own); the parallel passes skip such cells, by the level bounds of each cell.
+    // The bounds are exact at the start; within a pass refine and coarse widen those of the
+    // cells they change, and between passes those cells' bounds are made exact again.
     const unsigned nt = pass_threads(*this);
     PhaseTimer pt;
     auto span = [&](int i, int& lo, int& hi) {
@@ -831,30 +965,38 @@ void Tree::solute_gridmodify(int lvl_amr) {         // solution.F90:537-556
     // A pass (gridmodify1 then gridmodify4 over all cells) that changes nothing leaves the tree
     // as it found it, so every later pass would change nothing either: those are skipped.
     std::atomic<long> mods{0};
+    size_t visits = 0;
     mod_count = &mods;
     struct Reset {
         Tree& t;
         ~Reset() { t.mod_count = nullptr; t.track_levels = false; }
     } reset{*this};
+    if (nt > 1) {
+        leaf_levels();
+        track_levels = true;
+        pt.mark("leaf levels");
+    }
+    auto report = [&](const char* phase) {
+        if (pt.on) std::fprintf(stderr, "[fdmamr]   %s: %zu cells visited, %ld changes\n", phase, visits, mods.load());
+        pt.mark(phase);
+    };
     for (int k = 1; k <= lvl_amr; k++) {
         mods = 0;
         if (nt > 1) {
-            track_levels = true;
-            leaf_levels();
-            run_fronts(*this, 1, nt, n_alloc() / 16, [&](int i) {
+            visits = run_rows(*this, 0, nt, n_alloc() / 16, [&](int i) {
                 int lo, hi;
                 span(i, lo, hi);
-                if (lo != hi) gridmodify1(i);
-            });
-            pt.mark("gridmodify1");
-            leaf_levels();
-            run_fronts(*this, 3, nt, n_alloc() / 16, [&](int i) {
+                return lo != hi;
+            }, [&](int i) { gridmodify1(i); });
+            leaf_levels_dirty();
+            report("gridmodify1");
+            visits = run_rows(*this, 2, nt, n_alloc() / 16, [&](int i) {
                 int lo, hi;
                 span(i, lo, hi);
-                if (cell_lmax[i] >= 2 && lo < cell_lmax[i] - 1) gridmodify4(i);
-            });
-            track_levels = false;
-            pt.mark("gridmodify4");
+                return cell_lmax[i] >= 2 && lo < cell_lmax[i] - 1;
+            }, [&](int i) { gridmodify4(i); });
+            leaf_levels_dirty();
+            report("gridmodify4");
         } else {
             for (int i = 0; i < total; i++) gridmodify1(i);
             pt.mark("gridmodify1");
@@ -863,6 +1005,7 @@ void Tree::solute_gridmodify(int lvl_amr) {         // solution.F90:537-556
         }
         if (mods.load() == 0) break;
     }
+    track_levels = false;
     compact();
     pt.mark("compact");
 }

@@ -73,7 +73,7 @@ struct Tree {
     // 1 for the nodes the current regrid made leaves (refined sons, coarsened parents): their
     // values are copies or means, not evolved states
     BigVec<uint8_t> fresh;
-    std::vector<int32_t> nb;                        // 8 cached neighbours per node (idnei)
+    BigVec<int32_t> nb;                             // 8 cached neighbours per node (idnei)
     std::vector<int32_t> free_blocks;               // first ids of released son blocks
 
     // ---- leaf lists (IDgrid) ----
@@ -98,11 +98,14 @@ struct Tree {
     std::shared_ptr<unsigned char[]> spare;         // compact()'s staging buffer, kept
     size_t spare_bytes = 0;
     // per base cell: bounds on the levels of the leaves below it (a lower bound on the least,
-    // an upper bound on the greatest), kept by refine and coarse while track_levels is set
+    // an upper bound on the greatest), kept by refine and coarse while track_levels is set,
+    // which also mark the cell in cell_dirty
     std::vector<int8_t> cell_lmin, cell_lmax;
+    std::vector<uint8_t> cell_dirty;
     bool track_levels = false;
     std::atomic<long>* mod_count = nullptr;         // refines + coarsens, counted when set
-    void leaf_levels();
+    void leaf_levels();                             // exact bounds of every base cell
+    void leaf_levels_dirty();                       // exact bounds of the marked cells; clears the marks
 
     // neighbour finders (neighbor.F90:6-396); -1 = null
     int init_up(int c) const;
@@ -134,8 +137,9 @@ struct Tree {
     void gridmodify1(int c);                        // mesh.F90:305-353
     void gridmodify4(int c);                        // mesh.F90:355-395
     void solute_gridmodify(int lvl_amr);            // solution.F90:537-556
-    // order in which rot_div visits interior leaves (solution.F90:433-465)
-    void rot_div_order(std::vector<int32_t>& out) const;
+    // Srot, Sdiv and Ne of rot_div: the sums of the squared sensors over the interior leaves,
+    // added in the order rot_div visits them (solution.F90:430-465)
+    void rot_div_sums(double& srot, double& sdiv, int& ne) const;
 
     // ---- initial conditions (host, driver support) ----
     void initial(int ic_kind, uint64_t seed, const double* ic_param8);

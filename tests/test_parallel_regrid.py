@@ -1,6 +1,6 @@
 """The host regrid on several threads against the same regrid on one thread and against the
 oracle's serial regrid (orc_regrid_sensors). On many threads the library runs solute_ref over
-the base cells in parallel and gridmodify1 / gridmodify4 over wavefronts of base cells, takes
+the base cells in parallel and gridmodify1 / gridmodify4 row by row over the base cells, takes
 son blocks from a shared pool and renumbers the nodes canonically afterwards; the trees must
 come out node for node (canonical numbering, with the leaves' values) as the serial loops make
 them (mesh.F90:305-395, solution.F90:468-556). Host-only contexts with caller-supplied sensors:
