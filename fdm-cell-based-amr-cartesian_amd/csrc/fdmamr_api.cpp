@@ -6,6 +6,7 @@
 #include <atomic>
 #include <array>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -1377,6 +1378,14 @@ int32_t fdmamr_selector_stats(fdmamr_ctx* c, int32_t* troubled, int32_t* judged,
         HIPCHK(c, hipMemcpyAsync(c->sel_cnt.p, z.data(), z.size() * 4, hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
+    return 0;
+}
+
+int32_t fdmamr_selector_judge_counter(fdmamr_ctx* c, int64_t set_to, uint32_t* current) {
+    if (!c) return -1;
+    if (set_to > (int64_t)UINT32_MAX) return fail(c, -1, "fdmamr_selector_judge_counter: value out of range");
+    if (current) *current = c->sel_step;
+    if (set_to >= 0) c->sel_step = (uint32_t)set_to;
     return 0;
 }
 

@@ -37,6 +37,7 @@ EXPORTS = (
     "fdmamr_selector_get_weights", "fdmamr_selector_config", "fdmamr_selector_infer",
     "fdmamr_selector_features", "fdmamr_selector_collect", "fdmamr_selector_clear_dataset",
     "fdmamr_selector_train", "fdmamr_selector_grad", "fdmamr_selector_stats",
+    "fdmamr_selector_judge_counter",
 )
 SELECTOR_NPARAM = 13569
 SELECTOR_NFEAT = 80
@@ -121,6 +122,7 @@ def lib():
         "fdmamr_selector_grad": ([P, C.POINTER(C.c_uint16), C.POINTER(C.c_uint8), C.c_int32,
                                   C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)], C.c_int32),
         "fdmamr_selector_stats": ([P, ip, ip, ip, C.c_int32], C.c_int32),
+        "fdmamr_selector_judge_counter": ([P, C.c_int64, C.POINTER(C.c_uint32)], C.c_int32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -507,6 +509,71 @@ class Solver:
         nc = np.zeros(1, np.int32)
         self._check(self.L.fdmamr_leaf_lists(self.ctx, _ip(lst), _ip(nc)), "leaf_lists")
         return lst[: int(nc[0])], lst[int(nc[0]):]
+
+    # -- checkpoint and data dumps (no reference counterpart: the reference keeps its state in
+    # memory and writes only Residual.dat and accuracy_error.dat, output_ADT.F90:7-75) --
+    TREE_KEYS = ("father", "son", "lvl", "sp", "num", "cross", "x", "y", "prim7")
+
+    def save(self, path):
+        """Write a checkpoint (numpy .npz, nothing pickled): the quadtree with U(1:7) of every
+        node, which holds the leaves' states; the driver's tt, it, tstep and running lambda;
+        and, when the context has a selector with weights, the network weights, the per-leaf
+        flags (with their sticky judge and regrid bits) and the judge's sample counter.
+        restore() on a context made with the same Config continues exactly where this one
+        stands (the selector's threshold, judge fraction and label bound are the caller's to
+        set again with selector_config)."""
+        t = self.tree()
+        d = {k: t[k] for k in self.TREE_KEYS}
+        d["dims"] = np.array(self.dims(t), np.int64)
+        d["scalars"] = np.array([self.tt, self.tstep, self.lam[0], self.lam[1]], np.float64)
+        d["it"] = np.array([self.it], np.int64)
+        if self.cfg.selector:
+            try:
+                d["sel_w"] = self.selector_get_weights()
+            except FdmError:
+                pass  # no weights yet: nothing of the selector to keep
+            else:
+                d["sel_flags"] = self.get_flags()
+                cnt = C.c_uint32()
+                self._check(self.L.fdmamr_selector_judge_counter(self.ctx, -1, C.byref(cnt)), "judge_counter")
+                d["sel_counter"] = np.array([cnt.value], np.int64)
+        np.savez(path, **d)
+
+    def restore(self, path):
+        """Load a checkpoint written by save() into this context (made with the same Config)."""
+        with np.load(path) as z:
+            m, n = (int(v) for v in z["dims"])
+            self.import_tree(m, n, {k: z[k] for k in self.TREE_KEYS})
+            self.build_tables()
+            self.tt, self.tstep, lx, ly = (float(v) for v in z["scalars"])
+            self.lam = np.array([lx, ly])
+            self.it = int(z["it"][0])
+            if "sel_w" in z.files:
+                self.selector_set_weights(z["sel_w"])
+                self.set_flags(z["sel_flags"], use=True)
+                self._check(self.L.fdmamr_selector_judge_counter(self.ctx, int(z["sel_counter"][0]), None),
+                            "judge_counter")
+        return self
+
+    @staticmethod
+    def dims(t):
+        """Base grid (m columns, n rows) of an exported tree: the largest base indices."""
+        return int(t["num"][:, 1].max()), int(t["num"][:, 0].max())
+
+    def write_selector_rows(self, path):
+        """Dump the selector's training rows of the current state (numpy .npz): the BF16
+        feature rows (uint16 bit patterns, SELECTOR_NFEAT per leaf, the network's input as the
+        device forms it), the labels (max |w1 - w1_lin| of the leaf's WCNS reconstructions)
+        and the leaves' centres and levels, all in cell_NS order."""
+        g = self.geometry()
+        np.savez(path, features=self.selector_features(), labels=self.labels(), x=g["x"], y=g["y"],
+                 lvl=g["lvl"])
+
+    @staticmethod
+    def read_selector_rows(path):
+        """The arrays write_selector_rows() wrote, as a dict."""
+        with np.load(path) as z:
+            return {k: z[k] for k in z.files}
 
     def table_info(self):
         v = np.zeros(11, np.int32)

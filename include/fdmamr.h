@@ -254,6 +254,10 @@ int32_t fdmamr_selector_grad(fdmamr_ctx* ctx, const uint16_t* x_bf16, const uint
 /* counters since the last reset: leaves flagged troubled, judged, rejected by the judge */
 int32_t fdmamr_selector_stats(fdmamr_ctx* ctx, int32_t* troubled, int32_t* judged,
                               int32_t* rejected, int32_t reset);
+/* the judge's sample counter (the step number its hash sample is drawn with; each learned step
+ * advances it): *current receives the value, then set_to >= 0 replaces it. A checkpoint keeps
+ * it so that a resumed run draws the same samples as an uninterrupted one. */
+int32_t fdmamr_selector_judge_counter(fdmamr_ctx* ctx, int64_t set_to, uint32_t* current);
 
 /* ---- measurement (no reference counterpart; the reference times only block refinement,
  * mesh.F90:55,76-77) ---- */

@@ -62,11 +62,10 @@ static void regrid(Tree& t, Tables& tb, int k, const char* threads) {
         t.rot[c] = s;
         t.div[c] = 0.7 * s;
     }
-    std::vector<int32_t> order;
-    t.rot_div_order(order);
     double sr = 0, sd = 0;
-    for (int c : order) { sr += t.rot[c] * t.rot[c]; sd += t.div[c] * t.div[c]; }
-    const double ar = std::sqrt(sr / order.size()), ad = std::sqrt(sd / order.size());
+    int ne = 0;
+    t.rot_div_sums(sr, sd, ne);
+    const double ar = std::sqrt(sr / ne), ad = std::sqrt(sd / ne);
     t.solute_ref(ar, ad, 3, 1.0, 1.0, 0.4, 0.4);
     t.solute_gridmodify(3);
     t.idneighbor();
