@@ -505,6 +505,7 @@ void Tree::refine(int c) {                          // mesh.F90:199-260
         cell_lmax[cell] = std::max<int8_t>(cell_lmax[cell], (int8_t)(lvl[c] + 1));
         cell_lmin[cell] = std::min<int8_t>(cell_lmin[cell], (int8_t)(lvl[c] + 1));
         cell_dirty[cell] = 1;
+        cell_mod[cell] = mod_phase;
     }
 }
 
@@ -533,6 +534,7 @@ void Tree::coarse(int c) {                          // mesh.F90:397-408
         const size_t cell = (size_t)(ni[c] - 1) * m + nj[c] - 1;
         cell_lmin[cell] = std::min<int8_t>(cell_lmin[cell], (int8_t)lvl[c]);
         cell_dirty[cell] = 1;
+        cell_mod[cell] = mod_phase;
     }
 }
 
@@ -550,6 +552,7 @@ void Tree::leaf_levels() {
     cell_lmin.assign(total, 127);
     cell_lmax.assign(total, 0);
     cell_dirty.assign(total, 0);
+    cell_mod.assign(total, INT32_MIN);
     parallel_for((size_t)total, [&](size_t b, size_t e) {
         std::vector<int32_t> st;
         for (size_t i = b; i < e; i++) {
@@ -953,6 +956,14 @@ void Tree::solute_gridmodify(int lvl_amr) {         // solution.F90:537-556
     // beside its own); the parallel passes skip such cells, by the level bounds of each cell.
     // The bounds are exact at the start; within a pass refine and coarse widen those of the
     // cells they change, and between passes those cells' bounds are made exact again.
+    //
+    // From the second pass on, a phase (gridmodify1 or gridmodify4 over the cells) also skips a
+    // cell when nothing it reads has changed since the same phase of the previous pass visited
+    // or skipped it: no cell of its read set (itself and the cells beside it for gridmodify1,
+    // the cells up to two away in its row and column for gridmodify4) changed in that phase,
+    // in the phase between, or so far in this one. Its visit then sees the tree it saw last
+    // time, after a visit that changed nothing (a change would have marked a cell of its read
+    // set), so it would change nothing again.
     const unsigned nt = pass_threads(*this);
     PhaseTimer pt;
     auto span = [&](int i, int& lo, int& hi) {
@@ -980,23 +991,40 @@ void Tree::solute_gridmodify(int lvl_amr) {         // solution.F90:537-556
         if (pt.on) std::fprintf(stderr, "[fdmamr]   %s: %zu cells visited, %ld changes\n", phase, visits, mods.load());
         pt.mark(phase);
     };
+    // true when no cell of the cross of the given radius around cell i changed in this phase or
+    // the two before it (never in the first pass)
+    auto settled = [&](int i, int radius) {
+        if (mod_phase < 2) return false;
+        const int r = i / m, c = i % m, since = mod_phase - 2;
+        if (cell_mod[i] >= since) return false;
+        for (int d = 1; d <= radius; d++)
+            if ((c - d >= 0 && cell_mod[i - d] >= since) || (c + d < m && cell_mod[i + d] >= since) ||
+                (r - d >= 0 && cell_mod[i - d * m] >= since) || (r + d < n && cell_mod[i + d * m] >= since))
+                return false;
+        return true;
+    };
+    mod_phase = 0;
     for (int k = 1; k <= lvl_amr; k++) {
         mods = 0;
         if (nt > 1) {
             visits = run_rows(*this, 0, nt, n_alloc() / 16, [&](int i) {
+                if (settled(i, 1)) return false;
                 int lo, hi;
                 span(i, lo, hi);
                 return lo != hi;
             }, [&](int i) { gridmodify1(i); });
             leaf_levels_dirty();
             report("gridmodify1");
+            mod_phase++;
             visits = run_rows(*this, 2, nt, n_alloc() / 16, [&](int i) {
+                if (settled(i, 2)) return false;
                 int lo, hi;
                 span(i, lo, hi);
                 return cell_lmax[i] >= 2 && lo < cell_lmax[i] - 1;
             }, [&](int i) { gridmodify4(i); });
             leaf_levels_dirty();
             report("gridmodify4");
+            mod_phase++;
         } else {
             for (int i = 0; i < total; i++) gridmodify1(i);
             pt.mark("gridmodify1");

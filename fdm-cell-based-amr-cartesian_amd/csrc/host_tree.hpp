@@ -102,6 +102,8 @@ struct Tree {
     // which also mark the cell in cell_dirty
     std::vector<int8_t> cell_lmin, cell_lmax;
     std::vector<uint8_t> cell_dirty;
+    std::vector<int32_t> cell_mod;                  // the last balance phase that changed the cell
+    int32_t mod_phase = 0;                          // the balance phase running
     bool track_levels = false;
     std::atomic<long>* mod_count = nullptr;         // refines + coarsens, counted when set
     void leaf_levels();                             // exact bounds of every base cell
