@@ -648,56 +648,95 @@ static const DiagRule kUR = {1, 2, 3, 0, 2, 4};
 static const DiagRule kDR = {4, 1, 2, 3, 3, 1};
 static const DiagRule kDL = {3, 0, 1, 2, 4, 2};
 
-// One corner's finder given the node's own H and V neighbours hn, vn (idnei has them already)
-template <int (Tree::*HFN)(int) const, int (Tree::*VFN)(int) const>
-static int diag(const Tree& t, const DiagRule& R, int c, int hn, int vn) {
+// One corner's finder given the node's own H and V neighbours hn, vn and the face finders H, V
+// of other nodes (the recursive finders, or idnei's stored results)
+template <class HF, class VF>
+static int diag(const Tree& t, const DiagRule& R, int c, int hn, int vn, const HF& H, const VF& V) {
     const int f = t.father[c];
-    if (f < 0) return (hn < 0 || vn < 0) ? -1 : (t.*VFN)(hn);
+    if (f < 0) return (hn < 0 || vn < 0) ? -1 : V(hn);
     if (t.sp[c] == R.sib_sp) return t.son[f] + R.sib_son;
     if (t.sp[c] == R.desc_sp) {
         if (hn < 0 || vn < 0) return -1;
-        int r = (t.*VFN)(hn);
+        int r = V(hn);
         while (r >= 0 && t.lvl[r] < t.lvl[c] && t.son[r] >= 0) r = t.son[r] + R.desc_son;
         return r;
     }
-    if (t.sp[c] == R.hside_sp) return (hn < 0 || t.lvl[hn] < t.lvl[c]) ? -1 : (t.*VFN)(hn);
-    if (t.sp[c] == R.vside_sp) return (vn < 0 || t.lvl[vn] < t.lvl[c]) ? -1 : (t.*HFN)(vn);
+    if (t.sp[c] == R.hside_sp) return (hn < 0 || t.lvl[hn] < t.lvl[c]) ? -1 : V(hn);
+    if (t.sp[c] == R.vside_sp) return (vn < 0 || t.lvl[vn] < t.lvl[c]) ? -1 : H(vn);
     return -1;
 }
-static int diag_ul(const Tree& t, int c, int hn, int vn) { return diag<&Tree::left, &Tree::up>(t, kUL, c, hn, vn); }
-static int diag_ur(const Tree& t, int c, int hn, int vn) { return diag<&Tree::right, &Tree::up>(t, kUR, c, hn, vn); }
-static int diag_dr(const Tree& t, int c, int hn, int vn) { return diag<&Tree::right, &Tree::down>(t, kDR, c, hn, vn); }
-static int diag_dl(const Tree& t, int c, int hn, int vn) { return diag<&Tree::left, &Tree::down>(t, kDL, c, hn, vn); }
+int Tree::upleft(int c) const {
+    return diag(*this, kUL, c, left(c), up(c), [&](int q) { return left(q); }, [&](int q) { return up(q); });
+}
+int Tree::upright(int c) const {
+    return diag(*this, kUR, c, right(c), up(c), [&](int q) { return right(q); }, [&](int q) { return up(q); });
+}
+int Tree::downright(int c) const {
+    return diag(*this, kDR, c, right(c), down(c), [&](int q) { return right(q); }, [&](int q) { return down(q); });
+}
+int Tree::downleft(int c) const {
+    return diag(*this, kDL, c, left(c), down(c), [&](int q) { return left(q); }, [&](int q) { return down(q); });
+}
 
-int Tree::upleft(int c) const { return diag_ul(*this, c, left(c), up(c)); }
-int Tree::upright(int c) const { return diag_ur(*this, c, right(c), up(c)); }
-int Tree::downright(int c) const { return diag_dr(*this, c, right(c), down(c)); }
-int Tree::downleft(int c) const { return diag_dl(*this, c, left(c), down(c)); }
+// The face finders' recursion (neighbor.F90:65-196) as a table: for a son in position sp of
+// its father, the neighbour in direction dir is the sibling kSib[dir][sp] when that is >= 0;
+// otherwise it follows from the father's neighbour fn in that direction: null if fn is null,
+// fn itself if fn is a leaf, else fn's son kAcross[dir][sp]. Directions in Nb order (up, down,
+// right, left), positions 1..4 (index 0 unused).
+static const int8_t kSib[4][5] = {{-1, 3, 2, -1, -1}, {-1, -1, -1, 1, 0}, {-1, 1, -1, -1, 2}, {-1, -1, 0, 3, -1}};
+static const int8_t kAcross[4][5] = {{-1, -1, -1, 1, 0}, {-1, 3, 2, -1, -1}, {-1, -1, 0, 3, -1}, {-1, 1, -1, -1, 2}};
 
-void Tree::idnei(int c) {                           // neighbor.F90:412-449
+// idnei (neighbor.F90:412-449) evaluated top-down: a node's face neighbours come from its
+// father's stored ones (the same values the recursive finders return), base cells' from the
+// base grid
+void Tree::idnei_faces(int c) {
     int32_t* q = &nb[8 * (size_t)c];
-    q[NB_UP] = up(c); q[NB_DOWN] = down(c); q[NB_RIGHT] = right(c); q[NB_LEFT] = left(c);
+    const int f = father[c];
+    if (f < 0) {
+        q[NB_UP] = init_up(c); q[NB_DOWN] = init_down(c); q[NB_RIGHT] = init_right(c); q[NB_LEFT] = init_left(c);
+    } else {
+        const int p = sp[c];
+        const int32_t* qf = &nb[8 * (size_t)f];
+        for (int dir = 0; dir < 4; dir++) {
+            if (kSib[dir][p] >= 0) { q[dir] = son[f] + kSib[dir][p]; continue; }
+            const int fn = qf[dir];
+            q[dir] = fn < 0 ? -1 : son[fn] < 0 ? fn : son[fn] + kAcross[dir][p];
+        }
+    }
+    if (son[c] >= 0) for (int k = 0; k < 4; k++) idnei_faces(son[c] + k);
+}
+// the corners from the stored face neighbours of the node and of its face neighbours
+void Tree::idnei_corners(int c) {
+    int32_t* q = &nb[8 * (size_t)c];
     if (cross[c] >= 0) {
-        q[NB_UPLEFT] = diag_ul(*this, c, q[NB_LEFT], q[NB_UP]);
-        q[NB_UPRIGHT] = diag_ur(*this, c, q[NB_RIGHT], q[NB_UP]);
-        q[NB_DOWNLEFT] = diag_dl(*this, c, q[NB_LEFT], q[NB_DOWN]);
-        q[NB_DOWNRIGHT] = diag_dr(*this, c, q[NB_RIGHT], q[NB_DOWN]);
+        auto up_of = [&](int r) { return nb[8 * (size_t)r + NB_UP]; };
+        auto down_of = [&](int r) { return nb[8 * (size_t)r + NB_DOWN]; };
+        auto left_of = [&](int r) { return nb[8 * (size_t)r + NB_LEFT]; };
+        auto right_of = [&](int r) { return nb[8 * (size_t)r + NB_RIGHT]; };
+        q[NB_UPLEFT] = diag(*this, kUL, c, q[NB_LEFT], q[NB_UP], left_of, up_of);
+        q[NB_UPRIGHT] = diag(*this, kUR, c, q[NB_RIGHT], q[NB_UP], right_of, up_of);
+        q[NB_DOWNLEFT] = diag(*this, kDL, c, q[NB_LEFT], q[NB_DOWN], left_of, down_of);
+        q[NB_DOWNRIGHT] = diag(*this, kDR, c, q[NB_RIGHT], q[NB_DOWN], right_of, down_of);
     } else {
         q[NB_UPLEFT] = q[NB_UPRIGHT] = q[NB_DOWNLEFT] = q[NB_DOWNRIGHT] = -1;
     }
-    if (son[c] >= 0) for (int k = 0; k < 4; k++) idnei(son[c] + k);
+    if (son[c] >= 0) for (int k = 0; k < 4; k++) idnei_corners(son[c] + k);
 }
 void Tree::idneighbor() {
-    // idnei writes all eight entries of every node below a base cell; only released nodes
+    // idnei_faces / idnei_corners write all eight entries of every node below a base cell; only released nodes
     // (father -2), which no walk reaches, are cleared here
     nb.resize(8 * n_alloc());
     parallel_for(n_alloc(), [&](size_t b, size_t e) {
         for (size_t c = b; c < e; c++)
             if (father[c] == -2) std::fill(nb.begin() + 8 * c, nb.begin() + 8 * c + 8, -1);
     });
-    // the finders only read the tree; each base cell's subtree writes its own entries
+    // each base cell's subtree writes its own entries: first every node's face neighbours, then
+    // (when all are stored) the corners, which read those of other nodes
     parallel_for((size_t)total, [&](size_t b, size_t e) {
-        for (size_t i = b; i < e; i++) idnei((int)i);
+        for (size_t i = b; i < e; i++) idnei_faces((int)i);
+    });
+    parallel_for((size_t)total, [&](size_t b, size_t e) {
+        for (size_t i = b; i < e; i++) idnei_corners((int)i);
     });
 }
 

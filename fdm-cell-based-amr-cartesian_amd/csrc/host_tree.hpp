@@ -151,7 +151,8 @@ struct Tree {
     int alloc_block();
     void ns_walk(int c, bool fill, int& cnt);
     void bc_walk(int c, bool fill, int& cnt);
-    void idnei(int c);
+    void idnei_faces(int c);
+    void idnei_corners(int c);
     void coarse_sub(int c, double arot, double adiv, int lvl_amr, double cg_rot, double cg_div);
     void ref_sub(int c, double arot, double adiv, int lvl_amr, double rg_rot, double rg_div);
     void rot_div_walk(int c, std::vector<int32_t>& out) const;

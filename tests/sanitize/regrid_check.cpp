@@ -25,6 +25,7 @@ static uint64_t digest(const Tree& t, const Tables& tb) {
     h = fnv(h, t.son.data(), t.son.size() * 4);
     h = fnv(h, t.x.data(), t.x.size() * 8);
     h = fnv(h, t.U.data(), t.U.size() * 56);
+    h = fnv(h, t.nb.data(), t.nb.size() * 4);
     h = fnv(h, tb.stencil.data(), tb.stencil.size() * 4);
     h = fnv(h, tb.wls_i.data(), tb.wls_i.size() * 4);
     h = fnv(h, tb.wls_d.data(), tb.wls_d.size() * 8);
@@ -33,6 +34,27 @@ static uint64_t digest(const Tree& t, const Tables& tb) {
     h = fnv(h, tb.sel_loc.data(), tb.sel_loc.size());
     h = fnv(h, tb.face_lo.data(), tb.face_lo.size() * 4);
     return h;
+}
+
+// the cached neighbours (idneighbor, evaluated top-down) against the recursive finders
+static bool neighbours_match_finders(const Tree& t) {
+    for (size_t c = 0; c < t.n_alloc(); c++) {
+        if (t.father[c] == -2) continue;
+        const int ci = (int)c;
+        int want[8] = {t.up(ci), t.down(ci), t.right(ci), t.left(ci), -1, -1, -1, -1};
+        if (t.cross[c] >= 0) {
+            want[NB_UPLEFT] = t.upleft(ci);
+            want[NB_UPRIGHT] = t.upright(ci);
+            want[NB_DOWNLEFT] = t.downleft(ci);
+            want[NB_DOWNRIGHT] = t.downright(ci);
+        }
+        for (int k = 0; k < 8; k++)
+            if (t.nb[8 * c + k] != want[k]) {
+                std::printf("node %zu neighbour %d: cached %d, finder %d\n", c, k, t.nb[8 * c + k], want[k]);
+                return false;
+            }
+    }
+    return true;
 }
 
 static void mesh(Tree& t) {
@@ -85,6 +107,7 @@ int main() {
         std::printf("regrid %d: %zu leaves, %d WLS slots, digest %016llx %s\n", k, a.ns.size(), ta.S_w,
                     (unsigned long long)ha, ha == hb ? "(= one thread)" : "(DIFFERS from one thread)");
         if (ha != hb) return 1;
+        if (!neighbours_match_finders(a) || !neighbours_match_finders(b)) return 1;
     }
     return 0;
 }

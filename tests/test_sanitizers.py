@@ -1,7 +1,8 @@
-"""The host regrid and table build (threaded wavefront passes, shared node pool, parallel table
-phases) built with g++ under AddressSanitizer + UndefinedBehaviorSanitizer and, separately,
-under ThreadSanitizer, and run on a mesh large enough for the parallel passes: no sanitizer
-report, and the threaded result equals the one-thread result (tests/sanitize/regrid_check.cpp).
+"""The host regrid and table build (threaded row-pipelined balance passes, shared node pool,
+parallel table phases) built with g++ under AddressSanitizer + UndefinedBehaviorSanitizer and,
+separately, under ThreadSanitizer, and run on a mesh large enough for the parallel passes: no
+sanitizer report, the threaded result equals the one-thread result, and the cached neighbours
+equal the recursive finders' (tests/sanitize/regrid_check.cpp).
 CPU only; the host sources are compiled directly, without the HIP files."""
 import os
 import shutil
