@@ -985,17 +985,23 @@ void build_tables(const Tree& t, Tables& tb) {
                 for (int a = -2; a <= 2; a++) {
                     if (a == 0 || b == 0) continue;
                     // A same-level leaf at a along x whose stencil has a same-level leaf at b
-                    // along y: that leaf has the point at its centre, so it is the leaf the
-                    // descent below would find.
-                    const size_t ex = (size_t)i * 8 + (a == 1 ? 0 : a == -1 ? 1 : a == 2 ? 4 : 5);
-                    if (tb.st_kind[ex] == ST_COPY) {
-                        const int32_t j = tb.state_of_node[tb.st_node[ex]];
-                        const size_t ey = (size_t)j * 8 + (b == 1 ? 2 : b == -1 ? 3 : b == 2 ? 6 : 7);
-                        if (j >= 0 && j < N && tb.st_kind[ey] == ST_COPY) {
-                            tb.off16[(size_t)q * N + i] = redirect(tb.state_of_node[tb.st_node[ey]]);
-                            q++;
-                            continue;
-                        }
+                    // along y (or the other way round): that leaf has the point at its centre,
+                    // so it is the leaf the descent below would find.
+                    const int kx = a == 1 ? 0 : a == -1 ? 1 : a == 2 ? 4 : 5;
+                    const int ky = b == 1 ? 2 : b == -1 ? 3 : b == 2 ? 6 : 7;
+                    int32_t found = -1;
+                    for (int pass = 0; pass < 2 && found < 0; pass++) {
+                        const size_t e1 = (size_t)i * 8 + (pass == 0 ? kx : ky);
+                        if (tb.st_kind[e1] != ST_COPY) continue;
+                        const int32_t j = tb.state_of_node[tb.st_node[e1]];
+                        if (j < 0 || j >= N) continue;
+                        const size_t e2 = (size_t)j * 8 + (pass == 0 ? ky : kx);
+                        if (tb.st_kind[e2] == ST_COPY) found = tb.st_node[e2];
+                    }
+                    if (found >= 0) {
+                        tb.off16[(size_t)q * N + i] = redirect(tb.state_of_node[found]);
+                        q++;
+                        continue;
                     }
                     if (na < 0) {
                         na = 0;
