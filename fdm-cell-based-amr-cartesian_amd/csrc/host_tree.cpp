@@ -405,7 +405,7 @@ static void run_fronts(Tree& t, int skew, unsigned nt, size_t room, const F& f) 
 template <class W, class V>
 static size_t run_rows(Tree& t, int lag, unsigned nt, size_t room, const W& want, const V& visit) {
     const int m = t.m, n = t.n;
-    constexpr int kPublish = 16;  // columns between progress stores
+    constexpr int kPublish = 16;  // skipped columns between progress stores (a visit always stores)
     PassPool pool;
     pool.used = t.n_alloc();
     t.resize_nodes(pool.used + std::max(room, (size_t)1 << 16));
@@ -416,6 +416,10 @@ static size_t run_rows(Tree& t, int lag, unsigned nt, size_t room, const W& want
     std::atomic<bool> abort{false};
     std::atomic<size_t> visited{0};
     std::vector<std::exception_ptr> errs(nt);
+    // with FDMAMR_TIMING: each worker's time inside visits
+    const bool timing = std::getenv("FDMAMR_TIMING") != nullptr;
+    std::vector<double> busy(nt, 0.0);
+    const double t_start = timing ? now_ms() : 0.0;
     auto worker = [&](unsigned w) {
         std::vector<int32_t> cache;
         long mods = 0;
@@ -433,18 +437,24 @@ static size_t run_rows(Tree& t, int lag, unsigned nt, size_t room, const W& want
                     if (seen < need && spin > 2000) std::this_thread::yield();
                 }
                 const int i = r * m + c;
+                bool visited_cell = false;
                 if (!abort.load(std::memory_order_relaxed) && want(i)) {
                     pool.grow.lock_shared();
                     visits++;
+                    visited_cell = true;
+                    const double v0 = timing ? now_ms() : 0.0;
                     try {
                         visit(i);
                     } catch (...) {
                         errs[w] = std::current_exception();
                         abort = true;
                     }
+                    if (timing) busy[w] += now_ms() - v0;
                     pool.grow.unlock_shared();
                 }
-                if ((c + 1) % kPublish == 0 || c + 1 == m) done[r].store(c + 1, std::memory_order_release);
+                // after a visit at once, so that the next row follows a slow stretch closely
+                if (visited_cell || (c + 1) % kPublish == 0 || c + 1 == m)
+                    done[r].store(c + 1, std::memory_order_release);
             }
         }
         {
@@ -463,6 +473,12 @@ static size_t run_rows(Tree& t, int lag, unsigned nt, size_t room, const W& want
     for (unsigned w = 1; w < nt; w++) th.emplace_back(worker, w);
     worker(0);
     for (auto& x : th) x.join();
+    if (timing) {
+        double sum = 0.0, mx = 0.0;
+        for (double b : busy) { sum += b; mx = std::max(mx, b); }
+        std::fprintf(stderr, "[fdmamr]   rows: %.1f ms, workers inside visits %.1f ms in all, at most %.1f ms each\n",
+                     now_ms() - t_start, sum, mx);
+    }
     t.pool = nullptr;
     t.resize_nodes(pool.used);
     for (auto& e : errs)
@@ -564,15 +580,17 @@ void Tree::leaf_levels() {
     });
 }
 void Tree::leaf_levels_dirty() {
-    std::vector<int32_t> st;
-    for (int i = 0; i < total; i++) {
-        if (!cell_dirty[i]) continue;
-        int lo, hi;
-        cell_levels(*this, i, st, lo, hi);
-        cell_lmin[i] = (int8_t)lo;
-        cell_lmax[i] = (int8_t)hi;
-        cell_dirty[i] = 0;
-    }
+    parallel_for((size_t)total, [&](size_t b, size_t e) {
+        std::vector<int32_t> st;
+        for (size_t i = b; i < e; i++) {
+            if (!cell_dirty[i]) continue;
+            int lo, hi;
+            cell_levels(*this, (int)i, st, lo, hi);
+            cell_lmin[i] = (int8_t)lo;
+            cell_lmax[i] = (int8_t)hi;
+            cell_dirty[i] = 0;
+        }
+    });
 }
 
 // ---------------------------------------------------------------- neighbours
