@@ -980,27 +980,33 @@ void build_tables(const Tree& t, Tables& tb) {
             // The ancestors with their half widths, nearest first (listed when first needed):
             int anc[64], na = -1;
             double ax[64], ay[64], ahx[64], ahy[64];
+            // A same-level leaf at a along x whose stencil has a same-level leaf at b along y
+            // (or the other way round): that leaf has the point at its centre, so it is the
+            // leaf the descent below would find, and its id is in that stencil row already
+            // (stencil ids of same-level leaves are their state ids, aliases resolved). The
+            // same-level interior leaves at a along x and at b along y, once per leaf:
+            static constexpr int kXe[4] = {5, 1, 0, 4}, kYe[4] = {7, 3, 2, 6};  // offsets -2, -1, 1, 2
+            auto same_level_leaf = [&](size_t e) -> int32_t {
+                if (tb.st_kind[e] != ST_COPY) return -1;
+                const int32_t j = tb.state_of_node[tb.st_node[e]];
+                return j >= 0 && j < N ? j : -1;
+            };
+            int32_t jx[4], jy[4];
+            for (int u = 0; u < 4; u++) {
+                jx[u] = same_level_leaf((size_t)i * 8 + kXe[u]);
+                jy[u] = same_level_leaf((size_t)i * 8 + kYe[u]);
+            }
             int q = 0;
-            for (int b = -2; b <= 2; b++)
-                for (int a = -2; a <= 2; a++) {
-                    if (a == 0 || b == 0) continue;
-                    // A same-level leaf at a along x whose stencil has a same-level leaf at b
-                    // along y (or the other way round): that leaf has the point at its centre,
-                    // so it is the leaf the descent below would find.
-                    const int kx = a == 1 ? 0 : a == -1 ? 1 : a == 2 ? 4 : 5;
-                    const int ky = b == 1 ? 2 : b == -1 ? 3 : b == 2 ? 6 : 7;
-                    int32_t found = -1;
-                    for (int pass = 0; pass < 2 && found < 0; pass++) {
-                        const size_t e1 = (size_t)i * 8 + (pass == 0 ? kx : ky);
-                        if (tb.st_kind[e1] != ST_COPY) continue;
-                        const int32_t j = tb.state_of_node[tb.st_node[e1]];
-                        if (j < 0 || j >= N) continue;
-                        const size_t e2 = (size_t)j * 8 + (pass == 0 ? ky : kx);
-                        if (tb.st_kind[e2] == ST_COPY) found = tb.st_node[e2];
-                    }
-                    if (found >= 0) {
-                        tb.off16[(size_t)q * N + i] = redirect(tb.state_of_node[found]);
-                        q++;
+            for (int vb = 0; vb < 4; vb++)
+                for (int ua = 0; ua < 4; ua++, q++) {
+                    const int a = ua < 2 ? ua - 2 : ua - 1, b = vb < 2 ? vb - 2 : vb - 1;
+                    size_t e2 = SIZE_MAX;
+                    if (jx[ua] >= 0 && tb.st_kind[(size_t)jx[ua] * 8 + kYe[vb]] == ST_COPY)
+                        e2 = (size_t)jx[ua] * 8 + kYe[vb];
+                    else if (jy[vb] >= 0 && tb.st_kind[(size_t)jy[vb] * 8 + kXe[ua]] == ST_COPY)
+                        e2 = (size_t)jy[vb] * 8 + kXe[ua];
+                    if (e2 != SIZE_MAX) {
+                        tb.off16[(size_t)q * N + i] = tb.stencil[e2];
                         continue;
                     }
                     if (na < 0) {
@@ -1027,7 +1033,6 @@ void build_tables(const Tree& t, Tables& tb) {
                         node = t.son[node] + (rt && upq ? 2 : rt ? 1 : upq ? 3 : 0);
                     }
                     tb.off16[(size_t)q * N + i] = redirect(tb.state_of_node[node]);
-                    q++;
                 }
         }
     });
