@@ -815,6 +815,76 @@ void build_tables(const Tree& t, Tables& tb) {
     }
 
     pt.mark("tables: restriction + fill");
+    // off-axis points of the 5x5 selector stencil (state ids; ghost aliases resolved below)
+    resize_fresh(tb.off16, 16 * (size_t)N);
+    parallel_for((size_t)N, [&](size_t b0, size_t e0) {
+        for (size_t ii = b0; ii < e0; ii++) {
+            const int i = (int)ii;
+            const int c = tb.leaf_node[i];
+            const double dy = t.h / pow2i(t.lvl[c]), dx = t.xf * dy;
+            // The points are cell centres at the leaf's level, so each lies at least half a
+            // leaf width inside every box of that level or coarser: the nearest ancestor whose
+            // box holds it is on the path from its base cell, and the descent can start there.
+            // The ancestors with their half widths, nearest first (listed when first needed):
+            int anc[64], na = -1;
+            double ax[64], ay[64], ahx[64], ahy[64];
+            // A same-level leaf at a along x whose stencil has a same-level leaf at b along y
+            // (or the other way round): that leaf has the point at its centre, so it is the
+            // leaf the descent below would find, and its id is in that stencil row already
+            // (a same-level leaf's stencil id is its state id; ghost aliases are resolved
+            // later, for these ids and the stencil's alike). The same-level interior leaves at
+            // a along x and at b along y, once per leaf:
+            static constexpr int kXe[4] = {5, 1, 0, 4}, kYe[4] = {7, 3, 2, 6};  // offsets -2, -1, 1, 2
+            auto same_level_leaf = [&](size_t e) -> int32_t {
+                const int32_t j = tb.stencil[e];
+                return tb.st_kind[e] == ST_COPY && j < N ? j : -1;
+            };
+            int32_t jx[4], jy[4];
+            for (int u = 0; u < 4; u++) {
+                jx[u] = same_level_leaf((size_t)i * 8 + kXe[u]);
+                jy[u] = same_level_leaf((size_t)i * 8 + kYe[u]);
+            }
+            int q = 0;
+            for (int vb = 0; vb < 4; vb++)
+                for (int ua = 0; ua < 4; ua++, q++) {
+                    const int a = ua < 2 ? ua - 2 : ua - 1, b = vb < 2 ? vb - 2 : vb - 1;
+                    size_t e2 = SIZE_MAX;
+                    if (jx[ua] >= 0 && tb.st_kind[(size_t)jx[ua] * 8 + kYe[vb]] == ST_COPY)
+                        e2 = (size_t)jx[ua] * 8 + kYe[vb];
+                    else if (jy[vb] >= 0 && tb.st_kind[(size_t)jy[vb] * 8 + kXe[ua]] == ST_COPY)
+                        e2 = (size_t)jy[vb] * 8 + kXe[ua];
+                    if (e2 != SIZE_MAX) {
+                        tb.off16[(size_t)q * N + i] = tb.stencil[e2];
+                        continue;
+                    }
+                    if (na < 0) {
+                        na = 0;
+                        for (int f = t.father[c], k = 1; f >= 0 && na < 64; f = t.father[f], k++) {
+                            anc[na] = f; ax[na] = t.x[f]; ay[na] = t.y[f];
+                            ahx[na] = 0.5 * dx * (double)(1 << k); ahy[na] = 0.5 * dy * (double)(1 << k);
+                            na++;
+                        }
+                    }
+                    const double px = t.x[c] + a * dx, py = t.y[c] + b * dy;
+                    int node = -1;
+                    for (int k = 0; k < na; k++)
+                        if (std::fabs(px - ax[k]) < ahx[k] && std::fabs(py - ay[k]) < ahy[k]) { node = anc[k]; break; }
+                    if (node < 0) {
+                        int nj1 = (int)std::floor((px - t.Xmin) / (t.xf * t.h)) + 1;
+                        int ni1 = (int)std::floor((py - t.Ymin) / t.h) + 1;
+                        nj1 = std::min(std::max(nj1, 1), t.m);
+                        ni1 = std::min(std::max(ni1, 1), t.n);
+                        node = (ni1 - 1) * t.m + nj1 - 1;
+                    }
+                    while (t.son[node] >= 0) {
+                        const bool rt = px > t.x[node], upq = py > t.y[node];
+                        node = t.son[node] + (rt && upq ? 2 : rt ? 1 : upq ? 3 : 0);
+                    }
+                    tb.off16[(size_t)q * N + i] = tb.state_of_node[node];
+                }
+        }
+    });
+    pt.mark("tables: off16");
     // Ghosts whose boundary partner is an interior leaf hold an exact copy of that leaf's state
     // after every fill, and all inflow ghosts hold the same inflow state, so every table reads
     // the leaf itself, or the first inflow ghost, instead. The fills still write every ghost
@@ -837,6 +907,9 @@ void build_tables(const Tree& t, Tables& tb) {
     };
     parallel_for(tb.stencil.size(), [&](size_t b0, size_t e0) {
         for (size_t k = b0; k < e0; k++) tb.stencil[k] = redirect(tb.stencil[k]);
+    });
+    parallel_for(tb.off16.size(), [&](size_t b0, size_t e0) {
+        for (size_t k = b0; k < e0; k++) tb.off16[k] = redirect(tb.off16[k]);
     });
     parallel_for((size_t)tb.S_w, [&](size_t b0, size_t e0) {
         for (size_t s = b0; s < e0; s++) {
@@ -967,76 +1040,6 @@ void build_tables(const Tree& t, Tables& tb) {
         if (k >= tb.G || tb.ghost_alias[k] < 0) tb.fill_fast.push_back(k);
 
     pt.mark("tables: aliases, faces, slots");
-    // off-axis points of the 5x5 selector stencil
-    resize_fresh(tb.off16, 16 * (size_t)N);
-    parallel_for((size_t)N, [&](size_t b0, size_t e0) {
-        for (size_t ii = b0; ii < e0; ii++) {
-            const int i = (int)ii;
-            const int c = tb.leaf_node[i];
-            const double dy = t.h / pow2i(t.lvl[c]), dx = t.xf * dy;
-            // The points are cell centres at the leaf's level, so each lies at least half a
-            // leaf width inside every box of that level or coarser: the nearest ancestor whose
-            // box holds it is on the path from its base cell, and the descent can start there.
-            // The ancestors with their half widths, nearest first (listed when first needed):
-            int anc[64], na = -1;
-            double ax[64], ay[64], ahx[64], ahy[64];
-            // A same-level leaf at a along x whose stencil has a same-level leaf at b along y
-            // (or the other way round): that leaf has the point at its centre, so it is the
-            // leaf the descent below would find, and its id is in that stencil row already
-            // (stencil ids of same-level leaves are their state ids, aliases resolved). The
-            // same-level interior leaves at a along x and at b along y, once per leaf:
-            static constexpr int kXe[4] = {5, 1, 0, 4}, kYe[4] = {7, 3, 2, 6};  // offsets -2, -1, 1, 2
-            auto same_level_leaf = [&](size_t e) -> int32_t {
-                if (tb.st_kind[e] != ST_COPY) return -1;
-                const int32_t j = tb.state_of_node[tb.st_node[e]];
-                return j >= 0 && j < N ? j : -1;
-            };
-            int32_t jx[4], jy[4];
-            for (int u = 0; u < 4; u++) {
-                jx[u] = same_level_leaf((size_t)i * 8 + kXe[u]);
-                jy[u] = same_level_leaf((size_t)i * 8 + kYe[u]);
-            }
-            int q = 0;
-            for (int vb = 0; vb < 4; vb++)
-                for (int ua = 0; ua < 4; ua++, q++) {
-                    const int a = ua < 2 ? ua - 2 : ua - 1, b = vb < 2 ? vb - 2 : vb - 1;
-                    size_t e2 = SIZE_MAX;
-                    if (jx[ua] >= 0 && tb.st_kind[(size_t)jx[ua] * 8 + kYe[vb]] == ST_COPY)
-                        e2 = (size_t)jx[ua] * 8 + kYe[vb];
-                    else if (jy[vb] >= 0 && tb.st_kind[(size_t)jy[vb] * 8 + kXe[ua]] == ST_COPY)
-                        e2 = (size_t)jy[vb] * 8 + kXe[ua];
-                    if (e2 != SIZE_MAX) {
-                        tb.off16[(size_t)q * N + i] = tb.stencil[e2];
-                        continue;
-                    }
-                    if (na < 0) {
-                        na = 0;
-                        for (int f = t.father[c], k = 1; f >= 0 && na < 64; f = t.father[f], k++) {
-                            anc[na] = f; ax[na] = t.x[f]; ay[na] = t.y[f];
-                            ahx[na] = 0.5 * dx * (double)(1 << k); ahy[na] = 0.5 * dy * (double)(1 << k);
-                            na++;
-                        }
-                    }
-                    const double px = t.x[c] + a * dx, py = t.y[c] + b * dy;
-                    int node = -1;
-                    for (int k = 0; k < na; k++)
-                        if (std::fabs(px - ax[k]) < ahx[k] && std::fabs(py - ay[k]) < ahy[k]) { node = anc[k]; break; }
-                    if (node < 0) {
-                        int nj1 = (int)std::floor((px - t.Xmin) / (t.xf * t.h)) + 1;
-                        int ni1 = (int)std::floor((py - t.Ymin) / t.h) + 1;
-                        nj1 = std::min(std::max(nj1, 1), t.m);
-                        ni1 = std::min(std::max(ni1, 1), t.n);
-                        node = (ni1 - 1) * t.m + nj1 - 1;
-                    }
-                    while (t.son[node] >= 0) {
-                        const bool rt = px > t.x[node], upq = py > t.y[node];
-                        node = t.son[node] + (rt && upq ? 2 : rt ? 1 : upq ? 3 : 0);
-                    }
-                    tb.off16[(size_t)q * N + i] = redirect(tb.state_of_node[node]);
-                }
-        }
-    });
-    pt.mark("tables: off16");
     // selector tile neighbourhoods (see Tables::sel_nbr)
     {
         const int ntiles = (N + 15) / 16;
