@@ -817,7 +817,9 @@ void build_tables(const Tree& t, Tables& tb) {
     pt.mark("tables: restriction + fill");
     // off-axis points of the 5x5 selector stencil (state ids; ghost aliases resolved below)
     resize_fresh(tb.off16, 16 * (size_t)N);
+    std::atomic<size_t> descents{0};
     parallel_for((size_t)N, [&](size_t b0, size_t e0) {
+        size_t nd = 0;
         for (size_t ii = b0; ii < e0; ii++) {
             const int i = (int)ii;
             const int c = tb.leaf_node[i];
@@ -881,9 +883,12 @@ void build_tables(const Tree& t, Tables& tb) {
                         node = t.son[node] + (rt && upq ? 2 : rt ? 1 : upq ? 3 : 0);
                     }
                     tb.off16[(size_t)q * N + i] = tb.state_of_node[node];
+                    nd++;
                 }
         }
+        descents += nd;
     });
+    if (pt.on) std::fprintf(stderr, "[fdmamr]   off16: %zu of %zu points by descent\n", descents.load(), 16 * (size_t)N);
     pt.mark("tables: off16");
     // Ghosts whose boundary partner is an interior leaf hold an exact copy of that leaf's state
     // after every fill, and all inflow ghosts hold the same inflow state, so every table reads
