@@ -65,3 +65,23 @@ def test_parallel_regrid_matches_serial_and_oracle(monkeypatch):
     assert set(np.unique(lv)) == {0, 1, 2, 3}
     kinds = _compare_stencils(par, o, limit=4000)
     assert kinds[1] > 0 and kinds[3] > 0
+
+
+def test_parallel_regrid_any_thread_count(monkeypatch):
+    # the row-pipelined balance passes and the skipping of settled cells must give the
+    # one-thread tree for any number of workers (odd counts, and more workers than cores)
+    cfg = _cfg(lvl_amr=4)
+    cfg.h = 1.0 / 96
+    ctx = {nt: host(cfg).setup() for nt in (1, 2, 3, 16)}
+    for k in range(5):
+        g = ctx[1].geometry()
+        rot, div = _sensors(g["x"], g["y"], 0.7 * k)
+        trees = {}
+        for nt, s in ctx.items():
+            monkeypatch.setenv("FDMAMR_HOST_THREADS", str(nt))
+            s.regrid_with_sensors(rot, div)
+            trees[nt] = s.tree()
+        for nt in (2, 3, 16):
+            for key in F.Solver.TREE_KEYS:
+                assert np.array_equal(trees[nt][key], trees[1][key]), (k, nt, key)
+    assert ctx[1].counts()["l_max"] == 4
