@@ -355,6 +355,8 @@ def main():
                     help="first run a plain device copy of this many MiB (scale for HBM counters)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter runs that measure HBM traffic")
+    ap.add_argument("--regrid-phases", action="store_true",
+                    help="print the phase times of the learned context's last timed regrid (stderr)")
     ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--mesh-file", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -402,14 +404,21 @@ def main():
         s.run_steps(n)  # synchronises at the end
         return time.perf_counter() - t0
 
-    def regrid_both():
-        # the FP64 context on its own sensors, the learned one on the same sensors (same mesh)
+    def regrid_both(phases=False):
+        # the FP64 context on its own sensors, the learned one on the same sensors (same mesh);
+        # phases: the library prints the learned context's regrid phases (FDMAMR_TIMING)
         t0 = time.perf_counter()
         rot, div = s64.sensors()
         s64.regrid_with_sensors(rot, div)
         t64 = time.perf_counter() - t0
+        if phases:
+            os.environ["FDMAMR_TIMING"] = "1"
         t0 = time.perf_counter()
-        sel.regrid_with_sensors(rot, div)
+        try:
+            sel.regrid_with_sensors(rot, div)
+        finally:
+            if phases:
+                os.environ.pop("FDMAMR_TIMING", None)
         return t64, time.perf_counter() - t0
 
     def judge_rate(st):
@@ -460,7 +469,7 @@ def main():
         upd += E * sel.n_leaf
         wall_l += run(sel, E)
         wall64 += run(s64, E)
-        a, b = regrid_both()
+        a, b = regrid_both(phases=args.regrid_phases and cyc == n_timed_cycles - 1)
         reg64 += a
         reg_l += b
     st = sel.selector_stats(reset=True)
