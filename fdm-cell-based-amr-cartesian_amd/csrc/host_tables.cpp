@@ -468,7 +468,7 @@ void build_tables(const Tree& t, Tables& tb) {
     // phase 2: the distinct level-jump keys, numbered in order of first occurrence. The jump
     // entries are sharded by key hash over the threads; each shard finds the first entry of
     // each of its keys, and a prefix count over the first entries numbers the keys.
-    std::vector<int64_t> jd;  // entries that are level jumps, ascending
+    BigVec<int64_t> jd;  // entries that are level jumps, ascending
     {
         const unsigned nch = 64;
         const size_t per = ((size_t)N + nch - 1) / nch;
@@ -478,10 +478,12 @@ void build_tables(const Tree& t, Tables& tb) {
                 for (size_t d = ch * per * 8; d < std::min((size_t)N, (ch + 1) * per) * 8; d++)
                     if (tb.st_kind[d] < 0) part[ch].push_back((int64_t)d);
         }, 1);
-        size_t tot = 0;
-        for (auto& v : part) tot += v.size();
-        jd.reserve(tot);
-        for (auto& v : part) jd.insert(jd.end(), v.begin(), v.end());
+        std::vector<size_t> at(nch + 1, 0);
+        for (unsigned ch = 0; ch < nch; ch++) at[ch + 1] = at[ch] + part[ch].size();
+        jd.resize(at[nch]);  // every element is copied below
+        parallel_for(nch, [&](size_t b, size_t e) {
+            for (size_t ch = b; ch < e; ch++) std::copy(part[ch].begin(), part[ch].end(), jd.begin() + at[ch]);
+        }, 1);
     }
     pt.mark("tables: jump entries");
     const size_t J = jd.size();
@@ -815,7 +817,27 @@ void build_tables(const Tree& t, Tables& tb) {
     }
 
     pt.mark("tables: restriction + fill");
-    // off-axis points of the 5x5 selector stencil (state ids; ghost aliases resolved below)
+    // Ghosts whose boundary partner is an interior leaf hold an exact copy of that leaf's state
+    // after every fill, and all inflow ghosts hold the same inflow state, so every table reads
+    // the leaf itself, or the first inflow ghost, instead. The fills still write every ghost
+    // (the exported state keeps them), but no stage reads the aliased ones, and the fast
+    // path's per-stage fill covers only the rest: the first inflow ghost, restriction means
+    // and restriction slots (fill_fast). A WLS slot that still reads a ghost marks the point
+    // in the ghost mask of its size word, so that the fast path can form the value from the
+    // ghost's fill list in the same launch that writes the ghost.
+    tb.ghost_alias.assign(tb.G, -1);
+    int inflow0 = -1;
+    for (int g = 0; g < tb.G; g++) {
+        if (tb.bc_ref[g] >= 0 && tb.bc_ref[g] < tb.N) tb.ghost_alias[g] = tb.bc_ref[g];
+        else if (tb.bc_ref[g] == REF_INFLOW) {
+            if (inflow0 < 0) inflow0 = g;
+            else tb.ghost_alias[g] = tb.N + inflow0;
+        }
+    }
+    auto redirect = [&](int32_t r) -> int32_t {
+        return (r >= tb.N && r < tb.N + tb.G && tb.ghost_alias[r - tb.N] >= 0) ? tb.ghost_alias[r - tb.N] : r;
+    };
+    // off-axis points of the 5x5 selector stencil (state ids, ghost aliases resolved)
     resize_fresh(tb.off16, 16 * (size_t)N);
     std::atomic<size_t> descents{0};
     parallel_for((size_t)N, [&](size_t b0, size_t e0) {
@@ -833,9 +855,9 @@ void build_tables(const Tree& t, Tables& tb) {
             // A same-level leaf at a along x whose stencil has a same-level leaf at b along y
             // (or the other way round): that leaf has the point at its centre, so it is the
             // leaf the descent below would find, and its id is in that stencil row already
-            // (a same-level leaf's stencil id is its state id; ghost aliases are resolved
-            // later, for these ids and the stencil's alike). The same-level interior leaves at
-            // a along x and at b along y, once per leaf:
+            // (a same-level leaf's stencil id is its state id; the stencil's ghost aliases are
+            // resolved after this). The same-level interior leaves at a along x and at b along
+            // y, once per leaf:
             static constexpr int kXe[4] = {5, 1, 0, 4}, kYe[4] = {7, 3, 2, 6};  // offsets -2, -1, 1, 2
             auto same_level_leaf = [&](size_t e) -> int32_t {
                 const int32_t j = tb.stencil[e];
@@ -856,7 +878,7 @@ void build_tables(const Tree& t, Tables& tb) {
                     else if (jy[vb] >= 0 && tb.st_kind[(size_t)jy[vb] * 8 + kXe[ua]] == ST_COPY)
                         e2 = (size_t)jy[vb] * 8 + kXe[ua];
                     if (e2 != SIZE_MAX) {
-                        tb.off16[(size_t)q * N + i] = tb.stencil[e2];
+                        tb.off16[(size_t)q * N + i] = redirect(tb.stencil[e2]);
                         continue;
                     }
                     if (na < 0) {
@@ -882,7 +904,7 @@ void build_tables(const Tree& t, Tables& tb) {
                         const bool rt = px > t.x[node], upq = py > t.y[node];
                         node = t.son[node] + (rt && upq ? 2 : rt ? 1 : upq ? 3 : 0);
                     }
-                    tb.off16[(size_t)q * N + i] = tb.state_of_node[node];
+                    tb.off16[(size_t)q * N + i] = redirect(tb.state_of_node[node]);
                     nd++;
                 }
         }
@@ -890,31 +912,8 @@ void build_tables(const Tree& t, Tables& tb) {
     });
     if (pt.on) std::fprintf(stderr, "[fdmamr]   off16: %zu of %zu points by descent\n", descents.load(), 16 * (size_t)N);
     pt.mark("tables: off16");
-    // Ghosts whose boundary partner is an interior leaf hold an exact copy of that leaf's state
-    // after every fill, and all inflow ghosts hold the same inflow state, so every table reads
-    // the leaf itself, or the first inflow ghost, instead. The fills still write every ghost
-    // (the exported state keeps them), but no stage reads the aliased ones, and the fast
-    // path's per-stage fill covers only the rest: the first inflow ghost, restriction means
-    // and restriction slots (fill_fast). A WLS slot that still reads a ghost marks the point
-    // in the ghost mask of its size word, so that the fast path can form the value from the
-    // ghost's fill list in the same launch that writes the ghost.
-    tb.ghost_alias.assign(tb.G, -1);
-    int inflow0 = -1;
-    for (int g = 0; g < tb.G; g++) {
-        if (tb.bc_ref[g] >= 0 && tb.bc_ref[g] < tb.N) tb.ghost_alias[g] = tb.bc_ref[g];
-        else if (tb.bc_ref[g] == REF_INFLOW) {
-            if (inflow0 < 0) inflow0 = g;
-            else tb.ghost_alias[g] = tb.N + inflow0;
-        }
-    }
-    auto redirect = [&](int32_t r) -> int32_t {
-        return (r >= tb.N && r < tb.N + tb.G && tb.ghost_alias[r - tb.N] >= 0) ? tb.ghost_alias[r - tb.N] : r;
-    };
     parallel_for(tb.stencil.size(), [&](size_t b0, size_t e0) {
         for (size_t k = b0; k < e0; k++) tb.stencil[k] = redirect(tb.stencil[k]);
-    });
-    parallel_for(tb.off16.size(), [&](size_t b0, size_t e0) {
-        for (size_t k = b0; k < e0; k++) tb.off16[k] = redirect(tb.off16[k]);
     });
     parallel_for((size_t)tb.S_w, [&](size_t b0, size_t e0) {
         for (size_t s = b0; s < e0; s++) {
