@@ -1063,16 +1063,23 @@ void build_tables(const Tree& t, Tables& tb) {
         std::vector<int32_t> tile_cnt(ntiles);
         resize_fresh(tb.sel_loc, (size_t)N * 32);
         parallel_for(nrun, [&](size_t r0, size_t r1) {
-            // open-addressing set of at most 16 x 28 keys (1024 slots, cleared after each tile)
+            // The tile's own leaves come first, at positions 0..15 (every leaf reads itself);
+            // the other points in order of first use, found through an open-addressing set of
+            // at most 16 x 28 keys (1024 slots, cleared after each tile).
             std::vector<int32_t> key(1024, -1), val(1024, 0), used, U;
             for (size_t r = r0; r < r1; r++)
                 for (size_t T = r * per; T < std::min((size_t)ntiles, (r + 1) * per); T++) {
                     U.clear();
                     const int i0 = (int)T * 16, i1 = std::min(N, i0 + 16);
+                    for (int i = i0; i < i1; i++) U.push_back(i);
                     for (int i = i0; i < i1; i++)
                         for (int g = 0; g < 4; g++)
                             for (int j = 0; j < 7; j++) {
                                 const int32_t src = point_src(i, g + 4 * j);
+                                if (src >= i0 && src < i1) {
+                                    tb.sel_loc[(size_t)i * 32 + g * 8 + j] = (uint8_t)(src - i0);
+                                    continue;
+                                }
                                 uint32_t h = ((uint32_t)src * 2654435761u) >> 22;
                                 while (key[h] != -1 && key[h] != src) h = (h + 1) & 1023;
                                 if (key[h] == -1) {
