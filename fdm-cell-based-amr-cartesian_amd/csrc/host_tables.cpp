@@ -881,6 +881,22 @@ void build_tables(const Tree& t, Tables& tb) {
                         tb.off16[(size_t)q * N + i] = redirect(tb.stencil[e2]);
                         continue;
                     }
+                    const double px = t.x[c] + a * dx, py = t.y[c] + b * dy;
+                    // Otherwise, when the x-neighbour (or y-neighbour) is a same-level leaf, the
+                    // point is the one its stencil entry along y (along x) names, and that
+                    // entry's node (a coarser leaf, or a same-level node whose sons the descent
+                    // enters) holds it: the descent starts there.
+                    int node = -1;
+                    if (jx[ua] >= 0) node = tb.st_node[(size_t)jx[ua] * 8 + kYe[vb]];
+                    else if (jy[vb] >= 0) node = tb.st_node[(size_t)jy[vb] * 8 + kXe[ua]];
+                    if (node >= 0) {
+                        while (t.son[node] >= 0) {
+                            const bool rt = px > t.x[node], upq = py > t.y[node];
+                            node = t.son[node] + (rt && upq ? 2 : rt ? 1 : upq ? 3 : 0);
+                        }
+                        tb.off16[(size_t)q * N + i] = redirect(tb.state_of_node[node]);
+                        continue;
+                    }
                     if (na < 0) {
                         na = 0;
                         for (int f = t.father[c], k = 1; f >= 0 && na < 64; f = t.father[f], k++) {
@@ -889,8 +905,6 @@ void build_tables(const Tree& t, Tables& tb) {
                             na++;
                         }
                     }
-                    const double px = t.x[c] + a * dx, py = t.y[c] + b * dy;
-                    int node = -1;
                     for (int k = 0; k < na; k++)
                         if (std::fabs(px - ax[k]) < ahx[k] && std::fabs(py - ay[k]) < ahy[k]) { node = anc[k]; break; }
                     if (node < 0) {
@@ -910,7 +924,7 @@ void build_tables(const Tree& t, Tables& tb) {
         }
         descents += nd;
     });
-    if (pt.on) std::fprintf(stderr, "[fdmamr]   off16: %zu of %zu points by descent\n", descents.load(), 16 * (size_t)N);
+    if (pt.on) std::fprintf(stderr, "[fdmamr]   off16: %zu of %zu points by descent from an ancestor\n", descents.load(), 16 * (size_t)N);
     pt.mark("tables: off16");
     parallel_for(tb.stencil.size(), [&](size_t b0, size_t e0) {
         for (size_t k = b0; k < e0; k++) tb.stencil[k] = redirect(tb.stencil[k]);
