@@ -306,25 +306,20 @@ void Tree::compact() {
     nb.clear();
 }
 
-// Run f(cell) for the base cells on host threads, front by front: front T holds the cells of
-// row r and column c (cell r*m + c) with c + skew * r == T, and starts when the previous front
-// has finished; skew < 0 makes fronts of consecutive cells. Cells of one front must neither
-// write what another reads nor read what another writes; then every cell sees the tree exactly
-// as in a serial loop over the cells in index order. New son blocks come from a PassPool whose
-// tail starts with room for `room` nodes.
+// Run f(cell) for every base cell on host threads, in chunks of consecutive cells taken from a
+// shared counter, for passes whose cells are independent (each reads and writes only its own
+// subtree), so any order gives the tree a serial loop gives. New son blocks come from a
+// PassPool whose tail starts with room for `room` nodes.
 template <class F>
-static void run_fronts(Tree& t, int skew, unsigned nt, size_t room, const F& f) {
-    const int m = t.m, n = t.n;
-    constexpr int kRun = 4096;  // cells per front when skew < 0
-    const int nfront = skew < 0 ? (t.total + kRun - 1) / kRun : (m - 1) + skew * (n - 1) + 1;
-    const int chunk = skew < 0 ? 32 : 4;
+static void run_cells(Tree& t, unsigned nt, size_t room, const F& f) {
+    constexpr int kChunk = 32;
     PassPool pool;
     pool.used = t.n_alloc();
     PhaseTimer pt;
     t.resize_nodes(pool.used + std::max(room, (size_t)1 << 16));
     pt.mark("  pass: node arrays");
     t.pool = &pool;
-    std::atomic<int> next{0}, arrived{0}, gen{0};
+    std::atomic<int> next{0};
     std::atomic<bool> abort{false};
     std::vector<std::exception_ptr> errs(nt);
     auto worker = [&](unsigned w) {
@@ -332,43 +327,17 @@ static void run_fronts(Tree& t, int skew, unsigned nt, size_t room, const F& f) 
         long mods = 0;
         tl_blocks = &cache;
         tl_mods = &mods;
-        for (int T = 0; T < nfront; T++) {
-            int r0 = 0, count = std::min(kRun, t.total - T * kRun);
-            if (skew >= 0) {
-                r0 = T > m - 1 ? (T - (m - 1) + skew - 1) / skew : 0;
-                const int r1 = std::min(n - 1, T / skew);
-                count = std::max(0, r1 - r0 + 1);
-            }
+        for (;;) {
+            const int k0 = next.fetch_add(kChunk, std::memory_order_relaxed);
+            if (k0 >= t.total || abort.load(std::memory_order_relaxed)) break;
             pool.grow.lock_shared();
-            if (!abort.load(std::memory_order_relaxed)) {
-                try {
-                    for (;;) {
-                        const int k0 = next.fetch_add(chunk, std::memory_order_relaxed);
-                        if (k0 >= count) break;
-                        for (int k = k0; k < std::min(count, k0 + chunk); k++) {
-                            if (skew < 0) f(T * kRun + k);
-                            else {
-                                const int r = r0 + k;
-                                f(r * m + (T - skew * r));
-                            }
-                        }
-                    }
-                } catch (...) {
-                    errs[w] = std::current_exception();
-                    abort = true;
-                }
+            try {
+                for (int k = k0; k < std::min(t.total, k0 + kChunk); k++) f(k);
+            } catch (...) {
+                errs[w] = std::current_exception();
+                abort = true;
             }
             pool.grow.unlock_shared();
-            // barrier: the last worker to arrive resets the cell counter and opens the next front
-            const int g = gen.load(std::memory_order_acquire);
-            if (arrived.fetch_add(1, std::memory_order_acq_rel) == (int)nt - 1) {
-                arrived.store(0, std::memory_order_relaxed);
-                next.store(0, std::memory_order_relaxed);
-                gen.fetch_add(1, std::memory_order_release);
-            } else {
-                for (int spin = 0; gen.load(std::memory_order_acquire) == g; spin++)
-                    if (spin > 2000) std::this_thread::yield();
-            }
         }
         {
             std::lock_guard<std::mutex> g(pool.mu);
@@ -962,7 +931,7 @@ void Tree::solute_ref(double arot, double adiv, int lvl_amr, double rg_rot, doub
             refines += k;
         });
         PhaseTimer pt;
-        run_fronts(*this, -1, nt, 4 * refines.load(), cell);
+        run_cells(*this, nt, 4 * refines.load(), cell);
         if (pt.on) std::fprintf(stderr, "[fdmamr]   refine candidates %zu\n", refines.load());
     } else {
         for (int i = 0; i < total; i++) cell(i);
