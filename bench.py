@@ -186,8 +186,9 @@ def measure_traffic(args, mesh_file):
             log(f"traffic: {counter} run failed (rc {r.returncode})")
             return None
         n_leaf = info[-1]["n_leaf"]
-        # one launch per stage: the troubled leaves' blocks (4 lanes per leaf), then the cleared ones
-        grid = ((4 * n_leaf + 255) // 256 + (n_leaf + 255) // 256) * 256
+        # one launch per stage: 384 claimer blocks for the troubled leaves, one block per 256
+        # cleared-leaf slots, 1536 claimer blocks (STAGE_EARLY, STAGE_LATE in fdmamr_device.hpp)
+        grid = (384 + (n_leaf + 255) // 256 + 1536) * 256
         rows = sorted(csv.DictReader(open(files[0])), key=lambda x: int(x["Dispatch_Id"]))
         shutil.rmtree(d, ignore_errors=True)
         # the calibration copies are the run's first three copy dispatches (the solver's own

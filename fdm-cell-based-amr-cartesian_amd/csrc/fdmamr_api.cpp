@@ -228,7 +228,9 @@ int upload_tables(fdmamr_ctx* c) {
     HIPCHK(c, ensure(c->flags, tb.N));
     HIPCHK(c, ensure(c->lists, tb.N));
     HIPCHK(c, ensure(c->list_cnt, list_blocks(tb.N)));
-    HIPCHK(c, ensure(c->list_meta, 2));
+    HIPCHK(c, ensure(c->list_meta, LIST_META_INTS));
+    // the mixed stage's claim counters start at zero (each launch leaves them so)
+    HIPCHK(c, hipMemsetAsync(c->list_meta.p, 0, LIST_META_INTS * sizeof(int), c->stream));
     HIPCHK(c, ensure(c->label, tb.N));
     HIPCHK(c, ensure(c->stencil, tb.stencil.size()));
     HIPCHK(c, ensure(c->lvl, tb.lvl_leaf.size()));

@@ -211,7 +211,15 @@ struct JudgeArgs {
 // label, each troubled leaf's largest weight deviation goes to label, and with jd.flags (lists
 // only) the judge runs on them in the same launch. posflag (or nullptr) is set to 1 when a
 // leaf's FP64 result has rho <= 0 or p <= 0.
-void launch_stage_fast(const DevTables& t, const int32_t* lists, const int* meta, const uint8_t* flags,
+// The mixed launch (lists != nullptr) takes its troubled units from claim counters kept in meta
+// after the list split: meta holds LIST_META_INTS ints, [CLAIM_OFF, LIST_META_INTS) zero before
+// the first launch (each launch leaves them zero). STAGE_EARLY claimer blocks come before the
+// linear blocks and STAGE_LATE after them (multiples of CLAIM_GROUPS, the XCD count).
+constexpr int CLAIM_GROUPS = 8, CLAIM_STRIDE = 16, CLAIM_OFF = 16;
+constexpr int LIST_META_INTS = CLAIM_OFF + CLAIM_STRIDE * CLAIM_GROUPS + CLAIM_STRIDE;
+constexpr int STAGE_EARLY = 384, STAGE_LATE = 1536;
+static_assert(STAGE_EARLY % CLAIM_GROUPS == 0 && STAGE_LATE % CLAIM_GROUPS == 0, "claimers per XCD group");
+void launch_stage_fast(const DevTables& t, const int32_t* lists, int* meta, const uint8_t* flags,
                        const double4* Qin, const float4* Qf_in, const double4* Qn, double4* Qout, float4* Qf_out,
                        const DevScalars* sc, double alpha, double beta, int reduce, double* partials,
                        double* lampart, float* label, const JudgeArgs& jd, int* posflag, hipStream_t st);

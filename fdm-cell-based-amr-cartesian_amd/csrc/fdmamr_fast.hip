@@ -286,28 +286,17 @@ __device__ __forceinline__ void lin_block(int blk, const DevTables& t, const uin
 // characteristic reconstruction fills four times as many lanes as there are leaves. The
 // x+1/2 and y+1/2 lanes form the flux derivatives with their partner faces, and lane 4p
 // performs the FP64 update. With LABEL the leaf's largest |w1 - w1_lin| goes to label.
+// Unit lb covers troubled leaves [64 lb, 64 lb + 64) of the n leaves lists[base ...] (or
+// leaves 64 lb ... when lists == nullptr); its residual partials go to partials[lb * 8 ...].
 template <bool LABEL>
-__device__ __forceinline__ void full_block(int blk, const DevTables& t, const int32_t* __restrict__ lists,
-                                           const int* __restrict__ meta, const double4* __restrict__ Qin,
-                                           const float4* __restrict__ Qf_in, const double4* Qn, double4* Qout,
-                                           float4* __restrict__ Qf_out, const DevScalars* __restrict__ sc,
-                                           double alpha, double beta, int reduce, double* __restrict__ partials,
-                                           double* __restrict__ lampart, int poff, float* __restrict__ label,
-                                           const JudgeArgs& jd, int* posflag) {
-    int n = t.N, base = 0;
-    if (lists) {
-        base = meta[0];
-        n = t.N - base;
-    }
-    // every leaf on this path counts as troubled for the selector statistics
-    if (LABEL && jd.flags && blk == 0 && threadIdx.x == 0 && n > 0) atomicAdd(&jd.cnt[CNT_TROUBLED], n);
-    const int nact = (4 * n + 255) >> 8;
-    if (blk >= nact) {
-        if (reduce) block_partials_zero(partials, poff + blk, lampart);
-        return;
-    }
-    const int lb = xcd_block_n(blk, nact);
-    const int tid = lb * 256 + (int)threadIdx.x;
+__device__ __forceinline__ void full_unit(int lb, int n, int base, const DevTables& t,
+                                          const int32_t* __restrict__ lists, const double4* __restrict__ Qin,
+                                          const float4* __restrict__ Qf_in, const double4* Qn, double4* Qout,
+                                          float4* __restrict__ Qf_out, const DevScalars* __restrict__ sc,
+                                          double alpha, double beta, int reduce, double* __restrict__ partials,
+                                          double* __restrict__ lampart, float* __restrict__ label,
+                                          const JudgeArgs& jd, int* posflag, int tx) {
+    const int tid = lb * 256 + tx;
     const int p = tid >> 2, q = tid & 3, dir = q >> 1, side = q & 1;
     const bool valid = p < n;
     // every lane of an active block runs to the quad exchanges (invalid ones on leaf 0)
@@ -387,38 +376,77 @@ __device__ __forceinline__ void full_block(int blk, const DevTables& t, const in
         }
         const uint64_t bj = __ballot(judged), br = __ballot(rej);
         if ((threadIdx.x & 63) == 0 && bj) {
-            int* slot = jd.cnt + (blk % CNT_SLOTS) * CNT_N;
+            int* slot = jd.cnt + (lb % CNT_SLOTS) * CNT_N;
             atomicAdd(&slot[CNT_JUDGED], (int)__popcll(bj));
             if (br) atomicAdd(&slot[CNT_REJECTED], (int)__popcll(br));
         }
     }
-    if (reduce) block_partials(rmax, rsum, partials, poff + lb, lampart, lx, ly);
+    if (reduce) block_partials(rmax, rsum, partials, lb, lampart, lx, ly);
 }
 
-// One launch for both lists: blocks [0, nb4) run the troubled leaves (four lanes each), blocks
-// [nb4, nb4 + nb) the cleared ones, so the compute-bound characteristic work and the
-// memory-bound linear work share the machine instead of draining it in turn (the long
-// characteristic blocks are dispatched first; interleaving the two kinds measured slower).
-// Held to 6 waves per SIMD (76 VGPRs), so the labelling instance keeps the occupancy of the
-// other. Partials: the full blocks at [0, nb4), the linear ones after them.
+// One launch for both lists. The troubled leaves (four lanes each) come in units of 64 that
+// "claimer" blocks take from counters; the cleared leaves (one lane each) in fixed blocks of
+// 256. Blocks [0, STAGE_EARLY) are claimers, the next nlin blocks linear, the last STAGE_LATE
+// claimers again: the early claimers (about 1.5 per CU) run the compute-bound characteristic
+// work beside the memory-bound linear blocks on every CU, and the late ones, dispatched once
+// every linear block has been, take what is left with the whole machine. The host launches the
+// same grid whatever the troubled count (it does not read it), and no block is spent on units
+// that do not exist. A claimer of XCD group x = block % 8 (blocks are dealt to the 8 XCDs in
+// turn) takes the units of the x-th eighth of the troubled list, so each XCD's L2 sees one
+// contiguous range. Held to 6 waves per SIMD (80 VGPRs), so the labelling instance keeps the
+// occupancy of the other. Partials: unit lb at [lb], the linear blocks at [poff_lin ...).
 template <bool LABEL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_stage_mixed(DevTables t, const int32_t* __restrict__ lists,
-                                                     const int* __restrict__ meta, const double4* __restrict__ Qin,
+                                                     const int* __restrict__ meta, int* __restrict__ claim,
+                                                     const double4* __restrict__ Qin,
                                                      const float4* __restrict__ Qf_in, const double4* Qn,
                                                      double4* Qout, float4* __restrict__ Qf_out,
                                                      const DevScalars* __restrict__ sc, double alpha, double beta,
                                                      int reduce, double* __restrict__ partials,
-                                                     double* __restrict__ lampart, int nb4, float* __restrict__ label,
-                                                     JudgeArgs jd, int* posflag, const uint8_t* flags) {
-    if ((int)blockIdx.x < nb4)
-        full_block<LABEL>(blockIdx.x, t, lists, meta, Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, reduce,
-                          partials, lampart, 0, label, jd, posflag);
-    else
-        lin_block(blockIdx.x - nb4, t, flags, Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials,
-                  lampart, nb4, posflag);
+                                                     double* __restrict__ lampart, int poff_lin, int nlin,
+                                                     float* __restrict__ label, JudgeArgs jd, int* posflag,
+                                                     const uint8_t* flags) {
+    const int b = blockIdx.x;
+    if (b >= STAGE_EARLY && b < STAGE_EARLY + nlin) {
+        lin_block(b - STAGE_EARLY, t, flags, Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials,
+                  lampart, poff_lin, posflag);
+        return;
+    }
+    const int base = meta[0], n = t.N - base;
+    const int nact = (4 * n + 255) >> 8;
+    const int x = b & (CLAIM_GROUPS - 1);
+    const int lo = (int)(((long long)nact * x) / CLAIM_GROUPS);
+    const int hi = (int)(((long long)nact * (x + 1)) / CLAIM_GROUPS);
+    __shared__ int s_u;
+    for (;;) {
+        // every claim advances the counter, so the loop ends once the group's range is spent
+        if (threadIdx.x == 0) s_u = lo + atomicAdd(&claim[CLAIM_STRIDE * x], 1);
+        __syncthreads();
+        const int u = __builtin_amdgcn_readfirstlane(s_u);  // uniform: keeps the unit's indexing scalar
+        __syncthreads();  // s_u is rewritten by the next claim
+        if (u >= hi) break;
+        // every leaf on this path counts as troubled for the selector statistics
+        if (LABEL && jd.flags && u == 0 && threadIdx.x == 0) atomicAdd(&jd.cnt[CNT_TROUBLED], n);
+        // gamma and the lane index, made opaque per claim: otherwise the compiler hoists the
+        // constants and lane roles derived from them out of the loop and spills the unit's
+        // registers
+        double ga = t.gamma;
+        int tx = (int)threadIdx.x;
+        asm volatile("" : "+s"(ga), "+v"(tx));
+        DevTables tl = t;
+        tl.gamma = ga;
+        full_unit<LABEL>(u, n, base, tl, lists, Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials,
+                         lampart, label, jd, posflag, tx);
+    }
+    // the last claimer out rewinds the counters for the next launch (every other claimer has
+    // made its final claim before counting itself out)
+    if (threadIdx.x == 0 && atomicAdd(&claim[CLAIM_STRIDE * CLAIM_GROUPS], 1) == STAGE_EARLY + STAGE_LATE - 1) {
+        for (int k = 0; k < CLAIM_GROUPS; k++) atomicExch(&claim[CLAIM_STRIDE * k], 0);
+        atomicExch(&claim[CLAIM_STRIDE * CLAIM_GROUPS], 0);
+    }
 }
 
-// every leaf on the full path (no flags)
+// every leaf on the full path (no flags): one unit per block
 template <bool LABEL>
 __global__ __launch_bounds__(256) void k_stage_full(DevTables t, const double4* __restrict__ Qin,
                                                     const float4* __restrict__ Qf_in, const double4* Qn,
@@ -427,8 +455,8 @@ __global__ __launch_bounds__(256) void k_stage_full(DevTables t, const double4* 
                                                     int reduce, double* __restrict__ partials,
                                                     double* __restrict__ lampart, float* __restrict__ label,
                                                     int* posflag) {
-    full_block<LABEL>(blockIdx.x, t, nullptr, nullptr, Qin, Qf_in, Qn, Qout, Qf_out, sc, alpha, beta, reduce,
-                      partials, lampart, 0, label, JudgeArgs{}, posflag);
+    full_unit<LABEL>(xcd_block_n(blockIdx.x, gridDim.x), t.N, 0, t, nullptr, Qin, Qf_in, Qn, Qout, Qf_out, sc,
+                     alpha, beta, reduce, partials, lampart, label, JudgeArgs{}, posflag, (int)threadIdx.x);
 }
 
 // ---------------------------------------------------------------- leaf lists from flags
@@ -875,20 +903,23 @@ void launch_slots_fill(const DevTables& t, const float* wf, double4* Q, float4* 
                            nitems, beg, leaf, w);
 }
 
-void launch_stage_fast(const DevTables& t, const int32_t* lists, const int* meta, const uint8_t* flags,
+void launch_stage_fast(const DevTables& t, const int32_t* lists, int* meta, const uint8_t* flags,
                        const double4* Qin, const float4* Qf_in, const double4* Qn, double4* Qout, float4* Qf_out,
                        const DevScalars* sc, double alpha, double beta, int reduce, double* partials,
                        double* lampart, float* label, const JudgeArgs& jd, int* posflag, hipStream_t st) {
     if (t.N <= 0) return;
     const int nb = nb_(t.N, 256), nb4 = nb_(4 * t.N, 256);
     if (lists) {
+        // the claim counters follow the list split in meta (LIST_META_INTS ints)
+        int* claim = meta + CLAIM_OFF;
+        const dim3 grid(STAGE_EARLY + nb + STAGE_LATE);
         if (label)
-            hipLaunchKernelGGL(k_stage_mixed<true>, dim3(nb4 + nb), dim3(256), 0, st, t, lists, meta, Qin, Qf_in,
-                               Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials, lampart, nb4, label, jd, posflag,
-                               flags);
+            hipLaunchKernelGGL(k_stage_mixed<true>, grid, dim3(256), 0, st, t, lists, meta, claim, Qin, Qf_in, Qn,
+                               Qout, Qf_out, sc, alpha, beta, reduce, partials, lampart, nb4, nb, label, jd,
+                               posflag, flags);
         else
-            hipLaunchKernelGGL(k_stage_mixed<false>, dim3(nb4 + nb), dim3(256), 0, st, t, lists, meta, Qin, Qf_in,
-                               Qn, Qout, Qf_out, sc, alpha, beta, reduce, partials, lampart, nb4, label,
+            hipLaunchKernelGGL(k_stage_mixed<false>, grid, dim3(256), 0, st, t, lists, meta, claim, Qin, Qf_in, Qn,
+                               Qout, Qf_out, sc, alpha, beta, reduce, partials, lampart, nb4, nb, label,
                                JudgeArgs{}, posflag, flags);
     } else if (label) {
         hipLaunchKernelGGL(k_stage_full<true>, dim3(nb4), dim3(256), 0, st, t, Qin, Qf_in, Qn, Qout, Qf_out, sc,

@@ -25,8 +25,12 @@ def pair(cfg_kw, steps):
     return a, b
 
 
-def test_all_troubled_flags_equal_unflagged_path():
-    kw = dict(h=10.0 / 64, localdt=1, cfl=0.5, t_end=1e9)
+@pytest.mark.parametrize("cells", [64, 512])
+def test_all_troubled_flags_equal_unflagged_path(cells):
+    # 64 x 64 leaves give fewer 64-leaf units than the mixed launch has claimer blocks, 512 x 512
+    # (4096 units) make each claimer take several in turn; the units' counters must rewind
+    # after every launch for the next stage to update every leaf again
+    kw = dict(h=10.0 / cells, localdt=1, cfl=0.5, t_end=1e9)
     b = F.Solver(F.Config(flux_precision=F.FLUX_FP32, **kw)).setup()
     c = F.Solver(F.Config(flux_precision=F.FLUX_FP32, **kw)).setup()
     c.set_flags(np.ones(c.n_leaf, np.uint8), use=True)
