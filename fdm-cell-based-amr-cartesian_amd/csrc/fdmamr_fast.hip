@@ -393,10 +393,11 @@ __device__ __forceinline__ void full_unit(int lb, int n, int base, const DevTabl
 // same grid whatever the troubled count (it does not read it), and no block is spent on units
 // that do not exist. A claimer of XCD group x = block % 8 (blocks are dealt to the 8 XCDs in
 // turn) takes the units of the x-th eighth of the troubled list, so each XCD's L2 sees one
-// contiguous range. Held to 6 waves per SIMD (80 VGPRs), so the labelling instance keeps the
-// occupancy of the other. Partials: unit lb at [lb], the linear blocks at [poff_lin ...).
+// contiguous range. Held to 6 waves per SIMD (80 VGPRs); the labelling instance (stage 1) at 5
+// (90 VGPRs): at 6 its claim loop spills 12 VGPRs to scratch, and 5 measured 2-3 % faster.
+// Partials: unit lb at [lb], the linear blocks at [poff_lin ...).
 template <bool LABEL>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_stage_mixed(DevTables t, const int32_t* __restrict__ lists,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(LABEL ? 5 : 6))) void k_stage_mixed(DevTables t, const int32_t* __restrict__ lists,
                                                      const int* __restrict__ meta, int* __restrict__ claim,
                                                      const double4* __restrict__ Qin,
                                                      const float4* __restrict__ Qf_in, const double4* Qn,
